@@ -1,0 +1,865 @@
+/*
+ * oracle.c -- FP64 CPU oracle for the density-fitting Coulomb path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
+ * Python package) links, loads or calls this file.  It is used by tests/,
+ * by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as
+ * the checker / CPU baseline, never as the thing measured or shipped.
+ *
+ * What it restates (reference: eljost/pysisyphus-addons):
+ *   - shell table from the bas_* arrays      common/shells.f90:97-187
+ *   - contracted-coefficient normalization   common/normalization.f90:10-38
+ *   - Schwarz norms sqrt(||(ab|ab)||_F), sqrt(||(C|C)||_F)
+ *                                            common/integrals/intor.f90:97-160
+ *   - the 2c2e metric (P|Q)                  common/integrals/intor.f90:40-95
+ *   - the screened, integral-direct density contraction loop
+ *                                            diabatization/intor.f90:108-183
+ *   - erfinv (Blair-Edwards-Johnson 1976), contracted centres/extents and
+ *     the QVL estimate                       common/erfinv.f90, common/screening.f90
+ *
+ * The integral arithmetic itself lives in code generated by the third-party
+ * sympleints (not in the reference tree), so it is written here from the
+ * mathematics: McMurchie-Davidson with Cartesian Hermite expansion on *all*
+ * centres (bra pair and ket), the Hermite Coulomb integrals R_tuv by the
+ * standard downward recursion, and a final Cartesian->real-solid-harmonic
+ * transform (m = -l..l) of every index.  The Boys function is evaluated by
+ * its convergent series plus downward recursion (T < 45) or erf plus upward
+ * recursion (T >= 45).
+ *
+ * Pinned by: mpmath Boys values, closed-form (ss|s)/(s|s), an independent
+ * Gaussian-transform quadrature of primitive Cartesian integrals, rotation
+ * invariance of shell-block norms, ORCA's overlap matrix in the bena2
+ * fixture, the reference's erfinv / contracted_center KATs and the
+ * reference's bena2 Coulomb-tensor golden values.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_LMAX 6                       /* largest shell L the oracle handles */
+#define ORC_NCART(l) (((l) + 1) * ((l) + 2) / 2)
+
+/* ------------------------------------------------------------------------ */
+/* Boys function                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* F_n(T) for n = 0..nmax into F[0..nmax]. */
+void orc_boys_array(int nmax, double T, double *F)
+{
+    int n;
+    if (T < 45.0) {
+        /* F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) */
+        double term = 1.0 / (2 * nmax + 1), sum = term;
+        double eT = exp(-T);
+        int k;
+        for (k = 1; k < 2000; k++) {
+            term *= 2.0 * T / (2 * nmax + 2 * k + 1);
+            sum += term;
+            if (term < 1e-18 * sum) break;
+        }
+        F[nmax] = eT * sum;
+        for (n = nmax - 1; n >= 0; n--)
+            F[n] = (2.0 * T * F[n + 1] + eT) / (2 * n + 1);
+    } else {
+        double eT = exp(-T);
+        F[0] = 0.5 * sqrt(M_PI / T) * erf(sqrt(T));
+        for (n = 0; n < nmax; n++)
+            F[n + 1] = ((2 * n + 1) * F[n] - eT) / (2.0 * T);
+    }
+}
+
+void orc_boys(int nmax, int npts, const double *T, double *F)
+{
+    int i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < npts; i++)
+        orc_boys_array(nmax, T[i], F + (size_t)i * (nmax + 1));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shells                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int L, nprim, sph_off;
+    double A[3];
+    double *exps;
+    double *coefs;   /* normalized contraction coefficients */
+} orc_shell;
+
+typedef struct {
+    int nshells, nbf, lmax;
+    orc_shell *sh;
+    double *store;
+} orc_basis;
+
+static double dfact(int n) /* (n)!!, with (-1)!! = 1 */
+{
+    double r = 1.0;
+    while (n > 1) { r *= n; n -= 2; }
+    return r;
+}
+
+/* common/normalization.f90:10-38 */
+static void norm_cgto_coeffs(double *c, const double *e, int n, int L)
+{
+    double N = 0.0;
+    int i, j;
+    for (i = 0; i < n; i++)
+        for (j = 0; j < n; j++)
+            N += c[i] * c[j] / pow(e[i] + e[j], L + 1.5) * pow(sqrt(e[i] * e[j]), L + 1.5);
+    for (i = 0; i < n; i++)
+        c[i] *= sqrt(pow(e[i], L + 1.5) / (pow(M_PI, 1.5) / pow(2.0, L) * N));
+}
+
+/* bas_centers (nshells,3) and bas_spec (nshells,5) are column-major int32
+ * with 1-based pointers into bas_data (common/shells.f90:97-187). */
+void *orc_basis_new(int nshells, const int32_t *bas_centers, const int32_t *bas_spec,
+                    const double *bas_data)
+{
+    orc_basis *b = (orc_basis *)calloc(1, sizeof(orc_basis));
+    int i, nprim_tot = 0, off = 0, sph = 0;
+    b->nshells = nshells;
+    b->sh = (orc_shell *)calloc(nshells, sizeof(orc_shell));
+    for (i = 0; i < nshells; i++) nprim_tot += bas_spec[i + 2 * nshells];
+    b->store = (double *)malloc(sizeof(double) * 2 * (nprim_tot > 0 ? nprim_tot : 1));
+    b->lmax = -1;
+    for (i = 0; i < nshells; i++) {
+        orc_shell *s = &b->sh[i];
+        int cptr = bas_centers[i + 2 * nshells] - 1;
+        int L = bas_spec[i + 1 * nshells];
+        int np = bas_spec[i + 2 * nshells];
+        int pptr = bas_spec[i + 4 * nshells] - 1;
+        s->A[0] = bas_data[cptr]; s->A[1] = bas_data[cptr + 1]; s->A[2] = bas_data[cptr + 2];
+        s->L = L; s->nprim = np; s->sph_off = sph;
+        s->coefs = b->store + off;
+        s->exps = b->store + nprim_tot + off;
+        memcpy(s->coefs, bas_data + pptr, sizeof(double) * np);
+        memcpy(s->exps, bas_data + pptr + np, sizeof(double) * np);
+        norm_cgto_coeffs(s->coefs, s->exps, np, L);
+        off += np;
+        sph += 2 * L + 1;
+        if (L > b->lmax) b->lmax = L;
+    }
+    b->nbf = sph;
+    return b;
+}
+
+void orc_basis_free(void *p)
+{
+    orc_basis *b = (orc_basis *)p;
+    if (!b) return;
+    free(b->sh); free(b->store); free(b);
+}
+
+int orc_basis_nbf(void *p) { return ((orc_basis *)p)->nbf; }
+int orc_basis_lmax(void *p) { return ((orc_basis *)p)->lmax; }
+int orc_basis_shell_L(void *p, int i) { return ((orc_basis *)p)->sh[i].L; }
+int orc_basis_shell_off(void *p, int i) { return ((orc_basis *)p)->sh[i].sph_off; }
+void orc_basis_normed_coefs(void *p, int i, double *out)
+{
+    orc_shell *s = &((orc_basis *)p)->sh[i];
+    memcpy(out, s->coefs, sizeof(double) * s->nprim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Real solid harmonics                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Cartesian order inside a shell: lx from L down to 0, then ly from L-lx
+ * down to 0, lz = L-lx-ly. */
+static int cart_index(int L, int lx, int ly, int lz)
+{
+    (void)ly;
+    return (L - lx) * (L - lx + 1) / 2 + lz;
+}
+
+static double binom(int n, int k)
+{
+    double r = 1.0;
+    int i;
+    if (k < 0 || k > n) return 0.0;
+    for (i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+static double fact(int n)
+{
+    double r = 1.0;
+    while (n > 1) r *= n--;
+    return r;
+}
+
+/* sph[m+L][cart] of the real solid harmonic S_lm (Helgaker, Jorgensen, Olsen,
+ * Molecular Electronic-Structure Theory, eqs. 6.4.47-6.4.50), scaled by
+ * 1/sqrt((2L-1)!!) so that it maps the Cartesian monomials of a contracted
+ * shell normalized as in norm_cgto_coeffs onto normalized functions. */
+static double SPH[ORC_LMAX + 1][2 * ORC_LMAX + 1][ORC_NCART(ORC_LMAX)];
+static int sph_ready = 0;
+
+static void build_sph(void)
+{
+    int L, m, t, u, v2;
+    if (sph_ready) return;
+    memset(SPH, 0, sizeof(SPH));
+    for (L = 0; L <= ORC_LMAX; L++) {
+        double scale = 1.0 / sqrt(dfact(2 * L - 1));
+        for (m = -L; m <= L; m++) {
+            int am = abs(m);
+            int vm2 = (m < 0) ? 1 : 0;
+            double N = 1.0 / (pow(2.0, am) * fact(L)) *
+                       sqrt(2.0 * fact(L + am) * fact(L - am) / (m == 0 ? 2.0 : 1.0));
+            for (t = 0; t <= (L - am) / 2; t++)
+                for (u = 0; u <= t; u++)
+                    for (v2 = vm2; v2 <= am; v2 += 2) {
+                        int lx = 2 * t + am - 2 * u - v2;
+                        int ly = 2 * u + v2;
+                        int lz = L - 2 * t - am;
+                        double sgn = (((t + (v2 - vm2) / 2) & 1) ? -1.0 : 1.0);
+                        double C = sgn * pow(0.25, t) * binom(L, t) * binom(L - t, am + t) *
+                                   binom(t, u) * binom(am, v2);
+                        SPH[L][m + L][cart_index(L, lx, ly, lz)] += N * C * scale;
+                    }
+        }
+    }
+    sph_ready = 1;
+}
+
+void orc_sph_coeffs(int L, double *out /* (2L+1) x ncart(L) */)
+{
+    int m, c;
+    build_sph();
+    for (m = 0; m < 2 * L + 1; m++)
+        for (c = 0; c < ORC_NCART(L); c++) out[m * ORC_NCART(L) + c] = SPH[L][m][c];
+}
+
+/* ------------------------------------------------------------------------ */
+/* McMurchie-Davidson primitives                                             */
+/* ------------------------------------------------------------------------ */
+
+/* E^{ij}_t for one Cartesian direction, i <= la, j <= lb, t <= i+j.
+ * Layout E[(i*(lb+1) + j)*(la+lb+1) + t].  E^{00}_0 = exp(-mu X_AB^2). */
+static void herm_E(int la, int lb, double a, double b, double A, double B, double *E)
+{
+    int i, j, t, nt = la + lb + 1;
+    double p = a + b, mu = a * b / p, X = A - B;
+    double P = (a * A + b * B) / p, XPA = P - A, XPB = P - B, oo2p = 0.5 / p;
+#define EI(i, j, t) E[((i) * (lb + 1) + (j)) * nt + (t)]
+    memset(E, 0, sizeof(double) * (la + 1) * (lb + 1) * nt);
+    EI(0, 0, 0) = exp(-mu * X * X);
+    for (i = 0; i <= la; i++)
+        for (j = 0; j <= lb; j++) {
+            if (i == 0 && j == 0) continue;
+            for (t = 0; t <= i + j; t++) {
+                double v = 0.0;
+                if (i > 0) {
+                    if (t > 0) v += oo2p * EI(i - 1, j, t - 1);
+                    if (t <= i - 1 + j) v += XPA * EI(i - 1, j, t);
+                    if (t + 1 <= i - 1 + j) v += (t + 1) * EI(i - 1, j, t + 1);
+                } else {
+                    if (t > 0) v += oo2p * EI(i, j - 1, t - 1);
+                    if (t <= i + j - 1) v += XPB * EI(i, j - 1, t);
+                    if (t + 1 <= i + j - 1) v += (t + 1) * EI(i, j - 1, t + 1);
+                }
+                EI(i, j, t) = v;
+            }
+        }
+#undef EI
+}
+
+/* Hermite Coulomb integrals R_tuv(alpha; X,Y,Z) for t+u+v <= L, dense
+ * (L+1)^3 layout R[(t*(L+1) + u)*(L+1) + v], by the downward recursion
+ * R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{t,u,v}. */
+static void herm_R(int L, double alpha, double X, double Y, double Z, double *R, double *work)
+{
+    int n, t, u, v, s, L1 = L + 1;
+    double F[4 * ORC_LMAX + 2];
+    double *cur = work, *nxt = R;
+    size_t sz = (size_t)L1 * L1 * L1;
+#define RI(a, t, u, v) a[((t) * L1 + (u)) * L1 + (v)]
+    orc_boys_array(L, alpha * (X * X + Y * Y + Z * Z), F);
+    /* alternate so that level 0 lands in R */
+    if (L % 2 == 0) { cur = R; nxt = work; }
+    memset(cur, 0, sizeof(double) * sz);
+    RI(cur, 0, 0, 0) = pow(-2.0 * alpha, L) * F[L];
+    for (n = L - 1; n >= 0; n--) {
+        memset(nxt, 0, sizeof(double) * sz);
+        RI(nxt, 0, 0, 0) = pow(-2.0 * alpha, n) * F[n];
+        for (s = 1; s <= L - n; s++)
+            for (t = s; t >= 0; t--)
+                for (u = s - t; u >= 0; u--) {
+                    double val;
+                    v = s - t - u;
+                    if (t > 0) {
+                        val = X * RI(cur, t - 1, u, v);
+                        if (t > 1) val += (t - 1) * RI(cur, t - 2, u, v);
+                    } else if (u > 0) {
+                        val = Y * RI(cur, t, u - 1, v);
+                        if (u > 1) val += (u - 1) * RI(cur, t, u - 2, v);
+                    } else {
+                        val = Z * RI(cur, t, u, v - 1);
+                        if (v > 1) val += (v - 1) * RI(cur, t, u, v - 2);
+                    }
+                    RI(nxt, t, u, v) = val;
+                }
+        { double *tmp = cur; cur = nxt; nxt = tmp; }
+    }
+#undef RI
+}
+
+/* A "side" of a two-electron integral: a primitive product of two Cartesian
+ * Gaussians (la at A, exponent a) x (lb at B, exponent b).  A single-centre
+ * function is the product with lb = 0, b = 0, B = A. */
+typedef struct {
+    int la, lb;
+    double a, b, A[3], B[3];
+} orc_side;
+
+/* Work buffer sizes for a side with la, lb <= ORC_LMAX. */
+#define ORC_ESZ ((ORC_LMAX + 1) * (ORC_LMAX + 1) * (2 * ORC_LMAX + 1))
+
+typedef struct {
+    double E[2][3][ORC_ESZ];
+    double *R, *Rw, *V, *scr;
+    size_t Rsz, Vsz, scrsz;
+} orc_work;
+
+static void work_init(orc_work *w, int Lmax_total)
+{
+    size_t L1 = Lmax_total + 1;
+    w->Rsz = L1 * L1 * L1;
+    w->R = (double *)malloc(sizeof(double) * w->Rsz);
+    w->Rw = (double *)malloc(sizeof(double) * w->Rsz);
+    w->Vsz = L1 * L1 * L1 * ORC_NCART(ORC_LMAX) * ORC_NCART(ORC_LMAX);
+    w->V = NULL;
+    w->scr = NULL;
+    w->scrsz = 0;
+}
+
+static void work_free(orc_work *w)
+{
+    free(w->R); free(w->Rw); free(w->V); free(w->scr);
+}
+
+/* Accumulates coef * [ab|cd] (Cartesian, unnormalized monomials) into
+ * out[((ia*ncb + ib)*ncc + ic)*ncd + id]. */
+static void prim_eri(const orc_side *bra, const orc_side *ket, double coef, double *out,
+                     orc_work *w)
+{
+    int d, la = bra->la, lb = bra->lb, lc = ket->la, ld = ket->lb;
+    int lab = la + lb, lcd = lc + ld, L = lab + lcd, L1 = L + 1;
+    int nca = ORC_NCART(la), ncb = ORC_NCART(lb), ncc = ORC_NCART(lc), ncd = ORC_NCART(ld);
+    double p = bra->a + bra->b, q = ket->a + ket->b, alpha = p * q / (p + q);
+    double P[3], Q[3];
+    double pref = 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q)) * coef;
+    int H = (lab + 1) * (lab + 1) * (lab + 1);
+    size_t vsz;
+    int ia, ib, ic, id;
+
+    for (d = 0; d < 3; d++) {
+        P[d] = (bra->a * bra->A[d] + bra->b * bra->B[d]) / p;
+        Q[d] = (ket->a * ket->A[d] + ket->b * ket->B[d]) / q;
+        herm_E(la, lb, bra->a, bra->b, bra->A[d], bra->B[d], w->E[0][d]);
+        herm_E(lc, ld, ket->a, ket->b, ket->A[d], ket->B[d], w->E[1][d]);
+    }
+    herm_R(L, alpha, P[0] - Q[0], P[1] - Q[1], P[2] - Q[2], w->R, w->Rw);
+
+    /* V[tuv][ic][id] = sum_{tau nu phi} (-1)^{tau+nu+phi} E^{cd}_{tau nu phi} R_{t+tau,...} */
+    vsz = (size_t)H * ncc * ncd;
+    if (w->V == NULL || vsz > w->Vsz) {
+        free(w->V);
+        w->Vsz = vsz > w->Vsz ? vsz : w->Vsz;
+        w->V = (double *)malloc(sizeof(double) * w->Vsz);
+    }
+    memset(w->V, 0, sizeof(double) * vsz);
+    {
+        int ntc = lcd + 1, ntb = lab + 1;
+        int cxs[ORC_NCART(ORC_LMAX)][3], dxs[ORC_NCART(ORC_LMAX)][3];
+        int k = 0, lx, ly;
+        for (lx = lc; lx >= 0; lx--)
+            for (ly = lc - lx; ly >= 0; ly--, k++) { cxs[k][0] = lx; cxs[k][1] = ly; cxs[k][2] = lc - lx - ly; }
+        k = 0;
+        for (lx = ld; lx >= 0; lx--)
+            for (ly = ld - lx; ly >= 0; ly--, k++) { dxs[k][0] = lx; dxs[k][1] = ly; dxs[k][2] = ld - lx - ly; }
+        for (ic = 0; ic < ncc; ic++)
+            for (id = 0; id < ncd; id++) {
+                const double *Ex = w->E[1][0] + (cxs[ic][0] * (ld + 1) + dxs[id][0]) * ntc;
+                const double *Ey = w->E[1][1] + (cxs[ic][1] * (ld + 1) + dxs[id][1]) * ntc;
+                const double *Ez = w->E[1][2] + (cxs[ic][2] * (ld + 1) + dxs[id][2]) * ntc;
+                int tmax = cxs[ic][0] + dxs[id][0], umax = cxs[ic][1] + dxs[id][1],
+                    vmax = cxs[ic][2] + dxs[id][2];
+                int t, u, v, tt, uu, vv;
+                for (t = 0; t <= lab; t++)
+                    for (u = 0; u <= lab - t; u++)
+                        for (v = 0; v <= lab - t - u; v++) {
+                            double acc = 0.0;
+                            for (tt = 0; tt <= tmax; tt++)
+                                for (uu = 0; uu <= umax; uu++) {
+                                    double exy = Ex[tt] * Ey[uu];
+                                    for (vv = 0; vv <= vmax; vv++) {
+                                        double sgn = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+                                        acc += sgn * exy * Ez[vv] *
+                                               w->R[((t + tt) * L1 + (u + uu)) * L1 + (v + vv)];
+                                    }
+                                }
+                            w->V[((size_t)((t * ntb + u) * ntb + v) * ncc + ic) * ncd + id] = acc;
+                        }
+            }
+        /* out[ab][cd] += pref * sum_tuv E^{ab}_tuv V[tuv][cd] */
+        {
+            int axs[ORC_NCART(ORC_LMAX)][3], bxs[ORC_NCART(ORC_LMAX)][3];
+            int ncd2 = ncc * ncd;
+            k = 0;
+            for (lx = la; lx >= 0; lx--)
+                for (ly = la - lx; ly >= 0; ly--, k++) { axs[k][0] = lx; axs[k][1] = ly; axs[k][2] = la - lx - ly; }
+            k = 0;
+            for (lx = lb; lx >= 0; lx--)
+                for (ly = lb - lx; ly >= 0; ly--, k++) { bxs[k][0] = lx; bxs[k][1] = ly; bxs[k][2] = lb - lx - ly; }
+            for (ia = 0; ia < nca; ia++)
+                for (ib = 0; ib < ncb; ib++) {
+                    const double *Ex = w->E[0][0] + (axs[ia][0] * (lb + 1) + bxs[ib][0]) * ntb;
+                    const double *Ey = w->E[0][1] + (axs[ia][1] * (lb + 1) + bxs[ib][1]) * ntb;
+                    const double *Ez = w->E[0][2] + (axs[ia][2] * (lb + 1) + bxs[ib][2]) * ntb;
+                    int tmax = axs[ia][0] + bxs[ib][0], umax = axs[ia][1] + bxs[ib][1],
+                        vmax = axs[ia][2] + bxs[ib][2];
+                    double *o = out + (size_t)(ia * ncb + ib) * ncd2;
+                    int t, u, v, j;
+                    for (t = 0; t <= tmax; t++)
+                        for (u = 0; u <= umax; u++) {
+                            double exy = pref * Ex[t] * Ey[u];
+                            for (v = 0; v <= vmax; v++) {
+                                double e = exy * Ez[v];
+                                const double *Vp = w->V + (size_t)((t * ntb + u) * ntb + v) * ncd2;
+                                for (j = 0; j < ncd2; j++) o[j] += e * Vp[j];
+                            }
+                        }
+                }
+        }
+    }
+}
+
+/* Contracted Cartesian (ab|cd) over shells sa, sb (bra) and sc, sd (ket);
+ * sb / sd may be NULL for a single-centre side. */
+static void contracted_eri(const orc_shell *sa, const orc_shell *sb, const orc_shell *sc,
+                           const orc_shell *sd, double *cart, orc_work *w)
+{
+    int la = sa->L, lb = sb ? sb->L : 0, lc = sc->L, ld = sd ? sd->L : 0;
+    int nb = sb ? sb->nprim : 1, nd = sd ? sd->nprim : 1;
+    size_t n = (size_t)ORC_NCART(la) * ORC_NCART(lb) * ORC_NCART(lc) * ORC_NCART(ld);
+    int i, j, k, l;
+    orc_side bra, ket;
+    memset(cart, 0, sizeof(double) * n);
+    bra.la = la; bra.lb = lb; ket.la = lc; ket.lb = ld;
+    memcpy(bra.A, sa->A, sizeof(bra.A));
+    memcpy(bra.B, sb ? sb->A : sa->A, sizeof(bra.B));
+    memcpy(ket.A, sc->A, sizeof(ket.A));
+    memcpy(ket.B, sd ? sd->A : sc->A, sizeof(ket.B));
+    for (i = 0; i < sa->nprim; i++)
+        for (j = 0; j < nb; j++) {
+            bra.a = sa->exps[i];
+            bra.b = sb ? sb->exps[j] : 0.0;
+            for (k = 0; k < sc->nprim; k++)
+                for (l = 0; l < nd; l++) {
+                    double c = sa->coefs[i] * (sb ? sb->coefs[j] : 1.0) * sc->coefs[k] *
+                               (sd ? sd->coefs[l] : 1.0);
+                    ket.a = sc->exps[k];
+                    ket.b = sd ? sd->exps[l] : 0.0;
+                    prim_eri(&bra, &ket, c, cart, w);
+                }
+        }
+}
+
+/* Cartesian -> spherical on all four indices, in place in buf (which holds
+ * the Cartesian block on entry); the result is
+ * buf[((ma*nb + mb)*nc + mc)*nd + md].  scr must hold a Cartesian block. */
+static void cart2sph4(int la, int lb, int lc, int ld, double *buf, double *scr)
+{
+    int Ls[4] = {la, lb, lc, ld};
+    int dims[4], k, d;
+    build_sph();
+    for (k = 0; k < 4; k++) dims[k] = ORC_NCART(Ls[k]);
+    for (k = 0; k < 4; k++) {
+        int pre = 1, post = 1, pidx, m, i, c;
+        int nck = ORC_NCART(Ls[k]), nsk = 2 * Ls[k] + 1;
+        for (d = 0; d < k; d++) pre *= dims[d];
+        for (d = k + 1; d < 4; d++) post *= dims[d];
+        for (pidx = 0; pidx < pre; pidx++)
+            for (m = 0; m < nsk; m++)
+                for (i = 0; i < post; i++) {
+                    double acc = 0.0;
+                    for (c = 0; c < nck; c++)
+                        acc += SPH[Ls[k]][m][c] * buf[((size_t)pidx * nck + c) * post + i];
+                    scr[((size_t)pidx * nsk + m) * post + i] = acc;
+                }
+        memcpy(buf, scr, sizeof(double) * (size_t)pre * nsk * post);
+        dims[k] = nsk;
+    }
+}
+
+/* Cartesian block size for shells la, lb, lc, ld. */
+static size_t block_cart_size(int la, int lb, int lc, int ld)
+{
+    return (size_t)ORC_NCART(la) * ORC_NCART(lb) * ORC_NCART(lc) * ORC_NCART(ld);
+}
+
+/* Spherical block of the contracted integral over up to four shells, written
+ * to out (which must hold the Cartesian block size). */
+static void sph_block(const orc_shell *sa, const orc_shell *sb, const orc_shell *sc,
+                      const orc_shell *sd, double *out, orc_work *w)
+{
+    int la = sa->L, lb = sb ? sb->L : 0, lc = sc->L, ld = sd ? sd->L : 0;
+    size_t n = block_cart_size(la, lb, lc, ld);
+    if (n > w->scrsz) {
+        free(w->scr);
+        w->scr = (double *)malloc(sizeof(double) * n);
+        w->scrsz = n;
+    }
+    contracted_eri(sa, sb, sc, sd, out, w);
+    cart2sph4(la, lb, lc, ld, out, w->scr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Public integral blocks                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* (ab|c) spherical block with the reference's consumption order
+ * res[(ia*nb + ib)*nc + ic] (diabatization/intor.f90:155-171). */
+void orc_int3c2e_block(void *bas, void *aux, int a, int b, int c, double *out)
+{
+    orc_basis *B = (orc_basis *)bas, *X = (orc_basis *)aux;
+    orc_work w;
+    work_init(&w, B->sh[a].L + B->sh[b].L + X->sh[c].L);
+    sph_block(&B->sh[a], &B->sh[b], &X->sh[c], NULL, out, &w);
+    work_free(&w);
+}
+
+/* (P|Q) spherical block, out[iP*nQ + iQ]. */
+void orc_int2c2e_block(void *aux, int P, int Q, double *out)
+{
+    orc_basis *X = (orc_basis *)aux;
+    orc_work w;
+    work_init(&w, X->sh[P].L + X->sh[Q].L);
+    sph_block(&X->sh[P], NULL, &X->sh[Q], NULL, out, &w);
+    work_free(&w);
+}
+
+/* (ab|cd) spherical block, out[((ia*nb+ib)*nc+ic)*nd+id]. */
+void orc_int4c2e_block(void *bas, int a, int b, int c, int d, double *out)
+{
+    orc_basis *B = (orc_basis *)bas;
+    orc_work w;
+    work_init(&w, B->sh[a].L + B->sh[b].L + B->sh[c].L + B->sh[d].L);
+    sph_block(&B->sh[a], &B->sh[b], &B->sh[c], &B->sh[d], out, &w);
+    work_free(&w);
+}
+
+static double frob(const double *x, size_t n)
+{
+    double s = 0.0;
+    size_t i;
+    for (i = 0; i < n; i++) s += x[i] * x[i];
+    return sqrt(s);
+}
+
+/* common/integrals/intor.f90:131-160: sqrt(||(ab|ab)||_F) for a >= b, packed
+ * index a*(a+1)/2 + b (0-based). */
+void orc_schwarz_bra(void *bas, double *out)
+{
+    orc_basis *B = (orc_basis *)bas;
+    int a;
+#pragma omp parallel for schedule(dynamic)
+    for (a = 0; a < B->nshells; a++) {
+        orc_work w;
+        int b;
+        work_init(&w, 4 * B->lmax);
+        for (b = 0; b <= a; b++) {
+            int na = 2 * B->sh[a].L + 1, nb = 2 * B->sh[b].L + 1;
+            size_t n = (size_t)na * nb * na * nb;
+            double *blk = (double *)malloc(sizeof(double) * block_cart_size(B->sh[a].L, B->sh[b].L, B->sh[a].L, B->sh[b].L));
+            sph_block(&B->sh[a], &B->sh[b], &B->sh[a], &B->sh[b], blk, &w);
+            out[(size_t)a * (a + 1) / 2 + b] = sqrt(frob(blk, n));
+            free(blk);
+        }
+        work_free(&w);
+    }
+}
+
+/* common/integrals/intor.f90:97-129: sqrt(||(C|C)||_F). */
+void orc_schwarz_aux(void *aux, double *out)
+{
+    orc_basis *X = (orc_basis *)aux;
+    int c;
+#pragma omp parallel for schedule(dynamic)
+    for (c = 0; c < X->nshells; c++) {
+        orc_work w;
+        int n = 2 * X->sh[c].L + 1;
+        double *blk = (double *)malloc(sizeof(double) * block_cart_size(X->sh[c].L, 0, X->sh[c].L, 0));
+        work_init(&w, 2 * X->lmax);
+        sph_block(&X->sh[c], NULL, &X->sh[c], NULL, blk, &w);
+        out[c] = sqrt(frob(blk, (size_t)n * n));
+        free(blk);
+        work_free(&w);
+    }
+}
+
+/* common/integrals/intor.f90:40-95: full symmetric (P|Q), column-major naux x naux. */
+void orc_metric(void *aux, double *M)
+{
+    orc_basis *X = (orc_basis *)aux;
+    int n = X->nbf, a;
+#pragma omp parallel for schedule(dynamic)
+    for (a = 0; a < X->nshells; a++) {
+        orc_work w;
+        int b;
+        work_init(&w, 2 * X->lmax);
+        for (b = a; b < X->nshells; b++) {
+            int na = 2 * X->sh[a].L + 1, nb = 2 * X->sh[b].L + 1, i, j;
+            double *blk = (double *)malloc(sizeof(double) * block_cart_size(X->sh[a].L, 0, X->sh[b].L, 0));
+            sph_block(&X->sh[a], NULL, &X->sh[b], NULL, blk, &w);
+            for (i = 0; i < na; i++)
+                for (j = 0; j < nb; j++) {
+                    int r = X->sh[a].sph_off + i, c = X->sh[b].sph_off + j;
+                    M[(size_t)r + (size_t)n * c] = blk[i * nb + j];
+                    M[(size_t)c + (size_t)n * r] = blk[i * nb + j];
+                }
+            free(blk);
+        }
+        work_free(&w);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* The screened, integral-direct density contraction                         */
+/* diabatization/intor.f90:108-183                                           */
+/* ------------------------------------------------------------------------ */
+
+/* densities: column-major (ndens, nbf, nbf); d: column-major (naux, ndens),
+ * accumulated into (caller zeroes it).  Aux shells c_begin, c_begin+c_step, ...
+ * < c_end are processed (the full loop is 0, nshells_aux, 1).  schwarz_bra /
+ * schwarz_aux are the Schwarz norms; a triplet is skipped when
+ * schwarz_bra[ab] * schwarz_aux[c] < thresh.  Returns the number of skipped
+ * triplets in *nskipped and the computed ones in *ncomputed. */
+void orc_df_contract(void *bas, void *aux, const double *schwarz_bra, const double *schwarz_aux,
+                     double thresh, int ndens, const double *dens, double *d, int c_begin,
+                     int c_end, int c_step, int64_t *nskipped, int64_t *ncomputed)
+{
+    orc_basis *B = (orc_basis *)bas, *X = (orc_basis *)aux;
+    int nbf = B->nbf, naux = X->nbf;
+    int64_t skipped = 0, computed = 0;
+    int c;
+    (void)naux;
+#pragma omp parallel for schedule(dynamic) reduction(+ : skipped, computed)
+    for (c = c_begin; c < c_end; c += c_step) {
+        const orc_shell *sc = &X->sh[c];
+        int nc = 2 * sc->L + 1, a;
+        orc_work w;
+        double *blk = (double *)malloc(sizeof(double) * block_cart_size(B->lmax, B->lmax, X->lmax, 0));
+        work_init(&w, 2 * B->lmax + X->lmax);
+        for (a = 0; a < B->nshells; a++) {
+            const orc_shell *sa = &B->sh[a];
+            int b;
+            for (b = 0; b <= a; b++) {
+                const orc_shell *sb = &B->sh[b];
+                int na = 2 * sa->L + 1, nb = 2 * sb->L + 1, I, i, j, k, idx;
+                if (schwarz_bra[(size_t)a * (a + 1) / 2 + b] * schwarz_aux[c] < thresh) {
+                    skipped++;
+                    continue;
+                }
+                computed++;
+                sph_block(sa, sb, sc, NULL, blk, &w);
+                for (I = 0; I < ndens; I++) {
+                    idx = 0;
+                    for (i = 0; i < na; i++)
+                        for (j = 0; j < nb; j++) {
+                            size_t ii = sa->sph_off + i, jj = sb->sph_off + j;
+                            double Dij = dens[I + (size_t)ndens * (ii + (size_t)nbf * jj)];
+                            double Dji = dens[I + (size_t)ndens * (jj + (size_t)nbf * ii)];
+                            double f = (a != b) ? Dij + Dji : Dij;
+                            for (k = 0; k < nc; k++, idx++)
+                                d[(size_t)(sc->sph_off + k) + (size_t)X->nbf * I] += blk[idx] * f;
+                        }
+                }
+            }
+        }
+        free(blk);
+        work_free(&w);
+    }
+    *nskipped = skipped;
+    *ncomputed = computed;
+}
+
+/* Overlap matrix (spherical, normalized), column-major nbf x nbf, used only to
+ * pin coordinates, normalization and ordering against ORCA's S-matrix. */
+void orc_overlap(void *bas, double *S)
+{
+    orc_basis *B = (orc_basis *)bas;
+    int n = B->nbf, a, b, d;
+    build_sph();
+    for (a = 0; a < B->nshells; a++)
+        for (b = 0; b < B->nshells; b++) {
+            const orc_shell *sa = &B->sh[a], *sb = &B->sh[b];
+            int la = sa->L, lb = sb->L, nca = ORC_NCART(la), ncb = ORC_NCART(lb);
+            double cart[ORC_NCART(ORC_LMAX) * ORC_NCART(ORC_LMAX)];
+            double E[3][ORC_ESZ];
+            int i, j, ia, ib, ma, mb, lx, ly, k;
+            int axs[ORC_NCART(ORC_LMAX)][3], bxs[ORC_NCART(ORC_LMAX)][3];
+            k = 0;
+            for (lx = la; lx >= 0; lx--)
+                for (ly = la - lx; ly >= 0; ly--, k++) { axs[k][0] = lx; axs[k][1] = ly; axs[k][2] = la - lx - ly; }
+            k = 0;
+            for (lx = lb; lx >= 0; lx--)
+                for (ly = lb - lx; ly >= 0; ly--, k++) { bxs[k][0] = lx; bxs[k][1] = ly; bxs[k][2] = lb - lx - ly; }
+            memset(cart, 0, sizeof(cart));
+            for (i = 0; i < sa->nprim; i++)
+                for (j = 0; j < sb->nprim; j++) {
+                    double p = sa->exps[i] + sb->exps[j];
+                    double f = sa->coefs[i] * sb->coefs[j] * pow(M_PI / p, 1.5);
+                    for (d = 0; d < 3; d++) herm_E(la, lb, sa->exps[i], sb->exps[j], sa->A[d], sb->A[d], E[d]);
+                    for (ia = 0; ia < nca; ia++)
+                        for (ib = 0; ib < ncb; ib++) {
+                            int nt = la + lb + 1;
+                            cart[ia * ncb + ib] += f * E[0][(axs[ia][0] * (lb + 1) + bxs[ib][0]) * nt] *
+                                                   E[1][(axs[ia][1] * (lb + 1) + bxs[ib][1]) * nt] *
+                                                   E[2][(axs[ia][2] * (lb + 1) + bxs[ib][2]) * nt];
+                        }
+                }
+            for (ma = 0; ma < 2 * la + 1; ma++)
+                for (mb = 0; mb < 2 * lb + 1; mb++) {
+                    double acc = 0.0;
+                    for (ia = 0; ia < nca; ia++)
+                        for (ib = 0; ib < ncb; ib++)
+                            acc += SPH[la][ma][ia] * SPH[lb][mb][ib] * cart[ia * ncb + ib];
+                    S[(size_t)(sa->sph_off + ma) + (size_t)n * (sb->sph_off + mb)] = acc;
+                }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Screening helpers (common/erfinv.f90, common/screening.f90)               */
+/* ------------------------------------------------------------------------ */
+
+/* Rational Chebyshev approximations of Blair, Edwards & Johnson (1976), the
+ * coefficient tables of their Tables 17, 37 and 57 as used by
+ * common/erfinv.f90:8-78.  These are published constants of that paper. */
+static const double BEJ_P1[7] = {0.160304955844066229311e2, -0.90784959262960326650e2,
+                                 0.18644914861620987391e3, -0.16900142734642382420e3,
+                                 0.6545466284794487048e2, -0.864213011587247794e1,
+                                 0.176058782139590e0};
+static const double BEJ_Q1[7] = {0.147806470715138316110e2, -0.91374167024260313936e2,
+                                 0.21015790486205317714e3, -0.22210254121855132366e3,
+                                 0.10760453916055123830e3, -0.206010730328265443e2, 1.0};
+static const double BEJ_P2[8] = {-0.152389263440726128e-1, 0.3444556924136125216e0,
+                                 -0.29344398672542478687e1, 0.11763505705217627302e2,
+                                 -0.22655292823101104193e2, 0.19121334396580330163e2,
+                                 -0.5478927619598318769e1, 0.237516689024448000e0};
+static const double BEJ_Q2[8] = {-0.108465169602059954e-1, 0.2610628885843078511e0,
+                                 -0.24068318104393757995e1, 0.10695129973387014469e2,
+                                 -0.23716715521596581025e2, 0.24640158943917284883e2,
+                                 -0.10014376349783070835e2, 1.0};
+static const double BEJ_P3[9] = {0.3100808562552958465e-4, 0.40974876030119399241e-2,
+                                 0.1214902662897276161664e0, 0.11091676946390282026357e1,
+                                 0.32283798556639239484387e1, 0.2881691815651598826211e1,
+                                 0.2047972087262996049729e1, 0.854592208197214832806e0,
+                                 0.3551095884622383139e-2};
+static const double BEJ_Q3[9] = {0.3100809298564522487e-4, 0.40975286786639145041e-2,
+                                 0.1215907800748757143068e0, 0.11186271676316964210428e1,
+                                 0.34323639843052901805054e1, 0.4140284677116202150554e1,
+                                 0.4119797271272204121285e1, 0.2162961962641434560920e1, 1.0};
+
+static double bej_ratio(double x, double base, const double *P, const double *Q, int n)
+{
+    double num = 0.0, den = 0.0, pw = 1.0;
+    int i;
+    for (i = 0; i < n; i++) { num += P[i] * pw; den += Q[i] * pw; pw *= base; }
+    return x * num / den;
+}
+
+double orc_erfinv(double x)
+{
+    double ax = fabs(x);
+    if (ax <= 0.75) return bej_ratio(x, x * x - 0.5625, BEJ_P1, BEJ_Q1, 7);
+    if (ax <= 0.9375) return bej_ratio(x, x * x - 0.87890625, BEJ_P2, BEJ_Q2, 8);
+    {
+        double base = 1.0 / sqrt(-log(1.0 - ax));
+        return bej_ratio((x < 0 ? -1.0 : 1.0) / base, base, BEJ_P3, BEJ_Q3, 9);
+    }
+}
+
+/* common/screening.f90:235-265: |d_a d_b|-weighted mean of primitive centres. */
+void orc_contracted_center(int na, const double *axs, const double *das, const double *A, int nb,
+                           const double *bxs, const double *dbs, const double *B, double *R)
+{
+    double den = 0.0;
+    int i, j, k;
+    R[0] = R[1] = R[2] = 0.0;
+    for (i = 0; i < na; i++)
+        for (j = 0; j < nb; j++) {
+            double px = axs[i] + bxs[j], w = fabs(das[i] * dbs[j]);
+            for (k = 0; k < 3; k++) R[k] += w * (axs[i] * A[k] + bxs[j] * B[k]) / px;
+            den += w;
+        }
+    for (k = 0; k < 3; k++) R[k] /= den;
+}
+
+/* common/screening.f90:198-232 (Maurer et al. 2012, eqs. B1-B4).  The
+ * reference evaluates erfinv(1.0 - thresh) with the single-precision literal
+ * 1.0, i.e. in double precision since thresh is double. */
+double orc_contracted_extent(int na, const double *axs, const double *das, const double *A,
+                             int nb, const double *bxs, const double *dbs, const double *B,
+                             double thresh_ws)
+{
+    double erfi = orc_erfinv(1.0 - thresh_ws), R[3], ext = 0.0;
+    int i, j, k;
+    orc_contracted_center(na, axs, das, A, nb, bxs, dbs, B, R);
+    for (i = 0; i < na; i++)
+        for (j = 0; j < nb; j++) {
+            double px = axs[i] + bxs[j], dist = 0.0, tmp = sqrt(2.0 / px) * erfi;
+            for (k = 0; k < 3; k++) {
+                double Pk = (axs[i] * A[k] + bxs[j] * B[k]) / px;
+                dist += (R[k] - Pk) * (R[k] - Pk);
+            }
+            dist = sqrt(dist);
+            if (tmp + dist > ext) ext = tmp + dist;
+        }
+    return ext;
+}
+
+/* common/screening.f90:168-196 */
+double orc_auxiliary_extent(int n, const double *exps, double prec)
+{
+    double ext = 0.0, pt = -log(prec);
+    int i;
+    for (i = 0; i < n; i++) {
+        double arg = (pt + 0.5 * log(exps[i])) / exps[i];
+        double e = arg < 0 ? 0.01 : sqrt(arg);
+        if (e > ext) ext = e;
+    }
+    return ext;
+}
+
+int orc_abi_version(void) { return 1; }
+
+/* Primitive Cartesian [ab|cd] over unnormalized monomials x^lx y^ly z^lz
+ * e^{-a r_A^2} (Cartesian order as cart_index), for checking the Hermite
+ * machinery against an independent quadrature.  ld = 0 with d = 0 and D = C
+ * gives a single-centre ket. */
+void orc_prim_cart_eri(int la, int lb, int lc, int ld, double a, double b, double c, double d,
+                       const double *A, const double *B, const double *C, const double *D,
+                       double *out)
+{
+    orc_side bra, ket;
+    orc_work w;
+    size_t n = block_cart_size(la, lb, lc, ld);
+    bra.la = la; bra.lb = lb; bra.a = a; bra.b = b;
+    ket.la = lc; ket.lb = ld; ket.a = c; ket.b = d;
+    memcpy(bra.A, A, sizeof(bra.A)); memcpy(bra.B, B, sizeof(bra.B));
+    memcpy(ket.A, C, sizeof(ket.A)); memcpy(ket.B, D, sizeof(ket.B));
+    work_init(&w, la + lb + lc + ld);
+    memset(out, 0, sizeof(double) * n);
+    prim_eri(&bra, &ket, 1.0, out, &w);
+    work_free(&w);
+}

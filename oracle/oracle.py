@@ -1,0 +1,263 @@
+"""FP64 CPU oracle for the density-fitting Coulomb path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker and the CPU baseline.  The
+product never imports it.
+
+The integrals come from oracle.c (McMurchie-Davidson on all centres plus a
+Cartesian->spherical transform, see its header).  This module restates the
+remaining steps of the reference driver:
+
+* ``inv_chol``: LAPACK dpotrf('L') then dtrtri('L','N')  (common/linalg.f90:60-82)
+* ``dtrmm('L','L','N','N')``: B = L^-1 d             (diabatization/intor.f90:198)
+* the 8-fold Coulomb-tensor assembly                 (diabatization/intor.f90:232-269)
+* the Schwarz skip rule with the single-precision literal 1e-10
+                                                     (diabatization/intor.f90:140)
+"""
+
+import ctypes
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+
+# diabatization/intor.f90:140 compares a double against the real(4) literal 1e-10
+SCHWARZ_THRESH = float(np.float32(1e-10))
+
+_lib = None
+
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_F64F = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.orc_basis_new.restype = vp
+        L.orc_basis_new.argtypes = [ci, _I32P, _I32P, _F64P]
+        L.orc_basis_free.argtypes = [vp]
+        L.orc_basis_nbf.argtypes = [vp]
+        L.orc_basis_lmax.argtypes = [vp]
+        L.orc_basis_shell_off.argtypes = [vp, ci]
+        L.orc_basis_shell_L.argtypes = [vp, ci]
+        L.orc_basis_normed_coefs.argtypes = [vp, ci, _F64P]
+        L.orc_boys.argtypes = [ci, ci, _F64P, _F64P]
+        L.orc_sph_coeffs.argtypes = [ci, _F64P]
+        L.orc_int3c2e_block.argtypes = [vp, vp, ci, ci, ci, _F64P]
+        L.orc_int2c2e_block.argtypes = [vp, ci, ci, _F64P]
+        L.orc_int4c2e_block.argtypes = [vp, ci, ci, ci, ci, _F64P]
+        L.orc_schwarz_bra.argtypes = [vp, _F64P]
+        L.orc_schwarz_aux.argtypes = [vp, _F64P]
+        L.orc_metric.argtypes = [vp, _F64F]
+        L.orc_overlap.argtypes = [vp, _F64F]
+        L.orc_df_contract.argtypes = [vp, vp, _F64P, _F64P, cd, ci, _F64F, _F64F, ci, ci, ci,
+                                      ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64)]
+        L.orc_erfinv.restype = cd
+        L.orc_erfinv.argtypes = [cd]
+        L.orc_contracted_center.argtypes = [ci, _F64P, _F64P, _F64P, ci, _F64P, _F64P, _F64P,
+                                            _F64P]
+        L.orc_contracted_extent.restype = cd
+        L.orc_contracted_extent.argtypes = [ci, _F64P, _F64P, _F64P, ci, _F64P, _F64P, _F64P, cd]
+        L.orc_auxiliary_extent.restype = cd
+        L.orc_auxiliary_extent.argtypes = [ci, _F64P, cd]
+        L.orc_prim_cart_eri.argtypes = [ci, ci, ci, ci, cd, cd, cd, cd, _F64P, _F64P, _F64P,
+                                        _F64P, _F64P]
+        _lib = L
+    return _lib
+
+
+def ncart(L):
+    return (L + 1) * (L + 2) // 2
+
+
+class Basis:
+    """Oracle-side shell table built from the reference basis arrays."""
+
+    def __init__(self, bas_centers, bas_spec, bas_data):
+        self.bas_centers = np.asfortranarray(bas_centers, dtype=np.int32)
+        self.bas_spec = np.asfortranarray(bas_spec, dtype=np.int32)
+        self.bas_data = np.ascontiguousarray(bas_data, dtype=np.float64)
+        self.nshells = len(self.bas_centers)
+        self.h = lib().orc_basis_new(self.nshells, self.bas_centers, self.bas_spec, self.bas_data)
+        self.nbf = lib().orc_basis_nbf(self.h)
+        self.lmax = lib().orc_basis_lmax(self.h)
+        self.L = np.array([lib().orc_basis_shell_L(self.h, i) for i in range(self.nshells)])
+        self.off = np.array([lib().orc_basis_shell_off(self.h, i) for i in range(self.nshells)])
+
+    @classmethod
+    def from_shells(cls, shells):
+        return cls(*shells.as_arrays(fortran=True))
+
+    def __del__(self):
+        try:
+            lib().orc_basis_free(self.h)
+        except Exception:
+            pass
+
+
+def boys(nmax, T):
+    T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+    F = np.zeros((T.size, nmax + 1))
+    lib().orc_boys(nmax, T.size, T, F)
+    return F
+
+
+def sph_coeffs(L):
+    out = np.zeros((2 * L + 1, ncart(L)))
+    lib().orc_sph_coeffs(L, out)
+    return out
+
+
+def int3c2e_block(bas, aux, a, b, c):
+    na, nb, nc = 2 * bas.L[a] + 1, 2 * bas.L[b] + 1, 2 * aux.L[c] + 1
+    out = np.zeros(ncart(bas.L[a]) * ncart(bas.L[b]) * ncart(aux.L[c]))
+    lib().orc_int3c2e_block(bas.h, aux.h, a, b, c, out)
+    return out[: na * nb * nc].reshape(na, nb, nc)
+
+
+def int2c2e_block(aux, P, Q):
+    n1, n2 = 2 * aux.L[P] + 1, 2 * aux.L[Q] + 1
+    out = np.zeros(ncart(aux.L[P]) * ncart(aux.L[Q]))
+    lib().orc_int2c2e_block(aux.h, P, Q, out)
+    return out[: n1 * n2].reshape(n1, n2)
+
+
+def int4c2e_block(bas, a, b, c, d):
+    dims = [2 * bas.L[i] + 1 for i in (a, b, c, d)]
+    out = np.zeros(int(np.prod([ncart(bas.L[i]) for i in (a, b, c, d)])))
+    lib().orc_int4c2e_block(bas.h, a, b, c, d, out)
+    return out[: int(np.prod(dims))].reshape(dims)
+
+
+def prim_cart_eri(la, lb, lc, ld, a, b, c, d, A, B, C, D):
+    out = np.zeros(ncart(la) * ncart(lb) * ncart(lc) * ncart(ld))
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_prim_cart_eri(la, lb, lc, ld, a, b, c, d, f(A), f(B), f(C), f(D), out)
+    return out.reshape(ncart(la), ncart(lb), ncart(lc), ncart(ld))
+
+
+def schwarz_bra(bas):
+    out = np.zeros(bas.nshells * (bas.nshells + 1) // 2)
+    lib().orc_schwarz_bra(bas.h, out)
+    return out
+
+
+def schwarz_aux(aux):
+    out = np.zeros(aux.nshells)
+    lib().orc_schwarz_aux(aux.h, out)
+    return out
+
+
+def metric(aux):
+    M = np.zeros((aux.nbf, aux.nbf), order="F")
+    lib().orc_metric(aux.h, M)
+    return M
+
+
+def overlap(bas):
+    S = np.zeros((bas.nbf, bas.nbf), order="F")
+    lib().orc_overlap(bas.h, S)
+    return S
+
+
+def df_contract(bas, aux, densities, qbra=None, qaux=None, thresh=SCHWARZ_THRESH,
+                c_begin=0, c_end=None, c_step=1):
+    """Screened integral-direct d(P, I) = sum (mu nu|P) D_I(mu, nu), the loop of
+    diabatization/intor.f90:108-183.  densities is (ndens, nbf, nbf) in any
+    memory order.  Returns (d (naux, ndens) F-order, nskipped, ncomputed)."""
+    if qbra is None:
+        qbra = schwarz_bra(bas)
+    if qaux is None:
+        qaux = schwarz_aux(aux)
+    dens = np.asfortranarray(densities, dtype=np.float64)
+    ndens = dens.shape[0]
+    d = np.zeros((aux.nbf, ndens), order="F")
+    ns, nc = ctypes.c_int64(0), ctypes.c_int64(0)
+    if c_end is None:
+        c_end = aux.nshells
+    lib().orc_df_contract(bas.h, aux.h, np.ascontiguousarray(qbra), np.ascontiguousarray(qaux),
+                          thresh, ndens, dens, d, c_begin, c_end, c_step, ctypes.byref(ns),
+                          ctypes.byref(nc))
+    return d, ns.value, nc.value
+
+
+def inv_chol(M):
+    """common/linalg.f90:60-82: L^-1 in the lower triangle (upper keeps M)."""
+    from scipy.linalg import lapack
+
+    c, info = lapack.dpotrf(np.asfortranarray(M), lower=1, clean=0)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"dpotrf info={info}")
+    inv, info = lapack.dtrtri(c, lower=1, unitdiag=0)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"dtrtri info={info}")
+    return inv
+
+
+def dtrmm_lower(Linv, d):
+    """dtrmm('L','L','N','N', naux, ndens, 1, Linv, naux, d, naux)."""
+    from scipy.linalg import blas
+
+    return blas.dtrmm(1.0, np.asfortranarray(Linv), np.asfortranarray(d), side=0, lower=1,
+                      trans_a=0, diag=0)
+
+
+def contract_coulomb_densities_2d(shells, shells_aux, densities, thresh=SCHWARZ_THRESH,
+                                  return_info=False):
+    bas = Basis.from_shells(shells)
+    aux = Basis.from_shells(shells_aux)
+    d, nskip, ncomp = df_contract(bas, aux, densities, thresh=thresh)
+    M = metric(aux)
+    Linv = inv_chol(M)
+    B = dtrmm_lower(Linv, d)
+    if return_info:
+        return B, dict(nskipped=nskip, ncomputed=ncomp, d=d, metric=M)
+    return B
+
+
+def assemble_4d(B, nstates):
+    """diabatization/intor.f90:241-269, 8-fold symmetric J from packed B."""
+    J = np.zeros((nstates,) * 4, order="F")
+    d1 = 0
+    for i in range(nstates):
+        for j in range(i + 1):
+            d2 = 0
+            for k in range(i + 1):
+                lmax = j if k == i else k
+                for l in range(lmax + 1):
+                    val = float(B[:, d1] @ B[:, d2])
+                    for idx in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                                (k, l, i, j), (k, l, j, i), (l, k, i, j), (l, k, j, i)):
+                        J[idx] = val
+                    d2 += 1
+            d1 += 1
+    return J
+
+
+def contract_coulomb_densities_4d(shells, shells_aux, densities, thresh=SCHWARZ_THRESH,
+                                  return_info=False):
+    ndens = densities.shape[0]
+    nstates = int(math.isqrt(8 * ndens + 1) / 2.0 - 0.5)
+    assert nstates * (nstates + 1) // 2 == ndens
+    B, info = contract_coulomb_densities_2d(shells, shells_aux, densities, thresh,
+                                            return_info=True)
+    J = assemble_4d(B, nstates)
+    if return_info:
+        info["B"] = B
+        return J, info
+    return J

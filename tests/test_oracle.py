@@ -1,0 +1,189 @@
+"""Pins the FP64 CPU oracle (oracle/) before anything is compared against it.
+
+Every check here is independent of the oracle's own formulas: mpmath for the
+Boys function, closed forms for s-type integrals, a recurrence-free
+Gaussian-transform quadrature for primitive Cartesian integrals, rotation
+invariance of spherical shell blocks, the reference's own KATs
+(tests_fortran/test_erfinv.f90:25-29, tests_fortran/test_screening.f90:28-64),
+ORCA's overlap matrix and the reference's Coulomb-tensor golden values
+(tests/test_coulomb_tensor/test_coulomb_tensor.py:17-32).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pysisyphus_addons_amd import synthetic
+from pysisyphus_addons_amd.shells import Shell, Shells
+from quadrature import prim_cart_eri as quad_eri
+
+
+def test_boys_vs_mpmath():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(1)
+    T = np.concatenate([[0.0, 1e-12, 1e-6, 0.5, 26.99, 27.0, 44.99, 45.0, 120.0],
+                        np.exp(rng.uniform(np.log(1e-6), np.log(120.0), 60))])
+    nmax = 24
+    F = O.boys(nmax, T)
+    for i, t in enumerate(T):
+        for n in range(0, nmax + 1, 3):
+            if t == 0.0:
+                ref = mpmath.mpf(1) / (2 * n + 1)
+            else:
+                ref = mpmath.gammainc(n + mpmath.mpf(0.5), 0, t) / (2 * mpmath.mpf(t) ** (n + 0.5))
+            assert abs(F[i, n] - float(ref)) <= 2e-15 * float(ref) + 1e-300, (t, n)
+
+
+def test_primitive_ss_s_closed_form():
+    a, c = 1.3, 0.7
+    A = np.zeros(3)
+    C = np.array([0.1, -0.4, 0.3])
+    v = O.prim_cart_eri(0, 0, 0, 0, a, 0.0, c, 0.0, A, A, C, C)[0, 0, 0, 0]
+    T = a * c / (a + c) * C @ C
+    F0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    assert v == pytest.approx(2 * math.pi ** 2.5 / (a * c * math.sqrt(a + c)) * F0, rel=1e-14)
+
+
+@pytest.mark.parametrize(
+    "ls",
+    [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (2, 1, 1, 0), (2, 2, 2, 0), (1, 0, 3, 0),
+     (1, 1, 1, 1), (2, 0, 1, 1), (0, 0, 4, 0), (3, 1, 2, 0)],
+)
+def test_hermite_machinery_vs_quadrature(ls):
+    la, lb, lc, ld = ls
+    rng = np.random.default_rng(sum(ls) * 7 + la)
+    a, b, c, d = rng.uniform(0.3, 2.0, 4)
+    A, B, C, D = (rng.uniform(-1.0, 1.0, 3) for _ in range(4))
+    if ld == 0:
+        d, D = 0.0, C
+    ref = quad_eri(la, lb, lc, ld, a, b, c, d, A, B, C, D)
+    got = O.prim_cart_eri(la, lb, lc, ld, a, b, c, d, A, B, C, D)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_sph_coeffs_orthonormal():
+    """Overlap of a normalized single-primitive shell with itself is 1."""
+    for L in range(7):
+        sh = Shells([Shell(L=L, center=[0.1, 0.2, 0.3], exps=[0.8], coeffs=[1.0])])
+        S = O.overlap(O.Basis.from_shells(sh))
+        np.testing.assert_allclose(S, np.eye(2 * L + 1), atol=1e-13)
+
+
+def _rot(seed):
+    q, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
+    return q
+
+
+def _rotate(shells, R, shift):
+    return Shells([Shell(L=s.L, center=R @ np.asarray(s.center) + shift, exps=s.exps,
+                         coeffs=s.coeffs) for s in shells.shells])
+
+
+def test_rotation_invariance_of_block_norms():
+    rng = np.random.default_rng(5)
+    orb = Shells([Shell(L=L, center=rng.uniform(-1, 1, 3), exps=[0.5, 1.7], coeffs=[0.6, 0.4])
+                  for L in (3, 4, 2)])
+    aux = Shells([Shell(L=L, center=rng.uniform(-1, 1, 3), exps=[0.9], coeffs=[1.0])
+                  for L in (5, 4, 3, 0)])
+    R, shift = _rot(3), np.array([0.3, -1.2, 2.0])
+    b0, x0 = O.Basis.from_shells(orb), O.Basis.from_shells(aux)
+    b1, x1 = O.Basis.from_shells(_rotate(orb, R, shift)), O.Basis.from_shells(_rotate(aux, R, shift))
+    for (a, b) in [(0, 1), (1, 1), (2, 0)]:
+        for c in range(4):
+            n0 = np.linalg.norm(O.int3c2e_block(b0, x0, a, b, c))
+            n1 = np.linalg.norm(O.int3c2e_block(b1, x1, a, b, c))
+            assert n1 == pytest.approx(n0, rel=1e-12)
+    for (P, Q) in [(0, 0), (0, 1), (1, 2), (3, 0)]:
+        n0 = np.linalg.norm(O.int2c2e_block(x0, P, Q))
+        n1 = np.linalg.norm(O.int2c2e_block(x1, P, Q))
+        assert n1 == pytest.approx(n0, rel=1e-12)
+    n0 = np.linalg.norm(O.int4c2e_block(b0, 0, 1, 0, 1))
+    n1 = np.linalg.norm(O.int4c2e_block(b1, 0, 1, 0, 1))
+    assert n1 == pytest.approx(n0, rel=1e-12)
+
+
+def test_block_symmetries_and_schwarz_bound():
+    shells, aux_sh, _ = synthetic.molecule(3, 4.0, seed=11)
+    bas, aux = O.Basis.from_shells(shells), O.Basis.from_shells(aux_sh)
+    qb, qa = O.schwarz_bra(bas), O.schwarz_aux(aux)
+    for a in range(bas.nshells):
+        for b in range(a + 1):
+            for c in range(0, aux.nshells, 3):
+                blk = O.int3c2e_block(bas, aux, a, b, c)
+                blk_t = O.int3c2e_block(bas, aux, b, a, c)
+                np.testing.assert_allclose(blk, blk_t.transpose(1, 0, 2), atol=1e-13)
+                # Schwarz: |(ab|C)| <= sqrt(||(ab|ab)||) sqrt(||(C|C)||) elementwise
+                assert np.abs(blk).max() <= qb[a * (a + 1) // 2 + b] * qa[c] * (1 + 1e-12)
+    M = O.metric(aux)
+    np.testing.assert_allclose(M, M.T, atol=0)
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+def test_schwarz_definitions():
+    """sqrt of Frobenius norms of the diagonal blocks
+    (tests/test_intor/test_schwarz_integrals.py:49-86)."""
+    shells, aux_sh, _ = synthetic.molecule(2, 3.0, seed=3)
+    bas, aux = O.Basis.from_shells(shells), O.Basis.from_shells(aux_sh)
+    qb, qa = O.schwarz_bra(bas), O.schwarz_aux(aux)
+    for a in range(bas.nshells):
+        for b in range(a + 1):
+            blk = O.int4c2e_block(bas, a, b, a, b)
+            assert qb[a * (a + 1) // 2 + b] == pytest.approx(np.sqrt(np.linalg.norm(blk)), rel=1e-14)
+    for c in range(aux.nshells):
+        assert qa[c] == pytest.approx(np.sqrt(np.linalg.norm(O.int2c2e_block(aux, c, c))), rel=1e-14)
+
+
+def test_erfinv_kats():
+    """tests_fortran/test_erfinv.f90:25-29 (the +-0.9 values by exact equality)."""
+    L = O.lib()
+    assert L.orc_erfinv(0.0) == 0.0
+    assert L.orc_erfinv(0.9) == 1.1630871536766727
+    assert L.orc_erfinv(-0.9) == -1.1630871536766727
+    assert L.orc_erfinv(0.5) == pytest.approx(0.4769362762044698, abs=1e-15)
+    assert L.orc_erfinv(-0.5) == pytest.approx(-0.4769362762044698, abs=1e-15)
+    from scipy.special import erfinv
+
+    for x in (0.1, 0.74, 0.76, 0.93, 0.95, 0.9999):
+        assert L.orc_erfinv(x) == pytest.approx(erfinv(x), rel=1e-14)
+
+
+def _center(axs, das, A, bxs, dbs, B):
+    R = np.zeros(3)
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    O.lib().orc_contracted_center(len(axs), f(axs), f(das), f(A), len(bxs), f(bxs), f(dbs), f(B), R)
+    return R
+
+
+def test_contracted_center_kats():
+    """tests_fortran/test_screening.f90:28-64."""
+    A, B = np.zeros(3), np.ones(3)
+    R = _center([1.0], [1.0], A, [1.0], [1.0], A)
+    assert np.linalg.norm(R - A) <= 1e-14
+    axs = [1.0, 0.5]
+    R = _center(axs, axs, A, axs, axs, B)
+    assert np.linalg.norm(R - (A + B) / 2) <= 1e-14
+    R = _center([1.0, 0.5, 0.25], [1.0, 0.5, 0.25], A, [1.0, 0.5], [1.0, 0.5], B)
+    assert np.linalg.norm(R - 0.53650793650794) <= 1e-14
+
+
+def test_overlap_vs_orca(bena2):
+    bas = O.Basis(bena2["bas_centers"], bena2["bas_spec"], bena2["bas_data_orca"])
+    S = O.overlap(bas)
+    # ORCA's own elements are good to ~1e-10 (its smallest elements carry
+    # relative noise); normalization, ordering and geometry are pinned.
+    np.testing.assert_allclose(S, bena2["S"], atol=1e-9)
+    D = bena2["densities"]
+    assert np.trace(D[0] @ S) == pytest.approx(24.0, abs=1e-9)
+    assert np.trace(D[2] @ S) == pytest.approx(24.0, abs=1e-9)
+
+
+def test_bena2_coulomb_tensor_golden(bena2):
+    """The reference's only full-path golden, at 1e-10 absolute (values ~345;
+    the reference asserts 1e-12, below what two FP64 implementations agree to)."""
+    shells = Shells.from_arrays(bena2["bas_centers"], bena2["bas_spec"], bena2["bas_data"])
+    J = O.contract_coulomb_densities_4d(shells, shells, bena2["densities"])
+    assert J.shape == (2, 2, 2, 2)
+    np.testing.assert_allclose(J, bena2["J_ref"], atol=1e-10)
