@@ -1,0 +1,152 @@
+"""Benchmark of the density-fitting Coulomb path on one MI355X.
+
+A step is one complete Coulomb-tensor evaluation (Schwarz norms, metric,
+Cholesky, screened 3c2e integrals contracted with the densities, B = L^-1 d,
+the 8-fold J assembly) for the 60-atom workload (22 bohr cube, orbital
+3s2p1d, aux 4s3p2d1f, 8 states = 36 densities, FP64), with basis and
+densities resident in HBM when the timed region starts.
+
+Prints one JSON line: value = screened shell-triplets per second of the
+whole step; roofline of the 3c2e kernels (algorithmic FP64 flops from
+pysisyphus_addons_amd.flops over their HIP-event time on the library stream);
+cpu_baseline = the FP64 oracle's integral-direct loop on a sample of the same
+workload on this host's cores.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "pysisyphus-addons_amd"))
+
+METRIC = "screened 3c2e shell-triplets/s + J-tensor time: mixed vs FP64 GPU at ≤1e-8 Eh"
+WORKLOAD = "60 atoms in a 22 bohr cube, orbital 3s2p1d (3-prim), aux 4s3p2d1f, 8 states, FP64"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_baseline(shells, aux, dens, step):
+    """Oracle integral-direct loop over every `step`-th aux shell."""
+    threads = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    bas, x = O.Basis.from_shells(shells), O.Basis.from_shells(aux)
+    qb, qa = O.schwarz_bra(bas), O.schwarz_aux(x)
+    t0 = time.perf_counter()
+    _, nskip, ncomp = O.df_contract(bas, x, dens, qbra=qb, qaux=qa, c_begin=0, c_end=x.nshells,
+                                    c_step=step)
+    dt = time.perf_counter() - t0
+    return {
+        "value": ncomp / dt,
+        "unit": "screened shell-triplets/s",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
+        "kind": "port",
+        "sample": f"every {step}th aux shell ({ncomp} screened triplets, {dt:.1f} s) of the same "
+                  "workload, 3c2e + density contraction loop",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-step", type=int, default=6, help="aux-shell stride of the CPU sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards over auxiliary shells, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pysisyphus_addons_amd import _native as N
+    from pysisyphus_addons_amd import flops, synthetic
+
+    shells, aux, dens = synthetic.workload(1)
+    ndens, nbf = dens.shape[0], dens.shape[1]
+    nstates = 8
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    dev = torch.device("cuda:0")
+    dens_d = torch.from_numpy(dens).to(dev)
+    torch.cuda.synchronize()
+    opt = N.default_options(dens_layout=N.DENS_C, dens_on_device=1)
+    J = np.zeros((nstates,) * 4, order="F")
+    st = N.Stats()
+
+    def step():
+        s = N.lib().dfhip_contract_coulomb_densities(
+            ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, dens_d.data_ptr(),
+            ctypes.byref(opt), None, nstates, J.ctypes.data, ctypes.byref(st))
+        N.check(s)
+        return st.as_dict()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        recs.append(step())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    r0 = recs[-1]
+    ncomp = r0["ncomputed"]
+    avg = lambda k: float(np.mean([r[k] for r in recs]))
+    ms_int = avg("ms_int3c2e")
+    alg_flops = flops.total_flops(r0["class_prim_triplets"])
+    achieved = alg_flops / (ms_int * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": ncomp / (ms_per_step * 1e-3),
+        "unit": "screened shell-triplets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": WORKLOAD,
+            "nbf": nbf,
+            "naux": aux.nbf,
+            "ndens": ndens,
+            "screened_triplets": ncomp,
+            "triplets": r0["ntriplets"],
+            "precision": "fp64",
+            "j_tensor_ms": ms_per_step,
+            "int3c2e_screened_triplets_per_s": ncomp / ((ms_int + avg("ms_contract")) * 1e-3),
+            "phases_ms": {k[3:]: avg(k) for k in r0 if k.startswith("ms_")},
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "int3c2e_kernel (all classes of one step)",
+            "flops_per_launch": alg_flops,
+            "ms_per_launch": ms_int,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(shells, aux, dens, args.cpu_step)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

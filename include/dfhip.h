@@ -1,0 +1,152 @@
+/*
+ * dfhip.h -- C ABI of libdfhip.so, the MI355X (gfx950) density-fitting
+ * Coulomb library.  Drop-in for the native symbols pysisyphus-addons'
+ * Cython layer binds, plus extended entry points.
+ *
+ * Basis arrays follow the reference ABI (common/shells.f90:97-110):
+ *   bas_centers  int32 column-major (nshells, 3): centre index, atomic
+ *                number, 1-based pointer to x, y, z (bohr) in bas_data
+ *   bas_spec     int32 column-major (nshells, 5): shell index, L, number of
+ *                primitives, number of contractions, 1-based pointer to the
+ *                contraction coefficients (exponents follow) in bas_data
+ *   bas_data     float64
+ * Spherical functions are ordered shell by shell, m = -l..l inside a shell.
+ *
+ * The reference's entry points return nothing and abort on fatal errors;
+ * here every call records a status readable with dfhip_last_status() and
+ * dfhip_last_error() (0 = success), and never aborts the process.
+ */
+#ifndef DFHIP_H
+#define DFHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference-compatible entry points -------------------------------- */
+
+/* One-time initialization of tables and device state.
+ * Replaces f_init, common/integrals/init.f90:13-27. */
+void f_init(void);
+
+/* B = L^-1 d with d(P, I) = sum_{mu nu} (mu nu|P) D_I(mu, nu), Schwarz-screened
+ * at the single-precision literal 1e-10, L the Cholesky factor of (P|Q).
+ * densities: column-major (ndens, nbfs, nbfs), packed lower-triangular state
+ * pairs (1,1),(2,1),(2,2),...; df_tensor: column-major (naux, ndens), written.
+ * naux must equal the number of spherical auxiliary functions.
+ * Replaces f_contract_coulomb_densities_2d, diabatization/intor_wrapper.f90:12-43
+ * (bound in diabatization/_intor.pyx:11-31). */
+void f_contract_coulomb_densities_2d(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
+                                     const int32_t *bas_spec, const double *bas_data,
+                                     int32_t nshells_aux, int32_t ndata_aux,
+                                     const int32_t *bas_centers_aux, const int32_t *bas_spec_aux,
+                                     const double *bas_data_aux, int32_t ndens, int32_t nbfs,
+                                     const double *densities, int32_t naux, double *df_tensor);
+
+/* Coulomb tensor J_IJKL = sum_P B(P, IJ) B(P, KL), column-major (nstates)^4,
+ * all 8 permutations written.
+ * Replaces f_contract_coulomb_densities_4d, diabatization/intor_wrapper.f90:45-76
+ * (bound in diabatization/_intor.pyx:32-52). */
+void f_contract_coulomb_densities_4d(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
+                                     const int32_t *bas_spec, const double *bas_data,
+                                     int32_t nshells_aux, int32_t ndata_aux,
+                                     const int32_t *bas_centers_aux, const int32_t *bas_spec_aux,
+                                     const double *bas_data_aux, int32_t ndens, int32_t nbfs,
+                                     const double *densities, int32_t nstates,
+                                     double *coulomb_tensor);
+
+/* integrals[c] = sqrt(||(C|C)||_F) per shell.
+ * Replaces f_int_schwarz_aux, intor/intor_wrapper.f90:12-24. */
+void f_int_schwarz_aux(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
+                       const int32_t *bas_spec, const double *bas_data, double *integrals);
+
+/* integrals[a(a+1)/2 + b] = sqrt(||(ab|ab)||_F) for a >= b (0-based).
+ * Replaces f_int_schwarz_bra, intor/intor_wrapper.f90:26-38. */
+void f_int_schwarz_bra(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
+                       const int32_t *bas_spec, const double *bas_data, double *integrals);
+
+/* ---- status ------------------------------------------------------------ */
+
+enum {
+    DFHIP_OK = 0,
+    DFHIP_ERR_ARG = 1,       /* invalid argument or basis data */
+    DFHIP_ERR_HIP = 2,       /* HIP runtime error / no device */
+    DFHIP_ERR_NOT_SPD = 3,   /* metric not positive definite */
+    DFHIP_ERR_UNSUPPORTED = 4,
+    DFHIP_ERR_MEMORY = 5
+};
+
+/* Status of the last call on this thread and its message. */
+int dfhip_last_status(void);
+const char *dfhip_last_error(void);
+
+/* ---- extended entry points (no reference counterpart) ------------------ */
+
+typedef struct {
+    int32_t nshells, ndata;
+    const int32_t *bas_centers; /* column-major (nshells, 3) */
+    const int32_t *bas_spec;    /* column-major (nshells, 5) */
+    const double *bas_data;
+} dfhip_basis;
+
+enum { DFHIP_FP64 = 0, DFHIP_MIXED = 1 };
+enum { DFHIP_SCREEN_SCHWARZ = 0, DFHIP_SCREEN_NONE = 1 };
+enum { DFHIP_DENS_FORTRAN = 0, DFHIP_DENS_C = 1 };
+
+typedef struct {
+    int32_t precision;   /* DFHIP_FP64 */
+    int32_t screening;   /* DFHIP_SCREEN_SCHWARZ (reference) or _NONE */
+    double thresh;       /* skip when Q_ab Q_C < thresh; default (double)1e-10f */
+    int32_t dens_layout; /* DFHIP_DENS_FORTRAN: (ndens,nbf,nbf) column-major,
+                            DFHIP_DENS_C: (ndens,nbf,nbf) row-major */
+    int32_t verbose;     /* print sizes, phase times and screening counts */
+    int64_t max_tile_bytes; /* cap of the (ab|P) staging matrix, 0 = automatic */
+    int32_t dens_on_device; /* densities is a device pointer (HBM-resident input) */
+    int32_t reserved;
+} dfhip_options;
+
+typedef struct {
+    int64_t ntriplets;   /* nshells(nshells+1)/2 * nshells_aux */
+    int64_t ncomputed;   /* screened-in triplets */
+    int64_t nskipped;
+    int64_t nbf, naux, npair_functions;
+    int32_t ncolumn_blocks;
+    /* phase times in ms (HIP events on the library stream) */
+    double ms_upload, ms_schwarz, ms_metric, ms_cholesky, ms_int3c2e, ms_pack, ms_contract,
+        ms_solve, ms_assemble, ms_download, ms_total;
+    double ms_h2d_densities; /* part of ms_pack spent copying host densities */
+    /* screened-in triplets and primitive triplets per class, index
+       (la*(la+1)/2 + lb) * 6 + lc with la >= lb the pair's angular momenta */
+    int64_t class_triplets[15 * 6];
+    int64_t class_prim_triplets[15 * 6];
+} dfhip_stats;
+
+void dfhip_default_options(dfhip_options *opt);
+
+/* Full path with options and statistics.  df_tensor (column-major naux x
+ * ndens) and coulomb_tensor (column-major nstates^4) may each be NULL.
+ * Returns the status. */
+int dfhip_contract_coulomb_densities(const dfhip_basis *bas, const dfhip_basis *aux, int32_t ndens,
+                                     int32_t nbfs, const double *densities,
+                                     const dfhip_options *opt, double *df_tensor,
+                                     int32_t nstates, double *coulomb_tensor, dfhip_stats *stats);
+
+/* Dense screened 3c2e tensor out[(i*nbf + j)*naux + P] (row-major, both
+ * (i,j) orders filled, screened-out triplets zero), for parity checks on
+ * small systems. */
+int dfhip_int3c2e_dense(const dfhip_basis *bas, const dfhip_basis *aux, const dfhip_options *opt,
+                        double *out, dfhip_stats *stats);
+
+/* Metric (P|Q), naux x naux (symmetric). */
+int dfhip_metric(const dfhip_basis *aux, double *out);
+
+/* F[i] = F_{n[i]}(T[i]), 0 <= n[i] <= 17, T[i] >= 0, on the device. */
+int dfhip_boys(int64_t npts, const int32_t *n, const double *T, double *F);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFHIP_H */

@@ -1,0 +1,1019 @@
+// Host orchestration and C ABI of libdfhip.so.
+//
+// The call sequence follows contract_coulomb_densities_{2d,4d}
+// (diabatization/intor.f90:28-270): shell tables, Schwarz norms, metric,
+// Cholesky, screened 3c2e contracted with the densities, B = L^-1 d, and the
+// 8-fold Coulomb-tensor assembly.  All arithmetic runs on the GPU; the host
+// parses the basis, sorts shell pairs / auxiliary shells by their Schwarz
+// norms and derives the screened prefix lengths.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/dfhip.h"
+#include "int3c2e_impl.h"
+#include "linalg.h"
+#include "schwarz4c_impl.h"
+
+namespace dfhip {
+
+void register_int3c2e_l0(Int3Table&);
+void register_int3c2e_l1(Int3Table&);
+void register_int3c2e_l2(Int3Table&);
+void register_int3c2e_l3(Int3Table&);
+void register_int3c2e_l4(Int3Table&);
+void register_int3c2e_l5(Int3Table&);
+void register_schwarz4c(Schwarz4Table&);
+
+namespace {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+thread_local int t_status = DFHIP_OK;
+thread_local std::string t_msg;
+
+#define HC(x)                                                                                     \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) throw Error{DFHIP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{DFHIP_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    }
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+template <class T>
+void upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t st) {
+  d.alloc(std::max<size_t>(h.size(), 1));
+  if (!h.empty()) HC(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// ---------------------------------------------------------------------------
+// Tables
+
+double dfact(int n) {
+  double r = 1.0;
+  while (n > 1) {
+    r *= n;
+    n -= 2;
+  }
+  return r;
+}
+double fact(int n) {
+  double r = 1.0;
+  while (n > 1) r *= n--;
+  return r;
+}
+double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  double r = 1.0;
+  for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+  return r;
+}
+
+// Real solid harmonics S_lm (m = -l..l) as Cartesian polynomials, normalized
+// like x^l and divided by sqrt((2l-1)!!) (Helgaker, Jorgensen, Olsen,
+// Molecular Electronic-Structure Theory, 6.4.47-6.4.50).
+void build_sph_table(SphTable& t) {
+  int n = 0;
+  for (int L = 0; L <= LMAX_SPH; L++) {
+    const double scale = 1.0 / std::sqrt(dfact(2 * L - 1));
+    for (int m = -L; m <= L; m++) {
+      t.start[L][m + L] = n;
+      const int am = std::abs(m), vm2 = m < 0 ? 1 : 0;
+      const double N = 1.0 / (std::pow(2.0, am) * fact(L)) *
+                       std::sqrt(2.0 * fact(L + am) * fact(L - am) / (m == 0 ? 2.0 : 1.0));
+      // accumulate per monomial
+      std::vector<double> coef((L + 1) * (L + 1), 0.0);  // index lx*(L+1)+ly
+      for (int tt = 0; tt <= (L - am) / 2; tt++)
+        for (int u = 0; u <= tt; u++)
+          for (int v2 = vm2; v2 <= am; v2 += 2) {
+            const int lx = 2 * tt + am - 2 * u - v2, ly = 2 * u + v2;
+            const double sgn = ((tt + (v2 - vm2) / 2) & 1) ? -1.0 : 1.0;
+            coef[lx * (L + 1) + ly] +=
+                sgn * std::pow(0.25, tt) * binom(L, tt) * binom(L - tt, am + tt) * binom(tt, u) * binom(am, v2);
+          }
+      for (int lx = L; lx >= 0; lx--)
+        for (int ly = L - lx; ly >= 0; ly--) {
+          const double c = coef[lx * (L + 1) + ly];
+          if (c == 0.0) continue;
+          if (n >= SPH_MAX_TERMS) throw Error{DFHIP_ERR_ARG, "solid-harmonic table overflow"};
+          t.terms[n++] = SphTerm{N * c * scale, lx, ly, L - lx - ly};
+        }
+    }
+    t.start[L][2 * L + 1] = n;
+  }
+}
+
+// F_n(k h) by the convergent series at the highest order and downward
+// recursion.
+void build_boys_table(std::vector<double>& tab) {
+  tab.assign((size_t)BOYS_NGRID * BOYS_NORD, 0.0);
+  const int nmax = BOYS_NORD - 1;
+  for (int k = 0; k < BOYS_NGRID; k++) {
+    const double T = k * BOYS_H;
+    double term = 1.0 / (2 * nmax + 1), sum = term;
+    for (int j = 1; j < 4000; j++) {
+      term *= 2.0 * T / (2 * nmax + 2 * j + 1);
+      sum += term;
+      if (term < 1e-18 * sum) break;
+    }
+    const double eT = std::exp(-T);
+    double* f = &tab[(size_t)k * BOYS_NORD];
+    f[nmax] = eT * sum;
+    for (int n = nmax - 1; n >= 0; n--) f[n] = (2.0 * T * f[n + 1] + eT) / (2 * n + 1);
+  }
+}
+
+struct Ctx {
+  bool ready = false;
+  int device = -1;
+  hipStream_t st = nullptr;
+  DBuf<double> boys;
+  DBuf<SphTable> sph;
+  Int3Table int3{};
+  Schwarz4Table s4{};
+};
+
+Ctx& ctx() {
+  static Ctx* c = new Ctx();  // lives until process end
+  return *c;
+}
+std::mutex& ctx_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+void ensure_ctx() {
+  Ctx& c = ctx();
+  if (c.ready) return;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error{DFHIP_ERR_HIP, "no HIP device available"};
+  HC(hipGetDevice(&c.device));
+  HC(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+  std::vector<double> tab;
+  build_boys_table(tab);
+  upload(c.boys, tab, c.st);
+  SphTable* h = new SphTable();
+  std::memset(h, 0, sizeof(SphTable));
+  build_sph_table(*h);
+  c.sph.alloc(1);
+  HC(hipMemcpyAsync(c.sph.p, h, sizeof(SphTable), hipMemcpyHostToDevice, c.st));
+  HC(hipStreamSynchronize(c.st));
+  delete h;
+  register_int3c2e_l0(c.int3);
+  register_int3c2e_l1(c.int3);
+  register_int3c2e_l2(c.int3);
+  register_int3c2e_l3(c.int3);
+  register_int3c2e_l4(c.int3);
+  register_int3c2e_l5(c.int3);
+  register_schwarz4c(c.s4);
+  for (int la = 0; la <= LMAX_AUX; la++)
+    for (int lb = 0; lb <= (la <= LMAX_ORB ? la : 0); lb++)
+      for (int lc = 0; lc <= LMAX_AUX; lc++)
+        if (!c.int3[la][lb][lc].fn) throw Error{DFHIP_ERR_UNSUPPORTED, "missing 3c2e class kernel"};
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++)
+      if (!c.s4[la][lb].fn) throw Error{DFHIP_ERR_UNSUPPORTED, "missing 4c2e class kernel"};
+  c.ready = true;
+}
+
+// ---------------------------------------------------------------------------
+// Shells
+
+struct HostShells {
+  int n = 0, nbf = 0, lmax = -1, maxprim = 0;
+  std::vector<int> L, nprim, pptr, f0;
+  std::vector<double> xyz, exps, coefs;
+};
+
+// common/normalization.f90:10-38
+void norm_cgto_coeffs(double* c, const double* e, int n, int L) {
+  double N = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      N += c[i] * c[j] / std::pow(e[i] + e[j], L + 1.5) * std::pow(std::sqrt(e[i] * e[j]), L + 1.5);
+  for (int i = 0; i < n; i++) c[i] *= std::sqrt(std::pow(e[i], L + 1.5) / (std::pow(PI, 1.5) / std::pow(2.0, L) * N));
+}
+
+// common/shells.f90:97-187
+HostShells parse_shells(const dfhip_basis& b, int lmax_allowed, const char* what) {
+  HostShells s;
+  if (b.nshells < 1 || !b.bas_centers || !b.bas_spec || !b.bas_data)
+    throw Error{DFHIP_ERR_ARG, std::string(what) + ": empty basis"};
+  const int n = b.nshells;
+  s.n = n;
+  s.L.resize(n);
+  s.nprim.resize(n);
+  s.pptr.resize(n);
+  s.f0.resize(n);
+  s.xyz.resize(3 * n);
+  for (int i = 0; i < n; i++) {
+    const int cptr = b.bas_centers[i + 2 * n] - 1;
+    const int L = b.bas_spec[i + n];
+    const int np = b.bas_spec[i + 2 * n];
+    const int pp = b.bas_spec[i + 4 * n] - 1;
+    if (L < 0 || L > lmax_allowed)
+      throw Error{L < 0 ? DFHIP_ERR_ARG : DFHIP_ERR_UNSUPPORTED,
+                  std::string(what) + ": shell " + std::to_string(i) + " has L=" + std::to_string(L) +
+                      " (supported 0.." + std::to_string(lmax_allowed) + ")"};
+    if (np < 1) throw Error{DFHIP_ERR_ARG, std::string(what) + ": shell without primitives"};
+    if (cptr < 0 || cptr + 3 > b.ndata || pp < 0 || pp + 2 * np > b.ndata)
+      throw Error{DFHIP_ERR_ARG, std::string(what) + ": pointer outside bas_data"};
+    s.L[i] = L;
+    s.nprim[i] = np;
+    s.pptr[i] = (int)s.exps.size();
+    s.f0[i] = s.nbf;
+    s.nbf += 2 * L + 1;
+    s.lmax = std::max(s.lmax, L);
+    s.maxprim = std::max(s.maxprim, np);
+    for (int d = 0; d < 3; d++) s.xyz[3 * i + d] = b.bas_data[cptr + d];
+    std::vector<double> c(b.bas_data + pp, b.bas_data + pp + np), e(b.bas_data + pp + np, b.bas_data + pp + 2 * np);
+    for (double x : e)
+      if (!(x > 0.0)) throw Error{DFHIP_ERR_ARG, std::string(what) + ": non-positive exponent"};
+    norm_cgto_coeffs(c.data(), e.data(), np, L);
+    s.exps.insert(s.exps.end(), e.begin(), e.end());
+    s.coefs.insert(s.coefs.end(), c.begin(), c.end());
+  }
+  return s;
+}
+
+struct DevShellBufs {
+  DBuf<int> L, nprim, pptr, f0;
+  DBuf<double> xyz, exps, coefs;
+  DevShells view{};
+  void upload_from(const HostShells& h, hipStream_t st) {
+    upload(L, h.L, st);
+    upload(nprim, h.nprim, st);
+    upload(pptr, h.pptr, st);
+    upload(f0, h.f0, st);
+    upload(xyz, h.xyz, st);
+    upload(exps, h.exps, st);
+    upload(coefs, h.coefs, st);
+    view = DevShells{h.n, L.p, nprim.p, pptr.p, f0.p, xyz.p, exps.p, coefs.p};
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Timing
+
+struct Timer {
+  hipStream_t st;
+  std::vector<hipEvent_t> ev;
+  explicit Timer(hipStream_t s) : st(s) {}
+  int mark() {
+    hipEvent_t e;
+    HC(hipEventCreate(&e));
+    HC(hipEventRecord(e, st));
+    ev.push_back(e);
+    return (int)ev.size() - 1;
+  }
+  double ms(int a, int b) {
+    float t = 0.f;
+    HC(hipEventElapsedTime(&t, ev[a], ev[b]));
+    return t;
+  }
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Pairs and classes
+
+constexpr int NPCLS = (LMAX_ORB + 1) * (LMAX_ORB + 2) / 2;
+inline int pcls(int la, int lb) { return la * (la + 1) / 2 + lb; }
+
+struct PairSet {
+  // global pair order: class by class; inside a class as given by the caller
+  std::vector<int> s1, s2, packed;
+  std::vector<int> cls_begin, cls_end;  // per class, into the global order
+  int n() const { return (int)s1.size(); }
+};
+
+// All a >= b pairs, oriented with L(s1) >= L(s2), grouped by class in packed order.
+PairSet make_pairs(const HostShells& o) {
+  std::vector<std::vector<int>> by(NPCLS);
+  for (int a = 0; a < o.n; a++)
+    for (int b = 0; b <= a; b++) {
+      const int la = std::max(o.L[a], o.L[b]), lb = std::min(o.L[a], o.L[b]);
+      by[pcls(la, lb)].push_back(a * (a + 1) / 2 + b);
+    }
+  PairSet ps;
+  ps.cls_begin.assign(NPCLS, 0);
+  ps.cls_end.assign(NPCLS, 0);
+  for (int c = 0; c < NPCLS; c++) {
+    ps.cls_begin[c] = ps.n();
+    for (int pk : by[c]) {
+      int a = (int)((std::sqrt(8.0 * pk + 1.0) - 1.0) / 2.0);
+      while ((a + 1) * (a + 2) / 2 <= pk) a++;
+      while (a * (a + 1) / 2 > pk) a--;
+      const int b = pk - a * (a + 1) / 2;
+      int s1 = a, s2 = b;
+      if (o.L[a] < o.L[b]) std::swap(s1, s2);
+      ps.s1.push_back(s1);
+      ps.s2.push_back(s2);
+      ps.packed.push_back(pk);
+    }
+    ps.cls_end[c] = ps.n();
+  }
+  return ps;
+}
+
+// Schwarz norms of all pairs, in the PairSet's order.
+std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps) {
+  hipStream_t st = c.st;
+  DBuf<int> s1, s2;
+  upload(s1, ps.s1, st);
+  upload(s2, ps.s2, st);
+  std::vector<size_t> poff(NPCLS + 1, 0);
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb);
+      poff[k + 1] = (size_t)(ps.cls_end[k] - ps.cls_begin[k]) * c.s4[la][lb].nch;
+    }
+  for (int k = 0; k < NPCLS; k++) poff[k + 1] += poff[k];
+  DBuf<double> part(std::max<size_t>(poff[NPCLS], 1));
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
+      if (np == 0) continue;
+      c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+      HC(hipGetLastError());
+    }
+  std::vector<double> hp(poff[NPCLS]);
+  if (!hp.empty()) HC(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  std::vector<double> q(ps.n());
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb), nch = c.s4[la][lb].nch;
+      for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++) {
+        double s = 0.0;
+        for (int z = 0; z < nch; z++) s += hp[poff[k] + (size_t)(p - ps.cls_begin[k]) * nch + z];
+        q[p] = std::sqrt(std::sqrt(s));
+      }
+    }
+  (void)o;
+  return q;
+}
+
+// Full metric (P|Q) (naux x naux) into M via the 2c2e form of the 3c2e kernels.
+void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M) {
+  hipStream_t st = c.st;
+  std::vector<std::vector<int>> byL(LMAX_AUX + 1);
+  for (int i = 0; i < x.n; i++) byL[x.L[i]].push_back(i);
+  std::vector<int> s1, s2, nsurv;
+  std::vector<int64_t> row;
+  std::vector<int> lbeg(LMAX_AUX + 2, 0);
+  std::vector<int> lsh;
+  std::vector<int64_t> lcol;
+  std::vector<int> kmax(LMAX_AUX + 1, 1);
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    lbeg[L] = (int)s1.size();
+    for (int i : byL[L]) {
+      s1.push_back(i);
+      s2.push_back(-1);
+      row.push_back(x.f0[i]);
+      nsurv.push_back(std::numeric_limits<int>::max());
+      lsh.push_back(i);
+      lcol.push_back(x.f0[i]);
+      kmax[L] = std::max(kmax[L], x.nprim[i]);
+    }
+  }
+  lbeg[LMAX_AUX + 1] = (int)s1.size();
+  DBuf<int> ds1, ds2, dns, dlsh;
+  DBuf<int64_t> drow, dlcol;
+  upload(ds1, s1, st);
+  upload(ds2, s2, st);
+  upload(dns, nsurv, st);
+  upload(drow, row, st);
+  upload(dlsh, lsh, st);
+  upload(dlcol, lcol, st);
+  for (int LP = 0; LP <= LMAX_AUX; LP++) {
+    const int np = lbeg[LP + 1] - lbeg[LP];
+    if (np == 0) continue;
+    PairArgs pa{ds1.p + lbeg[LP], ds2.p + lbeg[LP], drow.p + lbeg[LP], dns.p + lbeg[LP], np};
+    for (int LQ = 0; LQ <= LMAX_AUX; LQ++) {
+      const int nq = lbeg[LQ + 1] - lbeg[LQ];
+      if (nq == 0) continue;
+      AuxArgs xa{dlsh.p + lbeg[LQ], dlcol.p + lbeg[LQ], 0, nq, kmax[LQ]};
+      c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, x.nbf, st);
+      HC(hipGetLastError());
+    }
+  }
+  HC(hipStreamSynchronize(st));
+}
+
+dfhip_options defaults() {
+  dfhip_options o;
+  std::memset(&o, 0, sizeof(o));
+  o.precision = DFHIP_FP64;
+  o.screening = DFHIP_SCREEN_SCHWARZ;
+  o.thresh = (double)1e-10f;  // diabatization/intor.f90:140 (real(4) literal)
+  o.dens_layout = DFHIP_DENS_FORTRAN;
+  const char* v = std::getenv("DFHIP_VERBOSE");
+  o.verbose = (v && *v && *v != '0') ? 1 : 0;
+  o.max_tile_bytes = 0;
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// The contraction path
+
+struct Plan {
+  HostShells orb, aux;
+  PairSet ps;              // sorted by Q_ab descending inside each class
+  std::vector<double> qab, qc;
+  std::vector<int64_t> prow;            // per pair, row of its first function
+  int64_t npf = 0;
+  // aux lists per L, sorted by Q_C descending; permuted column of each list entry
+  std::vector<std::vector<int>> alist;
+  std::vector<std::vector<int64_t>> acol;
+  std::vector<int> orig;                // permuted column -> original aux function
+  std::vector<std::vector<int>> nsurv;  // [L][pair]
+  int64_t ncomputed = 0, ntriplets = 0;
+  int64_t class_trip[NPCLS * 6] = {}, class_prim[NPCLS * 6] = {};
+};
+
+void build_plan(Plan& pl, const dfhip_options& opt) {
+  const HostShells& o = pl.orb;
+  const HostShells& x = pl.aux;
+  // sort pairs inside each class by Q_ab descending (stable on packed index)
+  PairSet sorted;
+  sorted.cls_begin = pl.ps.cls_begin;
+  sorted.cls_end = pl.ps.cls_end;
+  std::vector<double> q2;
+  for (int k = 0; k < NPCLS; k++) {
+    std::vector<int> idx;
+    for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) idx.push_back(p);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return pl.qab[a] > pl.qab[b]; });
+    for (int p : idx) {
+      sorted.s1.push_back(pl.ps.s1[p]);
+      sorted.s2.push_back(pl.ps.s2[p]);
+      sorted.packed.push_back(pl.ps.packed[p]);
+      q2.push_back(pl.qab[p]);
+    }
+  }
+  pl.ps = std::move(sorted);
+  pl.qab = std::move(q2);
+  pl.prow.resize(pl.ps.n());
+  int64_t r = 0;
+  for (int p = 0; p < pl.ps.n(); p++) {
+    pl.prow[p] = r;
+    r += (int64_t)(2 * o.L[pl.ps.s1[p]] + 1) * (2 * o.L[pl.ps.s2[p]] + 1);
+  }
+  pl.npf = r;
+  // aux lists
+  pl.alist.assign(LMAX_AUX + 1, {});
+  pl.acol.assign(LMAX_AUX + 1, {});
+  for (int i = 0; i < x.n; i++) pl.alist[x.L[i]].push_back(i);
+  pl.orig.assign(x.nbf, 0);
+  int64_t col = 0;
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    auto& v = pl.alist[L];
+    std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return pl.qc[a] > pl.qc[b]; });
+    for (int sh : v) {
+      pl.acol[L].push_back(col);
+      for (int m = 0; m < 2 * L + 1; m++) pl.orig[col + m] = x.f0[sh] + m;
+      col += 2 * L + 1;
+    }
+  }
+  // screened prefix per (L, pair): skip when Q_ab * Q_C < thresh
+  pl.nsurv.assign(LMAX_AUX + 1, std::vector<int>(pl.ps.n(), 0));
+  pl.ncomputed = 0;
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    const auto& v = pl.alist[L];
+    for (int p = 0; p < pl.ps.n(); p++) {
+      int n;
+      if (opt.screening == DFHIP_SCREEN_NONE) {
+        n = (int)v.size();
+      } else {
+        const double qa = pl.qab[p];
+        int lo = 0, hi = (int)v.size();  // first index with qa*qc < thresh
+        while (lo < hi) {
+          const int mid = (lo + hi) / 2;
+          if (qa * pl.qc[v[mid]] < opt.thresh) hi = mid;
+          else lo = mid + 1;
+        }
+        n = lo;
+      }
+      pl.nsurv[L][p] = n;
+      pl.ncomputed += n;
+    }
+  }
+  // per-class counts of screened triplets and primitive triplets
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    const auto& v = pl.alist[L];
+    std::vector<int64_t> pref(v.size() + 1, 0);
+    for (size_t k = 0; k < v.size(); k++) pref[k + 1] = pref[k] + x.nprim[v[k]];
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb);
+        for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+          const int n = pl.nsurv[L][p];
+          pl.class_trip[k * 6 + L] += n;
+          pl.class_prim[k * 6 + L] += (int64_t)o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]] * pref[n];
+        }
+      }
+  }
+  pl.ntriplets = (int64_t)o.n * (o.n + 1) / 2 * x.n;
+}
+
+struct Outputs {
+  double* df = nullptr;         // host, column-major naux x ndens
+  double* J = nullptr;          // host, column-major nstates^4
+  int nstates = 0;
+  double* dense3c = nullptr;    // host, (nbf, nbf, naux) row-major
+};
+
+void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const double* dens,
+         const dfhip_options& opt, Outputs out, dfhip_stats* stats) {
+  if (opt.precision != DFHIP_FP64) throw Error{DFHIP_ERR_UNSUPPORTED, "only the FP64 path is available"};
+  if (opt.screening != DFHIP_SCREEN_SCHWARZ && opt.screening != DFHIP_SCREEN_NONE)
+    throw Error{DFHIP_ERR_UNSUPPORTED, "unknown screening mode"};
+  Plan pl;
+  pl.orb = parse_shells(B, LMAX_ORB, "orbital basis");
+  pl.aux = parse_shells(X, LMAX_AUX, "auxiliary basis");
+  const HostShells& o = pl.orb;
+  const HostShells& x = pl.aux;
+  const bool need_dens = out.df || out.J;
+  if (need_dens) {
+    if (ndens < 1) throw Error{DFHIP_ERR_ARG, "ndens must be >= 1"};
+    if (nbfs != o.nbf)
+      throw Error{DFHIP_ERR_ARG, "nbfs=" + std::to_string(nbfs) + " but the basis has " + std::to_string(o.nbf) +
+                                     " spherical functions"};
+    if (!dens) throw Error{DFHIP_ERR_ARG, "densities is NULL"};
+  }
+  if (out.J && out.nstates * (out.nstates + 1) / 2 != ndens)
+    throw Error{DFHIP_ERR_ARG, "ndens is not nstates(nstates+1)/2"};
+  const int64_t naux = x.nbf;
+
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  Ctx& c = ctx();
+  hipStream_t st = c.st;
+  Timer tm(st);
+  const int e0 = tm.mark();
+  DevShellBufs dorb, daux;
+  dorb.upload_from(o, st);
+  daux.upload_from(x, st);
+  const int e1 = tm.mark();
+
+  // Schwarz norms of the shell pairs (common/integrals/intor.f90:131-160)
+  pl.ps = make_pairs(o);
+  pl.qab = schwarz_bra_dev(c, o, dorb.view, pl.ps);
+  const int e2 = tm.mark();
+
+  // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal blocks
+  DBuf<double> M((size_t)naux * naux);
+  metric_dev(c, x, daux.view, M.p);
+  DBuf<double> dqc(x.n);
+  schwarz_aux_from_metric(M.p, naux, daux.L.p, daux.f0.p, x.n, dqc.p, st);
+  pl.qc.resize(x.n);
+  HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  const int e3 = tm.mark();
+
+  // Cholesky of the metric (common/linalg.f90:78)
+  const int nblk = (int)((naux + CHOL_BLOCK - 1) / CHOL_BLOCK);
+  DBuf<double> invblk((size_t)nblk * CHOL_BLOCK * CHOL_BLOCK);
+  DBuf<int> info(1);
+  HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
+  cholesky(M.p, naux, invblk.p, info.p, st);
+  HC(hipGetLastError());
+  const int e4 = tm.mark();
+  HC(hipStreamSynchronize(st));
+  int hinfo = 0;
+  HC(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (hinfo != 0)
+    throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " + std::to_string(hinfo) +
+                                       ")"};
+
+  build_plan(pl, opt);
+  if (opt.verbose) {
+    std::printf("Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
+    std::printf("Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
+    std::printf(" Densities: %6d\n", ndens);
+  }
+
+  // device plan arrays
+  DBuf<int> ps1, ps2;
+  DBuf<int64_t> prow;
+  upload(ps1, pl.ps.s1, st);
+  upload(ps2, pl.ps.s2, st);
+  upload(prow, pl.prow, st);
+  std::vector<DBuf<int>> dnsurv(LMAX_AUX + 1);
+  std::vector<DBuf<int>> dalist(LMAX_AUX + 1);
+  std::vector<DBuf<int64_t>> dacol(LMAX_AUX + 1);
+  std::vector<int> kmax(LMAX_AUX + 1, 1);
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    upload(dnsurv[L], pl.nsurv[L], st);
+    upload(dalist[L], pl.alist[L], st);
+    upload(dacol[L], pl.acol[L], st);
+    for (int sh : pl.alist[L]) kmax[L] = std::max(kmax[L], x.nprim[sh]);
+  }
+  DBuf<int> dorig;
+  upload(dorig, pl.orig, st);
+
+  // densities: upload and pair-pack (diabatization/intor.f90:155-176 symmetry)
+  DBuf<double> Dp;
+  double ms_h2d = 0.0;
+  if (need_dens) {
+    DBuf<double> Draw;
+    const double* Dsrc = dens;
+    if (!opt.dens_on_device) {
+      const int h0 = tm.mark();
+      Draw.alloc((size_t)ndens * nbfs * nbfs);
+      HC(hipMemcpyAsync(Draw.p, dens, Draw.n * sizeof(double), hipMemcpyHostToDevice, st));
+      const int h1 = tm.mark();
+      HC(hipStreamSynchronize(st));
+      ms_h2d = tm.ms(h0, h1);
+      Dsrc = Draw.p;
+    }
+    int64_t sI, si, sj;
+    if (opt.dens_layout == DFHIP_DENS_C) {
+      sI = (int64_t)nbfs * nbfs;
+      si = nbfs;
+      sj = 1;
+    } else {
+      sI = 1;
+      si = ndens;
+      sj = (int64_t)ndens * nbfs;
+    }
+    Dp.alloc((size_t)pl.npf * ndens);
+    pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp.p, ndens, st);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(st));
+  }
+  const int e5 = tm.mark();
+
+  // column blocks of the (pair function x aux function) staging matrix
+  size_t cap = (size_t)opt.max_tile_bytes;
+  if (cap == 0) {
+    size_t fr = 0, tot = 0;
+    HC(hipMemGetInfo(&fr, &tot));
+    cap = std::max<size_t>((size_t)(0.5 * fr), (size_t)1 << 28);
+  }
+  const int64_t rowbytes = std::max<int64_t>(pl.npf, 1) * (int64_t)sizeof(double);
+  int64_t maxcols = std::max<int64_t>(1, (int64_t)(cap / rowbytes));
+  // block boundaries at aux-shell granularity over the permuted column order
+  std::vector<int64_t> bounds{0};
+  {
+    int64_t cur = 0;
+    for (int L = 0; L <= LMAX_AUX; L++)
+      for (size_t k = 0; k < pl.alist[L].size(); k++) {
+        const int64_t w = 2 * L + 1;
+        if (cur + w - bounds.back() > maxcols && cur > bounds.back()) bounds.push_back(cur);
+        cur += w;
+      }
+    if (bounds.back() != cur) bounds.push_back(cur);
+  }
+  const int nblocks = (int)bounds.size() - 1;
+  int64_t widest = 0;
+  for (int b = 0; b < nblocks; b++) widest = std::max(widest, bounds[b + 1] - bounds[b]);
+  DBuf<double> T((size_t)std::max<int64_t>(pl.npf, 1) * std::max<int64_t>(widest, 1));
+  DBuf<double> dd;
+  DBuf<double> part;
+  int nsplit = 1;
+  if (need_dens) {
+    dd.alloc((size_t)naux * ndens);
+    HC(hipMemsetAsync(dd.p, 0, dd.n * sizeof(double), st));
+    const int64_t tiles = ((widest + 63) / 64) * ((ndens + 63) / 64);
+    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(contract_nsplit_max(), 2048 / std::max<int64_t>(tiles, 1)));
+    nsplit = (int)std::min<int64_t>(nsplit, std::max<int64_t>(1, (pl.npf + 15) / 16));
+    part.alloc((size_t)nsplit * widest * ndens);
+  }
+  double ms_int = 0.0, ms_con = 0.0;
+  std::vector<double> dense_stage;
+  for (int blk = 0; blk < nblocks; blk++) {
+    const int64_t cA = bounds[blk], cB = bounds[blk + 1], w = cB - cA;
+    HC(hipMemsetAsync(T.p, 0, (size_t)pl.npf * w * sizeof(double), st));
+    const int b0 = tm.mark();
+    for (int L = 0; L <= LMAX_AUX; L++) {
+      const auto& cols = pl.acol[L];
+      // list window with columns inside [cA, cB)
+      const int lo = (int)(std::lower_bound(cols.begin(), cols.end(), cA) - cols.begin());
+      const int hi = (int)(std::lower_bound(cols.begin(), cols.end(), cB) - cols.begin());
+      if (lo >= hi) continue;
+      for (int la = 0; la <= LMAX_ORB; la++)
+        for (int lb = 0; lb <= la; lb++) {
+          const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
+          if (np == 0) continue;
+          PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dnsurv[L].p + p0, np};
+          AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi, kmax[L]};
+          c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, T.p - cA, w, st);
+          HC(hipGetLastError());
+        }
+    }
+    const int b1 = tm.mark();
+    if (need_dens) {
+      contract_T(T.p, w, Dp.p, ndens, pl.npf, (int)w, ndens, part.p, nsplit, dorig.p + cA, dd.p, (int)naux, st);
+      HC(hipGetLastError());
+    }
+    const int b2 = tm.mark();
+    if (out.dense3c) {
+      dense_stage.resize((size_t)pl.npf * w);
+      HC(hipMemcpyAsync(dense_stage.data(), T.p, dense_stage.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      HC(hipStreamSynchronize(st));
+      const int64_t nbf = o.nbf;
+      for (int p = 0; p < pl.ps.n(); p++) {
+        const int s1 = pl.ps.s1[p], s2 = pl.ps.s2[p];
+        const int n1 = 2 * o.L[s1] + 1, n2 = 2 * o.L[s2] + 1;
+        for (int mu = 0; mu < n1; mu++)
+          for (int nu = 0; nu < n2; nu++) {
+            const double* src = &dense_stage[(size_t)(pl.prow[p] + mu * n2 + nu) * w];
+            const int64_t i = o.f0[s1] + mu, j = o.f0[s2] + nu;
+            for (int64_t col = 0; col < w; col++) {
+              const int64_t P = pl.orig[cA + col];
+              out.dense3c[(i * nbf + j) * naux + P] = src[col];
+              out.dense3c[(j * nbf + i) * naux + P] = src[col];
+            }
+          }
+      }
+    }
+    HC(hipStreamSynchronize(st));
+    ms_int += tm.ms(b0, b1);
+    ms_con += tm.ms(b1, b2);
+  }
+  const int e6 = tm.mark();
+
+  // B = L^-1 d (diabatization/intor.f90:198)
+  if (need_dens) {
+    forward_solve(M.p, invblk.p, dd.p, naux, ndens, st);
+    HC(hipGetLastError());
+  }
+  const int e7 = tm.mark();
+  DBuf<double> dJ;
+  if (out.J) {
+    const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;
+    dJ.alloc(nj);
+    HC(hipMemsetAsync(dJ.p, 0, nj * sizeof(double), st));
+    assemble_j(dd.p, naux, ndens, out.nstates, dJ.p, st);
+    HC(hipGetLastError());
+  }
+  const int e8 = tm.mark();
+  if (out.df)
+    HC(hipMemcpyAsync(out.df, dd.p, (size_t)naux * ndens * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (out.J) HC(hipMemcpyAsync(out.J, dJ.p, dJ.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  const int e9 = tm.mark();
+  HC(hipStreamSynchronize(st));
+
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ntriplets = pl.ntriplets;
+    stats->ncomputed = pl.ncomputed;
+    stats->nskipped = pl.ntriplets - pl.ncomputed;
+    stats->nbf = o.nbf;
+    stats->naux = naux;
+    stats->npair_functions = pl.npf;
+    stats->ncolumn_blocks = nblocks;
+    stats->ms_upload = tm.ms(e0, e1);
+    stats->ms_schwarz = tm.ms(e1, e2);
+    stats->ms_metric = tm.ms(e2, e3);
+    stats->ms_cholesky = tm.ms(e3, e4);
+    stats->ms_pack = tm.ms(e4, e5);
+    stats->ms_int3c2e = ms_int;
+    stats->ms_contract = ms_con;
+    stats->ms_solve = tm.ms(e6, e7);
+    stats->ms_assemble = tm.ms(e7, e8);
+    stats->ms_download = tm.ms(e8, e9);
+    stats->ms_total = tm.ms(e0, e9);
+    stats->ms_h2d_densities = ms_h2d;
+    for (int k = 0; k < NPCLS * 6; k++) {
+      stats->class_triplets[k] = pl.class_trip[k];
+      stats->class_prim_triplets[k] = pl.class_prim[k];
+    }
+  }
+  if (opt.verbose) {
+    std::printf("Schwarz norms took %8.4f s.\n", tm.ms(e1, e2) * 1e-3);
+    std::printf("Calculation of density fitting metric (P|Q) took %8.4f s.\n", tm.ms(e2, e3) * 1e-3);
+    std::printf("Cholesky factorization of (P|Q) took %8.4f s.\n", tm.ms(e3, e4) * 1e-3);
+    std::printf("Density contraction took %8.4f s.\n", (ms_int + ms_con) * 1e-3);
+    const int64_t skipped = pl.ntriplets - pl.ncomputed;
+    std::printf("Screened out %12lld of %12lld integral triplets (%8.2f%%)\n", (long long)skipped,
+                (long long)pl.ntriplets, 100.0 * (double)skipped / (double)std::max<int64_t>(pl.ntriplets, 1));
+  }
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    t_status = DFHIP_OK;
+    t_msg.clear();
+  } catch (const Error& e) {
+    t_status = e.code;
+    t_msg = e.msg;
+  } catch (const std::exception& e) {
+    t_status = DFHIP_ERR_ARG;
+    t_msg = e.what();
+  }
+  if (t_status != DFHIP_OK && std::getenv("DFHIP_QUIET") == nullptr)
+    std::fprintf(stderr, "dfhip error %d: %s\n", t_status, t_msg.c_str());
+  return t_status;
+}
+
+dfhip_basis mkbasis(int32_t n, int32_t nd, const int32_t* bc, const int32_t* bs, const double* bd) {
+  return dfhip_basis{n, nd, bc, bs, bd};
+}
+
+std::vector<double> schwarz_bra_host(const dfhip_basis& b) {
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  Ctx& c = ctx();
+  HostShells o = parse_shells(b, LMAX_ORB, "basis");
+  DevShellBufs d;
+  d.upload_from(o, c.st);
+  PairSet ps = make_pairs(o);
+  std::vector<double> q = schwarz_bra_dev(c, o, d.view, ps);
+  std::vector<double> outp(ps.n());
+  for (int p = 0; p < ps.n(); p++) outp[ps.packed[p]] = q[p];
+  return outp;
+}
+
+std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  Ctx& c = ctx();
+  HostShells x = parse_shells(b, LMAX_AUX, "basis");
+  DevShellBufs d;
+  d.upload_from(x, c.st);
+  DBuf<double> M((size_t)x.nbf * x.nbf);
+  metric_dev(c, x, d.view, M.p);
+  std::vector<double> h((size_t)x.nbf * x.nbf);
+  if (qc) {
+    DBuf<double> dq(x.n);
+    schwarz_aux_from_metric(M.p, x.nbf, d.L.p, d.f0.p, x.n, dq.p, c.st);
+    qc->resize(x.n);
+    HC(hipMemcpyAsync(qc->data(), dq.p, x.n * sizeof(double), hipMemcpyDeviceToHost, c.st));
+  }
+  HC(hipMemcpyAsync(h.data(), M.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.st));
+  HC(hipStreamSynchronize(c.st));
+  return h;
+}
+
+}  // namespace
+}  // namespace dfhip
+
+using namespace dfhip;
+
+extern "C" {
+
+void f_init(void) {
+  guarded([&] {
+    std::lock_guard<std::mutex> lock(ctx_mutex());
+    ensure_ctx();
+  });
+}
+
+int dfhip_last_status(void) { return t_status; }
+const char* dfhip_last_error(void) { return t_msg.c_str(); }
+
+void dfhip_default_options(dfhip_options* opt) {
+  if (opt) *opt = defaults();
+}
+
+int dfhip_contract_coulomb_densities(const dfhip_basis* bas, const dfhip_basis* aux, int32_t ndens, int32_t nbfs,
+                                     const double* densities, const dfhip_options* opt, double* df_tensor,
+                                     int32_t nstates, double* coulomb_tensor, dfhip_stats* stats) {
+  return guarded([&] {
+    if (!bas || !aux) throw Error{DFHIP_ERR_ARG, "basis is NULL"};
+    const dfhip_options o = opt ? *opt : defaults();
+    Outputs out;
+    out.df = df_tensor;
+    out.J = coulomb_tensor;
+    out.nstates = nstates;
+    run(*bas, *aux, ndens, nbfs, densities, o, out, stats);
+  });
+}
+
+int dfhip_int3c2e_dense(const dfhip_basis* bas, const dfhip_basis* aux, const dfhip_options* opt, double* out3,
+                        dfhip_stats* stats) {
+  return guarded([&] {
+    if (!bas || !aux || !out3) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    const dfhip_options o = opt ? *opt : defaults();
+    Outputs out;
+    out.dense3c = out3;
+    run(*bas, *aux, 0, 0, nullptr, o, out, stats);
+  });
+}
+
+int dfhip_metric(const dfhip_basis* aux, double* out) {
+  return guarded([&] {
+    if (!aux || !out) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    std::vector<double> M = metric_host(*aux, nullptr);
+    std::memcpy(out, M.data(), M.size() * sizeof(double));
+  });
+}
+
+int dfhip_boys(int64_t npts, const int32_t* n, const double* T, double* F) {
+  return guarded([&] {
+    if (npts < 0 || (npts > 0 && (!n || !T || !F))) throw Error{DFHIP_ERR_ARG, "bad arguments"};
+    for (int64_t i = 0; i < npts; i++)
+      if (n[i] < 0 || n[i] > BOYS_NORD - BOYS_NTAYLOR || !(T[i] >= 0.0))
+        throw Error{DFHIP_ERR_ARG, "order or argument out of range"};
+    std::lock_guard<std::mutex> lock(ctx_mutex());
+    ensure_ctx();
+    Ctx& c = ctx();
+    DBuf<int> dn(std::max<int64_t>(npts, 1));
+    DBuf<double> dT(std::max<int64_t>(npts, 1)), dF(std::max<int64_t>(npts, 1));
+    if (npts > 0) {
+      HC(hipMemcpyAsync(dn.p, n, npts * sizeof(int), hipMemcpyHostToDevice, c.st));
+      HC(hipMemcpyAsync(dT.p, T, npts * sizeof(double), hipMemcpyHostToDevice, c.st));
+      boys_probe(dn.p, dT.p, npts, c.boys.p, dF.p, c.st);
+      HC(hipGetLastError());
+      HC(hipMemcpyAsync(F, dF.p, npts * sizeof(double), hipMemcpyDeviceToHost, c.st));
+    }
+    HC(hipStreamSynchronize(c.st));
+  });
+}
+
+void f_contract_coulomb_densities_2d(int32_t nshells, int32_t ndata, const int32_t* bas_centers,
+                                     const int32_t* bas_spec, const double* bas_data, int32_t nshells_aux,
+                                     int32_t ndata_aux, const int32_t* bas_centers_aux, const int32_t* bas_spec_aux,
+                                     const double* bas_data_aux, int32_t ndens, int32_t nbfs, const double* densities,
+                                     int32_t naux, double* df_tensor) {
+  guarded([&] {
+    const dfhip_basis b = mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data);
+    const dfhip_basis x = mkbasis(nshells_aux, ndata_aux, bas_centers_aux, bas_spec_aux, bas_data_aux);
+    int nsph = 0;
+    for (int i = 0; i < nshells_aux; i++) nsph += 2 * bas_spec_aux[i + nshells_aux] + 1;
+    if (naux != nsph)
+      throw Error{DFHIP_ERR_ARG, "naux=" + std::to_string(naux) + " but the auxiliary basis has " +
+                                     std::to_string(nsph) + " spherical functions"};
+    const dfhip_options o = defaults();
+    Outputs out;
+    out.df = df_tensor;
+    run(b, x, ndens, nbfs, densities, o, out, nullptr);
+  });
+}
+
+void f_contract_coulomb_densities_4d(int32_t nshells, int32_t ndata, const int32_t* bas_centers,
+                                     const int32_t* bas_spec, const double* bas_data, int32_t nshells_aux,
+                                     int32_t ndata_aux, const int32_t* bas_centers_aux, const int32_t* bas_spec_aux,
+                                     const double* bas_data_aux, int32_t ndens, int32_t nbfs, const double* densities,
+                                     int32_t nstates, double* coulomb_tensor) {
+  guarded([&] {
+    const dfhip_basis b = mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data);
+    const dfhip_basis x = mkbasis(nshells_aux, ndata_aux, bas_centers_aux, bas_spec_aux, bas_data_aux);
+    const dfhip_options o = defaults();
+    Outputs out;
+    out.J = coulomb_tensor;
+    out.nstates = nstates;
+    run(b, x, ndens, nbfs, densities, o, out, nullptr);
+  });
+}
+
+void f_int_schwarz_aux(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                       const double* bas_data, double* integrals) {
+  guarded([&] {
+    std::vector<double> qc;
+    metric_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data), &qc);
+    std::memcpy(integrals, qc.data(), qc.size() * sizeof(double));
+  });
+}
+
+void f_int_schwarz_bra(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                       const double* bas_data, double* integrals) {
+  guarded([&] {
+    std::vector<double> q = schwarz_bra_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data));
+    std::memcpy(integrals, q.data(), q.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+// Screened 3c2e (ab|C) blocks, written to a dense (pair function x aux
+// function) matrix in HBM.  Also used for the 2c2e metric (P|Q) with a
+// single-centre "pair" (s2 < 0).
+//
+// One workgroup = one shell pair (a, b) x a chunk of NC auxiliary shells of
+// one angular momentum.  Per primitive pair ij the spherical bra matrix Et
+// (nsph(a) nsph(b) x nherm(la+lb)) is built once in LDS and reused by every
+// auxiliary shell of the chunk; per auxiliary primitive the closed-form
+// R_tuv and the Hobson-transformed W[C][tuv][m] are built in LDS; the
+// contracted block accumulates in registers as acc += Et . W.
+#pragma once
+
+#include "hermite.h"
+
+namespace dfhip {
+
+struct PairArgs {
+  const int* s1;       // first shell (L = LA) of each pair of the class
+  const int* s2;       // second shell (L = LB), or -1 for a single-centre bra
+  const int64_t* row;  // output row of the pair's first function
+  const int* nsurv;    // screened-in prefix length of the aux list, per pair
+  int npairs;
+};
+
+struct AuxArgs {
+  const int* shell;    // aux shells of one L in processing order
+  const int64_t* col;  // output column of each listed aux shell
+  int lo, hi;          // list window processed by this launch
+  int kmax;            // largest primitive count in the list
+};
+
+template <int LA, int LB, int LC>
+struct Cls3 {
+  static constexpr int LAB = LA + LB, L = LAB + LC;
+  static constexpr int NA = nsph(LA), NB = nsph(LB), NAB = NA * NB, NCC = nsph(LC);
+  static constexpr int H = nherm(LAB), NR = nherm(L), NRLO = (LC == 0) ? 0 : nherm(LC - 1);
+  static constexpr int ESZ = (LA + 1) * (LB + 1) * (LAB + 1);
+  static constexpr int NT = 128;
+  static constexpr int FIXED = 3 * ESZ + NAB * H;
+  static constexpr int PERC = 4 + 4 * (L + 1) + NR + H * NCC;
+  static constexpr int SMALL = 8192, LARGE = 20000;  // doubles (64 KB / 156 KB)
+  static constexpr int NC_LDS = (SMALL - FIXED) / PERC >= 2 ? (SMALL - FIXED) / PERC : (LARGE - FIXED) / PERC;
+  static constexpr int NC_OUT = (16 * NT) / (NAB * NCC) > 1 ? (16 * NT) / (NAB * NCC) : 1;
+  static constexpr int NC0 = NC_LDS < NC_OUT ? NC_LDS : NC_OUT;
+  static constexpr int NC = NC0 > 128 ? 128 : (NC0 < 1 ? 1 : NC0);
+  static constexpr int LDS = FIXED + NC * PERC;
+  static constexpr int OPT = (NAB * NC * NCC + NT - 1) / NT;
+  static_assert(NC_LDS >= 1, "class does not fit in LDS");
+  static_assert(LDS <= LARGE, "LDS budget");
+};
+
+template <int LA, int LB, int LC>
+__global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
+                                                      const SphTable* __restrict__ sph,
+                                                      const double* __restrict__ boys, double* __restrict__ out,
+                                                      int64_t ldo) {
+  using K = Cls3<LA, LB, LC>;
+  constexpr int L = K::L, H = K::H, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NT = K::NT;
+  __shared__ double lds[K::LDS];
+  __shared__ int cshell[K::NC];
+
+  const int pidx = blockIdx.x;
+  const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c0 = xa.lo + (int)blockIdx.y * K::NC;
+  if (c0 >= cend) return;
+  const int nc = min(K::NC, cend - c0);
+  const int tid = threadIdx.x;
+
+  double* Ex = lds;
+  double* Et = lds + 3 * K::ESZ;
+  double* cxyz = Et + NAB * H;           // NC*4
+  double* gN = cxyz + 4 * K::NC;         // NC*(L+1)
+  double* pw = gN + K::NC * (L + 1);     // NC*3*(L+1)
+  double* R = pw + 3 * K::NC * (L + 1);  // NC*NR
+  double* W = R + K::NC * NR;            // NC*H*NCC
+
+  for (int c = tid; c < nc; c += NT) {
+    const int sh = xa.shell[c0 + c];
+    cshell[c] = sh;
+    cxyz[4 * c + 0] = ket.xyz[3 * sh + 0];
+    cxyz[4 * c + 1] = ket.xyz[3 * sh + 1];
+    cxyz[4 * c + 2] = ket.xyz[3 * sh + 2];
+  }
+
+  double A[3], B[3];
+  for (int d = 0; d < 3; d++) A[d] = bra.xyz[3 * s1 + d];
+  const int npa = bra.nprim[s1], pa0 = bra.pptr[s1];
+  int npb = 1, pb0 = 0;
+  if (s2 >= 0) {
+    for (int d = 0; d < 3; d++) B[d] = bra.xyz[3 * s2 + d];
+    npb = bra.nprim[s2];
+    pb0 = bra.pptr[s2];
+  } else {
+    for (int d = 0; d < 3; d++) B[d] = A[d];
+  }
+
+  double acc[K::OPT];
+#pragma unroll
+  for (int r = 0; r < K::OPT; r++) acc[r] = 0.0;
+
+  const int nW = nc * H * NCC;
+  const double lc_sign = (LC & 1) ? -1.0 : 1.0;
+
+  for (int ij = 0; ij < npa * npb; ij++) {
+    const int i = ij / npb, j = ij - (ij / npb) * npb;
+    const double a = bra.exps[pa0 + i], ca = bra.coefs[pa0 + i];
+    const double b = (s2 >= 0) ? bra.exps[pb0 + j] : 0.0;
+    const double cb = (s2 >= 0) ? bra.coefs[pb0 + j] : 1.0;
+    const double p = a + b;
+    double P[3];
+    for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
+
+    __syncthreads();  // previous primitive pair is done with Et and W
+    if (tid < 3) herm_E_dir<LA, LB>(a, b, A[tid], B[tid], Ex + tid * K::ESZ);
+    for (int k = tid; k < nW; k += NT) W[k] = 0.0;
+    __syncthreads();
+    const double cab = ca * cb;
+    for (int k = tid; k < NAB * H; k += NT) {
+      const int munu = k / H, h = k - (k / H) * H;
+      const int mu = munu / K::NB, nu = munu - (munu / K::NB) * K::NB;
+      int t, u, v;
+      herm_tuv(h, t, u, v);
+      Et[k] = cab * bra_element<LA, LB>(sph, mu, nu, t, u, v, Ex, Ex + K::ESZ, Ex + 2 * K::ESZ);
+    }
+
+    for (int kp = 0; kp < xa.kmax; kp++) {
+      // per-C scaled Boys values and powers of 2(P-C)
+      for (int k = tid; k < nc * 4 * (L + 1); k += NT) {
+        const int c = k / (4 * (L + 1)), q = k - c * (4 * (L + 1));
+        const int sh = cshell[c];
+        const bool live = kp < ket.nprim[sh];
+        const double g = live ? ket.exps[ket.pptr[sh] + kp] : 1.0;
+        const double PC0 = P[0] - cxyz[4 * c + 0], PC1 = P[1] - cxyz[4 * c + 1], PC2 = P[2] - cxyz[4 * c + 2];
+        if (q <= L) {
+          double val = 0.0;
+          if (live) {
+            const double dk = ket.coefs[ket.pptr[sh] + kp];
+            const double alpha = p * g / (p + g);
+            const double T = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
+            double scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
+            const double inv2g = 0.5 / g;
+            for (int e = 0; e < LC; e++) scale *= inv2g;
+            for (int e = 0; e < q; e++) scale *= -alpha;
+            val = scale * boys_f(q, T, boys);
+          }
+          gN[c * (L + 1) + q] = val;
+        } else {
+          const int qq = q - (L + 1), d = qq / (L + 1), e = qq - d * (L + 1);
+          const double x = 2.0 * (d == 0 ? PC0 : (d == 1 ? PC1 : PC2));
+          double v = 1.0;
+          for (int z = 0; z < e; z++) v *= x;
+          pw[c * 3 * (L + 1) + d * (L + 1) + e] = v;
+        }
+      }
+      __syncthreads();
+      for (int k = tid; k < nc * (NR - K::NRLO); k += NT) {
+        const int c = k / (NR - K::NRLO), h = K::NRLO + (k - c * (NR - K::NRLO));
+        int t, u, v;
+        herm_tuv(h, t, u, v);
+        R[c * NR + h] = herm_R(t, u, v, L, pw + c * 3 * (L + 1), gN + c * (L + 1));
+      }
+      __syncthreads();
+      for (int k = tid; k < nW; k += NT) {
+        const int c = k / (H * NCC), rr = k - c * (H * NCC), h = rr / NCC, m = rr - (rr / NCC) * NCC;
+        int t, u, v;
+        herm_tuv(h, t, u, v);
+        double s = 0.0;
+        const double* Rc = R + c * NR;
+        for (int q = sph->start[LC][m]; q < sph->start[LC][m + 1]; q++) {
+          const SphTerm tm = sph->terms[q];
+          s += tm.c * Rc[herm_index(t + tm.lx, u + tm.ly, v + tm.lz)];
+        }
+        W[k] += s;
+      }
+      __syncthreads();
+    }
+
+    const int ncol = nc * NCC;
+#pragma unroll
+    for (int r = 0; r < K::OPT; r++) {
+      const int o = tid + r * NT;
+      if (o < NAB * ncol) {
+        const int munu = o / ncol, col = o - (o / ncol) * ncol;
+        const int c = col / NCC, m = col - (col / NCC) * NCC;
+        const double* e = Et + munu * H;
+        const double* w = W + c * H * NCC + m;
+        double s = 0.0;
+        for (int h = 0; h < H; h++) s += e[h] * w[h * NCC];
+        acc[r] += s;
+      }
+    }
+  }
+
+  const int ncol = nc * NCC;
+  const int64_t row0 = pa.row[pidx];
+#pragma unroll
+  for (int r = 0; r < K::OPT; r++) {
+    const int o = tid + r * NT;
+    if (o < NAB * ncol) {
+      const int munu = o / ncol, col = o - (o / ncol) * ncol;
+      const int c = col / NCC, m = col - (col / NCC) * NCC;
+      out[(row0 + munu) * ldo + xa.col[c0 + c] + m] = acc[r];
+    }
+  }
+}
+
+struct Int3Launch {
+  void (*fn)(DevShells, DevShells, PairArgs, AuxArgs, const SphTable*, const double*, double*, int64_t,
+             hipStream_t);
+  int nc;
+  int lds_bytes;
+};
+
+template <int LA, int LB, int LC>
+void launch_int3c2e(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa, const SphTable* sph,
+                    const double* boys, double* out, int64_t ldo, hipStream_t st) {
+  using K = Cls3<LA, LB, LC>;
+  const int nchunks = (xa.hi - xa.lo + K::NC - 1) / K::NC;
+  if (pa.npairs <= 0 || nchunks <= 0) return;
+  dim3 grid(pa.npairs, nchunks);
+  hipLaunchKernelGGL((int3c2e_kernel<LA, LB, LC>), grid, dim3(K::NT), 0, st, bra, ket, pa, xa, sph, boys, out,
+                     ldo);
+}
+
+template <int LA, int LB, int LC>
+Int3Launch make_int3() {
+  using K = Cls3<LA, LB, LC>;
+  return Int3Launch{&launch_int3c2e<LA, LB, LC>, K::NC, (int)(K::LDS * sizeof(double))};
+}
+
+using Int3Table = Int3Launch[LMAX_AUX + 1][LMAX_ORB + 1][LMAX_AUX + 1];
+
+}  // namespace dfhip

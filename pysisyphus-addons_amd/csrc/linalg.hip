@@ -1,0 +1,416 @@
+// Dense FP64 kernels of the density-fitting path:
+//   * pair-packing of the (transition) densities,
+//   * the contraction d(P, I) = sum_k T(k, P) Dp(k, I) (split-K GEMM),
+//   * blocked Cholesky of the metric (dpotrf('L') of common/linalg.f90:78),
+//   * the forward solve B = L^-1 d (what dtrtri + dtrmm of
+//     diabatization/intor.f90:198 compute),
+//   * the 8-fold Coulomb-tensor assembly (diabatization/intor.f90:241-269),
+//   * Schwarz norms of the auxiliary shells from the metric's diagonal blocks,
+//   * a Boys-function probe.
+#include "hermite.h"
+#include "linalg.h"
+
+namespace dfhip {
+
+// ---------------------------------------------------------------------------
+// Density packing: Dp[(row(p) + mu*n2 + nu) * ldd + I] =
+//   D_I(f1+mu, f2+nu) + D_I(f2+nu, f1+mu)   (s1 != s2)
+//   D_I(f1+mu, f2+nu)                        (s1 == s2)
+// (the symmetry use of diabatization/intor.f90:162-169).
+__global__ void pack_density_kernel(const int* __restrict__ s1v, const int* __restrict__ s2v,
+                                    const int64_t* __restrict__ rowv, const int* __restrict__ Lv,
+                                    const int* __restrict__ f0v, const double* __restrict__ D, int64_t sI, int64_t si,
+                                    int64_t sj, int ndens, double* __restrict__ Dp, int ldd) {
+  const int p = blockIdx.x;
+  const int s1 = s1v[p], s2 = s2v[p];
+  const int n1 = 2 * Lv[s1] + 1, n2 = 2 * Lv[s2] + 1;
+  const int64_t f1 = f0v[s1], f2 = f0v[s2];
+  const int64_t row = rowv[p];
+  const int n = n1 * n2 * ndens;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int mn = e / ndens, I = e - mn * ndens;
+    const int mu = mn / n2, nu = mn - mu * n2;
+    double v = D[I * sI + (f1 + mu) * si + (f2 + nu) * sj];
+    if (s1 != s2) v += D[I * sI + (f2 + nu) * si + (f1 + mu) * sj];
+    Dp[(row + mn) * ldd + I] = v;
+  }
+}
+
+void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, const int* L, const int* f0,
+                  const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
+                  hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(pack_density_kernel, dim3(npairs), dim3(128), 0, st, s1, s2, row, L, f0, D, sI, si, sj,
+                     ndens, Dp, ldd);
+}
+
+// ---------------------------------------------------------------------------
+// part[s][P][I] = sum_{k in split s} T[k][P] * Dp[k][I]
+// T: npf x ncol (row-major, ldt), Dp: npf x ndens (row-major, ldd).
+// 64x64 output tile per workgroup, 256 threads with 4x4 each, K step 16.
+constexpr int GT = 64, GK = 16;
+
+__global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __restrict__ T, int64_t ldt,
+                                                             const double* __restrict__ Dp, int ldd, int64_t npf,
+                                                             int ncol, int ndens, int64_t kchunk,
+                                                             double* __restrict__ part) {
+  __shared__ double sT[GK][GT + 1];
+  __shared__ double sD[GK][GT + 1];
+  const int tp = blockIdx.x * GT, ti = blockIdx.y * GT, s = blockIdx.z;
+  const int64_t k0 = (int64_t)s * kchunk, k1 = min(npf, k0 + kchunk);
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;  // outputs P = tp + ty*4 + a, I = ti + tx*4 + b
+  double acc[4][4] = {};
+  for (int64_t kb = k0; kb < k1; kb += GK) {
+    for (int e = tid; e < GK * GT; e += 256) {
+      const int kk = e / GT, c = e - kk * GT;
+      const int64_t k = kb + kk;
+      sT[kk][c] = (k < k1 && tp + c < ncol) ? T[k * ldt + tp + c] : 0.0;
+      sD[kk][c] = (k < k1 && ti + c < ndens) ? Dp[k * ldd + ti + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        a[q] = sT[kk][ty * 4 + q];
+        b[q] = sD[kk][tx * 4 + q];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++) {
+      const int P = tp + ty * 4 + x, I = ti + tx * 4 + y;
+      if (P < ncol && I < ndens) part[((size_t)s * ncol + P) * ndens + I] = acc[x][y];
+    }
+}
+
+// d[orig(P) + ldd_out * I] += sum_s part[s][P][I]   (fixed summation order)
+__global__ void reduce_unpermute_kernel(const double* __restrict__ part, int nsplit, int ncol, int ndens,
+                                        const int* __restrict__ orig, double* __restrict__ d, int ldd_out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)ncol * ndens) return;
+  const int P = (int)(e / ndens), I = (int)(e - (int64_t)P * ndens);
+  double s = 0.0;
+  for (int z = 0; z < nsplit; z++) s += part[((size_t)z * ncol + P) * ndens + I];
+  d[orig[P] + (int64_t)ldd_out * I] += s;
+}
+
+void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int64_t npf, int ncol, int ndens,
+                double* part, int nsplit, const int* orig, double* d, int ldd_out, hipStream_t st) {
+  if (npf <= 0 || ncol <= 0) return;
+  int64_t kchunk = (npf + nsplit - 1) / nsplit;
+  kchunk = (kchunk + GK - 1) / GK * GK;
+  const int ns = (int)((npf + kchunk - 1) / kchunk);
+  dim3 grid((ncol + GT - 1) / GT, (ndens + GT - 1) / GT, ns);
+  hipLaunchKernelGGL(gemm_tn_splitk_kernel, grid, dim3(256), 0, st, T, ldt, Dp, ldd, npf, ncol, ndens, kchunk,
+                     part);
+  const int64_t n = (int64_t)ncol * ndens;
+  hipLaunchKernelGGL(reduce_unpermute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, ns, ncol,
+                     ndens, orig, d, ldd_out);
+}
+
+int contract_nsplit_max() { return 512; }
+
+// ---------------------------------------------------------------------------
+// Blocked right-looking Cholesky, lower, column-major A (n x n, lda = n).
+constexpr int CB = 64;
+
+// Factor the diagonal block at k0 in LDS, store L11 and inv(L11).
+__global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, int64_t n, int k0, int nb,
+                                                        double* __restrict__ inv, int* __restrict__ info) {
+  __shared__ double S[CB][CB + 1];
+  __shared__ double Iv[CB][CB + 1];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
+    S[r][c] = A[(k0 + r) + n * (k0 + c)];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; j++) {
+    if (tid == 0) {
+      double dj = S[j][j];
+      if (!(dj > 0.0)) {
+        if (!bad) {
+          bad = 1;
+          atomicCAS(info, 0, k0 + j + 1);
+        }
+        dj = 1.0;
+      }
+      S[j][j] = sqrt(dj);
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < nb; i += blockDim.x) S[i][j] /= S[j][j];
+    __syncthreads();
+    const int m = nb - j - 1;
+    for (int e = tid; e < m * m; e += blockDim.x) {
+      const int r = j + 1 + e % m, c = j + 1 + e / m;
+      if (r >= c) S[r][c] -= S[r][j] * S[c][j];
+    }
+    __syncthreads();
+  }
+  // inverse of the lower-triangular L11, one column per thread
+  if (tid < nb) {
+    const int c = tid;
+    for (int i = 0; i < nb; i++) Iv[i][c] = 0.0;
+    Iv[c][c] = 1.0 / S[c][c];
+    for (int i = c + 1; i < nb; i++) {
+      double s = 0.0;
+      for (int k = c; k < i; k++) s += S[i][k] * Iv[k][c];
+      Iv[i][c] = -s / S[i][i];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
+    if (r >= c) A[(k0 + r) + n * (k0 + c)] = S[r][c];
+    inv[r + CB * c] = (r >= c) ? Iv[r][c] : 0.0;
+  }
+}
+
+// L21 = A21 inv(L11)^T for 64 rows per workgroup.
+__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, int64_t n, int k0, int nb,
+                                                         const double* __restrict__ inv) {
+  __shared__ double X[CB][CB + 1];
+  __shared__ double Iv[CB][CB + 1];
+  const int r0 = k0 + nb + blockIdx.x * CB;
+  const int nr = min(CB, (int)(n - r0));
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
+    Iv[r][c] = inv[r + CB * c];
+  }
+  for (int e = tid; e < nr * nb; e += blockDim.x) {
+    const int r = e % nr, c = e / nr;
+    X[r][c] = A[(r0 + r) + n * (k0 + c)];
+  }
+  __syncthreads();
+  for (int e = tid; e < nr * nb; e += blockDim.x) {
+    const int r = e % nr, c = e / nr;
+    double s = 0.0;
+    for (int l = 0; l <= c; l++) s += X[r][l] * Iv[c][l];
+    A[(r0 + r) + n * (k0 + c)] = s;
+  }
+}
+
+// A22 -= L21 L21^T on the lower 64x64 tiles of the trailing matrix.
+__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int64_t n, int k0, int nb) {
+  __shared__ double Pr[CB][CB + 1];
+  __shared__ double Pc[CB][CB + 1];
+  const int m0 = k0 + nb;
+  // blockIdx.x -> (tr, tc) with tr >= tc
+  const int q = blockIdx.x;
+  int tr = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+  while ((tr + 1) * (tr + 2) / 2 <= q) tr++;
+  while (tr * (tr + 1) / 2 > q) tr--;
+  const int tc = q - tr * (tr + 1) / 2;
+  const int r0 = m0 + tr * CB, c0 = m0 + tc * CB;
+  const int nr = min(CB, (int)(n - r0)), ncl = min(CB, (int)(n - c0));
+  const int tid = threadIdx.x;
+  for (int e = tid; e < CB * nb; e += blockDim.x) {
+    const int r = e % CB, l = e / CB;
+    Pr[r][l] = (r < nr) ? A[(r0 + r) + n * (k0 + l)] : 0.0;
+    Pc[r][l] = (r < ncl) ? A[(c0 + r) + n * (k0 + l)] : 0.0;
+  }
+  __syncthreads();
+  const int tx = tid & 15, ty = tid >> 4;
+  double acc[4][4] = {};
+  for (int l = 0; l < nb; l++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int z = 0; z < 4; z++) {
+      a[z] = Pr[ty * 4 + z][l];
+      b[z] = Pc[tx * 4 + z][l];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+  }
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++) {
+      const int r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
+      if (r < n && c < n && r >= c) A[r + n * c] -= acc[x][y];
+    }
+}
+
+void cholesky(double* A, int64_t n, double* invblk, int* info, hipStream_t st) {
+  for (int k0 = 0; k0 < n; k0 += CB) {
+    const int nb = (int)std::min<int64_t>(CB, n - k0);
+    double* inv = invblk + (size_t)(k0 / CB) * CB * CB;
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, A, n, k0, nb, inv, info);
+    const int64_t m = n - k0 - nb;
+    if (m > 0) {
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), 0, st, A, n, k0, nb,
+                         inv);
+      const int64_t t = (m + CB - 1) / CB;
+      hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(t * (t + 1) / 2)), dim3(256), 0, st, A, n, k0, nb);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward solve L B = d in place (d column-major n x nrhs, ldd = n).
+__global__ __launch_bounds__(256) void trsm_diag_kernel(const double* __restrict__ inv, double* __restrict__ d,
+                                                        int64_t n, int k0, int nb, int nrhs) {
+  __shared__ double Iv[CB][CB + 1];
+  __shared__ double X[CB][CB + 1];
+  const int c0 = blockIdx.x * CB;
+  const int nc = min(CB, nrhs - c0);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
+    Iv[r][c] = inv[r + CB * c];
+  }
+  for (int e = tid; e < nb * nc; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
+    X[r][c] = d[(k0 + r) + n * (c0 + c)];
+  }
+  __syncthreads();
+  for (int e = tid; e < nb * nc; e += blockDim.x) {
+    const int r = e % nb, c = e / nb;
+    double s = 0.0;
+    for (int l = 0; l <= r; l++) s += Iv[r][l] * X[l][c];
+    d[(k0 + r) + n * (c0 + c)] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void trsm_update_kernel(const double* __restrict__ A, double* __restrict__ d,
+                                                          int64_t n, int k0, int nb, int nrhs) {
+  __shared__ double Lb[CB][CB + 1];
+  __shared__ double Xb[CB][CB + 1];
+  const int r0 = k0 + nb + blockIdx.x * CB, c0 = blockIdx.y * CB;
+  const int nr = min(CB, (int)(n - r0)), nc = min(CB, nrhs - c0);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < CB * nb; e += blockDim.x) {
+    const int r = e % CB, l = e / CB;
+    Lb[r][l] = (r < nr) ? A[(r0 + r) + n * (k0 + l)] : 0.0;
+    Xb[l][r] = (r < nc) ? d[(k0 + l) + n * (c0 + r)] : 0.0;
+  }
+  __syncthreads();
+  const int tx = tid & 15, ty = tid >> 4;
+  double acc[4][4] = {};
+  for (int l = 0; l < nb; l++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int z = 0; z < 4; z++) {
+      a[z] = Lb[ty * 4 + z][l];
+      b[z] = Xb[l][tx * 4 + z];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+  }
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++) {
+      const int r = ty * 4 + x, c = tx * 4 + y;
+      if (r < nr && c < nc) d[(r0 + r) + n * (c0 + c)] -= acc[x][y];
+    }
+}
+
+void forward_solve(const double* A, const double* invblk, double* d, int64_t n, int nrhs, hipStream_t st) {
+  for (int k0 = 0; k0 < n; k0 += CB) {
+    const int nb = (int)std::min<int64_t>(CB, n - k0);
+    const double* inv = invblk + (size_t)(k0 / CB) * CB * CB;
+    hipLaunchKernelGGL(trsm_diag_kernel, dim3((nrhs + CB - 1) / CB), dim3(256), 0, st, inv, d, n, k0, nb, nrhs);
+    const int64_t m = n - k0 - nb;
+    if (m > 0)
+      hipLaunchKernelGGL(trsm_update_kernel, dim3((unsigned)((m + CB - 1) / CB), (nrhs + CB - 1) / CB), dim3(256),
+                         0, st, A, d, n, k0, nb, nrhs);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// J assembly: for packed density pairs I2 <= I1, J = B(:,I1) . B(:,I2), written
+// to the 8 index permutations of the column-major nstates^4 tensor.
+__device__ inline void unpack_pair(int q, int& i, int& j) {
+  int a = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+  while ((a + 1) * (a + 2) / 2 <= q) a++;
+  while (a * (a + 1) / 2 > q) a--;
+  i = a;
+  j = q - a * (a + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void assemble_j_kernel(const double* __restrict__ B, int64_t naux, int nstates,
+                                                         double* __restrict__ J) {
+  __shared__ double red[256];
+  int I1, I2;
+  unpack_pair(blockIdx.x, I1, I2);
+  const double* b1 = B + naux * I1;
+  const double* b2 = B + naux * I2;
+  double s = 0.0;
+  for (int64_t m = threadIdx.x; m < naux; m += 256) s = fma(b1[m], b2[m], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int i, j, k, l;
+    unpack_pair(I1, i, j);
+    unpack_pair(I2, k, l);
+    const int64_t ns = nstates;
+    auto at = [&](int a, int b, int c, int d) -> double& { return J[a + ns * (b + ns * (c + ns * d))]; };
+    const double v = red[0];
+    at(i, j, k, l) = v;
+    at(j, i, k, l) = v;
+    at(i, j, l, k) = v;
+    at(j, i, l, k) = v;
+    at(k, l, i, j) = v;
+    at(k, l, j, i) = v;
+    at(l, k, i, j) = v;
+    at(l, k, j, i) = v;
+  }
+}
+
+void assemble_j(const double* B, int64_t naux, int ndens, int nstates, double* J, hipStream_t st) {
+  const int npairs = ndens * (ndens + 1) / 2;
+  if (npairs > 0) hipLaunchKernelGGL(assemble_j_kernel, dim3(npairs), dim3(256), 0, st, B, naux, nstates, J);
+}
+
+// ---------------------------------------------------------------------------
+// Q_C = sqrt(||(C|C)||_F) from the diagonal blocks of the metric (row-major =
+// column-major, symmetric).
+__global__ void schwarz_aux_kernel(const double* __restrict__ M, int64_t n, const int* __restrict__ L,
+                                   const int* __restrict__ f0, int nsh, double* __restrict__ q) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nsh) return;
+  const int nc = 2 * L[c] + 1;
+  const int64_t o = f0[c];
+  double s = 0.0;
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < nc; j++) {
+      const double v = M[(o + i) * n + o + j];
+      s += v * v;
+    }
+  q[c] = sqrt(sqrt(s));
+}
+
+void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,
+                             hipStream_t st) {
+  if (nsh > 0) hipLaunchKernelGGL(schwarz_aux_kernel, dim3((nsh + 127) / 128), dim3(128), 0, st, M, n, L, f0, nsh, q);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void boys_probe_kernel(const int* __restrict__ nv, const double* __restrict__ T, int64_t npts,
+                                  const double* __restrict__ tab, double* __restrict__ F) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npts) F[i] = boys_f(nv[i], T[i], tab);
+}
+
+void boys_probe(const int* n, const double* T, int64_t npts, const double* tab, double* F, hipStream_t st) {
+  if (npts > 0)
+    hipLaunchKernelGGL(boys_probe_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, n, T, npts, tab, F);
+}
+
+}  // namespace dfhip

@@ -1,0 +1,203 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+The GPU evaluates integrals with spherical Hermite bra matrices, Hobson-
+transformed auxiliary functions and closed-form R_tuv; the oracle with
+Cartesian McMurchie-Davidson on every centre and a final spherical transform.
+Tolerances are written per test.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pysisyphus_addons_amd import _native as N
+from pysisyphus_addons_amd import synthetic
+from pysisyphus_addons_amd.diabatization import _intor, intor
+from pysisyphus_addons_amd.intor import intor as sintor
+from pysisyphus_addons_amd.shells import Shell, Shells
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def dense3c(shells, aux, screening=N.SCREEN_SCHWARZ):
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    out = np.zeros((shells.nbf, shells.nbf, aux.nbf))
+    opt = N.default_options(screening=screening)
+    st = N.Stats()
+    N.check(N.lib().dfhip_int3c2e_dense(ctypes.byref(b.struct), ctypes.byref(x.struct),
+                                        ctypes.byref(opt), out.ctypes.data, ctypes.byref(st)))
+    return out, st.as_dict()
+
+
+def oracle_dense3c(shells, aux, screened=True):
+    bas, x = O.Basis.from_shells(shells), O.Basis.from_shells(aux)
+    qb, qa = O.schwarz_bra(bas), O.schwarz_aux(x)
+    out = np.zeros((shells.nbf, shells.nbf, aux.nbf))
+    for a in range(bas.nshells):
+        for b in range(a + 1):
+            for c in range(x.nshells):
+                if screened and qb[a * (a + 1) // 2 + b] * qa[c] < O.SCHWARZ_THRESH:
+                    continue
+                blk = O.int3c2e_block(bas, x, a, b, c)
+                sa = slice(bas.off[a], bas.off[a] + blk.shape[0])
+                sb = slice(bas.off[b], bas.off[b] + blk.shape[1])
+                sc = slice(x.off[c], x.off[c] + blk.shape[2])
+                out[sa, sb, sc] = blk
+                out[sb, sa, sc] = blk.transpose(1, 0, 2)
+    return out
+
+
+def test_boys_probe():
+    """Boys F_m(T), m <= 12, T log-uniform in [1e-6, 120] (the Boys probe of
+    the 60-atom workload), plus orders to 17 and the table/asymptotic seam."""
+    rng = np.random.default_rng(20260001)
+    n = rng.integers(0, 13, 200000).astype(np.int32)
+    T = np.exp(rng.uniform(np.log(1e-6), np.log(120.0), n.size))
+    extra_T = np.array([0.0, 0.025, 39.97, 39.99, 40.0, 40.01, 60.0, 1e-14])
+    T = np.concatenate([T, np.repeat(extra_T, 18)])
+    n = np.concatenate([n, np.tile(np.arange(18, dtype=np.int32), extra_T.size)])
+    F = np.zeros_like(T)
+    N.check(N.lib().dfhip_boys(T.size, n.ctypes.data, T.ctypes.data, F.ctypes.data))
+    ref = np.array([O.boys(int(k), np.array([t]))[0, k] for k, t in zip(n[-200:], T[-200:])])
+    Fref = O.boys(22, T)[np.arange(T.size), n]
+    np.testing.assert_allclose(F[-200:], ref, rtol=1e-14, atol=0)
+    err = np.abs(F - Fref)
+    _say(f"boys: max abs {err.max():.2e}, max rel {(err / Fref).max():.2e}")
+    assert err.max() <= 1e-14
+    assert (err / Fref).max() <= 5e-14
+
+
+def _mixed_system(seed=3):
+    rng = np.random.default_rng(seed)
+    orb, aux = [], []
+    centers = rng.uniform(-1.5, 1.5, (3, 3))
+    for ci, C in enumerate(centers):
+        for L in (0, 1, 2, 3):
+            e = 0.3 * 3.0 ** np.arange(2) * rng.uniform(0.5, 2.0)
+            orb.append(Shell(L=L, center=C, exps=e, coeffs=rng.uniform(0.2, 1.0, 2), center_ind=ci))
+        for L in (0, 1, 2, 3, 4, 5):
+            aux.append(Shell(L=L, center=C, exps=[rng.uniform(0.2, 3.0)], coeffs=[1.0], center_ind=ci))
+    return Shells(orb), Shells(aux)
+
+
+def test_int3c2e_blocks_all_classes_unscreened():
+    """Every (la, lb, lc) class with la, lb <= f and lc <= h, no screening."""
+    shells, aux = _mixed_system()
+    got, st = dense3c(shells, aux, screening=N.SCREEN_NONE)
+    ref = oracle_dense3c(shells, aux, screened=False)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max()
+    _say(f"3c2e all classes: max abs err {err:.2e} (max |int| {scale:.2e})")
+    assert st["ncomputed"] == st["ntriplets"]
+    assert err <= 1e-12 * max(scale, 1.0)
+
+
+def test_int3c2e_blocks_contracted_aux():
+    """Contracted auxiliary shells (aux = orbital basis, as in the reference's
+    Coulomb-tensor test) with variable primitive counts."""
+    rng = np.random.default_rng(9)
+    sh = []
+    for ci, C in enumerate(rng.uniform(-1.0, 1.0, (2, 3))):
+        for L, np_ in ((0, 3), (1, 2), (2, 1), (0, 6), (2, 4)):
+            sh.append(Shell(L=L, center=C, exps=np.sort(rng.uniform(0.1, 8.0, np_))[::-1],
+                            coeffs=rng.uniform(0.1, 1.0, np_), center_ind=ci))
+    shells = Shells(sh)
+    got, _ = dense3c(shells, shells, screening=N.SCREEN_NONE)
+    ref = oracle_dense3c(shells, shells, screened=False)
+    err = np.abs(got - ref).max()
+    _say(f"3c2e contracted aux: max abs err {err:.2e}")
+    assert err <= 1e-12 * max(np.abs(ref).max(), 1.0)
+
+
+def test_metric_and_schwarz():
+    shells, aux = _mixed_system(4)
+    x = N.BasisArrays(*aux.as_arrays())
+    M = np.zeros((aux.nbf, aux.nbf))
+    N.check(N.lib().dfhip_metric(ctypes.byref(x.struct), M.ctypes.data))
+    Mref = O.metric(O.Basis.from_shells(aux))
+    err = np.abs(M - Mref).max()
+    _say(f"metric: max abs err {err:.2e}")
+    assert err <= 1e-12 * np.abs(Mref).max()
+    qa = sintor.int_schwarz_aux(aux)
+    np.testing.assert_allclose(qa, O.schwarz_aux(O.Basis.from_shells(aux)), rtol=1e-13)
+    qb = sintor.int_schwarz_bra(shells)
+    np.testing.assert_allclose(qb, O.schwarz_bra(O.Basis.from_shells(shells)), rtol=1e-12)
+
+
+def test_schwarz_bra_g_shells():
+    """(ab|ab) up to (gg|gg), including the chunked classes."""
+    rng = np.random.default_rng(12)
+    sh = [Shell(L=L, center=rng.uniform(-1, 1, 3), exps=[0.7, 2.1], coeffs=[0.5, 0.6])
+          for L in (4, 3, 4, 0, 2)]
+    shells = Shells(sh)
+    qb = sintor.int_schwarz_bra(shells)
+    np.testing.assert_allclose(qb, O.schwarz_bra(O.Basis.from_shells(shells)), rtol=1e-12)
+
+
+def test_workload0_df_and_j_vs_oracle():
+    shells, aux, dens = synthetic.workload(0)
+    _say(f"workload 0: nbf {shells.nbf} naux {aux.nbf} ndens {dens.shape[0]}")
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    df, J, st = _intor.contract(b, x, dens, nstates=3, want_df=True, want_j=True)
+    Jref, info = O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True)
+    assert st["ncomputed"] == info["ncomputed"]
+    assert st["nskipped"] == info["nskipped"]
+    # d before the metric solve has no conditioning amplification
+    errJ = np.abs(J - Jref).max()
+    _say(f"workload 0: max |dJ| {errJ:.2e}, |J|max {np.abs(Jref).max():.3e}")
+    assert errJ <= 1e-10
+    np.testing.assert_allclose(df, info["B"], atol=1e-6 * np.abs(info["B"]).max())
+
+
+def test_workload0_screened_blocks():
+    shells, aux, _ = synthetic.workload(0)
+    got, st = dense3c(shells, aux)
+    ref = oracle_dense3c(shells, aux, screened=True)
+    err = np.abs(got - ref).max()
+    _say(f"workload 0 screened (ab|P): max abs err {err:.2e}")
+    assert err <= 1e-12 * np.abs(ref).max()
+
+
+def test_bena2_golden_through_dropin(bena2):
+    shells = Shells.from_arrays(bena2["bas_centers"], bena2["bas_spec"], bena2["bas_data"])
+    J = intor.contract_coulomb_densities_4d(shells, shells, bena2["densities"])
+    err = np.abs(J - bena2["J_ref"]).max()
+    _say(f"bena2 golden: max |dJ| {err:.2e}")
+    np.testing.assert_allclose(J, bena2["J_ref"], atol=1e-10)
+
+
+def test_2d_entry_matches_4d(bena2):
+    shells = Shells.from_arrays(bena2["bas_centers"], bena2["bas_spec"], bena2["bas_data"])
+    B = intor.contract_coulomb_densities_2d(shells, shells, bena2["densities"])
+    assert B.shape == (shells.nbf, 3) and B.flags.f_contiguous
+    J = intor.contract_coulomb_densities_4d(shells, shells, bena2["densities"])
+    assert J[0, 1, 0, 1] == pytest.approx(B[:, 1] @ B[:, 1], rel=1e-12)
+
+
+def test_workload1_vs_oracle():
+    shells, aux, dens = synthetic.workload(1)
+    _say(f"workload 1: nbf {shells.nbf} naux {aux.nbf} ndens {dens.shape[0]}")
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    _, J, st = _intor.contract(b, x, dens, nstates=8, want_df=False, want_j=True)
+    Jref, info = O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True)
+    assert st["ncomputed"] == info["ncomputed"]
+    errJ = np.abs(J - Jref).max()
+    _say(f"workload 1: max |dJ| {errJ:.2e} |J|max {np.abs(Jref).max():.3e} "
+         f"triplets {st['ncomputed']} / {st['ntriplets']}")
+    assert errJ <= 1e-10
+
+
+def test_bitwise_reproducible():
+    shells, aux, dens = synthetic.workload(0)
+    J1 = intor.contract_coulomb_densities_4d(shells, aux, dens)
+    J2 = intor.contract_coulomb_densities_4d(shells, aux, dens)
+    assert np.array_equal(J1, J2)

@@ -1,0 +1,108 @@
+"""Host-side checks of libdfhip.so that need no GPU: every symbol declared in
+include/dfhip.h is exported, the Python shims validate their inputs like the
+reference's typed memoryviews, and a call without a usable device reports an
+error status instead of aborting (the reference aborts, common/normalization.f90:17-25)."""
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pysisyphus_addons_amd import _native as N
+from pysisyphus_addons_amd import synthetic
+from pysisyphus_addons_amd.shells import Shell, Shells
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "dfhip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(f_[a-z0-9_]+|dfhip_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_header_symbols():
+    L = N.lib()
+    declared = _declared_symbols()
+    assert {"f_contract_coulomb_densities_2d", "f_contract_coulomb_densities_4d", "f_init",
+            "f_int_schwarz_aux", "f_int_schwarz_bra"} <= declared
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(N.EXPORTED) <= declared
+
+
+def test_shell_arrays_roundtrip():
+    shells, aux, _ = synthetic.molecule(3, 5.0, seed=1)
+    bc, bs, bd = shells.as_arrays(fortran=True)
+    assert bc.dtype == np.int32 and bs.dtype == np.int32 and bd.dtype == np.float64
+    back = Shells.from_arrays(bc, bs, bd)
+    assert len(back) == len(shells)
+    for a, b in zip(back.shells, shells.shells):
+        assert a.L == b.L
+        np.testing.assert_array_equal(a.exps, b.exps)
+        np.testing.assert_array_equal(a.coeffs, b.coeffs)
+        np.testing.assert_array_equal(a.center, b.center)
+
+
+def test_shim_rejects_wrong_dtype_and_layout():
+    from pysisyphus_addons_amd.diabatization import _intor
+
+    shells, aux, dens = synthetic.molecule(2, 4.0, nstates=1, seed=2)
+    a = shells.as_arrays()
+    x = aux.as_arrays()
+    with pytest.raises(ValueError):
+        _intor.contract_coulomb_densities_4d(a[0].astype(np.int64), a[1], a[2], *x, dens)
+    with pytest.raises(ValueError):
+        _intor.contract_coulomb_densities_4d(*a, *x, np.asfortranarray(dens))
+    with pytest.raises(ValueError):
+        _intor.contract_coulomb_densities_4d(*a, *x, dens.astype(np.float32))
+
+
+def test_no_device_reports_status(monkeypatch):
+    """In a process without a visible GPU the library must return an error
+    status (no abort, no CPU fallback)."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from pysisyphus_addons_amd.diabatization import intor
+
+    shells, aux, dens = synthetic.molecule(2, 4.0, nstates=1, seed=2)
+    with pytest.raises(N.DfhipError) as e:
+        intor.contract_coulomb_densities_4d(shells, aux, dens)
+    assert e.value.status == 2
+
+
+def test_workload_recipes():
+    shells, aux, dens = synthetic.workload(0)
+    assert len(shells) == 8 * 6 and shells.nbf == 112
+    assert len(aux) == 8 * 10 and aux.nbf == 240
+    assert dens.shape == (6, 112, 112)
+    np.testing.assert_allclose(np.trace(dens, axis1=1, axis2=2), 1.0)
+    np.testing.assert_allclose(dens, dens.transpose(0, 2, 1))
+    ex = np.concatenate([s.exps for s in shells.shells])
+    assert ex.min() >= 0.1 - 1e-12 and ex.max() <= 0.1 * 3 ** 7 + 1e-9
+
+
+def test_invalid_basis_reports_status():
+    """Basis errors are reported before any device work (the reference
+    aborts the process, common/normalization.f90:17-25)."""
+    from pysisyphus_addons_amd.diabatization import intor
+
+    shells, aux, dens = synthetic.molecule(2, 4.0, nstates=1, seed=2)
+    bad = Shells([Shell(L=6, center=[0, 0, 0], exps=[1.0], coeffs=[1.0])] + shells.shells[1:])
+    with pytest.raises(N.DfhipError) as e:
+        intor.contract_coulomb_densities_4d(bad, aux, dens)
+    assert e.value.status == 4
+    neg = Shells([Shell(L=0, center=[0, 0, 0], exps=[-1.0], coeffs=[1.0])] + shells.shells[1:])
+    with pytest.raises(N.DfhipError) as e:
+        intor.contract_coulomb_densities_4d(neg, aux, dens)
+    assert e.value.status == 1
+    from pysisyphus_addons_amd.diabatization import _intor
+
+    a, x = shells.as_arrays(), aux.as_arrays()
+    with pytest.raises(N.DfhipError) as e:
+        _intor.contract_coulomb_densities_4d(*a, *x, np.ascontiguousarray(dens[:, :-1, :-1]))
+    assert e.value.status == 1
