@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/dfhip.h"
+#include "gen3c_classes.h"
 #include "int3c2e_impl.h"
 #include "linalg.h"
 #include "schwarz4c_impl.h"
@@ -168,6 +169,7 @@ struct Ctx {
   DBuf<SphTable> sph;
   Int3Table int3{};
   Schwarz4Table s4{};
+  Gen3Table gen3{};
 };
 
 Ctx& ctx() {
@@ -203,6 +205,7 @@ void ensure_ctx() {
   register_int3c2e_l4(c.int3);
   register_int3c2e_l5(c.int3);
   register_schwarz4c(c.s4);
+  register_gen3_all(c.gen3);
   for (int la = 0; la <= LMAX_AUX; la++)
     for (int lb = 0; lb <= (la <= LMAX_ORB ? la : 0); lb++)
       for (int lc = 0; lc <= LMAX_AUX; lc++)
@@ -393,6 +396,64 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
   return q;
 }
 
+bool use_generated() {
+  const char* v = std::getenv("DFHIP_NO_GEN");
+  return !(v && *v && *v != '0');
+}
+
+inline bool gen3_has(int la, int lb, int lc) {
+  return la <= GEN3_LA_MAX && lb <= la && lc <= GEN3_LC_MAX;
+}
+
+void launch_bra_records(int la, int lb, DevShells bra, const int* s1, const int* s2, const int64_t* recoff, int rs,
+                        int npairs, double* recs, hipStream_t st) {
+  if (npairs <= 0) return;
+#define DFHIP_BR(A, B)                                                                                          \
+  if (la == A && lb == B) {                                                                                    \
+    hipLaunchKernelGGL((bra_records_kernel<A, B>), dim3(npairs), dim3(64), 0, st, bra, s1, s2, recoff, rs, recs); \
+    return;                                                                                                    \
+  }
+  DFHIP_BR(0, 0) DFHIP_BR(1, 0) DFHIP_BR(1, 1) DFHIP_BR(2, 0) DFHIP_BR(2, 1) DFHIP_BR(2, 2)
+#undef DFHIP_BR
+  throw Error{DFHIP_ERR_UNSUPPORTED, "no bra-record kernel for this class"};
+}
+
+// Bra records of the generated kernels for one pair class: offsets (in
+// doubles, per pair) and the records themselves.
+struct BraRecords {
+  DBuf<int64_t> off;
+  DBuf<int> npp;
+  DBuf<double> recs;
+};
+
+void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevShells& dev, const int* ds1,
+                       const int* ds2, const std::vector<int>& s1, const std::vector<int>& s2, BraRecords& br) {
+  const int rs = c.gen3[la][lb][0].rs;
+  std::vector<int64_t> off(s1.size());
+  std::vector<int> npp(s1.size());
+  int64_t o = 0;
+  for (size_t p = 0; p < s1.size(); p++) {
+    off[p] = o;
+    npp[p] = bra.nprim[s1[p]] * (s2[p] >= 0 ? bra.nprim[s2[p]] : 1);
+    o += (int64_t)npp[p] * rs;
+  }
+  upload(br.off, off, c.st);
+  upload(br.npp, npp, c.st);
+  br.recs.alloc(std::max<int64_t>(o, 1));
+  launch_bra_records(la, lb, dev, ds1, ds2, br.off.p, rs, (int)s1.size(), br.recs.p, c.st);
+  HC(hipGetLastError());
+}
+
+void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells ket, PairArgs pa, AuxArgs xa,
+                 double* out, int64_t ldo) {
+  const Gen3Launch& g = c.gen3[la][lb][lc];
+  const int nchunks = (xa.hi - xa.lo + 63) / 64;
+  if (pa.npairs <= 0 || nchunks <= 0) return;
+  hipLaunchKernelGGL(g.kernel, dim3(pa.npairs, nchunks, g.groups), dim3(64), 0, c.st, br.recs.p, br.off.p, br.npp.p,
+                     ket, pa, xa, c.boys.p, out, ldo);
+  HC(hipGetLastError());
+}
+
 // Full metric (P|Q) (naux x naux) into M via the 2c2e form of the 3c2e kernels.
 void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M) {
   hipStream_t st = c.st;
@@ -425,16 +486,27 @@ void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M) {
   upload(drow, row, st);
   upload(dlsh, lsh, st);
   upload(dlcol, lcol, st);
+  const bool gen = use_generated();
+  std::vector<BraRecords> brs(LMAX_AUX + 1);
   for (int LP = 0; LP <= LMAX_AUX; LP++) {
     const int np = lbeg[LP + 1] - lbeg[LP];
     if (np == 0) continue;
     PairArgs pa{ds1.p + lbeg[LP], ds2.p + lbeg[LP], drow.p + lbeg[LP], dns.p + lbeg[LP], np};
+    if (gen && gen3_has(LP, 0, 0)) {
+      std::vector<int> a1(s1.begin() + lbeg[LP], s1.begin() + lbeg[LP + 1]);
+      std::vector<int> a2(s2.begin() + lbeg[LP], s2.begin() + lbeg[LP + 1]);
+      build_bra_records(c, LP, 0, x, dev, ds1.p + lbeg[LP], ds2.p + lbeg[LP], a1, a2, brs[LP]);
+    }
     for (int LQ = 0; LQ <= LMAX_AUX; LQ++) {
       const int nq = lbeg[LQ + 1] - lbeg[LQ];
       if (nq == 0) continue;
       AuxArgs xa{dlsh.p + lbeg[LQ], dlcol.p + lbeg[LQ], 0, nq, kmax[LQ]};
-      c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, x.nbf, st);
-      HC(hipGetLastError());
+      if (gen && gen3_has(LP, 0, LQ)) {
+        launch_gen3(c, LP, 0, LQ, brs[LP], dev, pa, xa, M, x.nbf);
+      } else {
+        c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, x.nbf, st);
+        HC(hipGetLastError());
+      }
     }
   }
   HC(hipStreamSynchronize(st));
@@ -649,6 +721,18 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   DBuf<int> dorig;
   upload(dorig, pl.orig, st);
+  const bool gen = use_generated();
+  std::vector<BraRecords> brs(NPCLS);
+  if (gen) {
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
+        if (p1 == p0 || !gen3_has(la, lb, 0)) continue;
+        std::vector<int> a1(pl.ps.s1.begin() + p0, pl.ps.s1.begin() + p1);
+        std::vector<int> a2(pl.ps.s2.begin() + p0, pl.ps.s2.begin() + p1);
+        build_bra_records(c, la, lb, o, dorb.view, ps1.p + p0, ps2.p + p0, a1, a2, brs[k]);
+      }
+  }
 
   // densities: upload and pair-pack (diabatization/intor.f90:155-176 symmetry)
   DBuf<double> Dp;
@@ -736,8 +820,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
           if (np == 0) continue;
           PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dnsurv[L].p + p0, np};
           AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi, kmax[L]};
-          c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, T.p - cA, w, st);
-          HC(hipGetLastError());
+          if (gen && gen3_has(la, lb, L)) {
+            launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, T.p - cA, w);
+          } else {
+            c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, T.p - cA, w, st);
+            HC(hipGetLastError());
+          }
         }
     }
     const int b1 = tm.mark();

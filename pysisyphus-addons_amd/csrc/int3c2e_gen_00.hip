@@ -1,0 +1,443 @@
+// Generated by gen3c.py -- 3c2e classes (00|lc), lc <= 3.
+#include "gen3c_common.h"
+
+namespace dfhip {
+
+// class (00|0): L = 0, record 5 doubles, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_000(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[1];
+#pragma unroll
+    for (int q = 0; q < 1; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 5;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      double F[1];
+      boys_array<0>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      double W[1];
+#pragma unroll
+      for (int q = 0; q < 1; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      {
+      const double y0_0_0 = py0 * z0_0;
+      { const double r = px0 * y0_0_0;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[0];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (00|1): L = 1, record 5 doubles, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_001(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 5;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g; }
+      double F[2];
+      boys_array<1>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      double W[3];
+#pragma unroll
+      for (int q = 0; q < 3; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      {
+      const double y0_0_1 = py0 * z0_1;
+      { const double r = px1 * y0_0_1;
+        W[2] = fma(1.0, r, W[2]);
+      }
+      }
+      {
+      const double y1_0_1 = py1 * z0_1;
+      { const double r = px0 * y1_0_1;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      {
+      const double y0_1_1 = py0 * z1_1;
+      { const double r = px0 * y0_1_1;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+        acc[2] = fma(e, W[2], acc[2]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[1];
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[2];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (00|2): L = 2, record 5 doubles, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_002(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 5;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= i2g * i2g; }
+      double F[3];
+      boys_array<2>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      double W[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      {
+      const double y0_0_2 = py0 * z0_2;
+      const double y0_0_1 = py0 * z0_1;
+      { const double r = px2 * y0_0_2 + 2.0 * px0 * y0_0_1;
+        W[2] = fma(-0.2886751345948129, r, W[2]);
+        W[4] = fma(0.5, r, W[4]);
+      }
+      }
+      {
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px1 * y1_0_2;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      }
+      {
+      const double y2_0_2 = py2 * z0_2 + 2.0 * py0 * z0_1;
+      { const double r = px0 * y2_0_2;
+        W[2] = fma(-0.2886751345948129, r, W[2]);
+        W[4] = fma(-0.5, r, W[4]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      {
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px1 * y0_1_2;
+        W[3] = fma(1.0, r, W[3]);
+      }
+      }
+      {
+      const double y1_1_2 = py1 * z1_2;
+      { const double r = px0 * y1_1_2;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      {
+      const double y0_2_2 = py0 * z2_2;
+      { const double r = px0 * y0_2_2;
+        W[2] = fma(0.5773502691896258, r, W[2]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+        acc[4] = fma(e, W[4], acc[4]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[1];
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[2];
+    out[(row0 + 0) * ldo + col + 3] = 1.0 * acc[3];
+    out[(row0 + 0) * ldo + col + 4] = 1.0 * acc[4];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (00|3): L = 3, record 5 doubles, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_003(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 5;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[4];
+      boys_array<3>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      double W[7];
+#pragma unroll
+      for (int q = 0; q < 7; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      {
+      const double y0_0_3 = py0 * z0_3;
+      const double y0_0_2 = py0 * z0_2;
+      { const double r = px3 * y0_0_3 + 6.0 * px1 * y0_0_2;
+        W[4] = fma(-0.15811388300841894, r, W[4]);
+        W[6] = fma(0.20412414523193148, r, W[6]);
+      }
+      }
+      {
+      const double y1_0_3 = py1 * z0_3;
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[0] = fma(0.6123724356957945, r, W[0]);
+        W[2] = fma(-0.15811388300841894, r, W[2]);
+      }
+      }
+      {
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[4] = fma(-0.15811388300841894, r, W[4]);
+        W[6] = fma(-0.6123724356957945, r, W[6]);
+      }
+      }
+      {
+      const double y3_0_3 = py3 * z0_3 + 6.0 * py1 * z0_2;
+      { const double r = px0 * y3_0_3;
+        W[0] = fma(-0.20412414523193148, r, W[0]);
+        W[2] = fma(-0.15811388300841894, r, W[2]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      {
+      const double y0_1_3 = py0 * z1_3;
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[3] = fma(-0.38729833462074165, r, W[3]);
+        W[5] = fma(0.5, r, W[5]);
+      }
+      }
+      {
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px1 * y1_1_3;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[3] = fma(-0.38729833462074165, r, W[3]);
+        W[5] = fma(-0.5, r, W[5]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      {
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px1 * y0_2_3;
+        W[4] = fma(0.6324555320336758, r, W[4]);
+      }
+      }
+      {
+      const double y1_2_3 = py1 * z2_3;
+      { const double r = px0 * y1_2_3;
+        W[2] = fma(0.6324555320336758, r, W[2]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      {
+      const double y0_3_3 = py0 * z3_3;
+      { const double r = px0 * y0_3_3;
+        W[3] = fma(0.2581988897471611, r, W[3]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+        acc[4] = fma(e, W[4], acc[4]);
+        acc[5] = fma(e, W[5], acc[5]);
+        acc[6] = fma(e, W[6], acc[6]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[1];
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[2];
+    out[(row0 + 0) * ldo + col + 3] = 1.0 * acc[3];
+    out[(row0 + 0) * ldo + col + 4] = 1.0 * acc[4];
+    out[(row0 + 0) * ldo + col + 5] = 1.0 * acc[5];
+    out[(row0 + 0) * ldo + col + 6] = 1.0 * acc[6];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+void register_gen3_00(Gen3Table& t) {
+  t[0][0][0] = Gen3Launch{&int3c2e_gen_000, 1, 5};
+  t[0][0][1] = Gen3Launch{&int3c2e_gen_001, 1, 5};
+  t[0][0][2] = Gen3Launch{&int3c2e_gen_002, 1, 5};
+  t[0][0][3] = Gen3Launch{&int3c2e_gen_003, 1, 5};
+}
+
+}  // namespace dfhip

@@ -1,0 +1,3380 @@
+// Generated by gen3c.py -- 3c2e classes (11|lc), lc <= 3.
+#include "gen3c_common.h"
+
+namespace dfhip {
+
+// class (11|0): L = 2, record 37 doubles, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      double F[3];
+      boys_array<2>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      double W[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      {
+      const double y0_0_0 = py0 * z0_0;
+      { const double r = px0 * y0_0_0;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      const double y0_0_1 = py0 * z0_1;
+      { const double r = px1 * y0_0_1;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      const double y0_0_2 = py0 * z0_2;
+      { const double r = px2 * y0_0_2 + 2.0 * px0 * y0_0_1;
+        W[4] = fma(1.0, r, W[4]);
+      }
+      }
+      {
+      const double y1_0_1 = py1 * z0_1;
+      { const double r = px0 * y1_0_1;
+        W[2] = fma(1.0, r, W[2]);
+      }
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px1 * y1_0_2;
+        W[5] = fma(1.0, r, W[5]);
+      }
+      }
+      {
+      const double y2_0_2 = py2 * z0_2 + 2.0 * py0 * z0_1;
+      { const double r = px0 * y2_0_2;
+        W[7] = fma(1.0, r, W[7]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      {
+      const double y0_1_1 = py0 * z1_1;
+      { const double r = px0 * y0_1_1;
+        W[3] = fma(1.0, r, W[3]);
+      }
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px1 * y0_1_2;
+        W[6] = fma(1.0, r, W[6]);
+      }
+      }
+      {
+      const double y1_1_2 = py1 * z1_2;
+      { const double r = px0 * y1_1_2;
+        W[8] = fma(1.0, r, W[8]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      {
+      const double y0_2_2 = py0 * z2_2;
+      { const double r = px0 * y0_2_2;
+        W[9] = fma(1.0, r, W[9]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[1], acc[0]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[4], acc[0]);
+      }
+      { const double e = rec[7];
+        acc[1] = fma(e, W[0], acc[1]);
+      }
+      { const double e = rec[8];
+        acc[1] = fma(e, W[2], acc[1]);
+      }
+      { const double e = rec[9];
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[10];
+        acc[1] = fma(e, W[5], acc[1]);
+      }
+      { const double e = rec[11];
+        acc[2] = fma(e, W[0], acc[2]);
+      }
+      { const double e = rec[12];
+        acc[2] = fma(e, W[3], acc[2]);
+      }
+      { const double e = rec[13];
+        acc[2] = fma(e, W[1], acc[2]);
+      }
+      { const double e = rec[14];
+        acc[2] = fma(e, W[6], acc[2]);
+      }
+      { const double e = rec[15];
+        acc[3] = fma(e, W[0], acc[3]);
+      }
+      { const double e = rec[16];
+        acc[3] = fma(e, W[2], acc[3]);
+      }
+      { const double e = rec[17];
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[18];
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[19];
+        acc[4] = fma(e, W[0], acc[4]);
+      }
+      { const double e = rec[20];
+        acc[4] = fma(e, W[2], acc[4]);
+      }
+      { const double e = rec[21];
+        acc[4] = fma(e, W[7], acc[4]);
+      }
+      { const double e = rec[22];
+        acc[5] = fma(e, W[0], acc[5]);
+      }
+      { const double e = rec[23];
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[24];
+        acc[5] = fma(e, W[2], acc[5]);
+      }
+      { const double e = rec[25];
+        acc[5] = fma(e, W[8], acc[5]);
+      }
+      { const double e = rec[26];
+        acc[6] = fma(e, W[0], acc[6]);
+      }
+      { const double e = rec[27];
+        acc[6] = fma(e, W[3], acc[6]);
+      }
+      { const double e = rec[28];
+        acc[6] = fma(e, W[1], acc[6]);
+      }
+      { const double e = rec[29];
+        acc[6] = fma(e, W[6], acc[6]);
+      }
+      { const double e = rec[30];
+        acc[7] = fma(e, W[0], acc[7]);
+      }
+      { const double e = rec[31];
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[32];
+        acc[7] = fma(e, W[2], acc[7]);
+      }
+      { const double e = rec[33];
+        acc[7] = fma(e, W[8], acc[7]);
+      }
+      { const double e = rec[34];
+        acc[8] = fma(e, W[0], acc[8]);
+      }
+      { const double e = rec[35];
+        acc[8] = fma(e, W[3], acc[8]);
+      }
+      { const double e = rec[36];
+        acc[8] = fma(e, W[9], acc[8]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[4];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[5];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[3];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[7];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[8];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[1];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[2];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (11|1): L = 3, record 37 doubles, 2 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g; }
+      double F[4];
+      boys_array<3>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      double W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      {
+      }
+      {
+      const double y1_0_1 = py1 * z0_1;
+      { const double r = px0 * y1_0_1;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px1 * y1_0_2;
+        W[2] = fma(1.0, r, W[2]);
+      }
+      const double y1_0_3 = py1 * z0_3;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[8] = fma(1.0, r, W[8]);
+      }
+      }
+      {
+      const double y2_0_2 = py2 * z0_2 + 2.0 * py0 * z0_1;
+      { const double r = px0 * y2_0_2;
+        W[4] = fma(1.0, r, W[4]);
+      }
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[10] = fma(1.0, r, W[10]);
+      }
+      }
+      {
+      const double y3_0_3 = py3 * z0_3 + 6.0 * py1 * z0_2;
+      { const double r = px0 * y3_0_3;
+        W[14] = fma(1.0, r, W[14]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      {
+      const double y0_1_1 = py0 * z1_1;
+      { const double r = px0 * y0_1_1;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px1 * y0_1_2;
+        W[3] = fma(1.0, r, W[3]);
+      }
+      const double y0_1_3 = py0 * z1_3;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[9] = fma(1.0, r, W[9]);
+      }
+      }
+      {
+      const double y1_1_2 = py1 * z1_2;
+      { const double r = px0 * y1_1_2;
+        W[6] = fma(1.0, r, W[6]);
+        W[5] = fma(1.0, r, W[5]);
+      }
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px1 * y1_1_3;
+        W[12] = fma(1.0, r, W[12]);
+        W[11] = fma(1.0, r, W[11]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[16] = fma(1.0, r, W[16]);
+        W[15] = fma(1.0, r, W[15]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      {
+      const double y0_2_2 = py0 * z2_2;
+      { const double r = px0 * y0_2_2;
+        W[7] = fma(1.0, r, W[7]);
+      }
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px1 * y0_2_3;
+        W[13] = fma(1.0, r, W[13]);
+      }
+      }
+      {
+      const double y1_2_3 = py1 * z2_3;
+      { const double r = px0 * y1_2_3;
+        W[18] = fma(1.0, r, W[18]);
+        W[17] = fma(1.0, r, W[17]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      {
+      const double y0_3_3 = py0 * z3_3;
+      { const double r = px0 * y0_3_3;
+        W[19] = fma(1.0, r, W[19]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[2], acc[0]);
+        acc[1] = fma(e, W[3], acc[1]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[8], acc[0]);
+        acc[1] = fma(e, W[9], acc[1]);
+      }
+      { const double e = rec[7];
+        acc[2] = fma(e, W[0], acc[2]);
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[8];
+        acc[2] = fma(e, W[4], acc[2]);
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[9];
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+      }
+      { const double e = rec[10];
+        acc[2] = fma(e, W[10], acc[2]);
+        acc[3] = fma(e, W[11], acc[3]);
+      }
+      { const double e = rec[11];
+        acc[4] = fma(e, W[0], acc[4]);
+        acc[5] = fma(e, W[1], acc[5]);
+      }
+      { const double e = rec[12];
+        acc[4] = fma(e, W[6], acc[4]);
+        acc[5] = fma(e, W[7], acc[5]);
+      }
+      { const double e = rec[13];
+        acc[4] = fma(e, W[2], acc[4]);
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[14];
+        acc[4] = fma(e, W[12], acc[4]);
+        acc[5] = fma(e, W[13], acc[5]);
+      }
+      { const double e = rec[15];
+        acc[6] = fma(e, W[0], acc[6]);
+        acc[7] = fma(e, W[1], acc[7]);
+      }
+      { const double e = rec[16];
+        acc[6] = fma(e, W[4], acc[6]);
+        acc[7] = fma(e, W[5], acc[7]);
+      }
+      { const double e = rec[17];
+        acc[6] = fma(e, W[2], acc[6]);
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[18];
+        acc[6] = fma(e, W[10], acc[6]);
+        acc[7] = fma(e, W[11], acc[7]);
+      }
+      { const double e = rec[19];
+        acc[8] = fma(e, W[0], acc[8]);
+        acc[9] = fma(e, W[1], acc[9]);
+      }
+      { const double e = rec[20];
+        acc[8] = fma(e, W[4], acc[8]);
+        acc[9] = fma(e, W[5], acc[9]);
+      }
+      { const double e = rec[21];
+        acc[8] = fma(e, W[14], acc[8]);
+        acc[9] = fma(e, W[15], acc[9]);
+      }
+      { const double e = rec[22];
+        acc[10] = fma(e, W[0], acc[10]);
+        acc[11] = fma(e, W[1], acc[11]);
+      }
+      { const double e = rec[23];
+        acc[10] = fma(e, W[6], acc[10]);
+        acc[11] = fma(e, W[7], acc[11]);
+      }
+      { const double e = rec[24];
+        acc[10] = fma(e, W[4], acc[10]);
+        acc[11] = fma(e, W[5], acc[11]);
+      }
+      { const double e = rec[25];
+        acc[10] = fma(e, W[16], acc[10]);
+        acc[11] = fma(e, W[17], acc[11]);
+      }
+      { const double e = rec[26];
+        acc[12] = fma(e, W[0], acc[12]);
+        acc[13] = fma(e, W[1], acc[13]);
+      }
+      { const double e = rec[27];
+        acc[12] = fma(e, W[6], acc[12]);
+        acc[13] = fma(e, W[7], acc[13]);
+      }
+      { const double e = rec[28];
+        acc[12] = fma(e, W[2], acc[12]);
+        acc[13] = fma(e, W[3], acc[13]);
+      }
+      { const double e = rec[29];
+        acc[12] = fma(e, W[12], acc[12]);
+        acc[13] = fma(e, W[13], acc[13]);
+      }
+      { const double e = rec[30];
+        acc[14] = fma(e, W[0], acc[14]);
+        acc[15] = fma(e, W[1], acc[15]);
+      }
+      { const double e = rec[31];
+        acc[14] = fma(e, W[6], acc[14]);
+        acc[15] = fma(e, W[7], acc[15]);
+      }
+      { const double e = rec[32];
+        acc[14] = fma(e, W[4], acc[14]);
+        acc[15] = fma(e, W[5], acc[15]);
+      }
+      { const double e = rec[33];
+        acc[14] = fma(e, W[16], acc[14]);
+        acc[15] = fma(e, W[17], acc[15]);
+      }
+      { const double e = rec[34];
+        acc[16] = fma(e, W[0], acc[16]);
+        acc[17] = fma(e, W[1], acc[17]);
+      }
+      { const double e = rec[35];
+        acc[16] = fma(e, W[6], acc[16]);
+        acc[17] = fma(e, W[7], acc[17]);
+      }
+      { const double e = rec[36];
+        acc[16] = fma(e, W[18], acc[16]);
+        acc[17] = fma(e, W[19], acc[17]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 1] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 1] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 1] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 1] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 1] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 1] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 1] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 1] = 1.0 * acc[1];
+    }
+    break;
+  }
+  case 1: {
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g; }
+      double F[4];
+      boys_array<3>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      double W[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      {
+      const double y0_0_1 = py0 * z0_1;
+      { const double r = px1 * y0_0_1;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      const double y0_0_2 = py0 * z0_2;
+      { const double r = px2 * y0_0_2 + 2.0 * px0 * y0_0_1;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      const double y0_0_3 = py0 * z0_3;
+      { const double r = px3 * y0_0_3 + 6.0 * px1 * y0_0_2;
+        W[4] = fma(1.0, r, W[4]);
+      }
+      }
+      {
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px1 * y1_0_2;
+        W[2] = fma(1.0, r, W[2]);
+      }
+      const double y1_0_3 = py1 * z0_3;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[5] = fma(1.0, r, W[5]);
+      }
+      }
+      {
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[7] = fma(1.0, r, W[7]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      {
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px1 * y0_1_2;
+        W[3] = fma(1.0, r, W[3]);
+      }
+      const double y0_1_3 = py0 * z1_3;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[6] = fma(1.0, r, W[6]);
+      }
+      }
+      {
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px1 * y1_1_3;
+        W[8] = fma(1.0, r, W[8]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      {
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px1 * y0_2_3;
+        W[9] = fma(1.0, r, W[9]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      {
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[1], acc[0]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[4], acc[0]);
+      }
+      { const double e = rec[7];
+        acc[1] = fma(e, W[0], acc[1]);
+      }
+      { const double e = rec[8];
+        acc[1] = fma(e, W[2], acc[1]);
+      }
+      { const double e = rec[9];
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[10];
+        acc[1] = fma(e, W[5], acc[1]);
+      }
+      { const double e = rec[11];
+        acc[2] = fma(e, W[0], acc[2]);
+      }
+      { const double e = rec[12];
+        acc[2] = fma(e, W[3], acc[2]);
+      }
+      { const double e = rec[13];
+        acc[2] = fma(e, W[1], acc[2]);
+      }
+      { const double e = rec[14];
+        acc[2] = fma(e, W[6], acc[2]);
+      }
+      { const double e = rec[15];
+        acc[3] = fma(e, W[0], acc[3]);
+      }
+      { const double e = rec[16];
+        acc[3] = fma(e, W[2], acc[3]);
+      }
+      { const double e = rec[17];
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[18];
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[19];
+        acc[4] = fma(e, W[0], acc[4]);
+      }
+      { const double e = rec[20];
+        acc[4] = fma(e, W[2], acc[4]);
+      }
+      { const double e = rec[21];
+        acc[4] = fma(e, W[7], acc[4]);
+      }
+      { const double e = rec[22];
+        acc[5] = fma(e, W[0], acc[5]);
+      }
+      { const double e = rec[23];
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[24];
+        acc[5] = fma(e, W[2], acc[5]);
+      }
+      { const double e = rec[25];
+        acc[5] = fma(e, W[8], acc[5]);
+      }
+      { const double e = rec[26];
+        acc[6] = fma(e, W[0], acc[6]);
+      }
+      { const double e = rec[27];
+        acc[6] = fma(e, W[3], acc[6]);
+      }
+      { const double e = rec[28];
+        acc[6] = fma(e, W[1], acc[6]);
+      }
+      { const double e = rec[29];
+        acc[6] = fma(e, W[6], acc[6]);
+      }
+      { const double e = rec[30];
+        acc[7] = fma(e, W[0], acc[7]);
+      }
+      { const double e = rec[31];
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[32];
+        acc[7] = fma(e, W[2], acc[7]);
+      }
+      { const double e = rec[33];
+        acc[7] = fma(e, W[8], acc[7]);
+      }
+      { const double e = rec[34];
+        acc[8] = fma(e, W[0], acc[8]);
+      }
+      { const double e = rec[35];
+        acc[8] = fma(e, W[3], acc[8]);
+      }
+      { const double e = rec[36];
+        acc[8] = fma(e, W[9], acc[8]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[4];
+    out[(row0 + 1) * ldo + col + 2] = 1.0 * acc[5];
+    out[(row0 + 2) * ldo + col + 2] = 1.0 * acc[3];
+    out[(row0 + 3) * ldo + col + 2] = 1.0 * acc[7];
+    out[(row0 + 4) * ldo + col + 2] = 1.0 * acc[8];
+    out[(row0 + 5) * ldo + col + 2] = 1.0 * acc[6];
+    out[(row0 + 6) * ldo + col + 2] = 1.0 * acc[1];
+    out[(row0 + 7) * ldo + col + 2] = 1.0 * acc[2];
+    out[(row0 + 8) * ldo + col + 2] = 1.0 * acc[0];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (11|2): L = 4, record 37 doubles, 3 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= i2g * i2g; }
+      double F[5];
+      boys_array<4>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      double W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      {
+      }
+      {
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px1 * y1_0_2;
+        W[0] = fma(1.0, r, W[0]);
+      }
+      const double y1_0_3 = py1 * z0_3;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[2] = fma(1.0, r, W[2]);
+      }
+      const double y1_0_4 = py1 * z0_4;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[8] = fma(1.0, r, W[8]);
+      }
+      }
+      {
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[4] = fma(1.0, r, W[4]);
+      }
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[10] = fma(1.0, r, W[10]);
+      }
+      }
+      {
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[14] = fma(1.0, r, W[14]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      {
+      }
+      {
+      const double y1_1_2 = py1 * z1_2;
+      { const double r = px0 * y1_1_2;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px1 * y1_1_3;
+        W[6] = fma(1.0, r, W[6]);
+        W[3] = fma(1.0, r, W[3]);
+      }
+      const double y1_1_4 = py1 * z1_4;
+      { const double r = px2 * y1_1_4 + 2.0 * px0 * y1_1_3;
+        W[12] = fma(1.0, r, W[12]);
+        W[9] = fma(1.0, r, W[9]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[5] = fma(1.0, r, W[5]);
+      }
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[16] = fma(1.0, r, W[16]);
+        W[11] = fma(1.0, r, W[11]);
+      }
+      }
+      {
+      const double y3_1_4 = py3 * z1_4 + 6.0 * py1 * z1_3;
+      { const double r = px0 * y3_1_4;
+        W[15] = fma(1.0, r, W[15]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      {
+      }
+      {
+      const double y1_2_3 = py1 * z2_3;
+      { const double r = px0 * y1_2_3;
+        W[7] = fma(1.0, r, W[7]);
+      }
+      const double y1_2_4 = py1 * z2_4;
+      { const double r = px1 * y1_2_4;
+        W[18] = fma(1.0, r, W[18]);
+        W[13] = fma(1.0, r, W[13]);
+      }
+      }
+      {
+      const double y2_2_4 = py2 * z2_4 + 2.0 * py0 * z2_3;
+      { const double r = px0 * y2_2_4;
+        W[17] = fma(1.0, r, W[17]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      {
+      }
+      {
+      const double y1_3_4 = py1 * z3_4;
+      { const double r = px0 * y1_3_4;
+        W[19] = fma(1.0, r, W[19]);
+      }
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      {
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[2], acc[0]);
+        acc[1] = fma(e, W[3], acc[1]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[8], acc[0]);
+        acc[1] = fma(e, W[9], acc[1]);
+      }
+      { const double e = rec[7];
+        acc[2] = fma(e, W[0], acc[2]);
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[8];
+        acc[2] = fma(e, W[4], acc[2]);
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[9];
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+      }
+      { const double e = rec[10];
+        acc[2] = fma(e, W[10], acc[2]);
+        acc[3] = fma(e, W[11], acc[3]);
+      }
+      { const double e = rec[11];
+        acc[4] = fma(e, W[0], acc[4]);
+        acc[5] = fma(e, W[1], acc[5]);
+      }
+      { const double e = rec[12];
+        acc[4] = fma(e, W[6], acc[4]);
+        acc[5] = fma(e, W[7], acc[5]);
+      }
+      { const double e = rec[13];
+        acc[4] = fma(e, W[2], acc[4]);
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[14];
+        acc[4] = fma(e, W[12], acc[4]);
+        acc[5] = fma(e, W[13], acc[5]);
+      }
+      { const double e = rec[15];
+        acc[6] = fma(e, W[0], acc[6]);
+        acc[7] = fma(e, W[1], acc[7]);
+      }
+      { const double e = rec[16];
+        acc[6] = fma(e, W[4], acc[6]);
+        acc[7] = fma(e, W[5], acc[7]);
+      }
+      { const double e = rec[17];
+        acc[6] = fma(e, W[2], acc[6]);
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[18];
+        acc[6] = fma(e, W[10], acc[6]);
+        acc[7] = fma(e, W[11], acc[7]);
+      }
+      { const double e = rec[19];
+        acc[8] = fma(e, W[0], acc[8]);
+        acc[9] = fma(e, W[1], acc[9]);
+      }
+      { const double e = rec[20];
+        acc[8] = fma(e, W[4], acc[8]);
+        acc[9] = fma(e, W[5], acc[9]);
+      }
+      { const double e = rec[21];
+        acc[8] = fma(e, W[14], acc[8]);
+        acc[9] = fma(e, W[15], acc[9]);
+      }
+      { const double e = rec[22];
+        acc[10] = fma(e, W[0], acc[10]);
+        acc[11] = fma(e, W[1], acc[11]);
+      }
+      { const double e = rec[23];
+        acc[10] = fma(e, W[6], acc[10]);
+        acc[11] = fma(e, W[7], acc[11]);
+      }
+      { const double e = rec[24];
+        acc[10] = fma(e, W[4], acc[10]);
+        acc[11] = fma(e, W[5], acc[11]);
+      }
+      { const double e = rec[25];
+        acc[10] = fma(e, W[16], acc[10]);
+        acc[11] = fma(e, W[17], acc[11]);
+      }
+      { const double e = rec[26];
+        acc[12] = fma(e, W[0], acc[12]);
+        acc[13] = fma(e, W[1], acc[13]);
+      }
+      { const double e = rec[27];
+        acc[12] = fma(e, W[6], acc[12]);
+        acc[13] = fma(e, W[7], acc[13]);
+      }
+      { const double e = rec[28];
+        acc[12] = fma(e, W[2], acc[12]);
+        acc[13] = fma(e, W[3], acc[13]);
+      }
+      { const double e = rec[29];
+        acc[12] = fma(e, W[12], acc[12]);
+        acc[13] = fma(e, W[13], acc[13]);
+      }
+      { const double e = rec[30];
+        acc[14] = fma(e, W[0], acc[14]);
+        acc[15] = fma(e, W[1], acc[15]);
+      }
+      { const double e = rec[31];
+        acc[14] = fma(e, W[6], acc[14]);
+        acc[15] = fma(e, W[7], acc[15]);
+      }
+      { const double e = rec[32];
+        acc[14] = fma(e, W[4], acc[14]);
+        acc[15] = fma(e, W[5], acc[15]);
+      }
+      { const double e = rec[33];
+        acc[14] = fma(e, W[16], acc[14]);
+        acc[15] = fma(e, W[17], acc[15]);
+      }
+      { const double e = rec[34];
+        acc[16] = fma(e, W[0], acc[16]);
+        acc[17] = fma(e, W[1], acc[17]);
+      }
+      { const double e = rec[35];
+        acc[16] = fma(e, W[6], acc[16]);
+        acc[17] = fma(e, W[7], acc[17]);
+      }
+      { const double e = rec[36];
+        acc[16] = fma(e, W[18], acc[16]);
+        acc[17] = fma(e, W[19], acc[17]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 1] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 1] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 1] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 1] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 1] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 1] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 1] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 1] = 1.0 * acc[1];
+    }
+    break;
+  }
+  case 1: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= i2g * i2g; }
+      double F[5];
+      boys_array<4>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      double W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      {
+      const double y0_0_2 = py0 * z0_2;
+      const double y0_0_1 = py0 * z0_1;
+      { const double r = px2 * y0_0_2 + 2.0 * px0 * y0_0_1;
+        W[0] = fma(-0.2886751345948129, r, W[0]);
+      }
+      const double y0_0_3 = py0 * z0_3;
+      { const double r = px3 * y0_0_3 + 6.0 * px1 * y0_0_2;
+        W[2] = fma(-0.2886751345948129, r, W[2]);
+      }
+      const double y0_0_4 = py0 * z0_4;
+      { const double r = px4 * y0_0_4 + 12.0 * px2 * y0_0_3 + 12.0 * px0 * y0_0_2;
+        W[8] = fma(-0.2886751345948129, r, W[8]);
+      }
+      }
+      {
+      const double y1_0_3 = py1 * z0_3;
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[4] = fma(-0.2886751345948129, r, W[4]);
+      }
+      const double y1_0_4 = py1 * z0_4;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[10] = fma(-0.2886751345948129, r, W[10]);
+      }
+      }
+      {
+      const double y2_0_2 = py2 * z0_2 + 2.0 * py0 * z0_1;
+      { const double r = px0 * y2_0_2;
+        W[0] = fma(-0.2886751345948129, r, W[0]);
+      }
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[2] = fma(-0.2886751345948129, r, W[2]);
+      }
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[14] = fma(-0.2886751345948129, r, W[14]);
+        W[8] = fma(-0.2886751345948129, r, W[8]);
+      }
+      }
+      {
+      const double y3_0_3 = py3 * z0_3 + 6.0 * py1 * z0_2;
+      { const double r = px0 * y3_0_3;
+        W[4] = fma(-0.2886751345948129, r, W[4]);
+      }
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[10] = fma(-0.2886751345948129, r, W[10]);
+      }
+      }
+      {
+      const double y4_0_4 = py4 * z0_4 + 12.0 * py2 * z0_3 + 12.0 * py0 * z0_2;
+      { const double r = px0 * y4_0_4;
+        W[14] = fma(-0.2886751345948129, r, W[14]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      {
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px1 * y0_1_2;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      const double y0_1_3 = py0 * z1_3;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[6] = fma(-0.2886751345948129, r, W[6]);
+        W[3] = fma(1.0, r, W[3]);
+      }
+      const double y0_1_4 = py0 * z1_4;
+      { const double r = px3 * y0_1_4 + 6.0 * px1 * y0_1_3;
+        W[12] = fma(-0.2886751345948129, r, W[12]);
+        W[9] = fma(1.0, r, W[9]);
+      }
+      }
+      {
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px1 * y1_1_3;
+        W[5] = fma(1.0, r, W[5]);
+      }
+      const double y1_1_4 = py1 * z1_4;
+      { const double r = px2 * y1_1_4 + 2.0 * px0 * y1_1_3;
+        W[16] = fma(-0.2886751345948129, r, W[16]);
+        W[11] = fma(1.0, r, W[11]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[6] = fma(-0.2886751345948129, r, W[6]);
+      }
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[12] = fma(-0.2886751345948129, r, W[12]);
+        W[15] = fma(1.0, r, W[15]);
+      }
+      }
+      {
+      const double y3_1_4 = py3 * z1_4 + 6.0 * py1 * z1_3;
+      { const double r = px0 * y3_1_4;
+        W[16] = fma(-0.2886751345948129, r, W[16]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      {
+      const double y0_2_2 = py0 * z2_2;
+      { const double r = px0 * y0_2_2;
+        W[0] = fma(0.5773502691896258, r, W[0]);
+      }
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px1 * y0_2_3;
+        W[2] = fma(0.5773502691896258, r, W[2]);
+        W[7] = fma(1.0, r, W[7]);
+      }
+      const double y0_2_4 = py0 * z2_4;
+      { const double r = px2 * y0_2_4 + 2.0 * px0 * y0_2_3;
+        W[18] = fma(-0.2886751345948129, r, W[18]);
+        W[8] = fma(0.5773502691896258, r, W[8]);
+        W[13] = fma(1.0, r, W[13]);
+      }
+      }
+      {
+      const double y1_2_3 = py1 * z2_3;
+      { const double r = px0 * y1_2_3;
+        W[4] = fma(0.5773502691896258, r, W[4]);
+      }
+      const double y1_2_4 = py1 * z2_4;
+      { const double r = px1 * y1_2_4;
+        W[10] = fma(0.5773502691896258, r, W[10]);
+        W[17] = fma(1.0, r, W[17]);
+      }
+      }
+      {
+      const double y2_2_4 = py2 * z2_4 + 2.0 * py0 * z2_3;
+      { const double r = px0 * y2_2_4;
+        W[18] = fma(-0.2886751345948129, r, W[18]);
+        W[14] = fma(0.5773502691896258, r, W[14]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      {
+      const double y0_3_3 = py0 * z3_3;
+      { const double r = px0 * y0_3_3;
+        W[6] = fma(0.5773502691896258, r, W[6]);
+      }
+      const double y0_3_4 = py0 * z3_4;
+      { const double r = px1 * y0_3_4;
+        W[12] = fma(0.5773502691896258, r, W[12]);
+        W[19] = fma(1.0, r, W[19]);
+      }
+      }
+      {
+      const double y1_3_4 = py1 * z3_4;
+      { const double r = px0 * y1_3_4;
+        W[16] = fma(0.5773502691896258, r, W[16]);
+      }
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      {
+      const double y0_4_4 = py0 * z4_4;
+      { const double r = px0 * y0_4_4;
+        W[18] = fma(0.5773502691896258, r, W[18]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[2], acc[0]);
+        acc[1] = fma(e, W[3], acc[1]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[8], acc[0]);
+        acc[1] = fma(e, W[9], acc[1]);
+      }
+      { const double e = rec[7];
+        acc[2] = fma(e, W[0], acc[2]);
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[8];
+        acc[2] = fma(e, W[4], acc[2]);
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[9];
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+      }
+      { const double e = rec[10];
+        acc[2] = fma(e, W[10], acc[2]);
+        acc[3] = fma(e, W[11], acc[3]);
+      }
+      { const double e = rec[11];
+        acc[4] = fma(e, W[0], acc[4]);
+        acc[5] = fma(e, W[1], acc[5]);
+      }
+      { const double e = rec[12];
+        acc[4] = fma(e, W[6], acc[4]);
+        acc[5] = fma(e, W[7], acc[5]);
+      }
+      { const double e = rec[13];
+        acc[4] = fma(e, W[2], acc[4]);
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[14];
+        acc[4] = fma(e, W[12], acc[4]);
+        acc[5] = fma(e, W[13], acc[5]);
+      }
+      { const double e = rec[15];
+        acc[6] = fma(e, W[0], acc[6]);
+        acc[7] = fma(e, W[1], acc[7]);
+      }
+      { const double e = rec[16];
+        acc[6] = fma(e, W[4], acc[6]);
+        acc[7] = fma(e, W[5], acc[7]);
+      }
+      { const double e = rec[17];
+        acc[6] = fma(e, W[2], acc[6]);
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[18];
+        acc[6] = fma(e, W[10], acc[6]);
+        acc[7] = fma(e, W[11], acc[7]);
+      }
+      { const double e = rec[19];
+        acc[8] = fma(e, W[0], acc[8]);
+        acc[9] = fma(e, W[1], acc[9]);
+      }
+      { const double e = rec[20];
+        acc[8] = fma(e, W[4], acc[8]);
+        acc[9] = fma(e, W[5], acc[9]);
+      }
+      { const double e = rec[21];
+        acc[8] = fma(e, W[14], acc[8]);
+        acc[9] = fma(e, W[15], acc[9]);
+      }
+      { const double e = rec[22];
+        acc[10] = fma(e, W[0], acc[10]);
+        acc[11] = fma(e, W[1], acc[11]);
+      }
+      { const double e = rec[23];
+        acc[10] = fma(e, W[6], acc[10]);
+        acc[11] = fma(e, W[7], acc[11]);
+      }
+      { const double e = rec[24];
+        acc[10] = fma(e, W[4], acc[10]);
+        acc[11] = fma(e, W[5], acc[11]);
+      }
+      { const double e = rec[25];
+        acc[10] = fma(e, W[16], acc[10]);
+        acc[11] = fma(e, W[17], acc[11]);
+      }
+      { const double e = rec[26];
+        acc[12] = fma(e, W[0], acc[12]);
+        acc[13] = fma(e, W[1], acc[13]);
+      }
+      { const double e = rec[27];
+        acc[12] = fma(e, W[6], acc[12]);
+        acc[13] = fma(e, W[7], acc[13]);
+      }
+      { const double e = rec[28];
+        acc[12] = fma(e, W[2], acc[12]);
+        acc[13] = fma(e, W[3], acc[13]);
+      }
+      { const double e = rec[29];
+        acc[12] = fma(e, W[12], acc[12]);
+        acc[13] = fma(e, W[13], acc[13]);
+      }
+      { const double e = rec[30];
+        acc[14] = fma(e, W[0], acc[14]);
+        acc[15] = fma(e, W[1], acc[15]);
+      }
+      { const double e = rec[31];
+        acc[14] = fma(e, W[6], acc[14]);
+        acc[15] = fma(e, W[7], acc[15]);
+      }
+      { const double e = rec[32];
+        acc[14] = fma(e, W[4], acc[14]);
+        acc[15] = fma(e, W[5], acc[15]);
+      }
+      { const double e = rec[33];
+        acc[14] = fma(e, W[16], acc[14]);
+        acc[15] = fma(e, W[17], acc[15]);
+      }
+      { const double e = rec[34];
+        acc[16] = fma(e, W[0], acc[16]);
+        acc[17] = fma(e, W[1], acc[17]);
+      }
+      { const double e = rec[35];
+        acc[16] = fma(e, W[6], acc[16]);
+        acc[17] = fma(e, W[7], acc[17]);
+      }
+      { const double e = rec[36];
+        acc[16] = fma(e, W[18], acc[16]);
+        acc[17] = fma(e, W[19], acc[17]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 3] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 2] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 3] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 2] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 3] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 2] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 3] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 2] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 3] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 2] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 3] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 2] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 3] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 2] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 3] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 2] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 3] = 1.0 * acc[1];
+    }
+    break;
+  }
+  case 2: {
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= i2g * i2g; }
+      double F[5];
+      boys_array<4>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      double W[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      {
+      const double y0_0_2 = py0 * z0_2;
+      const double y0_0_1 = py0 * z0_1;
+      { const double r = px2 * y0_0_2 + 2.0 * px0 * y0_0_1;
+        W[0] = fma(0.5, r, W[0]);
+      }
+      const double y0_0_3 = py0 * z0_3;
+      { const double r = px3 * y0_0_3 + 6.0 * px1 * y0_0_2;
+        W[1] = fma(0.5, r, W[1]);
+      }
+      const double y0_0_4 = py0 * z0_4;
+      { const double r = px4 * y0_0_4 + 12.0 * px2 * y0_0_3 + 12.0 * px0 * y0_0_2;
+        W[4] = fma(0.5, r, W[4]);
+      }
+      }
+      {
+      const double y1_0_3 = py1 * z0_3;
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[2] = fma(0.5, r, W[2]);
+      }
+      const double y1_0_4 = py1 * z0_4;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[5] = fma(0.5, r, W[5]);
+      }
+      }
+      {
+      const double y2_0_2 = py2 * z0_2 + 2.0 * py0 * z0_1;
+      { const double r = px0 * y2_0_2;
+        W[0] = fma(-0.5, r, W[0]);
+      }
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[1] = fma(-0.5, r, W[1]);
+      }
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[7] = fma(0.5, r, W[7]);
+        W[4] = fma(-0.5, r, W[4]);
+      }
+      }
+      {
+      const double y3_0_3 = py3 * z0_3 + 6.0 * py1 * z0_2;
+      { const double r = px0 * y3_0_3;
+        W[2] = fma(-0.5, r, W[2]);
+      }
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[5] = fma(-0.5, r, W[5]);
+      }
+      }
+      {
+      const double y4_0_4 = py4 * z0_4 + 12.0 * py2 * z0_3 + 12.0 * py0 * z0_2;
+      { const double r = px0 * y4_0_4;
+        W[7] = fma(-0.5, r, W[7]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      {
+      const double y0_1_3 = py0 * z1_3;
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[3] = fma(0.5, r, W[3]);
+      }
+      const double y0_1_4 = py0 * z1_4;
+      { const double r = px3 * y0_1_4 + 6.0 * px1 * y0_1_3;
+        W[6] = fma(0.5, r, W[6]);
+      }
+      }
+      {
+      const double y1_1_4 = py1 * z1_4;
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px2 * y1_1_4 + 2.0 * px0 * y1_1_3;
+        W[8] = fma(0.5, r, W[8]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[3] = fma(-0.5, r, W[3]);
+      }
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[6] = fma(-0.5, r, W[6]);
+      }
+      }
+      {
+      const double y3_1_4 = py3 * z1_4 + 6.0 * py1 * z1_3;
+      { const double r = px0 * y3_1_4;
+        W[8] = fma(-0.5, r, W[8]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      {
+      const double y0_2_4 = py0 * z2_4;
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px2 * y0_2_4 + 2.0 * px0 * y0_2_3;
+        W[9] = fma(0.5, r, W[9]);
+      }
+      }
+      {
+      }
+      {
+      const double y2_2_4 = py2 * z2_4 + 2.0 * py0 * z2_3;
+      { const double r = px0 * y2_2_4;
+        W[9] = fma(-0.5, r, W[9]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      {
+      }
+      {
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      {
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[1], acc[0]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[4], acc[0]);
+      }
+      { const double e = rec[7];
+        acc[1] = fma(e, W[0], acc[1]);
+      }
+      { const double e = rec[8];
+        acc[1] = fma(e, W[2], acc[1]);
+      }
+      { const double e = rec[9];
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[10];
+        acc[1] = fma(e, W[5], acc[1]);
+      }
+      { const double e = rec[11];
+        acc[2] = fma(e, W[0], acc[2]);
+      }
+      { const double e = rec[12];
+        acc[2] = fma(e, W[3], acc[2]);
+      }
+      { const double e = rec[13];
+        acc[2] = fma(e, W[1], acc[2]);
+      }
+      { const double e = rec[14];
+        acc[2] = fma(e, W[6], acc[2]);
+      }
+      { const double e = rec[15];
+        acc[3] = fma(e, W[0], acc[3]);
+      }
+      { const double e = rec[16];
+        acc[3] = fma(e, W[2], acc[3]);
+      }
+      { const double e = rec[17];
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[18];
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[19];
+        acc[4] = fma(e, W[0], acc[4]);
+      }
+      { const double e = rec[20];
+        acc[4] = fma(e, W[2], acc[4]);
+      }
+      { const double e = rec[21];
+        acc[4] = fma(e, W[7], acc[4]);
+      }
+      { const double e = rec[22];
+        acc[5] = fma(e, W[0], acc[5]);
+      }
+      { const double e = rec[23];
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[24];
+        acc[5] = fma(e, W[2], acc[5]);
+      }
+      { const double e = rec[25];
+        acc[5] = fma(e, W[8], acc[5]);
+      }
+      { const double e = rec[26];
+        acc[6] = fma(e, W[0], acc[6]);
+      }
+      { const double e = rec[27];
+        acc[6] = fma(e, W[3], acc[6]);
+      }
+      { const double e = rec[28];
+        acc[6] = fma(e, W[1], acc[6]);
+      }
+      { const double e = rec[29];
+        acc[6] = fma(e, W[6], acc[6]);
+      }
+      { const double e = rec[30];
+        acc[7] = fma(e, W[0], acc[7]);
+      }
+      { const double e = rec[31];
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[32];
+        acc[7] = fma(e, W[2], acc[7]);
+      }
+      { const double e = rec[33];
+        acc[7] = fma(e, W[8], acc[7]);
+      }
+      { const double e = rec[34];
+        acc[8] = fma(e, W[0], acc[8]);
+      }
+      { const double e = rec[35];
+        acc[8] = fma(e, W[3], acc[8]);
+      }
+      { const double e = rec[36];
+        acc[8] = fma(e, W[9], acc[8]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 4] = 1.0 * acc[4];
+    out[(row0 + 1) * ldo + col + 4] = 1.0 * acc[5];
+    out[(row0 + 2) * ldo + col + 4] = 1.0 * acc[3];
+    out[(row0 + 3) * ldo + col + 4] = 1.0 * acc[7];
+    out[(row0 + 4) * ldo + col + 4] = 1.0 * acc[8];
+    out[(row0 + 5) * ldo + col + 4] = 1.0 * acc[6];
+    out[(row0 + 6) * ldo + col + 4] = 1.0 * acc[1];
+    out[(row0 + 7) * ldo + col + 4] = 1.0 * acc[2];
+    out[(row0 + 8) * ldo + col + 4] = 1.0 * acc[0];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (11|3): L = 5, record 37 doubles, 4 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
+    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
+    double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
+  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
+  const bool active = c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[6];
+      boys_array<5>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      sc *= ma; const double g5 = sc * F[5];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double px5 = px4 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double py5 = py4 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      const double pz5 = pz4 * pz1;
+      double W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      const double z0_5 = pz0 * g5;
+      {
+      }
+      {
+      const double y1_0_3 = py1 * z0_3;
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[0] = fma(0.6123724356957945, r, W[0]);
+      }
+      const double y1_0_4 = py1 * z0_4;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[2] = fma(0.6123724356957945, r, W[2]);
+      }
+      const double y1_0_5 = py1 * z0_5;
+      { const double r = px4 * y1_0_5 + 12.0 * px2 * y1_0_4 + 12.0 * px0 * y1_0_3;
+        W[8] = fma(0.6123724356957945, r, W[8]);
+      }
+      }
+      {
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[4] = fma(0.6123724356957945, r, W[4]);
+      }
+      const double y2_0_5 = py2 * z0_5 + 2.0 * py0 * z0_4;
+      { const double r = px3 * y2_0_5 + 6.0 * px1 * y2_0_4;
+        W[10] = fma(0.6123724356957945, r, W[10]);
+      }
+      }
+      {
+      const double y3_0_3 = py3 * z0_3 + 6.0 * py1 * z0_2;
+      { const double r = px0 * y3_0_3;
+        W[0] = fma(-0.20412414523193148, r, W[0]);
+      }
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[2] = fma(-0.20412414523193148, r, W[2]);
+      }
+      const double y3_0_5 = py3 * z0_5 + 6.0 * py1 * z0_4;
+      { const double r = px2 * y3_0_5 + 2.0 * px0 * y3_0_4;
+        W[14] = fma(0.6123724356957945, r, W[14]);
+        W[8] = fma(-0.20412414523193148, r, W[8]);
+      }
+      }
+      {
+      const double y4_0_4 = py4 * z0_4 + 12.0 * py2 * z0_3 + 12.0 * py0 * z0_2;
+      { const double r = px0 * y4_0_4;
+        W[4] = fma(-0.20412414523193148, r, W[4]);
+      }
+      const double y4_0_5 = py4 * z0_5 + 12.0 * py2 * z0_4 + 12.0 * py0 * z0_3;
+      { const double r = px1 * y4_0_5;
+        W[10] = fma(-0.20412414523193148, r, W[10]);
+      }
+      }
+      {
+      const double y5_0_5 = py5 * z0_5 + 20.0 * py3 * z0_4 + 60.0 * py1 * z0_3;
+      { const double r = px0 * y5_0_5;
+        W[14] = fma(-0.20412414523193148, r, W[14]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      const double z1_5 = pz1 * g5;
+      {
+      }
+      {
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px1 * y1_1_3;
+        W[1] = fma(1.0, r, W[1]);
+      }
+      const double y1_1_4 = py1 * z1_4;
+      { const double r = px2 * y1_1_4 + 2.0 * px0 * y1_1_3;
+        W[6] = fma(0.6123724356957945, r, W[6]);
+        W[3] = fma(1.0, r, W[3]);
+      }
+      const double y1_1_5 = py1 * z1_5;
+      { const double r = px3 * y1_1_5 + 6.0 * px1 * y1_1_4;
+        W[12] = fma(0.6123724356957945, r, W[12]);
+        W[9] = fma(1.0, r, W[9]);
+      }
+      }
+      {
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[5] = fma(1.0, r, W[5]);
+      }
+      const double y2_1_5 = py2 * z1_5 + 2.0 * py0 * z1_4;
+      { const double r = px2 * y2_1_5 + 2.0 * px0 * y2_1_4;
+        W[16] = fma(0.6123724356957945, r, W[16]);
+        W[11] = fma(1.0, r, W[11]);
+      }
+      }
+      {
+      const double y3_1_4 = py3 * z1_4 + 6.0 * py1 * z1_3;
+      { const double r = px0 * y3_1_4;
+        W[6] = fma(-0.20412414523193148, r, W[6]);
+      }
+      const double y3_1_5 = py3 * z1_5 + 6.0 * py1 * z1_4;
+      { const double r = px1 * y3_1_5;
+        W[12] = fma(-0.20412414523193148, r, W[12]);
+        W[15] = fma(1.0, r, W[15]);
+      }
+      }
+      {
+      const double y4_1_5 = py4 * z1_5 + 12.0 * py2 * z1_4 + 12.0 * py0 * z1_3;
+      { const double r = px0 * y4_1_5;
+        W[16] = fma(-0.20412414523193148, r, W[16]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      const double z2_5 = pz2 * g5 + 2.0 * pz0 * g4;
+      {
+      }
+      {
+      const double y1_2_4 = py1 * z2_4;
+      { const double r = px1 * y1_2_4;
+        W[7] = fma(1.0, r, W[7]);
+      }
+      const double y1_2_5 = py1 * z2_5;
+      { const double r = px2 * y1_2_5 + 2.0 * px0 * y1_2_4;
+        W[18] = fma(0.6123724356957945, r, W[18]);
+        W[13] = fma(1.0, r, W[13]);
+      }
+      }
+      {
+      const double y2_2_5 = py2 * z2_5 + 2.0 * py0 * z2_4;
+      { const double r = px1 * y2_2_5;
+        W[17] = fma(1.0, r, W[17]);
+      }
+      }
+      {
+      const double y3_2_5 = py3 * z2_5 + 6.0 * py1 * z2_4;
+      { const double r = px0 * y3_2_5;
+        W[18] = fma(-0.20412414523193148, r, W[18]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      const double z3_5 = pz3 * g5 + 6.0 * pz1 * g4;
+      {
+      }
+      {
+      const double y1_3_5 = py1 * z3_5;
+      { const double r = px1 * y1_3_5;
+        W[19] = fma(1.0, r, W[19]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      const double z4_5 = pz4 * g5 + 12.0 * pz2 * g4 + 12.0 * pz0 * g3;
+      {
+      }
+      {
+      }
+      }
+      {
+      const double z5_5 = pz5 * g5 + 20.0 * pz3 * g4 + 60.0 * pz1 * g3;
+      {
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[2], acc[0]);
+        acc[1] = fma(e, W[3], acc[1]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[8], acc[0]);
+        acc[1] = fma(e, W[9], acc[1]);
+      }
+      { const double e = rec[7];
+        acc[2] = fma(e, W[0], acc[2]);
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[8];
+        acc[2] = fma(e, W[4], acc[2]);
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[9];
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+      }
+      { const double e = rec[10];
+        acc[2] = fma(e, W[10], acc[2]);
+        acc[3] = fma(e, W[11], acc[3]);
+      }
+      { const double e = rec[11];
+        acc[4] = fma(e, W[0], acc[4]);
+        acc[5] = fma(e, W[1], acc[5]);
+      }
+      { const double e = rec[12];
+        acc[4] = fma(e, W[6], acc[4]);
+        acc[5] = fma(e, W[7], acc[5]);
+      }
+      { const double e = rec[13];
+        acc[4] = fma(e, W[2], acc[4]);
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[14];
+        acc[4] = fma(e, W[12], acc[4]);
+        acc[5] = fma(e, W[13], acc[5]);
+      }
+      { const double e = rec[15];
+        acc[6] = fma(e, W[0], acc[6]);
+        acc[7] = fma(e, W[1], acc[7]);
+      }
+      { const double e = rec[16];
+        acc[6] = fma(e, W[4], acc[6]);
+        acc[7] = fma(e, W[5], acc[7]);
+      }
+      { const double e = rec[17];
+        acc[6] = fma(e, W[2], acc[6]);
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[18];
+        acc[6] = fma(e, W[10], acc[6]);
+        acc[7] = fma(e, W[11], acc[7]);
+      }
+      { const double e = rec[19];
+        acc[8] = fma(e, W[0], acc[8]);
+        acc[9] = fma(e, W[1], acc[9]);
+      }
+      { const double e = rec[20];
+        acc[8] = fma(e, W[4], acc[8]);
+        acc[9] = fma(e, W[5], acc[9]);
+      }
+      { const double e = rec[21];
+        acc[8] = fma(e, W[14], acc[8]);
+        acc[9] = fma(e, W[15], acc[9]);
+      }
+      { const double e = rec[22];
+        acc[10] = fma(e, W[0], acc[10]);
+        acc[11] = fma(e, W[1], acc[11]);
+      }
+      { const double e = rec[23];
+        acc[10] = fma(e, W[6], acc[10]);
+        acc[11] = fma(e, W[7], acc[11]);
+      }
+      { const double e = rec[24];
+        acc[10] = fma(e, W[4], acc[10]);
+        acc[11] = fma(e, W[5], acc[11]);
+      }
+      { const double e = rec[25];
+        acc[10] = fma(e, W[16], acc[10]);
+        acc[11] = fma(e, W[17], acc[11]);
+      }
+      { const double e = rec[26];
+        acc[12] = fma(e, W[0], acc[12]);
+        acc[13] = fma(e, W[1], acc[13]);
+      }
+      { const double e = rec[27];
+        acc[12] = fma(e, W[6], acc[12]);
+        acc[13] = fma(e, W[7], acc[13]);
+      }
+      { const double e = rec[28];
+        acc[12] = fma(e, W[2], acc[12]);
+        acc[13] = fma(e, W[3], acc[13]);
+      }
+      { const double e = rec[29];
+        acc[12] = fma(e, W[12], acc[12]);
+        acc[13] = fma(e, W[13], acc[13]);
+      }
+      { const double e = rec[30];
+        acc[14] = fma(e, W[0], acc[14]);
+        acc[15] = fma(e, W[1], acc[15]);
+      }
+      { const double e = rec[31];
+        acc[14] = fma(e, W[6], acc[14]);
+        acc[15] = fma(e, W[7], acc[15]);
+      }
+      { const double e = rec[32];
+        acc[14] = fma(e, W[4], acc[14]);
+        acc[15] = fma(e, W[5], acc[15]);
+      }
+      { const double e = rec[33];
+        acc[14] = fma(e, W[16], acc[14]);
+        acc[15] = fma(e, W[17], acc[15]);
+      }
+      { const double e = rec[34];
+        acc[16] = fma(e, W[0], acc[16]);
+        acc[17] = fma(e, W[1], acc[17]);
+      }
+      { const double e = rec[35];
+        acc[16] = fma(e, W[6], acc[16]);
+        acc[17] = fma(e, W[7], acc[17]);
+      }
+      { const double e = rec[36];
+        acc[16] = fma(e, W[18], acc[16]);
+        acc[17] = fma(e, W[19], acc[17]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 1] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 1] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 1] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 1] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 1] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 1] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 1] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 1] = 1.0 * acc[1];
+    }
+    break;
+  }
+  case 1: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[6];
+      boys_array<5>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      sc *= ma; const double g5 = sc * F[5];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double px5 = px4 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double py5 = py4 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      const double pz5 = pz4 * pz1;
+      double W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      const double z0_5 = pz0 * g5;
+      {
+      }
+      {
+      const double y1_0_3 = py1 * z0_3;
+      const double y1_0_2 = py1 * z0_2;
+      { const double r = px2 * y1_0_3 + 2.0 * px0 * y1_0_2;
+        W[0] = fma(-0.15811388300841894, r, W[0]);
+      }
+      const double y1_0_4 = py1 * z0_4;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[2] = fma(-0.15811388300841894, r, W[2]);
+      }
+      const double y1_0_5 = py1 * z0_5;
+      { const double r = px4 * y1_0_5 + 12.0 * px2 * y1_0_4 + 12.0 * px0 * y1_0_3;
+        W[8] = fma(-0.15811388300841894, r, W[8]);
+      }
+      }
+      {
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[4] = fma(-0.15811388300841894, r, W[4]);
+      }
+      const double y2_0_5 = py2 * z0_5 + 2.0 * py0 * z0_4;
+      { const double r = px3 * y2_0_5 + 6.0 * px1 * y2_0_4;
+        W[10] = fma(-0.15811388300841894, r, W[10]);
+      }
+      }
+      {
+      const double y3_0_3 = py3 * z0_3 + 6.0 * py1 * z0_2;
+      { const double r = px0 * y3_0_3;
+        W[0] = fma(-0.15811388300841894, r, W[0]);
+      }
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[2] = fma(-0.15811388300841894, r, W[2]);
+      }
+      const double y3_0_5 = py3 * z0_5 + 6.0 * py1 * z0_4;
+      { const double r = px2 * y3_0_5 + 2.0 * px0 * y3_0_4;
+        W[14] = fma(-0.15811388300841894, r, W[14]);
+        W[8] = fma(-0.15811388300841894, r, W[8]);
+      }
+      }
+      {
+      const double y4_0_4 = py4 * z0_4 + 12.0 * py2 * z0_3 + 12.0 * py0 * z0_2;
+      { const double r = px0 * y4_0_4;
+        W[4] = fma(-0.15811388300841894, r, W[4]);
+      }
+      const double y4_0_5 = py4 * z0_5 + 12.0 * py2 * z0_4 + 12.0 * py0 * z0_3;
+      { const double r = px1 * y4_0_5;
+        W[10] = fma(-0.15811388300841894, r, W[10]);
+      }
+      }
+      {
+      const double y5_0_5 = py5 * z0_5 + 20.0 * py3 * z0_4 + 60.0 * py1 * z0_3;
+      { const double r = px0 * y5_0_5;
+        W[14] = fma(-0.15811388300841894, r, W[14]);
+      }
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      const double z1_5 = pz1 * g5;
+      {
+      const double y0_1_3 = py0 * z1_3;
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[1] = fma(-0.38729833462074165, r, W[1]);
+      }
+      const double y0_1_4 = py0 * z1_4;
+      { const double r = px3 * y0_1_4 + 6.0 * px1 * y0_1_3;
+        W[3] = fma(-0.38729833462074165, r, W[3]);
+      }
+      const double y0_1_5 = py0 * z1_5;
+      { const double r = px4 * y0_1_5 + 12.0 * px2 * y0_1_4 + 12.0 * px0 * y0_1_3;
+        W[9] = fma(-0.38729833462074165, r, W[9]);
+      }
+      }
+      {
+      const double y1_1_4 = py1 * z1_4;
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px2 * y1_1_4 + 2.0 * px0 * y1_1_3;
+        W[6] = fma(-0.15811388300841894, r, W[6]);
+        W[5] = fma(-0.38729833462074165, r, W[5]);
+      }
+      const double y1_1_5 = py1 * z1_5;
+      { const double r = px3 * y1_1_5 + 6.0 * px1 * y1_1_4;
+        W[12] = fma(-0.15811388300841894, r, W[12]);
+        W[11] = fma(-0.38729833462074165, r, W[11]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[1] = fma(-0.38729833462074165, r, W[1]);
+      }
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[3] = fma(-0.38729833462074165, r, W[3]);
+      }
+      const double y2_1_5 = py2 * z1_5 + 2.0 * py0 * z1_4;
+      { const double r = px2 * y2_1_5 + 2.0 * px0 * y2_1_4;
+        W[16] = fma(-0.15811388300841894, r, W[16]);
+        W[15] = fma(-0.38729833462074165, r, W[15]);
+        W[9] = fma(-0.38729833462074165, r, W[9]);
+      }
+      }
+      {
+      const double y3_1_4 = py3 * z1_4 + 6.0 * py1 * z1_3;
+      { const double r = px0 * y3_1_4;
+        W[6] = fma(-0.15811388300841894, r, W[6]);
+        W[5] = fma(-0.38729833462074165, r, W[5]);
+      }
+      const double y3_1_5 = py3 * z1_5 + 6.0 * py1 * z1_4;
+      { const double r = px1 * y3_1_5;
+        W[12] = fma(-0.15811388300841894, r, W[12]);
+        W[11] = fma(-0.38729833462074165, r, W[11]);
+      }
+      }
+      {
+      const double y4_1_5 = py4 * z1_5 + 12.0 * py2 * z1_4 + 12.0 * py0 * z1_3;
+      { const double r = px0 * y4_1_5;
+        W[16] = fma(-0.15811388300841894, r, W[16]);
+        W[15] = fma(-0.38729833462074165, r, W[15]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      const double z2_5 = pz2 * g5 + 2.0 * pz0 * g4;
+      {
+      const double y0_2_4 = py0 * z2_4;
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px2 * y0_2_4 + 2.0 * px0 * y0_2_3;
+        W[7] = fma(-0.38729833462074165, r, W[7]);
+      }
+      const double y0_2_5 = py0 * z2_5;
+      { const double r = px3 * y0_2_5 + 6.0 * px1 * y0_2_4;
+        W[13] = fma(-0.38729833462074165, r, W[13]);
+      }
+      }
+      {
+      const double y1_2_3 = py1 * z2_3;
+      { const double r = px0 * y1_2_3;
+        W[0] = fma(0.6324555320336758, r, W[0]);
+      }
+      const double y1_2_4 = py1 * z2_4;
+      { const double r = px1 * y1_2_4;
+        W[2] = fma(0.6324555320336758, r, W[2]);
+      }
+      const double y1_2_5 = py1 * z2_5;
+      { const double r = px2 * y1_2_5 + 2.0 * px0 * y1_2_4;
+        W[18] = fma(-0.15811388300841894, r, W[18]);
+        W[8] = fma(0.6324555320336758, r, W[8]);
+        W[17] = fma(-0.38729833462074165, r, W[17]);
+      }
+      }
+      {
+      const double y2_2_4 = py2 * z2_4 + 2.0 * py0 * z2_3;
+      { const double r = px0 * y2_2_4;
+        W[4] = fma(0.6324555320336758, r, W[4]);
+        W[7] = fma(-0.38729833462074165, r, W[7]);
+      }
+      const double y2_2_5 = py2 * z2_5 + 2.0 * py0 * z2_4;
+      { const double r = px1 * y2_2_5;
+        W[10] = fma(0.6324555320336758, r, W[10]);
+        W[13] = fma(-0.38729833462074165, r, W[13]);
+      }
+      }
+      {
+      const double y3_2_5 = py3 * z2_5 + 6.0 * py1 * z2_4;
+      { const double r = px0 * y3_2_5;
+        W[18] = fma(-0.15811388300841894, r, W[18]);
+        W[14] = fma(0.6324555320336758, r, W[14]);
+        W[17] = fma(-0.38729833462074165, r, W[17]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      const double z3_5 = pz3 * g5 + 6.0 * pz1 * g4;
+      {
+      const double y0_3_3 = py0 * z3_3;
+      { const double r = px0 * y0_3_3;
+        W[1] = fma(0.2581988897471611, r, W[1]);
+      }
+      const double y0_3_4 = py0 * z3_4;
+      { const double r = px1 * y0_3_4;
+        W[3] = fma(0.2581988897471611, r, W[3]);
+      }
+      const double y0_3_5 = py0 * z3_5;
+      { const double r = px2 * y0_3_5 + 2.0 * px0 * y0_3_4;
+        W[19] = fma(-0.38729833462074165, r, W[19]);
+        W[9] = fma(0.2581988897471611, r, W[9]);
+      }
+      }
+      {
+      const double y1_3_4 = py1 * z3_4;
+      { const double r = px0 * y1_3_4;
+        W[6] = fma(0.6324555320336758, r, W[6]);
+        W[5] = fma(0.2581988897471611, r, W[5]);
+      }
+      const double y1_3_5 = py1 * z3_5;
+      { const double r = px1 * y1_3_5;
+        W[12] = fma(0.6324555320336758, r, W[12]);
+        W[11] = fma(0.2581988897471611, r, W[11]);
+      }
+      }
+      {
+      const double y2_3_5 = py2 * z3_5 + 2.0 * py0 * z3_4;
+      { const double r = px0 * y2_3_5;
+        W[16] = fma(0.6324555320336758, r, W[16]);
+        W[19] = fma(-0.38729833462074165, r, W[19]);
+        W[15] = fma(0.2581988897471611, r, W[15]);
+      }
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      const double z4_5 = pz4 * g5 + 12.0 * pz2 * g4 + 12.0 * pz0 * g3;
+      {
+      const double y0_4_4 = py0 * z4_4;
+      { const double r = px0 * y0_4_4;
+        W[7] = fma(0.2581988897471611, r, W[7]);
+      }
+      const double y0_4_5 = py0 * z4_5;
+      { const double r = px1 * y0_4_5;
+        W[13] = fma(0.2581988897471611, r, W[13]);
+      }
+      }
+      {
+      const double y1_4_5 = py1 * z4_5;
+      { const double r = px0 * y1_4_5;
+        W[18] = fma(0.6324555320336758, r, W[18]);
+        W[17] = fma(0.2581988897471611, r, W[17]);
+      }
+      }
+      }
+      {
+      const double z5_5 = pz5 * g5 + 20.0 * pz3 * g4 + 60.0 * pz1 * g3;
+      {
+      const double y0_5_5 = py0 * z5_5;
+      { const double r = px0 * y0_5_5;
+        W[19] = fma(0.2581988897471611, r, W[19]);
+      }
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[2], acc[0]);
+        acc[1] = fma(e, W[3], acc[1]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[8], acc[0]);
+        acc[1] = fma(e, W[9], acc[1]);
+      }
+      { const double e = rec[7];
+        acc[2] = fma(e, W[0], acc[2]);
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[8];
+        acc[2] = fma(e, W[4], acc[2]);
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[9];
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+      }
+      { const double e = rec[10];
+        acc[2] = fma(e, W[10], acc[2]);
+        acc[3] = fma(e, W[11], acc[3]);
+      }
+      { const double e = rec[11];
+        acc[4] = fma(e, W[0], acc[4]);
+        acc[5] = fma(e, W[1], acc[5]);
+      }
+      { const double e = rec[12];
+        acc[4] = fma(e, W[6], acc[4]);
+        acc[5] = fma(e, W[7], acc[5]);
+      }
+      { const double e = rec[13];
+        acc[4] = fma(e, W[2], acc[4]);
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[14];
+        acc[4] = fma(e, W[12], acc[4]);
+        acc[5] = fma(e, W[13], acc[5]);
+      }
+      { const double e = rec[15];
+        acc[6] = fma(e, W[0], acc[6]);
+        acc[7] = fma(e, W[1], acc[7]);
+      }
+      { const double e = rec[16];
+        acc[6] = fma(e, W[4], acc[6]);
+        acc[7] = fma(e, W[5], acc[7]);
+      }
+      { const double e = rec[17];
+        acc[6] = fma(e, W[2], acc[6]);
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[18];
+        acc[6] = fma(e, W[10], acc[6]);
+        acc[7] = fma(e, W[11], acc[7]);
+      }
+      { const double e = rec[19];
+        acc[8] = fma(e, W[0], acc[8]);
+        acc[9] = fma(e, W[1], acc[9]);
+      }
+      { const double e = rec[20];
+        acc[8] = fma(e, W[4], acc[8]);
+        acc[9] = fma(e, W[5], acc[9]);
+      }
+      { const double e = rec[21];
+        acc[8] = fma(e, W[14], acc[8]);
+        acc[9] = fma(e, W[15], acc[9]);
+      }
+      { const double e = rec[22];
+        acc[10] = fma(e, W[0], acc[10]);
+        acc[11] = fma(e, W[1], acc[11]);
+      }
+      { const double e = rec[23];
+        acc[10] = fma(e, W[6], acc[10]);
+        acc[11] = fma(e, W[7], acc[11]);
+      }
+      { const double e = rec[24];
+        acc[10] = fma(e, W[4], acc[10]);
+        acc[11] = fma(e, W[5], acc[11]);
+      }
+      { const double e = rec[25];
+        acc[10] = fma(e, W[16], acc[10]);
+        acc[11] = fma(e, W[17], acc[11]);
+      }
+      { const double e = rec[26];
+        acc[12] = fma(e, W[0], acc[12]);
+        acc[13] = fma(e, W[1], acc[13]);
+      }
+      { const double e = rec[27];
+        acc[12] = fma(e, W[6], acc[12]);
+        acc[13] = fma(e, W[7], acc[13]);
+      }
+      { const double e = rec[28];
+        acc[12] = fma(e, W[2], acc[12]);
+        acc[13] = fma(e, W[3], acc[13]);
+      }
+      { const double e = rec[29];
+        acc[12] = fma(e, W[12], acc[12]);
+        acc[13] = fma(e, W[13], acc[13]);
+      }
+      { const double e = rec[30];
+        acc[14] = fma(e, W[0], acc[14]);
+        acc[15] = fma(e, W[1], acc[15]);
+      }
+      { const double e = rec[31];
+        acc[14] = fma(e, W[6], acc[14]);
+        acc[15] = fma(e, W[7], acc[15]);
+      }
+      { const double e = rec[32];
+        acc[14] = fma(e, W[4], acc[14]);
+        acc[15] = fma(e, W[5], acc[15]);
+      }
+      { const double e = rec[33];
+        acc[14] = fma(e, W[16], acc[14]);
+        acc[15] = fma(e, W[17], acc[15]);
+      }
+      { const double e = rec[34];
+        acc[16] = fma(e, W[0], acc[16]);
+        acc[17] = fma(e, W[1], acc[17]);
+      }
+      { const double e = rec[35];
+        acc[16] = fma(e, W[6], acc[16]);
+        acc[17] = fma(e, W[7], acc[17]);
+      }
+      { const double e = rec[36];
+        acc[16] = fma(e, W[18], acc[16]);
+        acc[17] = fma(e, W[19], acc[17]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 3] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 2] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 3] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 2] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 3] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 2] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 3] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 2] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 3] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 2] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 3] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 2] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 3] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 2] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 3] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 2] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 3] = 1.0 * acc[1];
+    }
+    break;
+  }
+  case 2: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[6];
+      boys_array<5>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      sc *= ma; const double g5 = sc * F[5];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double px5 = px4 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double py5 = py4 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      const double pz5 = pz4 * pz1;
+      double W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      const double z0_5 = pz0 * g5;
+      {
+      const double y0_0_3 = py0 * z0_3;
+      const double y0_0_2 = py0 * z0_2;
+      { const double r = px3 * y0_0_3 + 6.0 * px1 * y0_0_2;
+        W[0] = fma(-0.15811388300841894, r, W[0]);
+      }
+      const double y0_0_4 = py0 * z0_4;
+      { const double r = px4 * y0_0_4 + 12.0 * px2 * y0_0_3 + 12.0 * px0 * y0_0_2;
+        W[2] = fma(-0.15811388300841894, r, W[2]);
+      }
+      const double y0_0_5 = py0 * z0_5;
+      { const double r = px5 * y0_0_5 + 20.0 * px3 * y0_0_4 + 60.0 * px1 * y0_0_3;
+        W[8] = fma(-0.15811388300841894, r, W[8]);
+      }
+      }
+      {
+      const double y1_0_4 = py1 * z0_4;
+      const double y1_0_3 = py1 * z0_3;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[4] = fma(-0.15811388300841894, r, W[4]);
+      }
+      const double y1_0_5 = py1 * z0_5;
+      { const double r = px4 * y1_0_5 + 12.0 * px2 * y1_0_4 + 12.0 * px0 * y1_0_3;
+        W[10] = fma(-0.15811388300841894, r, W[10]);
+      }
+      }
+      {
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[0] = fma(-0.15811388300841894, r, W[0]);
+      }
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[2] = fma(-0.15811388300841894, r, W[2]);
+      }
+      const double y2_0_5 = py2 * z0_5 + 2.0 * py0 * z0_4;
+      { const double r = px3 * y2_0_5 + 6.0 * px1 * y2_0_4;
+        W[14] = fma(-0.15811388300841894, r, W[14]);
+        W[8] = fma(-0.15811388300841894, r, W[8]);
+      }
+      }
+      {
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[4] = fma(-0.15811388300841894, r, W[4]);
+      }
+      const double y3_0_5 = py3 * z0_5 + 6.0 * py1 * z0_4;
+      { const double r = px2 * y3_0_5 + 2.0 * px0 * y3_0_4;
+        W[10] = fma(-0.15811388300841894, r, W[10]);
+      }
+      }
+      {
+      const double y4_0_5 = py4 * z0_5 + 12.0 * py2 * z0_4 + 12.0 * py0 * z0_3;
+      { const double r = px1 * y4_0_5;
+        W[14] = fma(-0.15811388300841894, r, W[14]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      const double z1_5 = pz1 * g5;
+      {
+      const double y0_1_3 = py0 * z1_3;
+      const double y0_1_2 = py0 * z1_2;
+      { const double r = px2 * y0_1_3 + 2.0 * px0 * y0_1_2;
+        W[1] = fma(0.5, r, W[1]);
+      }
+      const double y0_1_4 = py0 * z1_4;
+      { const double r = px3 * y0_1_4 + 6.0 * px1 * y0_1_3;
+        W[6] = fma(-0.15811388300841894, r, W[6]);
+        W[3] = fma(0.5, r, W[3]);
+      }
+      const double y0_1_5 = py0 * z1_5;
+      { const double r = px4 * y0_1_5 + 12.0 * px2 * y0_1_4 + 12.0 * px0 * y0_1_3;
+        W[12] = fma(-0.15811388300841894, r, W[12]);
+        W[9] = fma(0.5, r, W[9]);
+      }
+      }
+      {
+      const double y1_1_4 = py1 * z1_4;
+      const double y1_1_3 = py1 * z1_3;
+      { const double r = px2 * y1_1_4 + 2.0 * px0 * y1_1_3;
+        W[5] = fma(0.5, r, W[5]);
+      }
+      const double y1_1_5 = py1 * z1_5;
+      { const double r = px3 * y1_1_5 + 6.0 * px1 * y1_1_4;
+        W[16] = fma(-0.15811388300841894, r, W[16]);
+        W[11] = fma(0.5, r, W[11]);
+      }
+      }
+      {
+      const double y2_1_3 = py2 * z1_3 + 2.0 * py0 * z1_2;
+      { const double r = px0 * y2_1_3;
+        W[1] = fma(-0.5, r, W[1]);
+      }
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[6] = fma(-0.15811388300841894, r, W[6]);
+        W[3] = fma(-0.5, r, W[3]);
+      }
+      const double y2_1_5 = py2 * z1_5 + 2.0 * py0 * z1_4;
+      { const double r = px2 * y2_1_5 + 2.0 * px0 * y2_1_4;
+        W[12] = fma(-0.15811388300841894, r, W[12]);
+        W[15] = fma(0.5, r, W[15]);
+        W[9] = fma(-0.5, r, W[9]);
+      }
+      }
+      {
+      const double y3_1_4 = py3 * z1_4 + 6.0 * py1 * z1_3;
+      { const double r = px0 * y3_1_4;
+        W[5] = fma(-0.5, r, W[5]);
+      }
+      const double y3_1_5 = py3 * z1_5 + 6.0 * py1 * z1_4;
+      { const double r = px1 * y3_1_5;
+        W[16] = fma(-0.15811388300841894, r, W[16]);
+        W[11] = fma(-0.5, r, W[11]);
+      }
+      }
+      {
+      const double y4_1_5 = py4 * z1_5 + 12.0 * py2 * z1_4 + 12.0 * py0 * z1_3;
+      { const double r = px0 * y4_1_5;
+        W[15] = fma(-0.5, r, W[15]);
+      }
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      const double z2_5 = pz2 * g5 + 2.0 * pz0 * g4;
+      {
+      const double y0_2_3 = py0 * z2_3;
+      { const double r = px1 * y0_2_3;
+        W[0] = fma(0.6324555320336758, r, W[0]);
+      }
+      const double y0_2_4 = py0 * z2_4;
+      { const double r = px2 * y0_2_4 + 2.0 * px0 * y0_2_3;
+        W[2] = fma(0.6324555320336758, r, W[2]);
+        W[7] = fma(0.5, r, W[7]);
+      }
+      const double y0_2_5 = py0 * z2_5;
+      { const double r = px3 * y0_2_5 + 6.0 * px1 * y0_2_4;
+        W[18] = fma(-0.15811388300841894, r, W[18]);
+        W[8] = fma(0.6324555320336758, r, W[8]);
+        W[13] = fma(0.5, r, W[13]);
+      }
+      }
+      {
+      const double y1_2_4 = py1 * z2_4;
+      { const double r = px1 * y1_2_4;
+        W[4] = fma(0.6324555320336758, r, W[4]);
+      }
+      const double y1_2_5 = py1 * z2_5;
+      { const double r = px2 * y1_2_5 + 2.0 * px0 * y1_2_4;
+        W[10] = fma(0.6324555320336758, r, W[10]);
+        W[17] = fma(0.5, r, W[17]);
+      }
+      }
+      {
+      const double y2_2_4 = py2 * z2_4 + 2.0 * py0 * z2_3;
+      { const double r = px0 * y2_2_4;
+        W[7] = fma(-0.5, r, W[7]);
+      }
+      const double y2_2_5 = py2 * z2_5 + 2.0 * py0 * z2_4;
+      { const double r = px1 * y2_2_5;
+        W[18] = fma(-0.15811388300841894, r, W[18]);
+        W[14] = fma(0.6324555320336758, r, W[14]);
+        W[13] = fma(-0.5, r, W[13]);
+      }
+      }
+      {
+      const double y3_2_5 = py3 * z2_5 + 6.0 * py1 * z2_4;
+      { const double r = px0 * y3_2_5;
+        W[17] = fma(-0.5, r, W[17]);
+      }
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      const double z3_5 = pz3 * g5 + 6.0 * pz1 * g4;
+      {
+      const double y0_3_4 = py0 * z3_4;
+      { const double r = px1 * y0_3_4;
+        W[6] = fma(0.6324555320336758, r, W[6]);
+      }
+      const double y0_3_5 = py0 * z3_5;
+      { const double r = px2 * y0_3_5 + 2.0 * px0 * y0_3_4;
+        W[12] = fma(0.6324555320336758, r, W[12]);
+        W[19] = fma(0.5, r, W[19]);
+      }
+      }
+      {
+      const double y1_3_5 = py1 * z3_5;
+      { const double r = px1 * y1_3_5;
+        W[16] = fma(0.6324555320336758, r, W[16]);
+      }
+      }
+      {
+      const double y2_3_5 = py2 * z3_5 + 2.0 * py0 * z3_4;
+      { const double r = px0 * y2_3_5;
+        W[19] = fma(-0.5, r, W[19]);
+      }
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      const double z4_5 = pz4 * g5 + 12.0 * pz2 * g4 + 12.0 * pz0 * g3;
+      {
+      const double y0_4_5 = py0 * z4_5;
+      { const double r = px1 * y0_4_5;
+        W[18] = fma(0.6324555320336758, r, W[18]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z5_5 = pz5 * g5 + 20.0 * pz3 * g4 + 60.0 * pz1 * g3;
+      {
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[2], acc[0]);
+        acc[1] = fma(e, W[3], acc[1]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[8], acc[0]);
+        acc[1] = fma(e, W[9], acc[1]);
+      }
+      { const double e = rec[7];
+        acc[2] = fma(e, W[0], acc[2]);
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[8];
+        acc[2] = fma(e, W[4], acc[2]);
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[9];
+        acc[2] = fma(e, W[2], acc[2]);
+        acc[3] = fma(e, W[3], acc[3]);
+      }
+      { const double e = rec[10];
+        acc[2] = fma(e, W[10], acc[2]);
+        acc[3] = fma(e, W[11], acc[3]);
+      }
+      { const double e = rec[11];
+        acc[4] = fma(e, W[0], acc[4]);
+        acc[5] = fma(e, W[1], acc[5]);
+      }
+      { const double e = rec[12];
+        acc[4] = fma(e, W[6], acc[4]);
+        acc[5] = fma(e, W[7], acc[5]);
+      }
+      { const double e = rec[13];
+        acc[4] = fma(e, W[2], acc[4]);
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[14];
+        acc[4] = fma(e, W[12], acc[4]);
+        acc[5] = fma(e, W[13], acc[5]);
+      }
+      { const double e = rec[15];
+        acc[6] = fma(e, W[0], acc[6]);
+        acc[7] = fma(e, W[1], acc[7]);
+      }
+      { const double e = rec[16];
+        acc[6] = fma(e, W[4], acc[6]);
+        acc[7] = fma(e, W[5], acc[7]);
+      }
+      { const double e = rec[17];
+        acc[6] = fma(e, W[2], acc[6]);
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[18];
+        acc[6] = fma(e, W[10], acc[6]);
+        acc[7] = fma(e, W[11], acc[7]);
+      }
+      { const double e = rec[19];
+        acc[8] = fma(e, W[0], acc[8]);
+        acc[9] = fma(e, W[1], acc[9]);
+      }
+      { const double e = rec[20];
+        acc[8] = fma(e, W[4], acc[8]);
+        acc[9] = fma(e, W[5], acc[9]);
+      }
+      { const double e = rec[21];
+        acc[8] = fma(e, W[14], acc[8]);
+        acc[9] = fma(e, W[15], acc[9]);
+      }
+      { const double e = rec[22];
+        acc[10] = fma(e, W[0], acc[10]);
+        acc[11] = fma(e, W[1], acc[11]);
+      }
+      { const double e = rec[23];
+        acc[10] = fma(e, W[6], acc[10]);
+        acc[11] = fma(e, W[7], acc[11]);
+      }
+      { const double e = rec[24];
+        acc[10] = fma(e, W[4], acc[10]);
+        acc[11] = fma(e, W[5], acc[11]);
+      }
+      { const double e = rec[25];
+        acc[10] = fma(e, W[16], acc[10]);
+        acc[11] = fma(e, W[17], acc[11]);
+      }
+      { const double e = rec[26];
+        acc[12] = fma(e, W[0], acc[12]);
+        acc[13] = fma(e, W[1], acc[13]);
+      }
+      { const double e = rec[27];
+        acc[12] = fma(e, W[6], acc[12]);
+        acc[13] = fma(e, W[7], acc[13]);
+      }
+      { const double e = rec[28];
+        acc[12] = fma(e, W[2], acc[12]);
+        acc[13] = fma(e, W[3], acc[13]);
+      }
+      { const double e = rec[29];
+        acc[12] = fma(e, W[12], acc[12]);
+        acc[13] = fma(e, W[13], acc[13]);
+      }
+      { const double e = rec[30];
+        acc[14] = fma(e, W[0], acc[14]);
+        acc[15] = fma(e, W[1], acc[15]);
+      }
+      { const double e = rec[31];
+        acc[14] = fma(e, W[6], acc[14]);
+        acc[15] = fma(e, W[7], acc[15]);
+      }
+      { const double e = rec[32];
+        acc[14] = fma(e, W[4], acc[14]);
+        acc[15] = fma(e, W[5], acc[15]);
+      }
+      { const double e = rec[33];
+        acc[14] = fma(e, W[16], acc[14]);
+        acc[15] = fma(e, W[17], acc[15]);
+      }
+      { const double e = rec[34];
+        acc[16] = fma(e, W[0], acc[16]);
+        acc[17] = fma(e, W[1], acc[17]);
+      }
+      { const double e = rec[35];
+        acc[16] = fma(e, W[6], acc[16]);
+        acc[17] = fma(e, W[7], acc[17]);
+      }
+      { const double e = rec[36];
+        acc[16] = fma(e, W[18], acc[16]);
+        acc[17] = fma(e, W[19], acc[17]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 4] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 5] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 4] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 5] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 4] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 5] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 4] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 5] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 4] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 5] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 4] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 5] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 4] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 5] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 4] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 5] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 4] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 5] = 1.0 * acc[1];
+    }
+    break;
+  }
+  case 3: {
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[6];
+      boys_array<5>(T, boys, F);
+      const double ma = -alpha;
+      const double g0 = sc * F[0];
+      sc *= ma; const double g1 = sc * F[1];
+      sc *= ma; const double g2 = sc * F[2];
+      sc *= ma; const double g3 = sc * F[3];
+      sc *= ma; const double g4 = sc * F[4];
+      sc *= ma; const double g5 = sc * F[5];
+      const double px0 = 1.0, px1 = 2.0 * PCx;
+      const double px2 = px1 * px1;
+      const double px3 = px2 * px1;
+      const double px4 = px3 * px1;
+      const double px5 = px4 * px1;
+      const double py0 = 1.0, py1 = 2.0 * PCy;
+      const double py2 = py1 * py1;
+      const double py3 = py2 * py1;
+      const double py4 = py3 * py1;
+      const double py5 = py4 * py1;
+      const double pz0 = 1.0, pz1 = 2.0 * PCz;
+      const double pz2 = pz1 * pz1;
+      const double pz3 = pz2 * pz1;
+      const double pz4 = pz3 * pz1;
+      const double pz5 = pz4 * pz1;
+      double W[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) W[q] = 0.0;
+      {
+      const double z0_0 = pz0 * g0;
+      const double z0_1 = pz0 * g1;
+      const double z0_2 = pz0 * g2;
+      const double z0_3 = pz0 * g3;
+      const double z0_4 = pz0 * g4;
+      const double z0_5 = pz0 * g5;
+      {
+      const double y0_0_3 = py0 * z0_3;
+      const double y0_0_2 = py0 * z0_2;
+      { const double r = px3 * y0_0_3 + 6.0 * px1 * y0_0_2;
+        W[0] = fma(0.20412414523193148, r, W[0]);
+      }
+      const double y0_0_4 = py0 * z0_4;
+      { const double r = px4 * y0_0_4 + 12.0 * px2 * y0_0_3 + 12.0 * px0 * y0_0_2;
+        W[1] = fma(0.20412414523193148, r, W[1]);
+      }
+      const double y0_0_5 = py0 * z0_5;
+      { const double r = px5 * y0_0_5 + 20.0 * px3 * y0_0_4 + 60.0 * px1 * y0_0_3;
+        W[4] = fma(0.20412414523193148, r, W[4]);
+      }
+      }
+      {
+      const double y1_0_4 = py1 * z0_4;
+      const double y1_0_3 = py1 * z0_3;
+      { const double r = px3 * y1_0_4 + 6.0 * px1 * y1_0_3;
+        W[2] = fma(0.20412414523193148, r, W[2]);
+      }
+      const double y1_0_5 = py1 * z0_5;
+      { const double r = px4 * y1_0_5 + 12.0 * px2 * y1_0_4 + 12.0 * px0 * y1_0_3;
+        W[5] = fma(0.20412414523193148, r, W[5]);
+      }
+      }
+      {
+      const double y2_0_3 = py2 * z0_3 + 2.0 * py0 * z0_2;
+      { const double r = px1 * y2_0_3;
+        W[0] = fma(-0.6123724356957945, r, W[0]);
+      }
+      const double y2_0_4 = py2 * z0_4 + 2.0 * py0 * z0_3;
+      { const double r = px2 * y2_0_4 + 2.0 * px0 * y2_0_3;
+        W[1] = fma(-0.6123724356957945, r, W[1]);
+      }
+      const double y2_0_5 = py2 * z0_5 + 2.0 * py0 * z0_4;
+      { const double r = px3 * y2_0_5 + 6.0 * px1 * y2_0_4;
+        W[7] = fma(0.20412414523193148, r, W[7]);
+        W[4] = fma(-0.6123724356957945, r, W[4]);
+      }
+      }
+      {
+      const double y3_0_4 = py3 * z0_4 + 6.0 * py1 * z0_3;
+      { const double r = px1 * y3_0_4;
+        W[2] = fma(-0.6123724356957945, r, W[2]);
+      }
+      const double y3_0_5 = py3 * z0_5 + 6.0 * py1 * z0_4;
+      { const double r = px2 * y3_0_5 + 2.0 * px0 * y3_0_4;
+        W[5] = fma(-0.6123724356957945, r, W[5]);
+      }
+      }
+      {
+      const double y4_0_5 = py4 * z0_5 + 12.0 * py2 * z0_4 + 12.0 * py0 * z0_3;
+      { const double r = px1 * y4_0_5;
+        W[7] = fma(-0.6123724356957945, r, W[7]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z1_1 = pz1 * g1;
+      const double z1_2 = pz1 * g2;
+      const double z1_3 = pz1 * g3;
+      const double z1_4 = pz1 * g4;
+      const double z1_5 = pz1 * g5;
+      {
+      const double y0_1_4 = py0 * z1_4;
+      const double y0_1_3 = py0 * z1_3;
+      { const double r = px3 * y0_1_4 + 6.0 * px1 * y0_1_3;
+        W[3] = fma(0.20412414523193148, r, W[3]);
+      }
+      const double y0_1_5 = py0 * z1_5;
+      { const double r = px4 * y0_1_5 + 12.0 * px2 * y0_1_4 + 12.0 * px0 * y0_1_3;
+        W[6] = fma(0.20412414523193148, r, W[6]);
+      }
+      }
+      {
+      const double y1_1_5 = py1 * z1_5;
+      const double y1_1_4 = py1 * z1_4;
+      { const double r = px3 * y1_1_5 + 6.0 * px1 * y1_1_4;
+        W[8] = fma(0.20412414523193148, r, W[8]);
+      }
+      }
+      {
+      const double y2_1_4 = py2 * z1_4 + 2.0 * py0 * z1_3;
+      { const double r = px1 * y2_1_4;
+        W[3] = fma(-0.6123724356957945, r, W[3]);
+      }
+      const double y2_1_5 = py2 * z1_5 + 2.0 * py0 * z1_4;
+      { const double r = px2 * y2_1_5 + 2.0 * px0 * y2_1_4;
+        W[6] = fma(-0.6123724356957945, r, W[6]);
+      }
+      }
+      {
+      const double y3_1_5 = py3 * z1_5 + 6.0 * py1 * z1_4;
+      { const double r = px1 * y3_1_5;
+        W[8] = fma(-0.6123724356957945, r, W[8]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z2_2 = pz2 * g2 + 2.0 * pz0 * g1;
+      const double z2_3 = pz2 * g3 + 2.0 * pz0 * g2;
+      const double z2_4 = pz2 * g4 + 2.0 * pz0 * g3;
+      const double z2_5 = pz2 * g5 + 2.0 * pz0 * g4;
+      {
+      const double y0_2_5 = py0 * z2_5;
+      const double y0_2_4 = py0 * z2_4;
+      { const double r = px3 * y0_2_5 + 6.0 * px1 * y0_2_4;
+        W[9] = fma(0.20412414523193148, r, W[9]);
+      }
+      }
+      {
+      }
+      {
+      const double y2_2_5 = py2 * z2_5 + 2.0 * py0 * z2_4;
+      { const double r = px1 * y2_2_5;
+        W[9] = fma(-0.6123724356957945, r, W[9]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const double z3_3 = pz3 * g3 + 6.0 * pz1 * g2;
+      const double z3_4 = pz3 * g4 + 6.0 * pz1 * g3;
+      const double z3_5 = pz3 * g5 + 6.0 * pz1 * g4;
+      {
+      }
+      {
+      }
+      {
+      }
+      }
+      {
+      const double z4_4 = pz4 * g4 + 12.0 * pz2 * g3 + 12.0 * pz0 * g2;
+      const double z4_5 = pz4 * g5 + 12.0 * pz2 * g4 + 12.0 * pz0 * g3;
+      {
+      }
+      {
+      }
+      }
+      {
+      const double z5_5 = pz5 * g5 + 20.0 * pz3 * g4 + 60.0 * pz1 * g3;
+      {
+      }
+      }
+      { const double e = rec[4];
+        acc[0] = fma(e, W[0], acc[0]);
+      }
+      { const double e = rec[5];
+        acc[0] = fma(e, W[1], acc[0]);
+      }
+      { const double e = rec[6];
+        acc[0] = fma(e, W[4], acc[0]);
+      }
+      { const double e = rec[7];
+        acc[1] = fma(e, W[0], acc[1]);
+      }
+      { const double e = rec[8];
+        acc[1] = fma(e, W[2], acc[1]);
+      }
+      { const double e = rec[9];
+        acc[1] = fma(e, W[1], acc[1]);
+      }
+      { const double e = rec[10];
+        acc[1] = fma(e, W[5], acc[1]);
+      }
+      { const double e = rec[11];
+        acc[2] = fma(e, W[0], acc[2]);
+      }
+      { const double e = rec[12];
+        acc[2] = fma(e, W[3], acc[2]);
+      }
+      { const double e = rec[13];
+        acc[2] = fma(e, W[1], acc[2]);
+      }
+      { const double e = rec[14];
+        acc[2] = fma(e, W[6], acc[2]);
+      }
+      { const double e = rec[15];
+        acc[3] = fma(e, W[0], acc[3]);
+      }
+      { const double e = rec[16];
+        acc[3] = fma(e, W[2], acc[3]);
+      }
+      { const double e = rec[17];
+        acc[3] = fma(e, W[1], acc[3]);
+      }
+      { const double e = rec[18];
+        acc[3] = fma(e, W[5], acc[3]);
+      }
+      { const double e = rec[19];
+        acc[4] = fma(e, W[0], acc[4]);
+      }
+      { const double e = rec[20];
+        acc[4] = fma(e, W[2], acc[4]);
+      }
+      { const double e = rec[21];
+        acc[4] = fma(e, W[7], acc[4]);
+      }
+      { const double e = rec[22];
+        acc[5] = fma(e, W[0], acc[5]);
+      }
+      { const double e = rec[23];
+        acc[5] = fma(e, W[3], acc[5]);
+      }
+      { const double e = rec[24];
+        acc[5] = fma(e, W[2], acc[5]);
+      }
+      { const double e = rec[25];
+        acc[5] = fma(e, W[8], acc[5]);
+      }
+      { const double e = rec[26];
+        acc[6] = fma(e, W[0], acc[6]);
+      }
+      { const double e = rec[27];
+        acc[6] = fma(e, W[3], acc[6]);
+      }
+      { const double e = rec[28];
+        acc[6] = fma(e, W[1], acc[6]);
+      }
+      { const double e = rec[29];
+        acc[6] = fma(e, W[6], acc[6]);
+      }
+      { const double e = rec[30];
+        acc[7] = fma(e, W[0], acc[7]);
+      }
+      { const double e = rec[31];
+        acc[7] = fma(e, W[3], acc[7]);
+      }
+      { const double e = rec[32];
+        acc[7] = fma(e, W[2], acc[7]);
+      }
+      { const double e = rec[33];
+        acc[7] = fma(e, W[8], acc[7]);
+      }
+      { const double e = rec[34];
+        acc[8] = fma(e, W[0], acc[8]);
+      }
+      { const double e = rec[35];
+        acc[8] = fma(e, W[3], acc[8]);
+      }
+      { const double e = rec[36];
+        acc[8] = fma(e, W[9], acc[8]);
+      }
+      }
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 6] = 1.0 * acc[4];
+    out[(row0 + 1) * ldo + col + 6] = 1.0 * acc[5];
+    out[(row0 + 2) * ldo + col + 6] = 1.0 * acc[3];
+    out[(row0 + 3) * ldo + col + 6] = 1.0 * acc[7];
+    out[(row0 + 4) * ldo + col + 6] = 1.0 * acc[8];
+    out[(row0 + 5) * ldo + col + 6] = 1.0 * acc[6];
+    out[(row0 + 6) * ldo + col + 6] = 1.0 * acc[1];
+    out[(row0 + 7) * ldo + col + 6] = 1.0 * acc[2];
+    out[(row0 + 8) * ldo + col + 6] = 1.0 * acc[0];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+void register_gen3_11(Gen3Table& t) {
+  t[1][1][0] = Gen3Launch{&int3c2e_gen_110, 1, 37};
+  t[1][1][1] = Gen3Launch{&int3c2e_gen_111, 2, 37};
+  t[1][1][2] = Gen3Launch{&int3c2e_gen_112, 3, 37};
+  t[1][1][3] = Gen3Launch{&int3c2e_gen_113, 4, 37};
+}
+
+}  // namespace dfhip

@@ -201,3 +201,16 @@ def test_bitwise_reproducible():
     J1 = intor.contract_coulomb_densities_4d(shells, aux, dens)
     J2 = intor.contract_coulomb_densities_4d(shells, aux, dens)
     assert np.array_equal(J1, J2)
+
+
+def test_generated_vs_lds_kernels(monkeypatch):
+    """The register-resident generated kernels and the LDS kernels agree on
+    every class both implement (workload 0 plus an all-class system)."""
+    shells, aux = _mixed_system(6)
+    a, _ = dense3c(shells, aux, screening=N.SCREEN_NONE)
+    monkeypatch.setenv("DFHIP_NO_GEN", "1")
+    b, _ = dense3c(shells, aux, screening=N.SCREEN_NONE)
+    monkeypatch.delenv("DFHIP_NO_GEN")
+    err = np.abs(a - b).max()
+    _say(f"generated vs LDS kernels: max abs diff {err:.2e}")
+    assert err <= 1e-13 * max(np.abs(b).max(), 1.0)
