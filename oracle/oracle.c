@@ -229,6 +229,10 @@ static void build_sph(void)
     sph_ready = 1;
 }
 
+/* The table is filled once when the library is loaded, before any OpenMP
+ * region can reach build_sph() concurrently. */
+__attribute__((constructor)) static void orc_init_tables(void) { build_sph(); }
+
 void orc_sph_coeffs(int L, double *out /* (2L+1) x ncart(L) */)
 {
     int m, c;

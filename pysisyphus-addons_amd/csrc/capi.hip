@@ -35,6 +35,7 @@ void register_int3c2e_l3(Int3Table&);
 void register_int3c2e_l4(Int3Table&);
 void register_int3c2e_l5(Int3Table&);
 void register_schwarz4c(Schwarz4Table&);
+void register_schwarz4c_cached(Schwarz4CachedTable&);
 
 namespace {
 
@@ -169,6 +170,7 @@ struct Ctx {
   DBuf<SphTable> sph;
   Int3Table int3{};
   Schwarz4Table s4{};
+  Schwarz4CachedTable s4c{};
   Gen3Table gen3{};
 };
 
@@ -205,6 +207,7 @@ void ensure_ctx() {
   register_int3c2e_l4(c.int3);
   register_int3c2e_l5(c.int3);
   register_schwarz4c(c.s4);
+  register_schwarz4c_cached(c.s4c);
   register_gen3_all(c.gen3);
   for (int la = 0; la <= LMAX_AUX; la++)
     for (int lb = 0; lb <= (la <= LMAX_ORB ? la : 0); lb++)
@@ -358,17 +361,27 @@ PairSet make_pairs(const HostShells& o) {
   return ps;
 }
 
-// Schwarz norms of all pairs, in the PairSet's order.
+// Schwarz norms of all pairs, in the PairSet's order.  Classes whose pairs all
+// fit the primitive-pair cache use the cached kernel (one partial per pair),
+// the others the chunked kernel.
 std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps) {
   hipStream_t st = c.st;
   DBuf<int> s1, s2;
   upload(s1, ps.s1, st);
   upload(s2, ps.s2, st);
   std::vector<size_t> poff(NPCLS + 1, 0);
+  std::vector<int> nch(NPCLS, 1), cached(NPCLS, 0);
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb);
-      poff[k + 1] = (size_t)(ps.cls_end[k] - ps.cls_begin[k]) * c.s4[la][lb].nch;
+      int maxnpp = 0;
+      for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++)
+        maxnpp = std::max(maxnpp, o.nprim[ps.s1[p]] * o.nprim[ps.s2[p]]);
+      const char* force = std::getenv("DFHIP_SCHWARZ_CHUNKED");
+      cached[k] = !(force && *force && *force != '0') && c.s4c[la][lb].fn != nullptr &&
+                  maxnpp <= c.s4c[la][lb].nppc;
+      nch[k] = cached[k] ? 1 : c.s4[la][lb].nch;
+      poff[k + 1] = (size_t)(ps.cls_end[k] - ps.cls_begin[k]) * nch[k];
     }
   for (int k = 0; k < NPCLS; k++) poff[k + 1] += poff[k];
   DBuf<double> part(std::max<size_t>(poff[NPCLS], 1));
@@ -376,23 +389,23 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
       if (np == 0) continue;
-      c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+      if (cached[k])
+        c.s4c[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+      else
+        c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       HC(hipGetLastError());
     }
   std::vector<double> hp(poff[NPCLS]);
   if (!hp.empty()) HC(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HC(hipStreamSynchronize(st));
   std::vector<double> q(ps.n());
-  for (int la = 0; la <= LMAX_ORB; la++)
-    for (int lb = 0; lb <= la; lb++) {
-      const int k = pcls(la, lb), nch = c.s4[la][lb].nch;
-      for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++) {
-        double s = 0.0;
-        for (int z = 0; z < nch; z++) s += hp[poff[k] + (size_t)(p - ps.cls_begin[k]) * nch + z];
-        q[p] = std::sqrt(std::sqrt(s));
-      }
+  for (int k = 0; k < NPCLS; k++)
+    for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++) {
+      double s = 0.0;
+      for (int z = 0; z < nch[k]; z++) s += hp[poff[k] + (size_t)(p - ps.cls_begin[k]) * nch[k] + z];
+      q[p] = std::sqrt(std::sqrt(s));
+      if (!std::isfinite(q[p])) throw Error{DFHIP_ERR_HIP, "non-finite Schwarz norm"};
     }
-  (void)o;
   return q;
 }
 
@@ -455,7 +468,7 @@ void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells
 }
 
 // Full metric (P|Q) (naux x naux) into M via the 2c2e form of the 3c2e kernels.
-void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M) {
+void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, int64_t ldm) {
   hipStream_t st = c.st;
   std::vector<std::vector<int>> byL(LMAX_AUX + 1);
   for (int i = 0; i < x.n; i++) byL[x.L[i]].push_back(i);
@@ -502,9 +515,9 @@ void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M) {
       if (nq == 0) continue;
       AuxArgs xa{dlsh.p + lbeg[LQ], dlcol.p + lbeg[LQ], 0, nq, kmax[LQ]};
       if (gen && gen3_has(LP, 0, LQ)) {
-        launch_gen3(c, LP, 0, LQ, brs[LP], dev, pa, xa, M, x.nbf);
+        launch_gen3(c, LP, 0, LQ, brs[LP], dev, pa, xa, M, ldm);
       } else {
-        c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, x.nbf, st);
+        c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, ldm, st);
         HC(hipGetLastError());
       }
     }
@@ -672,29 +685,20 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.qab = schwarz_bra_dev(c, o, dorb.view, pl.ps);
   const int e2 = tm.mark();
 
-  // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal blocks
-  DBuf<double> M((size_t)naux * naux);
-  metric_dev(c, x, daux.view, M.p);
+  // metric (common/integrals/intor.f90:40-95) in the leading block of the
+  // bordered matrix [[M, .], [d^T, 0]] (column-major, lda = naux + ndens),
+  // and Q_C from its diagonal blocks
+  const int64_t ntot = naux + (need_dens ? ndens : 0), lda = ntot;
+  DBuf<double> A((size_t)ntot * ntot);
+  HC(hipMemsetAsync(A.p, 0, A.n * sizeof(double), st));
+  metric_dev(c, x, daux.view, A.p, lda);
   DBuf<double> dqc(x.n);
-  schwarz_aux_from_metric(M.p, naux, daux.L.p, daux.f0.p, x.n, dqc.p, st);
+  schwarz_aux_from_metric(A.p, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
   pl.qc.resize(x.n);
   HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
-  const int e3 = tm.mark();
-
-  // Cholesky of the metric (common/linalg.f90:78)
-  const int nblk = (int)((naux + CHOL_BLOCK - 1) / CHOL_BLOCK);
-  DBuf<double> invblk((size_t)nblk * CHOL_BLOCK * CHOL_BLOCK);
-  DBuf<int> info(1);
-  HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
-  cholesky(M.p, naux, invblk.p, info.p, st);
-  HC(hipGetLastError());
-  const int e4 = tm.mark();
   HC(hipStreamSynchronize(st));
-  int hinfo = 0;
-  HC(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (hinfo != 0)
-    throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " + std::to_string(hinfo) +
-                                       ")"};
+  const int e3 = tm.mark();
+  const int e4 = e3;
 
   build_plan(pl, opt);
   if (opt.verbose) {
@@ -791,12 +795,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   int64_t widest = 0;
   for (int b = 0; b < nblocks; b++) widest = std::max(widest, bounds[b + 1] - bounds[b]);
   DBuf<double> T((size_t)std::max<int64_t>(pl.npf, 1) * std::max<int64_t>(widest, 1));
-  DBuf<double> dd;
   DBuf<double> part;
   int nsplit = 1;
   if (need_dens) {
-    dd.alloc((size_t)naux * ndens);
-    HC(hipMemsetAsync(dd.p, 0, dd.n * sizeof(double), st));
     const int64_t tiles = ((widest + 63) / 64) * ((ndens + 63) / 64);
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(contract_nsplit_max(), 2048 / std::max<int64_t>(tiles, 1)));
     nsplit = (int)std::min<int64_t>(nsplit, std::max<int64_t>(1, (pl.npf + 15) / 16));
@@ -830,7 +831,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     }
     const int b1 = tm.mark();
     if (need_dens) {
-      contract_T(T.p, w, Dp.p, ndens, pl.npf, (int)w, ndens, part.p, nsplit, dorig.p + cA, dd.p, (int)naux, st);
+      // d^T into the border rows: A[(naux + I) + lda * P]
+      contract_T(T.p, w, Dp.p, ndens, pl.npf, (int)w, ndens, part.p, nsplit, dorig.p + cA, A.p + naux, lda, 1, st);
       HC(hipGetLastError());
     }
     const int b2 = tm.mark();
@@ -860,23 +862,39 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   const int e6 = tm.mark();
 
-  // B = L^-1 d (diabatization/intor.f90:198)
+  // Cholesky of the metric bordered by d: L, B^T = (L^-1 d)^T and -B^T B
+  // (common/linalg.f90:78, diabatization/intor.f90:198, :241-269)
+  int e7 = e6, e8 = e6;
+  DBuf<double> dB, dJ;
   if (need_dens) {
-    forward_solve(M.p, invblk.p, dd.p, naux, ndens, st);
+    const int nblk = (int)((naux + CHOL_BLOCK - 1) / CHOL_BLOCK);
+    DBuf<double> invblk((size_t)nblk * CHOL_BLOCK * CHOL_BLOCK);
+    DBuf<int> info(1);
+    HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
+    cholesky_bordered(A.p, lda, naux, ntot, invblk.p, info.p, st);
     HC(hipGetLastError());
+    e7 = tm.mark();
+    HC(hipStreamSynchronize(st));
+    int hinfo = 0;
+    HC(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (hinfo != 0)
+      throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
+                                         std::to_string(hinfo) + ")"};
+    if (out.df) {
+      dB.alloc((size_t)naux * ndens);
+      border_to_b(A.p, lda, naux, ndens, dB.p, st);
+      HC(hipGetLastError());
+    }
+    if (out.J) {
+      const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;
+      dJ.alloc(nj);
+      HC(hipMemsetAsync(dJ.p, 0, nj * sizeof(double), st));
+      assemble_j(A.p, lda, naux, ndens, out.nstates, dJ.p, st);
+      HC(hipGetLastError());
+    }
+    e8 = tm.mark();
   }
-  const int e7 = tm.mark();
-  DBuf<double> dJ;
-  if (out.J) {
-    const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;
-    dJ.alloc(nj);
-    HC(hipMemsetAsync(dJ.p, 0, nj * sizeof(double), st));
-    assemble_j(dd.p, naux, ndens, out.nstates, dJ.p, st);
-    HC(hipGetLastError());
-  }
-  const int e8 = tm.mark();
-  if (out.df)
-    HC(hipMemcpyAsync(out.df, dd.p, (size_t)naux * ndens * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (out.df) HC(hipMemcpyAsync(out.df, dB.p, (size_t)naux * ndens * sizeof(double), hipMemcpyDeviceToHost, st));
   if (out.J) HC(hipMemcpyAsync(out.J, dJ.p, dJ.n * sizeof(double), hipMemcpyDeviceToHost, st));
   const int e9 = tm.mark();
   HC(hipStreamSynchronize(st));
@@ -893,12 +911,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     stats->ms_upload = tm.ms(e0, e1);
     stats->ms_schwarz = tm.ms(e1, e2);
     stats->ms_metric = tm.ms(e2, e3);
-    stats->ms_cholesky = tm.ms(e3, e4);
+    stats->ms_cholesky = tm.ms(e6, e7);
     stats->ms_pack = tm.ms(e4, e5);
     stats->ms_int3c2e = ms_int;
     stats->ms_contract = ms_con;
-    stats->ms_solve = tm.ms(e6, e7);
+    stats->ms_solve = 0.0;
     stats->ms_assemble = tm.ms(e7, e8);
+    (void)e4;
     stats->ms_download = tm.ms(e8, e9);
     stats->ms_total = tm.ms(e0, e9);
     stats->ms_h2d_densities = ms_h2d;
@@ -910,7 +929,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (opt.verbose) {
     std::printf("Schwarz norms took %8.4f s.\n", tm.ms(e1, e2) * 1e-3);
     std::printf("Calculation of density fitting metric (P|Q) took %8.4f s.\n", tm.ms(e2, e3) * 1e-3);
-    std::printf("Cholesky factorization of (P|Q) took %8.4f s.\n", tm.ms(e3, e4) * 1e-3);
+    std::printf("Cholesky factorization of (P|Q) bordered by d took %8.4f s.\n", tm.ms(e6, e7) * 1e-3);
     std::printf("Density contraction took %8.4f s.\n", (ms_int + ms_con) * 1e-3);
     const int64_t skipped = pl.ntriplets - pl.ncomputed;
     std::printf("Screened out %12lld of %12lld integral triplets (%8.2f%%)\n", (long long)skipped,
@@ -962,7 +981,7 @@ std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
   DevShellBufs d;
   d.upload_from(x, c.st);
   DBuf<double> M((size_t)x.nbf * x.nbf);
-  metric_dev(c, x, d.view, M.p);
+  metric_dev(c, x, d.view, M.p, x.nbf);
   std::vector<double> h((size_t)x.nbf * x.nbf);
   if (qc) {
     DBuf<double> dq(x.n);

@@ -12,11 +12,11 @@ void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, 
                   const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
                   hipStream_t st);
 void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int64_t npf, int ncol, int ndens,
-                double* part, int nsplit, const int* orig, double* d, int ldd_out, hipStream_t st);
+                double* part, int nsplit, const int* orig, double* d, int64_t sP, int64_t sI, hipStream_t st);
 int contract_nsplit_max();
-void cholesky(double* A, int64_t n, double* invblk, int* info, hipStream_t st);
-void forward_solve(const double* A, const double* invblk, double* d, int64_t n, int nrhs, hipStream_t st);
-void assemble_j(const double* B, int64_t naux, int ndens, int nstates, double* J, hipStream_t st);
+void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, double* invblk, int* info, hipStream_t st);
+void border_to_b(const double* A, int64_t lda, int64_t n, int ndens, double* B, hipStream_t st);
+void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st);
 void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,
                              hipStream_t st);
 void boys_probe(const int* n, const double* T, int64_t npts, const double* tab, double* F, hipStream_t st);

@@ -1,9 +1,9 @@
 // Dense FP64 kernels of the density-fitting path:
 //   * pair-packing of the (transition) densities,
 //   * the contraction d(P, I) = sum_k T(k, P) Dp(k, I) (split-K GEMM),
-//   * blocked Cholesky of the metric (dpotrf('L') of common/linalg.f90:78),
-//   * the forward solve B = L^-1 d (what dtrtri + dtrmm of
-//     diabatization/intor.f90:198 compute),
+//   * the blocked Cholesky of the metric bordered by d, which yields
+//     L (dpotrf('L'), common/linalg.f90:78), B^T = (L^-1 d)^T
+//     (dtrtri + dtrmm, diabatization/intor.f90:198) and -B^T B,
 //   * the 8-fold Coulomb-tensor assembly (diabatization/intor.f90:241-269),
 //   * Schwarz norms of the auxiliary shells from the metric's diagonal blocks,
 //   * a Boys-function probe.
@@ -91,19 +91,20 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
     }
 }
 
-// d[orig(P) + ldd_out * I] += sum_s part[s][P][I]   (fixed summation order)
+// d[orig(P) * sP + I * sI] += sum_s part[s][P][I]   (fixed summation order)
 __global__ void reduce_unpermute_kernel(const double* __restrict__ part, int nsplit, int ncol, int ndens,
-                                        const int* __restrict__ orig, double* __restrict__ d, int ldd_out) {
+                                        const int* __restrict__ orig, double* __restrict__ d, int64_t sP,
+                                        int64_t sI) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)ncol * ndens) return;
   const int P = (int)(e / ndens), I = (int)(e - (int64_t)P * ndens);
   double s = 0.0;
   for (int z = 0; z < nsplit; z++) s += part[((size_t)z * ncol + P) * ndens + I];
-  d[orig[P] + (int64_t)ldd_out * I] += s;
+  d[orig[P] * sP + I * sI] += s;
 }
 
 void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int64_t npf, int ncol, int ndens,
-                double* part, int nsplit, const int* orig, double* d, int ldd_out, hipStream_t st) {
+                double* part, int nsplit, const int* orig, double* d, int64_t sP, int64_t sI, hipStream_t st) {
   if (npf <= 0 || ncol <= 0) return;
   int64_t kchunk = (npf + nsplit - 1) / nsplit;
   kchunk = (kchunk + GK - 1) / GK * GK;
@@ -113,76 +114,80 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int64_t
                      part);
   const int64_t n = (int64_t)ncol * ndens;
   hipLaunchKernelGGL(reduce_unpermute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, ns, ncol,
-                     ndens, orig, d, ldd_out);
+                     ndens, orig, d, sP, sI);
 }
 
 int contract_nsplit_max() { return 512; }
 
 // ---------------------------------------------------------------------------
-// Blocked right-looking Cholesky, lower, column-major A (n x n, lda = n).
+// Blocked right-looking Cholesky of the bordered matrix
+//     [ M   . ]      M = (P|Q), n x n
+//     [ d^T 0 ]      d = (ab|P) D, naux x ndens
+// stored lower-column-major in A (ntot = n + ndens rows/cols, lda).  Only the
+// first n columns are factored.  Afterwards the leading block holds L
+// (M = L L^T, dpotrf('L') of common/linalg.f90:78), the border rows hold
+// B^T = (L^-1 d)^T (what dtrtri + dtrmm of diabatization/intor.f90:198
+// compute) and the trailing ndens x ndens block holds -B^T B, the packed
+// Coulomb tensor of diabatization/intor.f90:241-269 with a minus sign.
 constexpr int CB = 64;
 
-// Factor the diagonal block at k0 in LDS, store L11 and inv(L11).
-__global__ __launch_bounds__(256) void chol_diag_kernel(double* __restrict__ A, int64_t n, int k0, int nb,
-                                                        double* __restrict__ inv, int* __restrict__ info) {
+// Factor the diagonal block at k0 (left-looking, one wave, row r per lane)
+// and store L11 and inv(L11).
+__global__ __launch_bounds__(64) void chol_diag_kernel(double* __restrict__ A, int64_t lda, int k0, int nb,
+                                                       double* __restrict__ inv, int* __restrict__ info) {
   __shared__ double S[CB][CB + 1];
   __shared__ double Iv[CB][CB + 1];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
-  if (tid == 0) bad = 0;
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    S[r][c] = A[(k0 + r) + n * (k0 + c)];
-  }
+  __shared__ double piv;
+  const int r = threadIdx.x;
+  for (int c = 0; c < nb; c++)
+    if (r < nb) S[r][c] = A[(k0 + r) + lda * (k0 + c)];
   __syncthreads();
   for (int j = 0; j < nb; j++) {
-    if (tid == 0) {
-      double dj = S[j][j];
+    double t = 0.0;
+    if (r >= j && r < nb) {
+      t = S[r][j];
+      for (int k = 0; k < j; k++) t = fma(-S[r][k], S[j][k], t);
+    }
+    if (r == j) {
+      double dj = t;
       if (!(dj > 0.0)) {
-        if (!bad) {
-          bad = 1;
-          atomicCAS(info, 0, k0 + j + 1);
-        }
+        atomicCAS(info, 0, k0 + j + 1);
         dj = 1.0;
       }
-      S[j][j] = sqrt(dj);
+      piv = sqrt(dj);
     }
     __syncthreads();
-    for (int i = j + 1 + tid; i < nb; i += blockDim.x) S[i][j] /= S[j][j];
-    __syncthreads();
-    const int m = nb - j - 1;
-    for (int e = tid; e < m * m; e += blockDim.x) {
-      const int r = j + 1 + e % m, c = j + 1 + e / m;
-      if (r >= c) S[r][c] -= S[r][j] * S[c][j];
-    }
+    const double d = piv;
+    if (r == j) S[j][j] = d;
+    else if (r > j && r < nb) S[r][j] = t / d;
     __syncthreads();
   }
-  // inverse of the lower-triangular L11, one column per thread
-  if (tid < nb) {
-    const int c = tid;
-    for (int i = 0; i < nb; i++) Iv[i][c] = 0.0;
+  // column c of inv(L11) by forward substitution in lane c
+  if (r < nb) {
+    const int c = r;
+    for (int i = 0; i < c; i++) Iv[i][c] = 0.0;
     Iv[c][c] = 1.0 / S[c][c];
     for (int i = c + 1; i < nb; i++) {
       double s = 0.0;
-      for (int k = c; k < i; k++) s += S[i][k] * Iv[k][c];
+      for (int k = c; k < i; k++) s = fma(S[i][k], Iv[k][c], s);
       Iv[i][c] = -s / S[i][i];
     }
   }
   __syncthreads();
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    if (r >= c) A[(k0 + r) + n * (k0 + c)] = S[r][c];
-    inv[r + CB * c] = (r >= c) ? Iv[r][c] : 0.0;
-  }
+  for (int c = 0; c < nb; c++)
+    if (r < nb) {
+      if (r >= c) A[(k0 + r) + lda * (k0 + c)] = S[r][c];
+      inv[r + CB * c] = (r >= c) ? Iv[r][c] : 0.0;
+    }
 }
 
-// L21 = A21 inv(L11)^T for 64 rows per workgroup.
-__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, int64_t n, int k0, int nb,
-                                                         const double* __restrict__ inv) {
+// L21 = A21 inv(L11)^T for 64 rows per workgroup (rows k0+nb .. ntot-1).
+__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, int64_t lda, int64_t ntot, int k0,
+                                                         int nb, const double* __restrict__ inv) {
   __shared__ double X[CB][CB + 1];
   __shared__ double Iv[CB][CB + 1];
-  const int r0 = k0 + nb + blockIdx.x * CB;
-  const int nr = min(CB, (int)(n - r0));
+  const int64_t r0 = k0 + nb + (int64_t)blockIdx.x * CB;
+  const int nr = (int)min((int64_t)CB, ntot - r0);
   const int tid = threadIdx.x;
   for (int e = tid; e < nb * nb; e += blockDim.x) {
     const int r = e % nb, c = e / nb;
@@ -190,35 +195,36 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A,
   }
   for (int e = tid; e < nr * nb; e += blockDim.x) {
     const int r = e % nr, c = e / nr;
-    X[r][c] = A[(r0 + r) + n * (k0 + c)];
+    X[r][c] = A[(r0 + r) + lda * (k0 + c)];
   }
   __syncthreads();
   for (int e = tid; e < nr * nb; e += blockDim.x) {
     const int r = e % nr, c = e / nr;
     double s = 0.0;
-    for (int l = 0; l <= c; l++) s += X[r][l] * Iv[c][l];
-    A[(r0 + r) + n * (k0 + c)] = s;
+    for (int l = 0; l <= c; l++) s = fma(X[r][l], Iv[c][l], s);
+    A[(r0 + r) + lda * (k0 + c)] = s;
   }
 }
 
-// A22 -= L21 L21^T on the lower 64x64 tiles of the trailing matrix.
-__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int64_t n, int k0, int nb) {
+// A22 -= L21 L21^T on the lower 64x64 tiles of the trailing matrix
+// (rows/cols k0+nb .. ntot-1).
+__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int64_t lda, int64_t ntot, int k0,
+                                                          int nb) {
   __shared__ double Pr[CB][CB + 1];
   __shared__ double Pc[CB][CB + 1];
-  const int m0 = k0 + nb;
-  // blockIdx.x -> (tr, tc) with tr >= tc
+  const int64_t m0 = k0 + nb;
   const int q = blockIdx.x;
   int tr = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
   while ((tr + 1) * (tr + 2) / 2 <= q) tr++;
   while (tr * (tr + 1) / 2 > q) tr--;
   const int tc = q - tr * (tr + 1) / 2;
-  const int r0 = m0 + tr * CB, c0 = m0 + tc * CB;
-  const int nr = min(CB, (int)(n - r0)), ncl = min(CB, (int)(n - c0));
+  const int64_t r0 = m0 + (int64_t)tr * CB, c0 = m0 + (int64_t)tc * CB;
+  const int nr = (int)min((int64_t)CB, ntot - r0), ncl = (int)min((int64_t)CB, ntot - c0);
   const int tid = threadIdx.x;
   for (int e = tid; e < CB * nb; e += blockDim.x) {
     const int r = e % CB, l = e / CB;
-    Pr[r][l] = (r < nr) ? A[(r0 + r) + n * (k0 + l)] : 0.0;
-    Pc[r][l] = (r < ncl) ? A[(c0 + r) + n * (k0 + l)] : 0.0;
+    Pr[r][l] = (r < nr) ? A[(r0 + r) + lda * (k0 + l)] : 0.0;
+    Pc[r][l] = (r < ncl) ? A[(c0 + r) + lda * (k0 + l)] : 0.0;
   }
   __syncthreads();
   const int tx = tid & 15, ty = tid >> 4;
@@ -237,101 +243,46 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
   }
   for (int x = 0; x < 4; x++)
     for (int y = 0; y < 4; y++) {
-      const int r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
-      if (r < n && c < n && r >= c) A[r + n * c] -= acc[x][y];
+      const int64_t r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
+      if (r < ntot && c < ntot && r >= c) A[r + lda * c] -= acc[x][y];
     }
 }
 
-void cholesky(double* A, int64_t n, double* invblk, int* info, hipStream_t st) {
+void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, double* invblk, int* info, hipStream_t st) {
   for (int k0 = 0; k0 < n; k0 += CB) {
     const int nb = (int)std::min<int64_t>(CB, n - k0);
     double* inv = invblk + (size_t)(k0 / CB) * CB * CB;
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, A, n, k0, nb, inv, info);
-    const int64_t m = n - k0 - nb;
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(64), 0, st, A, lda, k0, nb, inv, info);
+    const int64_t m = ntot - k0 - nb;
     if (m > 0) {
-      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), 0, st, A, n, k0, nb,
-                         inv);
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), 0, st, A, lda, ntot, k0,
+                         nb, inv);
       const int64_t t = (m + CB - 1) / CB;
-      hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(t * (t + 1) / 2)), dim3(256), 0, st, A, n, k0, nb);
+      hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(t * (t + 1) / 2)), dim3(256), 0, st, A, lda, ntot, k0,
+                         nb);
     }
   }
+}
+
+// df[P + n I] = A[(n + I) + lda P]   (B from the border rows)
+__global__ void border_to_b_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int ndens,
+                                   double* __restrict__ B) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ndens) return;
+  const int64_t P = e % n, I = e / n;
+  B[P + n * I] = A[(n + I) + lda * P];
+}
+
+void border_to_b(const double* A, int64_t lda, int64_t n, int ndens, double* B, hipStream_t st) {
+  const int64_t tot = n * ndens;
+  if (tot > 0)
+    hipLaunchKernelGGL(border_to_b_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, lda, n, ndens, B);
 }
 
 // ---------------------------------------------------------------------------
-// Forward solve L B = d in place (d column-major n x nrhs, ldd = n).
-__global__ __launch_bounds__(256) void trsm_diag_kernel(const double* __restrict__ inv, double* __restrict__ d,
-                                                        int64_t n, int k0, int nb, int nrhs) {
-  __shared__ double Iv[CB][CB + 1];
-  __shared__ double X[CB][CB + 1];
-  const int c0 = blockIdx.x * CB;
-  const int nc = min(CB, nrhs - c0);
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    Iv[r][c] = inv[r + CB * c];
-  }
-  for (int e = tid; e < nb * nc; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    X[r][c] = d[(k0 + r) + n * (c0 + c)];
-  }
-  __syncthreads();
-  for (int e = tid; e < nb * nc; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    double s = 0.0;
-    for (int l = 0; l <= r; l++) s += Iv[r][l] * X[l][c];
-    d[(k0 + r) + n * (c0 + c)] = s;
-  }
-}
-
-__global__ __launch_bounds__(256) void trsm_update_kernel(const double* __restrict__ A, double* __restrict__ d,
-                                                          int64_t n, int k0, int nb, int nrhs) {
-  __shared__ double Lb[CB][CB + 1];
-  __shared__ double Xb[CB][CB + 1];
-  const int r0 = k0 + nb + blockIdx.x * CB, c0 = blockIdx.y * CB;
-  const int nr = min(CB, (int)(n - r0)), nc = min(CB, nrhs - c0);
-  const int tid = threadIdx.x;
-  for (int e = tid; e < CB * nb; e += blockDim.x) {
-    const int r = e % CB, l = e / CB;
-    Lb[r][l] = (r < nr) ? A[(r0 + r) + n * (k0 + l)] : 0.0;
-    Xb[l][r] = (r < nc) ? d[(k0 + l) + n * (c0 + r)] : 0.0;
-  }
-  __syncthreads();
-  const int tx = tid & 15, ty = tid >> 4;
-  double acc[4][4] = {};
-  for (int l = 0; l < nb; l++) {
-    double a[4], b[4];
-#pragma unroll
-    for (int z = 0; z < 4; z++) {
-      a[z] = Lb[ty * 4 + z][l];
-      b[z] = Xb[l][tx * 4 + z];
-    }
-#pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-      for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
-  }
-  for (int x = 0; x < 4; x++)
-    for (int y = 0; y < 4; y++) {
-      const int r = ty * 4 + x, c = tx * 4 + y;
-      if (r < nr && c < nc) d[(r0 + r) + n * (c0 + c)] -= acc[x][y];
-    }
-}
-
-void forward_solve(const double* A, const double* invblk, double* d, int64_t n, int nrhs, hipStream_t st) {
-  for (int k0 = 0; k0 < n; k0 += CB) {
-    const int nb = (int)std::min<int64_t>(CB, n - k0);
-    const double* inv = invblk + (size_t)(k0 / CB) * CB * CB;
-    hipLaunchKernelGGL(trsm_diag_kernel, dim3((nrhs + CB - 1) / CB), dim3(256), 0, st, inv, d, n, k0, nb, nrhs);
-    const int64_t m = n - k0 - nb;
-    if (m > 0)
-      hipLaunchKernelGGL(trsm_update_kernel, dim3((unsigned)((m + CB - 1) / CB), (nrhs + CB - 1) / CB), dim3(256),
-                         0, st, A, d, n, k0, nb, nrhs);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// J assembly: for packed density pairs I2 <= I1, J = B(:,I1) . B(:,I2), written
-// to the 8 index permutations of the column-major nstates^4 tensor.
+// J assembly: the trailing block of the bordered factorization holds
+// G(I1, I2) = -B(:,I1) . B(:,I2) for I2 <= I1; J is written to the 8 index
+// permutations of the column-major nstates^4 tensor.
 __device__ inline void unpack_pair(int q, int& i, int& j) {
   int a = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
   while ((a + 1) * (a + 2) / 2 <= q) a++;
@@ -340,42 +291,32 @@ __device__ inline void unpack_pair(int q, int& i, int& j) {
   j = q - a * (a + 1) / 2;
 }
 
-__global__ __launch_bounds__(256) void assemble_j_kernel(const double* __restrict__ B, int64_t naux, int nstates,
-                                                         double* __restrict__ J) {
-  __shared__ double red[256];
+__global__ void assemble_j_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int npairs, int nstates,
+                                  double* __restrict__ J) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npairs) return;
   int I1, I2;
-  unpack_pair(blockIdx.x, I1, I2);
-  const double* b1 = B + naux * I1;
-  const double* b2 = B + naux * I2;
-  double s = 0.0;
-  for (int64_t m = threadIdx.x; m < naux; m += 256) s = fma(b1[m], b2[m], s);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    int i, j, k, l;
-    unpack_pair(I1, i, j);
-    unpack_pair(I2, k, l);
-    const int64_t ns = nstates;
-    auto at = [&](int a, int b, int c, int d) -> double& { return J[a + ns * (b + ns * (c + ns * d))]; };
-    const double v = red[0];
-    at(i, j, k, l) = v;
-    at(j, i, k, l) = v;
-    at(i, j, l, k) = v;
-    at(j, i, l, k) = v;
-    at(k, l, i, j) = v;
-    at(k, l, j, i) = v;
-    at(l, k, i, j) = v;
-    at(l, k, j, i) = v;
-  }
+  unpack_pair(q, I1, I2);
+  const double v = -A[(n + I1) + lda * (n + I2)];
+  int i, j, k, l;
+  unpack_pair(I1, i, j);
+  unpack_pair(I2, k, l);
+  const int64_t ns = nstates;
+  auto at = [&](int a, int b, int c, int d) -> double& { return J[a + ns * (b + ns * (c + ns * d))]; };
+  at(i, j, k, l) = v;
+  at(j, i, k, l) = v;
+  at(i, j, l, k) = v;
+  at(j, i, l, k) = v;
+  at(k, l, i, j) = v;
+  at(k, l, j, i) = v;
+  at(l, k, i, j) = v;
+  at(l, k, j, i) = v;
 }
 
-void assemble_j(const double* B, int64_t naux, int ndens, int nstates, double* J, hipStream_t st) {
+void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st) {
   const int npairs = ndens * (ndens + 1) / 2;
-  if (npairs > 0) hipLaunchKernelGGL(assemble_j_kernel, dim3(npairs), dim3(256), 0, st, B, naux, nstates, J);
+  if (npairs > 0)
+    hipLaunchKernelGGL(assemble_j_kernel, dim3((npairs + 127) / 128), dim3(128), 0, st, A, lda, n, npairs, nstates, J);
 }
 
 // ---------------------------------------------------------------------------

@@ -14,6 +14,16 @@ struct Reg {
 };
 }  // namespace
 
+template <int LA, int LB>
+struct RegC {
+  static void run(Schwarz4CachedTable& t) {
+    t[LA][LB] = make_schwarz4_cached<LA, LB>();
+    if constexpr (LB > 0) RegC<LA, LB - 1>::run(t);
+    else if constexpr (LA > 0) RegC<LA - 1, LA - 1>::run(t);
+  }
+};
+
 void register_schwarz4c(Schwarz4Table& t) { Reg<LMAX_ORB, LMAX_ORB>::run(t); }
+void register_schwarz4c_cached(Schwarz4CachedTable& t) { RegC<LMAX_ORB, LMAX_ORB>::run(t); }
 
 }  // namespace dfhip
