@@ -117,6 +117,7 @@ typedef struct {
     double ms_upload, ms_schwarz, ms_metric, ms_cholesky, ms_int3c2e, ms_pack, ms_contract,
         ms_solve, ms_assemble, ms_download, ms_total;
     double ms_h2d_densities; /* part of ms_pack spent copying host densities */
+    double ms_host_plan;     /* host time: pair sorting, screening prefixes, tables */
     /* screened-in triplets and primitive triplets per class, index
        (la*(la+1)/2 + lb) * 6 + lc with la >= lb the pair's angular momenta */
     int64_t class_triplets[15 * 6];
@@ -138,6 +139,10 @@ int dfhip_contract_coulomb_densities(const dfhip_basis *bas, const dfhip_basis *
  * small systems. */
 int dfhip_int3c2e_dense(const dfhip_basis *bas, const dfhip_basis *aux, const dfhip_options *opt,
                         double *out, dfhip_stats *stats);
+
+/* Frees the device workspace the library keeps between calls (staging
+ * matrix, bordered metric, packed densities); the next call re-allocates. */
+int dfhip_release_workspace(void);
 
 /* Metric (P|Q), naux x naux (symmetric). */
 int dfhip_metric(const dfhip_basis *aux, double *out);

@@ -172,7 +172,17 @@ struct Ctx {
   Schwarz4Table s4{};
   Schwarz4CachedTable s4c{};
   Gen3Table gen3{};
+  // device workspace reused across calls (grown on demand, freed by
+  // dfhip_release_workspace)
+  DBuf<double> wsT, wsA, wsDp, wsPart, wsDraw;
 };
+
+template <class T>
+T* reserve(DBuf<T>& b, size_t n) {
+  n = std::max<size_t>(n, 1);
+  if (b.n < n) b.alloc(n);
+  return b.p;
+}
 
 Ctx& ctx() {
   static Ctx* c = new Ctx();  // lives until process end
@@ -564,11 +574,14 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   sorted.cls_begin = pl.ps.cls_begin;
   sorted.cls_end = pl.ps.cls_end;
   std::vector<double> q2;
+  std::vector<std::pair<double, int>> key;
   for (int k = 0; k < NPCLS; k++) {
-    std::vector<int> idx;
-    for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) idx.push_back(p);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return pl.qab[a] > pl.qab[b]; });
-    for (int p : idx) {
+    // descending Q_ab, ties in the original (packed) order
+    key.clear();
+    for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) key.emplace_back(-pl.qab[p], p);
+    std::sort(key.begin(), key.end());
+    for (const auto& kv : key) {
+      const int p = kv.second;
       sorted.s1.push_back(pl.ps.s1[p]);
       sorted.s2.push_back(pl.ps.s2[p]);
       sorted.packed.push_back(pl.ps.packed[p]);
@@ -592,34 +605,35 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   int64_t col = 0;
   for (int L = 0; L <= LMAX_AUX; L++) {
     auto& v = pl.alist[L];
-    std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return pl.qc[a] > pl.qc[b]; });
+    key.clear();
+    for (int sh : v) key.emplace_back(-pl.qc[sh], sh);
+    std::sort(key.begin(), key.end());
+    for (size_t k = 0; k < v.size(); k++) v[k] = key[k].second;
     for (int sh : v) {
       pl.acol[L].push_back(col);
       for (int m = 0; m < 2 * L + 1; m++) pl.orig[col + m] = x.f0[sh] + m;
       col += 2 * L + 1;
     }
   }
-  // screened prefix per (L, pair): skip when Q_ab * Q_C < thresh
+  // screened prefix per (L, pair): skip when Q_ab * Q_C < thresh.  Inside a
+  // pair class Q_ab is non-increasing, so the prefix length is too and one
+  // downward sweep per (class, L) finds all of them.
   pl.nsurv.assign(LMAX_AUX + 1, std::vector<int>(pl.ps.n(), 0));
   pl.ncomputed = 0;
   for (int L = 0; L <= LMAX_AUX; L++) {
     const auto& v = pl.alist[L];
-    for (int p = 0; p < pl.ps.n(); p++) {
-      int n;
-      if (opt.screening == DFHIP_SCREEN_NONE) {
-        n = (int)v.size();
-      } else {
-        const double qa = pl.qab[p];
-        int lo = 0, hi = (int)v.size();  // first index with qa*qc < thresh
-        while (lo < hi) {
-          const int mid = (lo + hi) / 2;
-          if (qa * pl.qc[v[mid]] < opt.thresh) hi = mid;
-          else lo = mid + 1;
+    const int nv = (int)v.size();
+    for (int k = 0; k < NPCLS; k++) {
+      int n = nv;
+      for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+        if (opt.screening != DFHIP_SCREEN_NONE) {
+          const double qa = pl.qab[p];
+          // the product is monotone in Q_C, so the kept set is a prefix
+          while (n > 0 && qa * pl.qc[v[n - 1]] < opt.thresh) n--;
         }
-        n = lo;
+        pl.nsurv[L][p] = n;
+        pl.ncomputed += n;
       }
-      pl.nsurv[L][p] = n;
-      pl.ncomputed += n;
     }
   }
   // per-class counts of screened triplets and primitive triplets
@@ -689,18 +703,21 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // bordered matrix [[M, .], [d^T, 0]] (column-major, lda = naux + ndens),
   // and Q_C from its diagonal blocks
   const int64_t ntot = naux + (need_dens ? ndens : 0), lda = ntot;
-  DBuf<double> A((size_t)ntot * ntot);
-  HC(hipMemsetAsync(A.p, 0, A.n * sizeof(double), st));
-  metric_dev(c, x, daux.view, A.p, lda);
+  double* Ap = reserve(c.wsA, (size_t)ntot * ntot);
+  HC(hipMemsetAsync(Ap, 0, (size_t)ntot * ntot * sizeof(double), st));
+  metric_dev(c, x, daux.view, Ap, lda);
   DBuf<double> dqc(x.n);
-  schwarz_aux_from_metric(A.p, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
+  schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
   pl.qc.resize(x.n);
   HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
   HC(hipStreamSynchronize(st));
   const int e3 = tm.mark();
   const int e4 = e3;
 
+  const auto hp0 = std::chrono::steady_clock::now();
   build_plan(pl, opt);
+  const double ms_plan =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
   if (opt.verbose) {
     std::printf("Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
     std::printf("Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
@@ -739,19 +756,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
 
   // densities: upload and pair-pack (diabatization/intor.f90:155-176 symmetry)
-  DBuf<double> Dp;
+  double* Dp = nullptr;
   double ms_h2d = 0.0;
   if (need_dens) {
-    DBuf<double> Draw;
     const double* Dsrc = dens;
     if (!opt.dens_on_device) {
+      const size_t nd = (size_t)ndens * nbfs * nbfs;
+      double* Draw = reserve(c.wsDraw, nd);
       const int h0 = tm.mark();
-      Draw.alloc((size_t)ndens * nbfs * nbfs);
-      HC(hipMemcpyAsync(Draw.p, dens, Draw.n * sizeof(double), hipMemcpyHostToDevice, st));
+      HC(hipMemcpyAsync(Draw, dens, nd * sizeof(double), hipMemcpyHostToDevice, st));
       const int h1 = tm.mark();
       HC(hipStreamSynchronize(st));
       ms_h2d = tm.ms(h0, h1);
-      Dsrc = Draw.p;
+      Dsrc = Draw;
     }
     int64_t sI, si, sj;
     if (opt.dens_layout == DFHIP_DENS_C) {
@@ -763,8 +780,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       si = ndens;
       sj = (int64_t)ndens * nbfs;
     }
-    Dp.alloc((size_t)pl.npf * ndens);
-    pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp.p, ndens, st);
+    Dp = reserve(c.wsDp, (size_t)pl.npf * ndens);
+    pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp, ndens, st);
     HC(hipGetLastError());
     HC(hipStreamSynchronize(st));
   }
@@ -794,20 +811,91 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   const int nblocks = (int)bounds.size() - 1;
   int64_t widest = 0;
   for (int b = 0; b < nblocks; b++) widest = std::max(widest, bounds[b + 1] - bounds[b]);
-  DBuf<double> T((size_t)std::max<int64_t>(pl.npf, 1) * std::max<int64_t>(widest, 1));
-  DBuf<double> part;
+  double* Tp = reserve(c.wsT, (size_t)std::max<int64_t>(pl.npf, 1) * std::max<int64_t>(widest, 1));
+  double* partp = nullptr;
   int nsplit = 1;
+  // screened-prefix mask of T (the 3c2e kernels write only screened-in blocks)
+  ScreenMask mk{};
+  DBuf<int> drowpair, dcolL, dcolia, dnsflat;
+  if (need_dens) {
+    std::vector<int> rowpair(std::max<int64_t>(pl.npf, 1)), colL(std::max<int64_t>(naux, 1)),
+        colia(std::max<int64_t>(naux, 1));
+    for (int p = 0; p < pl.ps.n(); p++) {
+      const int64_t r1 = (p + 1 < pl.ps.n()) ? pl.prow[p + 1] : pl.npf;
+      for (int64_t r = pl.prow[p]; r < r1; r++) rowpair[r] = p;
+    }
+    for (int L = 0; L <= LMAX_AUX; L++)
+      for (size_t k = 0; k < pl.acol[L].size(); k++)
+        for (int m = 0; m < 2 * L + 1; m++) {
+          colL[pl.acol[L][k] + m] = L;
+          colia[pl.acol[L][k] + m] = (int)k;
+        }
+    std::vector<int> nsflat((size_t)(LMAX_AUX + 1) * pl.ps.n());
+    for (int L = 0; L <= LMAX_AUX; L++)
+      std::copy(pl.nsurv[L].begin(), pl.nsurv[L].end(), nsflat.begin() + (size_t)L * pl.ps.n());
+    upload(drowpair, rowpair, st);
+    upload(dcolL, colL, st);
+    upload(dcolia, colia, st);
+    upload(dnsflat, nsflat, st);
+    mk = ScreenMask{drowpair.p, dcolL.p, dcolia.p, dnsflat.p, pl.ps.n()};
+  }
   if (need_dens) {
     const int64_t tiles = ((widest + 63) / 64) * ((ndens + 63) / 64);
-    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(contract_nsplit_max(), 2048 / std::max<int64_t>(tiles, 1)));
+    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(512, 2048 / std::max<int64_t>(tiles, 1)));
     nsplit = (int)std::min<int64_t>(nsplit, std::max<int64_t>(1, (pl.npf + 15) / 16));
-    part.alloc((size_t)nsplit * widest * ndens);
+    partp = reserve(c.wsPart, (size_t)nsplit * widest * ndens);
   }
+  // Non-zero row segments of T per column tile: for each pair class, the
+  // pairs whose screened prefix reaches the tile form a prefix of the class
+  // (pairs are sorted by Q_ab and the prefix lengths are non-increasing).
+  const int GT = contract_tile();
+  auto tile_segments = [&](int64_t cA, int64_t w, std::vector<int64_t>& seg) {
+    const int ntile = (int)((w + GT - 1) / GT);
+    seg.assign((size_t)ntile * (2 * NPCLS + 1), 0);
+    for (int t = 0; t < ntile; t++) {
+      const int64_t c0 = cA + (int64_t)t * GT, c1 = std::min(cA + w, c0 + GT);
+      int64_t* sg = &seg[(size_t)t * (2 * NPCLS + 1)];
+      int64_t tot = 0;
+      for (int k = 0; k < NPCLS; k++) {
+        const int p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
+        int count = 0;
+        if (p1 > p0) {
+          for (int L = 0; L <= LMAX_AUX; L++) {
+            const auto& cols = pl.acol[L];
+            if (cols.empty()) continue;
+            // first list entry whose columns reach past c0, and whether it starts before c1
+            int ia = (int)(std::upper_bound(cols.begin(), cols.end(), c0) - cols.begin()) - 1;
+            if (ia < 0) ia = 0;
+            if (cols[ia] + 2 * L + 1 <= c0) ia++;
+            if (ia >= (int)cols.size() || cols[ia] >= c1) continue;
+            // pairs with nsurv[L][p] > ia, a prefix of the class
+            const auto& ns = pl.nsurv[L];
+            int lo = p0, hi = p1;
+            while (lo < hi) {
+              const int mid = (lo + hi) / 2;
+              if (ns[mid] > ia) lo = mid + 1;
+              else hi = mid;
+            }
+            count = std::max(count, lo - p0);
+          }
+        }
+        const int64_t rlo = p1 > p0 ? pl.prow[p0] : 0;
+        const int64_t rhi = count > 0 ? (p0 + count < pl.ps.n() ? pl.prow[p0 + count] : pl.npf) : rlo;
+        sg[1 + 2 * k] = rlo;
+        sg[2 + 2 * k] = rhi;
+        tot += rhi - rlo;
+      }
+      sg[0] = tot;
+    }
+  };
+  std::vector<int64_t> hseg;
+  DBuf<int64_t> dseg;
   double ms_int = 0.0, ms_con = 0.0;
   std::vector<double> dense_stage;
   for (int blk = 0; blk < nblocks; blk++) {
     const int64_t cA = bounds[blk], cB = bounds[blk + 1], w = cB - cA;
-    HC(hipMemsetAsync(T.p, 0, (size_t)pl.npf * w * sizeof(double), st));
+    // only the dense dump reads screened-out entries; the contraction masks them
+    if (out.dense3c) HC(hipMemsetAsync(Tp, 0, (size_t)pl.npf * w * sizeof(double), st));
     const int b0 = tm.mark();
     for (int L = 0; L <= LMAX_AUX; L++) {
       const auto& cols = pl.acol[L];
@@ -822,9 +910,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
           PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dnsurv[L].p + p0, np};
           AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi, kmax[L]};
           if (gen && gen3_has(la, lb, L)) {
-            launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, T.p - cA, w);
+            launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
           } else {
-            c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, T.p - cA, w, st);
+            c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, Tp - cA, w, st);
             HC(hipGetLastError());
           }
         }
@@ -832,13 +920,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     const int b1 = tm.mark();
     if (need_dens) {
       // d^T into the border rows: A[(naux + I) + lda * P]
-      contract_T(T.p, w, Dp.p, ndens, pl.npf, (int)w, ndens, part.p, nsplit, dorig.p + cA, A.p + naux, lda, 1, st);
+      tile_segments(cA, w, hseg);
+      upload(dseg, hseg, st);
+      ScreenMask mb = mk;
+      mb.colL = dcolL.p + cA;
+      mb.colia = dcolia.p + cA;
+      contract_T(Tp, w, Dp, ndens, (int)w, ndens, NPCLS, dseg.p, mb, partp, nsplit, dorig.p + cA, Ap + naux, lda, 1,
+                 st);
       HC(hipGetLastError());
     }
     const int b2 = tm.mark();
     if (out.dense3c) {
       dense_stage.resize((size_t)pl.npf * w);
-      HC(hipMemcpyAsync(dense_stage.data(), T.p, dense_stage.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      HC(hipMemcpyAsync(dense_stage.data(), Tp, dense_stage.size() * sizeof(double), hipMemcpyDeviceToHost, st));
       HC(hipStreamSynchronize(st));
       const int64_t nbf = o.nbf;
       for (int p = 0; p < pl.ps.n(); p++) {
@@ -871,7 +965,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     DBuf<double> invblk((size_t)nblk * CHOL_BLOCK * CHOL_BLOCK);
     DBuf<int> info(1);
     HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
-    cholesky_bordered(A.p, lda, naux, ntot, invblk.p, info.p, st);
+    cholesky_bordered(Ap, lda, naux, ntot, invblk.p, info.p, st);
     HC(hipGetLastError());
     e7 = tm.mark();
     HC(hipStreamSynchronize(st));
@@ -882,14 +976,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
                                          std::to_string(hinfo) + ")"};
     if (out.df) {
       dB.alloc((size_t)naux * ndens);
-      border_to_b(A.p, lda, naux, ndens, dB.p, st);
+      border_to_b(Ap, lda, naux, ndens, dB.p, st);
       HC(hipGetLastError());
     }
     if (out.J) {
       const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;
       dJ.alloc(nj);
       HC(hipMemsetAsync(dJ.p, 0, nj * sizeof(double), st));
-      assemble_j(A.p, lda, naux, ndens, out.nstates, dJ.p, st);
+      assemble_j(Ap, lda, naux, ndens, out.nstates, dJ.p, st);
       HC(hipGetLastError());
     }
     e8 = tm.mark();
@@ -921,6 +1015,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     stats->ms_download = tm.ms(e8, e9);
     stats->ms_total = tm.ms(e0, e9);
     stats->ms_h2d_densities = ms_h2d;
+    stats->ms_host_plan = ms_plan;
     for (int k = 0; k < NPCLS * 6; k++) {
       stats->class_triplets[k] = pl.class_trip[k];
       stats->class_prim_triplets[k] = pl.class_prim[k];
@@ -1009,6 +1104,18 @@ void f_init(void) {
 }
 
 int dfhip_last_status(void) { return t_status; }
+
+int dfhip_release_workspace(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lock(ctx_mutex());
+    Ctx& c = ctx();
+    c.wsT.release();
+    c.wsA.release();
+    c.wsDp.release();
+    c.wsPart.release();
+    c.wsDraw.release();
+  });
+}
 const char* dfhip_last_error(void) { return t_msg.c_str(); }
 
 void dfhip_default_options(dfhip_options* opt) {

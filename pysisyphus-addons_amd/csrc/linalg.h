@@ -11,9 +11,21 @@ namespace dfhip {
 void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, const int* L, const int* f0,
                   const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
                   hipStream_t st);
-void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int64_t npf, int ncol, int ndens,
-                double* part, int nsplit, const int* orig, double* d, int64_t sP, int64_t sI, hipStream_t st);
-int contract_nsplit_max();
+// Screened-prefix mask of the staging matrix T: entry (row k, column P) is
+// live iff colia[P] < nsurv[colL[P] * npairs + rowpair[k]] (column arrays
+// relative to the column block).
+struct ScreenMask {
+  const int* rowpair;
+  const int* colL;
+  const int* colia;
+  const int* nsurv;
+  int npairs;
+};
+
+void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int ncol, int ndens, int nseg,
+                const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
+                int64_t sP, int64_t sI, hipStream_t st);
+int contract_tile();
 void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, double* invblk, int* info, hipStream_t st);
 void border_to_b(const double* A, int64_t lda, int64_t n, int ndens, double* B, hipStream_t st);
 void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st);

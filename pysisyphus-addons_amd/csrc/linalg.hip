@@ -47,42 +47,84 @@ void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, 
 // ---------------------------------------------------------------------------
 // part[s][P][I] = sum_{k in split s} T[k][P] * Dp[k][I]
 // T: npf x ncol (row-major, ldt), Dp: npf x ndens (row-major, ldd).
+// Only rows that can be non-zero for a column tile are visited: the rows of
+// the tile are the concatenation of nseg row segments [seg_lo, seg_hi) per
+// column tile (host-computed from the screened prefixes), split evenly over
+// the split index s.
 // 64x64 output tile per workgroup, 256 threads with 4x4 each, K step 16.
 constexpr int GT = 64, GK = 16;
 
 __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __restrict__ T, int64_t ldt,
-                                                             const double* __restrict__ Dp, int ldd, int64_t npf,
-                                                             int ncol, int ndens, int64_t kchunk,
+                                                             const double* __restrict__ Dp, int ldd, int ncol,
+                                                             int ndens, int nseg, const int64_t* __restrict__ seg,
+                                                             int nsplit, const ScreenMask mk,
                                                              double* __restrict__ part) {
   __shared__ double sT[GK][GT + 1];
   __shared__ double sD[GK][GT + 1];
+  const int tcol = blockIdx.x;
   const int tp = blockIdx.x * GT, ti = blockIdx.y * GT, s = blockIdx.z;
-  const int64_t k0 = (int64_t)s * kchunk, k1 = min(npf, k0 + kchunk);
+  const int64_t* sg = seg + (size_t)tcol * (2 * nseg + 1);  // [total, lo0, hi0, lo1, hi1, ...]
+  const int64_t tot = sg[0];
+  const int64_t chunk = ((tot + nsplit - 1) / nsplit + GK - 1) / GK * GK;
+  int64_t q0 = (int64_t)s * chunk, q1 = min(tot, q0 + chunk);
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;  // outputs P = tp + ty*4 + a, I = ti + tx*4 + b
+  // this lane always loads column tp + (tid % GT) of T: its screened prefix test
+  const int mcol = tp + (tid % GT);
+  const bool colok = mcol < ncol;
+  const int* __restrict__ ns_col = colok ? mk.nsurv + (size_t)mk.colL[mcol] * mk.npairs : mk.nsurv;
+  const int ia_col = colok ? mk.colia[mcol] : 0;
   double acc[4][4] = {};
-  for (int64_t kb = k0; kb < k1; kb += GK) {
-    for (int e = tid; e < GK * GT; e += 256) {
-      const int kk = e / GT, c = e - kk * GT;
-      const int64_t k = kb + kk;
-      sT[kk][c] = (k < k1 && tp + c < ncol) ? T[k * ldt + tp + c] : 0.0;
-      sD[kk][c] = (k < k1 && ti + c < ndens) ? Dp[k * ldd + ti + c] : 0.0;
+  // walk the segments overlapping [q0, q1) of the concatenated row range
+  int64_t base = 0;
+  for (int g = 0; g < nseg && q0 < q1; g++) {
+    const int64_t lo = sg[1 + 2 * g], hi = sg[2 + 2 * g], len = hi - lo;
+    if (q0 >= base + len) {
+      base += len;
+      continue;
     }
-    __syncthreads();
+    const int64_t a0 = lo + (q0 - base), a1 = lo + min(len, q1 - base);
+    for (int64_t kb = a0; kb < a1; kb += GK) {
+      // unconditional loads from clamped (in-bounds) addresses, then selects
+      const int mcolc = colok ? mcol : ncol - 1;
+      constexpr int NZ = GK * GT / 256;
+      double tv[NZ], dv[NZ];
+      int ns[NZ];
 #pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      double a[4], b[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        a[q] = sT[kk][ty * 4 + q];
-        b[q] = sD[kk][tx * 4 + q];
+      for (int z = 0; z < NZ; z++) {
+        const int e = tid + 256 * z, kk = e / GT, c = e - kk * GT;
+        const int64_t kc = min(kb + kk, a1 - 1);
+        tv[z] = T[kc * ldt + mcolc];
+        dv[z] = Dp[kc * ldd + min(ti + c, ndens - 1)];
+        ns[z] = ns_col[mk.rowpair[kc]];
       }
 #pragma unroll
-      for (int x = 0; x < 4; x++)
+      for (int z = 0; z < NZ; z++) {
+        const int e = tid + 256 * z, kk = e / GT, c = e - kk * GT;
+        const int64_t k = kb + kk;
+        // entries of screened-out triplets are never written: mask them here
+        const bool live = k < a1 && colok && ia_col < ns[z];
+        sT[kk][c] = live ? tv[z] : 0.0;
+        sD[kk][c] = (k < a1 && ti + c < ndens) ? dv[z] : 0.0;
+      }
+      __syncthreads();
 #pragma unroll
-        for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+      for (int kk = 0; kk < GK; kk++) {
+        double a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          a[q] = sT[kk][ty * 4 + q];
+          b[q] = sD[kk][tx * 4 + q];
+        }
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+          for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    q0 = base + len;
+    base += len;
   }
   for (int x = 0; x < 4; x++)
     for (int y = 0; y < 4; y++) {
@@ -103,21 +145,20 @@ __global__ void reduce_unpermute_kernel(const double* __restrict__ part, int nsp
   d[orig[P] * sP + I * sI] += s;
 }
 
-void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int64_t npf, int ncol, int ndens,
-                double* part, int nsplit, const int* orig, double* d, int64_t sP, int64_t sI, hipStream_t st) {
-  if (npf <= 0 || ncol <= 0) return;
-  int64_t kchunk = (npf + nsplit - 1) / nsplit;
-  kchunk = (kchunk + GK - 1) / GK * GK;
-  const int ns = (int)((npf + kchunk - 1) / kchunk);
-  dim3 grid((ncol + GT - 1) / GT, (ndens + GT - 1) / GT, ns);
-  hipLaunchKernelGGL(gemm_tn_splitk_kernel, grid, dim3(256), 0, st, T, ldt, Dp, ldd, npf, ncol, ndens, kchunk,
-                     part);
+void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int ncol, int ndens, int nseg,
+                const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
+                int64_t sP, int64_t sI, hipStream_t st) {
+  if (ncol <= 0) return;
+  dim3 grid((ncol + GT - 1) / GT, (ndens + GT - 1) / GT, nsplit);
+  hipLaunchKernelGGL(gemm_tn_splitk_kernel, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg, nsplit,
+                     mk, part);
   const int64_t n = (int64_t)ncol * ndens;
-  hipLaunchKernelGGL(reduce_unpermute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, ns, ncol,
+  hipLaunchKernelGGL(reduce_unpermute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, nsplit, ncol,
                      ndens, orig, d, sP, sI);
 }
 
-int contract_nsplit_max() { return 512; }
+int contract_tile() { return GT; }
+
 
 // ---------------------------------------------------------------------------
 // Blocked right-looking Cholesky of the bordered matrix
@@ -132,53 +173,66 @@ int contract_nsplit_max() { return 512; }
 constexpr int CB = 64;
 
 // Factor the diagonal block at k0 (left-looking, one wave, row r per lane)
-// and store L11 and inv(L11).
+// and store L11 and inv(L11).  Fully unrolled: each lane keeps its row of L
+// (and later its column of inv(L11)) in registers; the row of step j is read
+// from LDS as broadcast loads that are all independent.  Rows r >= nb of the
+// last, partial block are padded with the identity.
 __global__ __launch_bounds__(64) void chol_diag_kernel(double* __restrict__ A, int64_t lda, int k0, int nb,
                                                        double* __restrict__ inv, int* __restrict__ info) {
-  __shared__ double S[CB][CB + 1];
-  __shared__ double Iv[CB][CB + 1];
-  __shared__ double piv;
+  __shared__ double S[CB][CB + 1];  // rows of L as they are completed
+  __shared__ double dinv[CB];       // 1 / L[j][j]
   const int r = threadIdx.x;
-  for (int c = 0; c < nb; c++)
-    if (r < nb) S[r][c] = A[(k0 + r) + lda * (k0 + c)];
-  __syncthreads();
-  for (int j = 0; j < nb; j++) {
-    double t = 0.0;
-    if (r >= j && r < nb) {
-      t = S[r][j];
-      for (int k = 0; k < j; k++) t = fma(-S[r][k], S[j][k], t);
+  const int rr = min(r, nb - 1);
+  double a[CB];
+#pragma unroll
+  for (int c = 0; c < CB; c++) a[c] = A[(k0 + rr) + lda * (k0 + min(c, nb - 1))];
+#pragma unroll
+  for (int c = 0; c < CB; c++) a[c] = (r < nb && c < nb) ? a[c] : (r == c ? 1.0 : 0.0);
+  // left-looking: L[r][j] = (a[r][j] - sum_{k<j} L[r][k] L[j][k]) / L[j][j]
+#pragma unroll
+  for (int j = 0; j < CB; j++) {
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < j; k += 2) {
+      t0 = fma(a[k], S[j][k], t0);
+      if (k + 1 < j) t1 = fma(a[k + 1], S[j][k + 1], t1);
     }
+    const double t = a[j] - (t0 + t1);
     if (r == j) {
       double dj = t;
       if (!(dj > 0.0)) {
-        atomicCAS(info, 0, k0 + j + 1);
+        if (j < nb) atomicCAS(info, 0, k0 + j + 1);
         dj = 1.0;
       }
-      piv = sqrt(dj);
+      const double d = sqrt(dj);
+      dinv[j] = 1.0 / d;
+      a[j] = d;
     }
     __syncthreads();
-    const double d = piv;
-    if (r == j) S[j][j] = d;
-    else if (r > j && r < nb) S[r][j] = t / d;
+    if (r > j) a[j] = t * dinv[j];
+    if (r >= j) S[r][j] = a[j];
     __syncthreads();
   }
-  // column c of inv(L11) by forward substitution in lane c
+  // column r of inv(L11): iv[i] = -(sum_{k<i} L[i][k] iv[k]) / L[i][i], iv[k<r] = 0
+  double iv[CB];
+#pragma unroll
+  for (int i = 0; i < CB; i++) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < i; k += 2) {
+      s0 = fma(S[i][k], iv[k], s0);
+      if (k + 1 < i) s1 = fma(S[i][k + 1], iv[k + 1], s1);
+    }
+    iv[i] = (i < r) ? 0.0 : (i == r ? dinv[i] : -(s0 + s1) * dinv[i]);
+  }
   if (r < nb) {
-    const int c = r;
-    for (int i = 0; i < c; i++) Iv[i][c] = 0.0;
-    Iv[c][c] = 1.0 / S[c][c];
-    for (int i = c + 1; i < nb; i++) {
-      double s = 0.0;
-      for (int k = c; k < i; k++) s = fma(S[i][k], Iv[k][c], s);
-      Iv[i][c] = -s / S[i][i];
-    }
+#pragma unroll
+    for (int c = 0; c < CB; c++)
+      if (c < nb && r >= c) A[(k0 + r) + lda * (k0 + c)] = a[c];
   }
-  __syncthreads();
-  for (int c = 0; c < nb; c++)
-    if (r < nb) {
-      if (r >= c) A[(k0 + r) + lda * (k0 + c)] = S[r][c];
-      inv[r + CB * c] = (r >= c) ? Iv[r][c] : 0.0;
-    }
+  // inv is stored column-major: inv[i + CB * c]; lane r owns column r
+#pragma unroll
+  for (int i = 0; i < CB; i++) inv[i + CB * r] = (i < nb && r < nb) ? iv[i] : 0.0;
 }
 
 // L21 = A21 inv(L11)^T for 64 rows per workgroup (rows k0+nb .. ntot-1).
@@ -189,13 +243,20 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A,
   const int64_t r0 = k0 + nb + (int64_t)blockIdx.x * CB;
   const int nr = (int)min((int64_t)CB, ntot - r0);
   const int tid = threadIdx.x;
-  for (int e = tid; e < nb * nb; e += blockDim.x) {
-    const int r = e % nb, c = e / nb;
-    Iv[r][c] = inv[r + CB * c];
-  }
-  for (int e = tid; e < nr * nb; e += blockDim.x) {
-    const int r = e % nr, c = e / nr;
-    X[r][c] = A[(r0 + r) + lda * (k0 + c)];
+  {
+    double v[CB * CB / 256], w[CB * CB / 256];
+#pragma unroll
+    for (int z = 0; z < CB * CB / 256; z++) {
+      const int e = tid + 256 * z, r = e % CB, c = e / CB;
+      w[z] = inv[r + CB * c];
+      v[z] = A[(r0 + min(r, nr - 1)) + lda * (k0 + min(c, nb - 1))];
+    }
+#pragma unroll
+    for (int z = 0; z < CB * CB / 256; z++) {
+      const int e = tid + 256 * z, r = e % CB, c = e / CB;
+      Iv[r][c] = w[z];
+      X[r][c] = (r < nr && c < nb) ? v[z] : 0.0;
+    }
   }
   __syncthreads();
   for (int e = tid; e < nr * nb; e += blockDim.x) {
@@ -221,10 +282,21 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
   const int64_t r0 = m0 + (int64_t)tr * CB, c0 = m0 + (int64_t)tc * CB;
   const int nr = (int)min((int64_t)CB, ntot - r0), ncl = (int)min((int64_t)CB, ntot - c0);
   const int tid = threadIdx.x;
-  for (int e = tid; e < CB * nb; e += blockDim.x) {
-    const int r = e % CB, l = e / CB;
-    Pr[r][l] = (r < nr) ? A[(r0 + r) + lda * (k0 + l)] : 0.0;
-    Pc[r][l] = (r < ncl) ? A[(c0 + r) + lda * (k0 + l)] : 0.0;
+  {
+    double vr[CB * CB / 256], vc[CB * CB / 256];
+#pragma unroll
+    for (int z = 0; z < CB * CB / 256; z++) {
+      const int e = tid + 256 * z, r = e % CB, l = e / CB;
+      const int ll = min(l, nb - 1);
+      vr[z] = A[(r0 + min(r, nr - 1)) + lda * (k0 + ll)];
+      vc[z] = A[(c0 + min(r, ncl - 1)) + lda * (k0 + ll)];
+    }
+#pragma unroll
+    for (int z = 0; z < CB * CB / 256; z++) {
+      const int e = tid + 256 * z, r = e % CB, l = e / CB;
+      Pr[r][l] = (r < nr && l < nb) ? vr[z] : 0.0;
+      Pc[r][l] = (r < ncl && l < nb) ? vc[z] : 0.0;
+    }
   }
   __syncthreads();
   const int tx = tid & 15, ty = tid >> 4;
@@ -241,10 +313,20 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
 #pragma unroll
       for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
   }
+  double old[4][4];
+#pragma unroll
   for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int64_t r = min(r0 + ty * 4 + x, ntot - 1), c = min(c0 + tx * 4 + y, ntot - 1);
+      old[x][y] = A[r + lda * c];
+    }
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
     for (int y = 0; y < 4; y++) {
       const int64_t r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
-      if (r < ntot && c < ntot && r >= c) A[r + lda * c] -= acc[x][y];
+      if (r < ntot && c < ntot && r >= c) A[r + lda * c] = old[x][y] - acc[x][y];
     }
 }
 

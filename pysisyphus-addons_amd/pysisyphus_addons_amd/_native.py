@@ -54,7 +54,7 @@ class Stats(ctypes.Structure):
     ] + [(n, ctypes.c_double) for n in (
         "ms_upload", "ms_schwarz", "ms_metric", "ms_cholesky", "ms_int3c2e", "ms_pack",
         "ms_contract", "ms_solve", "ms_assemble", "ms_download", "ms_total",
-        "ms_h2d_densities")] + [
+        "ms_h2d_densities", "ms_host_plan")] + [
         ("class_triplets", ctypes.c_int64 * 90),
         ("class_prim_triplets", ctypes.c_int64 * 90),
     ]
@@ -75,7 +75,7 @@ EXPORTED = (
     "f_init", "f_contract_coulomb_densities_2d", "f_contract_coulomb_densities_4d",
     "f_int_schwarz_aux", "f_int_schwarz_bra", "dfhip_last_status", "dfhip_last_error",
     "dfhip_default_options", "dfhip_contract_coulomb_densities", "dfhip_int3c2e_dense",
-    "dfhip_metric", "dfhip_boys",
+    "dfhip_metric", "dfhip_boys", "dfhip_release_workspace",
 )
 
 
