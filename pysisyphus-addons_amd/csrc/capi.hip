@@ -25,6 +25,7 @@
 #include "int3c2e_impl.h"
 #include "linalg.h"
 #include "schwarz4c_impl.h"
+#include "schwarz4m_impl.h"
 
 namespace dfhip {
 
@@ -36,6 +37,7 @@ void register_int3c2e_l4(Int3Table&);
 void register_int3c2e_l5(Int3Table&);
 void register_schwarz4c(Schwarz4Table&);
 void register_schwarz4c_cached(Schwarz4CachedTable&);
+void register_schwarz4m(Schwarz4MTable&);
 
 namespace {
 
@@ -171,6 +173,7 @@ struct Ctx {
   Int3Table int3{};
   Schwarz4Table s4{};
   Schwarz4CachedTable s4c{};
+  Schwarz4MTable s4m{};
   Gen3Table gen3{};
   // device workspace reused across calls (grown on demand, freed by
   // dfhip_release_workspace)
@@ -218,6 +221,7 @@ void ensure_ctx() {
   register_int3c2e_l5(c.int3);
   register_schwarz4c(c.s4);
   register_schwarz4c_cached(c.s4c);
+  register_schwarz4m(c.s4m);
   register_gen3_all(c.gen3);
   for (int la = 0; la <= LMAX_AUX; la++)
     for (int lb = 0; lb <= (la <= LMAX_ORB ? la : 0); lb++)
@@ -380,17 +384,22 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
   upload(s1, ps.s1, st);
   upload(s2, ps.s2, st);
   std::vector<size_t> poff(NPCLS + 1, 0);
-  std::vector<int> nch(NPCLS, 1), cached(NPCLS, 0);
+  // kernel per class: 2 = matrix cores, 1 = cached VALU, 0 = chunked VALU
+  // (DFHIP_SCHWARZ_KERNEL caps the choice, for tests)
+  std::vector<int> nch(NPCLS, 1), kind(NPCLS, 0);
+  const char* cap_env = std::getenv("DFHIP_SCHWARZ_KERNEL");
+  const int cap = (cap_env && *cap_env) ? std::atoi(cap_env) : 2;
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb);
       int maxnpp = 0;
       for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++)
         maxnpp = std::max(maxnpp, o.nprim[ps.s1[p]] * o.nprim[ps.s2[p]]);
-      const char* force = std::getenv("DFHIP_SCHWARZ_CHUNKED");
-      cached[k] = !(force && *force && *force != '0') && c.s4c[la][lb].fn != nullptr &&
-                  maxnpp <= c.s4c[la][lb].nppc;
-      nch[k] = cached[k] ? 1 : c.s4[la][lb].nch;
+      if (cap >= 2 && c.s4m[la][lb].fn != nullptr && maxnpp <= c.s4m[la][lb].nppm)
+        kind[k] = 2;
+      else if (cap >= 1 && c.s4c[la][lb].fn != nullptr && maxnpp <= c.s4c[la][lb].nppc)
+        kind[k] = 1;
+      nch[k] = kind[k] ? 1 : c.s4[la][lb].nch;
       poff[k + 1] = (size_t)(ps.cls_end[k] - ps.cls_begin[k]) * nch[k];
     }
   for (int k = 0; k < NPCLS; k++) poff[k + 1] += poff[k];
@@ -399,7 +408,9 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
       if (np == 0) continue;
-      if (cached[k])
+      if (kind[k] == 2)
+        c.s4m[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+      else if (kind[k] == 1)
         c.s4c[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       else
         c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
@@ -566,6 +577,36 @@ struct Plan {
   int64_t class_trip[NPCLS * 6] = {}, class_prim[NPCLS * 6] = {};
 };
 
+// Stable LSD radix sort of (key, value) by ascending 64-bit key, 8 bits per
+// pass; passes whose digit is the same for every key are skipped.
+void radix_sort(std::vector<uint64_t>& key, std::vector<int>& val) {
+  const size_t n = key.size();
+  if (n < 2) return;
+  std::vector<uint64_t> k2(n);
+  std::vector<int> v2(n);
+  for (int shift = 0; shift < 64; shift += 8) {
+    size_t cnt[257] = {0};
+    for (size_t i = 0; i < n; i++) cnt[((key[i] >> shift) & 255) + 1]++;
+    if (cnt[((key[0] >> shift) & 255) + 1] == n) continue;
+    for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; i++) {
+      const size_t pos = cnt[(key[i] >> shift) & 255]++;
+      k2[pos] = key[i];
+      v2[pos] = val[i];
+    }
+    key.swap(k2);
+    val.swap(v2);
+  }
+}
+
+// Sort key of a non-negative finite double for descending order: the IEEE
+// bits of x >= 0 increase with x, so their complement decreases.
+inline uint64_t desc_key(double x) {
+  uint64_t b;
+  std::memcpy(&b, &x, sizeof b);
+  return ~b;
+}
+
 void build_plan(Plan& pl, const dfhip_options& opt) {
   const HostShells& o = pl.orb;
   const HostShells& x = pl.aux;
@@ -574,14 +615,22 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   sorted.cls_begin = pl.ps.cls_begin;
   sorted.cls_end = pl.ps.cls_end;
   std::vector<double> q2;
-  std::vector<std::pair<double, int>> key;
+  q2.reserve(pl.ps.n());
+  sorted.s1.reserve(pl.ps.n());
+  sorted.s2.reserve(pl.ps.n());
+  sorted.packed.reserve(pl.ps.n());
+  std::vector<uint64_t> key;
+  std::vector<int> idx;
   for (int k = 0; k < NPCLS; k++) {
     // descending Q_ab, ties in the original (packed) order
     key.clear();
-    for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) key.emplace_back(-pl.qab[p], p);
-    std::sort(key.begin(), key.end());
-    for (const auto& kv : key) {
-      const int p = kv.second;
+    idx.clear();
+    for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+      key.push_back(desc_key(pl.qab[p]));
+      idx.push_back(p);
+    }
+    radix_sort(key, idx);
+    for (const int p : idx) {
       sorted.s1.push_back(pl.ps.s1[p]);
       sorted.s2.push_back(pl.ps.s2[p]);
       sorted.packed.push_back(pl.ps.packed[p]);
@@ -606,9 +655,8 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   for (int L = 0; L <= LMAX_AUX; L++) {
     auto& v = pl.alist[L];
     key.clear();
-    for (int sh : v) key.emplace_back(-pl.qc[sh], sh);
-    std::sort(key.begin(), key.end());
-    for (size_t k = 0; k < v.size(); k++) v[k] = key[k].second;
+    for (int sh : v) key.push_back(desc_key(pl.qc[sh]));
+    radix_sort(key, v);
     for (int sh : v) {
       pl.acol[L].push_back(col);
       for (int m = 0; m < 2 * L + 1; m++) pl.orig[col + m] = x.f0[sh] + m;
@@ -711,6 +759,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.qc.resize(x.n);
   HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
   HC(hipStreamSynchronize(st));
+  for (const double q : pl.qc)
+    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
   const int e3 = tm.mark();
   const int e4 = e3;
 

@@ -141,6 +141,33 @@ def test_schwarz_bra_g_shells():
     np.testing.assert_allclose(qb, O.schwarz_bra(O.Basis.from_shells(shells)), rtol=1e-12)
 
 
+def _schwarz_systems():
+    rng = np.random.default_rng(21)
+    mixed_npp = []  # 1-, 2- and 3-primitive shells up to g in one class each
+    for ci, C in enumerate(rng.uniform(-1.2, 1.2, (3, 3))):
+        for L in (0, 1, 2, 3, 4):
+            k = 1 + (ci + L) % 3
+            e = rng.uniform(0.2, 0.6) * 3.0 ** np.arange(k)
+            mixed_npp.append(Shell(L=L, center=C, exps=e, coeffs=rng.uniform(0.2, 1.0, k), center_ind=ci))
+    wide = [Shell(L=L, center=rng.uniform(-1, 1, 3), exps=[0.3, 0.9, 2.7, 8.1], coeffs=[0.4, 0.5, 0.6, 0.7])
+            for L in (0, 1, 2)]  # 16 primitive pairs: above the matrix-core kernel's limit
+    return [("mixed", _mixed_system()[0]), ("workload0", synthetic.workload(0)[0]),
+            ("mixed_npp", Shells(mixed_npp)), ("wide", Shells(wide))]
+
+
+@pytest.mark.parametrize("kernel", [2, 1, 0])
+def test_schwarz_kernel_variants(kernel, monkeypatch):
+    """Matrix-core (2), cached (1) and chunked (0) Schwarz kernels against the
+    oracle's explicit (ab|ab) blocks."""
+    monkeypatch.setenv("DFHIP_SCHWARZ_KERNEL", str(kernel))
+    for name, shells in _schwarz_systems():
+        qb = sintor.int_schwarz_bra(shells)
+        ref = O.schwarz_bra(O.Basis.from_shells(shells))
+        rel = np.abs(qb - ref) / np.maximum(ref, 1e-300)
+        _say(f"schwarz kernel {kernel} {name}: max rel err {rel.max():.2e}")
+        np.testing.assert_allclose(qb, ref, rtol=1e-12)
+
+
 def test_workload0_df_and_j_vs_oracle():
     shells, aux, dens = synthetic.workload(0)
     _say(f"workload 0: nbf {shells.nbf} naux {aux.nbf} ndens {dens.shape[0]}")
