@@ -92,13 +92,15 @@ typedef struct {
 } dfhip_basis;
 
 enum { DFHIP_FP64 = 0, DFHIP_MIXED = 1 };
-enum { DFHIP_SCREEN_SCHWARZ = 0, DFHIP_SCREEN_NONE = 1 };
+enum { DFHIP_SCREEN_SCHWARZ = 0, DFHIP_SCREEN_NONE = 1, DFHIP_SCREEN_QVL = 2 };
 enum { DFHIP_DENS_FORTRAN = 0, DFHIP_DENS_C = 1 };
 
 typedef struct {
     int32_t precision;   /* DFHIP_FP64 */
-    int32_t screening;   /* DFHIP_SCREEN_SCHWARZ (reference) or _NONE */
-    double thresh;       /* skip when Q_ab Q_C < thresh; default (double)1e-10f */
+    int32_t screening;   /* DFHIP_SCREEN_SCHWARZ (reference), _NONE, or _QVL: the
+                            screener's qvl_estimate (common/screening.f90:267-299)
+                            with thresh_ws = 1e-4 and aux extents at 1e-8 */
+    double thresh;       /* skip when the estimate is < thresh; default (double)1e-10f */
     int32_t dens_layout; /* DFHIP_DENS_FORTRAN: (ndens,nbf,nbf) column-major,
                             DFHIP_DENS_C: (ndens,nbf,nbf) row-major */
     int32_t verbose;     /* print sizes, phase times and screening counts */

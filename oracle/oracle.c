@@ -641,13 +641,30 @@ void orc_metric(void *aux, double *M)
 /* diabatization/intor.f90:108-183                                           */
 /* ------------------------------------------------------------------------ */
 
+/* common/screening.f90:267-299, the QVL estimate of one triplet in the
+ * reference's operation order.  pr = pair record [centre x, y, z, extent,
+ * (min a + min b)^(1/4), Q_ab], xr = aux record [extent, beta, Q_C], C = aux
+ * centre, L = aux angular momentum.  R^(L+1) is a chain of products. */
+double orc_qvl_estimate(const double *pr, const double *xr, const double *C, int L)
+{
+    double dx = pr[0] - C[0], dy = pr[1] - C[1], dz = pr[2] - C[2];
+    double R = sqrt(dx * dx + dy * dy + dz * dz), Rp = R;
+    int i;
+    if (R <= pr[3] + xr[0]) return pr[5] * xr[2];
+    for (i = 0; i < L; i++) Rp = Rp * R;
+    return pr[5] * M_PI * sqrt(2.0) * xr[1] / (pr[4] * Rp);
+}
+
 /* densities: column-major (ndens, nbf, nbf); d: column-major (naux, ndens),
  * accumulated into (caller zeroes it).  Aux shells c_begin, c_begin+c_step, ...
  * < c_end are processed (the full loop is 0, nshells_aux, 1).  schwarz_bra /
  * schwarz_aux are the Schwarz norms; a triplet is skipped when
- * schwarz_bra[ab] * schwarz_aux[c] < thresh.  Returns the number of skipped
+ * schwarz_bra[ab] * schwarz_aux[c] < thresh, or, with qvl_pair (8 doubles per
+ * packed pair, record as in orc_qvl_estimate) and qvl_aux (4 per aux shell)
+ * given, when the QVL estimate is < thresh.  Returns the number of skipped
  * triplets in *nskipped and the computed ones in *ncomputed. */
 void orc_df_contract(void *bas, void *aux, const double *schwarz_bra, const double *schwarz_aux,
+                     const double *qvl_pair, const double *qvl_aux,
                      double thresh, int ndens, const double *dens, double *d, int c_begin,
                      int c_end, int c_step, int64_t *nskipped, int64_t *ncomputed)
 {
@@ -669,7 +686,11 @@ void orc_df_contract(void *bas, void *aux, const double *schwarz_bra, const doub
             for (b = 0; b <= a; b++) {
                 const orc_shell *sb = &B->sh[b];
                 int na = 2 * sa->L + 1, nb = 2 * sb->L + 1, I, i, j, k, idx;
-                if (schwarz_bra[(size_t)a * (a + 1) / 2 + b] * schwarz_aux[c] < thresh) {
+                size_t kab = (size_t)a * (a + 1) / 2 + b;
+                double est = qvl_pair ? orc_qvl_estimate(qvl_pair + 8 * kab, qvl_aux + 4 * (size_t)c,
+                                                         sc->A, sc->L)
+                                      : schwarz_bra[kab] * schwarz_aux[c];
+                if (est < thresh) {
                     skipped++;
                     continue;
                 }
@@ -843,6 +864,40 @@ double orc_auxiliary_extent(int n, const double *exps, double prec)
         if (e > ext) ext = e;
     }
     return ext;
+}
+
+/* common/screening.f90:101-163: the screener's per-pair and per-aux-shell
+ * quantities as QVL records (layout in orc_qvl_estimate).  pr: 8 doubles per
+ * packed pair k = a(a+1)/2 + b, a >= b; xr: 4 doubles per aux shell. */
+void orc_qvl_records(void *bas, void *aux, const double *schwarz_bra, const double *schwarz_aux,
+                     double thresh_ws, double prec, double *pr, double *xr)
+{
+    orc_basis *B = (orc_basis *)bas, *X = (orc_basis *)aux;
+    int a, b, c, i;
+    for (c = 0; c < X->nshells; c++) {
+        const orc_shell *sc = &X->sh[c];
+        double emin = sc->exps[0];
+        for (i = 1; i < sc->nprim; i++) emin = fmin(emin, sc->exps[i]);
+        xr[4 * c + 0] = orc_auxiliary_extent(sc->nprim, sc->exps, prec);
+        xr[4 * c + 1] = pow(emin, -(2.0 * sc->L + 3.0) / 4.0) * sqrt(dfact(2 * sc->L - 1));
+        xr[4 * c + 2] = schwarz_aux[c];
+        xr[4 * c + 3] = 0.0;
+    }
+    for (a = 0; a < B->nshells; a++)
+        for (b = 0; b <= a; b++) {
+            const orc_shell *sa = &B->sh[a], *sb = &B->sh[b];
+            size_t k = (size_t)a * (a + 1) / 2 + b;
+            double *r = pr + 8 * k, amin = sa->exps[0], bmin = sb->exps[0];
+            for (i = 1; i < sa->nprim; i++) amin = fmin(amin, sa->exps[i]);
+            for (i = 1; i < sb->nprim; i++) bmin = fmin(bmin, sb->exps[i]);
+            orc_contracted_center(sa->nprim, sa->exps, sa->coefs, sa->A, sb->nprim, sb->exps, sb->coefs,
+                                  sb->A, r);
+            r[3] = orc_contracted_extent(sa->nprim, sa->exps, sa->coefs, sa->A, sb->nprim, sb->exps,
+                                         sb->coefs, sb->A, thresh_ws);
+            r[4] = sqrt(sqrt(amin + bmin)); /* (..)**0.25d0 */
+            r[5] = schwarz_bra[k];
+            r[6] = r[7] = 0.0;
+        }
 }
 
 int orc_abi_version(void) { return 1; }

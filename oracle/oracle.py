@@ -64,7 +64,10 @@ def lib():
         L.orc_schwarz_aux.argtypes = [vp, _F64P]
         L.orc_metric.argtypes = [vp, _F64F]
         L.orc_overlap.argtypes = [vp, _F64F]
-        L.orc_df_contract.argtypes = [vp, vp, _F64P, _F64P, cd, ci, _F64F, _F64F, ci, ci, ci,
+        L.orc_qvl_records.argtypes = [vp, vp, _F64P, _F64P, cd, cd, _F64P, _F64P]
+        L.orc_qvl_estimate.restype = cd
+        L.orc_qvl_estimate.argtypes = [_F64P, _F64P, _F64P, ci]
+        L.orc_df_contract.argtypes = [vp, vp, _F64P, _F64P, vp, vp, cd, ci, _F64F, _F64F, ci, ci, ci,
                                       ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64)]
         L.orc_erfinv.restype = cd
@@ -175,11 +178,30 @@ def overlap(bas):
     return S
 
 
+QVL_THRESH_WS = 1e-4  # t_df_screener(shells, shells_aux, 1d-4), diabatization/intor.f90:84
+QVL_AUX_PREC = 1e-8   # auxiliary_extent(exps, 1d-8), common/screening.f90:109
+
+
+def qvl_records(bas, aux, qbra, qaux):
+    """Screener set-up of common/screening.f90:101-163 as QVL records: per
+    packed pair k = a(a+1)/2 + b (a >= b) [centre x, y, z, contracted extent,
+    (min a + min b)^(1/4), Q_ab, 0, 0]; per aux shell [auxiliary extent,
+    beta = min exp^(-(2L+3)/4) sqrt((2L-1)!!), Q_C, 0]."""
+    pr = np.zeros((bas.nshells * (bas.nshells + 1) // 2, 8))
+    xr = np.zeros((aux.nshells, 4))
+    lib().orc_qvl_records(bas.h, aux.h, np.ascontiguousarray(qbra, dtype=np.float64),
+                          np.ascontiguousarray(qaux, dtype=np.float64), QVL_THRESH_WS, QVL_AUX_PREC,
+                          pr, xr)
+    return pr, xr
+
+
 def df_contract(bas, aux, densities, qbra=None, qaux=None, thresh=SCHWARZ_THRESH,
-                c_begin=0, c_end=None, c_step=1):
+                c_begin=0, c_end=None, c_step=1, qvl=None):
     """Screened integral-direct d(P, I) = sum (mu nu|P) D_I(mu, nu), the loop of
     diabatization/intor.f90:108-183.  densities is (ndens, nbf, nbf) in any
-    memory order.  Returns (d (naux, ndens) F-order, nskipped, ncomputed)."""
+    memory order.  qvl = (pair records, aux records) of qvl_records switches
+    the screen to the QVL estimate.  Returns (d (naux, ndens) F-order,
+    nskipped, ncomputed)."""
     if qbra is None:
         qbra = schwarz_bra(bas)
     if qaux is None:
@@ -190,7 +212,12 @@ def df_contract(bas, aux, densities, qbra=None, qaux=None, thresh=SCHWARZ_THRESH
     ns, nc = ctypes.c_int64(0), ctypes.c_int64(0)
     if c_end is None:
         c_end = aux.nshells
+    qp = qx = None
+    if qvl is not None:
+        qp, qx = (np.ascontiguousarray(r, dtype=np.float64) for r in qvl)
     lib().orc_df_contract(bas.h, aux.h, np.ascontiguousarray(qbra), np.ascontiguousarray(qaux),
+                          None if qp is None else qp.ctypes.data,
+                          None if qx is None else qx.ctypes.data,
                           thresh, ndens, dens, d, c_begin, c_end, c_step, ctypes.byref(ns),
                           ctypes.byref(nc))
     return d, ns.value, nc.value
@@ -218,10 +245,15 @@ def dtrmm_lower(Linv, d):
 
 
 def contract_coulomb_densities_2d(shells, shells_aux, densities, thresh=SCHWARZ_THRESH,
-                                  return_info=False):
+                                  return_info=False, screening="schwarz"):
     bas = Basis.from_shells(shells)
     aux = Basis.from_shells(shells_aux)
-    d, nskip, ncomp = df_contract(bas, aux, densities, thresh=thresh)
+    qvl = None
+    if screening == "qvl":
+        qvl = qvl_records(bas, aux, schwarz_bra(bas), schwarz_aux(aux))
+    elif screening != "schwarz":
+        raise ValueError(screening)
+    d, nskip, ncomp = df_contract(bas, aux, densities, thresh=thresh, qvl=qvl)
     M = metric(aux)
     Linv = inv_chol(M)
     B = dtrmm_lower(Linv, d)
@@ -250,12 +282,12 @@ def assemble_4d(B, nstates):
 
 
 def contract_coulomb_densities_4d(shells, shells_aux, densities, thresh=SCHWARZ_THRESH,
-                                  return_info=False):
+                                  return_info=False, screening="schwarz"):
     ndens = densities.shape[0]
     nstates = int(math.isqrt(8 * ndens + 1) / 2.0 - 0.5)
     assert nstates * (nstates + 1) // 2 == ndens
     B, info = contract_coulomb_densities_2d(shells, shells_aux, densities, thresh,
-                                            return_info=True)
+                                            return_info=True, screening=screening)
     J = assemble_4d(B, nstates)
     if return_info:
         info["B"] = B

@@ -668,6 +668,8 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   // downward sweep per (class, L) finds all of them.
   pl.nsurv.assign(LMAX_AUX + 1, std::vector<int>(pl.ps.n(), 0));
   pl.ncomputed = 0;
+  pl.ntriplets = (int64_t)o.n * (o.n + 1) / 2 * x.n;
+  if (opt.screening == DFHIP_SCREEN_QVL) return;  // prefixes and counts from apply_qvl
   for (int L = 0; L <= LMAX_AUX; L++) {
     const auto& v = pl.alist[L];
     const int nv = (int)v.size();
@@ -699,7 +701,77 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
         }
       }
   }
-  pl.ntriplets = (int64_t)o.n * (o.n + 1) / 2 * x.n;
+}
+
+// erfinv(y) for 0 < y < 1 by Newton's method on erf (quadratic convergence
+// from sqrt(-log(1 - y)), which lies above the root).
+double erfinv_pos(double y) {
+  double x = std::sqrt(-std::log(1.0 - y));
+  for (int it = 0; it < 100; it++) {
+    const double dx = (std::erf(x) - y) / (2.0 / std::sqrt(M_PI) * std::exp(-x * x));
+    x -= dx;
+    if (std::fabs(dx) <= 1e-16 * x) break;
+  }
+  return x;
+}
+
+// QVL screening (common/screening.f90:101-163 set-up, :267-299 estimate):
+// aux records on the host, pair records and the screened-in prefixes of every
+// (pair, aux L) on the device; fills pl.nsurv and the counts.  The returned
+// buffers are the records the 3c2e kernels test each triplet against.
+struct QvlRecords {
+  DBuf<double> pair, aux;
+};
+
+void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int* ps1, const int* ps2,
+               const std::vector<DBuf<int>>& dalist, double thresh, QvlRecords& qr, hipStream_t st) {
+  const HostShells& o = pl.orb;
+  const HostShells& x = pl.aux;
+  const int np = pl.ps.n();
+  std::vector<double> xr((size_t)4 * x.n, 0.0);
+  for (int sh = 0; sh < x.n; sh++) {
+    const int L = x.L[sh];
+    double ext = 0.0, emin = x.exps[x.pptr[sh]];
+    const double pt = -std::log(1e-8);  // auxiliary_extent(exps, 1d-8), :168-196
+    for (int i = 0; i < x.nprim[sh]; i++) {
+      const double e = x.exps[x.pptr[sh] + i];
+      const double arg = (pt + 0.5 * std::log(e)) / e;
+      ext = std::max(ext, arg < 0 ? 0.01 : std::sqrt(arg));
+      emin = std::min(emin, e);
+    }
+    double df = 1.0;  // (2L-1)!!
+    for (int k = 2 * L - 1; k > 1; k -= 2) df *= k;
+    xr[4 * sh + 0] = ext;
+    xr[4 * sh + 1] = std::pow(emin, -(2.0 * L + 3.0) / 4.0) * std::sqrt(df);
+    xr[4 * sh + 2] = pl.qc[sh];
+  }
+  upload(qr.aux, xr, st);
+  DBuf<double> dq;
+  upload(dq, pl.qab, st);
+  qr.pair.alloc((size_t)8 * std::max(np, 1));
+  qvl_pair_records(dorb, ps1, ps2, dq.p, np, erfinv_pos(1.0 - 1e-4), qr.pair.p, st);
+  HC(hipGetLastError());
+  DBuf<int> ns((size_t)3 * std::max(np, 1));
+  std::vector<int> h((size_t)3 * np);
+  pl.ncomputed = 0;
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    const int nl = (int)pl.alist[L].size();
+    if (nl == 0 || np == 0) continue;
+    qvl_screen(qr.pair.p, np, daux, dalist[L].p, nl, L, qr.aux.p, thresh, ns.p, ns.p + np, ns.p + 2 * np, st);
+    HC(hipGetLastError());
+    HC(hipMemcpyAsync(h.data(), ns.p, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    std::copy(h.begin(), h.begin() + np, pl.nsurv[L].begin());
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb);
+        for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+          pl.class_trip[k * 6 + L] += h[np + p];
+          pl.class_prim[k * 6 + L] += (int64_t)o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]] * h[2 * np + p];
+          pl.ncomputed += h[np + p];
+        }
+      }
+  }
 }
 
 struct Outputs {
@@ -712,7 +784,8 @@ struct Outputs {
 void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const double* dens,
          const dfhip_options& opt, Outputs out, dfhip_stats* stats) {
   if (opt.precision != DFHIP_FP64) throw Error{DFHIP_ERR_UNSUPPORTED, "only the FP64 path is available"};
-  if (opt.screening != DFHIP_SCREEN_SCHWARZ && opt.screening != DFHIP_SCREEN_NONE)
+  if (opt.screening != DFHIP_SCREEN_SCHWARZ && opt.screening != DFHIP_SCREEN_NONE &&
+      opt.screening != DFHIP_SCREEN_QVL)
     throw Error{DFHIP_ERR_UNSUPPORTED, "unknown screening mode"};
   Plan pl;
   pl.orb = parse_shells(B, LMAX_ORB, "orbital basis");
@@ -785,11 +858,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   std::vector<DBuf<int64_t>> dacol(LMAX_AUX + 1);
   std::vector<int> kmax(LMAX_AUX + 1, 1);
   for (int L = 0; L <= LMAX_AUX; L++) {
-    upload(dnsurv[L], pl.nsurv[L], st);
     upload(dalist[L], pl.alist[L], st);
     upload(dacol[L], pl.acol[L], st);
     for (int sh : pl.alist[L]) kmax[L] = std::max(kmax[L], x.nprim[sh]);
   }
+  QvlRecords qvl;
+  const bool use_qvl = opt.screening == DFHIP_SCREEN_QVL;
+  if (use_qvl) apply_qvl(pl, dorb.view, daux.view, ps1.p, ps2.p, dalist, opt.thresh, qvl, st);
+  for (int L = 0; L <= LMAX_AUX; L++) upload(dnsurv[L], pl.nsurv[L], st);
   DBuf<int> dorig;
   upload(dorig, pl.orig, st);
   const bool gen = use_generated();
@@ -896,9 +972,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     partp = reserve(c.wsPart, (size_t)nsplit * widest * ndens);
   }
   // Non-zero row segments of T per column tile: for each pair class, the
-  // pairs whose screened prefix reaches the tile form a prefix of the class
-  // (pairs are sorted by Q_ab and the prefix lengths are non-increasing).
+  // rows from the class start to the last pair whose screened prefix reaches
+  // the tile (under Schwarz screening these pairs are exactly a prefix of
+  // the class: pairs are sorted by Q_ab and the prefixes are non-increasing).
   const int GT = contract_tile();
+  // suffix maxima of the prefixes inside each class: equal to nsurv under
+  // Schwarz screening (non-increasing), an upper envelope under QVL
+  std::vector<std::vector<int>> nsmax(LMAX_AUX + 1);
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    nsmax[L] = pl.nsurv[L];
+    for (int k = 0; k < NPCLS; k++)
+      for (int p = pl.ps.cls_end[k] - 2; p >= pl.ps.cls_begin[k]; p--)
+        nsmax[L][p] = std::max(nsmax[L][p], nsmax[L][p + 1]);
+  }
   auto tile_segments = [&](int64_t cA, int64_t w, std::vector<int64_t>& seg) {
     const int ntile = (int)((w + GT - 1) / GT);
     seg.assign((size_t)ntile * (2 * NPCLS + 1), 0);
@@ -918,8 +1004,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             if (ia < 0) ia = 0;
             if (cols[ia] + 2 * L + 1 <= c0) ia++;
             if (ia >= (int)cols.size() || cols[ia] >= c1) continue;
-            // pairs with nsurv[L][p] > ia, a prefix of the class
-            const auto& ns = pl.nsurv[L];
+            // pairs up to the last one with nsurv[L][p] > ia
+            const auto& ns = nsmax[L];
             int lo = p0, hi = p1;
             while (lo < hi) {
               const int mid = (lo + hi) / 2;
@@ -959,6 +1045,11 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
           if (np == 0) continue;
           PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dnsurv[L].p + p0, np};
           AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi, kmax[L]};
+          if (use_qvl) {
+            pa.qvl = qvl.pair.p + (size_t)8 * p0;
+            xa.qvl = qvl.aux.p;
+            xa.thresh = opt.thresh;
+          }
           if (gen && gen3_has(la, lb, L)) {
             launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
           } else {

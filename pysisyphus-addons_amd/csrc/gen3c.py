@@ -257,7 +257,8 @@ def gen_class(la, lb, lc):
     w(f"  const bool active = c < cend;")
     w(f"  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];")
     w(f"  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];")
-    w(f"  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];")
+    w(f"  const bool live = active && triplet_kept(pa, xa, pidx, sh, {lc}, Cx, Cy, Cz);")
+    w(f"  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];")
     w(f"  const int npp = npp_v[pidx];")
     w(f"  const double* __restrict__ recp = recs + recoff[pidx];")
     w(f"  const int64_t row0 = pa.row[pidx];")

@@ -139,4 +139,27 @@ __device__ inline double herm_R(int t, int u, int v, int L, const double* pw, co
   return sum;
 }
 
+// QVL screening estimate (common/screening.f90:267-299): with the pair
+// record pr = [centre x, y, z, extent, (min a + min b)^(1/4), Q_ab] and the aux
+// record xr = [extent, beta, Q_C], R = |centre - C|; for R <= ext_ab + ext_C
+// the Schwarz product Q_ab Q_C, else Q_ab pi sqrt(2) beta / ((..)^(1/4) R^(L+1)).
+// The triplet is kept unless the estimate is below thresh.  Evaluated in
+// the reference's operation order without contraction, so host and device
+// take the same decisions.
+__host__ __device__ inline bool qvl_keep(const double* pr, const double* xr, double Cx, double Cy, double Cz, int L,
+                                         double thresh) {
+#pragma clang fp contract(off)
+  const double dx = pr[0] - Cx, dy = pr[1] - Cy, dz = pr[2] - Cz;
+  const double R = sqrt(dx * dx + dy * dy + dz * dz);
+  double est;
+  if (R <= pr[3] + xr[0]) {
+    est = pr[5] * xr[2];
+  } else {
+    double Rp = R;
+    for (int i = 0; i < L; i++) Rp = Rp * R;
+    est = pr[5] * PI * sqrt(2.0) * xr[1] / (pr[4] * Rp);
+  }
+  return !(est < thresh);
+}
+
 }  // namespace dfhip

@@ -14,7 +14,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_000(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 0, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -77,7 +78,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_001(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 1, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -162,7 +164,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_002(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 2, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -281,7 +284,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_003(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 3, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];

@@ -14,7 +14,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_200(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 0, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_201(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 1, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -506,7 +508,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_202(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 2, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -1183,7 +1186,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_203(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 3, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];

@@ -14,7 +14,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_210(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 0, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -530,7 +531,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_211(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 1, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -2102,7 +2104,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_212(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 2, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
@@ -5031,7 +5034,8 @@ __global__ __launch_bounds__(64) void int3c2e_gen_213(const double* __restrict__
   const bool active = c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
-  const int nprim_c = active ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 3, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
   const int npp = npp_v[pidx];
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];

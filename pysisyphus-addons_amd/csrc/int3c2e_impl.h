@@ -20,6 +20,7 @@ struct PairArgs {
   const int64_t* row;  // output row of the pair's first function
   const int* nsurv;    // screened-in prefix length of the aux list, per pair
   int npairs;
+  const double* qvl = nullptr;  // QVL pair records (8 doubles per pair), or null for Schwarz
 };
 
 struct AuxArgs {
@@ -27,7 +28,16 @@ struct AuxArgs {
   const int64_t* col;  // output column of each listed aux shell
   int lo, hi;          // list window processed by this launch
   int kmax;            // largest primitive count in the list
+  const double* qvl = nullptr;  // QVL aux records (4 doubles per aux shell), used with PairArgs::qvl
+  double thresh = 0.0;
 };
+
+// Whether triplet (pair pidx, aux shell sh at C) survives QVL screening; always
+// true under Schwarz screening, whose screened-in set is the nsurv prefix.
+__device__ inline bool triplet_kept(const PairArgs& pa, const AuxArgs& xa, int pidx, int sh, int LC, double Cx,
+                                    double Cy, double Cz) {
+  return pa.qvl == nullptr || qvl_keep(pa.qvl + 8 * (size_t)pidx, xa.qvl + 4 * (size_t)sh, Cx, Cy, Cz, LC, xa.thresh);
+}
 
 template <int LA, int LB, int LC>
 struct Cls3 {
@@ -58,6 +68,7 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
   constexpr int L = K::L, H = K::H, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NT = K::NT;
   __shared__ double lds[K::LDS];
   __shared__ int cshell[K::NC];
+  __shared__ int ckeep[K::NC];
 
   const int pidx = blockIdx.x;
   const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
@@ -81,6 +92,7 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
     cxyz[4 * c + 0] = ket.xyz[3 * sh + 0];
     cxyz[4 * c + 1] = ket.xyz[3 * sh + 1];
     cxyz[4 * c + 2] = ket.xyz[3 * sh + 2];
+    ckeep[c] = triplet_kept(pa, xa, pidx, sh, LC, ket.xyz[3 * sh + 0], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2]);
   }
 
   double A[3], B[3];
@@ -129,7 +141,7 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
       for (int k = tid; k < nc * 4 * (L + 1); k += NT) {
         const int c = k / (4 * (L + 1)), q = k - c * (4 * (L + 1));
         const int sh = cshell[c];
-        const bool live = kp < ket.nprim[sh];
+        const bool live = kp < ket.nprim[sh] && ckeep[c];
         const double g = live ? ket.exps[ket.pptr[sh] + kp] : 1.0;
         const double PC0 = P[0] - cxyz[4 * c + 0], PC1 = P[1] - cxyz[4 * c + 1], PC2 = P[2] - cxyz[4 * c + 2];
         if (q <= L) {

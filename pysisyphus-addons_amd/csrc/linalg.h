@@ -6,6 +6,8 @@
 
 #include <algorithm>
 
+#include "dfhip_common.h"
+
 namespace dfhip {
 
 void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, const int* L, const int* f0,
@@ -32,6 +34,14 @@ void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates,
 void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,
                              hipStream_t st);
 void boys_probe(const int* n, const double* T, int64_t npts, const double* tab, double* F, hipStream_t st);
+
+// QVL screening (screen.hip): pair records (8 doubles per pair) and, per aux
+// list of angular momentum L, the screened-in prefix, kept shells and kept
+// primitives of every pair.
+void qvl_pair_records(const DevShells& bra, const int* s1, const int* s2, const double* qab, int npairs, double erfi,
+                      double* rec, hipStream_t st);
+void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int* alist, int nlist, int L,
+                const double* xrec, double thresh, int* nsurv, int* nkeep, int* nprimkeep, hipStream_t st);
 
 constexpr int CHOL_BLOCK = 64;
 

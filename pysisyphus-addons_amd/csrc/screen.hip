@@ -1,0 +1,98 @@
+// QVL screening on the device: pair records and the screened-in prefixes.
+#include "hermite.h"
+#include "linalg.h"
+
+namespace dfhip {
+
+// Pair record [centre x, y, z, extent, (min a + min b)^(1/4), Q_ab, 0, 0] of
+// each shell pair (common/screening.f90:135-146): |d_a d_b|-weighted mean of
+// the primitive centres (:235-265) and the largest primitive extent
+// sqrt(2/p) erfinv(1 - thresh_ws) + |R - P| (:198-232).  One thread per pair.
+__global__ void qvl_pair_kernel(DevShells bra, const int* __restrict__ s1v, const int* __restrict__ s2v,
+                                const double* __restrict__ qab, int npairs, double erfi, double* __restrict__ rec) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int s1 = s1v[p], s2 = s2v[p];
+  const int na = bra.nprim[s1], nb = bra.nprim[s2], a0 = bra.pptr[s1], b0 = bra.pptr[s2];
+  const double A[3] = {bra.xyz[3 * s1], bra.xyz[3 * s1 + 1], bra.xyz[3 * s1 + 2]};
+  const double B[3] = {bra.xyz[3 * s2], bra.xyz[3 * s2 + 1], bra.xyz[3 * s2 + 2]};
+  double Rc[3] = {0.0, 0.0, 0.0}, den = 0.0, amin = bra.exps[a0], bmin = bra.exps[b0];
+  {
+#pragma clang fp contract(off)
+    for (int i = 0; i < na; i++) {
+      const double ax = bra.exps[a0 + i];
+      amin = fmin(amin, ax);
+      for (int j = 0; j < nb; j++) {
+        const double bx = bra.exps[b0 + j], px = ax + bx;
+        const double w = fabs(bra.coefs[a0 + i] * bra.coefs[b0 + j]);
+        for (int k = 0; k < 3; k++) Rc[k] += w * (ax * A[k] + bx * B[k]) / px;
+        den += w;
+      }
+    }
+    for (int j = 0; j < nb; j++) bmin = fmin(bmin, bra.exps[b0 + j]);
+    for (int k = 0; k < 3; k++) Rc[k] /= den;
+    double ext = 0.0;
+    for (int i = 0; i < na; i++)
+      for (int j = 0; j < nb; j++) {
+        const double ax = bra.exps[a0 + i], bx = bra.exps[b0 + j], px = ax + bx;
+        const double tmp = sqrt(2.0 / px) * erfi;
+        double dist = 0.0;
+        for (int k = 0; k < 3; k++) {
+          const double Pk = (ax * A[k] + bx * B[k]) / px;
+          dist += (Rc[k] - Pk) * (Rc[k] - Pk);
+        }
+        dist = sqrt(dist);
+        if (tmp + dist > ext) ext = tmp + dist;
+      }
+    double* r = rec + 8 * (size_t)p;
+    r[0] = Rc[0];
+    r[1] = Rc[1];
+    r[2] = Rc[2];
+    r[3] = ext;
+    r[4] = sqrt(sqrt(amin + bmin));
+    r[5] = qab[p];
+    r[6] = 0.0;
+    r[7] = 0.0;
+  }
+}
+
+// Screened-in prefix of the aux list of angular momentum L for every pair
+// (the last kept list entry + 1), the number of kept shells and their total
+// primitive count.  One thread per pair, entries in list order.
+__global__ void qvl_screen_kernel(const double* __restrict__ prec, int npairs, DevShells ket,
+                                  const int* __restrict__ alist, int nlist, int L, const double* __restrict__ xrec,
+                                  double thresh, int* __restrict__ nsurv, int* __restrict__ nkeep,
+                                  int* __restrict__ nprimkeep) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  double pr[6];
+  for (int k = 0; k < 6; k++) pr[k] = prec[8 * (size_t)p + k];
+  int last = 0, cnt = 0, pc = 0;
+  for (int i = 0; i < nlist; i++) {
+    const int sh = alist[i];
+    if (qvl_keep(pr, xrec + 4 * (size_t)sh, ket.xyz[3 * sh], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2], L, thresh)) {
+      last = i + 1;
+      cnt++;
+      pc += ket.nprim[sh];
+    }
+  }
+  nsurv[p] = last;
+  nkeep[p] = cnt;
+  nprimkeep[p] = pc;
+}
+
+void qvl_pair_records(const DevShells& bra, const int* s1, const int* s2, const double* qab, int npairs, double erfi,
+                      double* rec, hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(qvl_pair_kernel, dim3((npairs + 127) / 128), dim3(128), 0, st, bra, s1, s2, qab, npairs, erfi,
+                     rec);
+}
+
+void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int* alist, int nlist, int L,
+                const double* xrec, double thresh, int* nsurv, int* nkeep, int* nprimkeep, hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(qvl_screen_kernel, dim3((npairs + 127) / 128), dim3(128), 0, st, prec, npairs, ket, alist, nlist,
+                     L, xrec, thresh, nsurv, nkeep, nprimkeep);
+}
+
+}  // namespace dfhip

@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
 
 
 FP64, MIXED = 0, 1
-SCREEN_SCHWARZ, SCREEN_NONE = 0, 1
+SCREEN_SCHWARZ, SCREEN_NONE, SCREEN_QVL = 0, 1, 2
 DENS_FORTRAN, DENS_C = 0, 1
 
 EXPORTED = (

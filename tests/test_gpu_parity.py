@@ -36,14 +36,22 @@ def dense3c(shells, aux, screening=N.SCREEN_SCHWARZ):
     return out, st.as_dict()
 
 
-def oracle_dense3c(shells, aux, screened=True):
+def oracle_dense3c(shells, aux, screened=True, qvl=False):
     bas, x = O.Basis.from_shells(shells), O.Basis.from_shells(aux)
     qb, qa = O.schwarz_bra(bas), O.schwarz_aux(x)
+    if qvl:
+        pr, xr = O.qvl_records(bas, x, qb, qa)
     out = np.zeros((shells.nbf, shells.nbf, aux.nbf))
     for a in range(bas.nshells):
         for b in range(a + 1):
+            k = a * (a + 1) // 2 + b
             for c in range(x.nshells):
-                if screened and qb[a * (a + 1) // 2 + b] * qa[c] < O.SCHWARZ_THRESH:
+                if qvl:
+                    est = O.lib().orc_qvl_estimate(np.ascontiguousarray(pr[k]), np.ascontiguousarray(xr[c]),
+                                                   np.ascontiguousarray(aux.shells[c].center), aux.shells[c].L)
+                    if est < O.SCHWARZ_THRESH:
+                        continue
+                elif screened and qb[k] * qa[c] < O.SCHWARZ_THRESH:
                     continue
                 blk = O.int3c2e_block(bas, x, a, b, c)
                 sa = slice(bas.off[a], bas.off[a] + blk.shape[0])
@@ -182,6 +190,39 @@ def test_workload0_df_and_j_vs_oracle():
     _say(f"workload 0: max |dJ| {errJ:.2e}, |J|max {np.abs(Jref).max():.3e}")
     assert errJ <= 1e-10
     np.testing.assert_allclose(df, info["B"], atol=1e-6 * np.abs(info["B"]).max())
+
+
+def test_qvl_workload0_vs_oracle():
+    """QVL screening (the screener's qvl_estimate) through the drop-in path:
+    same kept/skipped triplets as the oracle and J at 1e-10."""
+    shells, aux, dens = synthetic.workload(0)
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    opt = N.default_options(screening=N.SCREEN_QVL)
+    _, J, st = _intor.contract(b, x, dens, nstates=3, want_df=False, want_j=True, options=opt)
+    Jref, info = O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True, screening="qvl")
+    errJ = np.abs(J - Jref).max()
+    _say(f"workload 0 QVL: max |dJ| {errJ:.2e}, computed {st['ncomputed']} / {st['ntriplets']}")
+    assert st["ncomputed"] == info["ncomputed"]
+    assert st["nskipped"] == info["nskipped"]
+    assert errJ <= 1e-10
+
+
+def test_qvl_screened_blocks():
+    """Every class up to (ff|h) with QVL screening: the kept blocks match the
+    oracle, the screened ones are zero."""
+    shells, aux = _mixed_system(seed=5)
+    # spread the centres so that the far-field branch of the estimate is taken
+    sh = Shells([Shell(L=s.L, center=s.center * 8.0, exps=s.exps * 0.5, coeffs=s.coeffs, center_ind=s.center_ind)
+                 for s in shells.shells])
+    ax = Shells([Shell(L=s.L, center=s.center * 8.0, exps=s.exps, coeffs=s.coeffs, center_ind=s.center_ind)
+                 for s in aux.shells])
+    got, st = dense3c(sh, ax, screening=N.SCREEN_QVL)
+    ref = oracle_dense3c(sh, ax, qvl=True)
+    err = np.abs(got - ref).max()
+    _say(f"QVL screened blocks: max abs err {err:.2e}, computed {st['ncomputed']} / {st['ntriplets']}")
+    assert 0 < st["ncomputed"] < st["ntriplets"]
+    assert err <= 1e-12 * max(np.abs(ref).max(), 1.0)
 
 
 def test_workload0_screened_blocks():

@@ -169,6 +169,60 @@ def test_contracted_center_kats():
     assert np.linalg.norm(R - 0.53650793650794) <= 1e-14
 
 
+def _qvl_ratio(la, lb, Lc, g, R):
+    """QVL estimate / ||(ab|C)||_F for a two-centre contracted pair and an aux
+    shell of angular momentum Lc and exponent g at distance R."""
+    A, B = np.zeros(3), np.array([0.9, -0.4, 0.3])
+    orb = Shells([Shell(L=la, center=A, exps=[0.4, 1.5, 5.0], coeffs=[0.5, 0.4, 0.3]),
+                  Shell(L=lb, center=B, exps=[0.3, 1.1], coeffs=[0.6, 0.5])])
+    aux = Shells([Shell(L=Lc, center=A + R * np.array([0.48, 0.6, 0.64]), exps=[g], coeffs=[1.0])])
+    bas, x = O.Basis.from_shells(orb), O.Basis.from_shells(aux)
+    pr, xr = O.qvl_records(bas, x, O.schwarz_bra(bas), O.schwarz_aux(x))
+    est = O.lib().orc_qvl_estimate(np.ascontiguousarray(pr[1]), np.ascontiguousarray(xr[0]),
+                                   np.ascontiguousarray(aux.shells[0].center), Lc)
+    return est / np.linalg.norm(O.int3c2e_block(bas, x, 1, 0, 0))
+
+
+def test_qvl_estimate_far_field():
+    """common/screening.f90:267-299.  Far from the pair, ||(ab|C)|| falls as
+    R^-(L+1) times the aux multipole moment, which scales as g^-(2L+3)/4;
+    the estimate must track that exactly: the ratio is independent of the
+    aux exponent (pins beta's exponent), converges in R (pins R^(L+1)), is
+    nearly independent of L (pins sqrt((2L-1)!!)) and stays a modest
+    overestimate (pins pi sqrt(2) to within the bounds)."""
+    for la, lb in [(0, 0), (1, 0), (2, 1)]:
+        far = []
+        for Lc in range(4):
+            r = {(g, R): _qvl_ratio(la, lb, Lc, g, R) for g in (0.3, 1.2) for R in (40.0, 60.0)}
+            for R in (40.0, 60.0):
+                assert r[(0.3, R)] == pytest.approx(r[(1.2, R)], rel=1e-6)
+            assert r[(0.3, 60.0)] == pytest.approx(r[(0.3, 40.0)], rel=0.02)
+            assert 1.0 < r[(0.3, 60.0)] < 1.6
+            far.append(r[(0.3, 60.0)])
+        assert max(far) / min(far) < 1.06
+
+
+def test_qvl_estimate_near_is_schwarz():
+    """Inside the combined extents the estimate is the Schwarz product."""
+    pr = np.array([0.1, 0.2, 0.3, 2.0, 1.3, 0.7, 0.0, 0.0])
+    xr = np.array([1.5, 9.0, 0.4, 0.0])
+    for C in ([0.1, 0.2, 0.3], [0.1, 0.2, 3.8]):
+        assert O.lib().orc_qvl_estimate(pr, xr, np.array(C), 2) == 0.7 * 0.4
+    far = O.lib().orc_qvl_estimate(pr, xr, np.array([0.1, 0.2, 3.9]), 2)
+    assert far == pytest.approx(0.7 * math.pi * math.sqrt(2.0) * 9.0 / (1.3 * 3.6 ** 3), rel=1e-15)
+
+
+def test_qvl_screening_workload0():
+    """QVL screens a subset of triplets whose contributions are negligible:
+    J moves by far less than 1e-8 against the Schwarz-screened result."""
+    sh, aux, dens = synthetic.workload(0)
+    J1, i1 = O.contract_coulomb_densities_4d(sh, aux, dens, return_info=True)
+    J2, i2 = O.contract_coulomb_densities_4d(sh, aux, dens, return_info=True, screening="qvl")
+    assert i1["ncomputed"] + i1["nskipped"] == i2["ncomputed"] + i2["nskipped"]
+    assert i2["ncomputed"] < i1["ncomputed"]
+    assert np.abs(J1 - J2).max() < 1e-9
+
+
 def test_overlap_vs_orca(bena2):
     bas = O.Basis(bena2["bas_centers"], bena2["bas_spec"], bena2["bas_data_orca"])
     S = O.overlap(bas)
