@@ -9,8 +9,10 @@ densities resident in HBM when the timed region starts.
 Prints one JSON line: value = screened shell-triplets per second of the
 whole step; roofline of the 3c2e kernels (algorithmic FP64 flops from
 pysisyphus_addons_amd.flops over their HIP-event time on the library stream);
-cpu_baseline = the FP64 oracle's integral-direct loop on a sample of the same
-workload on this host's cores.
+mixed = the same step with the mixed-precision path (FP32 integral arithmetic
+for triplets with Schwarz product below 1e-3), its throughput and the largest
+deviation of its J from the FP64 path's; cpu_baseline = the FP64 oracle's
+integral-direct loop on a sample of the same workload on this host's cores.
 """
 
 import argparse
@@ -59,6 +61,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-step", type=int, default=6, help="aux-shell stride of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
+    ap.add_argument("--fp32-below", type=float, default=None, help="mixed path: FP32 below this Schwarz product")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -79,27 +83,47 @@ def main():
     dev = torch.device("cuda:0")
     dens_d = torch.from_numpy(dens).to(dev)
     torch.cuda.synchronize()
-    opt = N.default_options(dens_layout=N.DENS_C, dens_on_device=1)
     J = np.zeros((nstates,) * 4, order="F")
     st = N.Stats()
 
-    def step():
-        s = N.lib().dfhip_contract_coulomb_densities(
-            ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, dens_d.data_ptr(),
-            ctypes.byref(opt), None, nstates, J.ctypes.data, ctypes.byref(st))
-        N.check(s)
-        return st.as_dict()
+    def run(opt):
+        def step():
+            s = N.lib().dfhip_contract_coulomb_densities(
+                ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, dens_d.data_ptr(),
+                ctypes.byref(opt), None, nstates, J.ctypes.data, ctypes.byref(st))
+            N.check(s)
+            return st.as_dict()
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    recs = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        recs.append(step())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt / args.steps * 1e3
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        recs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            recs.append(step())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return dt / args.steps * 1e3, recs, J.copy()
+
+    ms_per_step, recs, J64 = run(N.default_options(dens_layout=N.DENS_C, dens_on_device=1))
+    print(f"fp64: {ms_per_step:.2f} ms per step", file=sys.stderr, flush=True)
+    mixed = None
+    if not args.no_mixed:
+        mopt = N.default_options(dens_layout=N.DENS_C, dens_on_device=1, precision=N.MIXED)
+        if args.fp32_below is not None:
+            mopt.fp32_below = args.fp32_below
+        ms_mixed, mrecs, Jm = run(mopt)
+        print(f"mixed: {ms_mixed:.2f} ms per step", file=sys.stderr, flush=True)
+        m0 = mrecs[-1]
+        mixed = {
+            "value": m0["ncomputed"] / (ms_mixed * 1e-3),
+            "ms_per_step": ms_mixed,
+            "speedup_vs_fp64": ms_per_step / ms_mixed,
+            "fp32_below": mopt.fp32_below,
+            "fp32_triplets": m0["ncomputed_fp32"],
+            "max_abs_dJ_vs_fp64": float(np.abs(Jm - J64).max()),
+            "phases_ms": {k[3:]: float(np.mean([r[k] for r in mrecs])) for k in m0 if k.startswith("ms_")},
+        }
     r0 = recs[-1]
     ncomp = r0["ncomputed"]
     avg = lambda k: float(np.mean([r[k] for r in recs]))
@@ -143,6 +167,8 @@ def main():
             "ms_per_launch": ms_int,
         },
     }
+    if mixed is not None:
+        out["mixed"] = mixed
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(shells, aux, dens, args.cpu_step)
     print(json.dumps(out), flush=True)

@@ -96,7 +96,10 @@ enum { DFHIP_SCREEN_SCHWARZ = 0, DFHIP_SCREEN_NONE = 1, DFHIP_SCREEN_QVL = 2 };
 enum { DFHIP_DENS_FORTRAN = 0, DFHIP_DENS_C = 1 };
 
 typedef struct {
-    int32_t precision;   /* DFHIP_FP64 */
+    int32_t precision;   /* DFHIP_FP64, or DFHIP_MIXED: 3c2e triplets whose Schwarz
+                            product Q_ab Q_C is below fp32_below are evaluated
+                            with FP32 arithmetic (FP64 Boys values, prefactors
+                            and primitive-pair sums); everything else is FP64 */
     int32_t screening;   /* DFHIP_SCREEN_SCHWARZ (reference), _NONE, or _QVL: the
                             screener's qvl_estimate (common/screening.f90:267-299)
                             with thresh_ws = 1e-4 and aux extents at 1e-8 */
@@ -107,6 +110,7 @@ typedef struct {
     int64_t max_tile_bytes; /* cap of the (ab|P) staging matrix, 0 = automatic */
     int32_t dens_on_device; /* densities is a device pointer (HBM-resident input) */
     int32_t reserved;
+    double fp32_below;   /* DFHIP_MIXED: FP32 below this Schwarz product (default 1e-3) */
 } dfhip_options;
 
 typedef struct {
@@ -124,6 +128,7 @@ typedef struct {
        (la*(la+1)/2 + lb) * 6 + lc with la >= lb the pair's angular momenta */
     int64_t class_triplets[15 * 6];
     int64_t class_prim_triplets[15 * 6];
+    int64_t ncomputed_fp32; /* DFHIP_MIXED: triplets inside the FP32 windows */
 } dfhip_stats;
 
 void dfhip_default_options(dfhip_options *opt);
@@ -142,8 +147,9 @@ int dfhip_contract_coulomb_densities(const dfhip_basis *bas, const dfhip_basis *
 int dfhip_int3c2e_dense(const dfhip_basis *bas, const dfhip_basis *aux, const dfhip_options *opt,
                         double *out, dfhip_stats *stats);
 
-/* Frees the device workspace the library keeps between calls (staging
- * matrix, bordered metric, packed densities); the next call re-allocates. */
+/* Frees the device memory the library keeps between calls (staging matrix,
+ * bordered metric, packed densities and the cached scratch blocks of earlier
+ * calls); the next call re-allocates. */
 int dfhip_release_workspace(void);
 
 /* Metric (P|Q), naux x naux (symmetric). */

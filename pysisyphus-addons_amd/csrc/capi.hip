@@ -55,28 +55,89 @@ thread_local std::string t_msg;
     if (e_ != hipSuccess) throw Error{DFHIP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)}; \
   } while (0)
 
+// Cache of device blocks in power-of-two size classes (256 B .. 256 MiB):
+// blocks a call frees are kept for the next call, which then pays neither
+// hipMalloc nor hipFree (the latter synchronizes the device).  Larger blocks
+// are allocated and freed exactly.  All device work runs on the library's
+// one stream, so a block handed out again is only touched by work queued
+// after the work of its previous owner.  dfhip_release_workspace empties it.
+class DevPool {
+ public:
+  static constexpr int KMIN = 8, KMAX = 28;
+  void* get(size_t bytes, size_t& cap) {
+    std::lock_guard<std::mutex> g(m_);
+    void* p = nullptr;
+    if (bytes > ((size_t)1 << KMAX)) {
+      cap = bytes;
+    } else {
+      int k = KMIN;
+      while (((size_t)1 << k) < bytes) k++;
+      cap = (size_t)1 << k;
+      if (!free_[k].empty()) {
+        p = free_[k].back();
+        free_[k].pop_back();
+        return p;
+      }
+    }
+    if (hipMalloc(&p, cap) != hipSuccess) {
+      // give the cached blocks back to the runtime and try once more
+      release_locked();
+      if (hipMalloc(&p, cap) != hipSuccess) return nullptr;
+    }
+    return p;
+  }
+  void put(void* p, size_t cap) {
+    std::lock_guard<std::mutex> g(m_);
+    if (cap > ((size_t)1 << KMAX)) {
+      (void)hipFree(p);
+      return;
+    }
+    int k = KMIN;
+    while (((size_t)1 << k) < cap) k++;
+    free_[k].push_back(p);
+  }
+  void release() {
+    std::lock_guard<std::mutex> g(m_);
+    release_locked();
+  }
+
+ private:
+  void release_locked() {
+    for (auto& v : free_) {
+      for (void* p : v) (void)hipFree(p);
+      v.clear();
+    }
+  }
+  std::mutex m_;
+  std::vector<void*> free_[KMAX + 1];
+};
+
+DevPool& dev_pool() {
+  static DevPool* pool = new DevPool();  // lives until process end
+  return *pool;
+}
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cap = 0;  // bytes of the pool block
   DBuf() = default;
   explicit DBuf(size_t count) { alloc(count); }
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   void alloc(size_t count) {
     release();
-    n = count;
     if (count == 0) return;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      throw Error{DFHIP_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
-    }
+    p = (T*)dev_pool().get(count * sizeof(T), cap);
+    if (!p) throw Error{DFHIP_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    n = count;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_pool().put(p, cap);
     p = nullptr;
     n = 0;
+    cap = 0;
   }
   ~DBuf() { release(); }
 };
@@ -458,10 +519,18 @@ struct BraRecords {
   DBuf<int64_t> off;
   DBuf<int> npp;
   DBuf<double> recs;
+  DBuf<float> recs32;  // mixed precision only
 };
 
+// FP32 copy of the bra records for the FP32 class kernels.
+__global__ void records_to_f32_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
 void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevShells& dev, const int* ds1,
-                       const int* ds2, const std::vector<int>& s1, const std::vector<int>& s2, BraRecords& br) {
+                       const int* ds2, const std::vector<int>& s1, const std::vector<int>& s2, BraRecords& br,
+                       bool fp32 = false) {
   const int rs = c.gen3[la][lb][0].rs;
   std::vector<int64_t> off(s1.size());
   std::vector<int> npp(s1.size());
@@ -476,15 +545,23 @@ void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevS
   br.recs.alloc(std::max<int64_t>(o, 1));
   launch_bra_records(la, lb, dev, ds1, ds2, br.off.p, rs, (int)s1.size(), br.recs.p, c.st);
   HC(hipGetLastError());
+  if (fp32 && o > 0) {
+    br.recs32.alloc(o);
+    hipLaunchKernelGGL(records_to_f32_kernel, dim3((unsigned)((o + 255) / 256)), dim3(256), 0, c.st, br.recs.p,
+                       br.recs32.p, o);
+    HC(hipGetLastError());
+  }
 }
 
 void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells ket, PairArgs pa, AuxArgs xa,
-                 double* out, int64_t ldo) {
+                 double* out, int64_t ldo, bool fp32 = false) {
   const Gen3Launch& g = c.gen3[la][lb][lc];
   const int nchunks = (xa.hi - xa.lo + 63) / 64;
   if (pa.npairs <= 0 || nchunks <= 0) return;
-  hipLaunchKernelGGL(g.kernel, dim3(pa.npairs, nchunks, g.groups), dim3(64), 0, c.st, br.recs.p, br.off.p, br.npp.p,
-                     ket, pa, xa, c.boys.p, out, ldo);
+  if (fp32 && !br.recs32.p) throw Error{DFHIP_ERR_HIP, "FP32 bra records missing"};
+  hipLaunchKernelGGL(fp32 ? g.kernel32 : g.kernel, dim3(pa.npairs, nchunks, fp32 ? g.groups32 : g.groups), dim3(64),
+                     0, c.st, br.recs.p,
+                     br.recs32.p, br.off.p, br.npp.p, ket, pa, xa, c.boys.p, out, ldo);
   HC(hipGetLastError());
 }
 
@@ -556,6 +633,7 @@ dfhip_options defaults() {
   const char* v = std::getenv("DFHIP_VERBOSE");
   o.verbose = (v && *v && *v != '0') ? 1 : 0;
   o.max_tile_bytes = 0;
+  o.fp32_below = 1e-3;
   return o;
 }
 
@@ -573,7 +651,8 @@ struct Plan {
   std::vector<std::vector<int64_t>> acol;
   std::vector<int> orig;                // permuted column -> original aux function
   std::vector<std::vector<int>> nsurv;  // [L][pair]
-  int64_t ncomputed = 0, ntriplets = 0;
+  std::vector<std::vector<int>> n64;    // [L][pair] FP64 part of the prefix (mixed precision)
+  int64_t ncomputed = 0, ntriplets = 0, ncomputed32 = 0;
   int64_t class_trip[NPCLS * 6] = {}, class_prim[NPCLS * 6] = {};
 };
 
@@ -703,6 +782,29 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   }
 }
 
+// Mixed precision: the FP64 part of each screened prefix, the list entries
+// with Q_ab Q_C >= tau (a prefix again, non-increasing along a class), for the
+// classes with FP32 kernels; the rest of the prefix runs in FP32.
+void split_precision(Plan& pl, double tau, bool gen) {
+  pl.n64 = pl.nsurv;
+  pl.ncomputed32 = 0;
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    const auto& v = pl.alist[L];
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        if (!gen || !gen3_has(la, lb, L)) continue;
+        const int k = pcls(la, lb);
+        int n = (int)v.size();
+        for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+          const double qa = pl.qab[p];
+          while (n > 0 && qa * pl.qc[v[n - 1]] < tau) n--;
+          pl.n64[L][p] = std::min(n, pl.nsurv[L][p]);
+          pl.ncomputed32 += pl.nsurv[L][p] - pl.n64[L][p];
+        }
+      }
+  }
+}
+
 // erfinv(y) for 0 < y < 1 by Newton's method on erf (quadratic convergence
 // from sqrt(-log(1 - y)), which lies above the root).
 double erfinv_pos(double y) {
@@ -783,7 +885,10 @@ struct Outputs {
 
 void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const double* dens,
          const dfhip_options& opt, Outputs out, dfhip_stats* stats) {
-  if (opt.precision != DFHIP_FP64) throw Error{DFHIP_ERR_UNSUPPORTED, "only the FP64 path is available"};
+  if (opt.precision != DFHIP_FP64 && opt.precision != DFHIP_MIXED)
+    throw Error{DFHIP_ERR_UNSUPPORTED, "unknown precision"};
+  const bool mixed = opt.precision == DFHIP_MIXED;
+  if (mixed && !(opt.fp32_below >= 0.0)) throw Error{DFHIP_ERR_ARG, "fp32_below must be >= 0"};
   if (opt.screening != DFHIP_SCREEN_SCHWARZ && opt.screening != DFHIP_SCREEN_NONE &&
       opt.screening != DFHIP_SCREEN_QVL)
     throw Error{DFHIP_ERR_UNSUPPORTED, "unknown screening mode"};
@@ -866,9 +971,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   const bool use_qvl = opt.screening == DFHIP_SCREEN_QVL;
   if (use_qvl) apply_qvl(pl, dorb.view, daux.view, ps1.p, ps2.p, dalist, opt.thresh, qvl, st);
   for (int L = 0; L <= LMAX_AUX; L++) upload(dnsurv[L], pl.nsurv[L], st);
+  const bool gen = use_generated();
+  std::vector<DBuf<int>> dn64(LMAX_AUX + 1);
+  if (mixed) {
+    split_precision(pl, opt.fp32_below, gen);
+    for (int L = 0; L <= LMAX_AUX; L++) upload(dn64[L], pl.n64[L], st);
+  }
   DBuf<int> dorig;
   upload(dorig, pl.orig, st);
-  const bool gen = use_generated();
   std::vector<BraRecords> brs(NPCLS);
   if (gen) {
     for (int la = 0; la <= LMAX_ORB; la++)
@@ -877,7 +987,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         if (p1 == p0 || !gen3_has(la, lb, 0)) continue;
         std::vector<int> a1(pl.ps.s1.begin() + p0, pl.ps.s1.begin() + p1);
         std::vector<int> a2(pl.ps.s2.begin() + p0, pl.ps.s2.begin() + p1);
-        build_bra_records(c, la, lb, o, dorb.view, ps1.p + p0, ps2.p + p0, a1, a2, brs[k]);
+        build_bra_records(c, la, lb, o, dorb.view, ps1.p + p0, ps2.p + p0, a1, a2, brs[k], mixed);
       }
   }
 
@@ -1050,7 +1160,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             xa.qvl = qvl.aux.p;
             xa.thresh = opt.thresh;
           }
-          if (gen && gen3_has(la, lb, L)) {
+          if (gen && gen3_has(la, lb, L) && mixed) {
+            PairArgs p64 = pa, p32 = pa;
+            p64.nsurv = dn64[L].p + p0;  // [0, n64): FP64
+            p32.nbeg = dn64[L].p + p0;   // [n64, nsurv): FP32
+            launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);
+            launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
+          } else if (gen && gen3_has(la, lb, L)) {
             launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
           } else {
             c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, Tp - cA, w, st);
@@ -1138,6 +1254,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     std::memset(stats, 0, sizeof(*stats));
     stats->ntriplets = pl.ntriplets;
     stats->ncomputed = pl.ncomputed;
+    stats->ncomputed_fp32 = pl.ncomputed32;
     stats->nskipped = pl.ntriplets - pl.ncomputed;
     stats->nbf = o.nbf;
     stats->naux = naux;
@@ -1255,6 +1372,7 @@ int dfhip_release_workspace(void) {
     c.wsDp.release();
     c.wsPart.release();
     c.wsDraw.release();
+    dev_pool().release();
   });
 }
 const char* dfhip_last_error(void) { return t_msg.c_str(); }

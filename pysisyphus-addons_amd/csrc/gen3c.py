@@ -16,6 +16,12 @@ wave's shell pair and per auxiliary primitive:
 After the primitive loops the Cartesian (a, b) indices are transformed to
 normalized real solid harmonics and the block is stored.
 
+Each class is emitted twice: int3c2e_gen_* in FP64 and int3c2e_gen32_* for the
+mixed-precision path, where steps 2-4 run in FP32 (bra E-products from an FP32
+copy of the records) while the Boys values, the prefactors and the sum over
+primitive pairs stay FP64 (per primitive pair the FP32 accumulators are added
+into FP64 ones).  Both take the aux-list window [nbeg, nsurv) of each pair.
+
 Usage: python gen3c.py <outdir>   (writes int3c2e_gen_<la><lb>.hip and gen3c_classes.h)
 """
 
@@ -27,7 +33,8 @@ from pathlib import Path
 # classes generated: la <= LA_MAX, lb <= la, lc <= LC_MAX
 LA_MAX = 2
 LC_MAX = 3
-ACC_LIMIT = 20     # Cartesian accumulators per lane (ncart_a ncart_b x m per group)
+ACC_LIMIT = 20     # Cartesian accumulators per lane (ncart_a ncart_b x m per group), FP64
+ACC_LIMIT32 = 40   # the same for the FP32 variants (half the registers per value)
 
 
 def cart(l):
@@ -100,10 +107,10 @@ def fmt(x):
     return r
 
 
-def mgroups(la, lb, lc):
+def mgroups(la, lb, lc, limit=ACC_LIMIT):
     nab = len(cart(la)) * len(cart(lb))
     ncc = 2 * lc + 1
-    mg = max(1, min(ncc, ACC_LIMIT // nab))
+    mg = max(1, min(ncc, limit // nab))
     ng = -(-ncc // mg)
     mg = -(-ncc // ng)
     groups = []
@@ -114,20 +121,28 @@ def mgroups(la, lb, lc):
     return groups
 
 
-def gen_group(la, lb, lc, ms):
-    """Device function body for one group of aux components ms."""
+def fmtr(x, real):
+    return fmt(x) + ("f" if real == "float" else "")
+
+
+def gen_group(la, lb, lc, ms, real="double"):
+    """Device function body for one group of aux components ms, in the
+    arithmetic type real (double or float)."""
     L = la + lb + lc
     lab = la + lb
     H = herm_list(lab)
     hidx = {h: i for i, h in enumerate(H)}
     S = sph_coeffs(lc)
+    f32 = real == "float"
+    FMA = "fmaf" if f32 else "fma"
+    lit = lambda x: fmtr(x, real)
     lines = []
     w = lines.append
     # W accumulators
     nW = len(H) * len(ms)
-    w(f"      double W[{nW}];")
+    w(f"      {real} W[{nW}];")
     w(f"#pragma unroll")
-    w(f"      for (int q = 0; q < {nW}; q++) W[q] = 0.0;")
+    w(f"      for (int q = 0; q < {nW}; q++) W[q] = {lit(0.0)};")
     # memoised inner sums z[v][n] = sum_k c_vk pz[v-2k] g[n-k]
     zdone, ydone = {}, {}
 
@@ -137,9 +152,9 @@ def gen_group(la, lb, lc, ms):
             for k in range(v // 2 + 1):
                 c = ct(v, k)
                 pz = f"pz{v - 2 * k}"
-                terms.append(f"{fmt(c)} * {pz} * g{n - k}" if c != 1 else f"{pz} * g{n - k}")
+                terms.append(f"{lit(c)} * {pz} * g{n - k}" if c != 1 else f"{pz} * g{n - k}")
             name = f"z{v}_{n}"
-            w(f"      const double {name} = {' + '.join(terms)};")
+            w(f"      const {real} {name} = {' + '.join(terms)};")
             zdone[(v, n)] = name
         return zdone[(v, n)]
 
@@ -149,9 +164,9 @@ def gen_group(la, lb, lc, ms):
             for j in range(u // 2 + 1):
                 c = ct(u, j)
                 zn = zname(v, n - j)
-                terms.append(f"{fmt(c)} * py{u - 2 * j} * {zn}" if c != 1 else f"py{u - 2 * j} * {zn}")
+                terms.append(f"{lit(c)} * py{u - 2 * j} * {zn}" if c != 1 else f"py{u - 2 * j} * {zn}")
             name = f"y{u}_{v}_{n}"
-            w(f"      const double {name} = {' + '.join(terms)};")
+            w(f"      const {real} {name} = {' + '.join(terms)};")
             ydone[(u, v, n)] = name
         return ydone[(u, v, n)]
 
@@ -198,21 +213,23 @@ def gen_group(la, lb, lc, ms):
         for i in range(t // 2 + 1):
             c = ct(t, i)
             yn = yname(u, v, t + u + v - i)
-            terms.append(f"{fmt(c)} * px{t - 2 * i} * {yn}" if c != 1 else f"px{t - 2 * i} * {yn}")
-        w(f"      {{ const double r = {' + '.join(terms)};")
+            terms.append(f"{lit(c)} * px{t - 2 * i} * {yn}" if c != 1 else f"px{t - 2 * i} * {yn}")
+        w(f"      {{ const {real} r = {' + '.join(terms)};")
         for q, c in consumers:
-            w(f"        W[{q}] = fma({fmt(c)}, r, W[{q}]);")
+            w(f"        W[{q}] = {FMA}({lit(c)}, r, W[{q}]);")
         w("      }")
     if cur_v is not None:
         w("      }")
         w("      }")
     # contraction with the uniform bra record
     nb = len(cart(lb))
+    acc, rec = ("accf", "rf") if f32 else ("acc", "rec")
     for k, (ia, ib, t, u, v) in enumerate(record_layout(la, lb)):
         h = hidx[(t, u, v)]
-        w(f"      {{ const double e = rec[{4 + k}];")
+        w(f"      {{ const {real} e = {rec}[{4 + k}];")
         for mi in range(len(ms)):
-            w(f"        acc[{(ia * nb + ib) * len(ms) + mi}] = fma(e, W[{h * len(ms) + mi}], acc[{(ia * nb + ib) * len(ms) + mi}]);")
+            q = (ia * nb + ib) * len(ms) + mi
+            w(f"        {acc}[{q}] = {FMA}(e, W[{h * len(ms) + mi}], {acc}[{q}]);")
         w("      }")
     return lines
 
@@ -238,29 +255,34 @@ def gen_store(la, lb, lc, ms):
     return lines
 
 
-def gen_class(la, lb, lc):
+def gen_class(la, lb, lc, real="double"):
     L = la + lb + lc
-    groups = mgroups(la, lb, lc)
+    groups = mgroups(la, lb, lc, ACC_LIMIT32 if real == "float" else ACC_LIMIT)
     na, nb = len(cart(la)), len(cart(lb))
     rs = 4 + len(record_layout(la, lb))
+    f32 = real == "float"
+    lit = lambda x: fmtr(x, real)
     out = []
     w = out.append
-    name = f"int3c2e_gen_{la}{lb}{lc}"
-    w(f"// class ({la}{lb}|{lc}): L = {L}, record {rs} doubles, {len(groups)} m-group(s)")
-    w(f"__global__ __launch_bounds__(64) void {name}(const double* __restrict__ recs, const int64_t* __restrict__ recoff,")
-    w(f"    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,")
-    w(f"    double* __restrict__ out, int64_t ldo) {{")
+    name = f"int3c2e_gen{'32' if f32 else ''}_{la}{lb}{lc}"
+    w(f"// class ({la}{lb}|{lc}), {'FP32' if f32 else 'FP64'}: L = {L}, record {rs} values, {len(groups)} m-group(s)")
+    w(f"__global__ __launch_bounds__(64) void {name}(const double* __restrict__ recs, const float* __restrict__ recs32,")
+    w(f"    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,")
+    w(f"    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {{")
     w(f"  const int pidx = blockIdx.x;")
     w(f"  const int cend = min(xa.hi, pa.nsurv[pidx]);")
-    w(f"  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;")
-    w(f"  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;")
-    w(f"  const bool active = c < cend;")
+    w(f"  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;")
+    w(f"  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;")
+    w(f"  if (c0 >= cend || c0 + 64 <= cbeg) return;")
+    w(f"  const bool active = c >= cbeg && c < cend;")
     w(f"  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];")
     w(f"  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];")
     w(f"  const bool live = active && triplet_kept(pa, xa, pidx, sh, {lc}, Cx, Cy, Cz);")
     w(f"  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];")
     w(f"  const int npp = npp_v[pidx];")
     w(f"  const double* __restrict__ recp = recs + recoff[pidx];")
+    if f32:
+        w(f"  const float* __restrict__ recp32 = recs32 + recoff[pidx];")
     w(f"  const int64_t row0 = pa.row[pidx];")
     w(f"  const int64_t col = active ? xa.col[c] : 0;")
     w(f"  switch (blockIdx.z) {{")
@@ -273,26 +295,41 @@ def gen_class(la, lb, lc):
         w(f"    for (int ij = 0; ij < npp; ij++) {{")
         w(f"      const double* __restrict__ rec = recp + (size_t)ij * {rs};")
         w(f"      const double p = rec[0];")
-        w(f"      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;")
+        if f32:
+            w(f"      const float* __restrict__ rf = recp32 + (size_t)ij * {rs};")
+            w(f"      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;")
+            w(f"      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;")
+            w(f"      float accf[{nacc}];")
+            w(f"#pragma unroll")
+            w(f"      for (int q = 0; q < {nacc}; q++) accf[q] = 0.0f;")
+        else:
+            w(f"      const double PCx = rec[1] - Cx, PCy = rec[2] - Cy, PCz = rec[3] - Cz;")
         w(f"      for (int kp = 0; kp < nprim_c; kp++) {{")
         w(f"      const double gam = ket.exps[pc0 + kp];")
         w(f"      const double alpha = p * gam / (p + gam);")
-        w(f"      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);")
+        if f32:
+            w(f"      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);")
+        else:
+            w(f"      const double T = alpha * (PCx * PCx + PCy * PCy + PCz * PCz);")
         w(f"      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];")
         if lc > 0:
             w(f"      {{ const double i2g = 0.5 / gam; sc *= {'-' if lc % 2 else ''}{' * '.join(['i2g'] * lc)}; }}")
         w(f"      double F[{L + 1}];")
         w(f"      boys_array<{L}>(T, boys, F);")
         w(f"      const double ma = -alpha;")
-        w(f"      const double g0 = sc * F[0];")
+        cvt = "(float)" if f32 else ""
+        w(f"      const {real} g0 = {cvt}(sc * F[0]);")
         for n in range(1, L + 1):
-            w(f"      sc *= ma; const double g{n} = sc * F[{n}];")
+            w(f"      sc *= ma; const {real} g{n} = {cvt}(sc * F[{n}]);")
         for d, X in (("x", "PCx"), ("y", "PCy"), ("z", "PCz")):
-            w(f"      const double p{d}0 = 1.0, p{d}1 = 2.0 * {X};")
+            w(f"      const {real} p{d}0 = {lit(1.0)}, p{d}1 = {lit(2.0)} * {X};")
             for a in range(2, L + 1):
-                w(f"      const double p{d}{a} = p{d}{a - 1} * p{d}1;")
-        out.extend(gen_group(la, lb, lc, ms))
+                w(f"      const {real} p{d}{a} = p{d}{a - 1} * p{d}1;")
+        out.extend(gen_group(la, lb, lc, ms, real))
         w(f"      }}")  # kp
+        if f32:
+            w(f"#pragma unroll")
+            w(f"      for (int q = 0; q < {nacc}; q++) acc[q] += (double)accf[q];")
         w(f"    }}")  # ij
         w(f"    if (active) {{")
         out.extend(gen_store(la, lb, lc, ms))
@@ -317,7 +354,10 @@ def main(outdir):
             for lc in range(LC_MAX + 1):
                 body, G, rs = gen_class(la, lb, lc)
                 src.extend(body)
-                regs.append(f"  t[{la}][{lb}][{lc}] = Gen3Launch{{&int3c2e_gen_{la}{lb}{lc}, {G}, {rs}}};")
+                body32, G32, _ = gen_class(la, lb, lc, "float")
+                src.extend(body32)
+                regs.append(f"  t[{la}][{lb}][{lc}] = Gen3Launch{{&int3c2e_gen_{la}{lb}{lc}, "
+                            f"&int3c2e_gen32_{la}{lb}{lc}, {G}, {G32}, {rs}}};")
             src.append(f"void register_gen3_{la}{lb}(Gen3Table& t) {{")
             src.extend(regs)
             src.append("}")

@@ -74,14 +74,17 @@ __global__ __launch_bounds__(64) void bra_records_kernel(DevShells bra, const in
   }
 }
 
-using Gen3Kernel = void (*)(const double*, const int64_t*, const int*, DevShells, PairArgs, AuxArgs, const double*,
-                            double*, int64_t);
+using Gen3Kernel = void (*)(const double*, const float*, const int64_t*, const int*, DevShells, PairArgs, AuxArgs,
+                            const double*, double*, int64_t);
 
 struct Gen3Launch {
-  Gen3Kernel kernel;
-  int groups;
+  Gen3Kernel kernel;    // FP64
+  Gen3Kernel kernel32;  // FP32 integral arithmetic, FP64 Boys values and primitive-pair sums
+  int groups;           // aux-component groups (grid z) of kernel
+  int groups32;         // and of kernel32
   int rs;
 };
+
 
 using Gen3Table = Gen3Launch[GEN3_LA_MAX + 1][GEN3_LA_MAX + 1][GEN3_LC_MAX + 1];
 

@@ -3,15 +3,16 @@
 
 namespace dfhip {
 
-// class (11|0): L = 2, record 37 doubles, 1 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
-    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
-    double* __restrict__ out, int64_t ldo) {
+// class (11|0), FP64: L = 2, record 37 values, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
-  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
-  const bool active = c < cend;
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
   const bool live = active && triplet_kept(pa, xa, pidx, sh, 0, Cx, Cy, Cz);
@@ -37,9 +38,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__
       double F[3];
       boys_array<2>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double py0 = 1.0, py1 = 2.0 * PCy;
@@ -231,15 +232,253 @@ __global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__
   }
 }
 
-// class (11|1): L = 3, record 37 doubles, 2 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
-    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
-    double* __restrict__ out, int64_t ldo) {
+// class (11|0), FP32: L = 2, record 37 values, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen32_110(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
-  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
-  const bool active = c < cend;
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 0, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const float* __restrict__ recp32 = recs32 + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const float* __restrict__ rf = recp32 + (size_t)ij * 37;
+      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;
+      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;
+      float accf[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) accf[q] = 0.0f;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      double F[3];
+      boys_array<2>(T, boys, F);
+      const double ma = -alpha;
+      const float g0 = (float)(sc * F[0]);
+      sc *= ma; const float g1 = (float)(sc * F[1]);
+      sc *= ma; const float g2 = (float)(sc * F[2]);
+      const float px0 = 1.0f, px1 = 2.0f * PCx;
+      const float px2 = px1 * px1;
+      const float py0 = 1.0f, py1 = 2.0f * PCy;
+      const float py2 = py1 * py1;
+      const float pz0 = 1.0f, pz1 = 2.0f * PCz;
+      const float pz2 = pz1 * pz1;
+      float W[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) W[q] = 0.0f;
+      {
+      const float z0_0 = pz0 * g0;
+      const float z0_1 = pz0 * g1;
+      const float z0_2 = pz0 * g2;
+      {
+      const float y0_0_0 = py0 * z0_0;
+      { const float r = px0 * y0_0_0;
+        W[0] = fmaf(1.0f, r, W[0]);
+      }
+      const float y0_0_1 = py0 * z0_1;
+      { const float r = px1 * y0_0_1;
+        W[1] = fmaf(1.0f, r, W[1]);
+      }
+      const float y0_0_2 = py0 * z0_2;
+      { const float r = px2 * y0_0_2 + 2.0f * px0 * y0_0_1;
+        W[4] = fmaf(1.0f, r, W[4]);
+      }
+      }
+      {
+      const float y1_0_1 = py1 * z0_1;
+      { const float r = px0 * y1_0_1;
+        W[2] = fmaf(1.0f, r, W[2]);
+      }
+      const float y1_0_2 = py1 * z0_2;
+      { const float r = px1 * y1_0_2;
+        W[5] = fmaf(1.0f, r, W[5]);
+      }
+      }
+      {
+      const float y2_0_2 = py2 * z0_2 + 2.0f * py0 * z0_1;
+      { const float r = px0 * y2_0_2;
+        W[7] = fmaf(1.0f, r, W[7]);
+      }
+      }
+      }
+      {
+      const float z1_1 = pz1 * g1;
+      const float z1_2 = pz1 * g2;
+      {
+      const float y0_1_1 = py0 * z1_1;
+      { const float r = px0 * y0_1_1;
+        W[3] = fmaf(1.0f, r, W[3]);
+      }
+      const float y0_1_2 = py0 * z1_2;
+      { const float r = px1 * y0_1_2;
+        W[6] = fmaf(1.0f, r, W[6]);
+      }
+      }
+      {
+      const float y1_1_2 = py1 * z1_2;
+      { const float r = px0 * y1_1_2;
+        W[8] = fmaf(1.0f, r, W[8]);
+      }
+      }
+      }
+      {
+      const float z2_2 = pz2 * g2 + 2.0f * pz0 * g1;
+      {
+      const float y0_2_2 = py0 * z2_2;
+      { const float r = px0 * y0_2_2;
+        W[9] = fmaf(1.0f, r, W[9]);
+      }
+      }
+      }
+      { const float e = rf[4];
+        accf[0] = fmaf(e, W[0], accf[0]);
+      }
+      { const float e = rf[5];
+        accf[0] = fmaf(e, W[1], accf[0]);
+      }
+      { const float e = rf[6];
+        accf[0] = fmaf(e, W[4], accf[0]);
+      }
+      { const float e = rf[7];
+        accf[1] = fmaf(e, W[0], accf[1]);
+      }
+      { const float e = rf[8];
+        accf[1] = fmaf(e, W[2], accf[1]);
+      }
+      { const float e = rf[9];
+        accf[1] = fmaf(e, W[1], accf[1]);
+      }
+      { const float e = rf[10];
+        accf[1] = fmaf(e, W[5], accf[1]);
+      }
+      { const float e = rf[11];
+        accf[2] = fmaf(e, W[0], accf[2]);
+      }
+      { const float e = rf[12];
+        accf[2] = fmaf(e, W[3], accf[2]);
+      }
+      { const float e = rf[13];
+        accf[2] = fmaf(e, W[1], accf[2]);
+      }
+      { const float e = rf[14];
+        accf[2] = fmaf(e, W[6], accf[2]);
+      }
+      { const float e = rf[15];
+        accf[3] = fmaf(e, W[0], accf[3]);
+      }
+      { const float e = rf[16];
+        accf[3] = fmaf(e, W[2], accf[3]);
+      }
+      { const float e = rf[17];
+        accf[3] = fmaf(e, W[1], accf[3]);
+      }
+      { const float e = rf[18];
+        accf[3] = fmaf(e, W[5], accf[3]);
+      }
+      { const float e = rf[19];
+        accf[4] = fmaf(e, W[0], accf[4]);
+      }
+      { const float e = rf[20];
+        accf[4] = fmaf(e, W[2], accf[4]);
+      }
+      { const float e = rf[21];
+        accf[4] = fmaf(e, W[7], accf[4]);
+      }
+      { const float e = rf[22];
+        accf[5] = fmaf(e, W[0], accf[5]);
+      }
+      { const float e = rf[23];
+        accf[5] = fmaf(e, W[3], accf[5]);
+      }
+      { const float e = rf[24];
+        accf[5] = fmaf(e, W[2], accf[5]);
+      }
+      { const float e = rf[25];
+        accf[5] = fmaf(e, W[8], accf[5]);
+      }
+      { const float e = rf[26];
+        accf[6] = fmaf(e, W[0], accf[6]);
+      }
+      { const float e = rf[27];
+        accf[6] = fmaf(e, W[3], accf[6]);
+      }
+      { const float e = rf[28];
+        accf[6] = fmaf(e, W[1], accf[6]);
+      }
+      { const float e = rf[29];
+        accf[6] = fmaf(e, W[6], accf[6]);
+      }
+      { const float e = rf[30];
+        accf[7] = fmaf(e, W[0], accf[7]);
+      }
+      { const float e = rf[31];
+        accf[7] = fmaf(e, W[3], accf[7]);
+      }
+      { const float e = rf[32];
+        accf[7] = fmaf(e, W[2], accf[7]);
+      }
+      { const float e = rf[33];
+        accf[7] = fmaf(e, W[8], accf[7]);
+      }
+      { const float e = rf[34];
+        accf[8] = fmaf(e, W[0], accf[8]);
+      }
+      { const float e = rf[35];
+        accf[8] = fmaf(e, W[3], accf[8]);
+      }
+      { const float e = rf[36];
+        accf[8] = fmaf(e, W[9], accf[8]);
+      }
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc[q] += (double)accf[q];
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[4];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[5];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[3];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[7];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[8];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[1];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[2];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (11|1), FP64: L = 3, record 37 values, 2 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
   const bool live = active && triplet_kept(pa, xa, pidx, sh, 1, Cx, Cy, Cz);
@@ -266,10 +505,10 @@ __global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__
       double F[4];
       boys_array<3>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -561,10 +800,10 @@ __global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__
       double F[4];
       boys_array<3>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -773,15 +1012,403 @@ __global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__
   }
 }
 
-// class (11|2): L = 4, record 37 doubles, 3 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
-    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
-    double* __restrict__ out, int64_t ldo) {
+// class (11|1), FP32: L = 3, record 37 values, 1 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen32_111(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
-  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
-  const bool active = c < cend;
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 1, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const float* __restrict__ recp32 = recs32 + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const float* __restrict__ rf = recp32 + (size_t)ij * 37;
+      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;
+      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;
+      float accf[27];
+#pragma unroll
+      for (int q = 0; q < 27; q++) accf[q] = 0.0f;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g; }
+      double F[4];
+      boys_array<3>(T, boys, F);
+      const double ma = -alpha;
+      const float g0 = (float)(sc * F[0]);
+      sc *= ma; const float g1 = (float)(sc * F[1]);
+      sc *= ma; const float g2 = (float)(sc * F[2]);
+      sc *= ma; const float g3 = (float)(sc * F[3]);
+      const float px0 = 1.0f, px1 = 2.0f * PCx;
+      const float px2 = px1 * px1;
+      const float px3 = px2 * px1;
+      const float py0 = 1.0f, py1 = 2.0f * PCy;
+      const float py2 = py1 * py1;
+      const float py3 = py2 * py1;
+      const float pz0 = 1.0f, pz1 = 2.0f * PCz;
+      const float pz2 = pz1 * pz1;
+      const float pz3 = pz2 * pz1;
+      float W[30];
+#pragma unroll
+      for (int q = 0; q < 30; q++) W[q] = 0.0f;
+      {
+      const float z0_0 = pz0 * g0;
+      const float z0_1 = pz0 * g1;
+      const float z0_2 = pz0 * g2;
+      const float z0_3 = pz0 * g3;
+      {
+      const float y0_0_1 = py0 * z0_1;
+      { const float r = px1 * y0_0_1;
+        W[2] = fmaf(1.0f, r, W[2]);
+      }
+      const float y0_0_2 = py0 * z0_2;
+      { const float r = px2 * y0_0_2 + 2.0f * px0 * y0_0_1;
+        W[5] = fmaf(1.0f, r, W[5]);
+      }
+      const float y0_0_3 = py0 * z0_3;
+      { const float r = px3 * y0_0_3 + 6.0f * px1 * y0_0_2;
+        W[14] = fmaf(1.0f, r, W[14]);
+      }
+      }
+      {
+      const float y1_0_1 = py1 * z0_1;
+      { const float r = px0 * y1_0_1;
+        W[0] = fmaf(1.0f, r, W[0]);
+      }
+      const float y1_0_2 = py1 * z0_2;
+      { const float r = px1 * y1_0_2;
+        W[3] = fmaf(1.0f, r, W[3]);
+        W[8] = fmaf(1.0f, r, W[8]);
+      }
+      const float y1_0_3 = py1 * z0_3;
+      { const float r = px2 * y1_0_3 + 2.0f * px0 * y1_0_2;
+        W[12] = fmaf(1.0f, r, W[12]);
+        W[17] = fmaf(1.0f, r, W[17]);
+      }
+      }
+      {
+      const float y2_0_2 = py2 * z0_2 + 2.0f * py0 * z0_1;
+      { const float r = px0 * y2_0_2;
+        W[6] = fmaf(1.0f, r, W[6]);
+      }
+      const float y2_0_3 = py2 * z0_3 + 2.0f * py0 * z0_2;
+      { const float r = px1 * y2_0_3;
+        W[15] = fmaf(1.0f, r, W[15]);
+        W[23] = fmaf(1.0f, r, W[23]);
+      }
+      }
+      {
+      const float y3_0_3 = py3 * z0_3 + 6.0f * py1 * z0_2;
+      { const float r = px0 * y3_0_3;
+        W[21] = fmaf(1.0f, r, W[21]);
+      }
+      }
+      }
+      {
+      const float z1_1 = pz1 * g1;
+      const float z1_2 = pz1 * g2;
+      const float z1_3 = pz1 * g3;
+      {
+      const float y0_1_1 = py0 * z1_1;
+      { const float r = px0 * y0_1_1;
+        W[1] = fmaf(1.0f, r, W[1]);
+      }
+      const float y0_1_2 = py0 * z1_2;
+      { const float r = px1 * y0_1_2;
+        W[4] = fmaf(1.0f, r, W[4]);
+        W[11] = fmaf(1.0f, r, W[11]);
+      }
+      const float y0_1_3 = py0 * z1_3;
+      { const float r = px2 * y0_1_3 + 2.0f * px0 * y0_1_2;
+        W[13] = fmaf(1.0f, r, W[13]);
+        W[20] = fmaf(1.0f, r, W[20]);
+      }
+      }
+      {
+      const float y1_1_2 = py1 * z1_2;
+      { const float r = px0 * y1_1_2;
+        W[9] = fmaf(1.0f, r, W[9]);
+        W[7] = fmaf(1.0f, r, W[7]);
+      }
+      const float y1_1_3 = py1 * z1_3;
+      { const float r = px1 * y1_1_3;
+        W[18] = fmaf(1.0f, r, W[18]);
+        W[16] = fmaf(1.0f, r, W[16]);
+        W[26] = fmaf(1.0f, r, W[26]);
+      }
+      }
+      {
+      const float y2_1_3 = py2 * z1_3 + 2.0f * py0 * z1_2;
+      { const float r = px0 * y2_1_3;
+        W[24] = fmaf(1.0f, r, W[24]);
+        W[22] = fmaf(1.0f, r, W[22]);
+      }
+      }
+      }
+      {
+      const float z2_2 = pz2 * g2 + 2.0f * pz0 * g1;
+      const float z2_3 = pz2 * g3 + 2.0f * pz0 * g2;
+      {
+      const float y0_2_2 = py0 * z2_2;
+      { const float r = px0 * y0_2_2;
+        W[10] = fmaf(1.0f, r, W[10]);
+      }
+      const float y0_2_3 = py0 * z2_3;
+      { const float r = px1 * y0_2_3;
+        W[19] = fmaf(1.0f, r, W[19]);
+        W[29] = fmaf(1.0f, r, W[29]);
+      }
+      }
+      {
+      const float y1_2_3 = py1 * z2_3;
+      { const float r = px0 * y1_2_3;
+        W[27] = fmaf(1.0f, r, W[27]);
+        W[25] = fmaf(1.0f, r, W[25]);
+      }
+      }
+      }
+      {
+      const float z3_3 = pz3 * g3 + 6.0f * pz1 * g2;
+      {
+      const float y0_3_3 = py0 * z3_3;
+      { const float r = px0 * y0_3_3;
+        W[28] = fmaf(1.0f, r, W[28]);
+      }
+      }
+      }
+      { const float e = rf[4];
+        accf[0] = fmaf(e, W[0], accf[0]);
+        accf[1] = fmaf(e, W[1], accf[1]);
+        accf[2] = fmaf(e, W[2], accf[2]);
+      }
+      { const float e = rf[5];
+        accf[0] = fmaf(e, W[3], accf[0]);
+        accf[1] = fmaf(e, W[4], accf[1]);
+        accf[2] = fmaf(e, W[5], accf[2]);
+      }
+      { const float e = rf[6];
+        accf[0] = fmaf(e, W[12], accf[0]);
+        accf[1] = fmaf(e, W[13], accf[1]);
+        accf[2] = fmaf(e, W[14], accf[2]);
+      }
+      { const float e = rf[7];
+        accf[3] = fmaf(e, W[0], accf[3]);
+        accf[4] = fmaf(e, W[1], accf[4]);
+        accf[5] = fmaf(e, W[2], accf[5]);
+      }
+      { const float e = rf[8];
+        accf[3] = fmaf(e, W[6], accf[3]);
+        accf[4] = fmaf(e, W[7], accf[4]);
+        accf[5] = fmaf(e, W[8], accf[5]);
+      }
+      { const float e = rf[9];
+        accf[3] = fmaf(e, W[3], accf[3]);
+        accf[4] = fmaf(e, W[4], accf[4]);
+        accf[5] = fmaf(e, W[5], accf[5]);
+      }
+      { const float e = rf[10];
+        accf[3] = fmaf(e, W[15], accf[3]);
+        accf[4] = fmaf(e, W[16], accf[4]);
+        accf[5] = fmaf(e, W[17], accf[5]);
+      }
+      { const float e = rf[11];
+        accf[6] = fmaf(e, W[0], accf[6]);
+        accf[7] = fmaf(e, W[1], accf[7]);
+        accf[8] = fmaf(e, W[2], accf[8]);
+      }
+      { const float e = rf[12];
+        accf[6] = fmaf(e, W[9], accf[6]);
+        accf[7] = fmaf(e, W[10], accf[7]);
+        accf[8] = fmaf(e, W[11], accf[8]);
+      }
+      { const float e = rf[13];
+        accf[6] = fmaf(e, W[3], accf[6]);
+        accf[7] = fmaf(e, W[4], accf[7]);
+        accf[8] = fmaf(e, W[5], accf[8]);
+      }
+      { const float e = rf[14];
+        accf[6] = fmaf(e, W[18], accf[6]);
+        accf[7] = fmaf(e, W[19], accf[7]);
+        accf[8] = fmaf(e, W[20], accf[8]);
+      }
+      { const float e = rf[15];
+        accf[9] = fmaf(e, W[0], accf[9]);
+        accf[10] = fmaf(e, W[1], accf[10]);
+        accf[11] = fmaf(e, W[2], accf[11]);
+      }
+      { const float e = rf[16];
+        accf[9] = fmaf(e, W[6], accf[9]);
+        accf[10] = fmaf(e, W[7], accf[10]);
+        accf[11] = fmaf(e, W[8], accf[11]);
+      }
+      { const float e = rf[17];
+        accf[9] = fmaf(e, W[3], accf[9]);
+        accf[10] = fmaf(e, W[4], accf[10]);
+        accf[11] = fmaf(e, W[5], accf[11]);
+      }
+      { const float e = rf[18];
+        accf[9] = fmaf(e, W[15], accf[9]);
+        accf[10] = fmaf(e, W[16], accf[10]);
+        accf[11] = fmaf(e, W[17], accf[11]);
+      }
+      { const float e = rf[19];
+        accf[12] = fmaf(e, W[0], accf[12]);
+        accf[13] = fmaf(e, W[1], accf[13]);
+        accf[14] = fmaf(e, W[2], accf[14]);
+      }
+      { const float e = rf[20];
+        accf[12] = fmaf(e, W[6], accf[12]);
+        accf[13] = fmaf(e, W[7], accf[13]);
+        accf[14] = fmaf(e, W[8], accf[14]);
+      }
+      { const float e = rf[21];
+        accf[12] = fmaf(e, W[21], accf[12]);
+        accf[13] = fmaf(e, W[22], accf[13]);
+        accf[14] = fmaf(e, W[23], accf[14]);
+      }
+      { const float e = rf[22];
+        accf[15] = fmaf(e, W[0], accf[15]);
+        accf[16] = fmaf(e, W[1], accf[16]);
+        accf[17] = fmaf(e, W[2], accf[17]);
+      }
+      { const float e = rf[23];
+        accf[15] = fmaf(e, W[9], accf[15]);
+        accf[16] = fmaf(e, W[10], accf[16]);
+        accf[17] = fmaf(e, W[11], accf[17]);
+      }
+      { const float e = rf[24];
+        accf[15] = fmaf(e, W[6], accf[15]);
+        accf[16] = fmaf(e, W[7], accf[16]);
+        accf[17] = fmaf(e, W[8], accf[17]);
+      }
+      { const float e = rf[25];
+        accf[15] = fmaf(e, W[24], accf[15]);
+        accf[16] = fmaf(e, W[25], accf[16]);
+        accf[17] = fmaf(e, W[26], accf[17]);
+      }
+      { const float e = rf[26];
+        accf[18] = fmaf(e, W[0], accf[18]);
+        accf[19] = fmaf(e, W[1], accf[19]);
+        accf[20] = fmaf(e, W[2], accf[20]);
+      }
+      { const float e = rf[27];
+        accf[18] = fmaf(e, W[9], accf[18]);
+        accf[19] = fmaf(e, W[10], accf[19]);
+        accf[20] = fmaf(e, W[11], accf[20]);
+      }
+      { const float e = rf[28];
+        accf[18] = fmaf(e, W[3], accf[18]);
+        accf[19] = fmaf(e, W[4], accf[19]);
+        accf[20] = fmaf(e, W[5], accf[20]);
+      }
+      { const float e = rf[29];
+        accf[18] = fmaf(e, W[18], accf[18]);
+        accf[19] = fmaf(e, W[19], accf[19]);
+        accf[20] = fmaf(e, W[20], accf[20]);
+      }
+      { const float e = rf[30];
+        accf[21] = fmaf(e, W[0], accf[21]);
+        accf[22] = fmaf(e, W[1], accf[22]);
+        accf[23] = fmaf(e, W[2], accf[23]);
+      }
+      { const float e = rf[31];
+        accf[21] = fmaf(e, W[9], accf[21]);
+        accf[22] = fmaf(e, W[10], accf[22]);
+        accf[23] = fmaf(e, W[11], accf[23]);
+      }
+      { const float e = rf[32];
+        accf[21] = fmaf(e, W[6], accf[21]);
+        accf[22] = fmaf(e, W[7], accf[22]);
+        accf[23] = fmaf(e, W[8], accf[23]);
+      }
+      { const float e = rf[33];
+        accf[21] = fmaf(e, W[24], accf[21]);
+        accf[22] = fmaf(e, W[25], accf[22]);
+        accf[23] = fmaf(e, W[26], accf[23]);
+      }
+      { const float e = rf[34];
+        accf[24] = fmaf(e, W[0], accf[24]);
+        accf[25] = fmaf(e, W[1], accf[25]);
+        accf[26] = fmaf(e, W[2], accf[26]);
+      }
+      { const float e = rf[35];
+        accf[24] = fmaf(e, W[9], accf[24]);
+        accf[25] = fmaf(e, W[10], accf[25]);
+        accf[26] = fmaf(e, W[11], accf[26]);
+      }
+      { const float e = rf[36];
+        accf[24] = fmaf(e, W[27], accf[24]);
+        accf[25] = fmaf(e, W[28], accf[25]);
+        accf[26] = fmaf(e, W[29], accf[26]);
+      }
+      }
+#pragma unroll
+      for (int q = 0; q < 27; q++) acc[q] += (double)accf[q];
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[12];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[13];
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[14];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[15];
+    out[(row0 + 1) * ldo + col + 1] = 1.0 * acc[16];
+    out[(row0 + 1) * ldo + col + 2] = 1.0 * acc[17];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[9];
+    out[(row0 + 2) * ldo + col + 1] = 1.0 * acc[10];
+    out[(row0 + 2) * ldo + col + 2] = 1.0 * acc[11];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[21];
+    out[(row0 + 3) * ldo + col + 1] = 1.0 * acc[22];
+    out[(row0 + 3) * ldo + col + 2] = 1.0 * acc[23];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[24];
+    out[(row0 + 4) * ldo + col + 1] = 1.0 * acc[25];
+    out[(row0 + 4) * ldo + col + 2] = 1.0 * acc[26];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[18];
+    out[(row0 + 5) * ldo + col + 1] = 1.0 * acc[19];
+    out[(row0 + 5) * ldo + col + 2] = 1.0 * acc[20];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[3];
+    out[(row0 + 6) * ldo + col + 1] = 1.0 * acc[4];
+    out[(row0 + 6) * ldo + col + 2] = 1.0 * acc[5];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 7) * ldo + col + 1] = 1.0 * acc[7];
+    out[(row0 + 7) * ldo + col + 2] = 1.0 * acc[8];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 1] = 1.0 * acc[1];
+    out[(row0 + 8) * ldo + col + 2] = 1.0 * acc[2];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (11|2), FP64: L = 4, record 37 values, 3 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
   const bool live = active && triplet_kept(pa, xa, pidx, sh, 2, Cx, Cy, Cz);
@@ -808,11 +1435,11 @@ __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__
       double F[5];
       boys_array<4>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -1124,11 +1751,11 @@ __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__
       double F[5];
       boys_array<4>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -1501,11 +2128,11 @@ __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__
       double F[5];
       boys_array<4>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -1776,15 +2403,838 @@ __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__
   }
 }
 
-// class (11|3): L = 5, record 37 doubles, 4 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__ recs, const int64_t* __restrict__ recoff,
-    const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa, const double* __restrict__ boys,
-    double* __restrict__ out, int64_t ldo) {
+// class (11|2), FP32: L = 4, record 37 values, 2 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen32_112(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c = xa.lo + (int)blockIdx.y * 64 + (int)threadIdx.x;
-  if (xa.lo + (int)blockIdx.y * 64 >= cend) return;
-  const bool active = c < cend;
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 2, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const float* __restrict__ recp32 = recs32 + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const float* __restrict__ rf = recp32 + (size_t)ij * 37;
+      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;
+      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;
+      float accf[27];
+#pragma unroll
+      for (int q = 0; q < 27; q++) accf[q] = 0.0f;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= i2g * i2g; }
+      double F[5];
+      boys_array<4>(T, boys, F);
+      const double ma = -alpha;
+      const float g0 = (float)(sc * F[0]);
+      sc *= ma; const float g1 = (float)(sc * F[1]);
+      sc *= ma; const float g2 = (float)(sc * F[2]);
+      sc *= ma; const float g3 = (float)(sc * F[3]);
+      sc *= ma; const float g4 = (float)(sc * F[4]);
+      const float px0 = 1.0f, px1 = 2.0f * PCx;
+      const float px2 = px1 * px1;
+      const float px3 = px2 * px1;
+      const float px4 = px3 * px1;
+      const float py0 = 1.0f, py1 = 2.0f * PCy;
+      const float py2 = py1 * py1;
+      const float py3 = py2 * py1;
+      const float py4 = py3 * py1;
+      const float pz0 = 1.0f, pz1 = 2.0f * PCz;
+      const float pz2 = pz1 * pz1;
+      const float pz3 = pz2 * pz1;
+      const float pz4 = pz3 * pz1;
+      float W[30];
+#pragma unroll
+      for (int q = 0; q < 30; q++) W[q] = 0.0f;
+      {
+      const float z0_0 = pz0 * g0;
+      const float z0_1 = pz0 * g1;
+      const float z0_2 = pz0 * g2;
+      const float z0_3 = pz0 * g3;
+      const float z0_4 = pz0 * g4;
+      {
+      const float y0_0_2 = py0 * z0_2;
+      const float y0_0_1 = py0 * z0_1;
+      { const float r = px2 * y0_0_2 + 2.0f * px0 * y0_0_1;
+        W[2] = fmaf(-0.2886751345948129f, r, W[2]);
+      }
+      const float y0_0_3 = py0 * z0_3;
+      { const float r = px3 * y0_0_3 + 6.0f * px1 * y0_0_2;
+        W[5] = fmaf(-0.2886751345948129f, r, W[5]);
+      }
+      const float y0_0_4 = py0 * z0_4;
+      { const float r = px4 * y0_0_4 + 12.0f * px2 * y0_0_3 + 12.0f * px0 * y0_0_2;
+        W[14] = fmaf(-0.2886751345948129f, r, W[14]);
+      }
+      }
+      {
+      const float y1_0_2 = py1 * z0_2;
+      { const float r = px1 * y1_0_2;
+        W[0] = fmaf(1.0f, r, W[0]);
+      }
+      const float y1_0_3 = py1 * z0_3;
+      { const float r = px2 * y1_0_3 + 2.0f * px0 * y1_0_2;
+        W[3] = fmaf(1.0f, r, W[3]);
+        W[8] = fmaf(-0.2886751345948129f, r, W[8]);
+      }
+      const float y1_0_4 = py1 * z0_4;
+      { const float r = px3 * y1_0_4 + 6.0f * px1 * y1_0_3;
+        W[12] = fmaf(1.0f, r, W[12]);
+        W[17] = fmaf(-0.2886751345948129f, r, W[17]);
+      }
+      }
+      {
+      const float y2_0_2 = py2 * z0_2 + 2.0f * py0 * z0_1;
+      { const float r = px0 * y2_0_2;
+        W[2] = fmaf(-0.2886751345948129f, r, W[2]);
+      }
+      const float y2_0_3 = py2 * z0_3 + 2.0f * py0 * z0_2;
+      { const float r = px1 * y2_0_3;
+        W[6] = fmaf(1.0f, r, W[6]);
+        W[5] = fmaf(-0.2886751345948129f, r, W[5]);
+      }
+      const float y2_0_4 = py2 * z0_4 + 2.0f * py0 * z0_3;
+      { const float r = px2 * y2_0_4 + 2.0f * px0 * y2_0_3;
+        W[15] = fmaf(1.0f, r, W[15]);
+        W[23] = fmaf(-0.2886751345948129f, r, W[23]);
+        W[14] = fmaf(-0.2886751345948129f, r, W[14]);
+      }
+      }
+      {
+      const float y3_0_3 = py3 * z0_3 + 6.0f * py1 * z0_2;
+      { const float r = px0 * y3_0_3;
+        W[8] = fmaf(-0.2886751345948129f, r, W[8]);
+      }
+      const float y3_0_4 = py3 * z0_4 + 6.0f * py1 * z0_3;
+      { const float r = px1 * y3_0_4;
+        W[21] = fmaf(1.0f, r, W[21]);
+        W[17] = fmaf(-0.2886751345948129f, r, W[17]);
+      }
+      }
+      {
+      const float y4_0_4 = py4 * z0_4 + 12.0f * py2 * z0_3 + 12.0f * py0 * z0_2;
+      { const float r = px0 * y4_0_4;
+        W[23] = fmaf(-0.2886751345948129f, r, W[23]);
+      }
+      }
+      }
+      {
+      const float z1_1 = pz1 * g1;
+      const float z1_2 = pz1 * g2;
+      const float z1_3 = pz1 * g3;
+      const float z1_4 = pz1 * g4;
+      {
+      const float y0_1_3 = py0 * z1_3;
+      const float y0_1_2 = py0 * z1_2;
+      { const float r = px2 * y0_1_3 + 2.0f * px0 * y0_1_2;
+        W[11] = fmaf(-0.2886751345948129f, r, W[11]);
+      }
+      const float y0_1_4 = py0 * z1_4;
+      { const float r = px3 * y0_1_4 + 6.0f * px1 * y0_1_3;
+        W[20] = fmaf(-0.2886751345948129f, r, W[20]);
+      }
+      }
+      {
+      const float y1_1_2 = py1 * z1_2;
+      { const float r = px0 * y1_1_2;
+        W[1] = fmaf(1.0f, r, W[1]);
+      }
+      const float y1_1_3 = py1 * z1_3;
+      { const float r = px1 * y1_1_3;
+        W[9] = fmaf(1.0f, r, W[9]);
+        W[4] = fmaf(1.0f, r, W[4]);
+      }
+      const float y1_1_4 = py1 * z1_4;
+      { const float r = px2 * y1_1_4 + 2.0f * px0 * y1_1_3;
+        W[18] = fmaf(1.0f, r, W[18]);
+        W[13] = fmaf(1.0f, r, W[13]);
+        W[26] = fmaf(-0.2886751345948129f, r, W[26]);
+      }
+      }
+      {
+      const float y2_1_3 = py2 * z1_3 + 2.0f * py0 * z1_2;
+      { const float r = px0 * y2_1_3;
+        W[7] = fmaf(1.0f, r, W[7]);
+        W[11] = fmaf(-0.2886751345948129f, r, W[11]);
+      }
+      const float y2_1_4 = py2 * z1_4 + 2.0f * py0 * z1_3;
+      { const float r = px1 * y2_1_4;
+        W[24] = fmaf(1.0f, r, W[24]);
+        W[16] = fmaf(1.0f, r, W[16]);
+        W[20] = fmaf(-0.2886751345948129f, r, W[20]);
+      }
+      }
+      {
+      const float y3_1_4 = py3 * z1_4 + 6.0f * py1 * z1_3;
+      { const float r = px0 * y3_1_4;
+        W[22] = fmaf(1.0f, r, W[22]);
+        W[26] = fmaf(-0.2886751345948129f, r, W[26]);
+      }
+      }
+      }
+      {
+      const float z2_2 = pz2 * g2 + 2.0f * pz0 * g1;
+      const float z2_3 = pz2 * g3 + 2.0f * pz0 * g2;
+      const float z2_4 = pz2 * g4 + 2.0f * pz0 * g3;
+      {
+      const float y0_2_2 = py0 * z2_2;
+      { const float r = px0 * y0_2_2;
+        W[2] = fmaf(0.5773502691896258f, r, W[2]);
+      }
+      const float y0_2_3 = py0 * z2_3;
+      { const float r = px1 * y0_2_3;
+        W[5] = fmaf(0.5773502691896258f, r, W[5]);
+      }
+      const float y0_2_4 = py0 * z2_4;
+      { const float r = px2 * y0_2_4 + 2.0f * px0 * y0_2_3;
+        W[29] = fmaf(-0.2886751345948129f, r, W[29]);
+        W[14] = fmaf(0.5773502691896258f, r, W[14]);
+      }
+      }
+      {
+      const float y1_2_3 = py1 * z2_3;
+      { const float r = px0 * y1_2_3;
+        W[10] = fmaf(1.0f, r, W[10]);
+        W[8] = fmaf(0.5773502691896258f, r, W[8]);
+      }
+      const float y1_2_4 = py1 * z2_4;
+      { const float r = px1 * y1_2_4;
+        W[27] = fmaf(1.0f, r, W[27]);
+        W[19] = fmaf(1.0f, r, W[19]);
+        W[17] = fmaf(0.5773502691896258f, r, W[17]);
+      }
+      }
+      {
+      const float y2_2_4 = py2 * z2_4 + 2.0f * py0 * z2_3;
+      { const float r = px0 * y2_2_4;
+        W[25] = fmaf(1.0f, r, W[25]);
+        W[29] = fmaf(-0.2886751345948129f, r, W[29]);
+        W[23] = fmaf(0.5773502691896258f, r, W[23]);
+      }
+      }
+      }
+      {
+      const float z3_3 = pz3 * g3 + 6.0f * pz1 * g2;
+      const float z3_4 = pz3 * g4 + 6.0f * pz1 * g3;
+      {
+      const float y0_3_3 = py0 * z3_3;
+      { const float r = px0 * y0_3_3;
+        W[11] = fmaf(0.5773502691896258f, r, W[11]);
+      }
+      const float y0_3_4 = py0 * z3_4;
+      { const float r = px1 * y0_3_4;
+        W[20] = fmaf(0.5773502691896258f, r, W[20]);
+      }
+      }
+      {
+      const float y1_3_4 = py1 * z3_4;
+      { const float r = px0 * y1_3_4;
+        W[28] = fmaf(1.0f, r, W[28]);
+        W[26] = fmaf(0.5773502691896258f, r, W[26]);
+      }
+      }
+      }
+      {
+      const float z4_4 = pz4 * g4 + 12.0f * pz2 * g3 + 12.0f * pz0 * g2;
+      {
+      const float y0_4_4 = py0 * z4_4;
+      { const float r = px0 * y0_4_4;
+        W[29] = fmaf(0.5773502691896258f, r, W[29]);
+      }
+      }
+      }
+      { const float e = rf[4];
+        accf[0] = fmaf(e, W[0], accf[0]);
+        accf[1] = fmaf(e, W[1], accf[1]);
+        accf[2] = fmaf(e, W[2], accf[2]);
+      }
+      { const float e = rf[5];
+        accf[0] = fmaf(e, W[3], accf[0]);
+        accf[1] = fmaf(e, W[4], accf[1]);
+        accf[2] = fmaf(e, W[5], accf[2]);
+      }
+      { const float e = rf[6];
+        accf[0] = fmaf(e, W[12], accf[0]);
+        accf[1] = fmaf(e, W[13], accf[1]);
+        accf[2] = fmaf(e, W[14], accf[2]);
+      }
+      { const float e = rf[7];
+        accf[3] = fmaf(e, W[0], accf[3]);
+        accf[4] = fmaf(e, W[1], accf[4]);
+        accf[5] = fmaf(e, W[2], accf[5]);
+      }
+      { const float e = rf[8];
+        accf[3] = fmaf(e, W[6], accf[3]);
+        accf[4] = fmaf(e, W[7], accf[4]);
+        accf[5] = fmaf(e, W[8], accf[5]);
+      }
+      { const float e = rf[9];
+        accf[3] = fmaf(e, W[3], accf[3]);
+        accf[4] = fmaf(e, W[4], accf[4]);
+        accf[5] = fmaf(e, W[5], accf[5]);
+      }
+      { const float e = rf[10];
+        accf[3] = fmaf(e, W[15], accf[3]);
+        accf[4] = fmaf(e, W[16], accf[4]);
+        accf[5] = fmaf(e, W[17], accf[5]);
+      }
+      { const float e = rf[11];
+        accf[6] = fmaf(e, W[0], accf[6]);
+        accf[7] = fmaf(e, W[1], accf[7]);
+        accf[8] = fmaf(e, W[2], accf[8]);
+      }
+      { const float e = rf[12];
+        accf[6] = fmaf(e, W[9], accf[6]);
+        accf[7] = fmaf(e, W[10], accf[7]);
+        accf[8] = fmaf(e, W[11], accf[8]);
+      }
+      { const float e = rf[13];
+        accf[6] = fmaf(e, W[3], accf[6]);
+        accf[7] = fmaf(e, W[4], accf[7]);
+        accf[8] = fmaf(e, W[5], accf[8]);
+      }
+      { const float e = rf[14];
+        accf[6] = fmaf(e, W[18], accf[6]);
+        accf[7] = fmaf(e, W[19], accf[7]);
+        accf[8] = fmaf(e, W[20], accf[8]);
+      }
+      { const float e = rf[15];
+        accf[9] = fmaf(e, W[0], accf[9]);
+        accf[10] = fmaf(e, W[1], accf[10]);
+        accf[11] = fmaf(e, W[2], accf[11]);
+      }
+      { const float e = rf[16];
+        accf[9] = fmaf(e, W[6], accf[9]);
+        accf[10] = fmaf(e, W[7], accf[10]);
+        accf[11] = fmaf(e, W[8], accf[11]);
+      }
+      { const float e = rf[17];
+        accf[9] = fmaf(e, W[3], accf[9]);
+        accf[10] = fmaf(e, W[4], accf[10]);
+        accf[11] = fmaf(e, W[5], accf[11]);
+      }
+      { const float e = rf[18];
+        accf[9] = fmaf(e, W[15], accf[9]);
+        accf[10] = fmaf(e, W[16], accf[10]);
+        accf[11] = fmaf(e, W[17], accf[11]);
+      }
+      { const float e = rf[19];
+        accf[12] = fmaf(e, W[0], accf[12]);
+        accf[13] = fmaf(e, W[1], accf[13]);
+        accf[14] = fmaf(e, W[2], accf[14]);
+      }
+      { const float e = rf[20];
+        accf[12] = fmaf(e, W[6], accf[12]);
+        accf[13] = fmaf(e, W[7], accf[13]);
+        accf[14] = fmaf(e, W[8], accf[14]);
+      }
+      { const float e = rf[21];
+        accf[12] = fmaf(e, W[21], accf[12]);
+        accf[13] = fmaf(e, W[22], accf[13]);
+        accf[14] = fmaf(e, W[23], accf[14]);
+      }
+      { const float e = rf[22];
+        accf[15] = fmaf(e, W[0], accf[15]);
+        accf[16] = fmaf(e, W[1], accf[16]);
+        accf[17] = fmaf(e, W[2], accf[17]);
+      }
+      { const float e = rf[23];
+        accf[15] = fmaf(e, W[9], accf[15]);
+        accf[16] = fmaf(e, W[10], accf[16]);
+        accf[17] = fmaf(e, W[11], accf[17]);
+      }
+      { const float e = rf[24];
+        accf[15] = fmaf(e, W[6], accf[15]);
+        accf[16] = fmaf(e, W[7], accf[16]);
+        accf[17] = fmaf(e, W[8], accf[17]);
+      }
+      { const float e = rf[25];
+        accf[15] = fmaf(e, W[24], accf[15]);
+        accf[16] = fmaf(e, W[25], accf[16]);
+        accf[17] = fmaf(e, W[26], accf[17]);
+      }
+      { const float e = rf[26];
+        accf[18] = fmaf(e, W[0], accf[18]);
+        accf[19] = fmaf(e, W[1], accf[19]);
+        accf[20] = fmaf(e, W[2], accf[20]);
+      }
+      { const float e = rf[27];
+        accf[18] = fmaf(e, W[9], accf[18]);
+        accf[19] = fmaf(e, W[10], accf[19]);
+        accf[20] = fmaf(e, W[11], accf[20]);
+      }
+      { const float e = rf[28];
+        accf[18] = fmaf(e, W[3], accf[18]);
+        accf[19] = fmaf(e, W[4], accf[19]);
+        accf[20] = fmaf(e, W[5], accf[20]);
+      }
+      { const float e = rf[29];
+        accf[18] = fmaf(e, W[18], accf[18]);
+        accf[19] = fmaf(e, W[19], accf[19]);
+        accf[20] = fmaf(e, W[20], accf[20]);
+      }
+      { const float e = rf[30];
+        accf[21] = fmaf(e, W[0], accf[21]);
+        accf[22] = fmaf(e, W[1], accf[22]);
+        accf[23] = fmaf(e, W[2], accf[23]);
+      }
+      { const float e = rf[31];
+        accf[21] = fmaf(e, W[9], accf[21]);
+        accf[22] = fmaf(e, W[10], accf[22]);
+        accf[23] = fmaf(e, W[11], accf[23]);
+      }
+      { const float e = rf[32];
+        accf[21] = fmaf(e, W[6], accf[21]);
+        accf[22] = fmaf(e, W[7], accf[22]);
+        accf[23] = fmaf(e, W[8], accf[23]);
+      }
+      { const float e = rf[33];
+        accf[21] = fmaf(e, W[24], accf[21]);
+        accf[22] = fmaf(e, W[25], accf[22]);
+        accf[23] = fmaf(e, W[26], accf[23]);
+      }
+      { const float e = rf[34];
+        accf[24] = fmaf(e, W[0], accf[24]);
+        accf[25] = fmaf(e, W[1], accf[25]);
+        accf[26] = fmaf(e, W[2], accf[26]);
+      }
+      { const float e = rf[35];
+        accf[24] = fmaf(e, W[9], accf[24]);
+        accf[25] = fmaf(e, W[10], accf[25]);
+        accf[26] = fmaf(e, W[11], accf[26]);
+      }
+      { const float e = rf[36];
+        accf[24] = fmaf(e, W[27], accf[24]);
+        accf[25] = fmaf(e, W[28], accf[25]);
+        accf[26] = fmaf(e, W[29], accf[26]);
+      }
+      }
+#pragma unroll
+      for (int q = 0; q < 27; q++) acc[q] += (double)accf[q];
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[12];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[13];
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[14];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[15];
+    out[(row0 + 1) * ldo + col + 1] = 1.0 * acc[16];
+    out[(row0 + 1) * ldo + col + 2] = 1.0 * acc[17];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[9];
+    out[(row0 + 2) * ldo + col + 1] = 1.0 * acc[10];
+    out[(row0 + 2) * ldo + col + 2] = 1.0 * acc[11];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[21];
+    out[(row0 + 3) * ldo + col + 1] = 1.0 * acc[22];
+    out[(row0 + 3) * ldo + col + 2] = 1.0 * acc[23];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[24];
+    out[(row0 + 4) * ldo + col + 1] = 1.0 * acc[25];
+    out[(row0 + 4) * ldo + col + 2] = 1.0 * acc[26];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[18];
+    out[(row0 + 5) * ldo + col + 1] = 1.0 * acc[19];
+    out[(row0 + 5) * ldo + col + 2] = 1.0 * acc[20];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[3];
+    out[(row0 + 6) * ldo + col + 1] = 1.0 * acc[4];
+    out[(row0 + 6) * ldo + col + 2] = 1.0 * acc[5];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[6];
+    out[(row0 + 7) * ldo + col + 1] = 1.0 * acc[7];
+    out[(row0 + 7) * ldo + col + 2] = 1.0 * acc[8];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 1] = 1.0 * acc[1];
+    out[(row0 + 8) * ldo + col + 2] = 1.0 * acc[2];
+    }
+    break;
+  }
+  case 1: {
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const float* __restrict__ rf = recp32 + (size_t)ij * 37;
+      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;
+      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;
+      float accf[18];
+#pragma unroll
+      for (int q = 0; q < 18; q++) accf[q] = 0.0f;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= i2g * i2g; }
+      double F[5];
+      boys_array<4>(T, boys, F);
+      const double ma = -alpha;
+      const float g0 = (float)(sc * F[0]);
+      sc *= ma; const float g1 = (float)(sc * F[1]);
+      sc *= ma; const float g2 = (float)(sc * F[2]);
+      sc *= ma; const float g3 = (float)(sc * F[3]);
+      sc *= ma; const float g4 = (float)(sc * F[4]);
+      const float px0 = 1.0f, px1 = 2.0f * PCx;
+      const float px2 = px1 * px1;
+      const float px3 = px2 * px1;
+      const float px4 = px3 * px1;
+      const float py0 = 1.0f, py1 = 2.0f * PCy;
+      const float py2 = py1 * py1;
+      const float py3 = py2 * py1;
+      const float py4 = py3 * py1;
+      const float pz0 = 1.0f, pz1 = 2.0f * PCz;
+      const float pz2 = pz1 * pz1;
+      const float pz3 = pz2 * pz1;
+      const float pz4 = pz3 * pz1;
+      float W[20];
+#pragma unroll
+      for (int q = 0; q < 20; q++) W[q] = 0.0f;
+      {
+      const float z0_0 = pz0 * g0;
+      const float z0_1 = pz0 * g1;
+      const float z0_2 = pz0 * g2;
+      const float z0_3 = pz0 * g3;
+      const float z0_4 = pz0 * g4;
+      {
+      const float y0_0_2 = py0 * z0_2;
+      const float y0_0_1 = py0 * z0_1;
+      { const float r = px2 * y0_0_2 + 2.0f * px0 * y0_0_1;
+        W[1] = fmaf(0.5f, r, W[1]);
+      }
+      const float y0_0_3 = py0 * z0_3;
+      { const float r = px3 * y0_0_3 + 6.0f * px1 * y0_0_2;
+        W[3] = fmaf(0.5f, r, W[3]);
+      }
+      const float y0_0_4 = py0 * z0_4;
+      { const float r = px4 * y0_0_4 + 12.0f * px2 * y0_0_3 + 12.0f * px0 * y0_0_2;
+        W[9] = fmaf(0.5f, r, W[9]);
+      }
+      }
+      {
+      const float y1_0_3 = py1 * z0_3;
+      const float y1_0_2 = py1 * z0_2;
+      { const float r = px2 * y1_0_3 + 2.0f * px0 * y1_0_2;
+        W[5] = fmaf(0.5f, r, W[5]);
+      }
+      const float y1_0_4 = py1 * z0_4;
+      { const float r = px3 * y1_0_4 + 6.0f * px1 * y1_0_3;
+        W[11] = fmaf(0.5f, r, W[11]);
+      }
+      }
+      {
+      const float y2_0_2 = py2 * z0_2 + 2.0f * py0 * z0_1;
+      { const float r = px0 * y2_0_2;
+        W[1] = fmaf(-0.5f, r, W[1]);
+      }
+      const float y2_0_3 = py2 * z0_3 + 2.0f * py0 * z0_2;
+      { const float r = px1 * y2_0_3;
+        W[3] = fmaf(-0.5f, r, W[3]);
+      }
+      const float y2_0_4 = py2 * z0_4 + 2.0f * py0 * z0_3;
+      { const float r = px2 * y2_0_4 + 2.0f * px0 * y2_0_3;
+        W[15] = fmaf(0.5f, r, W[15]);
+        W[9] = fmaf(-0.5f, r, W[9]);
+      }
+      }
+      {
+      const float y3_0_3 = py3 * z0_3 + 6.0f * py1 * z0_2;
+      { const float r = px0 * y3_0_3;
+        W[5] = fmaf(-0.5f, r, W[5]);
+      }
+      const float y3_0_4 = py3 * z0_4 + 6.0f * py1 * z0_3;
+      { const float r = px1 * y3_0_4;
+        W[11] = fmaf(-0.5f, r, W[11]);
+      }
+      }
+      {
+      const float y4_0_4 = py4 * z0_4 + 12.0f * py2 * z0_3 + 12.0f * py0 * z0_2;
+      { const float r = px0 * y4_0_4;
+        W[15] = fmaf(-0.5f, r, W[15]);
+      }
+      }
+      }
+      {
+      const float z1_1 = pz1 * g1;
+      const float z1_2 = pz1 * g2;
+      const float z1_3 = pz1 * g3;
+      const float z1_4 = pz1 * g4;
+      {
+      const float y0_1_2 = py0 * z1_2;
+      { const float r = px1 * y0_1_2;
+        W[0] = fmaf(1.0f, r, W[0]);
+      }
+      const float y0_1_3 = py0 * z1_3;
+      { const float r = px2 * y0_1_3 + 2.0f * px0 * y0_1_2;
+        W[2] = fmaf(1.0f, r, W[2]);
+        W[7] = fmaf(0.5f, r, W[7]);
+      }
+      const float y0_1_4 = py0 * z1_4;
+      { const float r = px3 * y0_1_4 + 6.0f * px1 * y0_1_3;
+        W[8] = fmaf(1.0f, r, W[8]);
+        W[13] = fmaf(0.5f, r, W[13]);
+      }
+      }
+      {
+      const float y1_1_3 = py1 * z1_3;
+      { const float r = px1 * y1_1_3;
+        W[4] = fmaf(1.0f, r, W[4]);
+      }
+      const float y1_1_4 = py1 * z1_4;
+      { const float r = px2 * y1_1_4 + 2.0f * px0 * y1_1_3;
+        W[10] = fmaf(1.0f, r, W[10]);
+        W[17] = fmaf(0.5f, r, W[17]);
+      }
+      }
+      {
+      const float y2_1_3 = py2 * z1_3 + 2.0f * py0 * z1_2;
+      { const float r = px0 * y2_1_3;
+        W[7] = fmaf(-0.5f, r, W[7]);
+      }
+      const float y2_1_4 = py2 * z1_4 + 2.0f * py0 * z1_3;
+      { const float r = px1 * y2_1_4;
+        W[14] = fmaf(1.0f, r, W[14]);
+        W[13] = fmaf(-0.5f, r, W[13]);
+      }
+      }
+      {
+      const float y3_1_4 = py3 * z1_4 + 6.0f * py1 * z1_3;
+      { const float r = px0 * y3_1_4;
+        W[17] = fmaf(-0.5f, r, W[17]);
+      }
+      }
+      }
+      {
+      const float z2_2 = pz2 * g2 + 2.0f * pz0 * g1;
+      const float z2_3 = pz2 * g3 + 2.0f * pz0 * g2;
+      const float z2_4 = pz2 * g4 + 2.0f * pz0 * g3;
+      {
+      const float y0_2_3 = py0 * z2_3;
+      { const float r = px1 * y0_2_3;
+        W[6] = fmaf(1.0f, r, W[6]);
+      }
+      const float y0_2_4 = py0 * z2_4;
+      { const float r = px2 * y0_2_4 + 2.0f * px0 * y0_2_3;
+        W[12] = fmaf(1.0f, r, W[12]);
+        W[19] = fmaf(0.5f, r, W[19]);
+      }
+      }
+      {
+      const float y1_2_4 = py1 * z2_4;
+      { const float r = px1 * y1_2_4;
+        W[16] = fmaf(1.0f, r, W[16]);
+      }
+      }
+      {
+      const float y2_2_4 = py2 * z2_4 + 2.0f * py0 * z2_3;
+      { const float r = px0 * y2_2_4;
+        W[19] = fmaf(-0.5f, r, W[19]);
+      }
+      }
+      }
+      {
+      const float z3_3 = pz3 * g3 + 6.0f * pz1 * g2;
+      const float z3_4 = pz3 * g4 + 6.0f * pz1 * g3;
+      {
+      const float y0_3_4 = py0 * z3_4;
+      { const float r = px1 * y0_3_4;
+        W[18] = fmaf(1.0f, r, W[18]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const float z4_4 = pz4 * g4 + 12.0f * pz2 * g3 + 12.0f * pz0 * g2;
+      {
+      }
+      }
+      { const float e = rf[4];
+        accf[0] = fmaf(e, W[0], accf[0]);
+        accf[1] = fmaf(e, W[1], accf[1]);
+      }
+      { const float e = rf[5];
+        accf[0] = fmaf(e, W[2], accf[0]);
+        accf[1] = fmaf(e, W[3], accf[1]);
+      }
+      { const float e = rf[6];
+        accf[0] = fmaf(e, W[8], accf[0]);
+        accf[1] = fmaf(e, W[9], accf[1]);
+      }
+      { const float e = rf[7];
+        accf[2] = fmaf(e, W[0], accf[2]);
+        accf[3] = fmaf(e, W[1], accf[3]);
+      }
+      { const float e = rf[8];
+        accf[2] = fmaf(e, W[4], accf[2]);
+        accf[3] = fmaf(e, W[5], accf[3]);
+      }
+      { const float e = rf[9];
+        accf[2] = fmaf(e, W[2], accf[2]);
+        accf[3] = fmaf(e, W[3], accf[3]);
+      }
+      { const float e = rf[10];
+        accf[2] = fmaf(e, W[10], accf[2]);
+        accf[3] = fmaf(e, W[11], accf[3]);
+      }
+      { const float e = rf[11];
+        accf[4] = fmaf(e, W[0], accf[4]);
+        accf[5] = fmaf(e, W[1], accf[5]);
+      }
+      { const float e = rf[12];
+        accf[4] = fmaf(e, W[6], accf[4]);
+        accf[5] = fmaf(e, W[7], accf[5]);
+      }
+      { const float e = rf[13];
+        accf[4] = fmaf(e, W[2], accf[4]);
+        accf[5] = fmaf(e, W[3], accf[5]);
+      }
+      { const float e = rf[14];
+        accf[4] = fmaf(e, W[12], accf[4]);
+        accf[5] = fmaf(e, W[13], accf[5]);
+      }
+      { const float e = rf[15];
+        accf[6] = fmaf(e, W[0], accf[6]);
+        accf[7] = fmaf(e, W[1], accf[7]);
+      }
+      { const float e = rf[16];
+        accf[6] = fmaf(e, W[4], accf[6]);
+        accf[7] = fmaf(e, W[5], accf[7]);
+      }
+      { const float e = rf[17];
+        accf[6] = fmaf(e, W[2], accf[6]);
+        accf[7] = fmaf(e, W[3], accf[7]);
+      }
+      { const float e = rf[18];
+        accf[6] = fmaf(e, W[10], accf[6]);
+        accf[7] = fmaf(e, W[11], accf[7]);
+      }
+      { const float e = rf[19];
+        accf[8] = fmaf(e, W[0], accf[8]);
+        accf[9] = fmaf(e, W[1], accf[9]);
+      }
+      { const float e = rf[20];
+        accf[8] = fmaf(e, W[4], accf[8]);
+        accf[9] = fmaf(e, W[5], accf[9]);
+      }
+      { const float e = rf[21];
+        accf[8] = fmaf(e, W[14], accf[8]);
+        accf[9] = fmaf(e, W[15], accf[9]);
+      }
+      { const float e = rf[22];
+        accf[10] = fmaf(e, W[0], accf[10]);
+        accf[11] = fmaf(e, W[1], accf[11]);
+      }
+      { const float e = rf[23];
+        accf[10] = fmaf(e, W[6], accf[10]);
+        accf[11] = fmaf(e, W[7], accf[11]);
+      }
+      { const float e = rf[24];
+        accf[10] = fmaf(e, W[4], accf[10]);
+        accf[11] = fmaf(e, W[5], accf[11]);
+      }
+      { const float e = rf[25];
+        accf[10] = fmaf(e, W[16], accf[10]);
+        accf[11] = fmaf(e, W[17], accf[11]);
+      }
+      { const float e = rf[26];
+        accf[12] = fmaf(e, W[0], accf[12]);
+        accf[13] = fmaf(e, W[1], accf[13]);
+      }
+      { const float e = rf[27];
+        accf[12] = fmaf(e, W[6], accf[12]);
+        accf[13] = fmaf(e, W[7], accf[13]);
+      }
+      { const float e = rf[28];
+        accf[12] = fmaf(e, W[2], accf[12]);
+        accf[13] = fmaf(e, W[3], accf[13]);
+      }
+      { const float e = rf[29];
+        accf[12] = fmaf(e, W[12], accf[12]);
+        accf[13] = fmaf(e, W[13], accf[13]);
+      }
+      { const float e = rf[30];
+        accf[14] = fmaf(e, W[0], accf[14]);
+        accf[15] = fmaf(e, W[1], accf[15]);
+      }
+      { const float e = rf[31];
+        accf[14] = fmaf(e, W[6], accf[14]);
+        accf[15] = fmaf(e, W[7], accf[15]);
+      }
+      { const float e = rf[32];
+        accf[14] = fmaf(e, W[4], accf[14]);
+        accf[15] = fmaf(e, W[5], accf[15]);
+      }
+      { const float e = rf[33];
+        accf[14] = fmaf(e, W[16], accf[14]);
+        accf[15] = fmaf(e, W[17], accf[15]);
+      }
+      { const float e = rf[34];
+        accf[16] = fmaf(e, W[0], accf[16]);
+        accf[17] = fmaf(e, W[1], accf[17]);
+      }
+      { const float e = rf[35];
+        accf[16] = fmaf(e, W[6], accf[16]);
+        accf[17] = fmaf(e, W[7], accf[17]);
+      }
+      { const float e = rf[36];
+        accf[16] = fmaf(e, W[18], accf[16]);
+        accf[17] = fmaf(e, W[19], accf[17]);
+      }
+      }
+#pragma unroll
+      for (int q = 0; q < 18; q++) acc[q] += (double)accf[q];
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 3] = 1.0 * acc[8];
+    out[(row0 + 0) * ldo + col + 4] = 1.0 * acc[9];
+    out[(row0 + 1) * ldo + col + 3] = 1.0 * acc[10];
+    out[(row0 + 1) * ldo + col + 4] = 1.0 * acc[11];
+    out[(row0 + 2) * ldo + col + 3] = 1.0 * acc[6];
+    out[(row0 + 2) * ldo + col + 4] = 1.0 * acc[7];
+    out[(row0 + 3) * ldo + col + 3] = 1.0 * acc[14];
+    out[(row0 + 3) * ldo + col + 4] = 1.0 * acc[15];
+    out[(row0 + 4) * ldo + col + 3] = 1.0 * acc[16];
+    out[(row0 + 4) * ldo + col + 4] = 1.0 * acc[17];
+    out[(row0 + 5) * ldo + col + 3] = 1.0 * acc[12];
+    out[(row0 + 5) * ldo + col + 4] = 1.0 * acc[13];
+    out[(row0 + 6) * ldo + col + 3] = 1.0 * acc[2];
+    out[(row0 + 6) * ldo + col + 4] = 1.0 * acc[3];
+    out[(row0 + 7) * ldo + col + 3] = 1.0 * acc[4];
+    out[(row0 + 7) * ldo + col + 4] = 1.0 * acc[5];
+    out[(row0 + 8) * ldo + col + 3] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 4] = 1.0 * acc[1];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
+// class (11|3), FP64: L = 5, record 37 values, 4 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
   const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
   const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
   const bool live = active && triplet_kept(pa, xa, pidx, sh, 3, Cx, Cy, Cz);
@@ -1811,12 +3261,12 @@ __global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__
       double F[6];
       boys_array<5>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
-      sc *= ma; const double g5 = sc * F[5];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
+      sc *= ma; const double g5 = (sc * F[5]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -2187,12 +3637,12 @@ __global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__
       double F[6];
       boys_array<5>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
-      sc *= ma; const double g5 = sc * F[5];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
+      sc *= ma; const double g5 = (sc * F[5]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -2644,12 +4094,12 @@ __global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__
       double F[6];
       boys_array<5>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
-      sc *= ma; const double g5 = sc * F[5];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
+      sc *= ma; const double g5 = (sc * F[5]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -3075,12 +4525,12 @@ __global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__
       double F[6];
       boys_array<5>(T, boys, F);
       const double ma = -alpha;
-      const double g0 = sc * F[0];
-      sc *= ma; const double g1 = sc * F[1];
-      sc *= ma; const double g2 = sc * F[2];
-      sc *= ma; const double g3 = sc * F[3];
-      sc *= ma; const double g4 = sc * F[4];
-      sc *= ma; const double g5 = sc * F[5];
+      const double g0 = (sc * F[0]);
+      sc *= ma; const double g1 = (sc * F[1]);
+      sc *= ma; const double g2 = (sc * F[2]);
+      sc *= ma; const double g3 = (sc * F[3]);
+      sc *= ma; const double g4 = (sc * F[4]);
+      sc *= ma; const double g5 = (sc * F[5]);
       const double px0 = 1.0, px1 = 2.0 * PCx;
       const double px2 = px1 * px1;
       const double px3 = px2 * px1;
@@ -3374,11 +4824,1115 @@ __global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__
   }
 }
 
+// class (11|3), FP32: L = 5, record 37 values, 2 m-group(s)
+__global__ __launch_bounds__(64) void int3c2e_gen32_113(const double* __restrict__ recs, const float* __restrict__ recs32,
+    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
+    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
+  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
+  if (c0 >= cend || c0 + 64 <= cbeg) return;
+  const bool active = c >= cbeg && c < cend;
+  const int sh = active ? xa.shell[c] : xa.shell[xa.lo];
+  const double Cx = ket.xyz[3 * sh], Cy = ket.xyz[3 * sh + 1], Cz = ket.xyz[3 * sh + 2];
+  const bool live = active && triplet_kept(pa, xa, pidx, sh, 3, Cx, Cy, Cz);
+  const int nprim_c = live ? ket.nprim[sh] : 0, pc0 = ket.pptr[sh];
+  const int npp = npp_v[pidx];
+  const double* __restrict__ recp = recs + recoff[pidx];
+  const float* __restrict__ recp32 = recs32 + recoff[pidx];
+  const int64_t row0 = pa.row[pidx];
+  const int64_t col = active ? xa.col[c] : 0;
+  switch (blockIdx.z) {
+  case 0: {
+    double acc[36];
+#pragma unroll
+    for (int q = 0; q < 36; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const float* __restrict__ rf = recp32 + (size_t)ij * 37;
+      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;
+      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;
+      float accf[36];
+#pragma unroll
+      for (int q = 0; q < 36; q++) accf[q] = 0.0f;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[6];
+      boys_array<5>(T, boys, F);
+      const double ma = -alpha;
+      const float g0 = (float)(sc * F[0]);
+      sc *= ma; const float g1 = (float)(sc * F[1]);
+      sc *= ma; const float g2 = (float)(sc * F[2]);
+      sc *= ma; const float g3 = (float)(sc * F[3]);
+      sc *= ma; const float g4 = (float)(sc * F[4]);
+      sc *= ma; const float g5 = (float)(sc * F[5]);
+      const float px0 = 1.0f, px1 = 2.0f * PCx;
+      const float px2 = px1 * px1;
+      const float px3 = px2 * px1;
+      const float px4 = px3 * px1;
+      const float px5 = px4 * px1;
+      const float py0 = 1.0f, py1 = 2.0f * PCy;
+      const float py2 = py1 * py1;
+      const float py3 = py2 * py1;
+      const float py4 = py3 * py1;
+      const float py5 = py4 * py1;
+      const float pz0 = 1.0f, pz1 = 2.0f * PCz;
+      const float pz2 = pz1 * pz1;
+      const float pz3 = pz2 * pz1;
+      const float pz4 = pz3 * pz1;
+      const float pz5 = pz4 * pz1;
+      float W[40];
+#pragma unroll
+      for (int q = 0; q < 40; q++) W[q] = 0.0f;
+      {
+      const float z0_0 = pz0 * g0;
+      const float z0_1 = pz0 * g1;
+      const float z0_2 = pz0 * g2;
+      const float z0_3 = pz0 * g3;
+      const float z0_4 = pz0 * g4;
+      const float z0_5 = pz0 * g5;
+      {
+      }
+      {
+      const float y1_0_3 = py1 * z0_3;
+      const float y1_0_2 = py1 * z0_2;
+      { const float r = px2 * y1_0_3 + 2.0f * px0 * y1_0_2;
+        W[0] = fmaf(0.6123724356957945f, r, W[0]);
+        W[2] = fmaf(-0.15811388300841894f, r, W[2]);
+      }
+      const float y1_0_4 = py1 * z0_4;
+      { const float r = px3 * y1_0_4 + 6.0f * px1 * y1_0_3;
+        W[4] = fmaf(0.6123724356957945f, r, W[4]);
+        W[6] = fmaf(-0.15811388300841894f, r, W[6]);
+      }
+      const float y1_0_5 = py1 * z0_5;
+      { const float r = px4 * y1_0_5 + 12.0f * px2 * y1_0_4 + 12.0f * px0 * y1_0_3;
+        W[16] = fmaf(0.6123724356957945f, r, W[16]);
+        W[18] = fmaf(-0.15811388300841894f, r, W[18]);
+      }
+      }
+      {
+      const float y2_0_4 = py2 * z0_4 + 2.0f * py0 * z0_3;
+      const float y2_0_3 = py2 * z0_3 + 2.0f * py0 * z0_2;
+      { const float r = px2 * y2_0_4 + 2.0f * px0 * y2_0_3;
+        W[8] = fmaf(0.6123724356957945f, r, W[8]);
+        W[10] = fmaf(-0.15811388300841894f, r, W[10]);
+      }
+      const float y2_0_5 = py2 * z0_5 + 2.0f * py0 * z0_4;
+      { const float r = px3 * y2_0_5 + 6.0f * px1 * y2_0_4;
+        W[20] = fmaf(0.6123724356957945f, r, W[20]);
+        W[22] = fmaf(-0.15811388300841894f, r, W[22]);
+      }
+      }
+      {
+      const float y3_0_3 = py3 * z0_3 + 6.0f * py1 * z0_2;
+      { const float r = px0 * y3_0_3;
+        W[0] = fmaf(-0.20412414523193148f, r, W[0]);
+        W[2] = fmaf(-0.15811388300841894f, r, W[2]);
+      }
+      const float y3_0_4 = py3 * z0_4 + 6.0f * py1 * z0_3;
+      { const float r = px1 * y3_0_4;
+        W[4] = fmaf(-0.20412414523193148f, r, W[4]);
+        W[6] = fmaf(-0.15811388300841894f, r, W[6]);
+      }
+      const float y3_0_5 = py3 * z0_5 + 6.0f * py1 * z0_4;
+      { const float r = px2 * y3_0_5 + 2.0f * px0 * y3_0_4;
+        W[28] = fmaf(0.6123724356957945f, r, W[28]);
+        W[16] = fmaf(-0.20412414523193148f, r, W[16]);
+        W[30] = fmaf(-0.15811388300841894f, r, W[30]);
+        W[18] = fmaf(-0.15811388300841894f, r, W[18]);
+      }
+      }
+      {
+      const float y4_0_4 = py4 * z0_4 + 12.0f * py2 * z0_3 + 12.0f * py0 * z0_2;
+      { const float r = px0 * y4_0_4;
+        W[8] = fmaf(-0.20412414523193148f, r, W[8]);
+        W[10] = fmaf(-0.15811388300841894f, r, W[10]);
+      }
+      const float y4_0_5 = py4 * z0_5 + 12.0f * py2 * z0_4 + 12.0f * py0 * z0_3;
+      { const float r = px1 * y4_0_5;
+        W[20] = fmaf(-0.20412414523193148f, r, W[20]);
+        W[22] = fmaf(-0.15811388300841894f, r, W[22]);
+      }
+      }
+      {
+      const float y5_0_5 = py5 * z0_5 + 20.0f * py3 * z0_4 + 60.0f * py1 * z0_3;
+      { const float r = px0 * y5_0_5;
+        W[28] = fmaf(-0.20412414523193148f, r, W[28]);
+        W[30] = fmaf(-0.15811388300841894f, r, W[30]);
+      }
+      }
+      }
+      {
+      const float z1_1 = pz1 * g1;
+      const float z1_2 = pz1 * g2;
+      const float z1_3 = pz1 * g3;
+      const float z1_4 = pz1 * g4;
+      const float z1_5 = pz1 * g5;
+      {
+      const float y0_1_3 = py0 * z1_3;
+      const float y0_1_2 = py0 * z1_2;
+      { const float r = px2 * y0_1_3 + 2.0f * px0 * y0_1_2;
+        W[3] = fmaf(-0.38729833462074165f, r, W[3]);
+      }
+      const float y0_1_4 = py0 * z1_4;
+      { const float r = px3 * y0_1_4 + 6.0f * px1 * y0_1_3;
+        W[7] = fmaf(-0.38729833462074165f, r, W[7]);
+      }
+      const float y0_1_5 = py0 * z1_5;
+      { const float r = px4 * y0_1_5 + 12.0f * px2 * y0_1_4 + 12.0f * px0 * y0_1_3;
+        W[19] = fmaf(-0.38729833462074165f, r, W[19]);
+      }
+      }
+      {
+      const float y1_1_3 = py1 * z1_3;
+      { const float r = px1 * y1_1_3;
+        W[1] = fmaf(1.0f, r, W[1]);
+      }
+      const float y1_1_4 = py1 * z1_4;
+      { const float r = px2 * y1_1_4 + 2.0f * px0 * y1_1_3;
+        W[12] = fmaf(0.6123724356957945f, r, W[12]);
+        W[5] = fmaf(1.0f, r, W[5]);
+        W[14] = fmaf(-0.15811388300841894f, r, W[14]);
+        W[11] = fmaf(-0.38729833462074165f, r, W[11]);
+      }
+      const float y1_1_5 = py1 * z1_5;
+      { const float r = px3 * y1_1_5 + 6.0f * px1 * y1_1_4;
+        W[24] = fmaf(0.6123724356957945f, r, W[24]);
+        W[17] = fmaf(1.0f, r, W[17]);
+        W[26] = fmaf(-0.15811388300841894f, r, W[26]);
+        W[23] = fmaf(-0.38729833462074165f, r, W[23]);
+      }
+      }
+      {
+      const float y2_1_3 = py2 * z1_3 + 2.0f * py0 * z1_2;
+      { const float r = px0 * y2_1_3;
+        W[3] = fmaf(-0.38729833462074165f, r, W[3]);
+      }
+      const float y2_1_4 = py2 * z1_4 + 2.0f * py0 * z1_3;
+      { const float r = px1 * y2_1_4;
+        W[9] = fmaf(1.0f, r, W[9]);
+        W[7] = fmaf(-0.38729833462074165f, r, W[7]);
+      }
+      const float y2_1_5 = py2 * z1_5 + 2.0f * py0 * z1_4;
+      { const float r = px2 * y2_1_5 + 2.0f * px0 * y2_1_4;
+        W[32] = fmaf(0.6123724356957945f, r, W[32]);
+        W[21] = fmaf(1.0f, r, W[21]);
+        W[34] = fmaf(-0.15811388300841894f, r, W[34]);
+        W[31] = fmaf(-0.38729833462074165f, r, W[31]);
+        W[19] = fmaf(-0.38729833462074165f, r, W[19]);
+      }
+      }
+      {
+      const float y3_1_4 = py3 * z1_4 + 6.0f * py1 * z1_3;
+      { const float r = px0 * y3_1_4;
+        W[12] = fmaf(-0.20412414523193148f, r, W[12]);
+        W[14] = fmaf(-0.15811388300841894f, r, W[14]);
+        W[11] = fmaf(-0.38729833462074165f, r, W[11]);
+      }
+      const float y3_1_5 = py3 * z1_5 + 6.0f * py1 * z1_4;
+      { const float r = px1 * y3_1_5;
+        W[24] = fmaf(-0.20412414523193148f, r, W[24]);
+        W[29] = fmaf(1.0f, r, W[29]);
+        W[26] = fmaf(-0.15811388300841894f, r, W[26]);
+        W[23] = fmaf(-0.38729833462074165f, r, W[23]);
+      }
+      }
+      {
+      const float y4_1_5 = py4 * z1_5 + 12.0f * py2 * z1_4 + 12.0f * py0 * z1_3;
+      { const float r = px0 * y4_1_5;
+        W[32] = fmaf(-0.20412414523193148f, r, W[32]);
+        W[34] = fmaf(-0.15811388300841894f, r, W[34]);
+        W[31] = fmaf(-0.38729833462074165f, r, W[31]);
+      }
+      }
+      }
+      {
+      const float z2_2 = pz2 * g2 + 2.0f * pz0 * g1;
+      const float z2_3 = pz2 * g3 + 2.0f * pz0 * g2;
+      const float z2_4 = pz2 * g4 + 2.0f * pz0 * g3;
+      const float z2_5 = pz2 * g5 + 2.0f * pz0 * g4;
+      {
+      const float y0_2_4 = py0 * z2_4;
+      const float y0_2_3 = py0 * z2_3;
+      { const float r = px2 * y0_2_4 + 2.0f * px0 * y0_2_3;
+        W[15] = fmaf(-0.38729833462074165f, r, W[15]);
+      }
+      const float y0_2_5 = py0 * z2_5;
+      { const float r = px3 * y0_2_5 + 6.0f * px1 * y0_2_4;
+        W[27] = fmaf(-0.38729833462074165f, r, W[27]);
+      }
+      }
+      {
+      const float y1_2_3 = py1 * z2_3;
+      { const float r = px0 * y1_2_3;
+        W[2] = fmaf(0.6324555320336758f, r, W[2]);
+      }
+      const float y1_2_4 = py1 * z2_4;
+      { const float r = px1 * y1_2_4;
+        W[13] = fmaf(1.0f, r, W[13]);
+        W[6] = fmaf(0.6324555320336758f, r, W[6]);
+      }
+      const float y1_2_5 = py1 * z2_5;
+      { const float r = px2 * y1_2_5 + 2.0f * px0 * y1_2_4;
+        W[36] = fmaf(0.6123724356957945f, r, W[36]);
+        W[25] = fmaf(1.0f, r, W[25]);
+        W[38] = fmaf(-0.15811388300841894f, r, W[38]);
+        W[18] = fmaf(0.6324555320336758f, r, W[18]);
+        W[35] = fmaf(-0.38729833462074165f, r, W[35]);
+      }
+      }
+      {
+      const float y2_2_4 = py2 * z2_4 + 2.0f * py0 * z2_3;
+      { const float r = px0 * y2_2_4;
+        W[10] = fmaf(0.6324555320336758f, r, W[10]);
+        W[15] = fmaf(-0.38729833462074165f, r, W[15]);
+      }
+      const float y2_2_5 = py2 * z2_5 + 2.0f * py0 * z2_4;
+      { const float r = px1 * y2_2_5;
+        W[33] = fmaf(1.0f, r, W[33]);
+        W[22] = fmaf(0.6324555320336758f, r, W[22]);
+        W[27] = fmaf(-0.38729833462074165f, r, W[27]);
+      }
+      }
+      {
+      const float y3_2_5 = py3 * z2_5 + 6.0f * py1 * z2_4;
+      { const float r = px0 * y3_2_5;
+        W[36] = fmaf(-0.20412414523193148f, r, W[36]);
+        W[38] = fmaf(-0.15811388300841894f, r, W[38]);
+        W[30] = fmaf(0.6324555320336758f, r, W[30]);
+        W[35] = fmaf(-0.38729833462074165f, r, W[35]);
+      }
+      }
+      }
+      {
+      const float z3_3 = pz3 * g3 + 6.0f * pz1 * g2;
+      const float z3_4 = pz3 * g4 + 6.0f * pz1 * g3;
+      const float z3_5 = pz3 * g5 + 6.0f * pz1 * g4;
+      {
+      const float y0_3_3 = py0 * z3_3;
+      { const float r = px0 * y0_3_3;
+        W[3] = fmaf(0.2581988897471611f, r, W[3]);
+      }
+      const float y0_3_4 = py0 * z3_4;
+      { const float r = px1 * y0_3_4;
+        W[7] = fmaf(0.2581988897471611f, r, W[7]);
+      }
+      const float y0_3_5 = py0 * z3_5;
+      { const float r = px2 * y0_3_5 + 2.0f * px0 * y0_3_4;
+        W[39] = fmaf(-0.38729833462074165f, r, W[39]);
+        W[19] = fmaf(0.2581988897471611f, r, W[19]);
+      }
+      }
+      {
+      const float y1_3_4 = py1 * z3_4;
+      { const float r = px0 * y1_3_4;
+        W[14] = fmaf(0.6324555320336758f, r, W[14]);
+        W[11] = fmaf(0.2581988897471611f, r, W[11]);
+      }
+      const float y1_3_5 = py1 * z3_5;
+      { const float r = px1 * y1_3_5;
+        W[37] = fmaf(1.0f, r, W[37]);
+        W[26] = fmaf(0.6324555320336758f, r, W[26]);
+        W[23] = fmaf(0.2581988897471611f, r, W[23]);
+      }
+      }
+      {
+      const float y2_3_5 = py2 * z3_5 + 2.0f * py0 * z3_4;
+      { const float r = px0 * y2_3_5;
+        W[34] = fmaf(0.6324555320336758f, r, W[34]);
+        W[39] = fmaf(-0.38729833462074165f, r, W[39]);
+        W[31] = fmaf(0.2581988897471611f, r, W[31]);
+      }
+      }
+      }
+      {
+      const float z4_4 = pz4 * g4 + 12.0f * pz2 * g3 + 12.0f * pz0 * g2;
+      const float z4_5 = pz4 * g5 + 12.0f * pz2 * g4 + 12.0f * pz0 * g3;
+      {
+      const float y0_4_4 = py0 * z4_4;
+      { const float r = px0 * y0_4_4;
+        W[15] = fmaf(0.2581988897471611f, r, W[15]);
+      }
+      const float y0_4_5 = py0 * z4_5;
+      { const float r = px1 * y0_4_5;
+        W[27] = fmaf(0.2581988897471611f, r, W[27]);
+      }
+      }
+      {
+      const float y1_4_5 = py1 * z4_5;
+      { const float r = px0 * y1_4_5;
+        W[38] = fmaf(0.6324555320336758f, r, W[38]);
+        W[35] = fmaf(0.2581988897471611f, r, W[35]);
+      }
+      }
+      }
+      {
+      const float z5_5 = pz5 * g5 + 20.0f * pz3 * g4 + 60.0f * pz1 * g3;
+      {
+      const float y0_5_5 = py0 * z5_5;
+      { const float r = px0 * y0_5_5;
+        W[39] = fmaf(0.2581988897471611f, r, W[39]);
+      }
+      }
+      }
+      { const float e = rf[4];
+        accf[0] = fmaf(e, W[0], accf[0]);
+        accf[1] = fmaf(e, W[1], accf[1]);
+        accf[2] = fmaf(e, W[2], accf[2]);
+        accf[3] = fmaf(e, W[3], accf[3]);
+      }
+      { const float e = rf[5];
+        accf[0] = fmaf(e, W[4], accf[0]);
+        accf[1] = fmaf(e, W[5], accf[1]);
+        accf[2] = fmaf(e, W[6], accf[2]);
+        accf[3] = fmaf(e, W[7], accf[3]);
+      }
+      { const float e = rf[6];
+        accf[0] = fmaf(e, W[16], accf[0]);
+        accf[1] = fmaf(e, W[17], accf[1]);
+        accf[2] = fmaf(e, W[18], accf[2]);
+        accf[3] = fmaf(e, W[19], accf[3]);
+      }
+      { const float e = rf[7];
+        accf[4] = fmaf(e, W[0], accf[4]);
+        accf[5] = fmaf(e, W[1], accf[5]);
+        accf[6] = fmaf(e, W[2], accf[6]);
+        accf[7] = fmaf(e, W[3], accf[7]);
+      }
+      { const float e = rf[8];
+        accf[4] = fmaf(e, W[8], accf[4]);
+        accf[5] = fmaf(e, W[9], accf[5]);
+        accf[6] = fmaf(e, W[10], accf[6]);
+        accf[7] = fmaf(e, W[11], accf[7]);
+      }
+      { const float e = rf[9];
+        accf[4] = fmaf(e, W[4], accf[4]);
+        accf[5] = fmaf(e, W[5], accf[5]);
+        accf[6] = fmaf(e, W[6], accf[6]);
+        accf[7] = fmaf(e, W[7], accf[7]);
+      }
+      { const float e = rf[10];
+        accf[4] = fmaf(e, W[20], accf[4]);
+        accf[5] = fmaf(e, W[21], accf[5]);
+        accf[6] = fmaf(e, W[22], accf[6]);
+        accf[7] = fmaf(e, W[23], accf[7]);
+      }
+      { const float e = rf[11];
+        accf[8] = fmaf(e, W[0], accf[8]);
+        accf[9] = fmaf(e, W[1], accf[9]);
+        accf[10] = fmaf(e, W[2], accf[10]);
+        accf[11] = fmaf(e, W[3], accf[11]);
+      }
+      { const float e = rf[12];
+        accf[8] = fmaf(e, W[12], accf[8]);
+        accf[9] = fmaf(e, W[13], accf[9]);
+        accf[10] = fmaf(e, W[14], accf[10]);
+        accf[11] = fmaf(e, W[15], accf[11]);
+      }
+      { const float e = rf[13];
+        accf[8] = fmaf(e, W[4], accf[8]);
+        accf[9] = fmaf(e, W[5], accf[9]);
+        accf[10] = fmaf(e, W[6], accf[10]);
+        accf[11] = fmaf(e, W[7], accf[11]);
+      }
+      { const float e = rf[14];
+        accf[8] = fmaf(e, W[24], accf[8]);
+        accf[9] = fmaf(e, W[25], accf[9]);
+        accf[10] = fmaf(e, W[26], accf[10]);
+        accf[11] = fmaf(e, W[27], accf[11]);
+      }
+      { const float e = rf[15];
+        accf[12] = fmaf(e, W[0], accf[12]);
+        accf[13] = fmaf(e, W[1], accf[13]);
+        accf[14] = fmaf(e, W[2], accf[14]);
+        accf[15] = fmaf(e, W[3], accf[15]);
+      }
+      { const float e = rf[16];
+        accf[12] = fmaf(e, W[8], accf[12]);
+        accf[13] = fmaf(e, W[9], accf[13]);
+        accf[14] = fmaf(e, W[10], accf[14]);
+        accf[15] = fmaf(e, W[11], accf[15]);
+      }
+      { const float e = rf[17];
+        accf[12] = fmaf(e, W[4], accf[12]);
+        accf[13] = fmaf(e, W[5], accf[13]);
+        accf[14] = fmaf(e, W[6], accf[14]);
+        accf[15] = fmaf(e, W[7], accf[15]);
+      }
+      { const float e = rf[18];
+        accf[12] = fmaf(e, W[20], accf[12]);
+        accf[13] = fmaf(e, W[21], accf[13]);
+        accf[14] = fmaf(e, W[22], accf[14]);
+        accf[15] = fmaf(e, W[23], accf[15]);
+      }
+      { const float e = rf[19];
+        accf[16] = fmaf(e, W[0], accf[16]);
+        accf[17] = fmaf(e, W[1], accf[17]);
+        accf[18] = fmaf(e, W[2], accf[18]);
+        accf[19] = fmaf(e, W[3], accf[19]);
+      }
+      { const float e = rf[20];
+        accf[16] = fmaf(e, W[8], accf[16]);
+        accf[17] = fmaf(e, W[9], accf[17]);
+        accf[18] = fmaf(e, W[10], accf[18]);
+        accf[19] = fmaf(e, W[11], accf[19]);
+      }
+      { const float e = rf[21];
+        accf[16] = fmaf(e, W[28], accf[16]);
+        accf[17] = fmaf(e, W[29], accf[17]);
+        accf[18] = fmaf(e, W[30], accf[18]);
+        accf[19] = fmaf(e, W[31], accf[19]);
+      }
+      { const float e = rf[22];
+        accf[20] = fmaf(e, W[0], accf[20]);
+        accf[21] = fmaf(e, W[1], accf[21]);
+        accf[22] = fmaf(e, W[2], accf[22]);
+        accf[23] = fmaf(e, W[3], accf[23]);
+      }
+      { const float e = rf[23];
+        accf[20] = fmaf(e, W[12], accf[20]);
+        accf[21] = fmaf(e, W[13], accf[21]);
+        accf[22] = fmaf(e, W[14], accf[22]);
+        accf[23] = fmaf(e, W[15], accf[23]);
+      }
+      { const float e = rf[24];
+        accf[20] = fmaf(e, W[8], accf[20]);
+        accf[21] = fmaf(e, W[9], accf[21]);
+        accf[22] = fmaf(e, W[10], accf[22]);
+        accf[23] = fmaf(e, W[11], accf[23]);
+      }
+      { const float e = rf[25];
+        accf[20] = fmaf(e, W[32], accf[20]);
+        accf[21] = fmaf(e, W[33], accf[21]);
+        accf[22] = fmaf(e, W[34], accf[22]);
+        accf[23] = fmaf(e, W[35], accf[23]);
+      }
+      { const float e = rf[26];
+        accf[24] = fmaf(e, W[0], accf[24]);
+        accf[25] = fmaf(e, W[1], accf[25]);
+        accf[26] = fmaf(e, W[2], accf[26]);
+        accf[27] = fmaf(e, W[3], accf[27]);
+      }
+      { const float e = rf[27];
+        accf[24] = fmaf(e, W[12], accf[24]);
+        accf[25] = fmaf(e, W[13], accf[25]);
+        accf[26] = fmaf(e, W[14], accf[26]);
+        accf[27] = fmaf(e, W[15], accf[27]);
+      }
+      { const float e = rf[28];
+        accf[24] = fmaf(e, W[4], accf[24]);
+        accf[25] = fmaf(e, W[5], accf[25]);
+        accf[26] = fmaf(e, W[6], accf[26]);
+        accf[27] = fmaf(e, W[7], accf[27]);
+      }
+      { const float e = rf[29];
+        accf[24] = fmaf(e, W[24], accf[24]);
+        accf[25] = fmaf(e, W[25], accf[25]);
+        accf[26] = fmaf(e, W[26], accf[26]);
+        accf[27] = fmaf(e, W[27], accf[27]);
+      }
+      { const float e = rf[30];
+        accf[28] = fmaf(e, W[0], accf[28]);
+        accf[29] = fmaf(e, W[1], accf[29]);
+        accf[30] = fmaf(e, W[2], accf[30]);
+        accf[31] = fmaf(e, W[3], accf[31]);
+      }
+      { const float e = rf[31];
+        accf[28] = fmaf(e, W[12], accf[28]);
+        accf[29] = fmaf(e, W[13], accf[29]);
+        accf[30] = fmaf(e, W[14], accf[30]);
+        accf[31] = fmaf(e, W[15], accf[31]);
+      }
+      { const float e = rf[32];
+        accf[28] = fmaf(e, W[8], accf[28]);
+        accf[29] = fmaf(e, W[9], accf[29]);
+        accf[30] = fmaf(e, W[10], accf[30]);
+        accf[31] = fmaf(e, W[11], accf[31]);
+      }
+      { const float e = rf[33];
+        accf[28] = fmaf(e, W[32], accf[28]);
+        accf[29] = fmaf(e, W[33], accf[29]);
+        accf[30] = fmaf(e, W[34], accf[30]);
+        accf[31] = fmaf(e, W[35], accf[31]);
+      }
+      { const float e = rf[34];
+        accf[32] = fmaf(e, W[0], accf[32]);
+        accf[33] = fmaf(e, W[1], accf[33]);
+        accf[34] = fmaf(e, W[2], accf[34]);
+        accf[35] = fmaf(e, W[3], accf[35]);
+      }
+      { const float e = rf[35];
+        accf[32] = fmaf(e, W[12], accf[32]);
+        accf[33] = fmaf(e, W[13], accf[33]);
+        accf[34] = fmaf(e, W[14], accf[34]);
+        accf[35] = fmaf(e, W[15], accf[35]);
+      }
+      { const float e = rf[36];
+        accf[32] = fmaf(e, W[36], accf[32]);
+        accf[33] = fmaf(e, W[37], accf[33]);
+        accf[34] = fmaf(e, W[38], accf[34]);
+        accf[35] = fmaf(e, W[39], accf[35]);
+      }
+      }
+#pragma unroll
+      for (int q = 0; q < 36; q++) acc[q] += (double)accf[q];
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 0] = 1.0 * acc[16];
+    out[(row0 + 0) * ldo + col + 1] = 1.0 * acc[17];
+    out[(row0 + 0) * ldo + col + 2] = 1.0 * acc[18];
+    out[(row0 + 0) * ldo + col + 3] = 1.0 * acc[19];
+    out[(row0 + 1) * ldo + col + 0] = 1.0 * acc[20];
+    out[(row0 + 1) * ldo + col + 1] = 1.0 * acc[21];
+    out[(row0 + 1) * ldo + col + 2] = 1.0 * acc[22];
+    out[(row0 + 1) * ldo + col + 3] = 1.0 * acc[23];
+    out[(row0 + 2) * ldo + col + 0] = 1.0 * acc[12];
+    out[(row0 + 2) * ldo + col + 1] = 1.0 * acc[13];
+    out[(row0 + 2) * ldo + col + 2] = 1.0 * acc[14];
+    out[(row0 + 2) * ldo + col + 3] = 1.0 * acc[15];
+    out[(row0 + 3) * ldo + col + 0] = 1.0 * acc[28];
+    out[(row0 + 3) * ldo + col + 1] = 1.0 * acc[29];
+    out[(row0 + 3) * ldo + col + 2] = 1.0 * acc[30];
+    out[(row0 + 3) * ldo + col + 3] = 1.0 * acc[31];
+    out[(row0 + 4) * ldo + col + 0] = 1.0 * acc[32];
+    out[(row0 + 4) * ldo + col + 1] = 1.0 * acc[33];
+    out[(row0 + 4) * ldo + col + 2] = 1.0 * acc[34];
+    out[(row0 + 4) * ldo + col + 3] = 1.0 * acc[35];
+    out[(row0 + 5) * ldo + col + 0] = 1.0 * acc[24];
+    out[(row0 + 5) * ldo + col + 1] = 1.0 * acc[25];
+    out[(row0 + 5) * ldo + col + 2] = 1.0 * acc[26];
+    out[(row0 + 5) * ldo + col + 3] = 1.0 * acc[27];
+    out[(row0 + 6) * ldo + col + 0] = 1.0 * acc[4];
+    out[(row0 + 6) * ldo + col + 1] = 1.0 * acc[5];
+    out[(row0 + 6) * ldo + col + 2] = 1.0 * acc[6];
+    out[(row0 + 6) * ldo + col + 3] = 1.0 * acc[7];
+    out[(row0 + 7) * ldo + col + 0] = 1.0 * acc[8];
+    out[(row0 + 7) * ldo + col + 1] = 1.0 * acc[9];
+    out[(row0 + 7) * ldo + col + 2] = 1.0 * acc[10];
+    out[(row0 + 7) * ldo + col + 3] = 1.0 * acc[11];
+    out[(row0 + 8) * ldo + col + 0] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 1] = 1.0 * acc[1];
+    out[(row0 + 8) * ldo + col + 2] = 1.0 * acc[2];
+    out[(row0 + 8) * ldo + col + 3] = 1.0 * acc[3];
+    }
+    break;
+  }
+  case 1: {
+    double acc[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) acc[q] = 0.0;
+    for (int ij = 0; ij < npp; ij++) {
+      const double* __restrict__ rec = recp + (size_t)ij * 37;
+      const double p = rec[0];
+      const float* __restrict__ rf = recp32 + (size_t)ij * 37;
+      const double PCxd = rec[1] - Cx, PCyd = rec[2] - Cy, PCzd = rec[3] - Cz;
+      const float PCx = (float)PCxd, PCy = (float)PCyd, PCz = (float)PCzd;
+      float accf[27];
+#pragma unroll
+      for (int q = 0; q < 27; q++) accf[q] = 0.0f;
+      for (int kp = 0; kp < nprim_c; kp++) {
+      const double gam = ket.exps[pc0 + kp];
+      const double alpha = p * gam / (p + gam);
+      const double T = alpha * (PCxd * PCxd + PCyd * PCyd + PCzd * PCzd);
+      double sc = 2.0 * PI * PI * sqrt(PI) / (p * gam * sqrt(p + gam)) * ket.coefs[pc0 + kp];
+      { const double i2g = 0.5 / gam; sc *= -i2g * i2g * i2g; }
+      double F[6];
+      boys_array<5>(T, boys, F);
+      const double ma = -alpha;
+      const float g0 = (float)(sc * F[0]);
+      sc *= ma; const float g1 = (float)(sc * F[1]);
+      sc *= ma; const float g2 = (float)(sc * F[2]);
+      sc *= ma; const float g3 = (float)(sc * F[3]);
+      sc *= ma; const float g4 = (float)(sc * F[4]);
+      sc *= ma; const float g5 = (float)(sc * F[5]);
+      const float px0 = 1.0f, px1 = 2.0f * PCx;
+      const float px2 = px1 * px1;
+      const float px3 = px2 * px1;
+      const float px4 = px3 * px1;
+      const float px5 = px4 * px1;
+      const float py0 = 1.0f, py1 = 2.0f * PCy;
+      const float py2 = py1 * py1;
+      const float py3 = py2 * py1;
+      const float py4 = py3 * py1;
+      const float py5 = py4 * py1;
+      const float pz0 = 1.0f, pz1 = 2.0f * PCz;
+      const float pz2 = pz1 * pz1;
+      const float pz3 = pz2 * pz1;
+      const float pz4 = pz3 * pz1;
+      const float pz5 = pz4 * pz1;
+      float W[30];
+#pragma unroll
+      for (int q = 0; q < 30; q++) W[q] = 0.0f;
+      {
+      const float z0_0 = pz0 * g0;
+      const float z0_1 = pz0 * g1;
+      const float z0_2 = pz0 * g2;
+      const float z0_3 = pz0 * g3;
+      const float z0_4 = pz0 * g4;
+      const float z0_5 = pz0 * g5;
+      {
+      const float y0_0_3 = py0 * z0_3;
+      const float y0_0_2 = py0 * z0_2;
+      { const float r = px3 * y0_0_3 + 6.0f * px1 * y0_0_2;
+        W[0] = fmaf(-0.15811388300841894f, r, W[0]);
+        W[2] = fmaf(0.20412414523193148f, r, W[2]);
+      }
+      const float y0_0_4 = py0 * z0_4;
+      { const float r = px4 * y0_0_4 + 12.0f * px2 * y0_0_3 + 12.0f * px0 * y0_0_2;
+        W[3] = fmaf(-0.15811388300841894f, r, W[3]);
+        W[5] = fmaf(0.20412414523193148f, r, W[5]);
+      }
+      const float y0_0_5 = py0 * z0_5;
+      { const float r = px5 * y0_0_5 + 20.0f * px3 * y0_0_4 + 60.0f * px1 * y0_0_3;
+        W[12] = fmaf(-0.15811388300841894f, r, W[12]);
+        W[14] = fmaf(0.20412414523193148f, r, W[14]);
+      }
+      }
+      {
+      const float y1_0_4 = py1 * z0_4;
+      const float y1_0_3 = py1 * z0_3;
+      { const float r = px3 * y1_0_4 + 6.0f * px1 * y1_0_3;
+        W[6] = fmaf(-0.15811388300841894f, r, W[6]);
+        W[8] = fmaf(0.20412414523193148f, r, W[8]);
+      }
+      const float y1_0_5 = py1 * z0_5;
+      { const float r = px4 * y1_0_5 + 12.0f * px2 * y1_0_4 + 12.0f * px0 * y1_0_3;
+        W[15] = fmaf(-0.15811388300841894f, r, W[15]);
+        W[17] = fmaf(0.20412414523193148f, r, W[17]);
+      }
+      }
+      {
+      const float y2_0_3 = py2 * z0_3 + 2.0f * py0 * z0_2;
+      { const float r = px1 * y2_0_3;
+        W[0] = fmaf(-0.15811388300841894f, r, W[0]);
+        W[2] = fmaf(-0.6123724356957945f, r, W[2]);
+      }
+      const float y2_0_4 = py2 * z0_4 + 2.0f * py0 * z0_3;
+      { const float r = px2 * y2_0_4 + 2.0f * px0 * y2_0_3;
+        W[3] = fmaf(-0.15811388300841894f, r, W[3]);
+        W[5] = fmaf(-0.6123724356957945f, r, W[5]);
+      }
+      const float y2_0_5 = py2 * z0_5 + 2.0f * py0 * z0_4;
+      { const float r = px3 * y2_0_5 + 6.0f * px1 * y2_0_4;
+        W[21] = fmaf(-0.15811388300841894f, r, W[21]);
+        W[12] = fmaf(-0.15811388300841894f, r, W[12]);
+        W[23] = fmaf(0.20412414523193148f, r, W[23]);
+        W[14] = fmaf(-0.6123724356957945f, r, W[14]);
+      }
+      }
+      {
+      const float y3_0_4 = py3 * z0_4 + 6.0f * py1 * z0_3;
+      { const float r = px1 * y3_0_4;
+        W[6] = fmaf(-0.15811388300841894f, r, W[6]);
+        W[8] = fmaf(-0.6123724356957945f, r, W[8]);
+      }
+      const float y3_0_5 = py3 * z0_5 + 6.0f * py1 * z0_4;
+      { const float r = px2 * y3_0_5 + 2.0f * px0 * y3_0_4;
+        W[15] = fmaf(-0.15811388300841894f, r, W[15]);
+        W[17] = fmaf(-0.6123724356957945f, r, W[17]);
+      }
+      }
+      {
+      const float y4_0_5 = py4 * z0_5 + 12.0f * py2 * z0_4 + 12.0f * py0 * z0_3;
+      { const float r = px1 * y4_0_5;
+        W[21] = fmaf(-0.15811388300841894f, r, W[21]);
+        W[23] = fmaf(-0.6123724356957945f, r, W[23]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const float z1_1 = pz1 * g1;
+      const float z1_2 = pz1 * g2;
+      const float z1_3 = pz1 * g3;
+      const float z1_4 = pz1 * g4;
+      const float z1_5 = pz1 * g5;
+      {
+      const float y0_1_3 = py0 * z1_3;
+      const float y0_1_2 = py0 * z1_2;
+      { const float r = px2 * y0_1_3 + 2.0f * px0 * y0_1_2;
+        W[1] = fmaf(0.5f, r, W[1]);
+      }
+      const float y0_1_4 = py0 * z1_4;
+      { const float r = px3 * y0_1_4 + 6.0f * px1 * y0_1_3;
+        W[9] = fmaf(-0.15811388300841894f, r, W[9]);
+        W[4] = fmaf(0.5f, r, W[4]);
+        W[11] = fmaf(0.20412414523193148f, r, W[11]);
+      }
+      const float y0_1_5 = py0 * z1_5;
+      { const float r = px4 * y0_1_5 + 12.0f * px2 * y0_1_4 + 12.0f * px0 * y0_1_3;
+        W[18] = fmaf(-0.15811388300841894f, r, W[18]);
+        W[13] = fmaf(0.5f, r, W[13]);
+        W[20] = fmaf(0.20412414523193148f, r, W[20]);
+      }
+      }
+      {
+      const float y1_1_4 = py1 * z1_4;
+      const float y1_1_3 = py1 * z1_3;
+      { const float r = px2 * y1_1_4 + 2.0f * px0 * y1_1_3;
+        W[7] = fmaf(0.5f, r, W[7]);
+      }
+      const float y1_1_5 = py1 * z1_5;
+      { const float r = px3 * y1_1_5 + 6.0f * px1 * y1_1_4;
+        W[24] = fmaf(-0.15811388300841894f, r, W[24]);
+        W[16] = fmaf(0.5f, r, W[16]);
+        W[26] = fmaf(0.20412414523193148f, r, W[26]);
+      }
+      }
+      {
+      const float y2_1_3 = py2 * z1_3 + 2.0f * py0 * z1_2;
+      { const float r = px0 * y2_1_3;
+        W[1] = fmaf(-0.5f, r, W[1]);
+      }
+      const float y2_1_4 = py2 * z1_4 + 2.0f * py0 * z1_3;
+      { const float r = px1 * y2_1_4;
+        W[9] = fmaf(-0.15811388300841894f, r, W[9]);
+        W[4] = fmaf(-0.5f, r, W[4]);
+        W[11] = fmaf(-0.6123724356957945f, r, W[11]);
+      }
+      const float y2_1_5 = py2 * z1_5 + 2.0f * py0 * z1_4;
+      { const float r = px2 * y2_1_5 + 2.0f * px0 * y2_1_4;
+        W[18] = fmaf(-0.15811388300841894f, r, W[18]);
+        W[22] = fmaf(0.5f, r, W[22]);
+        W[13] = fmaf(-0.5f, r, W[13]);
+        W[20] = fmaf(-0.6123724356957945f, r, W[20]);
+      }
+      }
+      {
+      const float y3_1_4 = py3 * z1_4 + 6.0f * py1 * z1_3;
+      { const float r = px0 * y3_1_4;
+        W[7] = fmaf(-0.5f, r, W[7]);
+      }
+      const float y3_1_5 = py3 * z1_5 + 6.0f * py1 * z1_4;
+      { const float r = px1 * y3_1_5;
+        W[24] = fmaf(-0.15811388300841894f, r, W[24]);
+        W[16] = fmaf(-0.5f, r, W[16]);
+        W[26] = fmaf(-0.6123724356957945f, r, W[26]);
+      }
+      }
+      {
+      const float y4_1_5 = py4 * z1_5 + 12.0f * py2 * z1_4 + 12.0f * py0 * z1_3;
+      { const float r = px0 * y4_1_5;
+        W[22] = fmaf(-0.5f, r, W[22]);
+      }
+      }
+      }
+      {
+      const float z2_2 = pz2 * g2 + 2.0f * pz0 * g1;
+      const float z2_3 = pz2 * g3 + 2.0f * pz0 * g2;
+      const float z2_4 = pz2 * g4 + 2.0f * pz0 * g3;
+      const float z2_5 = pz2 * g5 + 2.0f * pz0 * g4;
+      {
+      const float y0_2_3 = py0 * z2_3;
+      { const float r = px1 * y0_2_3;
+        W[0] = fmaf(0.6324555320336758f, r, W[0]);
+      }
+      const float y0_2_4 = py0 * z2_4;
+      { const float r = px2 * y0_2_4 + 2.0f * px0 * y0_2_3;
+        W[3] = fmaf(0.6324555320336758f, r, W[3]);
+        W[10] = fmaf(0.5f, r, W[10]);
+      }
+      const float y0_2_5 = py0 * z2_5;
+      { const float r = px3 * y0_2_5 + 6.0f * px1 * y0_2_4;
+        W[27] = fmaf(-0.15811388300841894f, r, W[27]);
+        W[12] = fmaf(0.6324555320336758f, r, W[12]);
+        W[19] = fmaf(0.5f, r, W[19]);
+        W[29] = fmaf(0.20412414523193148f, r, W[29]);
+      }
+      }
+      {
+      const float y1_2_4 = py1 * z2_4;
+      { const float r = px1 * y1_2_4;
+        W[6] = fmaf(0.6324555320336758f, r, W[6]);
+      }
+      const float y1_2_5 = py1 * z2_5;
+      { const float r = px2 * y1_2_5 + 2.0f * px0 * y1_2_4;
+        W[15] = fmaf(0.6324555320336758f, r, W[15]);
+        W[25] = fmaf(0.5f, r, W[25]);
+      }
+      }
+      {
+      const float y2_2_4 = py2 * z2_4 + 2.0f * py0 * z2_3;
+      { const float r = px0 * y2_2_4;
+        W[10] = fmaf(-0.5f, r, W[10]);
+      }
+      const float y2_2_5 = py2 * z2_5 + 2.0f * py0 * z2_4;
+      { const float r = px1 * y2_2_5;
+        W[27] = fmaf(-0.15811388300841894f, r, W[27]);
+        W[21] = fmaf(0.6324555320336758f, r, W[21]);
+        W[19] = fmaf(-0.5f, r, W[19]);
+        W[29] = fmaf(-0.6123724356957945f, r, W[29]);
+      }
+      }
+      {
+      const float y3_2_5 = py3 * z2_5 + 6.0f * py1 * z2_4;
+      { const float r = px0 * y3_2_5;
+        W[25] = fmaf(-0.5f, r, W[25]);
+      }
+      }
+      }
+      {
+      const float z3_3 = pz3 * g3 + 6.0f * pz1 * g2;
+      const float z3_4 = pz3 * g4 + 6.0f * pz1 * g3;
+      const float z3_5 = pz3 * g5 + 6.0f * pz1 * g4;
+      {
+      const float y0_3_4 = py0 * z3_4;
+      { const float r = px1 * y0_3_4;
+        W[9] = fmaf(0.6324555320336758f, r, W[9]);
+      }
+      const float y0_3_5 = py0 * z3_5;
+      { const float r = px2 * y0_3_5 + 2.0f * px0 * y0_3_4;
+        W[18] = fmaf(0.6324555320336758f, r, W[18]);
+        W[28] = fmaf(0.5f, r, W[28]);
+      }
+      }
+      {
+      const float y1_3_5 = py1 * z3_5;
+      { const float r = px1 * y1_3_5;
+        W[24] = fmaf(0.6324555320336758f, r, W[24]);
+      }
+      }
+      {
+      const float y2_3_5 = py2 * z3_5 + 2.0f * py0 * z3_4;
+      { const float r = px0 * y2_3_5;
+        W[28] = fmaf(-0.5f, r, W[28]);
+      }
+      }
+      }
+      {
+      const float z4_4 = pz4 * g4 + 12.0f * pz2 * g3 + 12.0f * pz0 * g2;
+      const float z4_5 = pz4 * g5 + 12.0f * pz2 * g4 + 12.0f * pz0 * g3;
+      {
+      const float y0_4_5 = py0 * z4_5;
+      { const float r = px1 * y0_4_5;
+        W[27] = fmaf(0.6324555320336758f, r, W[27]);
+      }
+      }
+      {
+      }
+      }
+      {
+      const float z5_5 = pz5 * g5 + 20.0f * pz3 * g4 + 60.0f * pz1 * g3;
+      {
+      }
+      }
+      { const float e = rf[4];
+        accf[0] = fmaf(e, W[0], accf[0]);
+        accf[1] = fmaf(e, W[1], accf[1]);
+        accf[2] = fmaf(e, W[2], accf[2]);
+      }
+      { const float e = rf[5];
+        accf[0] = fmaf(e, W[3], accf[0]);
+        accf[1] = fmaf(e, W[4], accf[1]);
+        accf[2] = fmaf(e, W[5], accf[2]);
+      }
+      { const float e = rf[6];
+        accf[0] = fmaf(e, W[12], accf[0]);
+        accf[1] = fmaf(e, W[13], accf[1]);
+        accf[2] = fmaf(e, W[14], accf[2]);
+      }
+      { const float e = rf[7];
+        accf[3] = fmaf(e, W[0], accf[3]);
+        accf[4] = fmaf(e, W[1], accf[4]);
+        accf[5] = fmaf(e, W[2], accf[5]);
+      }
+      { const float e = rf[8];
+        accf[3] = fmaf(e, W[6], accf[3]);
+        accf[4] = fmaf(e, W[7], accf[4]);
+        accf[5] = fmaf(e, W[8], accf[5]);
+      }
+      { const float e = rf[9];
+        accf[3] = fmaf(e, W[3], accf[3]);
+        accf[4] = fmaf(e, W[4], accf[4]);
+        accf[5] = fmaf(e, W[5], accf[5]);
+      }
+      { const float e = rf[10];
+        accf[3] = fmaf(e, W[15], accf[3]);
+        accf[4] = fmaf(e, W[16], accf[4]);
+        accf[5] = fmaf(e, W[17], accf[5]);
+      }
+      { const float e = rf[11];
+        accf[6] = fmaf(e, W[0], accf[6]);
+        accf[7] = fmaf(e, W[1], accf[7]);
+        accf[8] = fmaf(e, W[2], accf[8]);
+      }
+      { const float e = rf[12];
+        accf[6] = fmaf(e, W[9], accf[6]);
+        accf[7] = fmaf(e, W[10], accf[7]);
+        accf[8] = fmaf(e, W[11], accf[8]);
+      }
+      { const float e = rf[13];
+        accf[6] = fmaf(e, W[3], accf[6]);
+        accf[7] = fmaf(e, W[4], accf[7]);
+        accf[8] = fmaf(e, W[5], accf[8]);
+      }
+      { const float e = rf[14];
+        accf[6] = fmaf(e, W[18], accf[6]);
+        accf[7] = fmaf(e, W[19], accf[7]);
+        accf[8] = fmaf(e, W[20], accf[8]);
+      }
+      { const float e = rf[15];
+        accf[9] = fmaf(e, W[0], accf[9]);
+        accf[10] = fmaf(e, W[1], accf[10]);
+        accf[11] = fmaf(e, W[2], accf[11]);
+      }
+      { const float e = rf[16];
+        accf[9] = fmaf(e, W[6], accf[9]);
+        accf[10] = fmaf(e, W[7], accf[10]);
+        accf[11] = fmaf(e, W[8], accf[11]);
+      }
+      { const float e = rf[17];
+        accf[9] = fmaf(e, W[3], accf[9]);
+        accf[10] = fmaf(e, W[4], accf[10]);
+        accf[11] = fmaf(e, W[5], accf[11]);
+      }
+      { const float e = rf[18];
+        accf[9] = fmaf(e, W[15], accf[9]);
+        accf[10] = fmaf(e, W[16], accf[10]);
+        accf[11] = fmaf(e, W[17], accf[11]);
+      }
+      { const float e = rf[19];
+        accf[12] = fmaf(e, W[0], accf[12]);
+        accf[13] = fmaf(e, W[1], accf[13]);
+        accf[14] = fmaf(e, W[2], accf[14]);
+      }
+      { const float e = rf[20];
+        accf[12] = fmaf(e, W[6], accf[12]);
+        accf[13] = fmaf(e, W[7], accf[13]);
+        accf[14] = fmaf(e, W[8], accf[14]);
+      }
+      { const float e = rf[21];
+        accf[12] = fmaf(e, W[21], accf[12]);
+        accf[13] = fmaf(e, W[22], accf[13]);
+        accf[14] = fmaf(e, W[23], accf[14]);
+      }
+      { const float e = rf[22];
+        accf[15] = fmaf(e, W[0], accf[15]);
+        accf[16] = fmaf(e, W[1], accf[16]);
+        accf[17] = fmaf(e, W[2], accf[17]);
+      }
+      { const float e = rf[23];
+        accf[15] = fmaf(e, W[9], accf[15]);
+        accf[16] = fmaf(e, W[10], accf[16]);
+        accf[17] = fmaf(e, W[11], accf[17]);
+      }
+      { const float e = rf[24];
+        accf[15] = fmaf(e, W[6], accf[15]);
+        accf[16] = fmaf(e, W[7], accf[16]);
+        accf[17] = fmaf(e, W[8], accf[17]);
+      }
+      { const float e = rf[25];
+        accf[15] = fmaf(e, W[24], accf[15]);
+        accf[16] = fmaf(e, W[25], accf[16]);
+        accf[17] = fmaf(e, W[26], accf[17]);
+      }
+      { const float e = rf[26];
+        accf[18] = fmaf(e, W[0], accf[18]);
+        accf[19] = fmaf(e, W[1], accf[19]);
+        accf[20] = fmaf(e, W[2], accf[20]);
+      }
+      { const float e = rf[27];
+        accf[18] = fmaf(e, W[9], accf[18]);
+        accf[19] = fmaf(e, W[10], accf[19]);
+        accf[20] = fmaf(e, W[11], accf[20]);
+      }
+      { const float e = rf[28];
+        accf[18] = fmaf(e, W[3], accf[18]);
+        accf[19] = fmaf(e, W[4], accf[19]);
+        accf[20] = fmaf(e, W[5], accf[20]);
+      }
+      { const float e = rf[29];
+        accf[18] = fmaf(e, W[18], accf[18]);
+        accf[19] = fmaf(e, W[19], accf[19]);
+        accf[20] = fmaf(e, W[20], accf[20]);
+      }
+      { const float e = rf[30];
+        accf[21] = fmaf(e, W[0], accf[21]);
+        accf[22] = fmaf(e, W[1], accf[22]);
+        accf[23] = fmaf(e, W[2], accf[23]);
+      }
+      { const float e = rf[31];
+        accf[21] = fmaf(e, W[9], accf[21]);
+        accf[22] = fmaf(e, W[10], accf[22]);
+        accf[23] = fmaf(e, W[11], accf[23]);
+      }
+      { const float e = rf[32];
+        accf[21] = fmaf(e, W[6], accf[21]);
+        accf[22] = fmaf(e, W[7], accf[22]);
+        accf[23] = fmaf(e, W[8], accf[23]);
+      }
+      { const float e = rf[33];
+        accf[21] = fmaf(e, W[24], accf[21]);
+        accf[22] = fmaf(e, W[25], accf[22]);
+        accf[23] = fmaf(e, W[26], accf[23]);
+      }
+      { const float e = rf[34];
+        accf[24] = fmaf(e, W[0], accf[24]);
+        accf[25] = fmaf(e, W[1], accf[25]);
+        accf[26] = fmaf(e, W[2], accf[26]);
+      }
+      { const float e = rf[35];
+        accf[24] = fmaf(e, W[9], accf[24]);
+        accf[25] = fmaf(e, W[10], accf[25]);
+        accf[26] = fmaf(e, W[11], accf[26]);
+      }
+      { const float e = rf[36];
+        accf[24] = fmaf(e, W[27], accf[24]);
+        accf[25] = fmaf(e, W[28], accf[25]);
+        accf[26] = fmaf(e, W[29], accf[26]);
+      }
+      }
+#pragma unroll
+      for (int q = 0; q < 27; q++) acc[q] += (double)accf[q];
+    }
+    if (active) {
+    out[(row0 + 0) * ldo + col + 4] = 1.0 * acc[12];
+    out[(row0 + 0) * ldo + col + 5] = 1.0 * acc[13];
+    out[(row0 + 0) * ldo + col + 6] = 1.0 * acc[14];
+    out[(row0 + 1) * ldo + col + 4] = 1.0 * acc[15];
+    out[(row0 + 1) * ldo + col + 5] = 1.0 * acc[16];
+    out[(row0 + 1) * ldo + col + 6] = 1.0 * acc[17];
+    out[(row0 + 2) * ldo + col + 4] = 1.0 * acc[9];
+    out[(row0 + 2) * ldo + col + 5] = 1.0 * acc[10];
+    out[(row0 + 2) * ldo + col + 6] = 1.0 * acc[11];
+    out[(row0 + 3) * ldo + col + 4] = 1.0 * acc[21];
+    out[(row0 + 3) * ldo + col + 5] = 1.0 * acc[22];
+    out[(row0 + 3) * ldo + col + 6] = 1.0 * acc[23];
+    out[(row0 + 4) * ldo + col + 4] = 1.0 * acc[24];
+    out[(row0 + 4) * ldo + col + 5] = 1.0 * acc[25];
+    out[(row0 + 4) * ldo + col + 6] = 1.0 * acc[26];
+    out[(row0 + 5) * ldo + col + 4] = 1.0 * acc[18];
+    out[(row0 + 5) * ldo + col + 5] = 1.0 * acc[19];
+    out[(row0 + 5) * ldo + col + 6] = 1.0 * acc[20];
+    out[(row0 + 6) * ldo + col + 4] = 1.0 * acc[3];
+    out[(row0 + 6) * ldo + col + 5] = 1.0 * acc[4];
+    out[(row0 + 6) * ldo + col + 6] = 1.0 * acc[5];
+    out[(row0 + 7) * ldo + col + 4] = 1.0 * acc[6];
+    out[(row0 + 7) * ldo + col + 5] = 1.0 * acc[7];
+    out[(row0 + 7) * ldo + col + 6] = 1.0 * acc[8];
+    out[(row0 + 8) * ldo + col + 4] = 1.0 * acc[0];
+    out[(row0 + 8) * ldo + col + 5] = 1.0 * acc[1];
+    out[(row0 + 8) * ldo + col + 6] = 1.0 * acc[2];
+    }
+    break;
+  }
+  default: break;
+  }
+}
+
 void register_gen3_11(Gen3Table& t) {
-  t[1][1][0] = Gen3Launch{&int3c2e_gen_110, 1, 37};
-  t[1][1][1] = Gen3Launch{&int3c2e_gen_111, 2, 37};
-  t[1][1][2] = Gen3Launch{&int3c2e_gen_112, 3, 37};
-  t[1][1][3] = Gen3Launch{&int3c2e_gen_113, 4, 37};
+  t[1][1][0] = Gen3Launch{&int3c2e_gen_110, &int3c2e_gen32_110, 1, 1, 37};
+  t[1][1][1] = Gen3Launch{&int3c2e_gen_111, &int3c2e_gen32_111, 2, 1, 37};
+  t[1][1][2] = Gen3Launch{&int3c2e_gen_112, &int3c2e_gen32_112, 3, 2, 37};
+  t[1][1][3] = Gen3Launch{&int3c2e_gen_113, &int3c2e_gen32_113, 4, 2, 37};
 }
 
 }  // namespace dfhip

@@ -21,6 +21,7 @@ struct PairArgs {
   const int* nsurv;    // screened-in prefix length of the aux list, per pair
   int npairs;
   const double* qvl = nullptr;  // QVL pair records (8 doubles per pair), or null for Schwarz
+  const int* nbeg = nullptr;    // first aux-list entry per pair (generated kernels; null = 0)
 };
 
 struct AuxArgs {

@@ -39,6 +39,7 @@ class Options(ctypes.Structure):
         ("max_tile_bytes", ctypes.c_int64),
         ("dens_on_device", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("fp32_below", ctypes.c_double),
     ]
 
 
@@ -57,6 +58,7 @@ class Stats(ctypes.Structure):
         "ms_h2d_densities", "ms_host_plan")] + [
         ("class_triplets", ctypes.c_int64 * 90),
         ("class_prim_triplets", ctypes.c_int64 * 90),
+        ("ncomputed_fp32", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -7,6 +7,7 @@ Tolerances are written per test.
 """
 
 import ctypes
+import functools
 
 import numpy as np
 import pytest
@@ -250,18 +251,62 @@ def test_2d_entry_matches_4d(bena2):
     assert J[0, 1, 0, 1] == pytest.approx(B[:, 1] @ B[:, 1], rel=1e-12)
 
 
+@functools.lru_cache(maxsize=None)
+def _oracle_j(wid, nstates):
+    shells, aux, dens = synthetic.workload(wid)
+    return O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True)
+
+
 def test_workload1_vs_oracle():
     shells, aux, dens = synthetic.workload(1)
     _say(f"workload 1: nbf {shells.nbf} naux {aux.nbf} ndens {dens.shape[0]}")
     b = N.BasisArrays(*shells.as_arrays())
     x = N.BasisArrays(*aux.as_arrays())
     _, J, st = _intor.contract(b, x, dens, nstates=8, want_df=False, want_j=True)
-    Jref, info = O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True)
+    Jref, info = _oracle_j(1, 8)
     assert st["ncomputed"] == info["ncomputed"]
     errJ = np.abs(J - Jref).max()
     _say(f"workload 1: max |dJ| {errJ:.2e} |J|max {np.abs(Jref).max():.3e} "
          f"triplets {st['ncomputed']} / {st['ntriplets']}")
     assert errJ <= 1e-10
+
+
+@pytest.mark.parametrize("wid,nstates", [(0, 3), (1, 8)])
+def test_mixed_precision_j(wid, nstates):
+    """Mixed precision (FP32 integral arithmetic below the Schwarz product
+    1e-3): Coulomb-tensor elements within 1e-8 Eh of the FP64 oracle."""
+    shells, aux, dens = synthetic.workload(wid)
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    opt = N.default_options(precision=N.MIXED)
+    _, J, st = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True, options=opt)
+    Jref, info = _oracle_j(wid, nstates)
+    errJ = np.abs(J - Jref).max()
+    _say(f"workload {wid} mixed: max |dJ| {errJ:.2e} max rel {(np.abs(J - Jref) / np.abs(Jref).max()).max():.2e} "
+         f"FP32 triplets {st['ncomputed_fp32']} / {st['ncomputed']}")
+    assert st["ncomputed"] == info["ncomputed"]
+    assert st["ncomputed_fp32"] > 0
+    assert errJ <= 1e-8
+
+
+def test_mixed_precision_blocks():
+    """(ab|P) blocks of the mixed path: FP64 blocks exact, FP32 blocks (all
+    with Schwarz product < 1e-3) within 2e-9 absolute."""
+    shells, aux, _ = synthetic.workload(0)
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    got = np.zeros((shells.nbf, shells.nbf, aux.nbf))
+    opt = N.default_options(precision=N.MIXED)
+    st = N.Stats()
+    N.check(N.lib().dfhip_int3c2e_dense(ctypes.byref(b.struct), ctypes.byref(x.struct), ctypes.byref(opt),
+                                        got.ctypes.data, ctypes.byref(st)))
+    ref = oracle_dense3c(shells, aux)
+    err = np.abs(got - ref)
+    nz = np.abs(ref) > 0
+    _say(f"mixed blocks: max abs err {err.max():.2e}, max rel err {(err[nz] / np.abs(ref[nz])).max():.2e}, "
+         f"FP32 triplets {st.ncomputed_fp32} / {st.ncomputed}")
+    assert st.ncomputed_fp32 > 0
+    assert err.max() <= 2e-9
 
 
 def test_bitwise_reproducible():
