@@ -1218,11 +1218,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   int e7 = e6, e8 = e6;
   DBuf<double> dB, dJ;
   if (need_dens) {
-    const int nblk = (int)((naux + CHOL_BLOCK - 1) / CHOL_BLOCK);
-    DBuf<double> invblk((size_t)nblk * CHOL_BLOCK * CHOL_BLOCK);
     DBuf<int> info(1);
     HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
-    cholesky_bordered(Ap, lda, naux, ntot, invblk.p, info.p, st);
+    cholesky_bordered(Ap, lda, naux, ntot, info.p, st);
     HC(hipGetLastError());
     e7 = tm.mark();
     HC(hipStreamSynchronize(st));

@@ -28,7 +28,7 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
                 int64_t sP, int64_t sI, hipStream_t st);
 int contract_tile();
-void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, double* invblk, int* info, hipStream_t st);
+void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, int* info, hipStream_t st);
 void border_to_b(const double* A, int64_t lda, int64_t n, int ndens, double* B, hipStream_t st);
 void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st);
 void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,

@@ -172,98 +172,129 @@ int contract_tile() { return GT; }
 // Coulomb tensor of diabatization/intor.f90:241-269 with a minus sign.
 constexpr int CB = 64;
 
-// Factor the diagonal block at k0 (left-looking, one wave, row r per lane)
-// and store L11 and inv(L11).  Fully unrolled: each lane keeps its row of L
-// (and later its column of inv(L11)) in registers; the row of step j is read
-// from LDS as broadcast loads that are all independent.  Rows r >= nb of the
-// last, partial block are padded with the identity.
-__global__ __launch_bounds__(64) void chol_diag_kernel(double* __restrict__ A, int64_t lda, int k0, int nb,
-                                                       double* __restrict__ inv, int* __restrict__ info) {
-  __shared__ double S[CB][CB + 1];  // rows of L as they are completed
-  __shared__ double dinv[CB];       // 1 / L[j][j]
-  const int r = threadIdx.x;
-  const int rr = min(r, nb - 1);
-  double a[CB];
+// One block column of the Cholesky factorization: the 64x64 diagonal block
+// A11 at k0 and 64 rows of the panel A21 below it, as the 128x64 stack
+// [A11; A21] factored right-looking in 4-column steps:
+//   L11 L11^T = A11,   L21 = A21 L11^-T.
+// Every workgroup factors A11 itself (the same arithmetic, so the same
+// result) instead of waiting for a separate diagonal kernel.  L11 is not
+// stored: nothing after this step reads it (the trailing updates use L21, the
+// results are the border rows and the trailing block), and leaving A11 as it
+// is keeps it intact for the workgroups still loading it.  512 threads: thread (ti, tk) keeps the 4x4 tile of rows 4ti..
+// (ti < 16: A11, lower tiles only; ti >= 16: the panel rows) and columns 4tk..
+// in registers.  Per step jb: the owner of diagonal tile jb factors it and
+// publishes it; the tiles of column block jb solve against it and publish
+// their rows; the tiles right of jb take the rank-4 update.  Rows and
+// columns past nb of a partial last block are padded with the identity.
+__global__ __launch_bounds__(512) void chol_panel_kernel(double* __restrict__ A, int64_t lda, int64_t ntot, int k0,
+                                                         int nb, int* __restrict__ info) {
+  __shared__ double Lb[16];                     // factored diagonal tile: L (lower) and 1/L[c][c] on the diagonal
+  __shared__ __align__(16) double P[128][4];    // solved rows of the current column block
+  const int tid = threadIdx.x, ti = tid >> 4, tk = tid & 15;
+  const int64_t r0 = k0 + nb + (int64_t)blockIdx.x * CB;  // first panel row of this workgroup
+  const int nr = (int)max((int64_t)0, min((int64_t)CB, ntot - r0));
+  const bool diag = ti < 16;
+  const bool active = diag ? ti >= tk : true;
+  // global row of tile row x (clamped for the loads)
+  double a[4][4];
 #pragma unroll
-  for (int c = 0; c < CB; c++) a[c] = A[(k0 + rr) + lda * (k0 + min(c, nb - 1))];
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-  for (int c = 0; c < CB; c++) a[c] = (r < nb && c < nb) ? a[c] : (r == c ? 1.0 : 0.0);
-  // left-looking: L[r][j] = (a[r][j] - sum_{k<j} L[r][k] L[j][k]) / L[j][j]
-#pragma unroll
-  for (int j = 0; j < CB; j++) {
-    double t0 = 0.0, t1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < j; k += 2) {
-      t0 = fma(a[k], S[j][k], t0);
-      if (k + 1 < j) t1 = fma(a[k + 1], S[j][k + 1], t1);
-    }
-    const double t = a[j] - (t0 + t1);
-    if (r == j) {
-      double dj = t;
-      if (!(dj > 0.0)) {
-        if (j < nb) atomicCAS(info, 0, k0 + j + 1);
-        dj = 1.0;
+    for (int y = 0; y < 4; y++) {
+      const int c = 4 * tk + y;
+      int64_t row;
+      bool ok;
+      if (diag) {
+        const int r = 4 * ti + x;
+        row = k0 + min(r, nb - 1);
+        ok = r < nb && c < nb;
+      } else {
+        const int r = 4 * (ti - 16) + x;
+        row = nr > 0 ? r0 + min(r, nr - 1) : k0;
+        ok = r < nr && c < nb;
       }
-      const double d = sqrt(dj);
-      dinv[j] = 1.0 / d;
-      a[j] = d;
+      a[x][y] = A[row + lda * (k0 + min(c, nb - 1))];
+      if (!ok) a[x][y] = (diag && 4 * ti + x == c) ? 1.0 : 0.0;
     }
-    __syncthreads();
-    if (r > j) a[j] = t * dinv[j];
-    if (r >= j) S[r][j] = a[j];
-    __syncthreads();
-  }
-  // column r of inv(L11): iv[i] = -(sum_{k<i} L[i][k] iv[k]) / L[i][i], iv[k<r] = 0
-  double iv[CB];
-#pragma unroll
-  for (int i = 0; i < CB; i++) {
-    double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < i; k += 2) {
-      s0 = fma(S[i][k], iv[k], s0);
-      if (k + 1 < i) s1 = fma(S[i][k + 1], iv[k + 1], s1);
-    }
-    iv[i] = (i < r) ? 0.0 : (i == r ? dinv[i] : -(s0 + s1) * dinv[i]);
-  }
-  if (r < nb) {
-#pragma unroll
-    for (int c = 0; c < CB; c++)
-      if (c < nb && r >= c) A[(k0 + r) + lda * (k0 + c)] = a[c];
-  }
-  // inv is stored column-major: inv[i + CB * c]; lane r owns column r
-#pragma unroll
-  for (int i = 0; i < CB; i++) inv[i + CB * r] = (i < nb && r < nb) ? iv[i] : 0.0;
-}
 
-// L21 = A21 inv(L11)^T for 64 rows per workgroup (rows k0+nb .. ntot-1).
-__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, int64_t lda, int64_t ntot, int k0,
-                                                         int nb, const double* __restrict__ inv) {
-  __shared__ double X[CB][CB + 1];
-  __shared__ double Iv[CB][CB + 1];
-  const int64_t r0 = k0 + nb + (int64_t)blockIdx.x * CB;
-  const int nr = (int)min((int64_t)CB, ntot - r0);
-  const int tid = threadIdx.x;
-  {
-    double v[CB * CB / 256], w[CB * CB / 256];
+  for (int jb = 0; jb < 16; jb++) {
+    if (diag && ti == jb && tk == jb) {
+      double L[4][4], dinv[4];
 #pragma unroll
-    for (int z = 0; z < CB * CB / 256; z++) {
-      const int e = tid + 256 * z, r = e % CB, c = e / CB;
-      w[z] = inv[r + CB * c];
-      v[z] = A[(r0 + min(r, nr - 1)) + lda * (k0 + min(c, nb - 1))];
+      for (int c = 0; c < 4; c++) {
+        double d = a[c][c];
+#pragma unroll
+        for (int z = 0; z < c; z++) d -= L[c][z] * L[c][z];
+        if (!(d > 0.0)) {
+          if (blockIdx.x == 0 && 4 * jb + c < nb) atomicCAS(info, 0, k0 + 4 * jb + c + 1);
+          d = 1.0;
+        }
+        L[c][c] = sqrt(d);
+        dinv[c] = 1.0 / L[c][c];
+#pragma unroll
+        for (int x = c + 1; x < 4; x++) {
+          double v = a[x][c];
+#pragma unroll
+          for (int z = 0; z < c; z++) v -= L[x][z] * L[c][z];
+          L[x][c] = v * dinv[c];
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+          a[x][y] = (y <= x) ? L[x][y] : 0.0;
+          Lb[4 * x + y] = (y < x) ? L[x][y] : (y == x ? dinv[x] : 0.0);
+        }
     }
+    __syncthreads();
+    if (tk == jb && ti > jb) {
+      double L[16];
 #pragma unroll
-    for (int z = 0; z < CB * CB / 256; z++) {
-      const int e = tid + 256 * z, r = e % CB, c = e / CB;
-      Iv[r][c] = w[z];
-      X[r][c] = (r < nr && c < nb) ? v[z] : 0.0;
+      for (int z = 0; z < 16; z++) L[z] = Lb[z];
+      // X L^T = a:  X[x][y] = (a[x][y] - sum_{z<y} X[x][z] L[y][z]) / L[y][y]
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+          double v = a[x][y];
+#pragma unroll
+          for (int z = 0; z < y; z++) v -= a[x][z] * L[4 * y + z];
+          a[x][y] = v * L[5 * y];
+        }
+        *reinterpret_cast<double4*>(&P[4 * ti + x][0]) = make_double4(a[x][0], a[x][1], a[x][2], a[x][3]);
+      }
+    }
+    __syncthreads();
+    if (active && tk > jb) {
+      double pr[4][4], pc[4][4];
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const double4 u = *reinterpret_cast<const double4*>(&P[4 * ti + x][0]);
+        const double4 v = *reinterpret_cast<const double4*>(&P[4 * tk + x][0]);
+        pr[x][0] = u.x; pr[x][1] = u.y; pr[x][2] = u.z; pr[x][3] = u.w;
+        pc[x][0] = v.x; pc[x][1] = v.y; pc[x][2] = v.z; pc[x][3] = v.w;
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+          double v = a[x][y];
+#pragma unroll
+          for (int z = 0; z < 4; z++) v = fma(-pr[x][z], pc[y][z], v);
+          a[x][y] = v;
+        }
     }
   }
-  __syncthreads();
-  for (int e = tid; e < nr * nb; e += blockDim.x) {
-    const int r = e % nr, c = e / nr;
-    double s = 0.0;
-    for (int l = 0; l <= c; l++) s = fma(X[r][l], Iv[c][l], s);
-    A[(r0 + r) + lda * (k0 + c)] = s;
+  // store the panel rows L21
+  if (!diag) {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const int c = 4 * tk + y, r = 4 * (ti - 16) + x;
+        if (c < nb && r < nr) A[(r0 + r) + lda * (k0 + c)] = a[x][y];
+      }
   }
 }
 
@@ -330,15 +361,13 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
     }
 }
 
-void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, double* invblk, int* info, hipStream_t st) {
+void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, int* info, hipStream_t st) {
   for (int k0 = 0; k0 < n; k0 += CB) {
     const int nb = (int)std::min<int64_t>(CB, n - k0);
-    double* inv = invblk + (size_t)(k0 / CB) * CB * CB;
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(64), 0, st, A, lda, k0, nb, inv, info);
     const int64_t m = ntot - k0 - nb;
+    const unsigned npanel = (unsigned)std::max<int64_t>(1, (m + CB - 1) / CB);
+    hipLaunchKernelGGL(chol_panel_kernel, dim3(npanel), dim3(512), 0, st, A, lda, ntot, k0, nb, info);
     if (m > 0) {
-      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((m + CB - 1) / CB)), dim3(256), 0, st, A, lda, ntot, k0,
-                         nb, inv);
       const int64_t t = (m + CB - 1) / CB;
       hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(t * (t + 1) / 2)), dim3(256), 0, st, A, lda, ntot, k0,
                          nb);
