@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,6 +27,7 @@
 #include "linalg.h"
 #include "schwarz4c_impl.h"
 #include "schwarz4m_impl.h"
+#include "schwarz4s_impl.h"
 
 namespace dfhip {
 
@@ -38,6 +40,7 @@ void register_int3c2e_l5(Int3Table&);
 void register_schwarz4c(Schwarz4Table&);
 void register_schwarz4c_cached(Schwarz4CachedTable&);
 void register_schwarz4m(Schwarz4MTable&);
+void register_schwarz4s(Schwarz4STable&);
 
 namespace {
 
@@ -235,6 +238,7 @@ struct Ctx {
   Schwarz4Table s4{};
   Schwarz4CachedTable s4c{};
   Schwarz4MTable s4m{};
+  Schwarz4STable s4s{};
   Gen3Table gen3{};
   // device workspace reused across calls (grown on demand, freed by
   // dfhip_release_workspace)
@@ -283,6 +287,7 @@ void ensure_ctx() {
   register_schwarz4c(c.s4);
   register_schwarz4c_cached(c.s4c);
   register_schwarz4m(c.s4m);
+  register_schwarz4s(c.s4s);
   register_gen3_all(c.gen3);
   for (int la = 0; la <= LMAX_AUX; la++)
     for (int lb = 0; lb <= (la <= LMAX_ORB ? la : 0); lb++)
@@ -386,6 +391,7 @@ struct Timer {
   }
   double ms(int a, int b) {
     float t = 0.f;
+    HC(hipEventSynchronize(ev[b]));
     HC(hipEventElapsedTime(&t, ev[a], ev[b]));
     return t;
   }
@@ -445,32 +451,43 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
   upload(s1, ps.s1, st);
   upload(s2, ps.s2, st);
   std::vector<size_t> poff(NPCLS + 1, 0);
-  // kernel per class: 2 = matrix cores, 1 = cached VALU, 0 = chunked VALU
-  // (DFHIP_SCHWARZ_KERNEL caps the choice, for tests)
+  // kernel per class: 3 = matrix cores with the tables in LDS, 2 = matrix
+  // cores streaming the tables from global scratch, 1 = cached VALU,
+  // 0 = chunked VALU (DFHIP_SCHWARZ_KERNEL caps the choice, for tests)
   std::vector<int> nch(NPCLS, 1), kind(NPCLS, 0);
   const char* cap_env = std::getenv("DFHIP_SCHWARZ_KERNEL");
-  const int cap = (cap_env && *cap_env) ? std::atoi(cap_env) : 2;
+  const int cap = (cap_env && *cap_env) ? std::atoi(cap_env) : 3;
+  size_t scr_need = 0;
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb);
       int maxnpp = 0;
       for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++)
         maxnpp = std::max(maxnpp, o.nprim[ps.s1[p]] * o.nprim[ps.s2[p]]);
-      if (cap >= 2 && c.s4m[la][lb].fn != nullptr && maxnpp <= c.s4m[la][lb].nppm)
+      const int np = ps.cls_end[k] - ps.cls_begin[k];
+      if (cap >= 3 && c.s4m[la][lb].fn != nullptr && maxnpp <= c.s4m[la][lb].nppm) {
+        kind[k] = 3;
+      } else if (cap >= 2 && c.s4s[la][lb].fn != nullptr && maxnpp <= c.s4s[la][lb].npps) {
         kind[k] = 2;
-      else if (cap >= 1 && c.s4c[la][lb].fn != nullptr && maxnpp <= c.s4c[la][lb].nppc)
+        scr_need = std::max(scr_need, (size_t)np * (size_t)c.s4s[la][lb].perpair);
+      } else if (cap >= 1 && c.s4c[la][lb].fn != nullptr && maxnpp <= c.s4c[la][lb].nppc) {
         kind[k] = 1;
+      }
       nch[k] = kind[k] ? 1 : c.s4[la][lb].nch;
       poff[k + 1] = (size_t)(ps.cls_end[k] - ps.cls_begin[k]) * nch[k];
     }
   for (int k = 0; k < NPCLS; k++) poff[k + 1] += poff[k];
   DBuf<double> part(std::max<size_t>(poff[NPCLS], 1));
+  // scratch of the streamed kernels: all pairs of the largest class, at most 1 GiB
+  DBuf<double> scr(std::min<size_t>(scr_need, (size_t)1 << 27));
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
       if (np == 0) continue;
-      if (kind[k] == 2)
+      if (kind[k] == 3)
         c.s4m[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+      else if (kind[k] == 2)
+        c.s4s[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], scr.p, scr.n, st);
       else if (kind[k] == 1)
         c.s4c[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       else
@@ -805,6 +822,18 @@ void split_precision(Plan& pl, double tau, bool gen) {
   }
 }
 
+// Verbose progress line, flushed at once so that a caller redirecting stdout
+// sees each phase as it finishes.
+void vlog(const dfhip_options& opt, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void vlog(const dfhip_options& opt, const char* fmt, ...) {
+  if (!opt.verbose) return;
+  va_list ap;
+  va_start(ap, fmt);
+  std::vprintf(fmt, ap);
+  va_end(ap);
+  std::fflush(stdout);
+}
+
 // erfinv(y) for 0 < y < 1 by Newton's method on erf (quadratic convergence
 // from sqrt(-log(1 - y)), which lies above the root).
 double erfinv_pos(double y) {
@@ -924,6 +953,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.ps = make_pairs(o);
   pl.qab = schwarz_bra_dev(c, o, dorb.view, pl.ps);
   const int e2 = tm.mark();
+  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
 
   // metric (common/integrals/intor.f90:40-95) in the leading block of the
   // bordered matrix [[M, .], [d^T, 0]] (column-major, lda = naux + ndens),
@@ -940,17 +970,16 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   for (const double q : pl.qc)
     if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
   const int e3 = tm.mark();
+  vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
   const int e4 = e3;
 
   const auto hp0 = std::chrono::steady_clock::now();
   build_plan(pl, opt);
   const double ms_plan =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
-  if (opt.verbose) {
-    std::printf("Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
-    std::printf("Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
-    std::printf(" Densities: %6d\n", ndens);
-  }
+  vlog(opt, "Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
+  vlog(opt, "Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
+  vlog(opt, " Densities: %6d\n", ndens);
 
   // device plan arrays
   DBuf<int> ps1, ps2;
@@ -1210,6 +1239,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipStreamSynchronize(st));
     ms_int += tm.ms(b0, b1);
     ms_con += tm.ms(b1, b2);
+    vlog(opt, "Column block %d of %d: integrals %.1f ms, contraction %.1f ms.\n", blk + 1, nblocks, tm.ms(b0, b1),
+         tm.ms(b1, b2));
   }
   const int e6 = tm.mark();
 
@@ -1278,6 +1309,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     }
   }
   if (opt.verbose) {
+    std::fflush(stdout);
     std::printf("Schwarz norms took %8.4f s.\n", tm.ms(e1, e2) * 1e-3);
     std::printf("Calculation of density fitting metric (P|Q) took %8.4f s.\n", tm.ms(e2, e3) * 1e-3);
     std::printf("Cholesky factorization of (P|Q) bordered by d took %8.4f s.\n", tm.ms(e6, e7) * 1e-3);
@@ -1285,6 +1317,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     const int64_t skipped = pl.ntriplets - pl.ncomputed;
     std::printf("Screened out %12lld of %12lld integral triplets (%8.2f%%)\n", (long long)skipped,
                 (long long)pl.ntriplets, 100.0 * (double)skipped / (double)std::max<int64_t>(pl.ntriplets, 1));
+    std::fflush(stdout);
   }
 }
 

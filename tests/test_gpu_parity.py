@@ -164,10 +164,11 @@ def _schwarz_systems():
             ("mixed_npp", Shells(mixed_npp)), ("wide", Shells(wide))]
 
 
-@pytest.mark.parametrize("kernel", [2, 1, 0])
+@pytest.mark.parametrize("kernel", [3, 2, 1, 0])
 def test_schwarz_kernel_variants(kernel, monkeypatch):
-    """Matrix-core (2), cached (1) and chunked (0) Schwarz kernels against the
-    oracle's explicit (ab|ab) blocks."""
+    """Matrix-core kernels with LDS-resident (3) and streamed (2) tables,
+    cached (1) and chunked (0) VALU kernels against the oracle's explicit
+    (ab|ab) blocks."""
     monkeypatch.setenv("DFHIP_SCHWARZ_KERNEL", str(kernel))
     for name, shells in _schwarz_systems():
         qb = sintor.int_schwarz_bra(shells)
