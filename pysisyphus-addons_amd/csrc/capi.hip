@@ -24,6 +24,7 @@
 #include "../../include/dfhip.h"
 #include "gen3c_classes.h"
 #include "int3c2e_impl.h"
+#include "int3c2e_mm_impl.h"
 #include "linalg.h"
 #include "schwarz4c_impl.h"
 #include "schwarz4m_impl.h"
@@ -37,6 +38,11 @@ void register_int3c2e_l2(Int3Table&);
 void register_int3c2e_l3(Int3Table&);
 void register_int3c2e_l4(Int3Table&);
 void register_int3c2e_l5(Int3Table&);
+void register_int3c2e_mm_l0(Int3MTable&, EtTable&);
+void register_int3c2e_mm_l1(Int3MTable&, EtTable&);
+void register_int3c2e_mm_l2(Int3MTable&, EtTable&);
+void register_int3c2e_mm_l3(Int3MTable&, EtTable&);
+void register_int3c2e_mm_l4(Int3MTable&, EtTable&);
 void register_schwarz4c(Schwarz4Table&);
 void register_schwarz4c_cached(Schwarz4CachedTable&);
 void register_schwarz4m(Schwarz4MTable&);
@@ -235,6 +241,8 @@ struct Ctx {
   DBuf<double> boys;
   DBuf<SphTable> sph;
   Int3Table int3{};
+  Int3MTable int3m{};
+  EtTable et{};
   Schwarz4Table s4{};
   Schwarz4CachedTable s4c{};
   Schwarz4MTable s4m{};
@@ -278,6 +286,11 @@ void ensure_ctx() {
   HC(hipMemcpyAsync(c.sph.p, h, sizeof(SphTable), hipMemcpyHostToDevice, c.st));
   HC(hipStreamSynchronize(c.st));
   delete h;
+  register_int3c2e_mm_l0(c.int3m, c.et);
+  register_int3c2e_mm_l1(c.int3m, c.et);
+  register_int3c2e_mm_l2(c.int3m, c.et);
+  register_int3c2e_mm_l3(c.int3m, c.et);
+  register_int3c2e_mm_l4(c.int3m, c.et);
   register_int3c2e_l0(c.int3);
   register_int3c2e_l1(c.int3);
   register_int3c2e_l2(c.int3);
@@ -506,6 +519,11 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
       if (!std::isfinite(q[p])) throw Error{DFHIP_ERR_HIP, "non-finite Schwarz norm"};
     }
   return q;
+}
+
+bool use_matrix_cores_3c() {
+  const char* v = std::getenv("DFHIP_NO_MM");
+  return !(v && *v && *v != '0');
 }
 
 bool use_generated() {
@@ -1114,6 +1132,53 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // rows from the class start to the last pair whose screened prefix reaches
   // the tile (under Schwarz screening these pairs are exactly a prefix of
   // the class: pairs are sorted by Q_ab and the prefixes are non-increasing).
+  // Et tables of the classes run on the matrix-core 3c2e kernels: the pairs
+  // with screened-in triplets in any such aux L, built once for all column
+  // blocks (a class whose tables would take more than a third of free HBM
+  // keeps the LDS kernels)
+  std::vector<DBuf<double>> etg(NPCLS);
+  std::vector<DBuf<int64_t>> etoff(NPCLS);
+  std::vector<std::vector<char>> use_mm(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
+  if (use_matrix_cores_3c()) {
+    size_t fr = 0, tot = 0;
+    HC(hipMemGetInfo(&fr, &tot));
+    const size_t et_cap = fr / 3;
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
+        if (np == 0 || c.et[la][lb].fn == nullptr) continue;
+        bool any = false;
+        for (int L = 0; L <= LMAX_AUX; L++) {
+          use_mm[k][L] = !(gen && gen3_has(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
+          any = any || use_mm[k][L];
+        }
+        if (!any) continue;
+        std::vector<int> list;
+        std::vector<int64_t> off(np, 0);
+        int64_t totd = 0;
+        for (int p = 0; p < np; p++) {
+          bool need = false;
+          for (int L = 0; L <= LMAX_AUX && !need; L++) need = use_mm[k][L] && pl.nsurv[L][p0 + p] > 0;
+          if (!need) continue;
+          list.push_back(p);
+          off[p] = totd;
+          totd += (int64_t)o.nprim[pl.ps.s1[p0 + p]] * o.nprim[pl.ps.s2[p0 + p]] * c.et[la][lb].etpp;
+        }
+        if ((size_t)totd * sizeof(double) > et_cap) {
+          for (int L = 0; L <= LMAX_AUX; L++) use_mm[k][L] = 0;
+          continue;
+        }
+        if (list.empty()) continue;
+        DBuf<int> dlist;
+        upload(dlist, list, st);
+        upload(etoff[k], off, st);
+        etg[k].alloc((size_t)totd);
+        c.et[la][lb].fn(dorb.view, ps1.p + p0, ps2.p + p0, dlist.p, etoff[k].p, (int)list.size(), c.sph.p, etg[k].p,
+                        st);
+        HC(hipGetLastError());
+      }
+  }
+
   const int GT = contract_tile();
   // suffix maxima of the prefixes inside each class: equal to nsurv under
   // Schwarz screening (non-increasing), an upper envelope under QVL
@@ -1197,6 +1262,10 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
           } else if (gen && gen3_has(la, lb, L)) {
             launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
+          } else if (use_mm[k][L] && etg[k].p) {
+            c.int3m[la][lb][L].fn(dorb.view, daux.view, pa, xa, etg[k].p, etoff[k].p, c.sph.p, c.boys.p, Tp - cA, w,
+                                  st);
+            HC(hipGetLastError());
           } else {
             c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, Tp - cA, w, st);
             HC(hipGetLastError());

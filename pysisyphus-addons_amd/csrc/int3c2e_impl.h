@@ -40,6 +40,11 @@ __device__ inline bool triplet_kept(const PairArgs& pa, const AuxArgs& xa, int p
   return pa.qvl == nullptr || qvl_keep(pa.qvl + 8 * (size_t)pidx, xa.qvl + 4 * (size_t)sh, Cx, Cy, Cz, LC, xa.thresh);
 }
 
+// Number of terms of all real solid harmonics of angular momentum L.
+__host__ __device__ constexpr int nsph_terms(int L) {
+  return L == 0 ? 1 : L == 1 ? 3 : L == 2 ? 8 : L == 3 ? 16 : L == 4 ? 28 : L == 5 ? 46 : 70;
+}
+
 template <int LA, int LB, int LC>
 struct Cls3 {
   static constexpr int LAB = LA + LB, L = LAB + LC;
@@ -49,7 +54,7 @@ struct Cls3 {
   static constexpr int NT = 128;
   static constexpr int FIXED = 3 * ESZ + NAB * H;
   static constexpr int PERC = 4 + 4 * (L + 1) + NR + H * NCC;
-  static constexpr int SMALL = 8192, LARGE = 20000;  // doubles (64 KB / 156 KB)
+  static constexpr int SMALL = 8192, LARGE = 19300;  // doubles (64 KB / 151 KB; ~5 KB more for tables)
   static constexpr int NC_LDS = (SMALL - FIXED) / PERC >= 2 ? (SMALL - FIXED) / PERC : (LARGE - FIXED) / PERC;
   static constexpr int NC_OUT = (16 * NT) / (NAB * NCC) > 1 ? (16 * NT) / (NAB * NCC) : 1;
   static constexpr int NC0 = NC_LDS < NC_OUT ? NC_LDS : NC_OUT;
@@ -70,6 +75,12 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
   __shared__ double lds[K::LDS];
   __shared__ int cshell[K::NC];
   __shared__ int ckeep[K::NC];
+  // tables: packed (t,u,v) per Hermite index, t!/(i!(t-2i)!), solid-harmonic terms of LA, LB, LC
+  constexpr int LH = L / 2 + 1;
+  __shared__ int tuv[NR];
+  __shared__ double hc[(L + 1) * LH];
+  __shared__ int sa[2 * LA + 2], sb[2 * LB + 2], sc[2 * LC + 2];
+  __shared__ SphTerm ta[nsph_terms(LA)], tb[nsph_terms(LB)], tc[nsph_terms(LC)];
 
   const int pidx = blockIdx.x;
   const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
@@ -87,6 +98,29 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
   double* R = pw + 3 * K::NC * (L + 1);  // NC*NR
   double* W = R + K::NC * NR;            // NC*H*NCC
 
+  for (int h = tid; h < NR; h += NT) {
+    int t, u, v;
+    herm_tuv(h, t, u, v);
+    tuv[h] = t | (u << 8) | (v << 16);
+  }
+  for (int e = tid; e < (L + 1) * LH; e += NT) {
+    const int t = e / LH, i = e - t * LH;
+    double c = 0.0;
+    if (2 * i <= t) {
+      c = 1.0;
+      for (int z = 0; z < i; z++) c *= (double)((t - 2 * z) * (t - 2 * z - 1)) / (z + 1);
+    }
+    hc[e] = c;
+  }
+  {
+    const int a0 = sph->start[LA][0], b0 = sph->start[LB][0], x0 = sph->start[LC][0];
+    if (tid < 2 * LA + 2) sa[tid] = sph->start[LA][tid] - a0;
+    if (tid < 2 * LB + 2) sb[tid] = sph->start[LB][tid] - b0;
+    if (tid < 2 * LC + 2) sc[tid] = sph->start[LC][tid] - x0;
+    for (int e = tid; e < nsph_terms(LA); e += NT) ta[e] = sph->terms[a0 + e];
+    for (int e = tid; e < nsph_terms(LB); e += NT) tb[e] = sph->terms[b0 + e];
+    for (int e = tid; e < nsph_terms(LC); e += NT) tc[e] = sph->terms[x0 + e];
+  }
   for (int c = tid; c < nc; c += NT) {
     const int sh = xa.shell[c0 + c];
     cshell[c] = sh;
@@ -132,9 +166,9 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
     for (int k = tid; k < NAB * H; k += NT) {
       const int munu = k / H, h = k - (k / H) * H;
       const int mu = munu / K::NB, nu = munu - (munu / K::NB) * K::NB;
-      int t, u, v;
-      herm_tuv(h, t, u, v);
-      Et[k] = cab * bra_element<LA, LB>(sph, mu, nu, t, u, v, Ex, Ex + K::ESZ, Ex + 2 * K::ESZ);
+      const int tv = tuv[h];
+      Et[k] = cab * bra_element_lds<LA, LB>(sa, ta, sb, tb, mu, nu, tv & 255, (tv >> 8) & 255, tv >> 16, Ex,
+                                            Ex + K::ESZ, Ex + 2 * K::ESZ);
     }
 
     for (int kp = 0; kp < xa.kmax; kp++) {
@@ -169,20 +203,19 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
       __syncthreads();
       for (int k = tid; k < nc * (NR - K::NRLO); k += NT) {
         const int c = k / (NR - K::NRLO), h = K::NRLO + (k - c * (NR - K::NRLO));
-        int t, u, v;
-        herm_tuv(h, t, u, v);
-        R[c * NR + h] = herm_R(t, u, v, L, pw + c * 3 * (L + 1), gN + c * (L + 1));
+        const int tv = tuv[h];
+        R[c * NR + h] = herm_R_tab(tv & 255, (tv >> 8) & 255, tv >> 16, pw + c * 3 * (L + 1), L, gN + c * (L + 1), hc, LH);
       }
       __syncthreads();
       for (int k = tid; k < nW; k += NT) {
         const int c = k / (H * NCC), rr = k - c * (H * NCC), h = rr / NCC, m = rr - (rr / NCC) * NCC;
-        int t, u, v;
-        herm_tuv(h, t, u, v);
+        const int tv = tuv[h];
+        const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
         double s = 0.0;
         const double* Rc = R + c * NR;
-        for (int q = sph->start[LC][m]; q < sph->start[LC][m + 1]; q++) {
-          const SphTerm tm = sph->terms[q];
-          s += tm.c * Rc[herm_index(t + tm.lx, u + tm.ly, v + tm.lz)];
+        for (int q = sc[m]; q < sc[m + 1]; q++) {
+          const SphTerm tm = tc[q];
+          s = fma(tm.c, Rc[herm_index(t + tm.lx, u + tm.ly, v + tm.lz)], s);
         }
         W[k] += s;
       }

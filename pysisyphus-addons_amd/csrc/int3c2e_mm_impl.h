@@ -1,0 +1,323 @@
+// 3c2e classes on the FP64 matrix cores (the classes without a generated
+// register kernel: an f or g orbital shell, or a g or h aux shell).
+//
+// The bra contraction dominates these classes: per primitive pair ij and aux
+// shell C, out[mu nu][m] += sum_h Et_ij[mu nu][h] W_ijC[h][m] with
+// Et (nsph(a) nsph(b) x nherm(la+lb)) up to 81 x 165.  Rebuilding Et per
+// chunk of aux shells (what the LDS kernel of int3c2e_impl.h does, with only
+// a few aux shells per chunk at these sizes) costs as much as the product, so
+//   * int3c2e_et_kernel writes, once per call, Et of every primitive pair of
+//     the pairs that have screened-in triplets to a global table, h-major and
+//     padded to MFMA fragments: EtG[ij][h < HK][mu nu < NABP];
+//   * int3c2e_mm_kernel takes one shell pair x NC aux shells of one L per
+//     workgroup (8 waves): per primitive pair and aux primitive it builds the
+//     Boys values, R_tuv and the Hobson-transformed W[h][(C, m)] for all NC
+//     shells in LDS, then the waves run the (NAB x H) . (H x NC nsph(lc))
+//     product on v_mfma_f64_16x16x4 with A fragments read straight from the
+//     global Et table and B fragments from LDS; the accumulators stay in
+//     registers over all primitive pairs.
+#pragma once
+
+#include "int3c2e_impl.h"
+
+namespace dfhip {
+
+typedef double dbl4m __attribute__((ext_vector_type(4)));
+
+template <int LA, int LB, int LC>
+struct Cls3m {
+  static constexpr int LAB = LA + LB, L = LAB + LC;
+  static constexpr int NA = nsph(LA), NB = nsph(LB), NAB = NA * NB, NCC = nsph(LC);
+  static constexpr int H = nherm(LAB), NR = nherm(L), NRLO = (LC == 0) ? 0 : nherm(LC - 1);
+  static constexpr int HK = (H + 3) / 4 * 4, NABP = (NAB + 15) / 16 * 16, MT = NABP / 16;
+  static constexpr int ESZ = (LA + 1) * (LB + 1) * (LAB + 1);
+  static constexpr int GW = 4 * (L + 1);  // scaled Boys values and powers of 2(P - C) per aux shell
+  static constexpr int BUD = 18500;       // LDS doubles for R, G and W
+  static constexpr int nc_fit(int nc) { return nc * (NR + GW) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD; }
+  static constexpr int nc_max() {
+    int nc = 32;
+    while (nc > 1 && !nc_fit(nc)) nc--;
+    return nc;
+  }
+  static constexpr int NC = nc_max();
+  static constexpr int NCOL = NC * NCC, NTL = (NCOL + 15) / 16, NCOLP = NTL * 16;
+  static constexpr int NW = 8, NT = 64 * NW;
+  // wave w works on row tile w / GRP with the column tiles w % GRP, w % GRP + GRP,
+  // ...: it loads its A fragments once per primitive pair and reuses them
+  static constexpr int GRP = NW / MT >= 1 ? NW / MT : 1;
+  static constexpr int NPW = (NTL + GRP - 1) / GRP;
+  static constexpr int KS = HK / 4;
+  static constexpr int LH = L / 2 + 1;
+  static constexpr int64_t ETPP = (int64_t)HK * NABP;  // Et table doubles per primitive pair
+  static constexpr bool OK = nc_fit(NC);
+};
+
+// Et tables of the listed pairs: EtG[off[p] + ij*ETPP + h*NABP + mu*NB + nu],
+// contraction coefficients folded in, zero padding.
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void int3c2e_et_kernel(DevShells bra, const int* __restrict__ s1v,
+                                                         const int* __restrict__ s2v,
+                                                         const int* __restrict__ list,
+                                                         const int64_t* __restrict__ off,
+                                                         const SphTable* __restrict__ sph, double* __restrict__ EtG) {
+  using K = Cls3m<LA, LB, 0>;
+  constexpr int H = K::H, HK = K::HK, NAB = K::NAB, NABP = K::NABP, NB = K::NB, ESZ = K::ESZ, NT = 256;
+  constexpr int NPPM = 16;  // primitive pairs per pass
+  __shared__ double E[NPPM * 3 * ESZ];
+  __shared__ int tuv[H];
+  __shared__ int sa[2 * LA + 2], sb[2 * LB + 2];
+  __shared__ SphTerm ta[nsph_terms(LA)], tb[nsph_terms(LB)];
+  const int tid = threadIdx.x;
+  const int p = list[blockIdx.x];
+  const int s1 = s1v[p], s2 = s2v[p];
+  const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb;
+  const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
+  double* out = EtG + off[p];
+  for (int h = tid; h < H; h += NT) {
+    int t, u, v;
+    herm_tuv(h, t, u, v);
+    tuv[h] = t | (u << 8) | (v << 16);
+  }
+  {
+    const int a0 = sph->start[LA][0], b0 = sph->start[LB][0];
+    if (tid < 2 * LA + 2) sa[tid] = sph->start[LA][tid] - a0;
+    if (tid < 2 * LB + 2) sb[tid] = sph->start[LB][tid] - b0;
+    for (int e = tid; e < nsph_terms(LA); e += NT) ta[e] = sph->terms[a0 + e];
+    for (int e = tid; e < nsph_terms(LB); e += NT) tb[e] = sph->terms[b0 + e];
+  }
+  for (int q0 = 0; q0 < npp; q0 += NPPM) {
+    const int nq = min(NPPM, npp - q0);
+    __syncthreads();
+    for (int e = tid; e < nq * 3; e += NT) {
+      const int ij = q0 + e / 3, d = e % 3, i = ij / npb, j = ij - i * npb;
+      herm_E_dir<LA, LB>(bra.exps[pa0 + i], bra.exps[pb0 + j], bra.xyz[3 * s1 + d], bra.xyz[3 * s2 + d],
+                         E + e * ESZ);
+    }
+    __syncthreads();
+    for (int64_t e = tid; e < (int64_t)nq * K::ETPP; e += NT) {
+      const int q = (int)(e / K::ETPP), r = (int)(e - (int64_t)q * K::ETPP), h = r / NABP, n = r - h * NABP;
+      const int ij = q0 + q, i = ij / npb, j = ij - i * npb;
+      double val = 0.0;
+      if (h < H && n < NAB) {
+        const int tv = tuv[h];
+        const double* Eq = E + q * 3 * ESZ;
+        val = bra.coefs[pa0 + i] * bra.coefs[pb0 + j] *
+              bra_element_lds<LA, LB>(sa, ta, sb, tb, n / NB, n % NB, tv & 255, (tv >> 8) & 255, tv >> 16, Eq,
+                                      Eq + ESZ, Eq + 2 * ESZ);
+      }
+      out[(int64_t)ij * K::ETPP + r] = val;
+    }
+  }
+}
+
+template <int LA, int LB, int LC>
+__global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
+                                                         const double* __restrict__ EtG,
+                                                         const int64_t* __restrict__ etoff,
+                                                         const SphTable* __restrict__ sph,
+                                                         const double* __restrict__ boys, double* __restrict__ out,
+                                                         int64_t ldo) {
+  using K = Cls3m<LA, LB, LC>;
+  constexpr int L = K::L, H = K::H, HK = K::HK, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NABP = K::NABP;
+  constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, LH = K::LH;
+  constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL;
+  __shared__ double G[NC * GW];        // per aux shell: scaled Boys values, then (2 PC_d)^e
+  __shared__ double R[NC * NR];        // per aux shell: R_tuv
+  __shared__ double W[HK * NCOLP];     // W[h][c NCC + m]
+  __shared__ double cxyz[NC * 3];
+  __shared__ int cshell[NC], ckeep[NC];
+  __shared__ int tuv[NR];
+  __shared__ double hc[(L + 1) * LH];
+  __shared__ int sc[2 * LC + 2];
+  __shared__ SphTerm tc[nsph_terms(LC)];
+
+  const int pidx = blockIdx.x;
+  const int cend = min(xa.hi, pa.nsurv[pidx]);
+  const int c0 = xa.lo + (int)blockIdx.y * NC;
+  if (c0 >= cend) return;
+  const int nc = min(NC, cend - c0);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
+  const double* Et0 = EtG + etoff[pidx];
+
+  for (int h = tid; h < NR; h += NT) {
+    int t, u, v;
+    herm_tuv(h, t, u, v);
+    tuv[h] = t | (u << 8) | (v << 16);
+  }
+  for (int e = tid; e < (L + 1) * LH; e += NT) {
+    const int t = e / LH, i = e - t * LH;
+    double c = 0.0;
+    if (2 * i <= t) {
+      c = 1.0;
+      for (int z = 0; z < i; z++) c *= (double)((t - 2 * z) * (t - 2 * z - 1)) / (z + 1);
+    }
+    hc[e] = c;
+  }
+  {
+    const int x0 = sph->start[LC][0];
+    if (tid < 2 * LC + 2) sc[tid] = sph->start[LC][tid] - x0;
+    for (int e = tid; e < nsph_terms(LC); e += NT) tc[e] = sph->terms[x0 + e];
+  }
+  for (int c = tid; c < nc; c += NT) {
+    const int sh = xa.shell[c0 + c];
+    cshell[c] = sh;
+    cxyz[3 * c + 0] = ket.xyz[3 * sh + 0];
+    cxyz[3 * c + 1] = ket.xyz[3 * sh + 1];
+    cxyz[3 * c + 2] = ket.xyz[3 * sh + 2];
+    ckeep[c] = triplet_kept(pa, xa, pidx, sh, LC, ket.xyz[3 * sh + 0], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2]);
+  }
+  for (int e = tid; e < HK * NCOLP; e += NT) W[e] = 0.0;  // padding rows and columns stay zero
+
+  double A[3], B[3];
+  for (int d = 0; d < 3; d++) {
+    A[d] = bra.xyz[3 * s1 + d];
+    B[d] = bra.xyz[3 * s2 + d];
+  }
+  const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb;
+  const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
+  const double lc_sign = (LC & 1) ? -1.0 : 1.0;
+
+  const int mt = wave / GRP, gi = wave % GRP;  // mt >= MT: this wave only helps with the tables
+  dbl4m acc[NPW];
+#pragma unroll
+  for (int t = 0; t < NPW; t++) acc[t] = dbl4m{0.0, 0.0, 0.0, 0.0};
+
+  for (int ij = 0; ij < npp; ij++) {
+    const int i = ij / npb, j = ij - i * npb;
+    const double a = bra.exps[pa0 + i], b = bra.exps[pb0 + j], p = a + b;
+    double P[3];
+    for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
+    const double* Et = Et0 + (int64_t)ij * K::ETPP;
+    for (int kp = 0; kp < xa.kmax; kp++) {
+      __syncthreads();  // tables built (first pass); previous product done with W
+      for (int k = tid; k < nc * GW; k += NT) {
+        const int c = k / GW, q = k - c * GW;
+        const int sh = cshell[c];
+        const bool live = kp < ket.nprim[sh] && ckeep[c];
+        const double PC0 = P[0] - cxyz[3 * c + 0], PC1 = P[1] - cxyz[3 * c + 1], PC2 = P[2] - cxyz[3 * c + 2];
+        double val;
+        if (q <= L) {
+          val = 0.0;
+          if (live) {
+            const double g = ket.exps[ket.pptr[sh] + kp], dk = ket.coefs[ket.pptr[sh] + kp];
+            const double alpha = p * g / (p + g);
+            const double T = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
+            double scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
+            const double inv2g = 0.5 / g;
+            for (int e = 0; e < LC; e++) scale *= inv2g;
+            for (int e = 0; e < q; e++) scale *= -alpha;
+            val = scale * boys_f(q, T, boys);
+          }
+        } else {
+          const int qq = q - (L + 1), d = qq / (L + 1), e = qq - d * (L + 1);
+          const double x = 2.0 * (d == 0 ? PC0 : (d == 1 ? PC1 : PC2));
+          val = 1.0;
+          for (int z = 0; z < e; z++) val *= x;
+        }
+        G[k] = val;
+      }
+      __syncthreads();
+      for (int k = tid; k < nc * (NR - K::NRLO); k += NT) {
+        const int c = k / (NR - K::NRLO), h = K::NRLO + (k - c * (NR - K::NRLO));
+        const int tv = tuv[h];
+        const double* g = G + c * GW;
+        R[c * NR + h] = herm_R_tab(tv & 255, (tv >> 8) & 255, tv >> 16, g + (L + 1), L, g, hc, LH);
+      }
+      __syncthreads();
+      for (int k = tid; k < H * nc * NCC; k += NT) {
+        const int h = k / (nc * NCC), col = k - h * (nc * NCC), c = col / NCC, m = col - c * NCC;
+        const int tv = tuv[h];
+        const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
+        const double* Rc = R + c * NR;
+        double s = 0.0;
+        for (int q = sc[m]; q < sc[m + 1]; q++) {
+          const SphTerm tm = tc[q];
+          s = fma(tm.c, Rc[herm_index(t + tm.lx, u + tm.ly, v + tm.lz)], s);
+        }
+        W[h * NCOLP + col] = s;
+      }
+      __syncthreads();
+      // acc[nt] += Et[16 mt.., h] W[h, 16 nt..]
+      if (mt < MT) {
+        double av[KS];
+        const double* ea = Et + lk * NABP + 16 * mt + lr;
+#pragma unroll
+        for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
+#pragma unroll
+        for (int t = 0; t < NPW; t++) {
+          const int nt = gi + GRP * t;
+          if (nt >= NTL) break;
+          const double* wb = W + lk * NCOLP + 16 * nt + lr;
+          dbl4m c4 = acc[t];
+#pragma unroll
+          for (int s = 0; s < KS; s++) c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], wb[4 * s * NCOLP], c4, 0, 0, 0);
+          acc[t] = c4;
+        }
+      }
+    }
+  }
+  // register r of lane l: row mu nu = 16 mt + l/16 + 4 r, column 16 nt + l%16
+  const int64_t row0 = pa.row[pidx];
+  if (mt >= MT) return;
+#pragma unroll
+  for (int t = 0; t < NPW; t++) {
+    const int nt = gi + GRP * t;
+    if (nt >= NTL) break;
+    const int col = 16 * nt + lr, c = col / NCC, m = col - c * NCC;
+    if (c >= nc) continue;
+    const int64_t ocol = xa.col[c0 + c] + m;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int munu = 16 * mt + lk + 4 * r;
+      if (munu < NAB) out[(row0 + munu) * ldo + ocol] = acc[t][r];
+    }
+  }
+}
+
+struct Int3MLaunch {
+  void (*fn)(DevShells, DevShells, PairArgs, AuxArgs, const double*, const int64_t*, const SphTable*, const double*,
+             double*, int64_t, hipStream_t);
+  int nc;
+};
+
+template <int LA, int LB, int LC>
+void launch_int3c2e_mm(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa, const double* EtG, const int64_t* etoff,
+                       const SphTable* sph, const double* boys, double* out, int64_t ldo, hipStream_t st) {
+  using K = Cls3m<LA, LB, LC>;
+  const int nchunks = (xa.hi - xa.lo + K::NC - 1) / K::NC;
+  if (pa.npairs <= 0 || nchunks <= 0) return;
+  hipLaunchKernelGGL((int3c2e_mm_kernel<LA, LB, LC>), dim3(pa.npairs, nchunks), dim3(K::NT), 0, st, bra, ket, pa, xa,
+                     EtG, etoff, sph, boys, out, ldo);
+}
+
+template <int LA, int LB, int LC>
+Int3MLaunch make_int3m() {
+  if constexpr (Cls3m<LA, LB, LC>::OK)
+    return Int3MLaunch{&launch_int3c2e_mm<LA, LB, LC>, Cls3m<LA, LB, LC>::NC};
+  else
+    return Int3MLaunch{nullptr, 0};
+}
+
+struct EtLaunch {
+  void (*fn)(DevShells, const int*, const int*, const int*, const int64_t*, int, const SphTable*, double*,
+             hipStream_t);
+  int64_t etpp;  // doubles per primitive pair
+};
+
+template <int LA, int LB>
+void launch_et(DevShells bra, const int* s1, const int* s2, const int* list, const int64_t* off, int nlist,
+               const SphTable* sph, double* EtG, hipStream_t st) {
+  if (nlist <= 0) return;
+  hipLaunchKernelGGL((int3c2e_et_kernel<LA, LB>), dim3(nlist), dim3(256), 0, st, bra, s1, s2, list, off, sph, EtG);
+}
+
+template <int LA, int LB>
+EtLaunch make_et() {
+  return EtLaunch{&launch_et<LA, LB>, Cls3m<LA, LB, 0>::ETPP};
+}
+
+using Int3MTable = Int3MLaunch[LMAX_ORB + 1][LMAX_ORB + 1][LMAX_AUX + 1];
+using EtTable = EtLaunch[LMAX_ORB + 1][LMAX_ORB + 1];
+
+}  // namespace dfhip

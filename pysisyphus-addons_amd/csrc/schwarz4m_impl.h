@@ -50,27 +50,6 @@ struct Cls4m {
   static constexpr bool OK = LDS <= 20000;
 };
 
-// R_tuv in closed form with the coefficient table hc[t*LH + i] = t!/(i!(t-2i)!).
-__device__ inline double herm_R_tab(int t, int u, int v, const double* __restrict__ pw, int L,
-                                    const double* __restrict__ g, const double* __restrict__ hc, int LH) {
-  const double* px = pw;
-  const double* py = pw + (L + 1);
-  const double* pz = pw + 2 * (L + 1);
-  double sum = 0.0;
-  for (int i = 0; 2 * i <= t; i++) {
-    const double xi = hc[t * LH + i] * px[t - 2 * i];
-    for (int j = 0; 2 * j <= u; j++) {
-      const double xij = xi * hc[u * LH + j] * py[u - 2 * j];
-      const double* gg = g + (t + u + v - i - j);
-      double inner = 0.0;
-      for (int k = 0; 2 * k <= v; k++) inner = fma(hc[v * LH + k] * pz[v - 2 * k], gg[-k], inner);
-      sum = fma(xij, inner, sum);
-    }
-  }
-  return sum;
-}
-
-// herm_E_dir computed in registers (fully unrolled) and stored once.
 template <int LA, int LB>
 __device__ inline void herm_E_dir_reg(double a, double b, double A, double B, double* __restrict__ out) {
   constexpr int NT = LA + LB + 1, ESZ = (LA + 1) * (LB + 1) * NT;
@@ -101,32 +80,6 @@ __device__ inline void herm_E_dir_reg(double a, double b, double A, double B, do
     }
 #pragma unroll
   for (int k = 0; k < ESZ; k++) out[k] = E[k];
-}
-
-// bra_element with the solid-harmonic terms of LA and LB staged in LDS:
-// terms of (LA, mu) are ta[sa[mu] .. sa[mu+1]), likewise for LB.
-template <int LA, int LB>
-__device__ inline double bra_element_lds(const int* sa, const SphTerm* ta, const int* sb, const SphTerm* tb, int mu,
-                                         int nu, int t, int u, int v, const double* Ex, const double* Ey,
-                                         const double* Ez) {
-  constexpr int NT = LA + LB + 1;
-  double val = 0.0;
-  const int b0 = sb[nu], b1 = sb[nu + 1];
-  for (int ia = sa[mu]; ia < sa[mu + 1]; ia++) {
-    const SphTerm A = ta[ia];
-    if (t > A.lx + LB || u > A.ly + LB || v > A.lz + LB) continue;
-    const double* ex = Ex + A.lx * (LB + 1) * NT + t;
-    const double* ey = Ey + A.ly * (LB + 1) * NT + u;
-    const double* ez = Ez + A.lz * (LB + 1) * NT + v;
-    double s = 0.0;
-    for (int ib = b0; ib < b1; ib++) {
-      const SphTerm B = tb[ib];
-      if (t > A.lx + B.lx || u > A.ly + B.ly || v > A.lz + B.lz) continue;
-      s = fma(B.c, ex[B.lx * NT] * ey[B.ly * NT] * ez[B.lz * NT], s);
-    }
-    val = fma(A.c, s, val);
-  }
-  return val;
 }
 
 template <int LA, int LB>

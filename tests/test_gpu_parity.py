@@ -108,6 +108,39 @@ def test_int3c2e_blocks_all_classes_unscreened():
     assert err <= 1e-12 * max(scale, 1.0)
 
 
+def test_int3c2e_blocks_g_orbitals():
+    """Classes with a g orbital shell (the matrix-core kernels), aux up to h,
+    unscreened."""
+    rng = np.random.default_rng(17)
+    orb, aux = [], []
+    for ci, C in enumerate(rng.uniform(-1.2, 1.2, (2, 3))):
+        for L in (0, 2, 4):
+            e = 0.4 * 3.0 ** np.arange(2) * rng.uniform(0.5, 2.0)
+            orb.append(Shell(L=L, center=C, exps=e, coeffs=rng.uniform(0.2, 1.0, 2), center_ind=ci))
+        for L in (0, 3, 4, 5):
+            aux.append(Shell(L=L, center=C, exps=[rng.uniform(0.3, 2.0)], coeffs=[1.0], center_ind=ci))
+    shells, auxs = Shells(orb), Shells(aux)
+    got, st = dense3c(shells, auxs, screening=N.SCREEN_NONE)
+    ref = oracle_dense3c(shells, auxs, screened=False)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max()
+    _say(f"3c2e g orbitals: max abs err {err:.2e} (max |int| {scale:.2e})")
+    assert err <= 1e-12 * max(scale, 1.0)
+
+
+def test_matrix_core_vs_lds_kernels(monkeypatch):
+    """The matrix-core class kernels and the LDS kernels agree on every class
+    both run (all classes up to (ff|h))."""
+    shells, aux = _mixed_system(8)
+    a, _ = dense3c(shells, aux, screening=N.SCREEN_NONE)
+    monkeypatch.setenv("DFHIP_NO_MM", "1")
+    b, _ = dense3c(shells, aux, screening=N.SCREEN_NONE)
+    monkeypatch.delenv("DFHIP_NO_MM")
+    err = np.abs(a - b).max()
+    _say(f"matrix-core vs LDS kernels: max abs diff {err:.2e}")
+    assert err <= 1e-12 * max(np.abs(a).max(), 1.0)
+
+
 def test_int3c2e_blocks_contracted_aux():
     """Contracted auxiliary shells (aux = orbital basis, as in the reference's
     Coulomb-tensor test) with variable primitive counts."""
