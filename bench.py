@@ -29,6 +29,19 @@ sys.path.insert(0, str(ROOT / "pysisyphus-addons_amd"))
 METRIC = "screened 3c2e shell-triplets/s + J-tensor time: mixed vs FP64 GPU at ≤1e-8 Eh"
 WORKLOAD = "60 atoms in a 22 bohr cube, orbital 3s2p1d (3-prim), aux 4s3p2d1f, 8 states, FP64"
 FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_SUMMARY = ROOT / "profiles" / "traffic_60atom_fp64.md"
+
+
+def measured_traffic():
+    """HBM bytes of the 3c2e kernels per step, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (None without it)."""
+    try:
+        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+            if line.startswith("int3c2e_hbm_bytes_per_step:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(shells, aux, dens, step):
@@ -59,7 +72,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-step", type=int, default=6, help="aux-shell stride of the CPU sample")
+    ap.add_argument("--cpu-step", type=int, default=1, help="aux-shell stride of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
     ap.add_argument("--fp32-below", type=float, default=None, help="mixed path: FP32 below this Schwarz product")
@@ -161,8 +174,9 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "int3c2e_kernel (all classes of one step)",
+            "traffic": measured_traffic(),
+            "traffic_source": "profiles/traffic_60atom_fp64.md (rocprofv3 FETCH_SIZE, WRITE_SIZE, copy-calibrated)",
+            "kernel": "int3c2e class kernels (all classes of one step)",
             "flops_per_launch": alg_flops,
             "ms_per_launch": ms_int,
         },
