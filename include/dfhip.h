@@ -141,6 +141,20 @@ int dfhip_contract_coulomb_densities(const dfhip_basis *bas, const dfhip_basis *
                                      const dfhip_options *opt, double *df_tensor,
                                      int32_t nstates, double *coulomb_tensor, dfhip_stats *stats);
 
+/* The batched form: nmol independent molecules.  Molecule k has the bases
+ * bas[k] / aux[k], ndens[k] densities of nbfs[k] functions at densities[k]
+ * (layout opt->dens_layout), and receives its Coulomb tensor
+ * (nstates[k]^4, column-major) in coulomb_tensors[k] and, if stats is not
+ * NULL, its statistics in stats[k].  The molecules run concurrently on
+ * nlanes streams (0 = 4, at most 8), each with its own workspace, one host
+ * thread per stream; the results equal those of one call per molecule.  The
+ * first failing molecule stops the batch; the message names it. */
+int dfhip_contract_coulomb_densities_batch(int32_t nmol, const dfhip_basis *bas, const dfhip_basis *aux,
+                                           const int32_t *ndens, const int32_t *nbfs,
+                                           const double *const *densities, const dfhip_options *opt,
+                                           const int32_t *nstates, double *const *coulomb_tensors,
+                                           dfhip_stats *stats, int32_t nlanes);
+
 /* Dense screened 3c2e tensor out[(i*nbf + j)*naux + P] (row-major, both
  * (i,j) orders filled, screened-out triplets zero), for parity checks on
  * small systems. */

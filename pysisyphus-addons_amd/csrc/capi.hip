@@ -16,7 +16,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <atomic>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -126,11 +129,16 @@ DevPool& dev_pool() {
   return *pool;
 }
 
+// Pool of the calling thread's lane (see Lane); the default lane uses dev_pool().
+thread_local DevPool* t_pool = nullptr;
+DevPool& cur_pool() { return t_pool ? *t_pool : dev_pool(); }
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
-  size_t cap = 0;  // bytes of the pool block
+  size_t cap = 0;            // bytes of the pool block
+  DevPool* pool = nullptr;   // the pool the block came from
   DBuf() = default;
   explicit DBuf(size_t count) { alloc(count); }
   DBuf(const DBuf&) = delete;
@@ -138,12 +146,13 @@ struct DBuf {
   void alloc(size_t count) {
     release();
     if (count == 0) return;
-    p = (T*)dev_pool().get(count * sizeof(T), cap);
+    pool = &cur_pool();
+    p = (T*)pool->get(count * sizeof(T), cap);
     if (!p) throw Error{DFHIP_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
     n = count;
   }
   void release() {
-    if (p) dev_pool().put(p, cap);
+    if (p) pool->put(p, cap);
     p = nullptr;
     n = 0;
     cap = 0;
@@ -234,10 +243,31 @@ void build_boys_table(std::vector<double>& tab) {
   }
 }
 
+// A stream with its own workspace and block pool.  Calls run on the default
+// lane; a batch runs its molecules concurrently on several lanes, one host
+// thread each.  Blocks are only reused within their lane, i.e. by work queued
+// on the same stream after the work of their previous owner.
+struct Lane {
+  hipStream_t st = nullptr;
+  DevPool pool;
+  // device workspace reused across calls (grown on demand, freed by
+  // dfhip_release_workspace)
+  DBuf<double> wsT, wsA, wsDp, wsPart, wsDraw;
+  void release() {
+    wsT.release();
+    wsA.release();
+    wsDp.release();
+    wsPart.release();
+    wsDraw.release();
+    pool.release();
+  }
+};
+
 struct Ctx {
   bool ready = false;
   int device = -1;
-  hipStream_t st = nullptr;
+  Lane lane0;                                 // default lane (its blocks come from dev_pool())
+  std::vector<std::unique_ptr<Lane>> lanes;   // batch lanes 1..
   DBuf<double> boys;
   DBuf<SphTable> sph;
   Int3Table int3{};
@@ -248,9 +278,6 @@ struct Ctx {
   Schwarz4MTable s4m{};
   Schwarz4STable s4s{};
   Gen3Table gen3{};
-  // device workspace reused across calls (grown on demand, freed by
-  // dfhip_release_workspace)
-  DBuf<double> wsT, wsA, wsDp, wsPart, wsDraw;
 };
 
 template <class T>
@@ -269,22 +296,26 @@ std::mutex& ctx_mutex() {
   return m;
 }
 
+thread_local Lane* t_lane = nullptr;
+Lane& cur_lane() { return t_lane ? *t_lane : ctx().lane0; }
+
 void ensure_ctx() {
   Ctx& c = ctx();
   if (c.ready) return;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error{DFHIP_ERR_HIP, "no HIP device available"};
   HC(hipGetDevice(&c.device));
-  HC(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+  HC(hipStreamCreateWithFlags(&c.lane0.st, hipStreamNonBlocking));
+  const hipStream_t st0 = c.lane0.st;
   std::vector<double> tab;
   build_boys_table(tab);
-  upload(c.boys, tab, c.st);
+  upload(c.boys, tab, st0);
   SphTable* h = new SphTable();
   std::memset(h, 0, sizeof(SphTable));
   build_sph_table(*h);
   c.sph.alloc(1);
-  HC(hipMemcpyAsync(c.sph.p, h, sizeof(SphTable), hipMemcpyHostToDevice, c.st));
-  HC(hipStreamSynchronize(c.st));
+  HC(hipMemcpyAsync(c.sph.p, h, sizeof(SphTable), hipMemcpyHostToDevice, st0));
+  HC(hipStreamSynchronize(st0));
   delete h;
   register_int3c2e_mm_l0(c.int3m, c.et);
   register_int3c2e_mm_l1(c.int3m, c.et);
@@ -459,7 +490,7 @@ PairSet make_pairs(const HostShells& o) {
 // fit the primitive-pair cache use the cached kernel (one partial per pair),
 // the others the chunked kernel.
 std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps) {
-  hipStream_t st = c.st;
+  hipStream_t st = cur_lane().st;
   DBuf<int> s1, s2;
   upload(s1, ps.s1, st);
   upload(s2, ps.s2, st);
@@ -575,14 +606,14 @@ void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevS
     npp[p] = bra.nprim[s1[p]] * (s2[p] >= 0 ? bra.nprim[s2[p]] : 1);
     o += (int64_t)npp[p] * rs;
   }
-  upload(br.off, off, c.st);
-  upload(br.npp, npp, c.st);
+  upload(br.off, off, cur_lane().st);
+  upload(br.npp, npp, cur_lane().st);
   br.recs.alloc(std::max<int64_t>(o, 1));
-  launch_bra_records(la, lb, dev, ds1, ds2, br.off.p, rs, (int)s1.size(), br.recs.p, c.st);
+  launch_bra_records(la, lb, dev, ds1, ds2, br.off.p, rs, (int)s1.size(), br.recs.p, cur_lane().st);
   HC(hipGetLastError());
   if (fp32 && o > 0) {
     br.recs32.alloc(o);
-    hipLaunchKernelGGL(records_to_f32_kernel, dim3((unsigned)((o + 255) / 256)), dim3(256), 0, c.st, br.recs.p,
+    hipLaunchKernelGGL(records_to_f32_kernel, dim3((unsigned)((o + 255) / 256)), dim3(256), 0, cur_lane().st, br.recs.p,
                        br.recs32.p, o);
     HC(hipGetLastError());
   }
@@ -595,14 +626,14 @@ void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells
   if (pa.npairs <= 0 || nchunks <= 0) return;
   if (fp32 && !br.recs32.p) throw Error{DFHIP_ERR_HIP, "FP32 bra records missing"};
   hipLaunchKernelGGL(fp32 ? g.kernel32 : g.kernel, dim3(pa.npairs, nchunks, fp32 ? g.groups32 : g.groups), dim3(64),
-                     0, c.st, br.recs.p,
+                     0, cur_lane().st, br.recs.p,
                      br.recs32.p, br.off.p, br.npp.p, ket, pa, xa, c.boys.p, out, ldo);
   HC(hipGetLastError());
 }
 
 // Full metric (P|Q) (naux x naux) into M via the 2c2e form of the 3c2e kernels.
 void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, int64_t ldm) {
-  hipStream_t st = c.st;
+  hipStream_t st = cur_lane().st;
   std::vector<std::vector<int>> byL(LMAX_AUX + 1);
   for (int i = 0; i < x.n; i++) byL[x.L[i]].push_back(i);
   std::vector<int> s1, s2, nsurv;
@@ -930,8 +961,10 @@ struct Outputs {
   double* dense3c = nullptr;    // host, (nbf, nbf, naux) row-major
 };
 
+// take_lock = false: the caller holds the library mutex and has set up the
+// context (a batch worker on its own lane).
 void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const double* dens,
-         const dfhip_options& opt, Outputs out, dfhip_stats* stats) {
+         const dfhip_options& opt, Outputs out, dfhip_stats* stats, bool take_lock = true) {
   if (opt.precision != DFHIP_FP64 && opt.precision != DFHIP_MIXED)
     throw Error{DFHIP_ERR_UNSUPPORTED, "unknown precision"};
   const bool mixed = opt.precision == DFHIP_MIXED;
@@ -956,10 +989,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     throw Error{DFHIP_ERR_ARG, "ndens is not nstates(nstates+1)/2"};
   const int64_t naux = x.nbf;
 
-  std::lock_guard<std::mutex> lock(ctx_mutex());
-  ensure_ctx();
+  std::unique_lock<std::mutex> lock(ctx_mutex(), std::defer_lock);
+  if (take_lock) {
+    lock.lock();
+    ensure_ctx();
+  }
   Ctx& c = ctx();
-  hipStream_t st = c.st;
+  hipStream_t st = cur_lane().st;
   Timer tm(st);
   const int e0 = tm.mark();
   DevShellBufs dorb, daux;
@@ -977,7 +1013,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // bordered matrix [[M, .], [d^T, 0]] (column-major, lda = naux + ndens),
   // and Q_C from its diagonal blocks
   const int64_t ntot = naux + (need_dens ? ndens : 0), lda = ntot;
-  double* Ap = reserve(c.wsA, (size_t)ntot * ntot);
+  double* Ap = reserve(cur_lane().wsA, (size_t)ntot * ntot);
   HC(hipMemsetAsync(Ap, 0, (size_t)ntot * ntot * sizeof(double), st));
   metric_dev(c, x, daux.view, Ap, lda);
   DBuf<double> dqc(x.n);
@@ -1045,7 +1081,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     const double* Dsrc = dens;
     if (!opt.dens_on_device) {
       const size_t nd = (size_t)ndens * nbfs * nbfs;
-      double* Draw = reserve(c.wsDraw, nd);
+      double* Draw = reserve(cur_lane().wsDraw, nd);
       const int h0 = tm.mark();
       HC(hipMemcpyAsync(Draw, dens, nd * sizeof(double), hipMemcpyHostToDevice, st));
       const int h1 = tm.mark();
@@ -1063,7 +1099,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       si = ndens;
       sj = (int64_t)ndens * nbfs;
     }
-    Dp = reserve(c.wsDp, (size_t)pl.npf * ndens);
+    Dp = reserve(cur_lane().wsDp, (size_t)pl.npf * ndens);
     pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp, ndens, st);
     HC(hipGetLastError());
     HC(hipStreamSynchronize(st));
@@ -1094,7 +1130,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   const int nblocks = (int)bounds.size() - 1;
   int64_t widest = 0;
   for (int b = 0; b < nblocks; b++) widest = std::max(widest, bounds[b + 1] - bounds[b]);
-  double* Tp = reserve(c.wsT, (size_t)std::max<int64_t>(pl.npf, 1) * std::max<int64_t>(widest, 1));
+  double* Tp = reserve(cur_lane().wsT, (size_t)std::max<int64_t>(pl.npf, 1) * std::max<int64_t>(widest, 1));
   double* partp = nullptr;
   int nsplit = 1;
   // screened-prefix mask of T (the 3c2e kernels write only screened-in blocks)
@@ -1126,7 +1162,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     const int64_t tiles = ((widest + 63) / 64) * ((ndens + 63) / 64);
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(512, 2048 / std::max<int64_t>(tiles, 1)));
     nsplit = (int)std::min<int64_t>(nsplit, std::max<int64_t>(1, (pl.npf + 15) / 16));
-    partp = reserve(c.wsPart, (size_t)nsplit * widest * ndens);
+    partp = reserve(cur_lane().wsPart, (size_t)nsplit * widest * ndens);
   }
   // Non-zero row segments of T per column tile: for each pair class, the
   // rows from the class start to the last pair whose screened prefix reaches
@@ -1418,7 +1454,7 @@ std::vector<double> schwarz_bra_host(const dfhip_basis& b) {
   Ctx& c = ctx();
   HostShells o = parse_shells(b, LMAX_ORB, "basis");
   DevShellBufs d;
-  d.upload_from(o, c.st);
+  d.upload_from(o, cur_lane().st);
   PairSet ps = make_pairs(o);
   std::vector<double> q = schwarz_bra_dev(c, o, d.view, ps);
   std::vector<double> outp(ps.n());
@@ -1432,18 +1468,18 @@ std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
   Ctx& c = ctx();
   HostShells x = parse_shells(b, LMAX_AUX, "basis");
   DevShellBufs d;
-  d.upload_from(x, c.st);
+  d.upload_from(x, cur_lane().st);
   DBuf<double> M((size_t)x.nbf * x.nbf);
   metric_dev(c, x, d.view, M.p, x.nbf);
   std::vector<double> h((size_t)x.nbf * x.nbf);
   if (qc) {
     DBuf<double> dq(x.n);
-    schwarz_aux_from_metric(M.p, x.nbf, d.L.p, d.f0.p, x.n, dq.p, c.st);
+    schwarz_aux_from_metric(M.p, x.nbf, d.L.p, d.f0.p, x.n, dq.p, cur_lane().st);
     qc->resize(x.n);
-    HC(hipMemcpyAsync(qc->data(), dq.p, x.n * sizeof(double), hipMemcpyDeviceToHost, c.st));
+    HC(hipMemcpyAsync(qc->data(), dq.p, x.n * sizeof(double), hipMemcpyDeviceToHost, cur_lane().st));
   }
-  HC(hipMemcpyAsync(h.data(), M.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.st));
-  HC(hipStreamSynchronize(c.st));
+  HC(hipMemcpyAsync(h.data(), M.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, cur_lane().st));
+  HC(hipStreamSynchronize(cur_lane().st));
   return h;
 }
 
@@ -1467,11 +1503,8 @@ int dfhip_release_workspace(void) {
   return guarded([&] {
     std::lock_guard<std::mutex> lock(ctx_mutex());
     Ctx& c = ctx();
-    c.wsT.release();
-    c.wsA.release();
-    c.wsDp.release();
-    c.wsPart.release();
-    c.wsDraw.release();
+    c.lane0.release();
+    for (auto& ln : c.lanes) ln->release();
     dev_pool().release();
   });
 }
@@ -1492,6 +1525,74 @@ int dfhip_contract_coulomb_densities(const dfhip_basis* bas, const dfhip_basis* 
     out.J = coulomb_tensor;
     out.nstates = nstates;
     run(*bas, *aux, ndens, nbfs, densities, o, out, stats);
+  });
+}
+
+int dfhip_contract_coulomb_densities_batch(int32_t nmol, const dfhip_basis* bas, const dfhip_basis* aux,
+                                           const int32_t* ndens, const int32_t* nbfs,
+                                           const double* const* densities, const dfhip_options* opt,
+                                           const int32_t* nstates, double* const* coulomb_tensors,
+                                           dfhip_stats* stats, int32_t nlanes) {
+  return guarded([&] {
+    if (nmol < 0) throw Error{DFHIP_ERR_ARG, "nmol < 0"};
+    if (nmol == 0) return;
+    if (!bas || !aux || !ndens || !nbfs || !densities || !nstates || !coulomb_tensors)
+      throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    const dfhip_options o = opt ? *opt : defaults();
+    std::lock_guard<std::mutex> lock(ctx_mutex());
+    ensure_ctx();
+    Ctx& c = ctx();
+    const int nl = std::max(1, std::min({nlanes > 0 ? (int)nlanes : 4, 8, (int)nmol}));
+    while ((int)c.lanes.size() < nl - 1) {
+      auto ln = std::make_unique<Lane>();
+      HC(hipStreamCreateWithFlags(&ln->st, hipStreamNonBlocking));
+      c.lanes.push_back(std::move(ln));
+    }
+    std::atomic<int> next(0);
+    std::mutex emu;
+    int ecode = DFHIP_OK;
+    std::string emsg;
+    auto worker = [&](int w) {
+      t_lane = w == 0 ? &c.lane0 : c.lanes[w - 1].get();
+      t_pool = w == 0 ? nullptr : &t_lane->pool;
+      if (hipSetDevice(c.device) != hipSuccess) {
+        std::lock_guard<std::mutex> g(emu);
+        if (ecode == DFHIP_OK) {
+          ecode = DFHIP_ERR_HIP;
+          emsg = "hipSetDevice failed in a batch worker";
+        }
+        return;
+      }
+      for (int k = next++; k < nmol; k = next++) {
+        try {
+          Outputs out;
+          out.J = coulomb_tensors[k];
+          out.nstates = nstates[k];
+          run(bas[k], aux[k], ndens[k], nbfs[k], densities[k], o, out, stats ? &stats[k] : nullptr, false);
+        } catch (const Error& e) {
+          std::lock_guard<std::mutex> g(emu);
+          if (ecode == DFHIP_OK) {
+            ecode = e.code;
+            emsg = "molecule " + std::to_string(k) + ": " + e.msg;
+          }
+          next = nmol;
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> g(emu);
+          if (ecode == DFHIP_OK) {
+            ecode = DFHIP_ERR_HIP;
+            emsg = "molecule " + std::to_string(k) + ": " + e.what();
+          }
+          next = nmol;
+        }
+      }
+      t_lane = nullptr;
+      t_pool = nullptr;
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nl; w++) th.emplace_back(worker, w);
+    worker(0);
+    for (auto& t : th) t.join();
+    if (ecode != DFHIP_OK) throw Error{ecode, emsg};
   });
 }
 
@@ -1526,13 +1627,13 @@ int dfhip_boys(int64_t npts, const int32_t* n, const double* T, double* F) {
     DBuf<int> dn(std::max<int64_t>(npts, 1));
     DBuf<double> dT(std::max<int64_t>(npts, 1)), dF(std::max<int64_t>(npts, 1));
     if (npts > 0) {
-      HC(hipMemcpyAsync(dn.p, n, npts * sizeof(int), hipMemcpyHostToDevice, c.st));
-      HC(hipMemcpyAsync(dT.p, T, npts * sizeof(double), hipMemcpyHostToDevice, c.st));
-      boys_probe(dn.p, dT.p, npts, c.boys.p, dF.p, c.st);
+      HC(hipMemcpyAsync(dn.p, n, npts * sizeof(int), hipMemcpyHostToDevice, cur_lane().st));
+      HC(hipMemcpyAsync(dT.p, T, npts * sizeof(double), hipMemcpyHostToDevice, cur_lane().st));
+      boys_probe(dn.p, dT.p, npts, c.boys.p, dF.p, cur_lane().st);
       HC(hipGetLastError());
-      HC(hipMemcpyAsync(F, dF.p, npts * sizeof(double), hipMemcpyDeviceToHost, c.st));
+      HC(hipMemcpyAsync(F, dF.p, npts * sizeof(double), hipMemcpyDeviceToHost, cur_lane().st));
     }
-    HC(hipStreamSynchronize(c.st));
+    HC(hipStreamSynchronize(cur_lane().st));
   });
 }
 

@@ -77,7 +77,7 @@ EXPORTED = (
     "f_init", "f_contract_coulomb_densities_2d", "f_contract_coulomb_densities_4d",
     "f_int_schwarz_aux", "f_int_schwarz_bra", "dfhip_last_status", "dfhip_last_error",
     "dfhip_default_options", "dfhip_contract_coulomb_densities", "dfhip_int3c2e_dense",
-    "dfhip_metric", "dfhip_boys", "dfhip_release_workspace",
+    "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
 )
 
 
@@ -96,6 +96,10 @@ def lib():
         L.dfhip_contract_coulomb_densities.argtypes = [
             ctypes.POINTER(Basis), ctypes.POINTER(Basis), i32, i32, vp, ctypes.POINTER(Options),
             vp, i32, vp, ctypes.POINTER(Stats)]
+        L.dfhip_contract_coulomb_densities_batch.restype = ctypes.c_int
+        L.dfhip_contract_coulomb_densities_batch.argtypes = [
+            i32, ctypes.POINTER(Basis), ctypes.POINTER(Basis), vp, vp, vp, ctypes.POINTER(Options), vp, vp,
+            ctypes.POINTER(Stats), i32]
         L.dfhip_int3c2e_dense.restype = ctypes.c_int
         L.dfhip_int3c2e_dense.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis),
                                           ctypes.POINTER(Options), vp, ctypes.POINTER(Stats)]

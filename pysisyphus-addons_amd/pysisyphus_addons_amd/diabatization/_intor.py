@@ -58,6 +58,31 @@ def contract(bas, aux, dens, nstates=0, want_df=True, want_j=False, options=None
     return df, J, st.as_dict()
 
 
+def contract_batch(mols, nstates, options=None, nlanes=0):
+    """Coulomb tensors of independent molecules in one batched call.
+    mols: list of (BasisArrays, BasisArrays, densities (ndens, nbf, nbf));
+    nstates: per molecule.  Returns (list of J (nstates,)*4 F-order, list of
+    stats dicts)."""
+    n = len(mols)
+    opt = options if options is not None else N.default_options()
+    opt.dens_layout = N.DENS_C
+    basT = (N.Basis * n)(*[m[0].struct for m in mols])
+    auxT = (N.Basis * n)(*[m[1].struct for m in mols])
+    dens = [np.ascontiguousarray(m[2], dtype=np.float64) for m in mols]
+    nd = np.array([d.shape[0] for d in dens], dtype=np.int32)
+    nb = np.array([d.shape[1] for d in dens], dtype=np.int32)
+    ns = np.array(nstates, dtype=np.int32)
+    Js = [np.zeros((int(k),) * 4, order="F") for k in ns]
+    dptr = (N.ctypes.c_void_p * n)(*[d.ctypes.data for d in dens])
+    jptr = (N.ctypes.c_void_p * n)(*[J.ctypes.data for J in Js])
+    stats = (N.Stats * n)()
+    status = N.lib().dfhip_contract_coulomb_densities_batch(
+        n, basT, auxT, nd.ctypes.data, nb.ctypes.data, dptr, N.ctypes.byref(opt), ns.ctypes.data, jptr, stats,
+        nlanes)
+    N.check(status)
+    return Js, [st.as_dict() for st in stats]
+
+
 def contract_coulomb_densities_2d(bas_centers, bas_spec, bas_data, bas_centers_aux, bas_spec_aux,
                                   bas_data_aux, densities):
     b, x, dens = _prep(bas_centers, bas_spec, bas_data, bas_centers_aux, bas_spec_aux,

@@ -73,6 +73,13 @@ def test_no_device_reports_status(monkeypatch):
     with pytest.raises(N.DfhipError) as e:
         intor.contract_coulomb_densities_4d(shells, aux, dens)
     assert e.value.status == 2
+    # the batched form marshals its pointer arrays and reports the same status
+    from pysisyphus_addons_amd.diabatization import _intor
+
+    mols = [(N.BasisArrays(*shells.as_arrays()), N.BasisArrays(*aux.as_arrays()), dens)] * 3
+    with pytest.raises(N.DfhipError) as e:
+        _intor.contract_batch(mols, [1, 1, 1])
+    assert e.value.status == 2
 
 
 def test_workload_recipes():
