@@ -1100,7 +1100,25 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       sj = (int64_t)ndens * nbfs;
     }
     Dp = reserve(cur_lane().wsDp, (size_t)pl.npf * ndens);
-    pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp, ndens, st);
+    if (sj == 1 && si == nbfs) {
+      // density index slowest: tiled transpose (pair lookup by shell pair, shell by function)
+      std::vector<int> pairof((size_t)o.n * o.n), fshell(o.nbf);
+      for (int p = 0; p < pl.ps.n(); p++) {
+        const int a = pl.ps.s1[p], b = pl.ps.s2[p];
+        pairof[(size_t)a * o.n + b] = p | (1 << 30);
+        if (a != b) pairof[(size_t)b * o.n + a] = p;
+      }
+      for (int sh = 0; sh < o.n; sh++)
+        for (int m = 0; m < 2 * o.L[sh] + 1; m++) fshell[o.f0[sh] + m] = sh;
+      DBuf<int> dpairof, dfshell;
+      upload(dpairof, pairof, st);
+      upload(dfshell, fshell, st);
+      pack_density_t(dpairof.p, dfshell.p, prow.p, dorb.L.p, dorb.f0.p, o.n, o.nbf, Dsrc, sI, ndens, Dp, ndens, st);
+      HC(hipGetLastError());
+      HC(hipStreamSynchronize(st));
+    } else {
+      pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp, ndens, st);
+    }
     HC(hipGetLastError());
     HC(hipStreamSynchronize(st));
   }

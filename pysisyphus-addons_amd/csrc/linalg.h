@@ -13,6 +13,11 @@ namespace dfhip {
 void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, const int* L, const int* f0,
                   const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
                   hipStream_t st);
+// The packing for the C layout (density index slowest) as a tiled transpose;
+// pairof[s*nsh + t] = pair index of {s, t} | (1 << 30 if oriented (s, t)),
+// fshell[j] = shell of function j.
+void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, const int* L, const int* f0, int nsh,
+                    int nbf, const double* D, int64_t sI, int ndens, double* Dp, int ldd, hipStream_t st);
 // Screened-prefix mask of the staging matrix T: entry (row k, column P) is
 // live iff colia[P] < nsurv[colL[P] * npairs + rowpair[k]] (column arrays
 // relative to the column block).

@@ -44,6 +44,66 @@ void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, 
                      ndens, Dp, ldd);
 }
 
+// The same packing for densities whose density index is the slowest
+// (D[I*sI + i*nbf + j], the C layout): a tiled transpose.  One workgroup =
+// one row shell s x 32 densities x 64 columns j.  For each row i = f(s) + mu
+// the tile D[I0..I0+32][i][j0..j0+64] is read along j (coalesced), turned in
+// LDS and written along I into the packed row of the element's pair:
+//   pass 0: elements whose pair has s as first shell, Dp = D (stored);
+//   pass 1: elements whose pair has s as second shell (s1 != s2), Dp += D^T.
+// Every packed entry gets its pass-0 value from one thread and, for s1 != s2,
+// its pass-1 addition from one thread of the next launch, so the result does
+// not depend on the schedule.  pairof[s*nsh + t] = pair index of shells {s, t}
+// with bit 30 set when the pair is oriented (s, t); fshell[j] = shell of
+// function j.
+constexpr int PK_I = 32, PK_J = 64;
+__global__ __launch_bounds__(256) void pack_density_t_kernel(const int* __restrict__ pairof,
+                                                             const int* __restrict__ fshell,
+                                                             const int64_t* __restrict__ rowv,
+                                                             const int* __restrict__ Lv, const int* __restrict__ f0v,
+                                                             int nsh, int nbf, const double* __restrict__ D,
+                                                             int64_t sI, int ndens, double* __restrict__ Dp, int ldd,
+                                                             int pass) {
+  __shared__ double tile[PK_I][PK_J + 1];
+  const int s = blockIdx.x, I0 = blockIdx.y * PK_I, j0 = blockIdx.z * PK_J;
+  const int ns = 2 * Lv[s] + 1, fs = f0v[s];
+  const int tid = threadIdx.x;
+  const int nI = min(PK_I, ndens - I0), nj = min(PK_J, nbf - j0);
+  for (int mu = 0; mu < ns; mu++) {
+    __syncthreads();
+    for (int e = tid; e < PK_I * PK_J; e += 256) {
+      const int I = e / PK_J, j = e % PK_J;
+      tile[I][j] = (I < nI && j < nj) ? D[(int64_t)(I0 + I) * sI + (int64_t)(fs + mu) * nbf + j0 + j] : 0.0;
+    }
+    __syncthreads();
+    // column j of the tile -> its pair, packed position; 32 densities per column
+    for (int e = tid; e < PK_I * PK_J; e += 256) {
+      const int j = e / PK_I, I = e % PK_I;
+      if (j >= nj || I >= nI) continue;
+      const int col = j0 + j, t = fshell[col];
+      const int code = pairof[s * nsh + t];
+      const bool first = (code >> 30) & 1;
+      const int p = code & ((1 << 30) - 1);
+      const int nt = 2 * Lv[t] + 1, nu = col - f0v[t];
+      if (pass == 0 && first) {
+        Dp[(rowv[p] + mu * nt + nu) * ldd + I0 + I] = tile[I][j];
+      } else if (pass == 1 && !first) {  // pair (t, s): position (nu, mu)
+        double* d = &Dp[(rowv[p] + nu * ns + mu) * ldd + I0 + I];
+        *d += tile[I][j];
+      }
+    }
+  }
+}
+
+void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, const int* L, const int* f0, int nsh,
+                    int nbf, const double* D, int64_t sI, int ndens, double* Dp, int ldd, hipStream_t st) {
+  if (nsh <= 0 || nbf <= 0 || ndens <= 0) return;
+  const dim3 grid(nsh, (ndens + PK_I - 1) / PK_I, (nbf + PK_J - 1) / PK_J);
+  for (int pass = 0; pass < 2; pass++)
+    hipLaunchKernelGGL(pack_density_t_kernel, grid, dim3(256), 0, st, pairof, fshell, row, L, f0, nsh, nbf, D, sI,
+                       ndens, Dp, ldd, pass);
+}
+
 // ---------------------------------------------------------------------------
 // part[s][P][I] = sum_{k in split s} T[k][P] * Dp[k][I]
 // T: npf x ncol (row-major, ldt), Dp: npf x ndens (row-major, ldd).
