@@ -44,6 +44,10 @@ __device__ inline bool triplet_kept(const PairArgs& pa, const AuxArgs& xa, int p
 __host__ __device__ constexpr int nsph_terms(int L) {
   return L == 0 ? 1 : L == 1 ? 3 : L == 2 ? 8 : L == 3 ? 16 : L == 4 ? 28 : L == 5 ? 46 : 70;
 }
+// Largest number of terms of one real solid harmonic (L, m).
+__host__ __device__ constexpr int sph_max_terms(int L) {
+  return L <= 1 ? 1 : L <= 3 ? 3 : L <= 5 ? 6 : 10;
+}
 
 template <int LA, int LB, int LC>
 struct Cls3 {

@@ -47,6 +47,9 @@ struct Cls3m {
   static constexpr int GRP = NW / MT >= 1 ? NW / MT : 1;
   static constexpr int NPW = (NTL + GRP - 1) / GRP;
   static constexpr int KS = HK / 4;
+  // Hobson step: one thread per (h, m) and group of aux shells
+  static constexpr int MAXT = sph_max_terms(LC);
+  static constexpr int CG = NT / (H * NCC) >= 1 ? NT / (H * NCC) : 1;
   static constexpr int LH = L / 2 + 1;
   static constexpr int64_t ETPP = (int64_t)HK * NABP;  // Et table doubles per primitive pair
   static constexpr bool OK = nc_fit(NC);
@@ -218,24 +221,37 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
         G[k] = val;
       }
       __syncthreads();
+      // aux shell fastest: the lanes of a wave share (t, u, v) and so the loop bounds
       for (int k = tid; k < nc * (NR - K::NRLO); k += NT) {
-        const int c = k / (NR - K::NRLO), h = K::NRLO + (k - c * (NR - K::NRLO));
+        const int hh = k / nc, c = k - hh * nc, h = K::NRLO + hh;
         const int tv = tuv[h];
         const double* g = G + c * GW;
         R[c * NR + h] = herm_R_tab(tv & 255, (tv >> 8) & 255, tv >> 16, g + (L + 1), L, g, hc, LH);
       }
       __syncthreads();
-      for (int k = tid; k < H * nc * NCC; k += NT) {
-        const int h = k / (nc * NCC), col = k - h * (nc * NCC), c = col / NCC, m = col - c * NCC;
+      // W[h][c NCC + m] = sum_q c_q R_c[(t, u, v) + l_q] over the solid-harmonic terms
+      // q of (LC, m): the term indices are resolved once per (h, m) and reused
+      // for every aux shell of the thread's group
+      for (int k = tid; k < H * NCC * K::CG; k += NT) {
+        const int hm = k % (H * NCC), cg = k / (H * NCC), h = hm / NCC, m = hm - h * NCC;
         const int tv = tuv[h];
         const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
-        const double* Rc = R + c * NR;
-        double s = 0.0;
-        for (int q = sc[m]; q < sc[m + 1]; q++) {
-          const SphTerm tm = tc[q];
-          s = fma(tm.c, Rc[herm_index(t + tm.lx, u + tm.ly, v + tm.lz)], s);
+        const int q0 = sc[m], nq = sc[m + 1] - q0;
+        double cq[K::MAXT];
+        int iq[K::MAXT];
+#pragma unroll
+        for (int z = 0; z < K::MAXT; z++) {
+          const SphTerm tm = tc[z < nq ? q0 + z : q0];
+          cq[z] = z < nq ? tm.c : 0.0;
+          iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
         }
-        W[h * NCOLP + col] = s;
+        for (int c = cg; c < nc; c += K::CG) {
+          const double* Rc = R + c * NR;
+          double s = 0.0;
+#pragma unroll
+          for (int z = 0; z < K::MAXT; z++) s = fma(cq[z], Rc[iq[z]], s);
+          W[h * NCOLP + c * NCC + m] = s;
+        }
       }
       __syncthreads();
       // acc[nt] += Et[16 mt.., h] W[h, 16 nt..]
