@@ -32,8 +32,10 @@ struct Cls3m {
   static constexpr int HK = (H + 3) / 4 * 4, NABP = (NAB + 15) / 16 * 16, MT = NABP / 16;
   static constexpr int ESZ = (LA + 1) * (LB + 1) * (LAB + 1);
   static constexpr int GW = 4 * (L + 1);  // scaled Boys values and powers of 2(P - C) per aux shell
+  // LDS stride per aux shell: odd, so lanes on consecutive aux shells read distinct banks
+  static constexpr int GWP = GW + 1;
   static constexpr int BUD = 18500;       // LDS doubles for R, G and W
-  static constexpr int nc_fit(int nc) { return nc * (NR + GW) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD; }
+  static constexpr int nc_fit(int nc) { return nc * (NR + GWP) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD; }
   static constexpr int nc_max() {
     int nc = 32;
     while (nc > 1 && !nc_fit(nc)) nc--;
@@ -47,6 +49,10 @@ struct Cls3m {
   static constexpr int GRP = NW / MT >= 1 ? NW / MT : 1;
   static constexpr int NPW = (NTL + GRP - 1) / GRP;
   static constexpr int KS = HK / 4;
+  static constexpr int NRH = NR - NRLO;  // Hermite indices of order >= LC
+  // R step: RP aux shells per thread, CPL lanes per Hermite index
+  static constexpr int RP = NC >= 8 ? 2 : 1;
+  static constexpr int CPL = (NC + RP - 1) / RP;
   // Hobson step: one thread per (h, m) and group of aux shells
   static constexpr int MAXT = sph_max_terms(LC);
   static constexpr int CG = NT / (H * NCC) >= 1 ? NT / (H * NCC) : 1;
@@ -122,14 +128,15 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
                                                          int64_t ldo) {
   using K = Cls3m<LA, LB, LC>;
   constexpr int L = K::L, H = K::H, HK = K::HK, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NABP = K::NABP;
-  constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, LH = K::LH;
+  constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, GWP = K::GWP, LH = K::LH;
   constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL;
-  __shared__ double G[NC * GW];        // per aux shell: scaled Boys values, then (2 PC_d)^e
+  __shared__ double G[NC * GWP];       // per aux shell: scaled Boys values, then (2 PC_d)^e
   __shared__ double R[NC * NR];        // per aux shell: R_tuv
   __shared__ double W[HK * NCOLP];     // W[h][c NCC + m]
   __shared__ double cxyz[NC * 3];
   __shared__ int cshell[NC], ckeep[NC];
   __shared__ int tuv[NR];
+  __shared__ int rlist[K::NRH];  // (t, u, v) of order LC..L, sorted by (t/2, u/2, v/2)
   __shared__ double hc[(L + 1) * LH];
   __shared__ int sc[2 * LC + 2];
   __shared__ SphTerm tc[nsph_terms(LC)];
@@ -147,6 +154,18 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     int t, u, v;
     herm_tuv(h, t, u, v);
     tuv[h] = t | (u << 8) | (v << 16);
+  }
+  if (tid == 0) {
+    int n = 0;
+    for (int a = 0; 2 * a <= L; a++)
+      for (int b = 0; 2 * (a + b) <= L; b++)
+        for (int d = 0; 2 * (a + b + d) <= L; d++)
+          for (int pt = 0; pt < 2; pt++)
+            for (int pu = 0; pu < 2; pu++)
+              for (int pv = 0; pv < 2; pv++) {
+                const int t = 2 * a + pt, u = 2 * b + pu, v = 2 * d + pv;
+                if (t + u + v <= L && t + u + v >= LC) rlist[n++] = t | (u << 8) | (v << 16);
+              }
   }
   for (int e = tid; e < (L + 1) * LH; e += NT) {
     const int t = e / LH, i = e - t * LH;
@@ -218,15 +237,47 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
           val = 1.0;
           for (int z = 0; z < e; z++) val *= x;
         }
-        G[k] = val;
+        G[c * GWP + q] = val;
       }
       __syncthreads();
-      // aux shell fastest: the lanes of a wave share (t, u, v) and so the loop bounds
-      for (int k = tid; k < nc * (NR - K::NRLO); k += NT) {
-        const int hh = k / nc, c = k - hh * nc, h = K::NRLO + hh;
-        const int tv = tuv[h];
-        const double* g = G + c * GW;
-        R[c * NR + h] = herm_R_tab(tv & 255, (tv >> 8) & 255, tv >> 16, g + (L + 1), L, g, hc, LH);
+      // R_tuv (closed form, as herm_R_tab) over the Hermite indices in signature
+      // order (t/2, u/2, v/2), aux shells fastest: neighbouring lanes share the
+      // loop trip counts.  Each thread takes RP aux shells of one (t, u, v), so
+      // the chains of LDS reads and FMAs of its RP sums overlap.
+      for (int k = tid; k < K::CPL * K::NRH; k += NT) {
+        const int hh = k / K::CPL, cl = k - hh * K::CPL;  // compile-time divisor
+        if (cl >= nc) continue;
+        const int tv = rlist[hh];
+        const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16, tuvs = t + u + v;
+        const double* gr[K::RP];
+        double acc[K::RP];
+#pragma unroll
+        for (int r = 0; r < K::RP; r++) {
+          gr[r] = G + min(cl + r * K::CPL, nc - 1) * GWP;
+          acc[r] = 0.0;
+        }
+        for (int i = 0; 2 * i <= t; i++) {
+          const double ci = hc[t * LH + i];
+          for (int j = 0; 2 * j <= u; j++) {
+            const double cij = ci * hc[u * LH + j];
+            double pxy[K::RP];
+#pragma unroll
+            for (int r = 0; r < K::RP; r++) pxy[r] = cij * gr[r][(L + 1) + t - 2 * i] * gr[r][2 * (L + 1) + u - 2 * j];
+            for (int q = 0; 2 * q <= v; q++) {
+              const double cq = hc[v * LH + q];
+              const int e = tuvs - i - j - q;
+#pragma unroll
+              for (int r = 0; r < K::RP; r++)
+                acc[r] = fma(pxy[r] * cq, gr[r][3 * (L + 1) + v - 2 * q] * gr[r][e], acc[r]);
+            }
+          }
+        }
+        const int h = herm_index(t, u, v);
+#pragma unroll
+        for (int r = 0; r < K::RP; r++) {
+          const int c = cl + r * K::CPL;
+          if (c < nc) R[c * NR + h] = acc[r];
+        }
       }
       __syncthreads();
       // W[h][c NCC + m] = sum_q c_q R_c[(t, u, v) + l_q] over the solid-harmonic terms
