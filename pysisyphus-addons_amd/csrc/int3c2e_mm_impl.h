@@ -34,8 +34,13 @@ struct Cls3m {
   static constexpr int GW = 4 * (L + 1);  // scaled Boys values and powers of 2(P - C) per aux shell
   // LDS stride per aux shell: odd, so lanes on consecutive aux shells read distinct banks
   static constexpr int GWP = GW + 1;
-  static constexpr int BUD = 18500;       // LDS doubles for R, G and W
-  static constexpr int nc_fit(int nc) { return nc * (NR + GWP) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD; }
+  // R^(n)_tuv by levels n = L-1 .. 0 in two buffers: even levels in R (all NR
+  // indices), odd levels in Rt (orders <= L-1); odd strides per aux shell
+  static constexpr int NRP = NR | 1, NRT = nherm(L - 1) | 1;
+  static constexpr int BUD = 18500;       // LDS doubles for R, Rt, G and W
+  static constexpr int nc_fit(int nc) {
+    return nc * (NRP + NRT + GWP) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD;
+  }
   static constexpr int nc_max() {
     int nc = 32;
     while (nc > 1 && !nc_fit(nc)) nc--;
@@ -49,10 +54,6 @@ struct Cls3m {
   static constexpr int GRP = NW / MT >= 1 ? NW / MT : 1;
   static constexpr int NPW = (NTL + GRP - 1) / GRP;
   static constexpr int KS = HK / 4;
-  static constexpr int NRH = NR - NRLO;  // Hermite indices of order >= LC
-  // R step: RP aux shells per thread, CPL lanes per Hermite index
-  static constexpr int RP = NC >= 8 ? 2 : 1;
-  static constexpr int CPL = (NC + RP - 1) / RP;
   // Hobson step: one thread per (h, m) and group of aux shells
   static constexpr int MAXT = sph_max_terms(LC);
   static constexpr int CG = NT / (H * NCC) >= 1 ? NT / (H * NCC) : 1;
@@ -128,16 +129,17 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
                                                          int64_t ldo) {
   using K = Cls3m<LA, LB, LC>;
   constexpr int L = K::L, H = K::H, HK = K::HK, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NABP = K::NABP;
-  constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, GWP = K::GWP, LH = K::LH;
+  constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, GWP = K::GWP;
+  constexpr int NRP = K::NRP, NRT = K::NRT;
   constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL;
   __shared__ double G[NC * GWP];       // per aux shell: scaled Boys values, then (2 PC_d)^e
-  __shared__ double R[NC * NR];        // per aux shell: R_tuv
+  __shared__ double R[NC * NRP];       // per aux shell: R^(n)_tuv of the even levels, finally R_tuv
+  __shared__ double Rt[NC * NRT];      // per aux shell: R^(n)_tuv of the odd levels
   __shared__ double W[HK * NCOLP];     // W[h][c NCC + m]
   __shared__ double cxyz[NC * 3];
   __shared__ int cshell[NC], ckeep[NC];
   __shared__ int tuv[NR];
-  __shared__ int rlist[K::NRH];  // (t, u, v) of order LC..L, sorted by (t/2, u/2, v/2)
-  __shared__ double hc[(L + 1) * LH];
+  __shared__ int rec[NR];  // recursion of index h: parents h1, h2, direction, coefficient
   __shared__ int sc[2 * LC + 2];
   __shared__ SphTerm tc[nsph_terms(LC)];
 
@@ -155,26 +157,20 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     herm_tuv(h, t, u, v);
     tuv[h] = t | (u << 8) | (v << 16);
   }
-  if (tid == 0) {
-    int n = 0;
-    for (int a = 0; 2 * a <= L; a++)
-      for (int b = 0; 2 * (a + b) <= L; b++)
-        for (int d = 0; 2 * (a + b + d) <= L; d++)
-          for (int pt = 0; pt < 2; pt++)
-            for (int pu = 0; pu < 2; pu++)
-              for (int pv = 0; pv < 2; pv++) {
-                const int t = 2 * a + pt, u = 2 * b + pu, v = 2 * d + pv;
-                if (t + u + v <= L && t + u + v >= LC) rlist[n++] = t | (u << 8) | (v << 16);
-              }
-  }
-  for (int e = tid; e < (L + 1) * LH; e += NT) {
-    const int t = e / LH, i = e - t * LH;
-    double c = 0.0;
-    if (2 * i <= t) {
-      c = 1.0;
-      for (int z = 0; z < i; z++) c *= (double)((t - 2 * z) * (t - 2 * z - 1)) / (z + 1);
+  // R^(n)_tuv = 2 (t-1) R^(n+1)_{t-2,u,v} + 2 PC_x R^(n+1)_{t-1,u,v} along the first
+  // non-zero index (here in the scale of the G table: R^(n)_000 = g_n)
+  for (int h = 1 + tid; h < NR; h += NT) {
+    int t, u, v;
+    herm_tuv(h, t, u, v);
+    int dir, h1, h2, cf;
+    if (t > 0) {
+      dir = 0, cf = t - 1, h1 = herm_index(t - 1, u, v), h2 = t > 1 ? herm_index(t - 2, u, v) : 0;
+    } else if (u > 0) {
+      dir = 1, cf = u - 1, h1 = herm_index(t, u - 1, v), h2 = u > 1 ? herm_index(t, u - 2, v) : 0;
+    } else {
+      dir = 2, cf = v - 1, h1 = herm_index(t, u, v - 1), h2 = v > 1 ? herm_index(t, u, v - 2) : 0;
     }
-    hc[e] = c;
+    rec[h] = h1 | (h2 << 10) | (dir << 20) | (cf << 22);
   }
   {
     const int x0 = sph->start[LC][0];
@@ -238,46 +234,33 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
           for (int z = 0; z < e; z++) val *= x;
         }
         G[c * GWP + q] = val;
+        if (q == L) (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = val;  // R^(L)_000
       }
       __syncthreads();
-      // R_tuv (closed form, as herm_R_tab) over the Hermite indices in signature
-      // order (t/2, u/2, v/2), aux shells fastest: neighbouring lanes share the
-      // loop trip counts.  Each thread takes RP aux shells of one (t, u, v), so
-      // the chains of LDS reads and FMAs of its RP sums overlap.
-      for (int k = tid; k < K::CPL * K::NRH; k += NT) {
-        const int hh = k / K::CPL, cl = k - hh * K::CPL;  // compile-time divisor
-        if (cl >= nc) continue;
-        const int tv = rlist[hh];
-        const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16, tuvs = t + u + v;
-        const double* gr[K::RP];
-        double acc[K::RP];
+      // levels n = L-1 .. 0: every entry of a level is independent (aux shells
+      // on lanes, odd strides); level n covers the orders t+u+v <= L-n
 #pragma unroll
-        for (int r = 0; r < K::RP; r++) {
-          gr[r] = G + min(cl + r * K::CPL, nc - 1) * GWP;
-          acc[r] = 0.0;
-        }
-        for (int i = 0; 2 * i <= t; i++) {
-          const double ci = hc[t * LH + i];
-          for (int j = 0; 2 * j <= u; j++) {
-            const double cij = ci * hc[u * LH + j];
-            double pxy[K::RP];
-#pragma unroll
-            for (int r = 0; r < K::RP; r++) pxy[r] = cij * gr[r][(L + 1) + t - 2 * i] * gr[r][2 * (L + 1) + u - 2 * j];
-            for (int q = 0; 2 * q <= v; q++) {
-              const double cq = hc[v * LH + q];
-              const int e = tuvs - i - j - q;
-#pragma unroll
-              for (int r = 0; r < K::RP; r++)
-                acc[r] = fma(pxy[r] * cq, gr[r][3 * (L + 1) + v - 2 * q] * gr[r][e], acc[r]);
-            }
+      for (int n = L - 1; n >= 0; n--) {
+        const double* So = (n % 2 == 0) ? Rt : R;  // level n+1
+        double* Sn = (n % 2 == 0) ? R : Rt;
+        const int so = (n % 2 == 0) ? NRT : NRP, sn = (n % 2 == 0) ? NRP : NRT;
+        const int nent = nherm(L - n);
+        for (int k = tid; k < nent * NC; k += NT) {
+          const int h = k / NC, c = k - h * NC;  // compile-time divisor
+          if (c >= nc) continue;
+          const double* gc = G + c * GWP;
+          double val;
+          if (h == 0) {
+            val = gc[n];
+          } else {
+            const int rc = rec[h];
+            const int h1 = rc & 1023, h2 = (rc >> 10) & 1023, dir = (rc >> 20) & 3, cf = rc >> 22;
+            const double* sc0 = So + c * so;
+            val = fma((double)(2 * cf), sc0[h2], gc[(L + 1) * (1 + dir) + 1] * sc0[h1]);
           }
+          Sn[c * sn + h] = val;
         }
-        const int h = herm_index(t, u, v);
-#pragma unroll
-        for (int r = 0; r < K::RP; r++) {
-          const int c = cl + r * K::CPL;
-          if (c < nc) R[c * NR + h] = acc[r];
-        }
+        if (n > 0) __syncthreads();
       }
       __syncthreads();
       // W[h][c NCC + m] = sum_q c_q R_c[(t, u, v) + l_q] over the solid-harmonic terms
@@ -297,7 +280,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
           iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
         }
         for (int c = cg; c < nc; c += K::CG) {
-          const double* Rc = R + c * NR;
+          const double* Rc = R + c * NRP;
           double s = 0.0;
 #pragma unroll
           for (int z = 0; z < K::MAXT; z++) s = fma(cq[z], Rc[iq[z]], s);
