@@ -6,6 +6,11 @@ the 8-fold J assembly) for the 60-atom workload (22 bohr cube, orbital
 3s2p1d, aux 4s3p2d1f, 8 states = 36 densities, FP64), with basis and
 densities resident in HBM when the timed region starts.
 
+`--workload 2` and `--workload 4` run the same step on the f/g-orbital
+60-atom system and on the 200-atom g/h system (QVL screening at 1e-10, 20
+states, densities drawn on the device); `--batch` runs 256 independent
+molecules in one batched call.
+
 Prints one JSON line: value = screened shell-triplets per second of the
 whole step; roofline of the 3c2e kernels (algorithmic FP64 flops from
 pysisyphus_addons_amd.flops over their HIP-event time on the library stream);
@@ -27,7 +32,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "pysisyphus-addons_amd"))
 
 METRIC = "screened 3c2e shell-triplets/s + J-tensor time: mixed vs FP64 GPU at ≤1e-8 Eh"
-WORKLOAD = "60 atoms in a 22 bohr cube, orbital 3s2p1d (3-prim), aux 4s3p2d1f, 8 states, FP64"
+WORKLOADS = {
+    1: "60 atoms in a 22 bohr cube, orbital 3s2p1d (3-prim), aux 4s3p2d1f, 8 states, FP64",
+    2: "60 atoms in a 22 bohr cube, orbital 3s2p1d1f (3-prim), aux 4s3p2d1f1g, 8 states, FP64",
+    4: "200 atoms in a 34 bohr cube, orbital 4s3p2d1f1g (3-prim), aux 5s4p3d2f1g1h, 20 states, QVL 1e-10, FP64",
+}
 FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_SUMMARY = ROOT / "profiles" / "traffic_60atom_fp64.md"
 
@@ -125,6 +134,9 @@ def main():
     ap.add_argument("--fp32-below", type=float, default=None, help="mixed path: FP32 below this Schwarz product")
     ap.add_argument("--batch", action="store_true",
                     help="the batched form instead: 256 molecules of 5-40 atoms, 4 states, one call")
+    ap.add_argument("--workload", type=int, default=1, choices=(1, 2, 4),
+                    help="1: 60 atoms s-d / aux s-f, 8 states (default); 2: 60 atoms s-f / aux s-g, 8 states; "
+                         "4: 200 atoms s-g / aux s-h, 20 states, QVL 1e-10, densities generated on the device")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -140,13 +152,31 @@ def main():
     if args.batch:
         return bench_batch(args)
 
-    shells, aux, dens = synthetic.workload(1)
-    ndens, nbf = dens.shape[0], dens.shape[1]
-    nstates = 8
+    dev = torch.device("cuda:0")
+    base_opt = dict(dens_layout=N.DENS_C, dens_on_device=1)
+    if args.workload == 4:
+        # 210 densities of 7800^2 (102 GB): drawn on the device, (A + A^T)/2 with
+        # A ~ U(0, 1), unit trace
+        shells, aux, _ = synthetic.workload(4, with_densities=False)
+        nstates = synthetic.WORKLOADS[4]["nstates"]
+        nbf, ndens = shells.nbf, nstates * (nstates + 1) // 2
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(synthetic.SEED_BASE + 4)
+        dens_d = torch.empty((ndens, nbf, nbf), dtype=torch.float64, device=dev)
+        for i in range(ndens):
+            a = torch.rand((nbf, nbf), dtype=torch.float64, device=dev, generator=gen)
+            a = 0.5 * (a + a.T)
+            dens_d[i] = a / torch.trace(a)
+            del a
+        dens = None
+        base_opt.update(screening=N.SCREEN_QVL, thresh=1e-10)
+    else:
+        shells, aux, dens = synthetic.workload(args.workload)
+        ndens, nbf = dens.shape[0], dens.shape[1]
+        nstates = synthetic.WORKLOADS[args.workload]["nstates"]
+        dens_d = torch.from_numpy(dens).to(dev)
     b = N.BasisArrays(*shells.as_arrays())
     x = N.BasisArrays(*aux.as_arrays())
-    dev = torch.device("cuda:0")
-    dens_d = torch.from_numpy(dens).to(dev)
     torch.cuda.synchronize()
     J = np.zeros((nstates,) * 4, order="F")
     st = N.Stats()
@@ -170,11 +200,11 @@ def main():
         dt = time.perf_counter() - t0
         return dt / args.steps * 1e3, recs, J.copy()
 
-    ms_per_step, recs, J64 = run(N.default_options(dens_layout=N.DENS_C, dens_on_device=1))
+    ms_per_step, recs, J64 = run(N.default_options(**base_opt))
     print(f"fp64: {ms_per_step:.2f} ms per step", file=sys.stderr, flush=True)
     mixed = None
     if not args.no_mixed:
-        mopt = N.default_options(dens_layout=N.DENS_C, dens_on_device=1, precision=N.MIXED)
+        mopt = N.default_options(precision=N.MIXED, **base_opt)
         if args.fp32_below is not None:
             mopt.fp32_below = args.fp32_below
         ms_mixed, mrecs, Jm = run(mopt)
@@ -209,7 +239,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOAD,
+            "workload": WORKLOADS[args.workload],
             "nbf": nbf,
             "naux": aux.nbf,
             "ndens": ndens,
@@ -235,7 +265,7 @@ def main():
     }
     if mixed is not None:
         out["mixed"] = mixed
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == 1:
         out["cpu_baseline"] = cpu_baseline(shells, aux, dens, args.cpu_step)
     print(json.dumps(out), flush=True)
 

@@ -1129,6 +1129,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (cap == 0) {
     size_t fr = 0, tot = 0;
     HC(hipMemGetInfo(&fr, &tot));
+    // the staging matrix this lane already holds counts as free: a repeated
+    // call gets the same column blocks as the first one
+    fr += cur_lane().wsT.n * sizeof(double);
     cap = std::max<size_t>((size_t)(0.5 * fr), (size_t)1 << 28);
   }
   const int64_t rowbytes = std::max<int64_t>(pl.npf, 1) * (int64_t)sizeof(double);

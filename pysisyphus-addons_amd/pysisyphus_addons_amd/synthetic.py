@@ -90,20 +90,21 @@ def random_densities(rng, nstates, nbf):
     return D
 
 
-def molecule(natoms, side, orb="spd", aux="spdf", nstates=3, alpha0="grid", seed=0):
+def molecule(natoms, side, orb="spd", aux="spdf", nstates=3, alpha0="grid", seed=0, with_densities=True):
     rng = np.random.Generator(np.random.PCG64(seed))
     coords = random_geometry(rng, natoms, side)
     shells = _orbital_shells(rng, coords, orb, alpha0)
     shells_aux = _aux_shells(rng, coords, aux)
-    dens = random_densities(rng, nstates, shells.nbf)
+    dens = random_densities(rng, nstates, shells.nbf) if with_densities else None
     return shells, shells_aux, dens
 
 
-def workload(wid):
-    """(shells, shells_aux, densities) of workload 0, 1, 2 or 4."""
+def workload(wid, with_densities=True):
+    """(shells, shells_aux, densities) of workload 0, 1, 2 or 4; densities None
+    with with_densities=False (workload 4 holds 102 GB of them)."""
     w = WORKLOADS[wid]
     return molecule(w["natoms"], w["side"], w["orb"], w["aux"], w["nstates"], w["alpha0"],
-                    seed=SEED_BASE + wid)
+                    seed=SEED_BASE + wid, with_densities=with_densities)
 
 
 def batch_molecule(i):
