@@ -722,28 +722,6 @@ struct Plan {
   int64_t class_trip[NPCLS * 6] = {}, class_prim[NPCLS * 6] = {};
 };
 
-// Stable LSD radix sort of (key, value) by ascending 64-bit key, 8 bits per
-// pass; passes whose digit is the same for every key are skipped.
-void radix_sort(std::vector<uint64_t>& key, std::vector<int>& val) {
-  const size_t n = key.size();
-  if (n < 2) return;
-  std::vector<uint64_t> k2(n);
-  std::vector<int> v2(n);
-  for (int shift = 0; shift < 64; shift += 8) {
-    size_t cnt[257] = {0};
-    for (size_t i = 0; i < n; i++) cnt[((key[i] >> shift) & 255) + 1]++;
-    if (cnt[((key[0] >> shift) & 255) + 1] == n) continue;
-    for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
-    for (size_t i = 0; i < n; i++) {
-      const size_t pos = cnt[(key[i] >> shift) & 255]++;
-      k2[pos] = key[i];
-      v2[pos] = val[i];
-    }
-    key.swap(k2);
-    val.swap(v2);
-  }
-}
-
 // Sort key of a non-negative finite double for descending order: the IEEE
 // bits of x >= 0 increase with x, so their complement decreases.
 inline uint64_t desc_key(double x) {
@@ -752,41 +730,115 @@ inline uint64_t desc_key(double x) {
   return ~b;
 }
 
+// idx (indices into q) reordered so that q[idx[]] is non-increasing; equal
+// values keep their order (stable).  An LSD radix sort on the upper 32 bits
+// of desc_key (8 bits per pass, the counts of all passes from one read,
+// passes with a constant digit skipped), then runs with equal upper bits
+// are ordered by the full key with a stable insertion sort.  The result is
+// that of a stable sort by the full 64-bit key.
+void sort_desc(const double* q, std::vector<int>& idx) {
+  const size_t n = idx.size();
+  if (n < 2) return;
+  std::vector<uint32_t> key(n), k2(n);
+  std::vector<int> v2(n);
+  uint32_t cnt[4][256] = {};
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t k = (uint32_t)(desc_key(q[idx[i]]) >> 32);
+    key[i] = k;
+    for (int d = 0; d < 4; d++) cnt[d][(k >> (8 * d)) & 255]++;
+  }
+  for (int d = 0; d < 4; d++) {
+    uint32_t* c = cnt[d];
+    const int shift = 8 * d;
+    if (c[(key[0] >> shift) & 255] == n) continue;
+    uint32_t sum = 0;
+    for (int b = 0; b < 256; b++) {
+      const uint32_t t = c[b];
+      c[b] = sum;
+      sum += t;
+    }
+    for (size_t i = 0; i < n; i++) {
+      const uint32_t pos = c[(key[i] >> shift) & 255]++;
+      k2[pos] = key[i];
+      v2[pos] = idx[i];
+    }
+    key.swap(k2);
+    idx.swap(v2);
+  }
+  for (size_t a = 0; a < n;) {
+    size_t b = a + 1;
+    while (b < n && key[b] == key[a]) b++;
+    for (size_t i = a + 1; i < b; i++) {
+      const int v = idx[i];
+      const uint64_t kv = desc_key(q[v]);
+      size_t j = i;
+      while (j > a && desc_key(q[idx[j - 1]]) > kv) {
+        idx[j] = idx[j - 1];
+        j--;
+      }
+      idx[j] = v;
+    }
+    a = b;
+  }
+}
+
+// Runs fn(t) for t = 0..n-1; the tasks with big[t] on separate host threads
+// (each task is independent and writes only its own outputs), the rest on
+// the calling thread.
+template <class F>
+void run_tasks(int n, const std::vector<char>& big, F fn) {
+  std::vector<std::thread> th;
+  for (int t = 0; t < n; t++)
+    if (big[t]) th.emplace_back(fn, t);
+  for (int t = 0; t < n; t++)
+    if (!big[t]) fn(t);
+  for (auto& t : th) t.join();
+}
+
 void build_plan(Plan& pl, const dfhip_options& opt) {
   const HostShells& o = pl.orb;
   const HostShells& x = pl.aux;
+  const int np = pl.ps.n();
+  // pair classes of at least 4096 pairs are sorted, and on such systems the
+  // aux L swept, on their own host threads
+  std::vector<char> big_cls(NPCLS, 0), big_l(LMAX_AUX + 1, 0);
+  int nbig = 0;
+  for (int k = 0; k < NPCLS; k++) nbig += big_cls[k] = pl.ps.cls_end[k] - pl.ps.cls_begin[k] >= 4096;
+  if (nbig > 1) {
+    std::vector<int> nl(LMAX_AUX + 1, 0);
+    for (int i = 0; i < x.n; i++) nl[x.L[i]]++;
+    for (int L = 0; L <= LMAX_AUX; L++) big_l[L] = nl[L] > 0;
+  } else {
+    std::fill(big_cls.begin(), big_cls.end(), 0);
+  }
   // sort pairs inside each class by Q_ab descending (stable on packed index)
   PairSet sorted;
   sorted.cls_begin = pl.ps.cls_begin;
   sorted.cls_end = pl.ps.cls_end;
-  std::vector<double> q2;
-  q2.reserve(pl.ps.n());
-  sorted.s1.reserve(pl.ps.n());
-  sorted.s2.reserve(pl.ps.n());
-  sorted.packed.reserve(pl.ps.n());
-  std::vector<uint64_t> key;
-  std::vector<int> idx;
-  for (int k = 0; k < NPCLS; k++) {
+  std::vector<double> q2(np);
+  sorted.s1.resize(np);
+  sorted.s2.resize(np);
+  sorted.packed.resize(np);
+  run_tasks(NPCLS, big_cls, [&](int k) {
     // descending Q_ab, ties in the original (packed) order
-    key.clear();
-    idx.clear();
-    for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
-      key.push_back(desc_key(pl.qab[p]));
-      idx.push_back(p);
+    const int p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
+    if (p1 == p0) return;
+    std::vector<int> idx(p1 - p0);
+    for (int p = p0; p < p1; p++) idx[p - p0] = p;
+    sort_desc(pl.qab.data(), idx);
+    for (int r = 0; r < p1 - p0; r++) {
+      const int p = idx[r];
+      sorted.s1[p0 + r] = pl.ps.s1[p];
+      sorted.s2[p0 + r] = pl.ps.s2[p];
+      sorted.packed[p0 + r] = pl.ps.packed[p];
+      q2[p0 + r] = pl.qab[p];
     }
-    radix_sort(key, idx);
-    for (const int p : idx) {
-      sorted.s1.push_back(pl.ps.s1[p]);
-      sorted.s2.push_back(pl.ps.s2[p]);
-      sorted.packed.push_back(pl.ps.packed[p]);
-      q2.push_back(pl.qab[p]);
-    }
-  }
+  });
   pl.ps = std::move(sorted);
   pl.qab = std::move(q2);
-  pl.prow.resize(pl.ps.n());
+  pl.prow.resize(np);
   int64_t r = 0;
-  for (int p = 0; p < pl.ps.n(); p++) {
+  for (int p = 0; p < np; p++) {
     pl.prow[p] = r;
     r += (int64_t)(2 * o.L[pl.ps.s1[p]] + 1) * (2 * o.L[pl.ps.s2[p]] + 1);
   }
@@ -799,53 +851,51 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   int64_t col = 0;
   for (int L = 0; L <= LMAX_AUX; L++) {
     auto& v = pl.alist[L];
-    key.clear();
-    for (int sh : v) key.push_back(desc_key(pl.qc[sh]));
-    radix_sort(key, v);
+    sort_desc(pl.qc.data(), v);
     for (int sh : v) {
       pl.acol[L].push_back(col);
       for (int m = 0; m < 2 * L + 1; m++) pl.orig[col + m] = x.f0[sh] + m;
       col += 2 * L + 1;
     }
   }
-  // screened prefix per (L, pair): skip when Q_ab * Q_C < thresh.  Inside a
-  // pair class Q_ab is non-increasing, so the prefix length is too and one
-  // downward sweep per (class, L) finds all of them.
-  pl.nsurv.assign(LMAX_AUX + 1, std::vector<int>(pl.ps.n(), 0));
+  pl.nsurv.assign(LMAX_AUX + 1, std::vector<int>(np, 0));
   pl.ncomputed = 0;
   pl.ntriplets = (int64_t)o.n * (o.n + 1) / 2 * x.n;
   if (opt.screening == DFHIP_SCREEN_QVL) return;  // prefixes and counts from apply_qvl
-  for (int L = 0; L <= LMAX_AUX; L++) {
+  // Per aux L: the screened prefix of every pair (skip when Q_ab * Q_C <
+  // thresh; inside a pair class Q_ab is non-increasing, so the prefix length
+  // is too and one downward sweep per class finds all of them), then the
+  // per-class counts of screened triplets and primitive triplets.
+  int64_t ncomp[LMAX_AUX + 1] = {};
+  run_tasks(LMAX_AUX + 1, big_l, [&](int L) {
     const auto& v = pl.alist[L];
     const int nv = (int)v.size();
+    std::vector<double> qcs(nv);
+    std::vector<int64_t> pref(nv + 1, 0);
+    for (int i = 0; i < nv; i++) {
+      qcs[i] = pl.qc[v[i]];
+      pref[i + 1] = pref[i] + x.nprim[v[i]];
+    }
+    int* ns = pl.nsurv[L].data();
     for (int k = 0; k < NPCLS; k++) {
       int n = nv;
+      int64_t trip = 0, prim = 0;
       for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
         if (opt.screening != DFHIP_SCREEN_NONE) {
           const double qa = pl.qab[p];
           // the product is monotone in Q_C, so the kept set is a prefix
-          while (n > 0 && qa * pl.qc[v[n - 1]] < opt.thresh) n--;
+          while (n > 0 && qa * qcs[n - 1] < opt.thresh) n--;
         }
-        pl.nsurv[L][p] = n;
-        pl.ncomputed += n;
+        ns[p] = n;
+        trip += n;
+        prim += (int64_t)o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]] * pref[n];
       }
+      pl.class_trip[k * 6 + L] += trip;
+      pl.class_prim[k * 6 + L] += prim;
+      ncomp[L] += trip;
     }
-  }
-  // per-class counts of screened triplets and primitive triplets
-  for (int L = 0; L <= LMAX_AUX; L++) {
-    const auto& v = pl.alist[L];
-    std::vector<int64_t> pref(v.size() + 1, 0);
-    for (size_t k = 0; k < v.size(); k++) pref[k + 1] = pref[k] + x.nprim[v[k]];
-    for (int la = 0; la <= LMAX_ORB; la++)
-      for (int lb = 0; lb <= la; lb++) {
-        const int k = pcls(la, lb);
-        for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
-          const int n = pl.nsurv[L][p];
-          pl.class_trip[k * 6 + L] += n;
-          pl.class_prim[k * 6 + L] += (int64_t)o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]] * pref[n];
-        }
-      }
-  }
+  });
+  for (int L = 0; L <= LMAX_AUX; L++) pl.ncomputed += ncomp[L];
 }
 
 // Mixed precision: the FP64 part of each screened prefix, the list entries
@@ -1100,8 +1150,11 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       sj = (int64_t)ndens * nbfs;
     }
     Dp = reserve(cur_lane().wsDp, (size_t)pl.npf * ndens);
-    if (sj == 1 && si == nbfs) {
-      // density index slowest: tiled transpose (pair lookup by shell pair, shell by function)
+    if (sj == 1 && si == nbfs && ndens > 48) {
+      // density index slowest and many densities: tiled transpose (pair lookup
+      // by shell pair, shell by function).  With few densities the direct
+      // packing below reads runs along nu and is faster (0.31 against 0.45 ms
+      // at 36 densities of 840 functions).
       std::vector<int> pairof((size_t)o.n * o.n), fshell(o.nbf);
       for (int p = 0; p < pl.ps.n(); p++) {
         const int a = pl.ps.s1[p], b = pl.ps.s2[p];
