@@ -119,7 +119,10 @@ typedef struct {
     int64_t nskipped;
     int64_t nbf, naux, npair_functions;
     int32_t ncolumn_blocks;
-    /* phase times in ms (HIP events on the library stream) */
+    /* phase times in ms (HIP events on the library stream); the metric
+       Cholesky runs on a second stream during the integrals, ms_cholesky is
+       the part of it still running after the contraction, ms_assemble covers
+       B^T = d^T L^-T, -B^T B and the J assembly */
     double ms_upload, ms_schwarz, ms_metric, ms_cholesky, ms_int3c2e, ms_pack, ms_contract,
         ms_solve, ms_assemble, ms_download, ms_total;
     double ms_h2d_densities; /* part of ms_pack spent copying host densities */

@@ -249,6 +249,16 @@ void build_boys_table(std::vector<double>& tab) {
 // on the same stream after the work of their previous owner.
 struct Lane {
   hipStream_t st = nullptr;
+  // side stream of the lane: the metric factorization runs there while the
+  // integrals run on st (created on first use)
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void ensure_side() {
+    if (st2) return;
+    HC(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    HC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  }
   DevPool pool;
   // device workspace reused across calls (grown on demand, freed by
   // dfhip_release_workspace)
@@ -489,16 +499,30 @@ PairSet make_pairs(const HostShells& o) {
 // Schwarz norms of all pairs, in the PairSet's order.  Classes whose pairs all
 // fit the primitive-pair cache use the cached kernel (one partial per pair),
 // the others the chunked kernel.
-std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps) {
-  hipStream_t st = cur_lane().st;
+// Schwarz norms in two parts, so that the caller can queue other work
+// behind the kernels: schwarz_bra_launch enqueues them on the lane's stream,
+// schwarz_bra_finish downloads their partial sums and reduces them to Q_ab.
+struct SchwarzPending {
   DBuf<int> s1, s2;
+  DBuf<double> part, scr;
+  std::vector<size_t> poff;
+  std::vector<int> nch;
+};
+
+void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps, SchwarzPending& sp) {
+  hipStream_t st = cur_lane().st;
+  DBuf<int>& s1 = sp.s1;
+  DBuf<int>& s2 = sp.s2;
   upload(s1, ps.s1, st);
   upload(s2, ps.s2, st);
-  std::vector<size_t> poff(NPCLS + 1, 0);
+  std::vector<size_t>& poff = sp.poff;
+  poff.assign(NPCLS + 1, 0);
   // kernel per class: 3 = matrix cores with the tables in LDS, 2 = matrix
   // cores streaming the tables from global scratch, 1 = cached VALU,
   // 0 = chunked VALU (DFHIP_SCHWARZ_KERNEL caps the choice, for tests)
-  std::vector<int> nch(NPCLS, 1), kind(NPCLS, 0);
+  std::vector<int>& nch = sp.nch;
+  nch.assign(NPCLS, 1);
+  std::vector<int> kind(NPCLS, 0);
   const char* cap_env = std::getenv("DFHIP_SCHWARZ_KERNEL");
   const int cap = (cap_env && *cap_env) ? std::atoi(cap_env) : 3;
   size_t scr_need = 0;
@@ -521,9 +545,11 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
       poff[k + 1] = (size_t)(ps.cls_end[k] - ps.cls_begin[k]) * nch[k];
     }
   for (int k = 0; k < NPCLS; k++) poff[k + 1] += poff[k];
-  DBuf<double> part(std::max<size_t>(poff[NPCLS], 1));
+  DBuf<double>& part = sp.part;
+  part.alloc(std::max<size_t>(poff[NPCLS], 1));
   // scratch of the streamed kernels: all pairs of the largest class, at most 1 GiB
-  DBuf<double> scr(std::min<size_t>(scr_need, (size_t)1 << 27));
+  DBuf<double>& scr = sp.scr;
+  scr.alloc(std::min<size_t>(scr_need, (size_t)1 << 27));
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
@@ -538,8 +564,14 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
         c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       HC(hipGetLastError());
     }
+}
+
+std::vector<double> schwarz_bra_finish(const PairSet& ps, SchwarzPending& sp) {
+  hipStream_t st = cur_lane().st;
+  const std::vector<size_t>& poff = sp.poff;
+  const std::vector<int>& nch = sp.nch;
   std::vector<double> hp(poff[NPCLS]);
-  if (!hp.empty()) HC(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (!hp.empty()) HC(hipMemcpyAsync(hp.data(), sp.part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HC(hipStreamSynchronize(st));
   std::vector<double> q(ps.n());
   for (int k = 0; k < NPCLS; k++)
@@ -550,6 +582,12 @@ std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells
       if (!std::isfinite(q[p])) throw Error{DFHIP_ERR_HIP, "non-finite Schwarz norm"};
     }
   return q;
+}
+
+std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps) {
+  SchwarzPending sp;
+  schwarz_bra_launch(c, o, dev, ps, sp);
+  return schwarz_bra_finish(ps, sp);
 }
 
 bool use_matrix_cores_3c() {
@@ -1053,28 +1091,59 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   daux.upload_from(x, st);
   const int e1 = tm.mark();
 
-  // Schwarz norms of the shell pairs (common/integrals/intor.f90:131-160)
+  // Schwarz norms of the shell pairs (common/integrals/intor.f90:131-160);
+  // the metric kernels queue behind them, their results are read after both
   pl.ps = make_pairs(o);
-  pl.qab = schwarz_bra_dev(c, o, dorb.view, pl.ps);
+  SchwarzPending spend;
+  schwarz_bra_launch(c, o, dorb.view, pl.ps, spend);
   const int e2 = tm.mark();
-  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
 
   // metric (common/integrals/intor.f90:40-95) in the leading block of the
   // bordered matrix [[M, .], [d^T, 0]] (column-major, lda = naux + ndens),
   // and Q_C from its diagonal blocks
-  const int64_t ntot = naux + (need_dens ? ndens : 0), lda = ntot;
-  double* Ap = reserve(cur_lane().wsA, (size_t)ntot * ntot);
-  HC(hipMemsetAsync(Ap, 0, (size_t)ntot * ntot * sizeof(double), st));
+  // A = [[M], [I], [D^T]]: naux columns, rows 0..naux-1 the metric, then the
+  // identity border whose factorization gives L^-T, then d^T (ndens rows)
+  const int64_t lda = need_dens ? 2 * naux + ndens : naux;
+  double* Ap = reserve(cur_lane().wsA, (size_t)lda * naux);
+  HC(hipMemsetAsync(Ap, 0, (size_t)lda * naux * sizeof(double), st));
   metric_dev(c, x, daux.view, Ap, lda);
   DBuf<double> dqc(x.n);
   schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
+  const int e3 = tm.mark();
   pl.qc.resize(x.n);
   HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
-  HC(hipStreamSynchronize(st));
+  pl.qab = schwarz_bra_finish(pl.ps, spend);
   for (const double q : pl.qc)
     if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
-  const int e3 = tm.mark();
+  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
   vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
+
+  // Cholesky of the metric bordered by the identity (common/linalg.f90:78 and
+  // the dtrtri of diabatization/intor.f90:198) on the lane's side stream,
+  // concurrent with the integrals: it touches only rows < 2 naux, the
+  // contraction writes d^T below them.  Ap and info outlive the join.
+  DBuf<int> info(1);
+  Lane& lane = cur_lane();
+  // on every way out of run() (errors included) the side stream is drained
+  // before Ap, info or the pool blocks can be reused
+  struct SideDrain {
+    Lane& l;
+    bool on = false;
+    ~SideDrain() {
+      if (on) (void)hipStreamSynchronize(l.st2);
+    }
+  } drain{lane};
+  if (need_dens) {
+    lane.ensure_side();
+    drain.on = true;
+    set_identity_rows(Ap, lda, naux, st);
+    HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
+    HC(hipEventRecord(lane.ev_fork, st));
+    HC(hipStreamWaitEvent(lane.st2, lane.ev_fork, 0));
+    cholesky_inverse_rows(Ap, lda, naux, info.p, lane.st2);
+    HC(hipGetLastError());
+    HC(hipEventRecord(lane.ev_join, lane.st2));
+  }
   const int e4 = e3;
 
   const auto hp0 = std::chrono::steady_clock::now();
@@ -1390,7 +1459,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       ScreenMask mb = mk;
       mb.colL = dcolL.p + cA;
       mb.colia = dcolia.p + cA;
-      contract_T(Tp, w, Dp, ndens, (int)w, ndens, NPCLS, dseg.p, mb, partp, nsplit, dorig.p + cA, Ap + naux, lda, 1,
+      contract_T(Tp, w, Dp, ndens, (int)w, ndens, NPCLS, dseg.p, mb, partp, nsplit, dorig.p + cA, Ap + 2 * naux, lda, 1,
                  st);
       HC(hipGetLastError());
     }
@@ -1423,15 +1492,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   const int e6 = tm.mark();
 
-  // Cholesky of the metric bordered by d: L, B^T = (L^-1 d)^T and -B^T B
-  // (common/linalg.f90:78, diabatization/intor.f90:198, :241-269)
+  // B^T = d^T L^-T (the dtrmm of diabatization/intor.f90:198) and the packed
+  // Coulomb tensor -B^T B (:241-269) once the factorization has joined
   int e7 = e6, e8 = e6;
-  DBuf<double> dB, dJ;
+  DBuf<double> dB, dJ, dBT, dG, dGp;
   if (need_dens) {
-    DBuf<int> info(1);
-    HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
-    cholesky_bordered(Ap, lda, naux, ntot, info.p, st);
-    HC(hipGetLastError());
+    HC(hipStreamWaitEvent(st, lane.ev_join, 0));
     e7 = tm.mark();
     HC(hipStreamSynchronize(st));
     int hinfo = 0;
@@ -1439,16 +1505,23 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     if (hinfo != 0)
       throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
                                          std::to_string(hinfo) + ")"};
+    dBT.alloc((size_t)naux * ndens);
+    DBuf<double> dLp((size_t)apply_linv_part_size(naux, ndens));
+    apply_linv(Ap + 2 * naux, lda, Ap + naux, lda, naux, ndens, dLp.p, dBT.p, st);
+    HC(hipGetLastError());
     if (out.df) {
       dB.alloc((size_t)naux * ndens);
-      border_to_b(Ap, lda, naux, ndens, dB.p, st);
+      bt_to_b(dBT.p, naux, ndens, dB.p, st);
       HC(hipGetLastError());
     }
     if (out.J) {
+      dGp.alloc((size_t)gram_chunks(naux) * ndens * ndens);
+      dG.alloc((size_t)ndens * ndens);
+      gram_neg(dBT.p, naux, ndens, dGp.p, dG.p, st);
       const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;
       dJ.alloc(nj);
       HC(hipMemsetAsync(dJ.p, 0, nj * sizeof(double), st));
-      assemble_j(Ap, lda, naux, ndens, out.nstates, dJ.p, st);
+      assemble_j(dG.p, ndens, 0, ndens, out.nstates, dJ.p, st);
       HC(hipGetLastError());
     }
     e8 = tm.mark();

@@ -231,6 +231,7 @@ int contract_tile() { return GT; }
 // compute) and the trailing ndens x ndens block holds -B^T B, the packed
 // Coulomb tensor of diabatization/intor.f90:241-269 with a minus sign.
 constexpr int CB = 64;
+static_assert(CB == 64, "the staging loops of the Cholesky kernels index with shifts by 6 and 7");
 
 // One block column of the Cholesky factorization: the 64x64 diagonal block
 // A11 at k0 and 64 rows of the panel A21 below it, as the 128x64 stack
@@ -250,31 +251,28 @@ __global__ __launch_bounds__(512) void chol_panel_kernel(double* __restrict__ A,
                                                          int nb, int* __restrict__ info) {
   __shared__ double Lb[16];                     // factored diagonal tile: L (lower) and 1/L[c][c] on the diagonal
   __shared__ __align__(16) double P[128][4];    // solved rows of the current column block
+  __shared__ double S[128][CB + 1];             // [A11; A21] staged for column-wise (coalesced) loads and stores
   const int tid = threadIdx.x, ti = tid >> 4, tk = tid & 15;
   const int64_t r0 = k0 + nb + (int64_t)blockIdx.x * CB;  // first panel row of this workgroup
   const int nr = (int)max((int64_t)0, min((int64_t)CB, ntot - r0));
   const bool diag = ti < 16;
   const bool active = diag ? ti >= tk : true;
-  // global row of tile row x (clamped for the loads)
+  // lanes along the rows of a column: stage row r (< 64: A11, else A21), column c
+  for (int e = tid; e < 128 * CB; e += 512) {
+    const int r = e & 127, c = e >> 7;
+    const int64_t row = r < CB ? k0 + min(r, nb - 1) : (nr > 0 ? r0 + min(r - CB, nr - 1) : k0);
+    S[r][c] = A[row + lda * (k0 + min(c, nb - 1))];
+  }
+  __syncthreads();
   double a[4][4];
 #pragma unroll
   for (int x = 0; x < 4; x++)
 #pragma unroll
     for (int y = 0; y < 4; y++) {
       const int c = 4 * tk + y;
-      int64_t row;
-      bool ok;
-      if (diag) {
-        const int r = 4 * ti + x;
-        row = k0 + min(r, nb - 1);
-        ok = r < nb && c < nb;
-      } else {
-        const int r = 4 * (ti - 16) + x;
-        row = nr > 0 ? r0 + min(r, nr - 1) : k0;
-        ok = r < nr && c < nb;
-      }
-      a[x][y] = A[row + lda * (k0 + min(c, nb - 1))];
-      if (!ok) a[x][y] = (diag && 4 * ti + x == c) ? 1.0 : 0.0;
+      const int r = diag ? 4 * ti + x : 4 * (ti - 16) + x;
+      const bool ok = c < nb && (diag ? r < nb : r < nr);
+      a[x][y] = ok ? S[diag ? r : CB + r][c] : ((diag && r == c) ? 1.0 : 0.0);
     }
 
   for (int jb = 0; jb < 16; jb++) {
@@ -346,32 +344,33 @@ __global__ __launch_bounds__(512) void chol_panel_kernel(double* __restrict__ A,
         }
     }
   }
-  // store the panel rows L21
+  // store the panel rows L21, through S for column-wise stores
   if (!diag) {
 #pragma unroll
     for (int x = 0; x < 4; x++)
 #pragma unroll
-      for (int y = 0; y < 4; y++) {
-        const int c = 4 * tk + y, r = 4 * (ti - 16) + x;
-        if (c < nb && r < nr) A[(r0 + r) + lda * (k0 + c)] = a[x][y];
-      }
+      for (int y = 0; y < 4; y++) S[CB + 4 * (ti - 16) + x][4 * tk + y] = a[x][y];
+  }
+  __syncthreads();
+  for (int e = tid; e < CB * CB; e += 512) {
+    const int r = e & (CB - 1), c = e >> 6;
+    if (c < nb && r < nr) A[(r0 + r) + lda * (k0 + c)] = S[CB + r][c];
   }
 }
 
-// A22 -= L21 L21^T on the lower 64x64 tiles of the trailing matrix
-// (rows/cols k0+nb .. ntot-1).
-__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int64_t lda, int64_t ntot, int k0,
-                                                          int nb) {
+// A22 -= L21 L21^T on the lower part of the trailing matrix: rows
+// k0+nb .. nrow_end-1, columns k0+nb .. ncol_end-1, one 64x64 tile per
+// workgroup (grid: row tiles x column tiles; tiles above the diagonal exit).
+__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int64_t lda, int64_t nrow_end,
+                                                          int k0, int nb, int64_t ncol_end) {
   __shared__ double Pr[CB][CB + 1];
   __shared__ double Pc[CB][CB + 1];
   const int64_t m0 = k0 + nb;
-  const int q = blockIdx.x;
-  int tr = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-  while ((tr + 1) * (tr + 2) / 2 <= q) tr++;
-  while (tr * (tr + 1) / 2 > q) tr--;
-  const int tc = q - tr * (tr + 1) / 2;
+  const int tr = blockIdx.x, tc = blockIdx.y;
+  if (tc > tr) return;
+  const int64_t ntot = nrow_end;
   const int64_t r0 = m0 + (int64_t)tr * CB, c0 = m0 + (int64_t)tc * CB;
-  const int nr = (int)min((int64_t)CB, ntot - r0), ncl = (int)min((int64_t)CB, ntot - c0);
+  const int nr = (int)min((int64_t)CB, ntot - r0), ncl = (int)min((int64_t)CB, ncol_end - c0);
   const int tid = threadIdx.x;
   {
     double vr[CB * CB / 256], vc[CB * CB / 256];
@@ -404,21 +403,25 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
 #pragma unroll
       for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
   }
-  double old[4][4];
+  // A22 -= acc through LDS, so that the read-modify-write runs along columns
+  __syncthreads();
 #pragma unroll
   for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int y = 0; y < 4; y++) {
-      const int64_t r = min(r0 + ty * 4 + x, ntot - 1), c = min(c0 + tx * 4 + y, ntot - 1);
-      old[x][y] = A[r + lda * c];
-    }
-#pragma unroll
-  for (int x = 0; x < 4; x++)
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-      const int64_t r = r0 + ty * 4 + x, c = c0 + tx * 4 + y;
-      if (r < ntot && c < ntot && r >= c) A[r + lda * c] = old[x][y] - acc[x][y];
-    }
+    for (int y = 0; y < 4; y++) Pr[ty * 4 + x][tx * 4 + y] = acc[x][y];
+  __syncthreads();
+  for (int e = tid; e < CB * CB; e += 256) {
+    const int rr = e & (CB - 1), cc = e >> 6;
+    const int64_t r = r0 + rr, c = c0 + cc;
+    if (r < ntot && c < ncol_end && r >= c) A[r + lda * c] -= Pr[rr][cc];
+  }
+}
+
+void chol_update(double* A, int64_t lda, int64_t nrow_end, int k0, int nb, int64_t ncol_end, hipStream_t st) {
+  const int64_t m0 = k0 + nb;
+  if (nrow_end <= m0 || ncol_end <= m0) return;
+  const unsigned ntr = (unsigned)((nrow_end - m0 + CB - 1) / CB), ntc = (unsigned)((ncol_end - m0 + CB - 1) / CB);
+  hipLaunchKernelGGL(chol_update_kernel, dim3(ntr, ntc), dim3(256), 0, st, A, lda, nrow_end, k0, nb, ncol_end);
 }
 
 void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, int* info, hipStream_t st) {
@@ -427,12 +430,190 @@ void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, int* inf
     const int64_t m = ntot - k0 - nb;
     const unsigned npanel = (unsigned)std::max<int64_t>(1, (m + CB - 1) / CB);
     hipLaunchKernelGGL(chol_panel_kernel, dim3(npanel), dim3(512), 0, st, A, lda, ntot, k0, nb, info);
-    if (m > 0) {
-      const int64_t t = (m + CB - 1) / CB;
-      hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(t * (t + 1) / 2)), dim3(256), 0, st, A, lda, ntot, k0,
-                         nb);
-    }
+    chol_update(A, lda, ntot, k0, nb, ntot, st);
   }
+}
+
+// Cholesky of M bordered by the identity, A = [[M], [I]] (column-major, n
+// columns, rows 0..2n-1 of lda): afterwards the lower triangle of the leading
+// block holds L (M = L L^T; the diagonal blocks keep their last update) and
+// the border rows hold L^-T: A[n + i + lda j] = (L^-1)[j][i] for j >= i.
+// Identity row i is still untouched while the factorization is left of column
+// i (its entries there are zero), so step k0 takes the border rows only up to
+// n + k0 + nb.
+void cholesky_inverse_rows(double* A, int64_t lda, int64_t n, int* info, hipStream_t st) {
+  for (int k0 = 0; k0 < n; k0 += CB) {
+    const int nb = (int)std::min<int64_t>(CB, n - k0);
+    const int64_t rend = n + k0 + nb;
+    const int64_t m = rend - k0 - nb;
+    const unsigned npanel = (unsigned)std::max<int64_t>(1, (m + CB - 1) / CB);
+    hipLaunchKernelGGL(chol_panel_kernel, dim3(npanel), dim3(512), 0, st, A, lda, rend, k0, nb, info);
+    chol_update(A, lda, rend, k0, nb, n, st);
+  }
+}
+
+// A[n + i + lda i] = 1 for i < n (the identity border of cholesky_inverse_rows).
+__global__ void set_identity_rows_kernel(double* __restrict__ A, int64_t lda, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[n + i + lda * i] = 1.0;
+}
+
+void set_identity_rows(double* A, int64_t lda, int64_t n, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(set_identity_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, lda, n);
+}
+
+// BT[I + ndens j] = sum_{i <= j} DT[I + ldd i] LT[i + ldl j]: the rows of
+// B^T = D^T L^-T with D^T (ndens x n, column-major, ldd) and the upper
+// triangular L^-T (ldl).  64x64 output tiles (64 densities x 64 columns j),
+// 4x4 per thread, K split into chunks of kc (grid z); each chunk writes its
+// partial tile, a second kernel adds the chunks in a fixed order.
+__global__ __launch_bounds__(256) void apply_linv_kernel(const double* __restrict__ DT, int64_t ldd,
+                                                         const double* __restrict__ LT, int64_t ldl, int64_t n,
+                                                         int ndens, int64_t kc, double* __restrict__ part) {
+  __shared__ double sD[16][64 + 1];  // [i][I]
+  __shared__ double sL[16][64 + 1];  // [i][j]
+  const int64_t j0 = (int64_t)blockIdx.x * 64;
+  const int I0 = blockIdx.y * 64;
+  const int64_t kb = (int64_t)blockIdx.z * kc, ke = min(min(n, j0 + 64), kb + kc);
+  if (kb >= ke) return;  // past the triangle of this column tile: the reduction skips the chunk
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;  // I = I0 + 4 ty + x, j = j0 + 4 tx + y
+  double acc[4][4] = {};
+  for (int64_t k0 = kb; k0 < ke; k0 += 16) {
+    for (int e = tid; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, c = e & 63;
+      const int64_t k = k0 + kk;
+      sD[kk][c] = (k < ke && I0 + c < ndens) ? DT[(I0 + c) + ldd * k] : 0.0;
+      sL[kk][c] = (k < ke && j0 + c < n && k <= j0 + c) ? LT[k + ldl * (j0 + c)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        a[q] = sD[kk][4 * ty + q];
+        b[q] = sL[kk][4 * tx + q];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+  double* pz = part + (size_t)blockIdx.z * ndens * n;
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int I = I0 + 4 * ty + x;
+      const int64_t j = j0 + 4 * tx + y;
+      if (I < ndens && j < n) pz[I + (int64_t)ndens * j] = acc[x][y];
+    }
+}
+
+__global__ void apply_linv_reduce_kernel(const double* __restrict__ part, int64_t n, int ndens, int64_t kc,
+                                         double* __restrict__ BT) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ndens) return;
+  const int64_t j = e / ndens, kend = min(n, (j / 64) * 64 + 64);
+  double s = 0.0;
+  for (int64_t z = 0; z * kc < kend; z++) s += part[(size_t)z * ndens * n + e];
+  BT[e] = s;
+}
+
+int64_t apply_linv_chunk(int64_t n) { return std::max<int64_t>(256, (n + 15) / 16); }
+int64_t apply_linv_part_size(int64_t n, int ndens) {
+  const int64_t kc = apply_linv_chunk(n);
+  return ((n + kc - 1) / kc) * ndens * n;
+}
+
+void apply_linv(const double* DT, int64_t ldd, const double* LT, int64_t ldl, int64_t n, int ndens, double* part,
+                double* BT, hipStream_t st) {
+  if (n <= 0 || ndens <= 0) return;
+  const int64_t kc = apply_linv_chunk(n), nkc = (n + kc - 1) / kc;
+  hipLaunchKernelGGL(apply_linv_kernel, dim3((unsigned)((n + 63) / 64), (ndens + 63) / 64, (unsigned)nkc), dim3(256),
+                     0, st, DT, ldd, LT, ldl, n, ndens, kc, part);
+  const int64_t tot = n * ndens;
+  hipLaunchKernelGGL(apply_linv_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, n, ndens,
+                     kc, BT);
+}
+
+// G[I + ndens K] = -sum_j BT[I + ndens j] BT[K + ndens j] (the packed Coulomb
+// tensor with a minus sign, as the trailing block of the bordered
+// factorization): partial sums over chunks of 512 columns j, then a reduction
+// over the chunks in a fixed order.
+constexpr int GJ = 512;
+__global__ __launch_bounds__(256) void gram_part_kernel(const double* __restrict__ BT, int64_t n, int ndens,
+                                                        double* __restrict__ part) {
+  __shared__ double sA[16][64 + 1], sB[16][64 + 1];
+  const int I0 = blockIdx.x * 64, K0 = blockIdx.y * 64;
+  const int64_t jc = (int64_t)blockIdx.z * GJ, jend = min(n, jc + GJ);
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  double acc[4][4] = {};
+  for (int64_t k0 = jc; k0 < jend; k0 += 16) {
+    for (int e = tid; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, c = e & 63;
+      const int64_t k = k0 + kk;
+      sA[kk][c] = (k < jend && I0 + c < ndens) ? BT[(I0 + c) + (int64_t)ndens * k] : 0.0;
+      sB[kk][c] = (k < jend && K0 + c < ndens) ? BT[(K0 + c) + (int64_t)ndens * k] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        a[q] = sA[kk][4 * ty + q];
+        b[q] = sB[kk][4 * tx + q];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int I = I0 + 4 * ty + x, K = K0 + 4 * tx + y;
+      if (I < ndens && K < ndens) part[((size_t)blockIdx.z * ndens + K) * ndens + I] = acc[x][y];
+    }
+}
+
+__global__ void gram_reduce_kernel(const double* __restrict__ part, int nchunk, int ndens, double* __restrict__ G) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ndens * ndens) return;
+  double s = 0.0;
+  for (int z = 0; z < nchunk; z++) s += part[(size_t)z * ndens * ndens + e];
+  G[e] = -s;
+}
+
+int gram_chunks(int64_t n) { return (int)std::max<int64_t>(1, (n + GJ - 1) / GJ); }
+
+void gram_neg(const double* BT, int64_t n, int ndens, double* part, double* G, hipStream_t st) {
+  if (ndens <= 0) return;
+  const int nchunk = gram_chunks(n);
+  const unsigned t = (unsigned)((ndens + 63) / 64);
+  hipLaunchKernelGGL(gram_part_kernel, dim3(t, t, nchunk), dim3(256), 0, st, BT, n, ndens, part);
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((ndens * ndens + 255) / 256), dim3(256), 0, st, part, nchunk, ndens, G);
+}
+
+// B[P + n I] = BT[I + ndens P]
+__global__ void bt_to_b_kernel(const double* __restrict__ BT, int64_t n, int ndens, double* __restrict__ B) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ndens) return;
+  const int64_t P = e % n, I = e / n;
+  B[P + n * I] = BT[I + ndens * P];
+}
+
+void bt_to_b(const double* BT, int64_t n, int ndens, double* B, hipStream_t st) {
+  const int64_t tot = n * ndens;
+  if (tot > 0)
+    hipLaunchKernelGGL(bt_to_b_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, BT, n, ndens, B);
 }
 
 // df[P + n I] = A[(n + I) + lda P]   (B from the border rows)
@@ -468,7 +649,7 @@ __global__ void assemble_j_kernel(const double* __restrict__ A, int64_t lda, int
   if (q >= npairs) return;
   int I1, I2;
   unpack_pair(q, I1, I2);
-  const double v = -A[(n + I1) + lda * (n + I2)];
+  const double v = -A[(n + I1) + lda * (n + I2)];  // G(I1, I2) at row n + I1, column n + I2
   int i, j, k, l;
   unpack_pair(I1, i, j);
   unpack_pair(I2, k, l);
