@@ -33,7 +33,6 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
                 int64_t sP, int64_t sI, hipStream_t st);
 int contract_tile();
-void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, int* info, hipStream_t st);
 // Cholesky of [[M], [I]] (n columns, 2n rows, lda): L in the leading block,
 // L^-T in the border rows n..2n-1.
 void cholesky_inverse_rows(double* A, int64_t lda, int64_t n, int* info, hipStream_t st);
@@ -47,7 +46,6 @@ void apply_linv(const double* DT, int64_t ldd, const double* LT, int64_t ldl, in
 int gram_chunks(int64_t n);
 void gram_neg(const double* BT, int64_t n, int ndens, double* part, double* G, hipStream_t st);
 void bt_to_b(const double* BT, int64_t n, int ndens, double* B, hipStream_t st);
-void border_to_b(const double* A, int64_t lda, int64_t n, int ndens, double* B, hipStream_t st);
 void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st);
 void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,
                              hipStream_t st);

@@ -221,15 +221,14 @@ int contract_tile() { return GT; }
 
 
 // ---------------------------------------------------------------------------
-// Blocked right-looking Cholesky of the bordered matrix
-//     [ M   . ]      M = (P|Q), n x n
-//     [ d^T 0 ]      d = (ab|P) D, naux x ndens
-// stored lower-column-major in A (ntot = n + ndens rows/cols, lda).  Only the
-// first n columns are factored.  Afterwards the leading block holds L
-// (M = L L^T, dpotrf('L') of common/linalg.f90:78), the border rows hold
-// B^T = (L^-1 d)^T (what dtrtri + dtrmm of diabatization/intor.f90:198
-// compute) and the trailing ndens x ndens block holds -B^T B, the packed
-// Coulomb tensor of diabatization/intor.f90:241-269 with a minus sign.
+// Blocked right-looking Cholesky of a matrix bordered by extra rows,
+//     [ M ]      M = (P|Q), n x n
+//     [ C ]      C: rows below M
+// stored lower-column-major in A (lda).  Only the first n columns are
+// factored: afterwards the leading block holds L (M = L L^T, dpotrf('L') of
+// common/linalg.f90:78) and the border rows hold C L^-T.  With C = I they
+// hold L^-T, the transpose of the inverse that the reference forms with
+// dtrtri (diabatization/intor.f90:198).
 constexpr int CB = 64;
 static_assert(CB == 64, "the staging loops of the Cholesky kernels index with shifts by 6 and 7");
 
@@ -424,16 +423,6 @@ void chol_update(double* A, int64_t lda, int64_t nrow_end, int k0, int nb, int64
   hipLaunchKernelGGL(chol_update_kernel, dim3(ntr, ntc), dim3(256), 0, st, A, lda, nrow_end, k0, nb, ncol_end);
 }
 
-void cholesky_bordered(double* A, int64_t lda, int64_t n, int64_t ntot, int* info, hipStream_t st) {
-  for (int k0 = 0; k0 < n; k0 += CB) {
-    const int nb = (int)std::min<int64_t>(CB, n - k0);
-    const int64_t m = ntot - k0 - nb;
-    const unsigned npanel = (unsigned)std::max<int64_t>(1, (m + CB - 1) / CB);
-    hipLaunchKernelGGL(chol_panel_kernel, dim3(npanel), dim3(512), 0, st, A, lda, ntot, k0, nb, info);
-    chol_update(A, lda, ntot, k0, nb, ntot, st);
-  }
-}
-
 // Cholesky of M bordered by the identity, A = [[M], [I]] (column-major, n
 // columns, rows 0..2n-1 of lda): afterwards the lower triangle of the leading
 // block holds L (M = L L^T; the diagonal blocks keep their last update) and
@@ -614,21 +603,6 @@ void bt_to_b(const double* BT, int64_t n, int ndens, double* B, hipStream_t st) 
   const int64_t tot = n * ndens;
   if (tot > 0)
     hipLaunchKernelGGL(bt_to_b_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, BT, n, ndens, B);
-}
-
-// df[P + n I] = A[(n + I) + lda P]   (B from the border rows)
-__global__ void border_to_b_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int ndens,
-                                   double* __restrict__ B) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * ndens) return;
-  const int64_t P = e % n, I = e / n;
-  B[P + n * I] = A[(n + I) + lda * P];
-}
-
-void border_to_b(const double* A, int64_t lda, int64_t n, int ndens, double* B, hipStream_t st) {
-  const int64_t tot = n * ndens;
-  if (tot > 0)
-    hipLaunchKernelGGL(border_to_b_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, lda, n, ndens, B);
 }
 
 // ---------------------------------------------------------------------------
