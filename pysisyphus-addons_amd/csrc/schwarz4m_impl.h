@@ -31,22 +31,29 @@ template <int LA, int LB>
 struct Cls4m {
   static constexpr int LAB = LA + LB, L = 2 * LAB;
   static constexpr int NAB = nsph(LA) * nsph(LB), NB = nsph(LB);
-  static constexpr int H = nherm(LAB), NR = nherm(L), NR1 = NR + 1;  // R[NR] = 0 pads h2 >= H
+  static constexpr int H = nherm(LAB), NR = nherm(L);
+  // R stride per quartet: R[NR] = 0 pads h2 >= H; odd, so that lanes on
+  // consecutive quartets hit distinct banks
+  static constexpr int NR1 = (NR + 1) | 1;
+  // odd levels of the R recursion (orders <= L-1) live in F's space before F is built
+  static constexpr int NRT = nherm(L - 1) | 1;
   static constexpr int HK = (H + 3) / 4 * 4, NS = HK / 4;              // K steps per ket primitive pair
   static constexpr int ESZ = (LA + 1) * (LB + 1) * (LAB + 1);
   static constexpr int NCT = (NAB + 15) / 16, ZL = 16 * NCT;
   static constexpr int NPPM = 9, NQM = NPPM * (NPPM + 1) / 2;
-  static constexpr int NRT = (NPPM * H + 15) / 16;
-  static constexpr int NW = NRT >= 16 ? 8 : (NRT >= 8 ? 4 : (NRT >= 4 ? 2 : 1));
+  static constexpr int NRTL = (NPPM * H + 15) / 16;  // row tiles of V
+  static constexpr int NW = NRTL >= 16 ? 8 : (NRTL >= 8 ? 4 : (NRTL >= 4 ? 2 : 1));
   static constexpr int NT = 64 * NW;
   static constexpr int LH = L / 2 + 1;
-  // F rows read: < 16 NRT + HK, columns < ZL past the row start
-  static constexpr int FSZ = (16 * NRT + HK) * NAB + ZL;
+  // F rows read: < 16 NRTL + HK, columns < ZL past the row start
+  static constexpr int FSZ0 = (16 * NRTL + HK) * NAB + ZL;
+  static constexpr int FSZ = FSZ0 > NQM * NRT ? FSZ0 : NQM * NRT;
   static constexpr int RSZ = NQM * NR1;
-  static constexpr int GW = 4 * (L + 1);  // scaled Boys values and (2 PQ_d)^e per quartet
+  // scaled Boys values and (2 PQ_d)^e per quartet; odd stride
+  static constexpr int GW = 4 * (L + 1) + 1;
   static constexpr int SCR0 = NPPM * 3 * ESZ + NQM * GW;
   static constexpr int SCR = SCR0 > ZL * ZL ? SCR0 : ZL * ZL;
-  static constexpr int LDS = FSZ + RSZ + SCR + (L + 1) * LH;
+  static constexpr int LDS = FSZ + RSZ + SCR + NR;  // + tuv and rec
   static constexpr bool OK = LDS <= 20000;
 };
 
@@ -90,12 +97,12 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
                                                                       double* __restrict__ part) {
   using K = Cls4m<LA, LB>;
   constexpr int L = K::L, H = K::H, NR = K::NR, NR1 = K::NR1, NAB = K::NAB, NB = K::NB, NT = K::NT, NW = K::NW;
-  constexpr int NCT = K::NCT, ZL = K::ZL, NS = K::NS, GW = K::GW, ESZ = K::ESZ, LH = K::LH;
+  constexpr int NCT = K::NCT, ZL = K::ZL, NS = K::NS, GW = K::GW, ESZ = K::ESZ;
   __shared__ double F[K::FSZ];
   __shared__ double R[K::RSZ];
   __shared__ double scr[K::SCR];
-  __shared__ double hc[(L + 1) * LH];
   __shared__ int tuv[NR];
+  __shared__ int rec[NR];  // recursion of index h: parents h1, h2, direction, coefficient
   __shared__ double red[NW];
   __shared__ int sa[2 * LA + 2], sb[2 * LB + 2];
   __shared__ SphTerm ta[nsph(LA) * ncart(LA)], tb[nsph(LB) * ncart(LB)];
@@ -120,14 +127,20 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
     for (int e = tid; e < na; e += NT) ta[e] = sph->terms[a0 + e];
     for (int e = tid; e < nb; e += NT) tb[e] = sph->terms[b0 + e];
   }
-  for (int e = tid; e < (L + 1) * LH; e += NT) {
-    const int t = e / LH, i = e - t * LH;
-    double c = 0.0;
-    if (2 * i <= t) {  // t! / (i! (t-2i)!)
-      c = 1.0;
-      for (int z = 0; z < i; z++) c *= (double)((t - 2 * z) * (t - 2 * z - 1)) / (z + 1);
+  // R^(n)_tuv = 2 (t-1) R^(n+1)_{t-2,u,v} + 2 PQ_x R^(n+1)_{t-1,u,v} along the first
+  // non-zero index (in the scale of G: R^(n)_000 = g_n)
+  for (int h = 1 + tid; h < NR; h += NT) {
+    int t, u, v;
+    herm_tuv(h, t, u, v);
+    int dir, h1, h2, cf;
+    if (t > 0) {
+      dir = 0, cf = t - 1, h1 = herm_index(t - 1, u, v), h2 = t > 1 ? herm_index(t - 2, u, v) : 0;
+    } else if (u > 0) {
+      dir = 1, cf = u - 1, h1 = herm_index(t, u - 1, v), h2 = u > 1 ? herm_index(t, u - 2, v) : 0;
+    } else {
+      dir = 2, cf = v - 1, h1 = herm_index(t, u, v - 1), h2 = v > 1 ? herm_index(t, u, v - 2) : 0;
     }
-    hc[e] = c;
+    rec[h] = h1 | (h2 << 10) | (dir << 20) | (cf << 22);
   }
   for (int e = tid; e < npp * 3 + nq; e += NT) {
     if (e < npp * 3) {
@@ -173,6 +186,35 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
     }
   }
   __syncthreads();
+  // R^(n) by levels n = L .. 0 (L is even: level n in R if n is even, else in
+  // F's space), quartets on lanes; every entry of a level is independent
+  {
+    double* Rt = F;
+#pragma unroll
+    for (int n = L; n >= 0; n--) {
+      const double* So = (n % 2 == 0) ? Rt : R;  // level n+1
+      double* Sn = (n % 2 == 0) ? R : Rt;
+      const int so = (n % 2 == 0) ? K::NRT : NR1, sn = (n % 2 == 0) ? NR1 : K::NRT;
+      const int nent = nherm(L - n);
+      for (int e = tid; e < nent * K::NQM; e += NT) {
+        const int h = e / K::NQM, q = e - h * K::NQM;  // compile-time divisor
+        if (q >= nq) continue;
+        const double* g = G + q * GW;
+        double val;
+        if (h == 0) {
+          val = g[n];
+        } else {
+          const int rc = rec[h];
+          const int h1 = rc & 1023, h2 = (rc >> 10) & 1023, dir = (rc >> 20) & 3, cf = rc >> 22;
+          const double* s0 = So + q * so;
+          val = fma((double)(2 * cf), s0[h2], g[(L + 1) * (1 + dir) + 1] * s0[h1]);
+        }
+        Sn[q * sn + h] = val;
+      }
+      __syncthreads();
+    }
+    for (int q = tid; q < nq; q += NT) R[q * NR1 + NR] = 0.0;  // the h2 >= H pad
+  }
   for (int e = tid; e < K::FSZ; e += NT) {
     double val = 0.0;
     if (e < npp * H * NAB) {
@@ -184,16 +226,6 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
                                     E + 2 * ESZ);
     }
     F[e] = val;
-  }
-  for (int e = tid; e < nq * NR1; e += NT) {
-    const int q = e / NR1, h = e - q * NR1;
-    double val = 0.0;
-    if (h < NR) {
-      const int tv = tuv[h];
-      const double* g = G + q * GW;
-      val = herm_R_tab(tv & 255, (tv >> 8) & 255, tv >> 16, g + (L + 1), L, g, hc, LH);
-    }
-    R[e] = val;
   }
   __syncthreads();
   for (int e = tid; e < ZL * ZL; e += NT) scr[e] = 0.0;  // Z (E tables and G are dead)
