@@ -18,6 +18,7 @@
 //     registers over all primitive pairs.
 #pragma once
 
+#include "gen3c_common.h"
 #include "int3c2e_impl.h"
 
 namespace dfhip {
@@ -209,32 +210,46 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     const double* Et = Et0 + (int64_t)ij * K::ETPP;
     for (int kp = 0; kp < xa.kmax; kp++) {
       __syncthreads();  // tables built (first pass); previous product done with W
-      for (int k = tid; k < nc * GW; k += NT) {
-        const int c = k / GW, q = k - c * GW;
-        const int sh = cshell[c];
-        const bool live = kp < ket.nprim[sh] && ckeep[c];
+      // G: per aux shell, lane 0 of 4 the scaled Boys values g_n = s (-alpha)^n
+      // F_n(T) (F_L from the table, the rest by downward recursion), lanes 1-3
+      // the powers (2 PC_d)^e
+      for (int k = tid; k < nc * 4; k += NT) {
+        const int c = k >> 2, part = k & 3;
         const double PC0 = P[0] - cxyz[3 * c + 0], PC1 = P[1] - cxyz[3 * c + 1], PC2 = P[2] - cxyz[3 * c + 2];
-        double val;
-        if (q <= L) {
-          val = 0.0;
+        double* gc = G + c * GWP;
+        if (part == 0) {
+          const int sh = cshell[c];
+          const bool live = kp < ket.nprim[sh] && ckeep[c];
+          double F[L + 1];
+          double scale = 0.0, alpha = 0.0;
           if (live) {
             const double g = ket.exps[ket.pptr[sh] + kp], dk = ket.coefs[ket.pptr[sh] + kp];
-            const double alpha = p * g / (p + g);
+            alpha = p * g / (p + g);
             const double T = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
-            double scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
+            scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
             const double inv2g = 0.5 / g;
+#pragma unroll
             for (int e = 0; e < LC; e++) scale *= inv2g;
-            for (int e = 0; e < q; e++) scale *= -alpha;
-            val = scale * boys_f(q, T, boys);
+            boys_array<L>(T, boys, F);
+          } else {
+#pragma unroll
+            for (int n = 0; n <= L; n++) F[n] = 0.0;
           }
+#pragma unroll
+          for (int n = 0; n <= L; n++) {
+            gc[n] = scale * F[n];
+            scale *= -alpha;
+          }
+          (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gc[L];  // R^(L)_000
         } else {
-          const int qq = q - (L + 1), d = qq / (L + 1), e = qq - d * (L + 1);
-          const double x = 2.0 * (d == 0 ? PC0 : (d == 1 ? PC1 : PC2));
-          val = 1.0;
-          for (int z = 0; z < e; z++) val *= x;
+          const double x = 2.0 * (part == 1 ? PC0 : (part == 2 ? PC1 : PC2));
+          double v = 1.0;
+#pragma unroll
+          for (int n = 0; n <= L; n++) {
+            gc[(L + 1) * part + n] = v;
+            v *= x;
+          }
         }
-        G[c * GWP + q] = val;
-        if (q == L) (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = val;  // R^(L)_000
       }
       __syncthreads();
       // levels n = L-1 .. 0: every entry of a level is independent (aux shells
