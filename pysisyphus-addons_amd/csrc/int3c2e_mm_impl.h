@@ -32,9 +32,9 @@ struct Cls3m {
   static constexpr int H = nherm(LAB), NR = nherm(L), NRLO = (LC == 0) ? 0 : nherm(LC - 1);
   static constexpr int HK = (H + 3) / 4 * 4, NABP = (NAB + 15) / 16 * 16, MT = NABP / 16;
   static constexpr int ESZ = (LA + 1) * (LB + 1) * (LAB + 1);
-  static constexpr int GW = 4 * (L + 1);  // scaled Boys values and powers of 2(P - C) per aux shell
+  static constexpr int GW = L + 4;  // per aux shell: scaled Boys values g_0..g_L, then 2 PC_x, 2 PC_y, 2 PC_z
   // LDS stride per aux shell: odd, so lanes on consecutive aux shells read distinct banks
-  static constexpr int GWP = GW + 1;
+  static constexpr int GWP = GW | 1;
   // R^(n)_tuv by levels n = L-1 .. 0 in two buffers: even levels in R (all NR
   // indices), odd levels in Rt (orders <= L-1); odd strides per aux shell
   static constexpr int NRP = NR | 1, NRT = nherm(L - 1) | 1;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
       __syncthreads();  // tables built (first pass); previous product done with W
       // G: per aux shell, lane 0 of 4 the scaled Boys values g_n = s (-alpha)^n
       // F_n(T) (F_L from the table, the rest by downward recursion), lanes 1-3
-      // the powers (2 PC_d)^e
+      // 2 PC_d
       for (int k = tid; k < nc * 4; k += NT) {
         const int c = k >> 2, part = k & 3;
         const double PC0 = P[0] - cxyz[3 * c + 0], PC1 = P[1] - cxyz[3 * c + 1], PC2 = P[2] - cxyz[3 * c + 2];
@@ -242,13 +242,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
           }
           (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gc[L];  // R^(L)_000
         } else {
-          const double x = 2.0 * (part == 1 ? PC0 : (part == 2 ? PC1 : PC2));
-          double v = 1.0;
-#pragma unroll
-          for (int n = 0; n <= L; n++) {
-            gc[(L + 1) * part + n] = v;
-            v *= x;
-          }
+          gc[L + part] = 2.0 * (part == 1 ? PC0 : (part == 2 ? PC1 : PC2));
         }
       }
       __syncthreads();
@@ -259,9 +253,11 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
         const double* So = (n % 2 == 0) ? Rt : R;  // level n+1
         double* Sn = (n % 2 == 0) ? R : Rt;
         const int so = (n % 2 == 0) ? NRT : NRP, sn = (n % 2 == 0) ? NRP : NRT;
-        const int nent = nherm(L - n);
+        // level 0: only the orders >= LC that the Hobson step reads
+        const int hbeg = n == 0 ? K::NRLO : 0, nent = nherm(L - n) - hbeg;
+#pragma unroll 2
         for (int k = tid; k < nent * NC; k += NT) {
-          const int h = k / NC, c = k - h * NC;  // compile-time divisor
+          const int h = hbeg + k / NC, c = k - (h - hbeg) * NC;  // compile-time divisor
           if (c >= nc) continue;
           const double* gc = G + c * GWP;
           double val;
@@ -271,7 +267,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
             const int rc = rec[h];
             const int h1 = rc & 1023, h2 = (rc >> 10) & 1023, dir = (rc >> 20) & 3, cf = rc >> 22;
             const double* sc0 = So + c * so;
-            val = fma((double)(2 * cf), sc0[h2], gc[(L + 1) * (1 + dir) + 1] * sc0[h1]);
+            val = fma((double)(2 * cf), sc0[h2], gc[L + 1 + dir] * sc0[h1]);
           }
           Sn[c * sn + h] = val;
         }
