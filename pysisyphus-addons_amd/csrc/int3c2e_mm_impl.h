@@ -39,8 +39,10 @@ struct Cls3m {
   // indices), odd levels in Rt (orders <= L-1); odd strides per aux shell
   static constexpr int NRP = NR | 1, NRT = nherm(L - 1) | 1;
   static constexpr int BUD = 18500;       // LDS doubles for R, Rt, G and W
+  // G of up to NPG primitive pairs is built once per workgroup (uncontracted aux)
+  static constexpr int NPG = 9;
   static constexpr int nc_fit(int nc) {
-    return nc * (NRP + NRT + GWP) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD;
+    return nc * (NRP + NRT + NPG * GWP) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD;
   }
   static constexpr int nc_max() {
     int nc = 32;
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, GWP = K::GWP;
   constexpr int NRP = K::NRP, NRT = K::NRT;
   constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL;
-  __shared__ double G[NC * GWP];       // per aux shell: scaled Boys values, then (2 PC_d)^e
+  __shared__ double G[K::NPG * NC * GWP];  // per primitive pair and aux shell: scaled Boys values, then 2 PC_d
   __shared__ double R[NC * NRP];       // per aux shell: R^(n)_tuv of the even levels, finally R_tuv
   __shared__ double Rt[NC * NRT];      // per aux shell: R^(n)_tuv of the odd levels
   __shared__ double W[HK * NCOLP];     // W[h][c NCC + m]
@@ -197,6 +199,56 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
   const double lc_sign = (LC & 1) ? -1.0 : 1.0;
 
+  // G of one (primitive pair, aux primitive, aux shell): lane part 0 of 4 the
+  // scaled Boys values g_n = s (-alpha)^n F_n(T) (F_L from the table, the rest
+  // by downward recursion), parts 1-3 2 PC_d; the R recursion is seeded with g_L
+  auto gstep = [&](int k, const double* P, double p, int kp, double* Gd) {
+    const int c = k >> 2, part = k & 3;
+    const double PC0 = P[0] - cxyz[3 * c + 0], PC1 = P[1] - cxyz[3 * c + 1], PC2 = P[2] - cxyz[3 * c + 2];
+    double* gc = Gd + c * GWP;
+    if (part == 0) {
+      const int sh = cshell[c];
+      const bool live = kp < ket.nprim[sh] && ckeep[c];
+      double F[L + 1];
+      double scale = 0.0, alpha = 0.0;
+      if (live) {
+        const double g = ket.exps[ket.pptr[sh] + kp], dk = ket.coefs[ket.pptr[sh] + kp];
+        alpha = p * g / (p + g);
+        const double T = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
+        scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
+        const double inv2g = 0.5 / g;
+#pragma unroll
+        for (int e = 0; e < LC; e++) scale *= inv2g;
+        boys_array<L>(T, boys, F);
+      } else {
+#pragma unroll
+        for (int n = 0; n <= L; n++) F[n] = 0.0;
+      }
+#pragma unroll
+      for (int n = 0; n <= L; n++) {
+        gc[n] = scale * F[n];
+        scale *= -alpha;
+      }
+      if (Gd == G) (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gc[L];  // R^(L)_000
+    } else {
+      gc[L + part] = 2.0 * (part == 1 ? PC0 : (part == 2 ? PC1 : PC2));
+    }
+  };
+  // uncontracted aux shells and at most NPG primitive pairs: G of every
+  // primitive pair at once (9x the parallelism, one wait on the Boys table)
+  const bool pre = xa.kmax == 1 && npp <= K::NPG;
+  if (pre) {
+    __syncthreads();  // cshell, cxyz and ckeep of the prologue are read below
+    for (int k = tid; k < npp * nc * 4; k += NT) {
+      const int ij = k / (nc * 4), kk = k - ij * (nc * 4);
+      const int i = ij / npb, j = ij - i * npb;
+      const double a = bra.exps[pa0 + i], b = bra.exps[pb0 + j], p = a + b;
+      double P[3];
+      for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
+      gstep(kk, P, p, 0, G + ij * NC * GWP);
+    }
+  }
+
   const int mt = wave / GRP, gi = wave % GRP;  // mt >= MT: this wave only helps with the tables
   dbl4m acc[NPW];
 #pragma unroll
@@ -210,40 +262,11 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     const double* Et = Et0 + (int64_t)ij * K::ETPP;
     for (int kp = 0; kp < xa.kmax; kp++) {
       __syncthreads();  // tables built (first pass); previous product done with W
-      // G: per aux shell, lane 0 of 4 the scaled Boys values g_n = s (-alpha)^n
-      // F_n(T) (F_L from the table, the rest by downward recursion), lanes 1-3
-      // 2 PC_d
-      for (int k = tid; k < nc * 4; k += NT) {
-        const int c = k >> 2, part = k & 3;
-        const double PC0 = P[0] - cxyz[3 * c + 0], PC1 = P[1] - cxyz[3 * c + 1], PC2 = P[2] - cxyz[3 * c + 2];
-        double* gc = G + c * GWP;
-        if (part == 0) {
-          const int sh = cshell[c];
-          const bool live = kp < ket.nprim[sh] && ckeep[c];
-          double F[L + 1];
-          double scale = 0.0, alpha = 0.0;
-          if (live) {
-            const double g = ket.exps[ket.pptr[sh] + kp], dk = ket.coefs[ket.pptr[sh] + kp];
-            alpha = p * g / (p + g);
-            const double T = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
-            scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
-            const double inv2g = 0.5 / g;
-#pragma unroll
-            for (int e = 0; e < LC; e++) scale *= inv2g;
-            boys_array<L>(T, boys, F);
-          } else {
-#pragma unroll
-            for (int n = 0; n <= L; n++) F[n] = 0.0;
-          }
-#pragma unroll
-          for (int n = 0; n <= L; n++) {
-            gc[n] = scale * F[n];
-            scale *= -alpha;
-          }
-          (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gc[L];  // R^(L)_000
-        } else {
-          gc[L + part] = 2.0 * (part == 1 ? PC0 : (part == 2 ? PC1 : PC2));
-        }
+      double* gb = pre ? G + ij * NC * GWP : G;
+      if (pre) {
+        for (int c = tid; c < nc; c += NT) (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gb[c * GWP + L];
+      } else {
+        for (int k = tid; k < nc * 4; k += NT) gstep(k, P, p, kp, G);
       }
       __syncthreads();
       // levels n = L-1 .. 0: every entry of a level is independent (aux shells
@@ -259,7 +282,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
         for (int k = tid; k < nent * NC; k += NT) {
           const int h = hbeg + k / NC, c = k - (h - hbeg) * NC;  // compile-time divisor
           if (c >= nc) continue;
-          const double* gc = G + c * GWP;
+          const double* gc = gb + c * GWP;
           double val;
           if (h == 0) {
             val = gc[n];

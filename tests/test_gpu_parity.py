@@ -128,6 +128,28 @@ def test_int3c2e_blocks_g_orbitals():
     assert err <= 1e-12 * max(scale, 1.0)
 
 
+def test_int3c2e_matrix_core_contracted():
+    """Matrix-core classes (f and g orbital shells, g aux) with contracted
+    aux shells and orbital shells of 4 primitives (16 primitive pairs): the
+    kernels' per-iteration G path, against the oracle."""
+    rng = np.random.default_rng(23)
+    orb, aux = [], []
+    for ci, C in enumerate(rng.uniform(-1.0, 1.0, (2, 3))):
+        for L, np_ in ((0, 4), (3, 2), (4, 4)):
+            e = np.sort(rng.uniform(0.2, 6.0, np_))[::-1]
+            orb.append(Shell(L=L, center=C, exps=e, coeffs=rng.uniform(0.2, 1.0, np_), center_ind=ci))
+        for L, np_ in ((1, 2), (4, 3)):
+            e = np.sort(rng.uniform(0.3, 4.0, np_))[::-1]
+            aux.append(Shell(L=L, center=C, exps=e, coeffs=rng.uniform(0.2, 1.0, np_), center_ind=ci))
+    shells, auxs = Shells(orb), Shells(aux)
+    got, _ = dense3c(shells, auxs, screening=N.SCREEN_NONE)
+    ref = oracle_dense3c(shells, auxs, screened=False)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max()
+    _say(f"3c2e matrix-core, contracted: max abs err {err:.2e} (max |int| {scale:.2e})")
+    assert err <= 1e-12 * max(scale, 1.0)
+
+
 def test_matrix_core_vs_lds_kernels(monkeypatch):
     """The matrix-core class kernels and the LDS kernels agree on every class
     both run (all classes up to (ff|h))."""
