@@ -106,7 +106,9 @@ typedef struct {
     double thresh;       /* skip when the estimate is < thresh; default (double)1e-10f */
     int32_t dens_layout; /* DFHIP_DENS_FORTRAN: (ndens,nbf,nbf) column-major,
                             DFHIP_DENS_C: (ndens,nbf,nbf) row-major */
-    int32_t verbose;     /* print sizes, phase times and screening counts */
+    int32_t verbose;     /* 1: print sizes, phase times and screening counts;
+                            2: also the wall time of each set-up step before the
+                            integrals (synchronizing, for diagnosis) */
     int64_t max_tile_bytes; /* cap of the (ab|P) staging matrix, 0 = automatic */
     int32_t dens_on_device; /* densities is a device pointer (HBM-resident input) */
     int32_t reserved;

@@ -1145,6 +1145,17 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipEventRecord(lane.ev_join, lane.st2));
   }
   const int e4 = e3;
+  // verbose >= 2: wall time of the steps between the metric and the integrals
+  // (each step synchronized first, so the times add up)
+  auto step_t0 = std::chrono::steady_clock::now();
+  auto vstep = [&](const char* what) {
+    if (opt.verbose < 2) return;
+    HC(hipStreamSynchronize(st));
+    const auto t = std::chrono::steady_clock::now();
+    vlog(opt, "  %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(t - step_t0).count());
+    step_t0 = t;
+  };
+  vstep("Schwarz/metric results, fork");
 
   const auto hp0 = std::chrono::steady_clock::now();
   build_plan(pl, opt);
@@ -1153,6 +1164,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   vlog(opt, "Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
   vlog(opt, "Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
   vlog(opt, " Densities: %6d\n", ndens);
+  vstep("host plan");
 
   // device plan arrays
   DBuf<int> ps1, ps2;
@@ -1160,7 +1172,6 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   upload(ps1, pl.ps.s1, st);
   upload(ps2, pl.ps.s2, st);
   upload(prow, pl.prow, st);
-  std::vector<DBuf<int>> dnsurv(LMAX_AUX + 1);
   std::vector<DBuf<int>> dalist(LMAX_AUX + 1);
   std::vector<DBuf<int64_t>> dacol(LMAX_AUX + 1);
   std::vector<int> kmax(LMAX_AUX + 1, 1);
@@ -1172,7 +1183,16 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   QvlRecords qvl;
   const bool use_qvl = opt.screening == DFHIP_SCREEN_QVL;
   if (use_qvl) apply_qvl(pl, dorb.view, daux.view, ps1.p, ps2.p, dalist, opt.thresh, qvl, st);
-  for (int L = 0; L <= LMAX_AUX; L++) upload(dnsurv[L], pl.nsurv[L], st);
+  // screened prefixes, one flat array [L][pair] for the kernels and the
+  // contraction's mask
+  DBuf<int> dnsflat;
+  {
+    std::vector<int> nsflat((size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1));
+    for (int L = 0; L <= LMAX_AUX; L++)
+      std::copy(pl.nsurv[L].begin(), pl.nsurv[L].end(), nsflat.begin() + (size_t)L * pl.ps.n());
+    upload(dnsflat, nsflat, st);
+  }
+  auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
   const bool gen = use_generated();
   std::vector<DBuf<int>> dn64(LMAX_AUX + 1);
   if (mixed) {
@@ -1181,6 +1201,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   DBuf<int> dorig;
   upload(dorig, pl.orig, st);
+  vstep("plan uploads");
   std::vector<BraRecords> brs(NPCLS);
   if (gen) {
     for (int la = 0; la <= LMAX_ORB; la++)
@@ -1193,6 +1214,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       }
   }
 
+  vstep("bra records");
   // densities: upload and pair-pack (diabatization/intor.f90:155-176 symmetry)
   double* Dp = nullptr;
   double ms_h2d = 0.0;
@@ -1236,14 +1258,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       upload(dpairof, pairof, st);
       upload(dfshell, fshell, st);
       pack_density_t(dpairof.p, dfshell.p, prow.p, dorb.L.p, dorb.f0.p, o.n, o.nbf, Dsrc, sI, ndens, Dp, ndens, st);
-      HC(hipGetLastError());
-      HC(hipStreamSynchronize(st));
     } else {
       pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp, ndens, st);
     }
+    // no wait here: the host sets up the integrals while the packing runs
     HC(hipGetLastError());
-    HC(hipStreamSynchronize(st));
   }
+  vstep("density packing");
   const int e5 = tm.mark();
 
   // column blocks of the (pair function x aux function) staging matrix
@@ -1278,27 +1299,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   int nsplit = 1;
   // screened-prefix mask of T (the 3c2e kernels write only screened-in blocks)
   ScreenMask mk{};
-  DBuf<int> drowpair, dcolL, dcolia, dnsflat;
+  DBuf<int> drowpair, dcolL, dcolia;
   if (need_dens) {
-    std::vector<int> rowpair(std::max<int64_t>(pl.npf, 1)), colL(std::max<int64_t>(naux, 1)),
-        colia(std::max<int64_t>(naux, 1));
-    for (int p = 0; p < pl.ps.n(); p++) {
-      const int64_t r1 = (p + 1 < pl.ps.n()) ? pl.prow[p + 1] : pl.npf;
-      for (int64_t r = pl.prow[p]; r < r1; r++) rowpair[r] = p;
-    }
+    std::vector<int> colL(std::max<int64_t>(naux, 1)), colia(std::max<int64_t>(naux, 1));
+    drowpair.alloc((size_t)std::max<int64_t>(pl.npf, 1));
+    fill_rowpair(prow.p, pl.ps.n(), pl.npf, drowpair.p, st);
     for (int L = 0; L <= LMAX_AUX; L++)
       for (size_t k = 0; k < pl.acol[L].size(); k++)
         for (int m = 0; m < 2 * L + 1; m++) {
           colL[pl.acol[L][k] + m] = L;
           colia[pl.acol[L][k] + m] = (int)k;
         }
-    std::vector<int> nsflat((size_t)(LMAX_AUX + 1) * pl.ps.n());
-    for (int L = 0; L <= LMAX_AUX; L++)
-      std::copy(pl.nsurv[L].begin(), pl.nsurv[L].end(), nsflat.begin() + (size_t)L * pl.ps.n());
-    upload(drowpair, rowpair, st);
     upload(dcolL, colL, st);
     upload(dcolia, colia, st);
-    upload(dnsflat, nsflat, st);
     mk = ScreenMask{drowpair.p, dcolL.p, dcolia.p, dnsflat.p, pl.ps.n()};
   }
   if (need_dens) {
@@ -1411,6 +1424,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   DBuf<int64_t> dseg;
   double ms_int = 0.0, ms_con = 0.0;
   std::vector<double> dense_stage;
+  vstep("integral set-up");
   for (int blk = 0; blk < nblocks; blk++) {
     const int64_t cA = bounds[blk], cB = bounds[blk + 1], w = cB - cA;
     // only the dense dump reads screened-out entries; the contraction masks them
@@ -1426,7 +1440,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         for (int lb = 0; lb <= la; lb++) {
           const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
           if (np == 0) continue;
-          PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dnsurv[L].p + p0, np};
+          PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dns(L) + p0, np};
           AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi, kmax[L]};
           if (use_qvl) {
             pa.qvl = qvl.pair.p + (size_t)8 * p0;

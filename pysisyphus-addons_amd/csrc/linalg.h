@@ -18,6 +18,8 @@ void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, 
 // fshell[j] = shell of function j.
 void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, const int* L, const int* f0, int nsh,
                     int nbf, const double* D, int64_t sI, int ndens, double* Dp, int ldd, hipStream_t st);
+// rowpair[r] = p for the rows of pair p (row[p] .. row[p+1]-1, the last ends at npf)
+void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, hipStream_t st);
 // Screened-prefix mask of the staging matrix T: entry (row k, column P) is
 // live iff colia[P] < nsurv[colL[P] * npairs + rowpair[k]] (column arrays
 // relative to the column block).

@@ -104,6 +104,19 @@ void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, co
                        ndens, Dp, ldd, pass);
 }
 
+// rowpair[r] = p for the rows r = row[p] .. row[p+1]-1 of pair p (the last
+// pair ends at npf): one workgroup per pair.
+__global__ void fill_rowpair_kernel(const int64_t* __restrict__ row, int npairs, int64_t npf,
+                                    int* __restrict__ rowpair) {
+  const int p = blockIdx.x;
+  const int64_t r0 = row[p], r1 = p + 1 < npairs ? row[p + 1] : npf;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) rowpair[r] = p;
+}
+
+void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, hipStream_t st) {
+  if (npairs > 0) hipLaunchKernelGGL(fill_rowpair_kernel, dim3(npairs), dim3(64), 0, st, row, npairs, npf, rowpair);
+}
+
 // ---------------------------------------------------------------------------
 // part[s][P][I] = sum_{k in split s} T[k][P] * Dp[k][I]
 // T: npf x ncol (row-major, ldt), Dp: npf x ndens (row-major, ldd).
