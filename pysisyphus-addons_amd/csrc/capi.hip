@@ -1315,7 +1315,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     mk = ScreenMask{drowpair.p, dcolL.p, dcolia.p, dnsflat.p, pl.ps.n()};
   }
   if (need_dens) {
-    const int64_t tiles = ((widest + 63) / 64) * ((ndens + 63) / 64);
+    const int64_t tiles = contract_ntiles(widest, ndens);
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(512, 2048 / std::max<int64_t>(tiles, 1)));
     nsplit = (int)std::min<int64_t>(nsplit, std::max<int64_t>(1, (pl.npf + 15) / 16));
     partp = reserve(cur_lane().wsPart, (size_t)nsplit * widest * ndens);

@@ -35,6 +35,8 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
                 int64_t sP, int64_t sI, hipStream_t st);
 int contract_tile();
+// output tiles of one contraction (column tiles x density tiles)
+int64_t contract_ntiles(int64_t ncol, int ndens);
 // Cholesky of [[M], [I]] (n columns, 2n rows, lda): L in the leading block,
 // L^-T in the border rows n..2n-1.
 void cholesky_inverse_rows(double* A, int64_t lda, int64_t n, int* info, hipStream_t st);

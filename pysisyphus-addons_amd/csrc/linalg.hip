@@ -124,30 +124,34 @@ void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, hip
 // the tile are the concatenation of nseg row segments [seg_lo, seg_hi) per
 // column tile (host-computed from the screened prefixes), split evenly over
 // the split index s.
-// 64x64 output tile per workgroup, 256 threads with 4x4 each, K step 16.
-constexpr int GT = 64, GK = 16;
+// Output tile per workgroup: GP = 128 columns P x 16 IPT densities, 256
+// threads with 8 x IPT each (IPT from the number of densities, so that 36
+// densities take one tile of 48), K step 16.
+constexpr int GP = 128, GK = 16;
 
+template <int IPT>
 __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __restrict__ T, int64_t ldt,
                                                              const double* __restrict__ Dp, int ldd, int ncol,
                                                              int ndens, int nseg, const int64_t* __restrict__ seg,
                                                              int nsplit, const ScreenMask mk,
                                                              double* __restrict__ part) {
-  __shared__ double sT[GK][GT + 1];
-  __shared__ double sD[GK][GT + 1];
+  constexpr int GI = 16 * IPT;
+  __shared__ double sT[GK][GP + 1];
+  __shared__ double sD[GK][GI + 1];
   const int tcol = blockIdx.x;
-  const int tp = blockIdx.x * GT, ti = blockIdx.y * GT, s = blockIdx.z;
+  const int tp = blockIdx.x * GP, ti = blockIdx.y * GI, s = blockIdx.z;
   const int64_t* sg = seg + (size_t)tcol * (2 * nseg + 1);  // [total, lo0, hi0, lo1, hi1, ...]
   const int64_t tot = sg[0];
   const int64_t chunk = ((tot + nsplit - 1) / nsplit + GK - 1) / GK * GK;
   int64_t q0 = (int64_t)s * chunk, q1 = min(tot, q0 + chunk);
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;  // outputs P = tp + ty*4 + a, I = ti + tx*4 + b
-  // this lane always loads column tp + (tid % GT) of T: its screened prefix test
-  const int mcol = tp + (tid % GT);
+  const int tx = tid & 15, ty = tid >> 4;  // outputs P = tp + ty*8 + a, I = ti + tx*IPT + b
+  // this lane always loads column tp + (tid % GP) of T: its screened prefix test
+  const int mcol = tp + (tid % GP);
   const bool colok = mcol < ncol;
   const int* __restrict__ ns_col = colok ? mk.nsurv + (size_t)mk.colL[mcol] * mk.npairs : mk.nsurv;
   const int ia_col = colok ? mk.colia[mcol] : 0;
-  double acc[4][4] = {};
+  double acc[8][IPT] = {};
   // walk the segments overlapping [q0, q1) of the concatenated row range
   int64_t base = 0;
   for (int g = 0; g < nseg && q0 < q1; g++) {
@@ -160,50 +164,66 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
     for (int64_t kb = a0; kb < a1; kb += GK) {
       // unconditional loads from clamped (in-bounds) addresses, then selects
       const int mcolc = colok ? mcol : ncol - 1;
-      constexpr int NZ = GK * GT / 256;
-      double tv[NZ], dv[NZ];
+      constexpr int NZ = GK * GP / 256, NZD = (GK * GI + 255) / 256;
+      double tv[NZ], dv[NZD];
       int ns[NZ];
 #pragma unroll
       for (int z = 0; z < NZ; z++) {
-        const int e = tid + 256 * z, kk = e / GT, c = e - kk * GT;
+        const int e = tid + 256 * z, kk = e / GP;
         const int64_t kc = min(kb + kk, a1 - 1);
         tv[z] = T[kc * ldt + mcolc];
-        dv[z] = Dp[kc * ldd + min(ti + c, ndens - 1)];
         ns[z] = ns_col[mk.rowpair[kc]];
       }
 #pragma unroll
+      for (int z = 0; z < NZD; z++) {
+        const int e = min(tid + 256 * z, GK * GI - 1), kk = e / GI, c = e - kk * GI;
+        const int64_t kc = min(kb + kk, a1 - 1);
+        dv[z] = Dp[kc * ldd + min(ti + c, ndens - 1)];
+      }
+#pragma unroll
       for (int z = 0; z < NZ; z++) {
-        const int e = tid + 256 * z, kk = e / GT, c = e - kk * GT;
+        const int e = tid + 256 * z, kk = e / GP, c = e - kk * GP;
         const int64_t k = kb + kk;
         // entries of screened-out triplets are never written: mask them here
         const bool live = k < a1 && colok && ia_col < ns[z];
         sT[kk][c] = live ? tv[z] : 0.0;
-        sD[kk][c] = (k < a1 && ti + c < ndens) ? dv[z] : 0.0;
+      }
+#pragma unroll
+      for (int z = 0; z < NZD; z++) {
+        const int e = tid + 256 * z, kk = e / GI, c = e - kk * GI;
+        if (e < GK * GI) sD[kk][c] = (kb + kk < a1 && ti + c < ndens) ? dv[z] : 0.0;
       }
       __syncthreads();
 #pragma unroll
       for (int kk = 0; kk < GK; kk++) {
-        double a[4], b[4];
+        double a[8], b[IPT];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          a[q] = sT[kk][ty * 4 + q];
-          b[q] = sD[kk][tx * 4 + q];
-        }
+        for (int q = 0; q < 8; q++) a[q] = sT[kk][ty * 8 + q];
 #pragma unroll
-        for (int x = 0; x < 4; x++)
+        for (int q = 0; q < IPT; q++) b[q] = sD[kk][tx * IPT + q];
 #pragma unroll
-          for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+        for (int x = 0; x < 8; x++)
+#pragma unroll
+          for (int y = 0; y < IPT; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
       }
       __syncthreads();
     }
     q0 = base + len;
     base += len;
   }
-  for (int x = 0; x < 4; x++)
-    for (int y = 0; y < 4; y++) {
-      const int P = tp + ty * 4 + x, I = ti + tx * 4 + y;
+  for (int x = 0; x < 8; x++)
+    for (int y = 0; y < IPT; y++) {
+      const int P = tp + ty * 8 + x, I = ti + tx * IPT + y;
       if (P < ncol && I < ndens) part[((size_t)s * ncol + P) * ndens + I] = acc[x][y];
     }
+}
+
+// densities per thread of the contraction's output tile
+int contract_ipt(int ndens) { return ndens <= 16 ? 1 : ndens <= 32 ? 2 : ndens <= 48 ? 3 : 4; }
+
+int64_t contract_ntiles(int64_t ncol, int ndens) {
+  const int gi = 16 * contract_ipt(ndens);
+  return ((ncol + GP - 1) / GP) * ((ndens + gi - 1) / gi);
 }
 
 // d[orig(P) * sP + I * sI] += sum_s part[s][P][I]   (fixed summation order)
@@ -222,15 +242,31 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
                 int64_t sP, int64_t sI, hipStream_t st) {
   if (ncol <= 0) return;
-  dim3 grid((ncol + GT - 1) / GT, (ndens + GT - 1) / GT, nsplit);
-  hipLaunchKernelGGL(gemm_tn_splitk_kernel, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg, nsplit,
-                     mk, part);
+  const int ipt = contract_ipt(ndens), gi = 16 * ipt;
+  dim3 grid((ncol + GP - 1) / GP, (ndens + gi - 1) / gi, nsplit);
+  switch (ipt) {
+    case 1:
+      hipLaunchKernelGGL(gemm_tn_splitk_kernel<1>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
+                         nsplit, mk, part);
+      break;
+    case 2:
+      hipLaunchKernelGGL(gemm_tn_splitk_kernel<2>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
+                         nsplit, mk, part);
+      break;
+    case 3:
+      hipLaunchKernelGGL(gemm_tn_splitk_kernel<3>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
+                         nsplit, mk, part);
+      break;
+    default:
+      hipLaunchKernelGGL(gemm_tn_splitk_kernel<4>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
+                         nsplit, mk, part);
+  }
   const int64_t n = (int64_t)ncol * ndens;
   hipLaunchKernelGGL(reduce_unpermute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, nsplit, ncol,
                      ndens, orig, d, sP, sI);
 }
 
-int contract_tile() { return GT; }
+int contract_tile() { return GP; }
 
 
 // ---------------------------------------------------------------------------
