@@ -161,12 +161,14 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
       continue;
     }
     const int64_t a0 = lo + (q0 - base), a1 = lo + min(len, q1 - base);
-    for (int64_t kb = a0; kb < a1; kb += GK) {
-      // unconditional loads from clamped (in-bounds) addresses, then selects
-      const int mcolc = colok ? mcol : ncol - 1;
-      constexpr int NZ = GK * GP / 256, NZD = (GK * GI + 255) / 256;
-      double tv[NZ], dv[NZD];
-      int ns[NZ];
+    // K blocks of GK rows; the global loads of block kb + GK are issued
+    // before the FMAs of block kb (unconditional loads from clamped, in-bounds
+    // addresses, then selects)
+    const int mcolc = colok ? mcol : ncol - 1;
+    constexpr int NZ = GK * GP / 256, NZD = (GK * GI + 255) / 256;
+    double tv[NZ], dv[NZD];
+    int ns[NZ];
+    auto load = [&](int64_t kb) {
 #pragma unroll
       for (int z = 0; z < NZ; z++) {
         const int e = tid + 256 * z, kk = e / GP;
@@ -180,6 +182,9 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
         const int64_t kc = min(kb + kk, a1 - 1);
         dv[z] = Dp[kc * ldd + min(ti + c, ndens - 1)];
       }
+    };
+    load(a0);
+    for (int64_t kb = a0; kb < a1; kb += GK) {
 #pragma unroll
       for (int z = 0; z < NZ; z++) {
         const int e = tid + 256 * z, kk = e / GP, c = e - kk * GP;
@@ -194,6 +199,7 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
         if (e < GK * GI) sD[kk][c] = (kb + kk < a1 && ti + c < ndens) ? dv[z] : 0.0;
       }
       __syncthreads();
+      if (kb + GK < a1) load(kb + GK);
 #pragma unroll
       for (int kk = 0; kk < GK; kk++) {
         double a[8], b[IPT];
