@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
 }
 
 // densities per thread of the contraction's output tile
-int contract_ipt(int ndens) { return ndens <= 16 ? 1 : ndens <= 32 ? 2 : ndens <= 48 ? 3 : 4; }
+int contract_ipt(int ndens) { return ndens <= 16 ? 1 : ndens <= 32 ? 2 : ndens <= 48 ? 3 : ndens <= 64 ? 4 : 7; }
 
 int64_t contract_ntiles(int64_t ncol, int ndens) {
   const int gi = 16 * contract_ipt(ndens);
@@ -263,8 +263,12 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
       hipLaunchKernelGGL(gemm_tn_splitk_kernel<3>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
                          nsplit, mk, part);
       break;
-    default:
+    case 4:
       hipLaunchKernelGGL(gemm_tn_splitk_kernel<4>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
+                         nsplit, mk, part);
+      break;
+    default:  // more than 64 densities: tiles of 112, T read half as often as with 64
+      hipLaunchKernelGGL(gemm_tn_splitk_kernel<7>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
                          nsplit, mk, part);
   }
   const int64_t n = (int64_t)ncol * ndens;

@@ -249,6 +249,27 @@ def test_workload0_df_and_j_vs_oracle():
     np.testing.assert_allclose(df, info["B"], atol=1e-6 * np.abs(info["B"]).max())
 
 
+@pytest.mark.parametrize("nstates", [12])
+def test_many_densities_vs_oracle(nstates):
+    """More than 64 densities (78 for 12 states): the tiled-transpose packing
+    of C-ordered densities and the widest contraction tiles, against the
+    oracle, with the densities in both layouts."""
+    shells, aux, _ = synthetic.workload(0)
+    rng = np.random.Generator(np.random.PCG64(77))
+    dens = synthetic.random_densities(rng, nstates, shells.nbf)
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    _, J, st = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True)
+    Jref, info = O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True)
+    errJ = np.abs(J - Jref).max()
+    _say(f"workload 0, {dens.shape[0]} densities: max |dJ| {errJ:.2e}, |J|max {np.abs(Jref).max():.3e}")
+    assert st["ncomputed"] == info["ncomputed"]
+    assert errJ <= 1e-10
+    # the reference's Fortran layout (nbf, nbf, ndens) through the drop-in entry point
+    J4 = intor.contract_coulomb_densities_4d(shells, aux, dens)
+    assert np.abs(J4 - Jref).max() <= 1e-10
+
+
 def test_qvl_workload0_vs_oracle():
     """QVL screening (the screener's qvl_estimate) through the drop-in path:
     same kept/skipped triplets as the oracle and J at 1e-10."""
