@@ -21,6 +21,7 @@ struct Cls4s {
   static constexpr int LAB = LA + LB, L = 2 * LAB;
   static constexpr int NAB = nsph(LA) * nsph(LB), NB = nsph(LB);
   static constexpr int H = nherm(LAB), NR = nherm(L), NR1 = NR + 1;
+  static constexpr int NRL = nherm(L - 1) | 1;  // odd levels of the R recursion, per quartet
   static constexpr int HK = (H + 3) / 4 * 4, NS = HK / 4;
   static constexpr int ESZ = (LA + 1) * (LB + 1) * (LAB + 1);
   static constexpr int NCT = (NAB + 15) / 16, ZL = 16 * NCT;  // n1 tiles of Z
@@ -29,17 +30,19 @@ struct Cls4s {
   static constexpr int NRT = (NPPS * H + 15) / 16;
   static constexpr int NW = NRT > 48 ? 16 : 8, NT = 64 * NW;  // at most 6 row tiles per wave
   static constexpr int MAXT = (NRT + NW - 1) / NW;
-  static constexpr int LH = L / 2 + 1, GW = 4 * (L + 1);
+  static constexpr int GW = (L + 4) | 1;  // per quartet: g_0..g_L, then 2 PQ_x, 2 PQ_y, 2 PQ_z (odd stride)
   // LDS of the main kernel: B chunk [HK][16], R of one ket pair [NPPS][NR1], Z [NAB][NAB]
   static constexpr int LDS = HK * 16 + NPPS * NR1 + NAB * NAB;
   static constexpr bool OK = LDS <= 19500;
   // global scratch per pair (doubles): F rows padded by HK - H + 16 for the
   // last ket pair's padded K steps and the Z operand's last row tile
   static constexpr int64_t FPAIR = (int64_t)(NPPS * H + HK + 16) * NAB + 16;
+  // the F region holds the odd levels of the R recursion before F is built
+  static constexpr int64_t FREG = FPAIR > (int64_t)NQS * NRL ? FPAIR : (int64_t)NQS * NRL;
   static constexpr int64_t RPAIR = (int64_t)NQS * NR1;
-  static constexpr int64_t PERPAIR = FPAIR + RPAIR;
+  static constexpr int64_t PERPAIR = FREG + RPAIR;
   // LDS of the prep kernel
-  static constexpr int PREP_LDS = NPPS * 3 * ESZ + NQS * GW + (L + 1) * LH;
+  static constexpr int PREP_LDS = NPPS * 3 * ESZ + NQS * GW + NR;
 };
 
 template <int LA, int LB>
@@ -49,11 +52,11 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
                                                              const double* __restrict__ boys, double* __restrict__ scr) {
   using K = Cls4s<LA, LB>;
   constexpr int L = K::L, H = K::H, NR = K::NR, NR1 = K::NR1, NAB = K::NAB, NB = K::NB, ESZ = K::ESZ, GW = K::GW;
-  constexpr int LH = K::LH, NT = 256;
+  constexpr int NT = 256;
   __shared__ double Esc[K::NPPS * 3 * ESZ];
   __shared__ double G[K::NQS * GW];
-  __shared__ double hc[(L + 1) * LH];
   __shared__ int tuv[NR];
+  __shared__ int rec[NR];  // recursion of index h: parents h1, h2, direction, coefficient
   __shared__ int sa[2 * LA + 2], sb[2 * LB + 2];
   __shared__ SphTerm ta[nsph(LA) * ncart(LA)], tb[nsph(LB) * ncart(LB)];
   const int tid = threadIdx.x, pair = blockIdx.x;
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
   const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb, nq = npp * (npp + 1) / 2;
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
   double* F = scr + (size_t)pair * K::PERPAIR;
-  double* R = F + K::FPAIR;
+  double* R = F + K::FREG;
 
   for (int h = tid; h < NR; h += NT) {
     int t, u, v;
@@ -76,14 +79,20 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
     for (int e = tid; e < na; e += NT) ta[e] = sph->terms[a0 + e];
     for (int e = tid; e < nb; e += NT) tb[e] = sph->terms[b0 + e];
   }
-  for (int e = tid; e < (L + 1) * LH; e += NT) {
-    const int t = e / LH, i = e - t * LH;
-    double c = 0.0;
-    if (2 * i <= t) {
-      c = 1.0;
-      for (int z = 0; z < i; z++) c *= (double)((t - 2 * z) * (t - 2 * z - 1)) / (z + 1);
+  // R^(n)_tuv = 2 (t-1) R^(n+1)_{t-2,u,v} + 2 PQ_x R^(n+1)_{t-1,u,v} along the first
+  // non-zero index (in the scale of G: R^(n)_000 = g_n)
+  for (int h = 1 + tid; h < NR; h += NT) {
+    int t, u, v;
+    herm_tuv(h, t, u, v);
+    int dir, h1, h2, cf;
+    if (t > 0) {
+      dir = 0, cf = t - 1, h1 = herm_index(t - 1, u, v), h2 = t > 1 ? herm_index(t - 2, u, v) : 0;
+    } else if (u > 0) {
+      dir = 1, cf = u - 1, h1 = herm_index(t, u - 1, v), h2 = u > 1 ? herm_index(t, u - 2, v) : 0;
+    } else {
+      dir = 2, cf = v - 1, h1 = herm_index(t, u, v - 1), h2 = v > 1 ? herm_index(t, u, v - 2) : 0;
     }
-    hc[e] = c;
+    rec[h] = h1 | (h2 << 11) | (dir << 22) | (cf << 24);
   }
   for (int e = tid; e < npp * 3 + nq; e += NT) {
     if (e < npp * 3) {
@@ -117,17 +126,40 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
         sc *= -alpha;
       }
 #pragma unroll
-      for (int z = 0; z < 3; z++) {
-        const double x = 2.0 * PQ[z];
-        double v = 1.0;
-        for (int n = 0; n <= L; n++) {
-          g[(L + 1) * (1 + z) + n] = v;
-          v *= x;
-        }
-      }
+      for (int z = 0; z < 3; z++) g[L + 1 + z] = 2.0 * PQ[z];
     }
   }
   __syncthreads();
+  // R^(n) by levels n = L .. 0 in the pair's scratch (L is even: level n in R
+  // if n is even, else in the F region, which is built afterwards), quartets
+  // on lanes; every entry of a level is independent.  Global memory written
+  // before __syncthreads() is visible to the whole workgroup after it.
+  {
+    double* Rt = F;
+    for (int n = L; n >= 0; n--) {
+      const double* So = (n % 2 == 0) ? Rt : R;  // level n+1
+      double* Sn = (n % 2 == 0) ? R : Rt;
+      const int so = (n % 2 == 0) ? K::NRL : NR1, sn = (n % 2 == 0) ? NR1 : K::NRL;
+      const int nent = nherm(L - n);
+      for (int e = tid; e < nent * K::NQS; e += NT) {
+        const int h = e / K::NQS, q = e - h * K::NQS;  // compile-time divisor
+        if (q >= nq) continue;
+        const double* g = G + q * GW;
+        double val;
+        if (h == 0) {
+          val = g[n];
+        } else {
+          const int rc = rec[h];
+          const int h1 = rc & 2047, h2 = (rc >> 11) & 2047, dir = (rc >> 22) & 3, cf = rc >> 24;
+          const double* s0 = So + (size_t)q * so;
+          val = fma((double)(2 * cf), s0[h2], g[L + 1 + dir] * s0[h1]);
+        }
+        Sn[(size_t)q * sn + h] = val;
+      }
+      __syncthreads();
+    }
+    for (int q = tid; q < nq; q += NT) R[(size_t)q * NR1 + NR] = 0.0;  // the h2 >= H pad
+  }
   // F rows beyond npp H (padding read by the main kernel) are zero
   for (int64_t e = tid; e < K::FPAIR; e += NT) {
     double val = 0.0;
@@ -140,16 +172,6 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
                                     E + 2 * ESZ);
     }
     F[e] = val;
-  }
-  for (int e = tid; e < nq * NR1; e += NT) {
-    const int q = e / NR1, h = e - q * NR1;
-    double val = 0.0;
-    if (h < NR) {
-      const int tv = tuv[h];
-      const double* g = G + q * GW;
-      val = herm_R_tab(tv & 255, (tv >> 8) & 255, tv >> 16, g + (L + 1), L, g, hc, LH);
-    }
-    R[e] = val;
   }
 }
 
@@ -169,7 +191,7 @@ __global__ __launch_bounds__((Cls4s<LA, LB>::NT)) void schwarz4s_kernel(DevShell
   const int s1 = s1v[pair], s2 = s2v[pair];
   const int npp = bra.nprim[s1] * bra.nprim[s2];
   const double* F = scr + (size_t)pair * K::PERPAIR;
-  const double* R = F + K::FPAIR;
+  const double* R = F + K::FREG;
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const int nrt = (npp * H + 15) / 16;
 
