@@ -469,30 +469,28 @@ struct PairSet {
 
 // All a >= b pairs, oriented with L(s1) >= L(s2), grouped by class in packed order.
 PairSet make_pairs(const HostShells& o) {
-  std::vector<std::vector<int>> by(NPCLS);
+  // pairs per class counted first, then placed directly (packed index order
+  // inside each class)
+  std::vector<int> cnt(NPCLS + 1, 0);
+  for (int a = 0; a < o.n; a++)
+    for (int b = 0; b <= a; b++) cnt[pcls(std::max(o.L[a], o.L[b]), std::min(o.L[a], o.L[b])) + 1]++;
+  for (int c = 0; c < NPCLS; c++) cnt[c + 1] += cnt[c];
+  PairSet ps;
+  const int np = cnt[NPCLS];
+  ps.s1.resize(np);
+  ps.s2.resize(np);
+  ps.packed.resize(np);
+  ps.cls_begin.assign(cnt.begin(), cnt.begin() + NPCLS);
+  ps.cls_end.assign(cnt.begin() + 1, cnt.end());
+  std::vector<int> pos(cnt.begin(), cnt.begin() + NPCLS);
   for (int a = 0; a < o.n; a++)
     for (int b = 0; b <= a; b++) {
-      const int la = std::max(o.L[a], o.L[b]), lb = std::min(o.L[a], o.L[b]);
-      by[pcls(la, lb)].push_back(a * (a + 1) / 2 + b);
+      const bool sw = o.L[a] < o.L[b];
+      const int p = pos[pcls(std::max(o.L[a], o.L[b]), std::min(o.L[a], o.L[b]))]++;
+      ps.s1[p] = sw ? b : a;
+      ps.s2[p] = sw ? a : b;
+      ps.packed[p] = a * (a + 1) / 2 + b;
     }
-  PairSet ps;
-  ps.cls_begin.assign(NPCLS, 0);
-  ps.cls_end.assign(NPCLS, 0);
-  for (int c = 0; c < NPCLS; c++) {
-    ps.cls_begin[c] = ps.n();
-    for (int pk : by[c]) {
-      int a = (int)((std::sqrt(8.0 * pk + 1.0) - 1.0) / 2.0);
-      while ((a + 1) * (a + 2) / 2 <= pk) a++;
-      while (a * (a + 1) / 2 > pk) a--;
-      const int b = pk - a * (a + 1) / 2;
-      int s1 = a, s2 = b;
-      if (o.L[a] < o.L[b]) std::swap(s1, s2);
-      ps.s1.push_back(s1);
-      ps.s2.push_back(s2);
-      ps.packed.push_back(pk);
-    }
-    ps.cls_end[c] = ps.n();
-  }
   return ps;
 }
 

@@ -215,17 +215,21 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
     }
     for (int q = tid; q < nq; q += NT) R[q * NR1 + NR] = 0.0;  // the h2 >= H pad
   }
+  // F[(pq H + h) NAB + n]: Hermite index on lanes, so that lanes share the
+  // solid-harmonic term loops of (mu, nu); rows past npp H stay zero
   for (int e = tid; e < K::FSZ; e += NT) {
-    double val = 0.0;
+    int f = e;
     if (e < npp * H * NAB) {
-      const int pq = e / (H * NAB), r = e - pq * (H * NAB), h = r / NAB, n = r - h * NAB;
+      const int pq = e / (H * NAB), r = e - pq * (H * NAB), n = r / H, h = r - n * H;
+      f = (pq * H + h) * NAB + n;
       const int i = pq / npb, j = pq - i * npb, tv = tuv[h];
       const double* E = Esc + pq * 3 * ESZ;
-      val = bra.coefs[pa0 + i] * bra.coefs[pb0 + j] *
-            bra_element_lds<LA, LB>(sa, ta, sb, tb, n / NB, n % NB, tv & 255, (tv >> 8) & 255, tv >> 16, E, E + ESZ,
-                                    E + 2 * ESZ);
+      F[f] = bra.coefs[pa0 + i] * bra.coefs[pb0 + j] *
+             bra_element_lds<LA, LB>(sa, ta, sb, tb, n / NB, n % NB, tv & 255, (tv >> 8) & 255, tv >> 16, E, E + ESZ,
+                                     E + 2 * ESZ);
+    } else {
+      F[f] = 0.0;
     }
-    F[e] = val;
   }
   __syncthreads();
   for (int e = tid; e < ZL * ZL; e += NT) scr[e] = 0.0;  // Z (E tables and G are dead)
