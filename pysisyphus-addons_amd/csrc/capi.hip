@@ -602,6 +602,12 @@ inline bool gen3_has(int la, int lb, int lc) {
   return la <= GEN3_LA_MAX && lb <= la && lc <= GEN3_LC_MAX;
 }
 
+// Classes with a generated kernel where the matrix-core kernel is faster on
+// MI355X (measured per class on the 60-atom s-d / s-f workload: (dd|d) 577
+// against 776 us, (dd|f) 388 against 557 us); in mixed mode their FP64
+// window goes to the matrix-core kernel, the FP32 window stays generated.
+inline bool prefer_mm(int la, int lb, int lc) { return la == 2 && lb == 2 && lc >= 2; }
+
 void launch_bra_records(int la, int lb, DevShells bra, const int* s1, const int* s2, const int64_t* recoff, int rs,
                         int npairs, double* recs, hipStream_t st) {
   if (npairs <= 0) return;
@@ -1339,7 +1345,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         if (np == 0 || c.et[la][lb].fn == nullptr) continue;
         bool any = false;
         for (int L = 0; L <= LMAX_AUX; L++) {
-          use_mm[k][L] = !(gen && gen3_has(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
+          use_mm[k][L] = (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
           any = any || use_mm[k][L];
         }
         if (!any) continue;
@@ -1445,18 +1451,23 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             xa.qvl = qvl.aux.p;
             xa.thresh = opt.thresh;
           }
+          const bool mmk = use_mm[k][L] && etg[k].p;
+          auto launch_mm = [&](const PairArgs& q) {
+            c.int3m[la][lb][L].fn(dorb.view, daux.view, q, xa, etg[k].p, etoff[k].p, c.sph.p, c.boys.p, Tp - cA, w,
+                                  st);
+            HC(hipGetLastError());
+          };
           if (gen && gen3_has(la, lb, L) && mixed) {
             PairArgs p64 = pa, p32 = pa;
             p64.nsurv = dn64[L].p + p0;  // [0, n64): FP64
             p32.nbeg = dn64[L].p + p0;   // [n64, nsurv): FP32
-            launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);
+            if (mmk) launch_mm(p64);
+            else launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);
             launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
+          } else if (mmk) {
+            launch_mm(pa);
           } else if (gen && gen3_has(la, lb, L)) {
             launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
-          } else if (use_mm[k][L] && etg[k].p) {
-            c.int3m[la][lb][L].fn(dorb.view, daux.view, pa, xa, etg[k].p, etoff[k].p, c.sph.p, c.boys.p, Tp - cA, w,
-                                  st);
-            HC(hipGetLastError());
           } else {
             c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, Tp - cA, w, st);
             HC(hipGetLastError());
