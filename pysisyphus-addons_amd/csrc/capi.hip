@@ -1198,10 +1198,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
   const bool gen = use_generated();
-  std::vector<DBuf<int>> dn64(LMAX_AUX + 1);
+  DBuf<int> dn64flat;  // FP64 windows of the prefixes, [L][pair] like dnsflat
   if (mixed) {
     split_precision(pl, opt.fp32_below, gen);
-    for (int L = 0; L <= LMAX_AUX; L++) upload(dn64[L], pl.n64[L], st);
+    std::vector<int> f((size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1));
+    for (int L = 0; L <= LMAX_AUX; L++)
+      std::copy(pl.n64[L].begin(), pl.n64[L].end(), f.begin() + (size_t)L * pl.ps.n());
+    upload(dn64flat, f, st);
   }
   DBuf<int> dorig;
   upload(dorig, pl.orig, st);
@@ -1459,8 +1462,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
           };
           if (gen && gen3_has(la, lb, L) && mixed) {
             PairArgs p64 = pa, p32 = pa;
-            p64.nsurv = dn64[L].p + p0;  // [0, n64): FP64
-            p32.nbeg = dn64[L].p + p0;   // [n64, nsurv): FP32
+            p64.nsurv = dn64flat.p + (size_t)L * pl.ps.n() + p0;  // [0, n64): FP64
+            p32.nbeg = dn64flat.p + (size_t)L * pl.ps.n() + p0;   // [n64, nsurv): FP32
             if (mmk) launch_mm(p64);
             else launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);
             launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
