@@ -146,9 +146,13 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   __shared__ int sc[2 * LC + 2];
   __shared__ SphTerm tc[nsph_terms(LC)];
 
-  const int pidx = blockIdx.x;
+  // XCD-aware order: workgroups go round-robin to the 8 XCDs by linear id, so
+  // slot blockIdx.x % 8 of a row of the grid keeps one pair and walks its aux
+  // chunks (blockIdx.x / 8): the chunks of a pair share its Et table in one L2
+  const int pidx = (int)blockIdx.y * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c0 = xa.lo + (int)blockIdx.y * NC;
+  const int c0 = xa.lo + (int)(blockIdx.x >> 3) * NC;
   if (c0 >= cend) return;
   const int nc = min(NC, cend - c0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
@@ -371,8 +375,8 @@ void launch_int3c2e_mm(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa, co
   using K = Cls3m<LA, LB, LC>;
   const int nchunks = (xa.hi - xa.lo + K::NC - 1) / K::NC;
   if (pa.npairs <= 0 || nchunks <= 0) return;
-  hipLaunchKernelGGL((int3c2e_mm_kernel<LA, LB, LC>), dim3(pa.npairs, nchunks), dim3(K::NT), 0, st, bra, ket, pa, xa,
-                     EtG, etoff, sph, boys, out, ldo);
+  hipLaunchKernelGGL((int3c2e_mm_kernel<LA, LB, LC>), dim3(8 * nchunks, (pa.npairs + 7) / 8), dim3(K::NT), 0, st, bra,
+                     ket, pa, xa, EtG, etoff, sph, boys, out, ldo);
 }
 
 template <int LA, int LB, int LC>
