@@ -253,8 +253,10 @@ struct Lane {
   // integrals run on st (created on first use)
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int* hinfo = nullptr;  // pinned host copy of the factorization's status
   void ensure_side() {
     if (st2) return;
+    HC(hipHostMalloc((void**)&hinfo, sizeof(int), hipHostMallocDefault));
     HC(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     HC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
@@ -1114,14 +1116,6 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   DBuf<double> dqc(x.n);
   schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
   const int e3 = tm.mark();
-  pl.qc.resize(x.n);
-  HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
-  pl.qab = schwarz_bra_finish(pl.ps, spend);
-  for (const double q : pl.qc)
-    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
-  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
-  vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
-
   // Cholesky of the metric bordered by the identity (common/linalg.f90:78 and
   // the dtrtri of diabatization/intor.f90:198) on the lane's side stream,
   // concurrent with the integrals: it touches only rows < 2 naux, the
@@ -1148,6 +1142,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipGetLastError());
     HC(hipEventRecord(lane.ev_join, lane.st2));
   }
+  pl.qc.resize(x.n);
+  HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  pl.qab = schwarz_bra_finish(pl.ps, spend);
+  for (const double q : pl.qc)
+    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
+  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
+  vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
+
   const int e4 = e3;
   // verbose >= 2: wall time of the steps between the metric and the integrals
   // (each step synchronized first, so the times add up)
@@ -1525,12 +1527,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (need_dens) {
     HC(hipStreamWaitEvent(st, lane.ev_join, 0));
     e7 = tm.mark();
-    HC(hipStreamSynchronize(st));
-    int hinfo = 0;
-    HC(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (hinfo != 0)
-      throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
-                                         std::to_string(hinfo) + ")"};
+    // the status is read after the final wait: no stop here between the join
+    // and the products
+    HC(hipMemcpyAsync(lane.hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
     dBT.alloc((size_t)naux * ndens);
     DBuf<double> dLp((size_t)apply_linv_part_size(naux, ndens));
     apply_linv(Ap + 2 * naux, lda, Ap + naux, lda, naux, ndens, dLp.p, dBT.p, st);
@@ -1556,6 +1555,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (out.J) HC(hipMemcpyAsync(out.J, dJ.p, dJ.n * sizeof(double), hipMemcpyDeviceToHost, st));
   const int e9 = tm.mark();
   HC(hipStreamSynchronize(st));
+  if (need_dens && lane.hinfo[0] != 0)
+    throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
+                                       std::to_string(lane.hinfo[0]) + ")"};
 
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
