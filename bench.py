@@ -134,6 +134,9 @@ def main():
     ap.add_argument("--fp32-below", type=float, default=None, help="mixed path: FP32 below this Schwarz product")
     ap.add_argument("--batch", action="store_true",
                     help="the batched form instead: 256 molecules of 5-40 atoms, 4 states, one call")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a 2 GiB device-to-device copy: the reference run that scales the FETCH_SIZE and "
+                         "WRITE_SIZE counters of the roofline's traffic figure")
     ap.add_argument("--workload", type=int, default=1, choices=(1, 2, 4),
                     help="1: 60 atoms s-d / aux s-f, 8 states (default); 2: 60 atoms s-f / aux s-g, 8 states; "
                          "4: 200 atoms s-g / aux s-h, 20 states, QVL 1e-10, densities generated on the device")
@@ -149,6 +152,12 @@ def main():
     from pysisyphus_addons_amd import _native as N
     from pysisyphus_addons_amd import flops, synthetic
 
+    if args.copy_calibration:
+        x = torch.ones(2 ** 28, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x).copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": x.numel() * 8, "check": float(y[-1])}), flush=True)
+        return None
     if args.batch:
         return bench_batch(args)
 
