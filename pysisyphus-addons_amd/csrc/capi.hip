@@ -247,12 +247,35 @@ void build_boys_table(std::vector<double>& tab) {
 // lane; a batch runs its molecules concurrently on several lanes, one host
 // thread each.  Blocks are only reused within their lane, i.e. by work queued
 // on the same stream after the work of their previous owner.
+// Pinned host buffer, grown on demand: a device-to-host copy into it returns
+// at once, one into pageable memory waits for the stream to reach it.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* get(size_t n) {
+    const size_t need = std::max<size_t>(n, 1) * sizeof(T);
+    if (need > bytes) {
+      release();
+      HC(hipHostMalloc(&p, need, hipHostMallocDefault));
+      bytes = need;
+    }
+    return (T*)p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 struct Lane {
   hipStream_t st = nullptr;
   // side stream of the lane: the metric factorization runs there while the
   // integrals run on st (created on first use)
   hipStream_t st2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // (and the density packing before it, concurrent with the Schwarz norms)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_pack = nullptr;
   int* hinfo = nullptr;  // pinned host copy of the factorization's status
   void ensure_side() {
     if (st2) return;
@@ -260,12 +283,25 @@ struct Lane {
     HC(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     HC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    HC(hipEventCreateWithFlags(&ev_pack, hipEventDisableTiming));
+  }
+  // the Schwarz partial sums and the aux norms come back through pinned
+  // buffers, each signalled by its own event, so that the host can build the
+  // pair part of the plan while the metric still runs
+  hipEvent_t ev_sch = nullptr, ev_met = nullptr;
+  PinnedBuf hpart, hqc;
+  void ensure_events() {
+    if (ev_sch) return;
+    HC(hipEventCreateWithFlags(&ev_sch, hipEventDisableTiming));
+    HC(hipEventCreateWithFlags(&ev_met, hipEventDisableTiming));
   }
   DevPool pool;
   // device workspace reused across calls (grown on demand, freed by
   // dfhip_release_workspace)
   DBuf<double> wsT, wsA, wsDp, wsPart, wsDraw;
   void release() {
+    hpart.release();
+    hqc.release();
     wsT.release();
     wsA.release();
     wsDp.release();
@@ -438,10 +474,10 @@ struct Timer {
   hipStream_t st;
   std::vector<hipEvent_t> ev;
   explicit Timer(hipStream_t s) : st(s) {}
-  int mark() {
+  int mark(hipStream_t on = nullptr) {
     hipEvent_t e;
     HC(hipEventCreate(&e));
-    HC(hipEventRecord(e, st));
+    HC(hipEventRecord(e, on ? on : st));
     ev.push_back(e);
     return (int)ev.size() - 1;
   }
@@ -564,15 +600,20 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
         c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       HC(hipGetLastError());
     }
+  Lane& lane = cur_lane();
+  lane.ensure_events();
+  double* hp = lane.hpart.get<double>(poff[NPCLS]);
+  if (poff[NPCLS] > 0) HC(hipMemcpyAsync(hp, part.p, poff[NPCLS] * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipEventRecord(lane.ev_sch, st));
 }
 
+// Waits for the Schwarz kernels only (work queued behind them may still run).
 std::vector<double> schwarz_bra_finish(const PairSet& ps, SchwarzPending& sp) {
-  hipStream_t st = cur_lane().st;
+  Lane& lane = cur_lane();
   const std::vector<size_t>& poff = sp.poff;
   const std::vector<int>& nch = sp.nch;
-  std::vector<double> hp(poff[NPCLS]);
-  if (!hp.empty()) HC(hipMemcpyAsync(hp.data(), sp.part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  HC(hipStreamSynchronize(st));
+  HC(hipEventSynchronize(lane.ev_sch));
+  const double* hp = (const double*)lane.hpart.p;
   std::vector<double> q(ps.n());
   for (int k = 0; k < NPCLS; k++)
     for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++) {
@@ -615,7 +656,9 @@ void launch_bra_records(int la, int lb, DevShells bra, const int* s1, const int*
   if (npairs <= 0) return;
 #define DFHIP_BR(A, B)                                                                                          \
   if (la == A && lb == B) {                                                                                    \
-    hipLaunchKernelGGL((bra_records_kernel<A, B>), dim3(npairs), dim3(64), 0, st, bra, s1, s2, recoff, rs, recs); \
+    if (rs != 4 + BraRecMap<A, B>::N) throw Error{DFHIP_ERR_HIP, "bra record size mismatch"};                    \
+    hipLaunchKernelGGL((bra_records_kernel<A, B>), dim3(npairs), dim3(BRA_THREADS), 0, st, bra, s1, s2, recoff, rs, \
+                       recs);                                                                                  \
     return;                                                                                                    \
   }
   DFHIP_BR(0, 0) DFHIP_BR(1, 0) DFHIP_BR(1, 1) DFHIP_BR(2, 0) DFHIP_BR(2, 1) DFHIP_BR(2, 2)
@@ -730,7 +773,8 @@ void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, in
       }
     }
   }
-  HC(hipStreamSynchronize(st));
+  // no wait: the buffers above go back to the lane's pool, whose blocks are
+  // only reused by work queued behind these kernels
 }
 
 dfhip_options defaults() {
@@ -741,7 +785,7 @@ dfhip_options defaults() {
   o.thresh = (double)1e-10f;  // diabatization/intor.f90:140 (real(4) literal)
   o.dens_layout = DFHIP_DENS_FORTRAN;
   const char* v = std::getenv("DFHIP_VERBOSE");
-  o.verbose = (v && *v && *v != '0') ? 1 : 0;
+  o.verbose = (v && *v) ? std::max(0, std::atoi(v)) : 0;
   o.max_tile_bytes = 0;
   o.fp32_below = 1e-3;
   return o;
@@ -755,6 +799,7 @@ struct Plan {
   PairSet ps;              // sorted by Q_ab descending inside each class
   std::vector<double> qab, qc;
   std::vector<int64_t> prow;            // per pair, row of its first function
+  std::vector<int> from;                // per pair, its index in the make_pairs order
   int64_t npf = 0;
   // aux lists per L, sorted by Q_C descending; permuted column of each list entry
   std::vector<std::vector<int>> alist;
@@ -839,22 +884,24 @@ void run_tasks(int n, const std::vector<char>& big, F fn) {
   for (auto& t : th) t.join();
 }
 
-void build_plan(Plan& pl, const dfhip_options& opt) {
-  const HostShells& o = pl.orb;
-  const HostShells& x = pl.aux;
-  const int np = pl.ps.n();
-  // pair classes of at least 4096 pairs are sorted, and on such systems the
-  // aux L swept, on their own host threads
-  std::vector<char> big_cls(NPCLS, 0), big_l(LMAX_AUX + 1, 0);
+// Pair classes of at least 4096 pairs are sorted, and on such systems the aux
+// L swept, on their own host threads.
+bool big_system(const PairSet& ps, std::vector<char>* big_cls) {
+  std::vector<char> b(NPCLS, 0);
   int nbig = 0;
-  for (int k = 0; k < NPCLS; k++) nbig += big_cls[k] = pl.ps.cls_end[k] - pl.ps.cls_begin[k] >= 4096;
-  if (nbig > 1) {
-    std::vector<int> nl(LMAX_AUX + 1, 0);
-    for (int i = 0; i < x.n; i++) nl[x.L[i]]++;
-    for (int L = 0; L <= LMAX_AUX; L++) big_l[L] = nl[L] > 0;
-  } else {
-    std::fill(big_cls.begin(), big_cls.end(), 0);
-  }
+  for (int k = 0; k < NPCLS; k++) nbig += b[k] = ps.cls_end[k] - ps.cls_begin[k] >= 4096;
+  if (nbig <= 1) std::fill(b.begin(), b.end(), 0);
+  if (big_cls) *big_cls = b;
+  return nbig > 1;
+}
+
+// Pair part of the plan (needs Q_ab only): pairs sorted inside each class,
+// their first rows in the staging matrix.
+void build_plan_pairs(Plan& pl) {
+  const HostShells& o = pl.orb;
+  const int np = pl.ps.n();
+  std::vector<char> big_cls;
+  big_system(pl.ps, &big_cls);
   // sort pairs inside each class by Q_ab descending (stable on packed index)
   PairSet sorted;
   sorted.cls_begin = pl.ps.cls_begin;
@@ -863,6 +910,7 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
   sorted.s1.resize(np);
   sorted.s2.resize(np);
   sorted.packed.resize(np);
+  pl.from.resize(np);
   run_tasks(NPCLS, big_cls, [&](int k) {
     // descending Q_ab, ties in the original (packed) order
     const int p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
@@ -875,6 +923,7 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
       sorted.s1[p0 + r] = pl.ps.s1[p];
       sorted.s2[p0 + r] = pl.ps.s2[p];
       sorted.packed[p0 + r] = pl.ps.packed[p];
+      pl.from[p0 + r] = p;
       q2[p0 + r] = pl.qab[p];
     }
   });
@@ -887,6 +936,20 @@ void build_plan(Plan& pl, const dfhip_options& opt) {
     r += (int64_t)(2 * o.L[pl.ps.s1[p]] + 1) * (2 * o.L[pl.ps.s2[p]] + 1);
   }
   pl.npf = r;
+}
+
+// Aux part of the plan (needs Q_C and the sorted pairs): aux lists sorted by
+// Q_C, their columns, the screened prefixes and the triplet counts.
+void build_plan_aux(Plan& pl, const dfhip_options& opt) {
+  const HostShells& o = pl.orb;
+  const HostShells& x = pl.aux;
+  const int np = pl.ps.n();
+  std::vector<char> big_l(LMAX_AUX + 1, 0);
+  if (big_system(pl.ps, nullptr)) {
+    std::vector<int> nl(LMAX_AUX + 1, 0);
+    for (int i = 0; i < x.n; i++) nl[x.L[i]]++;
+    for (int L = 0; L <= LMAX_AUX; L++) big_l[L] = nl[L] > 0;
+  }
   // aux lists
   pl.alist.assign(LMAX_AUX + 1, {});
   pl.acol.assign(LMAX_AUX + 1, {});
@@ -1097,16 +1160,99 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   daux.upload_from(x, st);
   const int e1 = tm.mark();
 
+  Lane& lane = cur_lane();
+  if (need_dens) lane.ensure_side();
+  DBuf<int> info(1);
+  // on every way out of run() (errors included) the lane's streams are
+  // drained before Ap, info, the pinned buffers or the pool blocks can be
+  // reused
+  struct StreamDrain {
+    Lane& l;
+    bool side = false;
+    ~StreamDrain() {
+      (void)hipStreamSynchronize(l.st);
+      if (side) (void)hipStreamSynchronize(l.st2);
+    }
+  } drain{lane, need_dens};
+
   // Schwarz norms of the shell pairs (common/integrals/intor.f90:131-160);
-  // the metric kernels queue behind them, their results are read after both
+  // the metric kernels queue behind them
   pl.ps = make_pairs(o);
   SchwarzPending spend;
   schwarz_bra_launch(c, o, dorb.view, pl.ps, spend);
   const int e2 = tm.mark();
 
-  // metric (common/integrals/intor.f90:40-95) in the leading block of the
-  // bordered matrix [[M, .], [d^T, 0]] (column-major, lda = naux + ndens),
-  // and Q_C from its diagonal blocks
+  // densities: upload and pair-pack (diabatization/intor.f90:155-176
+  // symmetry) on the side stream, concurrent with the Schwarz kernels.  The
+  // packed rows follow the make_pairs order; the contraction reads them
+  // through rowperm once the pairs are sorted.
+  double* Dp = nullptr;
+  int h2d0 = -1, h2d1 = -1;
+  std::vector<int64_t> prow0(pl.ps.n());
+  int64_t npf0 = 0;
+  for (int p = 0; p < pl.ps.n(); p++) {
+    prow0[p] = npf0;
+    npf0 += (int64_t)(2 * o.L[pl.ps.s1[p]] + 1) * (2 * o.L[pl.ps.s2[p]] + 1);
+  }
+  if (npf0 >= ((int64_t)1 << 31)) throw Error{DFHIP_ERR_UNSUPPORTED, "more than 2^31 pair functions"};
+  // device arrays of the side stream's packing: they live until run() ends,
+  // so that no work on st gets their blocks while the packing reads them
+  DBuf<int> ps1_0, ps2_0, rowpair0, dpairof, dfshell;
+  DBuf<int64_t> prow0_d;
+  if (need_dens) {
+    const hipStream_t sp = lane.st2;
+    HC(hipStreamWaitEvent(sp, tm.ev[e1], 0));  // the shell arrays are on the device
+    const double* Dsrc = dens;
+    if (!opt.dens_on_device) {
+      const size_t nd = (size_t)ndens * nbfs * nbfs;
+      double* Draw = reserve(cur_lane().wsDraw, nd);
+      h2d0 = tm.mark(sp);
+      HC(hipMemcpyAsync(Draw, dens, nd * sizeof(double), hipMemcpyHostToDevice, sp));
+      h2d1 = tm.mark(sp);
+      Dsrc = Draw;
+    }
+    int64_t sI, si, sj;
+    if (opt.dens_layout == DFHIP_DENS_C) {
+      sI = (int64_t)nbfs * nbfs;
+      si = nbfs;
+      sj = 1;
+    } else {
+      sI = 1;
+      si = ndens;
+      sj = (int64_t)ndens * nbfs;
+    }
+    Dp = reserve(cur_lane().wsDp, (size_t)npf0 * ndens);
+    upload(prow0_d, prow0, sp);
+    if (sj == 1 && si == nbfs && ndens > 48) {
+      // density index slowest and many densities: tiled transpose (pair lookup
+      // by shell pair, shell by function).  With few densities the packing
+      // by rows below is faster.
+      std::vector<int> pairof((size_t)o.n * o.n), fshell(o.nbf);
+      for (int p = 0; p < pl.ps.n(); p++) {
+        const int a = pl.ps.s1[p], b = pl.ps.s2[p];
+        pairof[(size_t)a * o.n + b] = p | (1 << 30);
+        if (a != b) pairof[(size_t)b * o.n + a] = p;
+      }
+      for (int sh = 0; sh < o.n; sh++)
+        for (int m = 0; m < 2 * o.L[sh] + 1; m++) fshell[o.f0[sh] + m] = sh;
+      upload(dpairof, pairof, sp);
+      upload(dfshell, fshell, sp);
+      pack_density_t(dpairof.p, dfshell.p, prow0_d.p, dorb.L.p, dorb.f0.p, o.n, o.nbf, Dsrc, sI, ndens, Dp, ndens,
+                     sp);
+    } else {
+      upload(ps1_0, pl.ps.s1, sp);
+      upload(ps2_0, pl.ps.s2, sp);
+      rowpair0.alloc((size_t)std::max<int64_t>(npf0, 1));
+      fill_rowpair(prow0_d.p, pl.ps.n(), npf0, rowpair0.p, nullptr, nullptr, sp);
+      pack_density(rowpair0.p, ps1_0.p, ps2_0.p, prow0_d.p, npf0, dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp,
+                   ndens, sp);
+    }
+    HC(hipGetLastError());
+    HC(hipEventRecord(lane.ev_pack, sp));
+  }
+
+  // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal
+  // blocks, queued behind the Schwarz kernels.
   // A = [[M], [I], [D^T]]: naux columns, rows 0..naux-1 the metric, then the
   // identity border whose factorization gives L^-T, then d^T (ndens rows)
   const int64_t lda = need_dens ? 2 * naux + ndens : naux;
@@ -1116,24 +1262,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   DBuf<double> dqc(x.n);
   schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
   const int e3 = tm.mark();
+  double* hqc = lane.hqc.get<double>(x.n);
+  HC(hipMemcpyAsync(hqc, dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipEventRecord(lane.ev_met, st));
   // Cholesky of the metric bordered by the identity (common/linalg.f90:78 and
   // the dtrtri of diabatization/intor.f90:198) on the lane's side stream,
   // concurrent with the integrals: it touches only rows < 2 naux, the
   // contraction writes d^T below them.  Ap and info outlive the join.
-  DBuf<int> info(1);
-  Lane& lane = cur_lane();
-  // on every way out of run() (errors included) the side stream is drained
-  // before Ap, info or the pool blocks can be reused
-  struct SideDrain {
-    Lane& l;
-    bool on = false;
-    ~SideDrain() {
-      if (on) (void)hipStreamSynchronize(l.st2);
-    }
-  } drain{lane};
   if (need_dens) {
-    lane.ensure_side();
-    drain.on = true;
     set_identity_rows(Ap, lda, naux, st);
     HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
     HC(hipEventRecord(lane.ev_fork, st));
@@ -1142,17 +1278,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipGetLastError());
     HC(hipEventRecord(lane.ev_join, lane.st2));
   }
-  pl.qc.resize(x.n);
-  HC(hipMemcpyAsync(pl.qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
-  pl.qab = schwarz_bra_finish(pl.ps, spend);
-  for (const double q : pl.qc)
-    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
-  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
-  vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
 
-  const int e4 = e3;
-  // verbose >= 2: wall time of the steps between the metric and the integrals
-  // (each step synchronized first, so the times add up)
+  // verbose >= 2: wall time of the host steps up to the integrals (each step
+  // synchronized first, so the times add up)
   auto step_t0 = std::chrono::steady_clock::now();
   auto vstep = [&](const char* what) {
     if (opt.verbose < 2) return;
@@ -1161,23 +1289,126 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     vlog(opt, "  %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(t - step_t0).count());
     step_t0 = t;
   };
-  vstep("Schwarz/metric results, fork");
-
+  // pair part of the plan as soon as the Schwarz norms are back, while the
+  // metric runs; the packing and the bra records queue behind the metric
+  pl.qab = schwarz_bra_finish(pl.ps, spend);
+  vstep("Schwarz norms");
   const auto hp0 = std::chrono::steady_clock::now();
-  build_plan(pl, opt);
-  const double ms_plan =
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
-  vlog(opt, "Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
-  vlog(opt, "Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
-  vlog(opt, " Densities: %6d\n", ndens);
-  vstep("host plan");
-
-  // device plan arrays
+  build_plan_pairs(pl);
+  double ms_plan = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
+  vstep("pair plan");
   DBuf<int> ps1, ps2;
   DBuf<int64_t> prow;
   upload(ps1, pl.ps.s1, st);
   upload(ps2, pl.ps.s2, st);
   upload(prow, pl.prow, st);
+  // per staging-matrix row (sorted pairs): its pair, for the contraction's
+  // mask, and the packed density row it reads
+  DBuf<int> drowpair, drowperm;
+  DBuf<int64_t> ddprow;
+  if (need_dens) {
+    if (pl.npf != npf0) throw Error{DFHIP_ERR_HIP, "pair-function count changed by the sort"};
+    std::vector<int64_t> dprow(pl.ps.n());
+    for (int p = 0; p < pl.ps.n(); p++) dprow[p] = prow0[pl.from[p]];
+    upload(ddprow, dprow, st);
+    drowpair.alloc((size_t)std::max<int64_t>(pl.npf, 1));
+    drowperm.alloc((size_t)std::max<int64_t>(pl.npf, 1));
+    fill_rowpair(prow.p, pl.ps.n(), pl.npf, drowpair.p, ddprow.p, drowperm.p, st);
+  }
+  const bool gen = use_generated();
+  std::vector<BraRecords> brs(NPCLS);
+  if (gen) {
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
+        if (p1 == p0 || !gen3_has(la, lb, 0)) continue;
+        std::vector<int> a1(pl.ps.s1.begin() + p0, pl.ps.s1.begin() + p1);
+        std::vector<int> a2(pl.ps.s2.begin() + p0, pl.ps.s2.begin() + p1);
+        build_bra_records(c, la, lb, o, dorb.view, ps1.p + p0, ps2.p + p0, a1, a2, brs[k], mixed);
+      }
+  }
+  vstep("bra records");
+
+  // aux part of the plan once Q_C is back (the packing may still run)
+  HC(hipEventSynchronize(lane.ev_met));
+  pl.qc.assign(hqc, hqc + x.n);
+  for (const double q : pl.qc)
+    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
+
+  // Et tables of the classes run on the matrix-core 3c2e kernels: the pairs
+  // with screened-in triplets in any such aux L, built once for all column
+  // blocks (a class whose tables would take more than a sixth of the free
+  // HBM, the staging matrix this lane holds counted as free, keeps the LDS
+  // kernels).  need(k, L, p): pair p (global index) of class k has
+  // screened-in triplets with aux L.
+  std::vector<DBuf<double>> etg(NPCLS);
+  std::vector<DBuf<int64_t>> etoff(NPCLS);
+  std::vector<std::vector<char>> use_mm(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
+  auto build_et = [&](auto need) {
+    if (!use_matrix_cores_3c()) return;
+    size_t fr = 0, tot = 0;
+    HC(hipMemGetInfo(&fr, &tot));
+    const size_t et_cap = (fr + cur_lane().wsT.n * sizeof(double)) / 6;
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
+        if (np == 0 || c.et[la][lb].fn == nullptr) continue;
+        bool any = false;
+        for (int L = 0; L <= LMAX_AUX; L++) {
+          use_mm[k][L] = (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
+          any = any || use_mm[k][L];
+        }
+        if (!any) continue;
+        std::vector<int> list;
+        std::vector<int64_t> off(np, 0);
+        int64_t totd = 0;
+        for (int p = 0; p < np; p++) {
+          bool nd = false;
+          for (int L = 0; L <= LMAX_AUX && !nd; L++) nd = use_mm[k][L] && need(k, L, p0 + p);
+          if (!nd) continue;
+          list.push_back(p);
+          off[p] = totd;
+          totd += (int64_t)o.nprim[pl.ps.s1[p0 + p]] * o.nprim[pl.ps.s2[p0 + p]] * c.et[la][lb].etpp;
+        }
+        if ((size_t)totd * sizeof(double) > et_cap) {
+          for (int L = 0; L <= LMAX_AUX; L++) use_mm[k][L] = 0;
+          continue;
+        }
+        if (list.empty()) continue;
+        DBuf<int> dlist;
+        upload(dlist, list, st);
+        upload(etoff[k], off, st);
+        etg[k].alloc((size_t)totd);
+        c.et[la][lb].fn(dorb.view, ps1.p + p0, ps2.p + p0, dlist.p, etoff[k].p, (int)list.size(), c.sph.p, etg[k].p,
+                        st);
+        HC(hipGetLastError());
+      }
+  };
+  // Under Schwarz screening (or none) the pairs with screened-in triplets
+  // are known before the aux plan: nsurv[L][p] > 0 exactly when Q_ab times
+  // the largest Q_C of aux L passes, the test of the plan's sweep; the Et
+  // kernels then run while the host builds the aux plan.
+  bool et_done = false;
+  if (opt.screening != DFHIP_SCREEN_QVL) {
+    std::vector<double> qcmax(LMAX_AUX + 1, -1.0);
+    for (int i = 0; i < x.n; i++) qcmax[x.L[i]] = std::max(qcmax[x.L[i]], pl.qc[i]);
+    const bool all = opt.screening == DFHIP_SCREEN_NONE;
+    build_et([&](int k, int L, int p) {
+      return qcmax[L] >= 0.0 && (all || !(pl.qab[p] * qcmax[L] < opt.thresh));
+    });
+    et_done = true;
+  }
+  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
+  vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
+  const auto hp1 = std::chrono::steady_clock::now();
+  build_plan_aux(pl, opt);
+  ms_plan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp1).count();
+  vlog(opt, "Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
+  vlog(opt, "Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
+  vlog(opt, " Densities: %6d\n", ndens);
+  vstep("aux plan");
+
+  // device plan arrays
   std::vector<DBuf<int>> dalist(LMAX_AUX + 1);
   std::vector<DBuf<int64_t>> dacol(LMAX_AUX + 1);
   std::vector<int> kmax(LMAX_AUX + 1, 1);
@@ -1199,7 +1430,6 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     upload(dnsflat, nsflat, st);
   }
   auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
-  const bool gen = use_generated();
   DBuf<int> dn64flat;  // FP64 windows of the prefixes, [L][pair] like dnsflat
   if (mixed) {
     split_precision(pl, opt.fp32_below, gen);
@@ -1211,69 +1441,6 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   DBuf<int> dorig;
   upload(dorig, pl.orig, st);
   vstep("plan uploads");
-  std::vector<BraRecords> brs(NPCLS);
-  if (gen) {
-    for (int la = 0; la <= LMAX_ORB; la++)
-      for (int lb = 0; lb <= la; lb++) {
-        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
-        if (p1 == p0 || !gen3_has(la, lb, 0)) continue;
-        std::vector<int> a1(pl.ps.s1.begin() + p0, pl.ps.s1.begin() + p1);
-        std::vector<int> a2(pl.ps.s2.begin() + p0, pl.ps.s2.begin() + p1);
-        build_bra_records(c, la, lb, o, dorb.view, ps1.p + p0, ps2.p + p0, a1, a2, brs[k], mixed);
-      }
-  }
-
-  vstep("bra records");
-  // densities: upload and pair-pack (diabatization/intor.f90:155-176 symmetry)
-  double* Dp = nullptr;
-  double ms_h2d = 0.0;
-  if (need_dens) {
-    const double* Dsrc = dens;
-    if (!opt.dens_on_device) {
-      const size_t nd = (size_t)ndens * nbfs * nbfs;
-      double* Draw = reserve(cur_lane().wsDraw, nd);
-      const int h0 = tm.mark();
-      HC(hipMemcpyAsync(Draw, dens, nd * sizeof(double), hipMemcpyHostToDevice, st));
-      const int h1 = tm.mark();
-      HC(hipStreamSynchronize(st));
-      ms_h2d = tm.ms(h0, h1);
-      Dsrc = Draw;
-    }
-    int64_t sI, si, sj;
-    if (opt.dens_layout == DFHIP_DENS_C) {
-      sI = (int64_t)nbfs * nbfs;
-      si = nbfs;
-      sj = 1;
-    } else {
-      sI = 1;
-      si = ndens;
-      sj = (int64_t)ndens * nbfs;
-    }
-    Dp = reserve(cur_lane().wsDp, (size_t)pl.npf * ndens);
-    if (sj == 1 && si == nbfs && ndens > 48) {
-      // density index slowest and many densities: tiled transpose (pair lookup
-      // by shell pair, shell by function).  With few densities the direct
-      // packing below reads runs along nu and is faster (0.31 against 0.45 ms
-      // at 36 densities of 840 functions).
-      std::vector<int> pairof((size_t)o.n * o.n), fshell(o.nbf);
-      for (int p = 0; p < pl.ps.n(); p++) {
-        const int a = pl.ps.s1[p], b = pl.ps.s2[p];
-        pairof[(size_t)a * o.n + b] = p | (1 << 30);
-        if (a != b) pairof[(size_t)b * o.n + a] = p;
-      }
-      for (int sh = 0; sh < o.n; sh++)
-        for (int m = 0; m < 2 * o.L[sh] + 1; m++) fshell[o.f0[sh] + m] = sh;
-      DBuf<int> dpairof, dfshell;
-      upload(dpairof, pairof, st);
-      upload(dfshell, fshell, st);
-      pack_density_t(dpairof.p, dfshell.p, prow.p, dorb.L.p, dorb.f0.p, o.n, o.nbf, Dsrc, sI, ndens, Dp, ndens, st);
-    } else {
-      pack_density(ps1.p, ps2.p, prow.p, pl.ps.n(), dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp, ndens, st);
-    }
-    // no wait here: the host sets up the integrals while the packing runs
-    HC(hipGetLastError());
-  }
-  vstep("density packing");
   const int e5 = tm.mark();
 
   // column blocks of the (pair function x aux function) staging matrix
@@ -1308,11 +1475,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   int nsplit = 1;
   // screened-prefix mask of T (the 3c2e kernels write only screened-in blocks)
   ScreenMask mk{};
-  DBuf<int> drowpair, dcolL, dcolia;
+  DBuf<int> dcolL, dcolia;
   if (need_dens) {
     std::vector<int> colL(std::max<int64_t>(naux, 1)), colia(std::max<int64_t>(naux, 1));
-    drowpair.alloc((size_t)std::max<int64_t>(pl.npf, 1));
-    fill_rowpair(prow.p, pl.ps.n(), pl.npf, drowpair.p, st);
     for (int L = 0; L <= LMAX_AUX; L++)
       for (size_t k = 0; k < pl.acol[L].size(); k++)
         for (int m = 0; m < 2 * L + 1; m++) {
@@ -1321,8 +1486,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         }
     upload(dcolL, colL, st);
     upload(dcolia, colia, st);
-    mk = ScreenMask{drowpair.p, dcolL.p, dcolia.p, dnsflat.p, pl.ps.n()};
+    mk = ScreenMask{drowpair.p, drowperm.p, dcolL.p, dcolia.p, dnsflat.p, pl.ps.n()};
   }
+  vstep("column blocks, mask");
   if (need_dens) {
     const int64_t tiles = contract_ntiles(widest, ndens);
     nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(512, 2048 / std::max<int64_t>(tiles, 1)));
@@ -1333,63 +1499,23 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // rows from the class start to the last pair whose screened prefix reaches
   // the tile (under Schwarz screening these pairs are exactly a prefix of
   // the class: pairs are sorted by Q_ab and the prefixes are non-increasing).
-  // Et tables of the classes run on the matrix-core 3c2e kernels: the pairs
-  // with screened-in triplets in any such aux L, built once for all column
-  // blocks (a class whose tables would take more than a third of free HBM
-  // keeps the LDS kernels)
-  std::vector<DBuf<double>> etg(NPCLS);
-  std::vector<DBuf<int64_t>> etoff(NPCLS);
-  std::vector<std::vector<char>> use_mm(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
-  if (use_matrix_cores_3c()) {
-    size_t fr = 0, tot = 0;
-    HC(hipMemGetInfo(&fr, &tot));
-    const size_t et_cap = fr / 3;
-    for (int la = 0; la <= LMAX_ORB; la++)
-      for (int lb = 0; lb <= la; lb++) {
-        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
-        if (np == 0 || c.et[la][lb].fn == nullptr) continue;
-        bool any = false;
-        for (int L = 0; L <= LMAX_AUX; L++) {
-          use_mm[k][L] = (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
-          any = any || use_mm[k][L];
-        }
-        if (!any) continue;
-        std::vector<int> list;
-        std::vector<int64_t> off(np, 0);
-        int64_t totd = 0;
-        for (int p = 0; p < np; p++) {
-          bool need = false;
-          for (int L = 0; L <= LMAX_AUX && !need; L++) need = use_mm[k][L] && pl.nsurv[L][p0 + p] > 0;
-          if (!need) continue;
-          list.push_back(p);
-          off[p] = totd;
-          totd += (int64_t)o.nprim[pl.ps.s1[p0 + p]] * o.nprim[pl.ps.s2[p0 + p]] * c.et[la][lb].etpp;
-        }
-        if ((size_t)totd * sizeof(double) > et_cap) {
-          for (int L = 0; L <= LMAX_AUX; L++) use_mm[k][L] = 0;
-          continue;
-        }
-        if (list.empty()) continue;
-        DBuf<int> dlist;
-        upload(dlist, list, st);
-        upload(etoff[k], off, st);
-        etg[k].alloc((size_t)totd);
-        c.et[la][lb].fn(dorb.view, ps1.p + p0, ps2.p + p0, dlist.p, etoff[k].p, (int)list.size(), c.sph.p, etg[k].p,
-                        st);
-        HC(hipGetLastError());
-      }
-  }
-
+  if (!et_done) build_et([&](int k, int L, int p) { return pl.nsurv[L][p] > 0; });
+  vstep("Et tables");
   const int GT = contract_tile();
-  // suffix maxima of the prefixes inside each class: equal to nsurv under
-  // Schwarz screening (non-increasing), an upper envelope under QVL
-  std::vector<std::vector<int>> nsmax(LMAX_AUX + 1);
-  for (int L = 0; L <= LMAX_AUX; L++) {
-    nsmax[L] = pl.nsurv[L];
-    for (int k = 0; k < NPCLS; k++)
-      for (int p = pl.ps.cls_end[k] - 2; p >= pl.ps.cls_begin[k]; p--)
-        nsmax[L][p] = std::max(nsmax[L][p], nsmax[L][p + 1]);
+  // suffix maxima of the prefixes inside each class: under Schwarz screening
+  // (or none) the prefixes are non-increasing and are their own suffix
+  // maxima; under QVL an upper envelope
+  std::vector<std::vector<int>> nsmax_qvl;
+  if (use_qvl) {
+    nsmax_qvl.assign(LMAX_AUX + 1, {});
+    for (int L = 0; L <= LMAX_AUX; L++) {
+      nsmax_qvl[L] = pl.nsurv[L];
+      for (int k = 0; k < NPCLS; k++)
+        for (int p = pl.ps.cls_end[k] - 2; p >= pl.ps.cls_begin[k]; p--)
+          nsmax_qvl[L][p] = std::max(nsmax_qvl[L][p], nsmax_qvl[L][p + 1]);
+    }
   }
+  const std::vector<std::vector<int>>& nsmax = use_qvl ? nsmax_qvl : pl.nsurv;
   auto tile_segments = [&](int64_t cA, int64_t w, std::vector<int64_t>& seg) {
     const int ntile = (int)((w + GT - 1) / GT);
     seg.assign((size_t)ntile * (2 * NPCLS + 1), 0);
@@ -1481,6 +1607,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     }
     const int b1 = tm.mark();
     if (need_dens) {
+      // the packed densities come from the side stream
+      if (blk == 0) HC(hipStreamWaitEvent(st, lane.ev_pack, 0));
       // d^T into the border rows: A[(naux + I) + lda * P]
       tile_segments(cA, w, hseg);
       upload(dseg, hseg, st);
@@ -1573,15 +1701,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     stats->ms_schwarz = tm.ms(e1, e2);
     stats->ms_metric = tm.ms(e2, e3);
     stats->ms_cholesky = tm.ms(e6, e7);
-    stats->ms_pack = tm.ms(e4, e5);
+    stats->ms_pack = tm.ms(e3, e5);
     stats->ms_int3c2e = ms_int;
     stats->ms_contract = ms_con;
     stats->ms_solve = 0.0;
     stats->ms_assemble = tm.ms(e7, e8);
-    (void)e4;
     stats->ms_download = tm.ms(e8, e9);
     stats->ms_total = tm.ms(e0, e9);
-    stats->ms_h2d_densities = ms_h2d;
+    stats->ms_h2d_densities = h2d0 >= 0 ? tm.ms(h2d0, h2d1) : 0.0;
     stats->ms_host_plan = ms_plan;
     for (int k = 0; k < NPCLS * 6; k++) {
       stats->class_triplets[k] = pl.class_trip[k];

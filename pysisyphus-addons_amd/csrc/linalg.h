@@ -10,21 +10,25 @@
 
 namespace dfhip {
 
-void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, const int* L, const int* f0,
-                  const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
+// Dp[r * ldd + I] for the npf rows r (pair functions; rowpair[r] = pair of r)
+void pack_density(const int* rowpair, const int* s1, const int* s2, const int64_t* row, int64_t npf, const int* L,
+                  const int* f0, const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
                   hipStream_t st);
 // The packing for the C layout (density index slowest) as a tiled transpose;
 // pairof[s*nsh + t] = pair index of {s, t} | (1 << 30 if oriented (s, t)),
 // fshell[j] = shell of function j.
 void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, const int* L, const int* f0, int nsh,
                     int nbf, const double* D, int64_t sI, int ndens, double* Dp, int ldd, hipStream_t st);
-// rowpair[r] = p for the rows of pair p (row[p] .. row[p+1]-1, the last ends at npf)
-void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, hipStream_t st);
+// rowpair[r] = p for the rows of pair p (row[p] .. row[p+1]-1, the last ends
+// at npf); with dprow: rowperm[r] = dprow[p] + r - row[p]
+void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, const int64_t* dprow, int* rowperm,
+                  hipStream_t st);
 // Screened-prefix mask of the staging matrix T: entry (row k, column P) is
 // live iff colia[P] < nsurv[colL[P] * npairs + rowpair[k]] (column arrays
 // relative to the column block).
 struct ScreenMask {
   const int* rowpair;
+  const int* rowperm;  // packed density row of each staging-matrix row
   const int* colL;
   const int* colia;
   const int* nsurv;

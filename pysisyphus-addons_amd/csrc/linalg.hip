@@ -17,31 +17,45 @@ namespace dfhip {
 //   D_I(f1+mu, f2+nu) + D_I(f2+nu, f1+mu)   (s1 != s2)
 //   D_I(f1+mu, f2+nu)                        (s1 == s2)
 // (the symmetry use of diabatization/intor.f90:162-169).
-__global__ void pack_density_kernel(const int* __restrict__ s1v, const int* __restrict__ s2v,
-                                    const int64_t* __restrict__ rowv, const int* __restrict__ Lv,
-                                    const int* __restrict__ f0v, const double* __restrict__ D, int64_t sI, int64_t si,
-                                    int64_t sj, int ndens, double* __restrict__ Dp, int ldd) {
-  const int p = blockIdx.x;
-  const int s1 = s1v[p], s2 = s2v[p];
-  const int n1 = 2 * Lv[s1] + 1, n2 = 2 * Lv[s2] + 1;
-  const int64_t f1 = f0v[s1], f2 = f0v[s2];
-  const int64_t row = rowv[p];
-  const int n = n1 * n2 * ndens;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const int mn = e / ndens, I = e - mn * ndens;
+// One thread per packed entry: x over densities, y over rows (pair
+// functions), PK_ROWS rows per thread; the pair of a row comes from rowpair.
+constexpr int PK_ROWS = 4;
+__global__ __launch_bounds__(256) void pack_density_kernel(const int* __restrict__ rowpair,
+                                                           const int* __restrict__ s1v, const int* __restrict__ s2v,
+                                                           const int64_t* __restrict__ rowv,
+                                                           const int* __restrict__ Lv, const int* __restrict__ f0v,
+                                                           const double* __restrict__ D, int64_t sI, int64_t si,
+                                                           int64_t sj, int ndens, int64_t npf,
+                                                           double* __restrict__ Dp, int ldd) {
+  const int64_t rb = ((int64_t)blockIdx.x * PK_ROWS) * blockDim.y + threadIdx.y;
+#pragma unroll
+  for (int k = 0; k < PK_ROWS; k++) {
+    const int64_t r = rb + (int64_t)k * blockDim.y;
+    if (r >= npf) return;
+    const int p = rowpair[r];
+    const int s1 = s1v[p], s2 = s2v[p];
+    const int n2 = 2 * Lv[s2] + 1;
+    const int mn = (int)(r - rowv[p]);
     const int mu = mn / n2, nu = mn - mu * n2;
-    double v = D[I * sI + (f1 + mu) * si + (f2 + nu) * sj];
-    if (s1 != s2) v += D[I * sI + (f2 + nu) * si + (f1 + mu) * sj];
-    Dp[(row + mn) * ldd + I] = v;
+    const int64_t i = f0v[s1] + mu, j = f0v[s2] + nu;
+    const double* __restrict__ Dij = D + i * si + j * sj;
+    const double* __restrict__ Dji = D + j * si + i * sj;
+    for (int I = threadIdx.x; I < ndens; I += blockDim.x) {
+      double v = Dij[I * sI];
+      if (s1 != s2) v += Dji[I * sI];
+      Dp[r * ldd + I] = v;
+    }
   }
 }
 
-void pack_density(const int* s1, const int* s2, const int64_t* row, int npairs, const int* L, const int* f0,
-                  const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
+void pack_density(const int* rowpair, const int* s1, const int* s2, const int64_t* row, int64_t npf, const int* L,
+                  const int* f0, const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
                   hipStream_t st) {
-  if (npairs <= 0) return;
-  hipLaunchKernelGGL(pack_density_kernel, dim3(npairs), dim3(128), 0, st, s1, s2, row, L, f0, D, sI, si, sj,
-                     ndens, Dp, ldd);
+  if (npf <= 0 || ndens <= 0) return;
+  const int bx = std::min(ndens, 64), by = std::max(1, 256 / bx);
+  const int64_t nblk = (npf + (int64_t)by * PK_ROWS - 1) / ((int64_t)by * PK_ROWS);
+  hipLaunchKernelGGL(pack_density_kernel, dim3((unsigned)nblk), dim3(bx, by), 0, st, rowpair, s1, s2, row, L, f0, D,
+                     sI, si, sj, ndens, npf, Dp, ldd);
 }
 
 // The same packing for densities whose density index is the slowest
@@ -105,20 +119,29 @@ void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, co
 }
 
 // rowpair[r] = p for the rows r = row[p] .. row[p+1]-1 of pair p (the last
-// pair ends at npf): one workgroup per pair.
+// pair ends at npf), and with dprow rowperm[r] = dprow[p] + r - row[p]: one
+// workgroup per pair.
 __global__ void fill_rowpair_kernel(const int64_t* __restrict__ row, int npairs, int64_t npf,
-                                    int* __restrict__ rowpair) {
+                                    int* __restrict__ rowpair, const int64_t* __restrict__ dprow,
+                                    int* __restrict__ rowperm) {
   const int p = blockIdx.x;
   const int64_t r0 = row[p], r1 = p + 1 < npairs ? row[p + 1] : npf;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) rowpair[r] = p;
+  const int64_t d0 = dprow ? dprow[p] : 0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    rowpair[r] = p;
+    if (dprow) rowperm[r] = (int)(d0 + (r - r0));
+  }
 }
 
-void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, hipStream_t st) {
-  if (npairs > 0) hipLaunchKernelGGL(fill_rowpair_kernel, dim3(npairs), dim3(64), 0, st, row, npairs, npf, rowpair);
+void fill_rowpair(const int64_t* row, int npairs, int64_t npf, int* rowpair, const int64_t* dprow, int* rowperm,
+                  hipStream_t st) {
+  if (npairs > 0)
+    hipLaunchKernelGGL(fill_rowpair_kernel, dim3(npairs), dim3(64), 0, st, row, npairs, npf, rowpair, dprow,
+                       rowperm);
 }
 
 // ---------------------------------------------------------------------------
-// part[s][P][I] = sum_{k in split s} T[k][P] * Dp[k][I]
+// part[s][P][I] = sum_{k in split s} T[k][P] * Dp[rowperm[k]][I]
 // T: npf x ncol (row-major, ldt), Dp: npf x ndens (row-major, ldd).
 // Only rows that can be non-zero for a column tile are visited: the rows of
 // the tile are the concatenation of nseg row segments [seg_lo, seg_hi) per
@@ -180,7 +203,7 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
       for (int z = 0; z < NZD; z++) {
         const int e = min(tid + 256 * z, GK * GI - 1), kk = e / GI, c = e - kk * GI;
         const int64_t kc = min(kb + kk, a1 - 1);
-        dv[z] = Dp[kc * ldd + min(ti + c, ndens - 1)];
+        dv[z] = Dp[(int64_t)mk.rowperm[kc] * ldd + min(ti + c, ndens - 1)];
       }
     };
     load(a0);
