@@ -108,7 +108,10 @@ typedef struct {
                             DFHIP_DENS_C: (ndens,nbf,nbf) row-major */
     int32_t verbose;     /* 1: print sizes, phase times and screening counts;
                             2: also the wall time of each set-up step before the
-                            integrals (synchronizing, for diagnosis) */
+                            integrals (synchronizing, for diagnosis); 3: instead
+                            of 2, when the host and when the stream reached each
+                            step, without waits (the env DFHIP_VERBOSE=n sets
+                            the default) */
     int64_t max_tile_bytes; /* cap of the (ab|P) staging matrix, 0 = automatic */
     int32_t dens_on_device; /* densities is a device pointer (HBM-resident input) */
     int32_t reserved;

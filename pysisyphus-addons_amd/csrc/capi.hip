@@ -166,6 +166,7 @@ void upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t st) {
   if (!h.empty()) HC(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
+
 // ---------------------------------------------------------------------------
 // Tables
 
@@ -269,6 +270,48 @@ struct PinnedBuf {
   }
 };
 
+// Bump allocator over pinned host memory for the uploads queued behind
+// running kernels.  A pageable upload goes through the runtime's staging
+// buffers, which are reused only once the copies holding them have run, so a
+// series of pageable uploads queued behind long kernels stalls the host;
+// copies from pinned memory return at once.  reset() only when no copy from
+// the arena is in flight (at the start of a call: every call ends with its
+// streams drained).
+struct PinnedArena {
+  std::vector<PinnedBuf> chunks;
+  size_t used = 0;   // bytes used in the last chunk
+  size_t total = 0;  // bytes handed out since the last reset
+  void reset() {
+    if (chunks.size() > 1) {
+      // one chunk for what the last call needed
+      const size_t need = total;
+      for (auto& c : chunks) c.release();
+      chunks.assign(1, PinnedBuf{});
+      chunks[0].get<char>(need);
+    }
+    used = 0;
+    total = 0;
+  }
+  template <class T>
+  T* alloc(size_t n) {
+    const size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+    if (chunks.empty() || used + b > chunks.back().bytes) {
+      chunks.emplace_back();
+      chunks.back().get<char>(std::max<size_t>(b, (size_t)4 << 20));
+      used = 0;
+    }
+    T* p = (T*)((char*)chunks.back().p + used);
+    used += b;
+    total += b;
+    return p;
+  }
+  void release() {
+    for (auto& c : chunks) c.release();
+    chunks.clear();
+    used = total = 0;
+  }
+};
+
 struct Lane {
   hipStream_t st = nullptr;
   // side stream of the lane: the metric factorization runs there while the
@@ -287,19 +330,39 @@ struct Lane {
   }
   // the Schwarz partial sums and the aux norms come back through pinned
   // buffers, each signalled by its own event, so that the host can build the
-  // pair part of the plan while the metric still runs
-  hipEvent_t ev_sch = nullptr, ev_met = nullptr;
+  // pair part of the plan while the Schwarz kernels of later classes and the
+  // metric still run
+  hipEvent_t ev_met = nullptr;
   PinnedBuf hpart, hqc;
+  PinnedArena arena;
+  // copy stream: the metric's inputs go up on it while the Schwarz kernels
+  // run (ev_mprep); the Schwarz sums come down on it in two shipments, all
+  // classes but the last as soon as their kernels end (ev_ship[0], on the
+  // host at ev_shipd[0]) while st goes on with the last class, then the last
+  // class; the sorted pairs go up and their bra records are built on it
+  // while the metric runs (ev_plan).  ev_pairs: the device pair list is
+  // ready.
+  hipStream_t st3 = nullptr;
+  hipEvent_t ev_mprep = nullptr, ev_pairs = nullptr, ev_plan = nullptr, ev_ship[2] = {}, ev_shipd[2] = {};
   void ensure_events() {
-    if (ev_sch) return;
-    HC(hipEventCreateWithFlags(&ev_sch, hipEventDisableTiming));
+    if (ev_met) return;
     HC(hipEventCreateWithFlags(&ev_met, hipEventDisableTiming));
+    HC(hipStreamCreateWithFlags(&st3, hipStreamNonBlocking));
+    HC(hipEventCreateWithFlags(&ev_mprep, hipEventDisableTiming));
+    HC(hipEventCreateWithFlags(&ev_pairs, hipEventDisableTiming));
+    HC(hipEventCreateWithFlags(&ev_plan, hipEventDisableTiming));
+    for (int i = 0; i < 2; i++) {
+      HC(hipEventCreateWithFlags(&ev_ship[i], hipEventDisableTiming));
+      HC(hipEventCreateWithFlags(&ev_shipd[i], hipEventDisableTiming));
+    }
   }
   DevPool pool;
   // device workspace reused across calls (grown on demand, freed by
   // dfhip_release_workspace)
-  DBuf<double> wsT, wsA, wsDp, wsPart, wsDraw;
+  DBuf<double> wsT, wsA, wsDp, wsPart, wsDraw, wsDT;
   void release() {
+    wsDT.release();
+    arena.release();
     hpart.release();
     hqc.release();
     wsT.release();
@@ -346,6 +409,61 @@ std::mutex& ctx_mutex() {
 
 thread_local Lane* t_lane = nullptr;
 Lane& cur_lane() { return t_lane ? *t_lane : ctx().lane0; }
+
+// Uploads through the lane's pinned arena, for uploads queued behind running
+// kernels (see PinnedArena).
+template <class T>
+void h2d_pinned(T* d, const T* h, size_t n, hipStream_t st) {
+  if (!n) return;
+  T* b = cur_lane().arena.alloc<T>(n);
+  std::memcpy(b, h, n * sizeof(T));
+  HC(hipMemcpyAsync(d, b, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+template <class T>
+void upload_pinned(DBuf<T>& d, const std::vector<T>& h, hipStream_t st) {
+  d.alloc(std::max<size_t>(h.size(), 1));
+  h2d_pinned(d.p, h.data(), h.size(), st);
+}
+
+// Several arrays in one device block, staged in one pinned block and sent
+// with one copy (each copy costs the stream a few microseconds whatever its
+// size): add() reserves an aligned range and returns its offset; after
+// allocate(), host<T>(off) is where the caller writes the array and
+// dev<T>(off) where the device will find it once commit() has queued the
+// copy.
+struct UploadBatch {
+  size_t size = 0;
+  char* h = nullptr;
+  DBuf<char> d;
+  template <class T>
+  size_t add(size_t n) {
+    const size_t off = size;
+    size += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+    return off;
+  }
+  void allocate() {
+    h = cur_lane().arena.alloc<char>(size);
+    d.alloc(std::max<size_t>(size, 1));
+  }
+  template <class T>
+  T* host(size_t off) {
+    return (T*)(h + off);
+  }
+  template <class T>
+  T* dev(size_t off) {
+    return (T*)(d.p + off);
+  }
+  void commit(hipStream_t st) {
+    if (size) HC(hipMemcpyAsync(d.p, h, size, hipMemcpyHostToDevice, st));
+  }
+};
+
+// A device address with the .p of a DBuf, for arrays inside an UploadBatch.
+template <class T>
+struct DView {
+  T* p = nullptr;
+};
 
 void ensure_ctx() {
   Ctx& c = ctx();
@@ -505,30 +623,77 @@ struct PairSet {
   int n() const { return (int)s1.size(); }
 };
 
-// All a >= b pairs, oriented with L(s1) >= L(s2), grouped by class in packed order.
-PairSet make_pairs(const HostShells& o) {
-  // pairs per class counted first, then placed directly (packed index order
-  // inside each class)
-  std::vector<int> cnt(NPCLS + 1, 0);
-  for (int a = 0; a < o.n; a++)
-    for (int b = 0; b <= a; b++) cnt[pcls(std::max(o.L[a], o.L[b]), std::min(o.L[a], o.L[b])) + 1]++;
-  for (int c = 0; c < NPCLS; c++) cnt[c + 1] += cnt[c];
+// All a >= b shell pairs, oriented with L(s1) >= L(s2) and grouped by class
+// (la, lb); the order inside a class is that of PairClassTable, which
+// pair_classes fills and gen_pairs follows on the device.
+struct PairClasses {
+  PairClassTable t;
+  std::vector<int> shflat;  // shells of each L, ascending, L by L
+};
+
+PairClasses pair_classes(const HostShells& o) {
+  PairClasses pc;
+  std::memset(&pc.t, 0, sizeof(pc.t));
+  std::vector<int> loff(LMAX_ORB + 2, 0), nl(LMAX_ORB + 1, 0);
+  for (int a = 0; a < o.n; a++) nl[o.L[a]]++;
+  for (int L = 0; L <= LMAX_ORB; L++) loff[L + 1] = loff[L] + nl[L];
+  pc.shflat.resize(o.n);
+  std::vector<int> pos(loff.begin(), loff.end() - 1);
+  for (int a = 0; a < o.n; a++) pc.shflat[pos[o.L[a]]++] = a;
+  PairClassTable& t = pc.t;
+  t.ncls = NPCLS;
+  int p = 0;
+  int64_t row = 0;
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb);
+      t.begin[k] = p;
+      t.la[k] = la;
+      t.lb[k] = lb;
+      t.off_a[k] = loff[la];
+      t.off_b[k] = loff[lb];
+      t.n_b[k] = nl[lb];
+      t.w[k] = (2 * la + 1) * (2 * lb + 1);
+      t.row0[k] = row;
+      const int n = la != lb ? nl[la] * nl[lb] : nl[la] * (nl[la] + 1) / 2;
+      p += n;
+      row += (int64_t)n * t.w[k];
+    }
+  t.begin[NPCLS] = p;
+  return pc;
+}
+
+// The host copy of the pair list in the order of pc (gen_pairs' order).
+PairSet make_pairs(const HostShells& o, const PairClasses& pc) {
+  const PairClassTable& t = pc.t;
   PairSet ps;
-  const int np = cnt[NPCLS];
+  const int np = t.begin[NPCLS];
   ps.s1.resize(np);
   ps.s2.resize(np);
   ps.packed.resize(np);
-  ps.cls_begin.assign(cnt.begin(), cnt.begin() + NPCLS);
-  ps.cls_end.assign(cnt.begin() + 1, cnt.end());
-  std::vector<int> pos(cnt.begin(), cnt.begin() + NPCLS);
-  for (int a = 0; a < o.n; a++)
-    for (int b = 0; b <= a; b++) {
-      const bool sw = o.L[a] < o.L[b];
-      const int p = pos[pcls(std::max(o.L[a], o.L[b]), std::min(o.L[a], o.L[b]))]++;
-      ps.s1[p] = sw ? b : a;
-      ps.s2[p] = sw ? a : b;
-      ps.packed[p] = a * (a + 1) / 2 + b;
+  ps.cls_begin.assign(t.begin, t.begin + NPCLS);
+  ps.cls_end.assign(t.begin + 1, t.begin + NPCLS + 1);
+  for (int k = 0; k < NPCLS; k++) {
+    int p = t.begin[k];
+    if (t.la[k] != t.lb[k]) {
+      const int na = t.n_b[k] ? (t.begin[k + 1] - t.begin[k]) / t.n_b[k] : 0;
+      for (int ia = 0; ia < na; ia++)
+        for (int ib = 0; ib < t.n_b[k]; ib++, p++) {
+          ps.s1[p] = pc.shflat[t.off_a[k] + ia];
+          ps.s2[p] = pc.shflat[t.off_b[k] + ib];
+        }
+    } else {
+      for (int ia = 0; p < t.begin[k + 1]; ia++)
+        for (int ib = 0; ib <= ia; ib++, p++) {
+          ps.s1[p] = pc.shflat[t.off_a[k] + ia];
+          ps.s2[p] = pc.shflat[t.off_b[k] + ib];
+        }
     }
+  }
+  for (int p = 0; p < np; p++) {
+    const int a = std::max(ps.s1[p], ps.s2[p]), b = std::min(ps.s1[p], ps.s2[p]);
+    ps.packed[p] = a * (a + 1) / 2 + b;
+  }
   return ps;
 }
 
@@ -536,21 +701,41 @@ PairSet make_pairs(const HostShells& o) {
 // fit the primitive-pair cache use the cached kernel (one partial per pair),
 // the others the chunked kernel.
 // Schwarz norms in two parts, so that the caller can queue other work
-// behind the kernels: schwarz_bra_launch enqueues them on the lane's stream,
-// schwarz_bra_finish downloads their partial sums and reduces them to Q_ab.
+// behind the kernels: schwarz_bra_launch enqueues them on the lane's stream
+// and, class by class, the copies of their partial sums to the host on the
+// lane's copy stream; schwarz_class_finish waits for one class's sums and
+// reduces them to Q_ab.
 struct SchwarzPending {
-  DBuf<int> s1, s2;
+  DBuf<int> s1, s2, shflat;
+  DBuf<int64_t> prow0;
   DBuf<double> part, scr;
   std::vector<size_t> poff;
   std::vector<int> nch;
+  std::vector<int> ship;  // per class: the shipment (0 or 1) that brings its sums
+  bool shipped[2] = {false, false};
 };
 
-void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps, SchwarzPending& sp) {
+void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const PairClasses& pc, SchwarzPending& sp) {
   hipStream_t st = cur_lane().st;
+  // the pair list (and each pair's first row) generated on the device
+  const int npairs = pc.t.begin[NPCLS];
   DBuf<int>& s1 = sp.s1;
   DBuf<int>& s2 = sp.s2;
-  upload(s1, ps.s1, st);
-  upload(s2, ps.s2, st);
+  s1.alloc(std::max(npairs, 1));
+  s2.alloc(std::max(npairs, 1));
+  sp.prow0.alloc(std::max(npairs, 1));
+  Lane& lane = cur_lane();
+  lane.ensure_events();
+  upload_pinned(sp.shflat, pc.shflat, st);
+  gen_pairs(pc.t, sp.shflat.p, s1.p, s2.p, sp.prow0.p, st);
+  HC(hipGetLastError());
+  HC(hipEventRecord(lane.ev_pairs, st));  // other streams may read the list after this
+  // shells of each L with the most primitives
+  int maxprim[LMAX_ORB + 1] = {};
+  for (int a = 0; a < o.n; a++) maxprim[o.L[a]] = std::max(maxprim[o.L[a]], o.nprim[a]);
+  PairSet ps;  // class ranges only
+  ps.cls_begin.assign(pc.t.begin, pc.t.begin + NPCLS);
+  ps.cls_end.assign(pc.t.begin + 1, pc.t.begin + NPCLS + 1);
   std::vector<size_t>& poff = sp.poff;
   poff.assign(NPCLS + 1, 0);
   // kernel per class: 3 = matrix cores with the tables in LDS, 2 = matrix
@@ -565,10 +750,8 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb);
-      int maxnpp = 0;
-      for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++)
-        maxnpp = std::max(maxnpp, o.nprim[ps.s1[p]] * o.nprim[ps.s2[p]]);
       const int np = ps.cls_end[k] - ps.cls_begin[k];
+      const int maxnpp = np > 0 ? maxprim[la] * maxprim[lb] : 0;
       if (cap >= 3 && c.s4m[la][lb].fn != nullptr && maxnpp <= c.s4m[la][lb].nppm) {
         kind[k] = 3;
       } else if (cap >= 2 && c.s4s[la][lb].fn != nullptr && maxnpp <= c.s4s[la][lb].npps) {
@@ -586,6 +769,23 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
   // scratch of the streamed kernels: all pairs of the largest class, at most 1 GiB
   DBuf<double>& scr = sp.scr;
   scr.alloc(std::min<size_t>(scr_need, (size_t)1 << 27));
+  double* hp = lane.hpart.get<double>(poff[NPCLS]);
+  // shipments: the classes before the last non-empty one, then the rest
+  int last = -1, prev = -1;
+  for (int k = 0; k < NPCLS; k++)
+    if (ps.cls_end[k] > ps.cls_begin[k]) {
+      prev = last;
+      last = k;
+    }
+  sp.ship.assign(NPCLS, 1);
+  for (int k = 0; k <= prev; k++) sp.ship[k] = 0;
+  auto ship = [&](int i, size_t b, size_t e) {
+    HC(hipEventRecord(lane.ev_ship[i], st));
+    HC(hipStreamWaitEvent(lane.st3, lane.ev_ship[i], 0));
+    if (e > b) HC(hipMemcpyAsync(hp + b, part.p + b, (e - b) * sizeof(double), hipMemcpyDeviceToHost, lane.st3));
+    HC(hipEventRecord(lane.ev_shipd[i], lane.st3));
+    sp.shipped[i] = true;
+  };
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
@@ -599,35 +799,37 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
       else
         c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       HC(hipGetLastError());
+      if (k == prev) ship(0, 0, poff[prev + 1]);
     }
-  Lane& lane = cur_lane();
-  lane.ensure_events();
-  double* hp = lane.hpart.get<double>(poff[NPCLS]);
-  if (poff[NPCLS] > 0) HC(hipMemcpyAsync(hp, part.p, poff[NPCLS] * sizeof(double), hipMemcpyDeviceToHost, st));
-  HC(hipEventRecord(lane.ev_sch, st));
+  ship(1, prev >= 0 ? poff[prev + 1] : 0, poff[NPCLS]);
 }
 
-// Waits for the Schwarz kernels only (work queued behind them may still run).
-std::vector<double> schwarz_bra_finish(const PairSet& ps, SchwarzPending& sp) {
+// Q_ab of the pairs of class k (make_pairs order) into q[cls_begin[k] ..];
+// waits for that class's sums only (work queued behind them may still run).
+void schwarz_class_finish(const PairSet& ps, const SchwarzPending& sp, int k, double* q) {
   Lane& lane = cur_lane();
   const std::vector<size_t>& poff = sp.poff;
-  const std::vector<int>& nch = sp.nch;
-  HC(hipEventSynchronize(lane.ev_sch));
+  const int nch = sp.nch[k];
+  if (sp.shipped[sp.ship[k]]) HC(hipEventSynchronize(lane.ev_shipd[sp.ship[k]]));
   const double* hp = (const double*)lane.hpart.p;
+  for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++) {
+    double s = 0.0;
+    for (int z = 0; z < nch; z++) s += hp[poff[k] + (size_t)(p - ps.cls_begin[k]) * nch + z];
+    q[p] = std::sqrt(std::sqrt(s));
+    if (!std::isfinite(q[p])) throw Error{DFHIP_ERR_HIP, "non-finite Schwarz norm"};
+  }
+}
+
+std::vector<double> schwarz_bra_finish(const PairSet& ps, SchwarzPending& sp) {
   std::vector<double> q(ps.n());
-  for (int k = 0; k < NPCLS; k++)
-    for (int p = ps.cls_begin[k]; p < ps.cls_end[k]; p++) {
-      double s = 0.0;
-      for (int z = 0; z < nch[k]; z++) s += hp[poff[k] + (size_t)(p - ps.cls_begin[k]) * nch[k] + z];
-      q[p] = std::sqrt(std::sqrt(s));
-      if (!std::isfinite(q[p])) throw Error{DFHIP_ERR_HIP, "non-finite Schwarz norm"};
-    }
+  for (int k = 0; k < NPCLS; k++) schwarz_class_finish(ps, sp, k, q.data());
   return q;
 }
 
-std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairSet& ps) {
+std::vector<double> schwarz_bra_dev(Ctx& c, const HostShells& o, const DevShells& dev, const PairClasses& pc,
+                                    const PairSet& ps) {
   SchwarzPending sp;
-  schwarz_bra_launch(c, o, dev, ps, sp);
+  schwarz_bra_launch(c, o, dev, pc, sp);
   return schwarz_bra_finish(ps, sp);
 }
 
@@ -667,13 +869,29 @@ void launch_bra_records(int la, int lb, DevShells bra, const int* s1, const int*
 }
 
 // Bra records of the generated kernels for one pair class: offsets (in
-// doubles, per pair) and the records themselves.
+// doubles, per pair), primitive-pair counts and the records themselves.  off
+// and npp point into off_buf / npp_buf or into a caller's upload batch.
 struct BraRecords {
-  DBuf<int64_t> off;
-  DBuf<int> npp;
+  DBuf<int64_t> off_buf;
+  DBuf<int> npp_buf;
+  const int64_t* off = nullptr;
+  const int* npp = nullptr;
   DBuf<double> recs;
   DBuf<float> recs32;  // mixed precision only
 };
+
+// Record layout of n pairs: off[p], npp[p] (host arrays); returns the total
+// number of doubles.
+int64_t bra_record_layout(int rs, const HostShells& bra, const int* s1, const int* s2, int n, int64_t* off,
+                          int* npp) {
+  int64_t o = 0;
+  for (int p = 0; p < n; p++) {
+    off[p] = o;
+    npp[p] = bra.nprim[s1[p]] * (s2[p] >= 0 ? bra.nprim[s2[p]] : 1);
+    o += (int64_t)npp[p] * rs;
+  }
+  return o;
+}
 
 // FP32 copy of the bra records for the FP32 class kernels.
 __global__ void records_to_f32_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
@@ -681,29 +899,35 @@ __global__ void records_to_f32_kernel(const double* __restrict__ in, float* __re
   if (i < n) out[i] = (float)in[i];
 }
 
-void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevShells& dev, const int* ds1,
-                       const int* ds2, const std::vector<int>& s1, const std::vector<int>& s2, BraRecords& br,
-                       bool fp32 = false) {
+// The records of n pairs whose layout (br.off, br.npp, total o doubles) is
+// on the device already.
+void fill_bra_records(Ctx& c, int la, int lb, const DevShells& dev, const int* ds1, const int* ds2, int n, int64_t o,
+                      BraRecords& br, bool fp32, hipStream_t s) {
   const int rs = c.gen3[la][lb][0].rs;
-  std::vector<int64_t> off(s1.size());
-  std::vector<int> npp(s1.size());
-  int64_t o = 0;
-  for (size_t p = 0; p < s1.size(); p++) {
-    off[p] = o;
-    npp[p] = bra.nprim[s1[p]] * (s2[p] >= 0 ? bra.nprim[s2[p]] : 1);
-    o += (int64_t)npp[p] * rs;
-  }
-  upload(br.off, off, cur_lane().st);
-  upload(br.npp, npp, cur_lane().st);
   br.recs.alloc(std::max<int64_t>(o, 1));
-  launch_bra_records(la, lb, dev, ds1, ds2, br.off.p, rs, (int)s1.size(), br.recs.p, cur_lane().st);
+  launch_bra_records(la, lb, dev, ds1, ds2, br.off, rs, n, br.recs.p, s);
   HC(hipGetLastError());
   if (fp32 && o > 0) {
     br.recs32.alloc(o);
-    hipLaunchKernelGGL(records_to_f32_kernel, dim3((unsigned)((o + 255) / 256)), dim3(256), 0, cur_lane().st, br.recs.p,
+    hipLaunchKernelGGL(records_to_f32_kernel, dim3((unsigned)((o + 255) / 256)), dim3(256), 0, s, br.recs.p,
                        br.recs32.p, o);
     HC(hipGetLastError());
   }
+}
+
+// Layout, upload and records of n pairs (host s1/s2, device ds1/ds2).
+void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevShells& dev, const int* ds1,
+                       const int* ds2, const int* s1, const int* s2, int n, BraRecords& br, bool fp32 = false,
+                       hipStream_t s = nullptr) {
+  if (!s) s = cur_lane().st;
+  std::vector<int64_t> off(n);
+  std::vector<int> npp(n);
+  const int64_t o = bra_record_layout(c.gen3[la][lb][0].rs, bra, s1, s2, n, off.data(), npp.data());
+  upload_pinned(br.off_buf, off, s);
+  upload_pinned(br.npp_buf, npp, s);
+  br.off = br.off_buf.p;
+  br.npp = br.npp_buf.p;
+  fill_bra_records(c, la, lb, dev, ds1, ds2, n, o, br, fp32, s);
 }
 
 void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells ket, PairArgs pa, AuxArgs xa,
@@ -714,23 +938,35 @@ void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells
   if (fp32 && !br.recs32.p) throw Error{DFHIP_ERR_HIP, "FP32 bra records missing"};
   hipLaunchKernelGGL(fp32 ? g.kernel32 : g.kernel, dim3(pa.npairs, nchunks, fp32 ? g.groups32 : g.groups), dim3(64),
                      0, cur_lane().st, br.recs.p,
-                     br.recs32.p, br.off.p, br.npp.p, ket, pa, xa, c.boys.p, out, ldo);
+                     br.recs32.p, br.off, br.npp, ket, pa, xa, c.boys.p, out, ldo);
   HC(hipGetLastError());
 }
 
-// Full metric (P|Q) (naux x naux) into M via the 2c2e form of the 3c2e kernels.
-void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, int64_t ldm) {
-  hipStream_t st = cur_lane().st;
+// Full metric (P|Q) (naux x naux) into M via the 2c2e form of the 3c2e
+// kernels, in two parts: metric_prepare uploads the aux "pair" lists and
+// builds their bra records on stream s (a side stream may run it while other
+// work runs), metric_launch queues the kernels on the lane's stream once the
+// caller has ordered that stream behind the preparation.
+struct MetricPrep {
+  DBuf<int> ds1, ds2, dns, dlsh;
+  DBuf<int64_t> drow, dlcol;
+  std::vector<BraRecords> brs;
+  std::vector<int> lbeg, kmax;
+  bool gen = false;
+  MetricPrep() : brs(LMAX_AUX + 1) {}
+};
+
+void metric_prepare(Ctx& c, const HostShells& x, const DevShells& dev, MetricPrep& mp, hipStream_t s) {
   std::vector<std::vector<int>> byL(LMAX_AUX + 1);
   for (int i = 0; i < x.n; i++) byL[x.L[i]].push_back(i);
   std::vector<int> s1, s2, nsurv;
   std::vector<int64_t> row;
-  std::vector<int> lbeg(LMAX_AUX + 2, 0);
   std::vector<int> lsh;
   std::vector<int64_t> lcol;
-  std::vector<int> kmax(LMAX_AUX + 1, 1);
+  mp.lbeg.assign(LMAX_AUX + 2, 0);
+  mp.kmax.assign(LMAX_AUX + 1, 1);
   for (int L = 0; L <= LMAX_AUX; L++) {
-    lbeg[L] = (int)s1.size();
+    mp.lbeg[L] = (int)s1.size();
     for (int i : byL[L]) {
       s1.push_back(i);
       s2.push_back(-1);
@@ -738,43 +974,52 @@ void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, in
       nsurv.push_back(std::numeric_limits<int>::max());
       lsh.push_back(i);
       lcol.push_back(x.f0[i]);
-      kmax[L] = std::max(kmax[L], x.nprim[i]);
+      mp.kmax[L] = std::max(mp.kmax[L], x.nprim[i]);
     }
   }
-  lbeg[LMAX_AUX + 1] = (int)s1.size();
-  DBuf<int> ds1, ds2, dns, dlsh;
-  DBuf<int64_t> drow, dlcol;
-  upload(ds1, s1, st);
-  upload(ds2, s2, st);
-  upload(dns, nsurv, st);
-  upload(drow, row, st);
-  upload(dlsh, lsh, st);
-  upload(dlcol, lcol, st);
-  const bool gen = use_generated();
-  std::vector<BraRecords> brs(LMAX_AUX + 1);
+  mp.lbeg[LMAX_AUX + 1] = (int)s1.size();
+  upload_pinned(mp.ds1, s1, s);
+  upload_pinned(mp.ds2, s2, s);
+  upload_pinned(mp.dns, nsurv, s);
+  upload_pinned(mp.drow, row, s);
+  upload_pinned(mp.dlsh, lsh, s);
+  upload_pinned(mp.dlcol, lcol, s);
+  mp.gen = use_generated();
   for (int LP = 0; LP <= LMAX_AUX; LP++) {
-    const int np = lbeg[LP + 1] - lbeg[LP];
+    const int np = mp.lbeg[LP + 1] - mp.lbeg[LP];
+    if (np > 0 && mp.gen && gen3_has(LP, 0, 0))
+      build_bra_records(c, LP, 0, x, dev, mp.ds1.p + mp.lbeg[LP], mp.ds2.p + mp.lbeg[LP], s1.data() + mp.lbeg[LP],
+                        s2.data() + mp.lbeg[LP], np, mp.brs[LP], false, s);
+  }
+}
+
+void metric_launch(Ctx& c, const DevShells& dev, const MetricPrep& mp, double* M, int64_t ldm) {
+  hipStream_t st = cur_lane().st;
+  for (int LP = 0; LP <= LMAX_AUX; LP++) {
+    const int np = mp.lbeg[LP + 1] - mp.lbeg[LP];
     if (np == 0) continue;
-    PairArgs pa{ds1.p + lbeg[LP], ds2.p + lbeg[LP], drow.p + lbeg[LP], dns.p + lbeg[LP], np};
-    if (gen && gen3_has(LP, 0, 0)) {
-      std::vector<int> a1(s1.begin() + lbeg[LP], s1.begin() + lbeg[LP + 1]);
-      std::vector<int> a2(s2.begin() + lbeg[LP], s2.begin() + lbeg[LP + 1]);
-      build_bra_records(c, LP, 0, x, dev, ds1.p + lbeg[LP], ds2.p + lbeg[LP], a1, a2, brs[LP]);
-    }
+    PairArgs pa{mp.ds1.p + mp.lbeg[LP], mp.ds2.p + mp.lbeg[LP], mp.drow.p + mp.lbeg[LP], mp.dns.p + mp.lbeg[LP], np};
     for (int LQ = 0; LQ <= LMAX_AUX; LQ++) {
-      const int nq = lbeg[LQ + 1] - lbeg[LQ];
+      const int nq = mp.lbeg[LQ + 1] - mp.lbeg[LQ];
       if (nq == 0) continue;
-      AuxArgs xa{dlsh.p + lbeg[LQ], dlcol.p + lbeg[LQ], 0, nq, kmax[LQ]};
-      if (gen && gen3_has(LP, 0, LQ)) {
-        launch_gen3(c, LP, 0, LQ, brs[LP], dev, pa, xa, M, ldm);
+      AuxArgs xa{mp.dlsh.p + mp.lbeg[LQ], mp.dlcol.p + mp.lbeg[LQ], 0, nq, mp.kmax[LQ]};
+      if (mp.gen && gen3_has(LP, 0, LQ)) {
+        launch_gen3(c, LP, 0, LQ, mp.brs[LP], dev, pa, xa, M, ldm);
       } else {
         c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, ldm, st);
         HC(hipGetLastError());
       }
     }
   }
-  // no wait: the buffers above go back to the lane's pool, whose blocks are
-  // only reused by work queued behind these kernels
+}
+
+// Both parts on the lane's stream.  No wait: the buffers go back to the
+// lane's pool, whose blocks are only reused by work queued behind these
+// kernels.
+void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, int64_t ldm) {
+  MetricPrep mp;
+  metric_prepare(c, x, dev, mp, cur_lane().st);
+  metric_launch(c, dev, mp, M, ldm);
 }
 
 dfhip_options defaults() {
@@ -895,62 +1140,35 @@ bool big_system(const PairSet& ps, std::vector<char>* big_cls) {
   return nbig > 1;
 }
 
-// Pair part of the plan (needs Q_ab only): pairs sorted inside each class,
-// their first rows in the staging matrix.
-void build_plan_pairs(Plan& pl) {
+// Pair part of the plan for class k (needs that class's Q_ab only): its
+// pairs sorted by Q_ab descending, ties in make_pairs order, into pl.ps,
+// pl.qab, pl.from and pl.prow at the class's range (these have their full
+// sizes).  ps0 / q0: the pairs in make_pairs order and their Q_ab; row0: the
+// class's first row (all pairs of a class have the same number of functions).
+void plan_pair_class(Plan& pl, const PairSet& ps0, const std::vector<double>& q0, int k, int64_t row0) {
   const HostShells& o = pl.orb;
-  const int np = pl.ps.n();
-  std::vector<char> big_cls;
-  big_system(pl.ps, &big_cls);
-  // sort pairs inside each class by Q_ab descending (stable on packed index)
-  PairSet sorted;
-  sorted.cls_begin = pl.ps.cls_begin;
-  sorted.cls_end = pl.ps.cls_end;
-  std::vector<double> q2(np);
-  sorted.s1.resize(np);
-  sorted.s2.resize(np);
-  sorted.packed.resize(np);
-  pl.from.resize(np);
-  run_tasks(NPCLS, big_cls, [&](int k) {
-    // descending Q_ab, ties in the original (packed) order
-    const int p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
-    if (p1 == p0) return;
-    std::vector<int> idx(p1 - p0);
-    for (int p = p0; p < p1; p++) idx[p - p0] = p;
-    sort_desc(pl.qab.data(), idx);
-    for (int r = 0; r < p1 - p0; r++) {
-      const int p = idx[r];
-      sorted.s1[p0 + r] = pl.ps.s1[p];
-      sorted.s2[p0 + r] = pl.ps.s2[p];
-      sorted.packed[p0 + r] = pl.ps.packed[p];
-      pl.from[p0 + r] = p;
-      q2[p0 + r] = pl.qab[p];
-    }
-  });
-  pl.ps = std::move(sorted);
-  pl.qab = std::move(q2);
-  pl.prow.resize(np);
-  int64_t r = 0;
-  for (int p = 0; p < np; p++) {
-    pl.prow[p] = r;
-    r += (int64_t)(2 * o.L[pl.ps.s1[p]] + 1) * (2 * o.L[pl.ps.s2[p]] + 1);
+  const int p0 = ps0.cls_begin[k], p1 = ps0.cls_end[k];
+  if (p1 == p0) return;
+  std::vector<int> idx(p1 - p0);
+  for (int p = p0; p < p1; p++) idx[p - p0] = p;
+  sort_desc(q0.data(), idx);
+  const int64_t w = (int64_t)(2 * o.L[ps0.s1[p0]] + 1) * (2 * o.L[ps0.s2[p0]] + 1);
+  for (int r = 0; r < p1 - p0; r++) {
+    const int p = idx[r];
+    pl.ps.s1[p0 + r] = ps0.s1[p];
+    pl.ps.s2[p0 + r] = ps0.s2[p];
+    pl.ps.packed[p0 + r] = ps0.packed[p];
+    pl.from[p0 + r] = p;
+    pl.qab[p0 + r] = q0[p];
+    pl.prow[p0 + r] = row0 + r * w;
   }
-  pl.npf = r;
 }
 
-// Aux part of the plan (needs Q_C and the sorted pairs): aux lists sorted by
-// Q_C, their columns, the screened prefixes and the triplet counts.
-void build_plan_aux(Plan& pl, const dfhip_options& opt) {
+// Aux lists of the plan (needs Q_C): per L sorted by Q_C descending, their
+// columns in the staging matrix and the original function of each column.
+void build_aux_lists(Plan& pl) {
   const HostShells& o = pl.orb;
   const HostShells& x = pl.aux;
-  const int np = pl.ps.n();
-  std::vector<char> big_l(LMAX_AUX + 1, 0);
-  if (big_system(pl.ps, nullptr)) {
-    std::vector<int> nl(LMAX_AUX + 1, 0);
-    for (int i = 0; i < x.n; i++) nl[x.L[i]]++;
-    for (int L = 0; L <= LMAX_AUX; L++) big_l[L] = nl[L] > 0;
-  }
-  // aux lists
   pl.alist.assign(LMAX_AUX + 1, {});
   pl.acol.assign(LMAX_AUX + 1, {});
   for (int i = 0; i < x.n; i++) pl.alist[x.L[i]].push_back(i);
@@ -965,15 +1183,31 @@ void build_plan_aux(Plan& pl, const dfhip_options& opt) {
       col += 2 * L + 1;
     }
   }
+  pl.ntriplets = (int64_t)o.n * (o.n + 1) / 2 * x.n;
+}
+
+// Screened prefixes of the plan (needs the aux lists and the sorted pairs)
+// and the triplet counts; under QVL they come from apply_qvl instead.
+void screen_sweep(Plan& pl, const dfhip_options& opt) {
+  const HostShells& o = pl.orb;
+  const HostShells& x = pl.aux;
+  const int np = pl.ps.n();
+  std::vector<char> big_l(LMAX_AUX + 1, 0);
+  if (big_system(pl.ps, nullptr)) {
+    std::vector<int> nl(LMAX_AUX + 1, 0);
+    for (int i = 0; i < x.n; i++) nl[x.L[i]]++;
+    for (int L = 0; L <= LMAX_AUX; L++) big_l[L] = nl[L] > 0;
+  }
   pl.nsurv.assign(LMAX_AUX + 1, std::vector<int>(np, 0));
   pl.ncomputed = 0;
-  pl.ntriplets = (int64_t)o.n * (o.n + 1) / 2 * x.n;
   if (opt.screening == DFHIP_SCREEN_QVL) return;  // prefixes and counts from apply_qvl
   // Per aux L: the screened prefix of every pair (skip when Q_ab * Q_C <
   // thresh; inside a pair class Q_ab is non-increasing, so the prefix length
   // is too and one downward sweep per class finds all of them), then the
   // per-class counts of screened triplets and primitive triplets.
   int64_t ncomp[LMAX_AUX + 1] = {};
+  std::vector<int> npp(np);
+  for (int p = 0; p < np; p++) npp[p] = o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]];
   run_tasks(LMAX_AUX + 1, big_l, [&](int L) {
     const auto& v = pl.alist[L];
     const int nv = (int)v.size();
@@ -995,7 +1229,7 @@ void build_plan_aux(Plan& pl, const dfhip_options& opt) {
         }
         ns[p] = n;
         trip += n;
-        prim += (int64_t)o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]] * pref[n];
+        prim += (int64_t)npp[p] * pref[n];
       }
       pl.class_trip[k * 6 + L] += trip;
       pl.class_prim[k * 6 + L] += prim;
@@ -1061,7 +1295,7 @@ struct QvlRecords {
 };
 
 void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int* ps1, const int* ps2,
-               const std::vector<DBuf<int>>& dalist, double thresh, QvlRecords& qr, hipStream_t st) {
+               const std::vector<DView<int>>& dalist, double thresh, QvlRecords& qr, hipStream_t st) {
   const HostShells& o = pl.orb;
   const HostShells& x = pl.aux;
   const int np = pl.ps.n();
@@ -1155,12 +1389,31 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   hipStream_t st = cur_lane().st;
   Timer tm(st);
   const int e0 = tm.mark();
+  // verbose 2: wall time of the host steps up to the integrals (each step
+  // synchronized first, so the times add up); verbose 3: no waits, each step
+  // logs when the host got there and when the stream did (printed at the end)
+  const auto run_t0 = std::chrono::steady_clock::now();
+  auto step_t0 = run_t0;
+  std::vector<std::pair<const char*, std::pair<double, int>>> vmarks;
+  auto vstep = [&](const char* what) {
+    if (opt.verbose < 2) return;
+    const auto t = std::chrono::steady_clock::now();
+    if (opt.verbose >= 3) {
+      vmarks.push_back({what, {std::chrono::duration<double, std::milli>(t - run_t0).count(), tm.mark()}});
+      return;
+    }
+    HC(hipStreamSynchronize(st));
+    const auto t2 = std::chrono::steady_clock::now();
+    vlog(opt, "  %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(t2 - step_t0).count());
+    step_t0 = t2;
+  };
   DevShellBufs dorb, daux;
   dorb.upload_from(o, st);
   daux.upload_from(x, st);
   const int e1 = tm.mark();
 
   Lane& lane = cur_lane();
+  lane.arena.reset();
   if (need_dens) lane.ensure_side();
   DBuf<int> info(1);
   // on every way out of run() (errors included) the lane's streams are
@@ -1172,15 +1425,27 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     ~StreamDrain() {
       (void)hipStreamSynchronize(l.st);
       if (side) (void)hipStreamSynchronize(l.st2);
+      if (l.st3) (void)hipStreamSynchronize(l.st3);
     }
   } drain{lane, need_dens};
 
   // Schwarz norms of the shell pairs (common/integrals/intor.f90:131-160);
   // the metric kernels queue behind them
-  pl.ps = make_pairs(o);
+  vstep("shells");
+  const PairClasses pcl = pair_classes(o);
   SchwarzPending spend;
-  schwarz_bra_launch(c, o, dorb.view, pl.ps, spend);
+  schwarz_bra_launch(c, o, dorb.view, pcl, spend);
   const int e2 = tm.mark();
+  vstep("Schwarz queued");
+  pl.ps = make_pairs(o, pcl);
+  vstep("pairs");
+
+  // the metric's inputs go up on the copy stream while the Schwarz kernels run
+  MetricPrep mprep;
+  HC(hipStreamWaitEvent(lane.st3, tm.ev[e1], 0));
+  metric_prepare(c, x, daux.view, mprep, lane.st3);
+  HC(hipEventRecord(lane.ev_mprep, lane.st3));
+  vstep("metric inputs queued");
 
   // densities: upload and pair-pack (diabatization/intor.f90:155-176
   // symmetry) on the side stream, concurrent with the Schwarz kernels.  The
@@ -1197,11 +1462,10 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (npf0 >= ((int64_t)1 << 31)) throw Error{DFHIP_ERR_UNSUPPORTED, "more than 2^31 pair functions"};
   // device arrays of the side stream's packing: they live until run() ends,
   // so that no work on st gets their blocks while the packing reads them
-  DBuf<int> ps1_0, ps2_0, rowpair0, dpairof, dfshell;
-  DBuf<int64_t> prow0_d;
+  DBuf<int> rowpair0, dpairof, dfshell;
   if (need_dens) {
     const hipStream_t sp = lane.st2;
-    HC(hipStreamWaitEvent(sp, tm.ev[e1], 0));  // the shell arrays are on the device
+    HC(hipStreamWaitEvent(sp, lane.ev_pairs, 0));  // the shell arrays and the pair list are on the device
     const double* Dsrc = dens;
     if (!opt.dens_on_device) {
       const size_t nd = (size_t)ndens * nbfs * nbfs;
@@ -1222,7 +1486,6 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       sj = (int64_t)ndens * nbfs;
     }
     Dp = reserve(cur_lane().wsDp, (size_t)npf0 * ndens);
-    upload(prow0_d, prow0, sp);
     if (sj == 1 && si == nbfs && ndens > 48) {
       // density index slowest and many densities: tiled transpose (pair lookup
       // by shell pair, shell by function).  With few densities the packing
@@ -1237,19 +1500,31 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         for (int m = 0; m < 2 * o.L[sh] + 1; m++) fshell[o.f0[sh] + m] = sh;
       upload(dpairof, pairof, sp);
       upload(dfshell, fshell, sp);
-      pack_density_t(dpairof.p, dfshell.p, prow0_d.p, dorb.L.p, dorb.f0.p, o.n, o.nbf, Dsrc, sI, ndens, Dp, ndens,
+      pack_density_t(dpairof.p, dfshell.p, spend.prow0.p, dorb.L.p, dorb.f0.p, o.n, o.nbf, Dsrc, sI, ndens, Dp, ndens,
                      sp);
     } else {
-      upload(ps1_0, pl.ps.s1, sp);
-      upload(ps2_0, pl.ps.s2, sp);
+      if (sj == 1 && si == nbfs && (size_t)ndens * nbfs * nbfs * sizeof(double) <= ((size_t)1 << 32)) {
+        // density index slowest, few densities: turn D into (i, j, I) first,
+        // so that the packing reads runs of densities (the direct reads along
+        // I of the C layout each take their own cache line, about ten times
+        // the bytes of D; the turn moves D twice)
+        const int64_t nn = (int64_t)nbfs * nbfs;
+        double* DT = reserve(cur_lane().wsDT, (size_t)ndens * nn);
+        transpose(Dsrc, ndens, nn, DT, sp);
+        Dsrc = DT;
+        sI = 1;
+        si = (int64_t)nbfs * ndens;
+        sj = ndens;
+      }
       rowpair0.alloc((size_t)std::max<int64_t>(npf0, 1));
-      fill_rowpair(prow0_d.p, pl.ps.n(), npf0, rowpair0.p, nullptr, nullptr, sp);
-      pack_density(rowpair0.p, ps1_0.p, ps2_0.p, prow0_d.p, npf0, dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp,
+      fill_rowpair(spend.prow0.p, pl.ps.n(), npf0, rowpair0.p, nullptr, nullptr, sp);
+      pack_density(rowpair0.p, spend.s1.p, spend.s2.p, spend.prow0.p, npf0, dorb.L.p, dorb.f0.p, Dsrc, sI, si, sj, ndens, Dp,
                    ndens, sp);
     }
     HC(hipGetLastError());
     HC(hipEventRecord(lane.ev_pack, sp));
   }
+  vstep("packing queued");
 
   // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal
   // blocks, queued behind the Schwarz kernels.
@@ -1258,7 +1533,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   const int64_t lda = need_dens ? 2 * naux + ndens : naux;
   double* Ap = reserve(cur_lane().wsA, (size_t)lda * naux);
   HC(hipMemsetAsync(Ap, 0, (size_t)lda * naux * sizeof(double), st));
-  metric_dev(c, x, daux.view, Ap, lda);
+  HC(hipStreamWaitEvent(st, lane.ev_mprep, 0));
+  metric_launch(c, daux.view, mprep, Ap, lda);
   DBuf<double> dqc(x.n);
   schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
   const int e3 = tm.mark();
@@ -1279,55 +1555,90 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipEventRecord(lane.ev_join, lane.st2));
   }
 
-  // verbose >= 2: wall time of the host steps up to the integrals (each step
-  // synchronized first, so the times add up)
-  auto step_t0 = std::chrono::steady_clock::now();
-  auto vstep = [&](const char* what) {
-    if (opt.verbose < 2) return;
-    HC(hipStreamSynchronize(st));
-    const auto t = std::chrono::steady_clock::now();
-    vlog(opt, "  %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(t - step_t0).count());
-    step_t0 = t;
-  };
-  // pair part of the plan as soon as the Schwarz norms are back, while the
-  // metric runs; the packing and the bra records queue behind the metric
-  pl.qab = schwarz_bra_finish(pl.ps, spend);
-  vstep("Schwarz norms");
-  const auto hp0 = std::chrono::steady_clock::now();
-  build_plan_pairs(pl);
-  double ms_plan = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
-  vstep("pair plan");
-  DBuf<int> ps1, ps2;
-  DBuf<int64_t> prow;
-  upload(ps1, pl.ps.s1, st);
-  upload(ps2, pl.ps.s2, st);
-  upload(prow, pl.prow, st);
+  vstep("metric queued");
+  // pair part of the plan class by class as the Schwarz sums arrive (the
+  // sorts overlap the Schwarz kernels of the later classes and the metric);
+  // the sorted pair arrays and the bra-record layouts go up in one copy
+  const PairSet ps0 = pl.ps;
+  const int npairs = ps0.n();
+  std::vector<double> q0(npairs);
+  pl.qab.assign(npairs, 0.0);
+  pl.from.assign(npairs, 0);
+  pl.prow.assign(npairs, 0);
+  const bool gen = use_generated();
+  UploadBatch pb;
+  const size_t o_s1 = pb.add<int>(npairs), o_s2 = pb.add<int>(npairs), o_prow = pb.add<int64_t>(npairs);
+  const size_t o_qab = pb.add<double>(npairs), o_dprow = pb.add<int64_t>(need_dens ? npairs : 0);
+  std::vector<size_t> o_boff(NPCLS, 0), o_bnpp(NPCLS, 0);
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb), n = ps0.cls_end[k] - ps0.cls_begin[k];
+      if (n > 0 && gen && gen3_has(la, lb, 0)) {
+        o_boff[k] = pb.add<int64_t>(n);
+        o_bnpp[k] = pb.add<int>(n);
+      }
+    }
+  pb.allocate();
+  DView<int> ps1{pb.dev<int>(o_s1)}, ps2{pb.dev<int>(o_s2)};
+  DView<int64_t> prow{pb.dev<int64_t>(o_prow)}, ddprow{pb.dev<int64_t>(o_dprow)};
+  DView<double> dqab{pb.dev<double>(o_qab)};
+  std::vector<BraRecords> brs(NPCLS);
+  std::vector<int64_t> brtot(NPCLS, 0);
+  double ms_plan = 0.0;
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb), p0 = ps0.cls_begin[k], n = ps0.cls_end[k] - p0;
+      schwarz_class_finish(ps0, spend, k, q0.data());
+      if (n == 0) continue;
+      static const char* const wname[] = {"class 0 sums", "class 1 sums", "class 2 sums", "class 3 sums",
+                                          "class 4 sums", "class 5 sums", "class 6+ sums"};
+      static const char* const sname[] = {"class 0 sorted", "class 1 sorted", "class 2 sorted", "class 3 sorted",
+                                          "class 4 sorted", "class 5 sorted", "class 6+ sorted"};
+      if (opt.verbose >= 3) vstep(wname[std::min(k, 6)]);
+      const auto hp0 = std::chrono::steady_clock::now();
+      plan_pair_class(pl, ps0, q0, k, prow0[p0]);
+      ms_plan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
+      if (opt.verbose >= 3) vstep(sname[std::min(k, 6)]);
+      std::memcpy(pb.host<int>(o_s1) + p0, pl.ps.s1.data() + p0, n * sizeof(int));
+      std::memcpy(pb.host<int>(o_s2) + p0, pl.ps.s2.data() + p0, n * sizeof(int));
+      std::memcpy(pb.host<int64_t>(o_prow) + p0, pl.prow.data() + p0, n * sizeof(int64_t));
+      std::memcpy(pb.host<double>(o_qab) + p0, pl.qab.data() + p0, n * sizeof(double));
+      if (need_dens) {
+        // packed density row of each sorted pair's first function
+        int64_t* dprow = pb.host<int64_t>(o_dprow);
+        for (int r = p0; r < p0 + n; r++) dprow[r] = prow0[pl.from[r]];
+      }
+      if (gen && gen3_has(la, lb, 0)) {
+        brtot[k] = bra_record_layout(c.gen3[la][lb][0].rs, o, pl.ps.s1.data() + p0, pl.ps.s2.data() + p0, n,
+                                     pb.host<int64_t>(o_boff[k]), pb.host<int>(o_bnpp[k]));
+        brs[k].off = pb.dev<int64_t>(o_boff[k]);
+        brs[k].npp = pb.dev<int>(o_bnpp[k]);
+      }
+    }
+  // the upload, the bra records and the row maps run on the copy stream,
+  // beside the metric; the lane's stream waits for them before its first use
+  // of the sorted pairs
+  const hipStream_t s3 = lane.st3;
+  pb.commit(s3);
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++) {
+      const int k = pcls(la, lb), p0 = ps0.cls_begin[k], n = ps0.cls_end[k] - p0;
+      if (n > 0 && gen && gen3_has(la, lb, 0))
+        fill_bra_records(c, la, lb, dorb.view, ps1.p + p0, ps2.p + p0, n, brtot[k], brs[k], mixed, s3);
+    }
+  pl.npf = npf0;
   // per staging-matrix row (sorted pairs): its pair, for the contraction's
   // mask, and the packed density row it reads
   DBuf<int> drowpair, drowperm;
-  DBuf<int64_t> ddprow;
   if (need_dens) {
-    if (pl.npf != npf0) throw Error{DFHIP_ERR_HIP, "pair-function count changed by the sort"};
-    std::vector<int64_t> dprow(pl.ps.n());
-    for (int p = 0; p < pl.ps.n(); p++) dprow[p] = prow0[pl.from[p]];
-    upload(ddprow, dprow, st);
     drowpair.alloc((size_t)std::max<int64_t>(pl.npf, 1));
     drowperm.alloc((size_t)std::max<int64_t>(pl.npf, 1));
-    fill_rowpair(prow.p, pl.ps.n(), pl.npf, drowpair.p, ddprow.p, drowperm.p, st);
+    fill_rowpair(prow.p, npairs, pl.npf, drowpair.p, ddprow.p, drowperm.p, s3);
   }
-  const bool gen = use_generated();
-  std::vector<BraRecords> brs(NPCLS);
-  if (gen) {
-    for (int la = 0; la <= LMAX_ORB; la++)
-      for (int lb = 0; lb <= la; lb++) {
-        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
-        if (p1 == p0 || !gen3_has(la, lb, 0)) continue;
-        std::vector<int> a1(pl.ps.s1.begin() + p0, pl.ps.s1.begin() + p1);
-        std::vector<int> a2(pl.ps.s2.begin() + p0, pl.ps.s2.begin() + p1);
-        build_bra_records(c, la, lb, o, dorb.view, ps1.p + p0, ps2.p + p0, a1, a2, brs[k], mixed);
-      }
-  }
-  vstep("bra records");
+  HC(hipGetLastError());
+  HC(hipEventRecord(lane.ev_plan, s3));
+  HC(hipStreamWaitEvent(st, lane.ev_plan, 0));
+  vstep("pair plan, bra records");
 
   // aux part of the plan once Q_C is back (the packing may still run)
   HC(hipEventSynchronize(lane.ev_met));
@@ -1376,8 +1687,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         }
         if (list.empty()) continue;
         DBuf<int> dlist;
-        upload(dlist, list, st);
-        upload(etoff[k], off, st);
+        upload_pinned(dlist, list, st);
+        upload_pinned(etoff[k], off, st);
         etg[k].alloc((size_t)totd);
         c.et[la][lb].fn(dorb.view, ps1.p + p0, ps2.p + p0, dlist.p, etoff[k].p, (int)list.size(), c.sph.p, etg[k].p,
                         st);
@@ -1401,22 +1712,64 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
   vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
   const auto hp1 = std::chrono::steady_clock::now();
-  build_plan_aux(pl, opt);
+  build_aux_lists(pl);
+  // Schwarz screening (or none) in FP64: the prefixes come from the device,
+  // and the host's sweep (the counts, and the prefixes that the contraction's
+  // row segments use) runs once the integrals are queued
+  const bool dev_prefix =
+      !mixed && (opt.screening == DFHIP_SCREEN_SCHWARZ || opt.screening == DFHIP_SCREEN_NONE);
+  if (!dev_prefix) screen_sweep(pl, opt);
   ms_plan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp1).count();
   vlog(opt, "Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
   vlog(opt, "Aux. basis: %6d shells, %6d basis functions\n", x.n, x.nbf);
   vlog(opt, " Densities: %6d\n", ndens);
   vstep("aux plan");
 
-  // device plan arrays
-  std::vector<DBuf<int>> dalist(LMAX_AUX + 1);
-  std::vector<DBuf<int64_t>> dacol(LMAX_AUX + 1);
-  std::vector<int> kmax(LMAX_AUX + 1, 1);
+  // device plan arrays of the aux side in one copy: the lists and their
+  // columns per L, the Q_C of the lists (for the device prefixes), the
+  // original aux function of each column and, for the contraction's mask,
+  // the L and list index of each column
+  UploadBatch ab;
+  std::vector<size_t> o_al(LMAX_AUX + 1), o_ac(LMAX_AUX + 1);
+  size_t qtot = 0;
   for (int L = 0; L <= LMAX_AUX; L++) {
-    upload(dalist[L], pl.alist[L], st);
-    upload(dacol[L], pl.acol[L], st);
-    for (int sh : pl.alist[L]) kmax[L] = std::max(kmax[L], x.nprim[sh]);
+    o_al[L] = ab.add<int>(pl.alist[L].size());
+    o_ac[L] = ab.add<int64_t>(pl.acol[L].size());
+    qtot += pl.alist[L].size();
   }
+  const size_t o_qcs = ab.add<double>(qtot), o_lists = ab.add<int>(2 * (LMAX_AUX + 1));
+  const size_t o_orig = ab.add<int>(pl.orig.size()), o_colL = ab.add<int>(naux), o_colia = ab.add<int>(naux);
+  ab.allocate();
+  std::vector<DView<int>> dalist(LMAX_AUX + 1);
+  std::vector<DView<int64_t>> dacol(LMAX_AUX + 1);
+  std::vector<int> kmax(LMAX_AUX + 1, 1);
+  {
+    int* hl = ab.host<int>(o_lists);
+    double* hq = ab.host<double>(o_qcs);
+    int* colL = ab.host<int>(o_colL);
+    int* colia = ab.host<int>(o_colia);
+    size_t q = 0;
+    for (int L = 0; L <= LMAX_AUX; L++) {
+      const size_t nl = pl.alist[L].size();
+      std::copy(pl.alist[L].begin(), pl.alist[L].end(), ab.host<int>(o_al[L]));
+      std::copy(pl.acol[L].begin(), pl.acol[L].end(), ab.host<int64_t>(o_ac[L]));
+      hl[L] = (int)q;
+      hl[LMAX_AUX + 1 + L] = (int)nl;
+      for (size_t i = 0; i < nl; i++) {
+        hq[q + i] = pl.qc[pl.alist[L][i]];
+        for (int m = 0; m < 2 * L + 1; m++) {
+          colL[pl.acol[L][i] + m] = L;
+          colia[pl.acol[L][i] + m] = (int)i;
+        }
+      }
+      q += nl;
+      dalist[L].p = ab.dev<int>(o_al[L]);
+      dacol[L].p = ab.dev<int64_t>(o_ac[L]);
+      for (int sh : pl.alist[L]) kmax[L] = std::max(kmax[L], x.nprim[sh]);
+    }
+    std::copy(pl.orig.begin(), pl.orig.end(), ab.host<int>(o_orig));
+  }
+  ab.commit(st);
   QvlRecords qvl;
   const bool use_qvl = opt.screening == DFHIP_SCREEN_QVL;
   if (use_qvl) apply_qvl(pl, dorb.view, daux.view, ps1.p, ps2.p, dalist, opt.thresh, qvl, st);
@@ -1424,10 +1777,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // contraction's mask
   DBuf<int> dnsflat;
   {
-    std::vector<int> nsflat((size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1));
-    for (int L = 0; L <= LMAX_AUX; L++)
-      std::copy(pl.nsurv[L].begin(), pl.nsurv[L].end(), nsflat.begin() + (size_t)L * pl.ps.n());
-    upload(dnsflat, nsflat, st);
+    const size_t nf = (size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1);
+    dnsflat.alloc(nf);
+    if (dev_prefix) {
+      const int* dl = ab.dev<int>(o_lists);
+      schwarz_prefixes(dqab.p, pl.ps.n(), ab.dev<double>(o_qcs), dl, dl + LMAX_AUX + 1, LMAX_AUX + 1, opt.thresh,
+                       opt.screening == DFHIP_SCREEN_NONE, dnsflat.p, st);
+      HC(hipGetLastError());
+    } else {
+      int* nsflat = lane.arena.alloc<int>(nf);
+      for (int L = 0; L <= LMAX_AUX; L++)
+        std::copy(pl.nsurv[L].begin(), pl.nsurv[L].end(), nsflat + (size_t)L * pl.ps.n());
+      HC(hipMemcpyAsync(dnsflat.p, nsflat, nf * sizeof(int), hipMemcpyHostToDevice, st));
+    }
   }
   auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
   DBuf<int> dn64flat;  // FP64 windows of the prefixes, [L][pair] like dnsflat
@@ -1436,10 +1798,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     std::vector<int> f((size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1));
     for (int L = 0; L <= LMAX_AUX; L++)
       std::copy(pl.n64[L].begin(), pl.n64[L].end(), f.begin() + (size_t)L * pl.ps.n());
-    upload(dn64flat, f, st);
+    upload_pinned(dn64flat, f, st);
   }
-  DBuf<int> dorig;
-  upload(dorig, pl.orig, st);
+  const DView<int> dorig{ab.dev<int>(o_orig)};
   vstep("plan uploads");
   const int e5 = tm.mark();
 
@@ -1475,19 +1836,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   int nsplit = 1;
   // screened-prefix mask of T (the 3c2e kernels write only screened-in blocks)
   ScreenMask mk{};
-  DBuf<int> dcolL, dcolia;
-  if (need_dens) {
-    std::vector<int> colL(std::max<int64_t>(naux, 1)), colia(std::max<int64_t>(naux, 1));
-    for (int L = 0; L <= LMAX_AUX; L++)
-      for (size_t k = 0; k < pl.acol[L].size(); k++)
-        for (int m = 0; m < 2 * L + 1; m++) {
-          colL[pl.acol[L][k] + m] = L;
-          colia[pl.acol[L][k] + m] = (int)k;
-        }
-    upload(dcolL, colL, st);
-    upload(dcolia, colia, st);
-    mk = ScreenMask{drowpair.p, drowperm.p, dcolL.p, dcolia.p, dnsflat.p, pl.ps.n()};
-  }
+  if (need_dens) mk = ScreenMask{drowpair.p, drowperm.p, ab.dev<int>(o_colL), ab.dev<int>(o_colia), dnsflat.p, pl.ps.n()};
   vstep("column blocks, mask");
   if (need_dens) {
     const int64_t tiles = contract_ntiles(widest, ndens);
@@ -1605,16 +1954,24 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
           }
         }
     }
+    if (blk == 0) {
+      vstep("integrals queued");
+      if (dev_prefix) {
+        const auto t = std::chrono::steady_clock::now();
+        screen_sweep(pl, opt);
+        ms_plan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+      }
+    }
     const int b1 = tm.mark();
     if (need_dens) {
       // the packed densities come from the side stream
       if (blk == 0) HC(hipStreamWaitEvent(st, lane.ev_pack, 0));
       // d^T into the border rows: A[(naux + I) + lda * P]
       tile_segments(cA, w, hseg);
-      upload(dseg, hseg, st);
+      upload_pinned(dseg, hseg, st);
       ScreenMask mb = mk;
-      mb.colL = dcolL.p + cA;
-      mb.colia = dcolia.p + cA;
+      mb.colL = mk.colL + cA;
+      mb.colia = mk.colia + cA;
       contract_T(Tp, w, Dp, ndens, (int)w, ndens, NPCLS, dseg.p, mb, partp, nsplit, dorig.p + cA, Ap + 2 * naux, lda, 1,
                  st);
       HC(hipGetLastError());
@@ -1715,6 +2072,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       stats->class_prim_triplets[k] = pl.class_prim[k];
     }
   }
+  if (opt.verbose >= 3)
+    for (const auto& m : vmarks)
+      vlog(opt, "  %-28s host %8.3f ms, stream %8.3f ms\n", m.first, m.second.first, tm.ms(e0, m.second.second));
   if (opt.verbose) {
     std::fflush(stdout);
     std::printf("Schwarz norms took %8.4f s.\n", tm.ms(e1, e2) * 1e-3);
@@ -1757,8 +2117,10 @@ std::vector<double> schwarz_bra_host(const dfhip_basis& b) {
   HostShells o = parse_shells(b, LMAX_ORB, "basis");
   DevShellBufs d;
   d.upload_from(o, cur_lane().st);
-  PairSet ps = make_pairs(o);
-  std::vector<double> q = schwarz_bra_dev(c, o, d.view, ps);
+  cur_lane().arena.reset();
+  const PairClasses pc = pair_classes(o);
+  PairSet ps = make_pairs(o, pc);
+  std::vector<double> q = schwarz_bra_dev(c, o, d.view, pc, ps);
   std::vector<double> outp(ps.n());
   for (int p = 0; p < ps.n(); p++) outp[ps.packed[p]] = q[p];
   return outp;
@@ -1769,6 +2131,7 @@ std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
   ensure_ctx();
   Ctx& c = ctx();
   HostShells x = parse_shells(b, LMAX_AUX, "basis");
+  cur_lane().arena.reset();
   DevShellBufs d;
   d.upload_from(x, cur_lane().st);
   DBuf<double> M((size_t)x.nbf * x.nbf);

@@ -10,6 +10,8 @@
 
 namespace dfhip {
 
+// out = in^T for a row-major nr x nc matrix
+void transpose(const double* in, int64_t nr, int64_t nc, double* out, hipStream_t st);
 // Dp[r * ldd + I] for the npf rows r (pair functions; rowpair[r] = pair of r)
 void pack_density(const int* rowpair, const int* s1, const int* s2, const int64_t* row, int64_t npf, const int* L,
                   const int* f0, const double* D, int64_t sI, int64_t si, int64_t sj, int ndens, double* Dp, int ldd,
@@ -66,6 +68,29 @@ void qvl_pair_records(const DevShells& bra, const int* s1, const int* s2, const 
                       double* rec, hipStream_t st);
 void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int* alist, int nlist, int L,
                 const double* xrec, double thresh, int* nsurv, int* nkeep, int* nprimkeep, hipStream_t st);
+
+// Schwarz screened prefixes (screen.hip) of every pair and aux list L into
+// nsflat[L * npairs + p]: qab per pair, qcs the Q_C of the lists (sorted
+// non-increasing, list L at loff[L], nl[L] entries); all: no screening.
+void schwarz_prefixes(const double* qab, int npairs, const double* qcs, const int* loff, const int* nl, int nL,
+                      double thresh, bool all, int* nsflat, hipStream_t st);
+
+// Shell-pair classes (la >= lb) of a basis, for generating the pair list on
+// the device: the shells of each L in one list (ascending shell index per
+// L); per class its first pair, its shells' offsets in that list, the number
+// of lb shells, the functions per pair and its first row.  Inside a class
+// the pairs run over the la shells, then the lb shells (for la == lb only
+// the lb shell <= the la shell).
+constexpr int PC_MAX = (LMAX_ORB + 1) * (LMAX_ORB + 2) / 2;
+struct PairClassTable {
+  int ncls;
+  int begin[PC_MAX + 1];
+  int la[PC_MAX], lb[PC_MAX];
+  int off_a[PC_MAX], off_b[PC_MAX], n_b[PC_MAX], w[PC_MAX];
+  int64_t row0[PC_MAX];
+};
+// s1[p], s2[p] and (if prow0) the first row prow0[p] of every pair
+void gen_pairs(const PairClassTable& t, const int* shflat, int* s1, int* s2, int64_t* prow0, hipStream_t st);
 
 constexpr int CHOL_BLOCK = 64;
 

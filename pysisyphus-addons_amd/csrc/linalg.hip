@@ -17,6 +17,31 @@ namespace dfhip {
 //   D_I(f1+mu, f2+nu) + D_I(f2+nu, f1+mu)   (s1 != s2)
 //   D_I(f1+mu, f2+nu)                        (s1 == s2)
 // (the symmetry use of diabatization/intor.f90:162-169).
+// out[c * nr + r] = in[r * nc + c] (an nr x nc row-major matrix turned):
+// 32 x 64 tiles through LDS, reads and writes along rows.
+constexpr int TR_R = 32, TR_C = 64;
+__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ in, int64_t nr, int64_t nc,
+                                                        double* __restrict__ out) {
+  __shared__ double tile[TR_R][TR_C + 1];
+  const int64_t c0 = (int64_t)blockIdx.x * TR_C, r0 = (int64_t)blockIdx.y * TR_R;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < TR_R * TR_C; e += 256) {
+    const int r = e / TR_C, c = e % TR_C;
+    if (r0 + r < nr && c0 + c < nc) tile[r][c] = in[(r0 + r) * nc + c0 + c];
+  }
+  __syncthreads();
+  for (int e = tid; e < TR_R * TR_C; e += 256) {
+    const int c = e / TR_R, r = e % TR_R;
+    if (r0 + r < nr && c0 + c < nc) out[(c0 + c) * nr + r0 + r] = tile[r][c];
+  }
+}
+
+void transpose(const double* in, int64_t nr, int64_t nc, double* out, hipStream_t st) {
+  if (nr <= 0 || nc <= 0) return;
+  const dim3 grid((unsigned)((nc + TR_C - 1) / TR_C), (unsigned)((nr + TR_R - 1) / TR_R));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, st, in, nr, nc, out);
+}
+
 // One thread per packed entry: x over densities, y over rows (pair
 // functions), PK_ROWS rows per thread; the pair of a row comes from rowpair.
 constexpr int PK_ROWS = 4;
@@ -116,6 +141,36 @@ void pack_density_t(const int* pairof, const int* fshell, const int64_t* row, co
   for (int pass = 0; pass < 2; pass++)
     hipLaunchKernelGGL(pack_density_t_kernel, grid, dim3(256), 0, st, pairof, fshell, row, L, f0, nsh, nbf, D, sI,
                        ndens, Dp, ldd, pass);
+}
+
+// One thread per pair: its class by a scan of the class starts, then its
+// shells by the class's index order (triangular for la == lb).
+__global__ void gen_pairs_kernel(PairClassTable t, const int* __restrict__ shflat, int* __restrict__ s1,
+                                 int* __restrict__ s2, int64_t* __restrict__ prow0) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= t.begin[t.ncls]) return;
+  int k = 0;
+  while (k + 1 < t.ncls && p >= t.begin[k + 1]) k++;
+  const int i = p - t.begin[k];
+  int ia, ib;
+  if (t.la[k] != t.lb[k]) {
+    ia = i / t.n_b[k];
+    ib = i - ia * t.n_b[k];
+  } else {
+    ia = (int)((sqrt(8.0 * i + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(ia + 1) * (ia + 2) / 2 <= i) ia++;
+    while ((int64_t)ia * (ia + 1) / 2 > i) ia--;
+    ib = i - ia * (ia + 1) / 2;
+  }
+  s1[p] = shflat[t.off_a[k] + ia];
+  s2[p] = shflat[t.off_b[k] + ib];
+  if (prow0) prow0[p] = t.row0[k] + (int64_t)i * t.w[k];
+}
+
+void gen_pairs(const PairClassTable& t, const int* shflat, int* s1, int* s2, int64_t* prow0, hipStream_t st) {
+  const int np = t.begin[t.ncls];
+  if (np > 0)
+    hipLaunchKernelGGL(gen_pairs_kernel, dim3((np + 255) / 256), dim3(256), 0, st, t, shflat, s1, s2, prow0);
 }
 
 // rowpair[r] = p for the rows r = row[p] .. row[p+1]-1 of pair p (the last

@@ -81,6 +81,41 @@ __global__ void qvl_screen_kernel(const double* __restrict__ prec, int npairs, D
   nprimkeep[p] = pc;
 }
 
+// Schwarz screened prefixes (common/screening.f90 test Q_ab Q_C >= thresh,
+// the comparison of the host sweep): for pair p and aux list L (Q_C sorted
+// non-increasing in qcs[loff[L] .. loff[L] + nl[L])), the number of list
+// entries with !(Q_ab Q_C < thresh).  The products are non-increasing along
+// the list, so the entries that pass are a prefix: binary search for its
+// end.  all: no screening (the whole list).  One thread per (pair, L).
+__global__ void schwarz_prefix_kernel(const double* __restrict__ qab, int npairs, const double* __restrict__ qcs,
+                                      const int* __restrict__ loff, const int* __restrict__ nl, double thresh,
+                                      int all, int* __restrict__ nsflat) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, L = blockIdx.y;
+  if (p >= npairs) return;
+  const int n = nl[L];
+  int lo = 0, hi = n;
+  if (!all) {
+    const double qa = qab[p];
+    const double* q = qcs + loff[L];
+    // invariant: entries < lo pass, entries >= hi fail
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (qa * q[mid] < thresh) hi = mid;
+      else lo = mid + 1;
+    }
+  } else {
+    lo = n;
+  }
+  nsflat[(size_t)L * npairs + p] = lo;
+}
+
+void schwarz_prefixes(const double* qab, int npairs, const double* qcs, const int* loff, const int* nl, int nL,
+                      double thresh, bool all, int* nsflat, hipStream_t st) {
+  if (npairs <= 0 || nL <= 0) return;
+  hipLaunchKernelGGL(schwarz_prefix_kernel, dim3((npairs + 255) / 256, nL), dim3(256), 0, st, qab, npairs, qcs, loff,
+                     nl, thresh, all ? 1 : 0, nsflat);
+}
+
 void qvl_pair_records(const DevShells& bra, const int* s1, const int* s2, const double* qab, int npairs, double erfi,
                       double* rec, hipStream_t st) {
   if (npairs <= 0) return;
