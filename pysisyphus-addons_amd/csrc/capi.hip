@@ -336,14 +336,16 @@ struct Lane {
   PinnedBuf hpart, hqc;
   PinnedArena arena;
   // copy stream: the metric's inputs go up on it while the Schwarz kernels
-  // run (ev_mprep); the Schwarz sums come down on it in two shipments, all
-  // classes but the last as soon as their kernels end (ev_ship[0], on the
-  // host at ev_shipd[0]) while st goes on with the last class, then the last
-  // class; the sorted pairs go up and their bra records are built on it
+  // run (ev_mprep); the Schwarz sums come down on it in NSHIP shipments,
+  // each as soon as its classes' kernels end (ev_ship[i], on the host at
+  // ev_shipd[i]) while st goes on with the next classes: all classes but the
+  // last NSHIP - 1, then those one by one; the sorted pairs go up and their
+  // bra records are built on it
   // while the metric runs (ev_plan).  ev_pairs: the device pair list is
   // ready.
   hipStream_t st3 = nullptr;
-  hipEvent_t ev_mprep = nullptr, ev_pairs = nullptr, ev_plan = nullptr, ev_ship[2] = {}, ev_shipd[2] = {};
+  static constexpr int NSHIP = 3;
+  hipEvent_t ev_mprep = nullptr, ev_pairs = nullptr, ev_plan = nullptr, ev_ship[NSHIP] = {}, ev_shipd[NSHIP] = {};
   void ensure_events() {
     if (ev_met) return;
     HC(hipEventCreateWithFlags(&ev_met, hipEventDisableTiming));
@@ -351,7 +353,7 @@ struct Lane {
     HC(hipEventCreateWithFlags(&ev_mprep, hipEventDisableTiming));
     HC(hipEventCreateWithFlags(&ev_pairs, hipEventDisableTiming));
     HC(hipEventCreateWithFlags(&ev_plan, hipEventDisableTiming));
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < NSHIP; i++) {
       HC(hipEventCreateWithFlags(&ev_ship[i], hipEventDisableTiming));
       HC(hipEventCreateWithFlags(&ev_shipd[i], hipEventDisableTiming));
     }
@@ -711,8 +713,8 @@ struct SchwarzPending {
   DBuf<double> part, scr;
   std::vector<size_t> poff;
   std::vector<int> nch;
-  std::vector<int> ship;  // per class: the shipment (0 or 1) that brings its sums
-  bool shipped[2] = {false, false};
+  std::vector<int> ship;  // per class: the shipment that brings its sums
+  bool shipped[Lane::NSHIP] = {};
 };
 
 void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const PairClasses& pc, SchwarzPending& sp) {
@@ -770,15 +772,24 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
   DBuf<double>& scr = sp.scr;
   scr.alloc(std::min<size_t>(scr_need, (size_t)1 << 27));
   double* hp = lane.hpart.get<double>(poff[NPCLS]);
-  // shipments: the classes before the last non-empty one, then the rest
-  int last = -1, prev = -1;
+  // shipments: the last NSHIP - 1 non-empty classes one each, the classes
+  // before them in the first (shipment i ends after class shipend[i])
+  constexpr int NS = Lane::NSHIP;
+  std::vector<int> nonempty;
   for (int k = 0; k < NPCLS; k++)
-    if (ps.cls_end[k] > ps.cls_begin[k]) {
-      prev = last;
-      last = k;
-    }
-  sp.ship.assign(NPCLS, 1);
-  for (int k = 0; k <= prev; k++) sp.ship[k] = 0;
+    if (ps.cls_end[k] > ps.cls_begin[k]) nonempty.push_back(k);
+  int shipend[NS];
+  for (int i = 0; i < NS; i++) {
+    const int j = (int)nonempty.size() - NS + i;  // shipment i ends after nonempty[j]
+    shipend[i] = j >= 0 ? nonempty[j] : -1;
+  }
+  sp.ship.assign(NPCLS, NS - 1);
+  for (int k = NPCLS - 1; k >= 0; k--)
+    for (int i = 0; i < NS; i++)
+      if (k <= shipend[i]) {
+        sp.ship[k] = i;
+        break;
+      }
   auto ship = [&](int i, size_t b, size_t e) {
     HC(hipEventRecord(lane.ev_ship[i], st));
     HC(hipStreamWaitEvent(lane.st3, lane.ev_ship[i], 0));
@@ -799,9 +810,10 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
       else
         c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       HC(hipGetLastError());
-      if (k == prev) ship(0, 0, poff[prev + 1]);
+      for (int i = 0; i < NS - 1; i++)
+        if (k == shipend[i]) ship(i, i > 0 && shipend[i - 1] >= 0 ? poff[shipend[i - 1] + 1] : 0, poff[k + 1]);
     }
-  ship(1, prev >= 0 ? poff[prev + 1] : 0, poff[NPCLS]);
+  ship(NS - 1, NS > 1 && shipend[NS - 2] >= 0 ? poff[shipend[NS - 2] + 1] : 0, poff[NPCLS]);
 }
 
 // Q_ab of the pairs of class k (make_pairs order) into q[cls_begin[k] ..];
@@ -948,8 +960,9 @@ void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells
 // work runs), metric_launch queues the kernels on the lane's stream once the
 // caller has ordered that stream behind the preparation.
 struct MetricPrep {
-  DBuf<int> ds1, ds2, dns, dlsh;
+  DBuf<int> ds1, ds2, dns, dlsh, diota, dfL;
   DBuf<int64_t> drow, dlcol;
+  int64_t nbf = 0;
   std::vector<BraRecords> brs;
   std::vector<int> lbeg, kmax;
   bool gen = false;
@@ -978,6 +991,14 @@ void metric_prepare(Ctx& c, const HostShells& x, const DevShells& dev, MetricPre
     }
   }
   mp.lbeg[LMAX_AUX + 1] = (int)s1.size();
+  // the window [i, n) of bra i in its own L's list, and the L of each function
+  std::vector<int> iota(std::max<size_t>(s1.size(), 1)), fL(std::max(x.nbf, 1));
+  std::iota(iota.begin(), iota.end(), 0);
+  for (int i = 0; i < x.n; i++)
+    for (int m = 0; m < 2 * x.L[i] + 1; m++) fL[x.f0[i] + m] = x.L[i];
+  mp.nbf = x.nbf;
+  upload_pinned(mp.diota, iota, s);
+  upload_pinned(mp.dfL, fL, s);
   upload_pinned(mp.ds1, s1, s);
   upload_pinned(mp.ds2, s2, s);
   upload_pinned(mp.dns, nsurv, s);
@@ -993,24 +1014,32 @@ void metric_prepare(Ctx& c, const HostShells& x, const DevShells& dev, MetricPre
   }
 }
 
+// One side of the symmetric metric: the blocks with bra L < ket L, and for
+// equal L the ket shells from the bra's own on (the window [i, n) of the
+// generated kernels; the other kernels take the whole list), then the other
+// side by symmetry.
 void metric_launch(Ctx& c, const DevShells& dev, const MetricPrep& mp, double* M, int64_t ldm) {
   hipStream_t st = cur_lane().st;
   for (int LP = 0; LP <= LMAX_AUX; LP++) {
     const int np = mp.lbeg[LP + 1] - mp.lbeg[LP];
     if (np == 0) continue;
     PairArgs pa{mp.ds1.p + mp.lbeg[LP], mp.ds2.p + mp.lbeg[LP], mp.drow.p + mp.lbeg[LP], mp.dns.p + mp.lbeg[LP], np};
-    for (int LQ = 0; LQ <= LMAX_AUX; LQ++) {
+    for (int LQ = LP; LQ <= LMAX_AUX; LQ++) {
       const int nq = mp.lbeg[LQ + 1] - mp.lbeg[LQ];
       if (nq == 0) continue;
       AuxArgs xa{mp.dlsh.p + mp.lbeg[LQ], mp.dlcol.p + mp.lbeg[LQ], 0, nq, mp.kmax[LQ]};
       if (mp.gen && gen3_has(LP, 0, LQ)) {
-        launch_gen3(c, LP, 0, LQ, mp.brs[LP], dev, pa, xa, M, ldm);
+        PairArgs pw = pa;
+        if (LQ == LP) pw.nbeg = mp.diota.p;
+        launch_gen3(c, LP, 0, LQ, mp.brs[LP], dev, pw, xa, M, ldm);
       } else {
         c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, ldm, st);
         HC(hipGetLastError());
       }
     }
   }
+  symmetrize_metric(M, ldm, mp.nbf, mp.dfL.p, st);
+  HC(hipGetLastError());
 }
 
 // Both parts on the lane's stream.  No wait: the buffers go back to the
@@ -1636,39 +1665,40 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     fill_rowpair(prow.p, npairs, pl.npf, drowpair.p, ddprow.p, drowperm.p, s3);
   }
   HC(hipGetLastError());
-  HC(hipEventRecord(lane.ev_plan, s3));
-  HC(hipStreamWaitEvent(st, lane.ev_plan, 0));
-  vstep("pair plan, bra records");
 
-  // aux part of the plan once Q_C is back (the packing may still run)
-  HC(hipEventSynchronize(lane.ev_met));
-  pl.qc.assign(hqc, hqc + x.n);
-  for (const double q : pl.qc)
-    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
-
-  // Et tables of the classes run on the matrix-core 3c2e kernels: the pairs
-  // with screened-in triplets in any such aux L, built once for all column
-  // blocks (a class whose tables would take more than a sixth of the free
-  // HBM, the staging matrix this lane holds counted as free, keeps the LDS
-  // kernels).  need(k, L, p): pair p (global index) of class k has
-  // screened-in triplets with aux L.
+  // Et tables of the classes run on the matrix-core 3c2e kernels, built once
+  // for all column blocks after the aux plan, for the pairs with screened-in
+  // triplets in any such aux L (a class whose tables would take more than a
+  // sixth of the free HBM, the staging matrix this lane holds counted as
+  // free, keeps the LDS kernels).  need(k, L, p): pair p (global index) of
+  // class k is to be tabled for aux L; defer: leave a class whose table is
+  // too large for a later call instead.
   std::vector<DBuf<double>> etg(NPCLS);
   std::vector<DBuf<int64_t>> etoff(NPCLS);
+  std::vector<DBuf<int>> etlist(NPCLS);
   std::vector<std::vector<char>> use_mm(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
-  auto build_et = [&](auto need) {
-    if (!use_matrix_cores_3c()) return;
-    size_t fr = 0, tot = 0;
-    HC(hipMemGetInfo(&fr, &tot));
-    const size_t et_cap = (fr + cur_lane().wsT.n * sizeof(double)) / 6;
+  size_t et_cap = 0;
+  if (use_matrix_cores_3c()) {
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb);
+        if (pl.ps.cls_end[k] == pl.ps.cls_begin[k] || c.et[la][lb].fn == nullptr) continue;
+        for (int L = 0; L <= LMAX_AUX; L++)
+          use_mm[k][L] = (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
+      }
+  }
+  auto build_et = [&](auto need, hipStream_t es, bool defer) {
+    if (et_cap == 0) {
+      size_t fr = 0, tot = 0;
+      HC(hipMemGetInfo(&fr, &tot));
+      et_cap = std::max<size_t>((fr + cur_lane().wsT.n * sizeof(double)) / 6, 1);
+    }
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
         const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
-        if (np == 0 || c.et[la][lb].fn == nullptr) continue;
+        if (np == 0 || etg[k].p) continue;
         bool any = false;
-        for (int L = 0; L <= LMAX_AUX; L++) {
-          use_mm[k][L] = (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
-          any = any || use_mm[k][L];
-        }
+        for (int L = 0; L <= LMAX_AUX; L++) any = any || use_mm[k][L];
         if (!any) continue;
         std::vector<int> list;
         std::vector<int64_t> off(np, 0);
@@ -1682,32 +1712,40 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
           totd += (int64_t)o.nprim[pl.ps.s1[p0 + p]] * o.nprim[pl.ps.s2[p0 + p]] * c.et[la][lb].etpp;
         }
         if ((size_t)totd * sizeof(double) > et_cap) {
-          for (int L = 0; L <= LMAX_AUX; L++) use_mm[k][L] = 0;
+          if (!defer)
+            for (int L = 0; L <= LMAX_AUX; L++) use_mm[k][L] = 0;
           continue;
         }
         if (list.empty()) continue;
-        DBuf<int> dlist;
-        upload_pinned(dlist, list, st);
-        upload_pinned(etoff[k], off, st);
+        upload_pinned(etlist[k], list, es);
+        upload_pinned(etoff[k], off, es);
         etg[k].alloc((size_t)totd);
-        c.et[la][lb].fn(dorb.view, ps1.p + p0, ps2.p + p0, dlist.p, etoff[k].p, (int)list.size(), c.sph.p, etg[k].p,
-                        st);
+        c.et[la][lb].fn(dorb.view, ps1.p + p0, ps2.p + p0, etlist[k].p, etoff[k].p, (int)list.size(), c.sph.p,
+                        etg[k].p, es);
         HC(hipGetLastError());
       }
   };
-  // Under Schwarz screening (or none) the pairs with screened-in triplets
-  // are known before the aux plan: nsurv[L][p] > 0 exactly when Q_ab times
-  // the largest Q_C of aux L passes, the test of the plan's sweep; the Et
-  // kernels then run while the host builds the aux plan.
-  bool et_done = false;
+
+  HC(hipEventRecord(lane.ev_plan, s3));
+  HC(hipStreamWaitEvent(st, lane.ev_plan, 0));
+  vstep("pair plan, bra records");
+
+  // aux part of the plan once Q_C is back (the packing may still run)
+  HC(hipEventSynchronize(lane.ev_met));
+  pl.qc.assign(hqc, hqc + x.n);
+  for (const double q : pl.qc)
+    if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
+
+  // the rest of the Et tables: the pairs with screened-in triplets (under
+  // Schwarz screening, or none, nsurv[L][p] > 0 exactly when Q_ab times the
+  // largest Q_C of aux L passes, the test of the plan's sweep)
   if (opt.screening != DFHIP_SCREEN_QVL) {
     std::vector<double> qcmax(LMAX_AUX + 1, -1.0);
     for (int i = 0; i < x.n; i++) qcmax[x.L[i]] = std::max(qcmax[x.L[i]], pl.qc[i]);
     const bool all = opt.screening == DFHIP_SCREEN_NONE;
-    build_et([&](int k, int L, int p) {
-      return qcmax[L] >= 0.0 && (all || !(pl.qab[p] * qcmax[L] < opt.thresh));
-    });
-    et_done = true;
+    build_et(
+        [&](int k, int L, int p) { return qcmax[L] >= 0.0 && (all || !(pl.qab[p] * qcmax[L] < opt.thresh)); }, st,
+        false);
   }
   vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
   vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
@@ -1848,7 +1886,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // rows from the class start to the last pair whose screened prefix reaches
   // the tile (under Schwarz screening these pairs are exactly a prefix of
   // the class: pairs are sorted by Q_ab and the prefixes are non-increasing).
-  if (!et_done) build_et([&](int k, int L, int p) { return pl.nsurv[L][p] > 0; });
+  if (opt.screening == DFHIP_SCREEN_QVL) build_et([&](int k, int L, int p) { return pl.nsurv[L][p] > 0; }, st, false);
   vstep("Et tables");
   const int GT = contract_tile();
   // suffix maxima of the prefixes inside each class: under Schwarz screening

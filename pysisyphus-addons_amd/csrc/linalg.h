@@ -75,6 +75,10 @@ void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int*
 void schwarz_prefixes(const double* qab, int npairs, const double* qcs, const int* loff, const int* nl, int nL,
                       double thresh, bool all, int* nsflat, hipStream_t st);
 
+// The metric from its computed side (linalg.hip): fL[u] = L of the shell of
+// aux function u.
+void symmetrize_metric(double* M, int64_t ldm, int64_t n, const int* fL, hipStream_t st);
+
 // Shell-pair classes (la >= lb) of a basis, for generating the pair list on
 // the device: the shells of each L in one list (ascending shell index per
 // L); per class its first pair, its shells' offsets in that list, the number

@@ -578,6 +578,39 @@ void cholesky_inverse_rows(double* A, int64_t lda, int64_t n, int* info, hipStre
   }
 }
 
+// The metric computed on one side only: for functions u != v the block of
+// the shell with the smaller L as bra (equal L: the smaller function index)
+// was written at M[bra * ldm + ket]; copy it to M[ket * ldm + bra].  32 x 32
+// tiles: the tile at (a0, b0) reads its mirror tile along rows into LDS.
+__global__ __launch_bounds__(256) void symmetrize_metric_kernel(double* __restrict__ M, int64_t ldm, int64_t n,
+                                                                const int* __restrict__ fL) {
+  __shared__ double t[32][33];
+  const int64_t a0 = (int64_t)blockIdx.y * 32, b0 = (int64_t)blockIdx.x * 32;
+  const int tid = threadIdx.x;
+  // mirror tile: element (a, b) of this tile is M[b * ldm + a]
+  for (int e = tid; e < 1024; e += 256) {
+    const int bi = e >> 5, ai = e & 31;
+    const int64_t a = a0 + ai, b = b0 + bi;
+    if (a < n && b < n) t[bi][ai] = M[b * ldm + a];
+  }
+  __syncthreads();
+  for (int e = tid; e < 1024; e += 256) {
+    const int ai = e >> 5, bi = e & 31;
+    const int64_t a = a0 + ai, b = b0 + bi;
+    if (a >= n || b >= n || a == b) continue;
+    const int La = fL[a], Lb = fL[b];
+    // (a, b) was computed when a is the bra: smaller L, or equal L and a < b
+    const bool computed = La < Lb || (La == Lb && a < b);
+    if (!computed) M[a * ldm + b] = t[bi][ai];
+  }
+}
+
+void symmetrize_metric(double* M, int64_t ldm, int64_t n, const int* fL, hipStream_t st) {
+  if (n <= 0) return;
+  const dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+  hipLaunchKernelGGL(symmetrize_metric_kernel, grid, dim3(256), 0, st, M, ldm, n, fL);
+}
+
 // A[n + i + lda i] = 1 for i < n (the identity border of cholesky_inverse_rows).
 __global__ void set_identity_rows_kernel(double* __restrict__ A, int64_t lda, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
