@@ -412,6 +412,12 @@ std::mutex& ctx_mutex() {
 thread_local Lane* t_lane = nullptr;
 Lane& cur_lane() { return t_lane ? *t_lane : ctx().lane0; }
 
+// A batch runs several lanes at once: their copy-stream work goes on each
+// lane's own stream instead, so that the streams of all lanes stay within
+// the hardware queues (work of streams that share a queue serializes).
+thread_local bool t_few_streams = false;
+hipStream_t copy_stream(Lane& l) { return t_few_streams ? l.st : l.st3; }
+
 // Uploads through the lane's pinned arena, for uploads queued behind running
 // kernels (see PinnedArena).
 template <class T>
@@ -792,9 +798,10 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
       }
   auto ship = [&](int i, size_t b, size_t e) {
     HC(hipEventRecord(lane.ev_ship[i], st));
-    HC(hipStreamWaitEvent(lane.st3, lane.ev_ship[i], 0));
-    if (e > b) HC(hipMemcpyAsync(hp + b, part.p + b, (e - b) * sizeof(double), hipMemcpyDeviceToHost, lane.st3));
-    HC(hipEventRecord(lane.ev_shipd[i], lane.st3));
+    const hipStream_t s3 = copy_stream(lane);
+    HC(hipStreamWaitEvent(s3, lane.ev_ship[i], 0));
+    if (e > b) HC(hipMemcpyAsync(hp + b, part.p + b, (e - b) * sizeof(double), hipMemcpyDeviceToHost, s3));
+    HC(hipEventRecord(lane.ev_shipd[i], s3));
     sp.shipped[i] = true;
   };
   for (int la = 0; la <= LMAX_ORB; la++)
@@ -1471,9 +1478,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
 
   // the metric's inputs go up on the copy stream while the Schwarz kernels run
   MetricPrep mprep;
-  HC(hipStreamWaitEvent(lane.st3, tm.ev[e1], 0));
-  metric_prepare(c, x, daux.view, mprep, lane.st3);
-  HC(hipEventRecord(lane.ev_mprep, lane.st3));
+  HC(hipStreamWaitEvent(copy_stream(lane), tm.ev[e1], 0));
+  metric_prepare(c, x, daux.view, mprep, copy_stream(lane));
+  HC(hipEventRecord(lane.ev_mprep, copy_stream(lane)));
   vstep("metric inputs queued");
 
   // densities: upload and pair-pack (diabatization/intor.f90:155-176
@@ -1647,7 +1654,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // the upload, the bra records and the row maps run on the copy stream,
   // beside the metric; the lane's stream waits for them before its first use
   // of the sorted pairs
-  const hipStream_t s3 = lane.st3;
+  const hipStream_t s3 = copy_stream(lane);
   pb.commit(s3);
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
@@ -2258,6 +2265,10 @@ int dfhip_contract_coulomb_densities_batch(int32_t nmol, const dfhip_basis* bas,
     auto worker = [&](int w) {
       t_lane = w == 0 ? &c.lane0 : c.lanes[w - 1].get();
       t_pool = w == 0 ? nullptr : &t_lane->pool;
+      t_few_streams = nl > 1;
+      struct Reset {
+        ~Reset() { t_few_streams = false; }
+      } reset_streams;
       if (hipSetDevice(c.device) != hipSuccess) {
         std::lock_guard<std::mutex> g(emu);
         if (ecode == DFHIP_OK) {
