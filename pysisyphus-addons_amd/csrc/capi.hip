@@ -2070,7 +2070,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       HC(hipGetLastError());
     }
     if (out.J) {
-      dGp.alloc((size_t)gram_chunks(naux) * ndens * ndens);
+      dGp.alloc((size_t)gram_chunks(naux, ndens) * ndens * ndens);
       dG.alloc((size_t)ndens * ndens);
       gram_neg(dBT.p, naux, ndens, dGp.p, dG.p, st);
       const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;

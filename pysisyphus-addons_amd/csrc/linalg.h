@@ -52,8 +52,8 @@ void set_identity_rows(double* A, int64_t lda, int64_t n, hipStream_t st);
 int64_t apply_linv_part_size(int64_t n, int ndens);
 void apply_linv(const double* DT, int64_t ldd, const double* LT, int64_t ldl, int64_t n, int ndens, double* part,
                 double* BT, hipStream_t st);
-// G (ndens x ndens, column-major) = -BT BT^T; part holds gram_chunks(n) ndens^2 doubles
-int gram_chunks(int64_t n);
+// G (ndens x ndens, column-major) = -BT BT^T; part holds gram_chunks(n, ndens) ndens^2 doubles
+int gram_chunks(int64_t n, int ndens);
 void gram_neg(const double* BT, int64_t n, int ndens, double* part, double* G, hipStream_t st);
 void bt_to_b(const double* BT, int64_t n, int ndens, double* B, hipStream_t st);
 void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st);

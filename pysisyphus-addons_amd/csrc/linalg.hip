@@ -682,7 +682,9 @@ __global__ void apply_linv_reduce_kernel(const double* __restrict__ part, int64_
   BT[e] = s;
 }
 
-int64_t apply_linv_chunk(int64_t n) { return std::max<int64_t>(256, (n + 15) / 16); }
+// K chunk of apply_linv: 64 at naux <= 2048 (a few hundred workgroups of
+// four K steps for the small triangle), at most 32 chunks above
+int64_t apply_linv_chunk(int64_t n) { return std::max<int64_t>(64, ((n + 31) / 32 + 15) / 16 * 16); }
 int64_t apply_linv_part_size(int64_t n, int ndens) {
   const int64_t kc = apply_linv_chunk(n);
   return ((n + kc - 1) / kc) * ndens * n;
@@ -701,14 +703,18 @@ void apply_linv(const double* DT, int64_t ldd, const double* LT, int64_t ldl, in
 
 // G[I + ndens K] = -sum_j BT[I + ndens j] BT[K + ndens j] (the packed Coulomb
 // tensor with a minus sign, as the trailing block of the bordered
-// factorization): partial sums over chunks of 512 columns j, then a reduction
-// over the chunks in a fixed order.
-constexpr int GJ = 512;
+// factorization): partial sums over chunks of gj columns j (about 512
+// workgroups in all), then a reduction over the chunks in a fixed order.
+int64_t gram_jchunk(int64_t n, int ndens) {
+  const int64_t t = (ndens + 63) / 64, want = std::max<int64_t>(1, 512 / (t * t));
+  return std::max<int64_t>(16, ((n + want - 1) / want + 15) / 16 * 16);
+}
+
 __global__ __launch_bounds__(256) void gram_part_kernel(const double* __restrict__ BT, int64_t n, int ndens,
-                                                        double* __restrict__ part) {
+                                                        int64_t gj, double* __restrict__ part) {
   __shared__ double sA[16][64 + 1], sB[16][64 + 1];
   const int I0 = blockIdx.x * 64, K0 = blockIdx.y * 64;
-  const int64_t jc = (int64_t)blockIdx.z * GJ, jend = min(n, jc + GJ);
+  const int64_t jc = (int64_t)blockIdx.z * gj, jend = min(n, jc + gj);
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   double acc[4][4] = {};
   for (int64_t k0 = jc; k0 < jend; k0 += 16) {
@@ -751,13 +757,17 @@ __global__ void gram_reduce_kernel(const double* __restrict__ part, int nchunk, 
   G[e] = -s;
 }
 
-int gram_chunks(int64_t n) { return (int)std::max<int64_t>(1, (n + GJ - 1) / GJ); }
+int gram_chunks(int64_t n, int ndens) {
+  const int64_t gj = gram_jchunk(n, ndens);
+  return (int)std::max<int64_t>(1, (n + gj - 1) / gj);
+}
 
 void gram_neg(const double* BT, int64_t n, int ndens, double* part, double* G, hipStream_t st) {
   if (ndens <= 0) return;
-  const int nchunk = gram_chunks(n);
+  const int nchunk = gram_chunks(n, ndens);
   const unsigned t = (unsigned)((ndens + 63) / 64);
-  hipLaunchKernelGGL(gram_part_kernel, dim3(t, t, nchunk), dim3(256), 0, st, BT, n, ndens, part);
+  hipLaunchKernelGGL(gram_part_kernel, dim3(t, t, nchunk), dim3(256), 0, st, BT, n, ndens, gram_jchunk(n, ndens),
+                     part);
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((ndens * ndens + 255) / 256), dim3(256), 0, st, part, nchunk, ndens, G);
 }
 
