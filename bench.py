@@ -196,7 +196,9 @@ def main():
                 ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, dens_d.data_ptr(),
                 ctypes.byref(opt), None, nstates, J.ctypes.data, ctypes.byref(st))
             N.check(s)
-            return st.as_dict()
+            # a raw copy of the stats: the conversion to a dict waits until
+            # the timed loop is over
+            return N.Stats.from_buffer_copy(st)
 
         for _ in range(args.warmup):
             step()
@@ -207,7 +209,7 @@ def main():
             recs.append(step())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        return dt / args.steps * 1e3, recs, J.copy()
+        return dt / args.steps * 1e3, [r.as_dict() for r in recs], J.copy()
 
     ms_per_step, recs, J64 = run(N.default_options(**base_opt))
     print(f"fp64: {ms_per_step:.2f} ms per step", file=sys.stderr, flush=True)

@@ -1392,6 +1392,7 @@ struct Outputs {
 // context (a batch worker on its own lane).
 void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const double* dens,
          const dfhip_options& opt, Outputs out, dfhip_stats* stats, bool take_lock = true) {
+  const auto entry_t = std::chrono::steady_clock::now();
   if (opt.precision != DFHIP_FP64 && opt.precision != DFHIP_MIXED)
     throw Error{DFHIP_ERR_UNSUPPORTED, "unknown precision"};
   const bool mixed = opt.precision == DFHIP_MIXED;
@@ -1685,13 +1686,16 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   std::vector<DBuf<int>> etlist(NPCLS);
   std::vector<std::vector<char>> use_mm(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
   size_t et_cap = 0;
+  std::vector<char> haveL(LMAX_AUX + 1, 0);  // aux L present in the basis
+  for (int i = 0; i < x.n; i++) haveL[x.L[i]] = 1;
   if (use_matrix_cores_3c()) {
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
         const int k = pcls(la, lb);
         if (pl.ps.cls_end[k] == pl.ps.cls_begin[k] || c.et[la][lb].fn == nullptr) continue;
         for (int L = 0; L <= LMAX_AUX; L++)
-          use_mm[k][L] = (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) && c.int3m[la][lb][L].fn != nullptr;
+          use_mm[k][L] = haveL[L] && (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) &&
+                         c.int3m[la][lb][L].fn != nullptr;
       }
   }
   auto build_et = [&](auto need, hipStream_t es, bool defer) {
@@ -1735,6 +1739,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
 
   HC(hipEventRecord(lane.ev_plan, s3));
   HC(hipStreamWaitEvent(st, lane.ev_plan, 0));
+  // Et tables of all pairs of a class where they fit, queued behind the
+  // metric while the host waits for it anyway (the rest after the aux plan)
+  if (opt.screening != DFHIP_SCREEN_QVL) build_et([](int, int, int) { return true; }, st, true);
   vstep("pair plan, bra records");
 
   // aux part of the plan once Q_C is back (the packing may still run)
@@ -1742,6 +1749,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.qc.assign(hqc, hqc + x.n);
   for (const double q : pl.qc)
     if (!std::isfinite(q)) throw Error{DFHIP_ERR_HIP, "non-finite auxiliary Schwarz norm"};
+  vstep("Q_C on the host");
 
   // the rest of the Et tables: the pairs with screened-in triplets (under
   // Schwarz screening, or none, nsurv[L][p] > 0 exactly when Q_ab times the
@@ -1754,8 +1762,11 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         [&](int k, int L, int p) { return qcmax[L] >= 0.0 && (all || !(pl.qab[p] * qcmax[L] < opt.thresh)); }, st,
         false);
   }
-  vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
-  vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
+  vstep("Et tables queued");
+  if (opt.verbose) {
+    vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
+    vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
+  }
   const auto hp1 = std::chrono::steady_clock::now();
   build_aux_lists(pl);
   // Schwarz screening (or none) in FP64: the prefixes come from the device,
@@ -2116,6 +2127,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       stats->class_triplets[k] = pl.class_trip[k];
       stats->class_prim_triplets[k] = pl.class_prim[k];
     }
+  }
+  if (opt.verbose >= 3) {
+    vlog(opt, "  %-28s host %8.3f ms before the first event\n", "call entry",
+         std::chrono::duration<double, std::milli>(run_t0 - entry_t).count());
+    vlog(opt, "  %-28s host %8.3f ms\n", "results on the host",
+         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - run_t0).count());
   }
   if (opt.verbose >= 3)
     for (const auto& m : vmarks)
