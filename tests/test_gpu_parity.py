@@ -412,6 +412,43 @@ def test_batch_matches_single_calls():
     assert errm <= 1e-8
 
 
+def test_column_blocks_and_device_densities():
+    """The staging matrix in several column blocks (a small max_tile_bytes)
+    and densities already in HBM (dens_on_device, both layouts) give the
+    one-block host-density result and the oracle's J."""
+    import torch
+
+    shells, aux, dens = synthetic.workload(0)
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    _, J0, st0 = _intor.contract(b, x, dens, nstates=3, want_df=False, want_j=True)
+    Jref = O.contract_coulomb_densities_4d(shells, aux, dens)
+    assert np.abs(J0 - Jref).max() <= 1e-10
+    # about five column blocks
+    tile = st0["npair_functions"] * 8 * max(1, aux.nbf // 5)
+    _, J1, st1 = _intor.contract(b, x, dens, nstates=3, want_df=False, want_j=True,
+                                 options=N.default_options(max_tile_bytes=tile))
+    _say(f"column blocks: {st1['ncolumn_blocks']}, max |dJ| vs one block {np.abs(J1 - J0).max():.2e}")
+    assert st1["ncolumn_blocks"] >= 3
+    assert st1["ncomputed"] == st0["ncomputed"]
+    assert np.abs(J1 - J0).max() <= 1e-12 * max(np.abs(J0).max(), 1.0)
+    ndens, nbf = dens.shape[0], dens.shape[1]
+    # the memory of (ndens, nbf, nbf) in C order and in column-major order
+    for layout, host in ((N.DENS_C, np.ascontiguousarray(dens).ravel()),
+                         (N.DENS_FORTRAN, np.asfortranarray(dens).ravel(order="F"))):
+        d = torch.from_numpy(host).to("cuda:0")
+        torch.cuda.synchronize()
+        J2 = np.zeros((3,) * 4, order="F")
+        st = N.Stats()
+        opt = N.default_options(dens_layout=layout, dens_on_device=1)
+        N.check(N.lib().dfhip_contract_coulomb_densities(
+            ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, d.data_ptr(), ctypes.byref(opt), None, 3,
+            J2.ctypes.data, ctypes.byref(st)))
+        err = np.abs(J2 - J0).max()
+        _say(f"device densities, layout {layout}: max |dJ| vs host densities {err:.2e}")
+        assert err <= 1e-12 * max(np.abs(J0).max(), 1.0)
+
+
 def test_bitwise_reproducible():
     shells, aux, dens = synthetic.workload(0)
     J1 = intor.contract_coulomb_densities_4d(shells, aux, dens)
