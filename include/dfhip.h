@@ -124,13 +124,16 @@ typedef struct {
     int64_t nskipped;
     int64_t nbf, naux, npair_functions;
     int32_t ncolumn_blocks;
-    /* phase times in ms (HIP events on the library stream); the metric
-       Cholesky runs on a second stream during the integrals, ms_cholesky is
-       the part of it still running after the contraction, ms_assemble covers
-       B^T = d^T L^-T, -B^T B and the J assembly */
+    /* phase times in ms (HIP events): ms_schwarz, ms_pack (the set-up from
+       the end of the Schwarz kernels to the first 3c2e kernel),
+       ms_int3c2e, ms_contract and ms_assemble (B^T = d^T L^-T, -B^T B and
+       the J assembly) on the library's stream; the density packing, the
+       metric (ms_metric) and its Cholesky run on a second stream beside the
+       Schwarz kernels and the integrals, ms_cholesky is the part of the
+       Cholesky still running after the contraction */
     double ms_upload, ms_schwarz, ms_metric, ms_cholesky, ms_int3c2e, ms_pack, ms_contract,
         ms_solve, ms_assemble, ms_download, ms_total;
-    double ms_h2d_densities; /* part of ms_pack spent copying host densities */
+    double ms_h2d_densities; /* copying host densities (second stream) */
     double ms_host_plan;     /* host time: pair sorting, screening prefixes, tables */
     /* screened-in triplets and primitive triplets per class, index
        (la*(la+1)/2 + lb) * 6 + lc with la >= lb the pair's angular momenta */

@@ -950,13 +950,14 @@ void build_bra_records(Ctx& c, int la, int lb, const HostShells& bra, const DevS
 }
 
 void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells ket, PairArgs pa, AuxArgs xa,
-                 double* out, int64_t ldo, bool fp32 = false) {
+                 double* out, int64_t ldo, bool fp32 = false, hipStream_t st = nullptr) {
+  if (!st) st = cur_lane().st;
   const Gen3Launch& g = c.gen3[la][lb][lc];
   const int nchunks = (xa.hi - xa.lo + 63) / 64;
   if (pa.npairs <= 0 || nchunks <= 0) return;
   if (fp32 && !br.recs32.p) throw Error{DFHIP_ERR_HIP, "FP32 bra records missing"};
   hipLaunchKernelGGL(fp32 ? g.kernel32 : g.kernel, dim3(pa.npairs, nchunks, fp32 ? g.groups32 : g.groups), dim3(64),
-                     0, cur_lane().st, br.recs.p,
+                     0, st, br.recs.p,
                      br.recs32.p, br.off, br.npp, ket, pa, xa, c.boys.p, out, ldo);
   HC(hipGetLastError());
 }
@@ -1025,8 +1026,9 @@ void metric_prepare(Ctx& c, const HostShells& x, const DevShells& dev, MetricPre
 // equal L the ket shells from the bra's own on (the window [i, n) of the
 // generated kernels; the other kernels take the whole list), then the other
 // side by symmetry.
-void metric_launch(Ctx& c, const DevShells& dev, const MetricPrep& mp, double* M, int64_t ldm) {
-  hipStream_t st = cur_lane().st;
+void metric_launch(Ctx& c, const DevShells& dev, const MetricPrep& mp, double* M, int64_t ldm,
+                   hipStream_t st = nullptr) {
+  if (!st) st = cur_lane().st;
   for (int LP = 0; LP <= LMAX_AUX; LP++) {
     const int np = mp.lbeg[LP + 1] - mp.lbeg[LP];
     if (np == 0) continue;
@@ -1038,7 +1040,7 @@ void metric_launch(Ctx& c, const DevShells& dev, const MetricPrep& mp, double* M
       if (mp.gen && gen3_has(LP, 0, LQ)) {
         PairArgs pw = pa;
         if (LQ == LP) pw.nbeg = mp.diota.p;
-        launch_gen3(c, LP, 0, LQ, mp.brs[LP], dev, pw, xa, M, ldm);
+        launch_gen3(c, LP, 0, LQ, mp.brs[LP], dev, pw, xa, M, ldm, false, st);
       } else {
         c.int3[LP][0][LQ].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, M, ldm, st);
         HC(hipGetLastError());
@@ -1564,32 +1566,35 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   vstep("packing queued");
 
   // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal
-  // blocks, queued behind the Schwarz kernels.
+  // blocks: on the side stream behind the packing (beside the Schwarz
+  // kernels; without densities on the lane's stream).
   // A = [[M], [I], [D^T]]: naux columns, rows 0..naux-1 the metric, then the
   // identity border whose factorization gives L^-T, then d^T (ndens rows)
+  const hipStream_t sm = need_dens ? lane.st2 : st;
   const int64_t lda = need_dens ? 2 * naux + ndens : naux;
   double* Ap = reserve(cur_lane().wsA, (size_t)lda * naux);
-  HC(hipMemsetAsync(Ap, 0, (size_t)lda * naux * sizeof(double), st));
-  HC(hipStreamWaitEvent(st, lane.ev_mprep, 0));
-  metric_launch(c, daux.view, mprep, Ap, lda);
+  HC(hipMemsetAsync(Ap, 0, (size_t)lda * naux * sizeof(double), sm));
+  HC(hipStreamWaitEvent(sm, lane.ev_mprep, 0));
+  const int e2m = tm.mark(sm);  // the metric's kernels start here
+  metric_launch(c, daux.view, mprep, Ap, lda, sm);
   DBuf<double> dqc(x.n);
-  schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, st);
-  const int e3 = tm.mark();
+  schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, sm);
+  const int e3 = tm.mark(sm);
   double* hqc = lane.hqc.get<double>(x.n);
-  HC(hipMemcpyAsync(hqc, dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
-  HC(hipEventRecord(lane.ev_met, st));
+  HC(hipMemcpyAsync(hqc, dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, sm));
+  HC(hipEventRecord(lane.ev_met, sm));
   // Cholesky of the metric bordered by the identity (common/linalg.f90:78 and
-  // the dtrtri of diabatization/intor.f90:198) on the lane's side stream,
-  // concurrent with the integrals: it touches only rows < 2 naux, the
-  // contraction writes d^T below them.  Ap and info outlive the join.
+  // the dtrtri of diabatization/intor.f90:198) behind it on the side stream,
+  // concurrent with the Schwarz kernels and the integrals: it touches only
+  // rows < 2 naux, the contraction writes d^T below them (after waiting for
+  // ev_met: the memset above clears those rows too).  Ap and info outlive the
+  // join.
   if (need_dens) {
-    set_identity_rows(Ap, lda, naux, st);
-    HC(hipMemsetAsync(info.p, 0, sizeof(int), st));
-    HC(hipEventRecord(lane.ev_fork, st));
-    HC(hipStreamWaitEvent(lane.st2, lane.ev_fork, 0));
-    cholesky_inverse_rows(Ap, lda, naux, info.p, lane.st2);
+    set_identity_rows(Ap, lda, naux, sm);
+    HC(hipMemsetAsync(info.p, 0, sizeof(int), sm));
+    cholesky_inverse_rows(Ap, lda, naux, info.p, sm);
     HC(hipGetLastError());
-    HC(hipEventRecord(lane.ev_join, lane.st2));
+    HC(hipEventRecord(lane.ev_join, sm));
   }
 
   vstep("metric queued");
@@ -1765,7 +1770,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   vstep("Et tables queued");
   if (opt.verbose) {
     vlog(opt, "Schwarz norms of %d shell pairs done (%.1f ms).\n", pl.ps.n(), tm.ms(e1, e2));
-    vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2, e3));
+    vlog(opt, "Metric (P|Q) of %d aux functions done (%.1f ms).\n", x.nbf, tm.ms(e2m, e3));
   }
   const auto hp1 = std::chrono::steady_clock::now();
   build_aux_lists(pl);
@@ -2020,8 +2025,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     }
     const int b1 = tm.mark();
     if (need_dens) {
-      // the packed densities come from the side stream
-      if (blk == 0) HC(hipStreamWaitEvent(st, lane.ev_pack, 0));
+      // the packed densities come from the side stream, and d^T's rows were
+      // cleared there before the metric
+      if (blk == 0) {
+        HC(hipStreamWaitEvent(st, lane.ev_pack, 0));
+        HC(hipStreamWaitEvent(st, lane.ev_met, 0));
+      }
       // d^T into the border rows: A[(naux + I) + lda * P]
       tile_segments(cA, w, hseg);
       upload_pinned(dseg, hseg, st);
@@ -2112,9 +2121,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     stats->ncolumn_blocks = nblocks;
     stats->ms_upload = tm.ms(e0, e1);
     stats->ms_schwarz = tm.ms(e1, e2);
-    stats->ms_metric = tm.ms(e2, e3);
+    stats->ms_metric = tm.ms(e2m, e3);
     stats->ms_cholesky = tm.ms(e6, e7);
-    stats->ms_pack = tm.ms(e3, e5);
+    stats->ms_pack = tm.ms(e2, e5);
     stats->ms_int3c2e = ms_int;
     stats->ms_contract = ms_con;
     stats->ms_solve = 0.0;
@@ -2140,7 +2149,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (opt.verbose) {
     std::fflush(stdout);
     std::printf("Schwarz norms took %8.4f s.\n", tm.ms(e1, e2) * 1e-3);
-    std::printf("Calculation of density fitting metric (P|Q) took %8.4f s.\n", tm.ms(e2, e3) * 1e-3);
+    std::printf("Calculation of density fitting metric (P|Q) took %8.4f s.\n", tm.ms(e2m, e3) * 1e-3);
     std::printf("Cholesky factorization of (P|Q) bordered by d took %8.4f s.\n", tm.ms(e6, e7) * 1e-3);
     std::printf("Density contraction took %8.4f s.\n", (ms_int + ms_con) * 1e-3);
     const int64_t skipped = pl.ntriplets - pl.ncomputed;

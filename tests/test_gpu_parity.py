@@ -416,7 +416,10 @@ def test_column_blocks_and_device_densities():
     """The staging matrix in several column blocks (a small max_tile_bytes)
     and densities already in HBM (dens_on_device, both layouts) give the
     one-block host-density result and the oracle's J."""
-    import torch
+    hip = ctypes.CDLL("libamdhip64.so")  # device buffers straight from the HIP runtime
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
 
     shells, aux, dens = synthetic.workload(0)
     b = N.BasisArrays(*shells.as_arrays())
@@ -436,14 +439,18 @@ def test_column_blocks_and_device_densities():
     # the memory of (ndens, nbf, nbf) in C order and in column-major order
     for layout, host in ((N.DENS_C, np.ascontiguousarray(dens).ravel()),
                          (N.DENS_FORTRAN, np.asfortranarray(dens).ravel(order="F"))):
-        d = torch.from_numpy(host).to("cuda:0")
-        torch.cuda.synchronize()
-        J2 = np.zeros((3,) * 4, order="F")
-        st = N.Stats()
-        opt = N.default_options(dens_layout=layout, dens_on_device=1)
-        N.check(N.lib().dfhip_contract_coulomb_densities(
-            ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, d.data_ptr(), ctypes.byref(opt), None, 3,
-            J2.ctypes.data, ctypes.byref(st)))
+        d = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(d), host.nbytes) == 0
+        try:
+            assert hip.hipMemcpy(d, host.ctypes.data, host.nbytes, 1) == 0  # host to device
+            J2 = np.zeros((3,) * 4, order="F")
+            st = N.Stats()
+            opt = N.default_options(dens_layout=layout, dens_on_device=1)
+            N.check(N.lib().dfhip_contract_coulomb_densities(
+                ctypes.byref(b.struct), ctypes.byref(x.struct), ndens, nbf, d, ctypes.byref(opt), None, 3,
+                J2.ctypes.data, ctypes.byref(st)))
+        finally:
+            hip.hipFree(d)
         err = np.abs(J2 - J0).max()
         _say(f"device densities, layout {layout}: max |dJ| vs host densities {err:.2e}")
         assert err <= 1e-12 * max(np.abs(J0).max(), 1.0)
