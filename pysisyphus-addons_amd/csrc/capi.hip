@@ -1479,15 +1479,39 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.ps = make_pairs(o, pcl);
   vstep("pairs");
 
-  // the metric's inputs go up on the copy stream while the Schwarz kernels run
+  // the metric's inputs go up on the side stream (the metric follows them
+  // there) while the Schwarz kernels run; not on the copy stream, whose
+  // shipments wait for the Schwarz classes
   MetricPrep mprep;
-  HC(hipStreamWaitEvent(copy_stream(lane), tm.ev[e1], 0));
-  metric_prepare(c, x, daux.view, mprep, copy_stream(lane));
-  HC(hipEventRecord(lane.ev_mprep, copy_stream(lane)));
+  const hipStream_t sm = need_dens ? lane.st2 : st;
+  HC(hipStreamWaitEvent(sm, tm.ev[e1], 0));
+  metric_prepare(c, x, daux.view, mprep, sm);
+  HC(hipEventRecord(lane.ev_mprep, sm));
   vstep("metric inputs queued");
 
+
+  // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal
+  // blocks: first on the side stream, beside the Schwarz kernels (without
+  // densities on the lane's stream).
+  // A = [[M], [I], [D^T]]: naux columns, rows 0..naux-1 the metric, then the
+  // identity border whose factorization gives L^-T, then d^T (ndens rows)
+  const int64_t lda = need_dens ? 2 * naux + ndens : naux;
+  double* Ap = reserve(cur_lane().wsA, (size_t)lda * naux);
+  HC(hipMemsetAsync(Ap, 0, (size_t)lda * naux * sizeof(double), sm));
+  HC(hipStreamWaitEvent(sm, lane.ev_mprep, 0));
+  const int e2m = tm.mark(sm);  // the metric's kernels start here
+  metric_launch(c, daux.view, mprep, Ap, lda, sm);
+  DBuf<double> dqc(x.n);
+  schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, sm);
+  const int e3 = tm.mark(sm);
+  double* hqc = lane.hqc.get<double>(x.n);
+  HC(hipMemcpyAsync(hqc, dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, sm));
+  HC(hipEventRecord(lane.ev_met, sm));
+  vstep("metric queued");
+
   // densities: upload and pair-pack (diabatization/intor.f90:155-176
-  // symmetry) on the side stream, concurrent with the Schwarz kernels.  The
+  // symmetry) on the side stream behind the metric, concurrent with the
+  // Schwarz kernels.  The
   // packed rows follow the make_pairs order; the contraction reads them
   // through rowperm once the pairs are sorted.
   double* Dp = nullptr;
@@ -1565,30 +1589,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   vstep("packing queued");
 
-  // metric (common/integrals/intor.f90:40-95) and Q_C from its diagonal
-  // blocks: on the side stream behind the packing (beside the Schwarz
-  // kernels; without densities on the lane's stream).
-  // A = [[M], [I], [D^T]]: naux columns, rows 0..naux-1 the metric, then the
-  // identity border whose factorization gives L^-T, then d^T (ndens rows)
-  const hipStream_t sm = need_dens ? lane.st2 : st;
-  const int64_t lda = need_dens ? 2 * naux + ndens : naux;
-  double* Ap = reserve(cur_lane().wsA, (size_t)lda * naux);
-  HC(hipMemsetAsync(Ap, 0, (size_t)lda * naux * sizeof(double), sm));
-  HC(hipStreamWaitEvent(sm, lane.ev_mprep, 0));
-  const int e2m = tm.mark(sm);  // the metric's kernels start here
-  metric_launch(c, daux.view, mprep, Ap, lda, sm);
-  DBuf<double> dqc(x.n);
-  schwarz_aux_from_metric(Ap, lda, daux.L.p, daux.f0.p, x.n, dqc.p, sm);
-  const int e3 = tm.mark(sm);
-  double* hqc = lane.hqc.get<double>(x.n);
-  HC(hipMemcpyAsync(hqc, dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, sm));
-  HC(hipEventRecord(lane.ev_met, sm));
   // Cholesky of the metric bordered by the identity (common/linalg.f90:78 and
-  // the dtrtri of diabatization/intor.f90:198) behind it on the side stream,
-  // concurrent with the Schwarz kernels and the integrals: it touches only
-  // rows < 2 naux, the contraction writes d^T below them (after waiting for
-  // ev_met: the memset above clears those rows too).  Ap and info outlive the
-  // join.
+  // the dtrtri of diabatization/intor.f90:198) behind the metric and the
+  // packing on the side stream, concurrent with the Schwarz kernels and the
+  // integrals: it touches only rows < 2 naux, the contraction writes d^T
+  // below them (after waiting for ev_met: the memset before the metric
+  // clears those rows too).  Ap and info outlive the join.
   if (need_dens) {
     set_identity_rows(Ap, lda, naux, sm);
     HC(hipMemsetAsync(info.p, 0, sizeof(int), sm));
@@ -1597,7 +1603,6 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipEventRecord(lane.ev_join, sm));
   }
 
-  vstep("metric queued");
   // pair part of the plan class by class as the Schwarz sums arrive (the
   // sorts overlap the Schwarz kernels of the later classes and the metric);
   // the sorted pair arrays and the bra-record layouts go up in one copy
