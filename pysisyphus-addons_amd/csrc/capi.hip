@@ -1596,6 +1596,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // below them (after waiting for ev_met: the memset before the metric
   // clears those rows too).  Ap and info outlive the join.
   if (need_dens) {
+    // not before the Schwarz kernels are done: its panel workgroups hold LDS
+    // that the Schwarz workgroups need, the 3c2e kernels mostly do not
+    HC(hipStreamWaitEvent(sm, tm.ev[e2], 0));
     set_identity_rows(Ap, lda, naux, sm);
     HC(hipMemsetAsync(info.p, 0, sizeof(int), sm));
     cholesky_inverse_rows(Ap, lda, naux, info.p, sm);
