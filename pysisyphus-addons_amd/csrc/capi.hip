@@ -1686,6 +1686,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     fill_rowpair(prow.p, npairs, pl.npf, drowpair.p, ddprow.p, drowperm.p, s3);
   }
   HC(hipGetLastError());
+  vstep("bra records queued");
 
   // Et tables of the classes run on the matrix-core 3c2e kernels, built once
   // for all column blocks after the aux plan, for the pairs with screened-in
@@ -1752,9 +1753,21 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
 
   HC(hipEventRecord(lane.ev_plan, s3));
   HC(hipStreamWaitEvent(st, lane.ev_plan, 0));
-  // Et tables of all pairs of a class where they fit, queued behind the
-  // metric while the host waits for it anyway (the rest after the aux plan)
-  if (opt.screening != DFHIP_SCREEN_QVL) build_et([](int, int, int) { return true; }, st, true);
+  vstep("early Et queued");
+  // Et tables of all pairs of the classes that run on the matrix cores for
+  // an aux L the generated kernels cover too ((dd|d), (dd|f): most of their
+  // pairs are needed), queued while the host waits for Q_C anyway; the rest
+  // after the aux plan, for the pairs with screened-in triplets only (for the
+  // f and g classes, or the g and h aux shells, tables of all pairs cost
+  // several times the needed ones)
+  if (opt.screening != DFHIP_SCREEN_QVL) {
+    std::vector<char> early(NPCLS, 0);
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++)
+        for (int L = 0; L <= LMAX_AUX; L++)
+          if (haveL[L] && gen && gen3_has(la, lb, L) && prefer_mm(la, lb, L)) early[pcls(la, lb)] = 1;
+    build_et([&](int k, int, int) { return early[k] != 0; }, st, true);
+  }
   vstep("pair plan, bra records");
 
   // aux part of the plan once Q_C is back (the packing may still run)

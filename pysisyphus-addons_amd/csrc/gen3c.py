@@ -35,6 +35,11 @@ LA_MAX = 2
 LC_MAX = 3
 ACC_LIMIT = 20     # Cartesian accumulators per lane (ncart_a ncart_b x m per group), FP64
 ACC_LIMIT32 = 40   # the same for the FP32 variants (half the registers per value)
+# FP64 classes compiled for 3 waves per SIMD (at most 168 VGPRs) instead of
+# the 2 the compiler picks by itself; small spills at most.  Measured on the
+# 60-atom s-d / aux s-f workload: 3c2e phase 4.36 -> 4.17 ms.  (dd|p) and the
+# larger classes spill too much at 3 waves and stay as they are.
+MIN_WAVES = {(1, 1, 3): 3, (2, 0, 2): 3, (2, 0, 3): 3, (2, 1, 2): 3, (2, 1, 3): 3, (2, 2, 0): 3}
 
 
 def cart(l):
@@ -266,7 +271,9 @@ def gen_class(la, lb, lc, real="double"):
     w = out.append
     name = f"int3c2e_gen{'32' if f32 else ''}_{la}{lb}{lc}"
     w(f"// class ({la}{lb}|{lc}), {'FP32' if f32 else 'FP64'}: L = {L}, record {rs} values, {len(groups)} m-group(s)")
-    w(f"__global__ __launch_bounds__(64) void {name}(const double* __restrict__ recs, const float* __restrict__ recs32,")
+    mw = None if f32 else MIN_WAVES.get((la, lb, lc))
+    bounds = f"64, {mw}" if mw else "64"
+    w(f"__global__ __launch_bounds__({bounds}) void {name}(const double* __restrict__ recs, const float* __restrict__ recs32,")
     w(f"    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,")
     w(f"    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {{")
     w(f"  const int pidx = blockIdx.x;")

@@ -3226,7 +3226,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_112(const double* __restrict
 }
 
 // class (11|3), FP64: L = 5, record 37 values, 4 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_113(const double* __restrict__ recs, const float* __restrict__ recs32,
+__global__ __launch_bounds__(64, 3) void int3c2e_gen_113(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;

@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_201(const double* __restrict
 }
 
 // class (20|2), FP64: L = 4, record 25 values, 2 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_202(const double* __restrict__ recs, const float* __restrict__ recs32,
+__global__ __launch_bounds__(64, 3) void int3c2e_gen_202(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
@@ -2166,7 +2166,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_202(const double* __restrict
 }
 
 // class (20|3), FP64: L = 5, record 25 values, 3 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_203(const double* __restrict__ recs, const float* __restrict__ recs32,
+__global__ __launch_bounds__(64, 3) void int3c2e_gen_203(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;

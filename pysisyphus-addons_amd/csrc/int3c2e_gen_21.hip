@@ -3867,7 +3867,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_211(const double* __restrict
 }
 
 // class (21|2), FP64: L = 5, record 112 values, 5 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_212(const double* __restrict__ recs, const float* __restrict__ recs32,
+__global__ __launch_bounds__(64, 3) void int3c2e_gen_212(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
@@ -8989,7 +8989,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_212(const double* __restrict
 }
 
 // class (21|3), FP64: L = 6, record 112 values, 7 m-group(s)
-__global__ __launch_bounds__(64) void int3c2e_gen_213(const double* __restrict__ recs, const float* __restrict__ recs32,
+__global__ __launch_bounds__(64, 3) void int3c2e_gen_213(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
   const int pidx = blockIdx.x;
