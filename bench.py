@@ -71,8 +71,9 @@ def cpu_baseline(shells, aux, dens, step):
         "unit": "screened shell-triplets/s",
         "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
         "kind": "port",
-        "sample": f"every {step}th aux shell ({ncomp} screened triplets, {dt:.1f} s) of the same "
-                  "workload, 3c2e + density contraction loop",
+        "sample": (("all aux shells" if step == 1 else f"every {step}th aux shell")
+                   + f" ({ncomp} screened triplets, {dt:.1f} s) of the same "
+                   "workload, 3c2e + density contraction loop"),
     }
 
 
