@@ -177,6 +177,15 @@ int dfhip_int3c2e_dense(const dfhip_basis *bas, const dfhip_basis *aux, const df
  * calls); the next call re-allocates. */
 int dfhip_release_workspace(void);
 
+/* The QVL screener's records as the device path computes them
+ * (common/screening.f90:101-163 set-up): pair_records[8 k + 0..7] for the
+ * packed pair k = a(a+1)/2 + b, a >= b (0-based) = contracted centre x, y, z,
+ * contracted extent (thresh_ws 1e-4), (min a + min b)^(1/4), Q_ab, 0, 0;
+ * aux_records[4 c + 0..3] = auxiliary extent (prec 1e-8),
+ * beta = min exp^(-(2L+3)/4) sqrt((2L-1)!!), Q_C, 0. */
+int dfhip_qvl_records(const dfhip_basis *bas, const dfhip_basis *aux, double *pair_records,
+                      double *aux_records);
+
 /* Metric (P|Q), naux x naux (symmetric). */
 int dfhip_metric(const dfhip_basis *aux, double *out);
 

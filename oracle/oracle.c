@@ -824,7 +824,8 @@ void orc_contracted_center(int na, const double *axs, const double *das, const d
     for (i = 0; i < na; i++)
         for (j = 0; j < nb; j++) {
             double px = axs[i] + bxs[j], w = fabs(das[i] * dbs[j]);
-            for (k = 0; k < 3; k++) R[k] += w * (axs[i] * A[k] + bxs[j] * B[k]) / px;
+            /* P first, then the weight: R = R + abs_dab*P (:256-259) */
+            for (k = 0; k < 3; k++) R[k] += w * ((axs[i] * A[k] + bxs[j] * B[k]) / px);
             den += w;
         }
     for (k = 0; k < 3; k++) R[k] /= den;
@@ -894,7 +895,7 @@ void orc_qvl_records(void *bas, void *aux, const double *schwarz_bra, const doub
                                   sb->A, r);
             r[3] = orc_contracted_extent(sa->nprim, sa->exps, sa->coefs, sa->A, sb->nprim, sb->exps,
                                          sb->coefs, sb->A, thresh_ws);
-            r[4] = sqrt(sqrt(amin + bmin)); /* (..)**0.25d0 */
+            r[4] = pow(amin + bmin, 0.25); /* (..)**0.25d0, :143 */
             r[5] = schwarz_bra[k];
             r[6] = r[7] = 0.0;
         }

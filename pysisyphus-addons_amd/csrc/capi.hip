@@ -1060,6 +1060,64 @@ void metric_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* M, in
   metric_launch(c, dev, mp, M, ldm);
 }
 
+// Q_C = sqrt(||(C|C)||_F) of every aux shell from the diagonal blocks alone
+// (common/integrals/intor.f90:97-129 evaluates int_2c2e(L, L, c, c) only):
+// per L, the 2c2e kernels with bra shell i limited to its own list entry
+// (window [i, i + 1)), writing into a compact buffer whose row is the shell's
+// function and whose column is m.  O(naux) work, against O(naux^2) for the
+// full metric.
+void schwarz_aux_dev(Ctx& c, const HostShells& x, const DevShells& dev, double* qc, hipStream_t st) {
+  constexpr int LDC = 2 * LMAX_AUX + 1;
+  std::vector<std::vector<int>> byL(LMAX_AUX + 1);
+  for (int i = 0; i < x.n; i++) byL[x.L[i]].push_back(i);
+  std::vector<int> s1, s2, nbeg, nend, lbeg(LMAX_AUX + 2, 0), kmax(LMAX_AUX + 1, 1);
+  std::vector<int64_t> row, col;
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    lbeg[L] = (int)s1.size();
+    for (size_t k = 0; k < byL[L].size(); k++) {
+      const int i = byL[L][k];
+      s1.push_back(i);
+      s2.push_back(-1);
+      row.push_back(x.f0[i]);
+      col.push_back(0);
+      nbeg.push_back((int)k);
+      nend.push_back((int)k + 1);
+      kmax[L] = std::max(kmax[L], x.nprim[i]);
+    }
+  }
+  lbeg[LMAX_AUX + 1] = (int)s1.size();
+  DBuf<int> ds1, ds2, dnb, dne;
+  DBuf<int64_t> drow, dcol;
+  upload_pinned(ds1, s1, st);
+  upload_pinned(ds2, s2, st);
+  upload_pinned(dnb, nbeg, st);
+  upload_pinned(dne, nend, st);
+  upload_pinned(drow, row, st);
+  upload_pinned(dcol, col, st);
+  DBuf<double> blk((size_t)std::max(x.nbf, 1) * LDC);
+  const bool gen = use_generated();
+  std::vector<BraRecords> brs(LMAX_AUX + 1);
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    const int b0 = lbeg[L], np = lbeg[L + 1] - b0;
+    if (np == 0) continue;
+    PairArgs pa{ds1.p + b0, ds2.p + b0, drow.p + b0, dne.p + b0, np};
+    pa.nbeg = dnb.p + b0;
+    AuxArgs xa{ds1.p + b0, dcol.p + b0, 0, np, kmax[L]};
+    if (gen && gen3_has(L, 0, L)) {
+      build_bra_records(c, L, 0, x, dev, ds1.p + b0, ds2.p + b0, s1.data() + b0, s2.data() + b0, np, brs[L], false,
+                        st);
+      launch_gen3(c, L, 0, L, brs[L], dev, pa, xa, blk.p, LDC, false, st);
+    } else {
+      c.int3[L][0][L].fn(dev, dev, pa, xa, c.sph.p, c.boys.p, blk.p, LDC, st);
+      HC(hipGetLastError());
+    }
+  }
+  schwarz_aux_from_metric(blk.p, LDC, dev.L, dev.f0, x.n, qc, st, true);
+  HC(hipGetLastError());
+  // the pool blocks above go back before the caller's copy of qc has run
+  HC(hipStreamSynchronize(st));
+}
+
 dfhip_options defaults() {
   dfhip_options o;
   std::memset(&o, 0, sizeof(o));
@@ -1312,16 +1370,43 @@ void vlog(const dfhip_options& opt, const char* fmt, ...) {
   std::fflush(stdout);
 }
 
-// erfinv(y) for 0 < y < 1 by Newton's method on erf (quadratic convergence
-// from sqrt(-log(1 - y)), which lies above the root).
+// erfinv(y), 0 <= y < 1, by the rational Chebyshev approximations of Blair,
+// Edwards and Johnson (Math. Comp. 30 (1976) 827-830), the method of
+// common/erfinv.f90:88-127: x R1(x^2 - 0.5625) for y <= 0.75,
+// x R2(x^2 - 0.87890625) for y <= 0.9375, and R3(t)/t with
+// t = 1/sqrt(-log(1 - y)) above.  The coefficients are the paper's published
+// tables; each R = P(z)/Q(z) is evaluated by Horner's rule.
 double erfinv_pos(double y) {
-  double x = std::sqrt(-std::log(1.0 - y));
-  for (int it = 0; it < 100; it++) {
-    const double dx = (std::erf(x) - y) / (2.0 / std::sqrt(M_PI) * std::exp(-x * x));
-    x -= dx;
-    if (std::fabs(dx) <= 1e-16 * x) break;
-  }
-  return x;
+  static const double p1[7] = {0.160304955844066229311e2, -0.90784959262960326650e2, 0.18644914861620987391e3,
+                               -0.16900142734642382420e3, 0.6545466284794487048e2,  -0.864213011587247794e1,
+                               0.176058782139590e0};
+  static const double q1[7] = {0.147806470715138316110e2, -0.91374167024260313936e2, 0.21015790486205317714e3,
+                               -0.22210254121855132366e3, 0.10760453916055123830e3,  -0.206010730328265443e2,
+                               1.0};
+  static const double p2[8] = {-0.152389263440726128e-1, 0.3444556924136125216e0,  -0.29344398672542478687e1,
+                               0.11763505705217627302e2, -0.22655292823101104193e2, 0.19121334396580330163e2,
+                               -0.5478927619598318769e1, 0.237516689024448000e0};
+  static const double q2[8] = {-0.108465169602059954e-1, 0.2610628885843078511e0,  -0.24068318104393757995e1,
+                               0.10695129973387014469e2, -0.23716715521596581025e2, 0.24640158943917284883e2,
+                               -0.10014376349783070835e2, 1.0};
+  static const double p3[9] = {0.3100808562552958465e-4,  0.40974876030119399241e-2, 0.1214902662897276161664e0,
+                               0.11091676946390282026357e1, 0.32283798556639239484387e1, 0.2881691815651598826211e1,
+                               0.2047972087262996049729e1, 0.854592208197214832806e0,  0.3551095884622383139e-2};
+  static const double q3[9] = {0.3100809298564522487e-4,  0.40975286786639145041e-2, 0.1215907800748757143068e0,
+                               0.11186271676316964210428e1, 0.34323639843052901805054e1, 0.4140284677116202150554e1,
+                               0.4119797271272204121285e1, 0.2162961962641434560920e1, 1.0};
+  auto ratio = [](const double* p, const double* q, int n, double z) {
+    double a = p[n - 1], b = q[n - 1];
+    for (int i = n - 2; i >= 0; i--) {
+      a = a * z + p[i];
+      b = b * z + q[i];
+    }
+    return a / b;
+  };
+  if (y <= 0.75) return y * ratio(p1, q1, 7, y * y - 0.5625);
+  if (y <= 0.9375) return y * ratio(p2, q2, 8, y * y - 0.87890625);
+  const double t = 1.0 / std::sqrt(-std::log(1.0 - y));
+  return ratio(p3, q3, 9, t) / t;
 }
 
 // QVL screening (common/screening.f90:101-163 set-up, :267-299 estimate):
@@ -1332,11 +1417,9 @@ struct QvlRecords {
   DBuf<double> pair, aux;
 };
 
-void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int* ps1, const int* ps2,
-               const std::vector<DView<int>>& dalist, double thresh, QvlRecords& qr, hipStream_t st) {
-  const HostShells& o = pl.orb;
-  const HostShells& x = pl.aux;
-  const int np = pl.ps.n();
+// Aux records [extent, beta, Q_C, 0] per aux shell (common/screening.f90:107-115):
+// auxiliary_extent(exps, 1d-8) (:168-196), beta = min exp^(-(2L+3)/4) sqrt((2L-1)!!).
+std::vector<double> aux_qvl_records(const HostShells& x, const std::vector<double>& qc) {
   std::vector<double> xr((size_t)4 * x.n, 0.0);
   for (int sh = 0; sh < x.n; sh++) {
     const int L = x.L[sh];
@@ -1352,13 +1435,24 @@ void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int
     for (int k = 2 * L - 1; k > 1; k -= 2) df *= k;
     xr[4 * sh + 0] = ext;
     xr[4 * sh + 1] = std::pow(emin, -(2.0 * L + 3.0) / 4.0) * std::sqrt(df);
-    xr[4 * sh + 2] = pl.qc[sh];
+    xr[4 * sh + 2] = qc[sh];
   }
-  upload(qr.aux, xr, st);
+  return xr;
+}
+
+// erfinv(1 - thresh_ws) of the pair extents, thresh_ws = 1d-4
+// (t_df_screener(shells, shells_aux, 1d-4), diabatization/intor.f90:84)
+double qvl_erfi() { return erfinv_pos(1.0 - 1e-4); }
+
+void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int* ps1, const int* ps2,
+               const std::vector<DView<int>>& dalist, double thresh, QvlRecords& qr, hipStream_t st) {
+  const HostShells& o = pl.orb;
+  const int np = pl.ps.n();
+  upload(qr.aux, aux_qvl_records(pl.aux, pl.qc), st);
   DBuf<double> dq;
   upload(dq, pl.qab, st);
   qr.pair.alloc((size_t)8 * std::max(np, 1));
-  qvl_pair_records(dorb, ps1, ps2, dq.p, np, erfinv_pos(1.0 - 1e-4), qr.pair.p, st);
+  qvl_pair_records(dorb, ps1, ps2, dq.p, np, qvl_erfi(), qr.pair.p, st);
   HC(hipGetLastError());
   DBuf<int> ns((size_t)3 * std::max(np, 1));
   std::vector<int> h((size_t)3 * np);
@@ -2218,6 +2312,59 @@ std::vector<double> schwarz_bra_host(const dfhip_basis& b) {
   return outp;
 }
 
+std::vector<double> schwarz_aux_host(const dfhip_basis& b) {
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  HostShells x = parse_shells(b, LMAX_AUX, "basis");
+  cur_lane().arena.reset();
+  const hipStream_t st = cur_lane().st;
+  DevShellBufs d;
+  d.upload_from(x, st);
+  DBuf<double> dq(x.n);
+  schwarz_aux_dev(ctx(), x, d.view, dq.p, st);
+  std::vector<double> q(x.n);
+  HC(hipMemcpyAsync(q.data(), dq.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  return q;
+}
+
+// The QVL screener records of the device path: per packed pair
+// k = a(a+1)/2 + b (a >= b) [centre x, y, z, extent, (min a + min b)^(1/4),
+// Q_ab, 0, 0] from qvl_pair_kernel, per aux shell [extent, beta, Q_C, 0].
+void qvl_records_host(const dfhip_basis& b, const dfhip_basis& xb, double* pair_out, double* aux_out) {
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  Ctx& c = ctx();
+  HostShells o = parse_shells(b, LMAX_ORB, "orbital basis");
+  HostShells x = parse_shells(xb, LMAX_AUX, "auxiliary basis");
+  cur_lane().arena.reset();
+  const hipStream_t st = cur_lane().st;
+  DevShellBufs dorb, daux;
+  dorb.upload_from(o, st);
+  daux.upload_from(x, st);
+  const PairClasses pc = pair_classes(o);
+  PairSet ps = make_pairs(o, pc);
+  const std::vector<double> qab = schwarz_bra_dev(c, o, dorb.view, pc, ps);
+  DBuf<double> dqc(x.n);
+  schwarz_aux_dev(c, x, daux.view, dqc.p, st);
+  std::vector<double> qc(x.n);
+  HC(hipMemcpyAsync(qc.data(), dqc.p, x.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  const int np = ps.n();
+  DBuf<int> ds1, ds2;
+  DBuf<double> dq, rec((size_t)8 * std::max(np, 1));
+  upload(ds1, ps.s1, st);
+  upload(ds2, ps.s2, st);
+  upload(dq, qab, st);
+  qvl_pair_records(dorb.view, ds1.p, ds2.p, dq.p, np, qvl_erfi(), rec.p, st);
+  HC(hipGetLastError());
+  std::vector<double> h((size_t)8 * np);
+  HC(hipMemcpyAsync(h.data(), rec.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  for (int p = 0; p < np; p++) std::memcpy(pair_out + (size_t)8 * ps.packed[p], &h[(size_t)8 * p], 8 * sizeof(double));
+  const std::vector<double> xr = aux_qvl_records(x, qc);
+  std::memcpy(aux_out, xr.data(), xr.size() * sizeof(double));
+}
+
 std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
   std::lock_guard<std::mutex> lock(ctx_mutex());
   ensure_ctx();
@@ -2368,6 +2515,13 @@ int dfhip_int3c2e_dense(const dfhip_basis* bas, const dfhip_basis* aux, const df
   });
 }
 
+int dfhip_qvl_records(const dfhip_basis* bas, const dfhip_basis* aux, double* pair_records, double* aux_records) {
+  return guarded([&] {
+    if (!bas || !aux || !pair_records || !aux_records) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    qvl_records_host(*bas, *aux, pair_records, aux_records);
+  });
+}
+
 int dfhip_metric(const dfhip_basis* aux, double* out) {
   return guarded([&] {
     if (!aux || !out) throw Error{DFHIP_ERR_ARG, "NULL argument"};
@@ -2437,8 +2591,8 @@ void f_contract_coulomb_densities_4d(int32_t nshells, int32_t ndata, const int32
 void f_int_schwarz_aux(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
                        const double* bas_data, double* integrals) {
   guarded([&] {
-    std::vector<double> qc;
-    metric_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data), &qc);
+    if (!integrals) throw Error{DFHIP_ERR_ARG, "integrals is NULL"};
+    std::vector<double> qc = schwarz_aux_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data));
     std::memcpy(integrals, qc.data(), qc.size() * sizeof(double));
   });
 }

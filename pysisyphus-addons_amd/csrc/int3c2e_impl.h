@@ -21,7 +21,7 @@ struct PairArgs {
   const int* nsurv;    // screened-in prefix length of the aux list, per pair
   int npairs;
   const double* qvl = nullptr;  // QVL pair records (8 doubles per pair), or null for Schwarz
-  const int* nbeg = nullptr;    // first aux-list entry per pair (generated kernels; null = 0)
+  const int* nbeg = nullptr;    // first aux-list entry per pair (generated and LDS kernels; null = 0)
 };
 
 struct AuxArgs {
@@ -89,9 +89,11 @@ __global__ __launch_bounds__(128) void int3c2e_kernel(DevShells bra, DevShells k
   const int pidx = blockIdx.x;
   const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c0 = xa.lo + (int)blockIdx.y * K::NC;
-  if (c0 >= cend) return;
-  const int nc = min(K::NC, cend - c0);
+  // this workgroup's chunk of the list, cut to the pair's window [nbeg, nsurv)
+  const int cchunk = xa.lo + (int)blockIdx.y * K::NC;
+  const int c0 = pa.nbeg ? max(cchunk, pa.nbeg[pidx]) : cchunk;
+  if (c0 >= min(cend, cchunk + K::NC)) return;
+  const int nc = min(cchunk + K::NC, cend) - c0;
   const int tid = threadIdx.x;
 
   double* Ex = lds;

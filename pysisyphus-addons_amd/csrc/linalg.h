@@ -58,7 +58,7 @@ void gram_neg(const double* BT, int64_t n, int ndens, double* part, double* G, h
 void bt_to_b(const double* BT, int64_t n, int ndens, double* B, hipStream_t st);
 void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates, double* J, hipStream_t st);
 void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,
-                             hipStream_t st);
+                             hipStream_t st, bool compact = false);
 void boys_probe(const int* n, const double* T, int64_t npts, const double* tab, double* F, hipStream_t st);
 
 // QVL screening (screen.hip): pair records (8 doubles per pair) and, per aux

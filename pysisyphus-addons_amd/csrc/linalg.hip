@@ -828,24 +828,28 @@ void assemble_j(const double* A, int64_t lda, int64_t n, int ndens, int nstates,
 // ---------------------------------------------------------------------------
 // Q_C = sqrt(||(C|C)||_F) from the diagonal blocks of the metric (row-major =
 // column-major, symmetric).
+// compact: the diagonal blocks sit at M[(f0 + i) n + j] (rows of the shell's
+// functions, columns m) instead of on the diagonal of the full metric
 __global__ void schwarz_aux_kernel(const double* __restrict__ M, int64_t n, const int* __restrict__ L,
-                                   const int* __restrict__ f0, int nsh, double* __restrict__ q) {
+                                   const int* __restrict__ f0, int nsh, int compact, double* __restrict__ q) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nsh) return;
   const int nc = 2 * L[c] + 1;
-  const int64_t o = f0[c];
+  const int64_t o = f0[c], oc = compact ? 0 : o;
   double s = 0.0;
   for (int i = 0; i < nc; i++)
     for (int j = 0; j < nc; j++) {
-      const double v = M[(o + i) * n + o + j];
+      const double v = M[(o + i) * n + oc + j];
       s += v * v;
     }
   q[c] = sqrt(sqrt(s));
 }
 
 void schwarz_aux_from_metric(const double* M, int64_t n, const int* L, const int* f0, int nsh, double* q,
-                             hipStream_t st) {
-  if (nsh > 0) hipLaunchKernelGGL(schwarz_aux_kernel, dim3((nsh + 127) / 128), dim3(128), 0, st, M, n, L, f0, nsh, q);
+                             hipStream_t st, bool compact) {
+  if (nsh > 0)
+    hipLaunchKernelGGL(schwarz_aux_kernel, dim3((nsh + 127) / 128), dim3(128), 0, st, M, n, L, f0, nsh,
+                       compact ? 1 : 0, q);
 }
 
 // ---------------------------------------------------------------------------

@@ -25,7 +25,8 @@ __global__ void qvl_pair_kernel(DevShells bra, const int* __restrict__ s1v, cons
       for (int j = 0; j < nb; j++) {
         const double bx = bra.exps[b0 + j], px = ax + bx;
         const double w = fabs(bra.coefs[a0 + i] * bra.coefs[b0 + j]);
-        for (int k = 0; k < 3; k++) Rc[k] += w * (ax * A[k] + bx * B[k]) / px;
+        // the primitive centre first, then its weight (R = R + abs_dab*P, :256-259)
+        for (int k = 0; k < 3; k++) Rc[k] += w * ((ax * A[k] + bx * B[k]) / px);
         den += w;
       }
     }
@@ -49,7 +50,7 @@ __global__ void qvl_pair_kernel(DevShells bra, const int* __restrict__ s1v, cons
     r[1] = Rc[1];
     r[2] = Rc[2];
     r[3] = ext;
-    r[4] = sqrt(sqrt(amin + bmin));
+    r[4] = pow(amin + bmin, 0.25);  // (..)**0.25d0, :143
     r[5] = qab[p];
     r[6] = 0.0;
     r[7] = 0.0;

@@ -107,6 +107,16 @@ def workload(wid, with_densities=True):
                     seed=SEED_BASE + wid, with_densities=with_densities)
 
 
+def workload4_reduced(natoms=10):
+    """The 200-atom recipe (orbital 4s3p2d1f1g, aux 5s4p3d2f1g1h, 20 states)
+    on natoms atoms at the same atom density (cube side 34 (natoms/200)^(1/3)
+    bohr): small enough for the oracle, with every class of the large run."""
+    w = WORKLOADS[4]
+    side = w["side"] * (natoms / w["natoms"]) ** (1.0 / 3.0)
+    return molecule(natoms, side, w["orb"], w["aux"], w["nstates"], w["alpha0"],
+                    seed=SEED_BASE + 4000 + natoms)
+
+
 def batch_molecule(i):
     """Molecule i (0..255) of the batched workload 3."""
     rng = np.random.Generator(np.random.PCG64(MOL_SEED_BASE + i))

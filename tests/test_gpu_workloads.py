@@ -1,0 +1,205 @@
+"""The benchmark workloads at their full sizes (and the 200-atom recipe at a
+size the oracle covers), through the C ABI, against the FP64 oracle and
+through size-independent properties:
+
+* the 60-atom f/g workload (orbital 3s2p1d1f, aux 4s3p2d1f1g, 8 states):
+  FP64 J against the oracle at 1e-10 with identical triplet counts, the
+  mixed path at 1e-8;
+* the 200-atom recipe (orbital 4s3p2d1f1g, aux 5s4p3d2f1g1h, 20 states,
+  QVL at 1e-10) on 10 atoms: FP64 at 1e-10 with identical kept/skipped
+  counts, mixed at 1e-8;
+* the 200-atom workload itself: FP64 against mixed at 1e-8, the 8-fold
+  symmetry of J, the class histogram summing to the screened count;
+* the QVL screener records of the device path against the oracle's.
+"""
+
+import ctypes
+import functools
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pysisyphus_addons_amd import _native as N
+from pysisyphus_addons_amd import synthetic
+from pysisyphus_addons_amd.diabatization import _intor
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _arrays(shells, aux):
+    return N.BasisArrays(*shells.as_arrays()), N.BasisArrays(*aux.as_arrays())
+
+
+@functools.lru_cache(maxsize=None)
+def _oracle(key):
+    if key == "w2":
+        shells, aux, dens = synthetic.workload(2)
+        screening = "schwarz"
+    else:
+        shells, aux, dens = synthetic.workload4_reduced(10)
+        screening = "qvl"
+    J, info = O.contract_coulomb_densities_4d(shells, aux, dens, return_info=True, screening=screening)
+    return J, info
+
+
+def _gpu(key, precision):
+    if key == "w2":
+        shells, aux, dens = synthetic.workload(2)
+        opt = N.default_options(precision=precision)
+    else:
+        shells, aux, dens = synthetic.workload4_reduced(10)
+        opt = N.default_options(precision=precision, screening=N.SCREEN_QVL, thresh=1e-10)
+    b, x = _arrays(shells, aux)
+    nstates = {"w2": 8, "w4r": 20}[key]
+    _, J, st = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True, options=opt)
+    return J, st
+
+
+def _check_counts(st):
+    assert st["ncomputed"] + st["nskipped"] == st["ntriplets"]
+    assert sum(st["class_triplets"]) == st["ncomputed"]
+
+
+@pytest.mark.parametrize("key", ["w2", "w4r"])
+def test_fp64_vs_oracle(key):
+    _say(f"{key}: oracle ...")
+    Jo, info = _oracle(key)
+    J, st = _gpu(key, N.FP64)
+    err = np.abs(J - Jo).max()
+    _say(f"{key} FP64: max |dJ| {err:.2e} (|J|max {np.abs(Jo).max():.3e}), computed {st['ncomputed']} "
+         f"(oracle {info['ncomputed']}), skipped {st['nskipped']} (oracle {info['nskipped']})")
+    assert st["ncomputed"] == info["ncomputed"]
+    assert st["nskipped"] == info["nskipped"]
+    _check_counts(st)
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("key", ["w2", "w4r"])
+def test_mixed_vs_oracle(key):
+    Jo, info = _oracle(key)
+    J, st = _gpu(key, N.MIXED)
+    err = np.abs(J - Jo)
+    _say(f"{key} mixed: max |dJ| {err.max():.2e}, max rel {(err.max() / np.abs(Jo).max()):.2e}, "
+         f"low-precision triplets {st['ncomputed_fp32']} / {st['ncomputed']}")
+    assert st["ncomputed"] == info["ncomputed"]
+    _check_counts(st)
+    assert err.max() <= 1e-8
+
+
+def test_mixed_blocks_fg_classes():
+    """(ab|P) blocks of the mixed path on the f/g-orbital, g/h-aux classes (the
+    200-atom recipe on 4 atoms), against the oracle: max abs and rel error."""
+    shells, aux, _ = synthetic.workload4_reduced(4)
+    b, x = _arrays(shells, aux)
+    got = np.zeros((shells.nbf, shells.nbf, aux.nbf))
+    opt = N.default_options(precision=N.MIXED, screening=N.SCREEN_QVL, thresh=1e-10)
+    st = N.Stats()
+    N.check(N.lib().dfhip_int3c2e_dense(ctypes.byref(b.struct), ctypes.byref(x.struct), ctypes.byref(opt),
+                                        got.ctypes.data, ctypes.byref(st)))
+    from test_gpu_parity import oracle_dense3c
+
+    ref = oracle_dense3c(shells, aux, qvl=True)
+    err = np.abs(got - ref)
+    nz = np.abs(ref) > 1e-12
+    _say(f"mixed blocks (200-atom recipe, 4 atoms): max abs {err.max():.2e}, max rel (|ref| > 1e-12) "
+         f"{(err[nz] / np.abs(ref[nz])).max():.2e}, low-precision triplets {st.ncomputed_fp32} / {st.ncomputed}")
+    assert err.max() <= 2e-9
+
+
+@pytest.mark.parametrize("key", ["w0", "w4r"])
+def test_qvl_records_vs_oracle(key):
+    """Contracted centres and extents (common/screening.f90:198-265),
+    (min a + min b)^(1/4), Q_ab, and the aux extents and betas of the device
+    path against the oracle's screener set-up."""
+    if key == "w0":
+        shells, aux, _ = synthetic.workload(0)
+    else:
+        shells, aux, _ = synthetic.workload4_reduced(10)
+    b, x = _arrays(shells, aux)
+    npk = len(shells.shells) * (len(shells.shells) + 1) // 2
+    pr = np.zeros((npk, 8))
+    xr = np.zeros((len(aux.shells), 4))
+    N.check(N.lib().dfhip_qvl_records(ctypes.byref(b.struct), ctypes.byref(x.struct), pr.ctypes.data,
+                                      xr.ctypes.data))
+    bas, xb = O.Basis.from_shells(shells), O.Basis.from_shells(aux)
+    po, xo = O.qvl_records(bas, xb, O.schwarz_bra(bas), O.schwarz_aux(xb))
+    dc = np.abs(pr[:, :3] - po[:, :3]).max()
+    de = np.abs(pr[:, 3] - po[:, 3]).max()
+    _say(f"{key} QVL records: centre max abs diff {dc:.2e}, extent {de:.2e}")
+    np.testing.assert_allclose(pr[:, :3], po[:, :3], rtol=0, atol=1e-13 * max(1.0, np.abs(po[:, :3]).max()))
+    np.testing.assert_allclose(pr[:, 3], po[:, 3], rtol=1e-14)
+    np.testing.assert_allclose(pr[:, 4], po[:, 4], rtol=1e-15)
+    np.testing.assert_allclose(pr[:, 5], po[:, 5], rtol=1e-12)
+    assert not pr[:, 6:].any()
+    np.testing.assert_array_equal(xr[:, :2], xo[:, :2])
+    np.testing.assert_allclose(xr[:, 2], xo[:, 2], rtol=1e-13)
+
+
+def _device_densities(nstates, nbf, seed):
+    """nstates(nstates+1)/2 symmetric (A + A^T)/2, A ~ U(0, 1), unit trace,
+    drawn on the device (the 200-atom workload holds 102 GB of them)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    ndens = nstates * (nstates + 1) // 2
+    D = torch.empty((ndens, nbf, nbf), dtype=torch.float64, device=dev)
+    for i in range(ndens):
+        a = torch.rand((nbf, nbf), dtype=torch.float64, device=dev, generator=gen)
+        a = 0.5 * (a + a.T)
+        D[i] = a / torch.trace(a)
+        del a
+    torch.cuda.synchronize()
+    return D
+
+
+def _eightfold(J):
+    perms = [(0, 1, 2, 3), (1, 0, 2, 3), (0, 1, 3, 2), (1, 0, 3, 2), (2, 3, 0, 1), (2, 3, 1, 0), (3, 2, 0, 1),
+             (3, 2, 1, 0)]
+    return all(np.array_equal(J, J.transpose(p)) for p in perms)
+
+
+def test_workload4_full_properties():
+    """The 200-atom workload (2,200 shells, 7,747,520,000 triplets, QVL 1e-10,
+    20 states) on device densities: mixed within 1e-8 of FP64, the 8-fold
+    symmetry of J, 64-bit counts, the class histogram summing to the
+    screened count, both precisions screening the same triplets."""
+    import torch
+
+    shells, aux, _ = synthetic.workload(4, with_densities=False)
+    b, x = _arrays(shells, aux)
+    nstates = 20
+    _say(f"workload 4: nbf {shells.nbf} naux {aux.nbf}, drawing densities on the device ...")
+    D = _device_densities(nstates, shells.nbf, synthetic.SEED_BASE + 4)
+    res = {}
+    try:
+        for name, prec in (("fp64", N.FP64), ("mixed", N.MIXED)):
+            J = np.zeros((nstates,) * 4, order="F")
+            st = N.Stats()
+            opt = N.default_options(precision=prec, screening=N.SCREEN_QVL, thresh=1e-10, dens_layout=N.DENS_C,
+                                    dens_on_device=1)
+            _say(f"workload 4 {name} ...")
+            N.check(N.lib().dfhip_contract_coulomb_densities(
+                ctypes.byref(b.struct), ctypes.byref(x.struct), D.shape[0], shells.nbf, D.data_ptr(),
+                ctypes.byref(opt), None, nstates, J.ctypes.data, ctypes.byref(st)))
+            res[name] = (J, st.as_dict())
+            _say(f"workload 4 {name}: {st.ms_total:.0f} ms, computed {st.ncomputed} / {st.ntriplets}")
+    finally:
+        del D
+        torch.cuda.empty_cache()
+    (J64, s64), (Jm, sm) = res["fp64"], res["mixed"]
+    assert s64["ntriplets"] == 7747520000
+    for s in (s64, sm):
+        _check_counts(s)
+    assert sm["ncomputed"] == s64["ncomputed"]
+    assert _eightfold(J64) and _eightfold(Jm)
+    err = np.abs(Jm - J64).max()
+    _say(f"workload 4: max |dJ| mixed vs FP64 {err:.2e} (|J|max {np.abs(J64).max():.3e}), low-precision "
+         f"triplets {sm['ncomputed_fp32']} / {sm['ncomputed']}")
+    assert err <= 1e-8

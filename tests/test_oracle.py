@@ -169,6 +169,65 @@ def test_contracted_center_kats():
     assert np.linalg.norm(R - 0.53650793650794) <= 1e-14
 
 
+def _extent(axs, das, A, bxs, dbs, B, thresh=1e-4):
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    return O.lib().orc_contracted_extent(len(axs), f(axs), f(das), f(A), len(bxs), f(bxs), f(dbs), f(B), thresh)
+
+
+def test_contracted_extent_closed_forms():
+    """common/screening.f90:198-232 (Maurer et al. 2012, eqs. B1-B4).
+    One primitive pair: the contracted centre is P itself, and the extent r
+    is where the pair distribution's tail falls to the threshold,
+    erfc(r sqrt(p/2)) = thresh_ws, whatever the centres and coefficients.
+    Two equally weighted primitive pairs with centres P1, P2: the centre is
+    the midpoint, and the extent is the larger primitive extent plus
+    |P1 - P2| / 2.  Symmetric in the two shells."""
+    from scipy.special import erfc
+
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        a, b = np.exp(rng.uniform(np.log(0.05), np.log(500.0), 2))
+        A, B = rng.uniform(-3.0, 3.0, 3), rng.uniform(-3.0, 3.0, 3)
+        da, db = rng.uniform(-1.0, 1.0, 2)
+        for thr in (1e-4, 1e-8):
+            r = _extent([a], [da], A, [b], [db], B, thr)
+            assert erfc(r * math.sqrt((a + b) / 2.0)) == pytest.approx(thr, rel=1e-12)
+        assert _extent([a], [da], A, [b], [db], B) == pytest.approx(_extent([b], [db], B, [a], [da], A),
+                                                                    rel=1e-15)
+    # two primitives on a (equal |d|), one on b: centres P1, P2 of the pairs
+    A, B = np.zeros(3), np.array([0.0, 0.0, 2.0])
+    a1, a2, b = 0.3, 2.7, 0.9
+    P1, P2 = b * B / (a1 + b), b * B / (a2 + b)
+    r1 = _extent([a1], [1.0], A, [b], [1.0], B)
+    r2 = _extent([a2], [1.0], A, [b], [1.0], B)
+    half = np.linalg.norm(P1 - P2) / 2.0
+    assert np.allclose(_center([a1, a2], [0.7, -0.7], A, [b], [0.4], B), (P1 + P2) / 2.0, rtol=0, atol=1e-15)
+    assert _extent([a1, a2], [0.7, -0.7], A, [b], [0.4], B) == pytest.approx(max(r1, r2) + half, rel=1e-14)
+    # translation invariance
+    s = np.array([3.0, -1.0, 0.5])
+    assert _extent([a1, a2], [0.7, 0.2], A + s, [b], [0.4], B + s) == pytest.approx(
+        _extent([a1, a2], [0.7, 0.2], A, [b], [0.4], B), rel=1e-13)
+
+
+def test_auxiliary_extent_branches():
+    """common/screening.f90:168-196.  For an exponent alpha the extent r is
+    where sqrt(alpha) exp(-alpha r^2) falls to prec (both branches: the
+    square-root branch for alpha > prec^2, the 0.01 floor below it, where
+    sqrt(alpha) < prec already at r = 0); a contracted shell takes the
+    largest primitive extent."""
+    L = O.lib()
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    for prec in (1e-8, 1e-6):
+        for alpha in (0.03, 0.5, 7.0, 500.0, 2.0 * prec ** 2):
+            r = L.orc_auxiliary_extent(1, f([alpha]), prec)
+            assert math.sqrt(alpha) * math.exp(-alpha * r * r) == pytest.approx(prec, rel=1e-10)
+        for alpha in (prec ** 2 / 2.0, 1e-30):
+            assert L.orc_auxiliary_extent(1, f([alpha]), prec) == 0.01
+    e = [0.04, 3.0, 90.0]
+    assert L.orc_auxiliary_extent(3, f(e), 1e-8) == max(L.orc_auxiliary_extent(1, f([x]), 1e-8) for x in e)
+    assert L.orc_auxiliary_extent(2, f([1e-30, 1e-31]), 1e-8) == 0.01
+
+
 def _qvl_ratio(la, lb, Lc, g, R):
     """QVL estimate / ||(ab|C)||_F for a two-centre contracted pair and an aux
     shell of angular momentum Lc and exponent g at distance R."""
