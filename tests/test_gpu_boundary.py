@@ -7,6 +7,7 @@ value, void return.  Statuses are read with dfhip_last_status().
 """
 
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -42,7 +43,7 @@ def call_4d(shells, aux, dens):
     keep_x, a_x = _fargs(aux)
     fd = np.asfortranarray(dens)  # _intor.pyx:147
     ndens, nbfs = dens.shape[0], dens.shape[1]
-    nstates = int(np.isqrt(8 * ndens + 1) // 2)
+    nstates = (math.isqrt(8 * ndens + 1) - 1) // 2
     assert nstates * (nstates + 1) // 2 == ndens
     J = np.zeros((nstates,) * 4, order="F")
     L.f_contract_coulomb_densities_4d(*a_o, *a_x, ndens, nbfs, fd.ctypes.data, nstates, J.ctypes.data)

@@ -26,6 +26,18 @@ from pysisyphus_addons_amd.diabatization import _intor
 
 pytestmark = pytest.mark.gpu
 
+# torch ships a HIP runtime of its own; it finds no device once the library's
+# HIP runtime has initialised the GPU in the same process, so the 200-atom test
+# (device densities drawn with torch) lets torch take the device first, at
+# collection, before any test calls the library (bench.py does the same).
+try:
+    import torch
+
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda:0")
+except ImportError:
+    torch = None
+
 
 def _say(msg):
     print(msg, flush=True)
@@ -143,8 +155,6 @@ def test_qvl_records_vs_oracle(key):
 def _device_densities(nstates, nbf, seed):
     """nstates(nstates+1)/2 symmetric (A + A^T)/2, A ~ U(0, 1), unit trace,
     drawn on the device (the 200-atom workload holds 102 GB of them)."""
-    import torch
-
     dev = torch.device("cuda:0")
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
@@ -170,8 +180,6 @@ def test_workload4_full_properties():
     20 states) on device densities: mixed within 1e-8 of FP64, the 8-fold
     symmetry of J, 64-bit counts, the class histogram summing to the
     screened count, both precisions screening the same triplets."""
-    import torch
-
     shells, aux, _ = synthetic.workload(4, with_densities=False)
     b, x = _arrays(shells, aux)
     nstates = 20
