@@ -186,6 +186,18 @@ int dfhip_release_workspace(void);
 int dfhip_qvl_records(const dfhip_basis *bas, const dfhip_basis *aux, double *pair_records,
                       double *aux_records);
 
+/* Allocation-free form of f_int_schwarz_aux: integrals[c] = sqrt(||(C|C)||_F)
+ * per aux shell, host arrays in and out, synchronous.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  It calls no hipMalloc, hipFree or
+ * pinned allocation: every device buffer lives in a static 16 MiB device array
+ * per GPU (88 bytes per aux function plus about 100 bytes per shell must fit,
+ * e.g. 170,000 functions in h shells; a larger basis is rejected with
+ * DFHIP_ERR_ARG).  f_init() must have run on the current device first.
+ * Calls in one process are serialized by the library; the array keeps the last
+ * call's contents. */
+int dfhip_int_schwarz_aux_noalloc(const dfhip_basis *aux, double *integrals);
+
 /* Metric (P|Q), naux x naux (symmetric). */
 int dfhip_metric(const dfhip_basis *aux, double *out);
 

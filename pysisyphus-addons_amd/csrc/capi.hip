@@ -376,9 +376,21 @@ struct Lane {
   }
 };
 
+// Device scratch of the allocation-free entry point
+// (dfhip_int_schwarz_aux_noalloc): every GPU the library runs on holds its own
+// copy of this array, at its own address, until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is only
+// a stand-in for the symbol): it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept per device in
+// Ctx::scratch_dev.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;  // 16 MiB per device
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct Ctx {
   bool ready = false;
   int device = -1;
+  unsigned char* scratch_dev[NOALLOC_MAX_DEVICES] = {};  // device address of `scratch`, per device
   Lane lane0;                                 // default lane (its blocks come from dev_pool())
   std::vector<std::unique_ptr<Lane>> lanes;   // batch lanes 1..
   DBuf<double> boys;
@@ -473,12 +485,26 @@ struct DView {
   T* p = nullptr;
 };
 
+// Device address of the allocation-free scratch array on device dev (the
+// current device), looked up once per device.
+unsigned char* resolve_scratch(Ctx& c, int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) throw Error{DFHIP_ERR_UNSUPPORTED, "device index out of range"};
+  if (!c.scratch_dev[dev]) {
+    void* p = nullptr;
+    HC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error{DFHIP_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array"};
+    c.scratch_dev[dev] = (unsigned char*)p;
+  }
+  return c.scratch_dev[dev];
+}
+
 void ensure_ctx() {
   Ctx& c = ctx();
   if (c.ready) return;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error{DFHIP_ERR_HIP, "no HIP device available"};
   HC(hipGetDevice(&c.device));
+  resolve_scratch(c, c.device);
   HC(hipStreamCreateWithFlags(&c.lane0.st, hipStreamNonBlocking));
   const hipStream_t st0 = c.lane0.st;
   std::vector<double> tab;
@@ -2328,6 +2354,124 @@ std::vector<double> schwarz_aux_host(const dfhip_basis& b) {
   return q;
 }
 
+// The allocation-free form of schwarz_aux_host for callers that cannot
+// allocate device memory (code under HIP graph capture, tight serving loops):
+// the same diagonal blocks (C|C) through the LDS 2c2e kernels, with every
+// device buffer (shell arrays, aux lists, the compact blocks, the norms)
+// carved out of the static scratch array; no hipMalloc, hipFree or pinned
+// allocation (host-side vectors only).  Needs f_init() on this device first,
+// which builds the tables and looks up the scratch array's address.  The
+// library's lock serializes calls in this process and each call waits for its
+// work before returning, so the scratch array is never used by two calls at
+// once here; the array keeps the last call's contents (nothing in it needs to
+// start at zero: every element a kernel reads is written earlier in the same
+// call).
+void schwarz_aux_noalloc(const dfhip_basis& b, double* integrals) {
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  Ctx& c = ctx();
+  if (!c.ready) throw Error{DFHIP_ERR_ARG, "call f_init() before the allocation-free entry point"};
+  int dev = -1;
+  HC(hipGetDevice(&dev));
+  if (dev != c.device)
+    throw Error{DFHIP_ERR_UNSUPPORTED, "the library was set up on device " + std::to_string(c.device) +
+                                           ", the current device is " + std::to_string(dev)};
+  unsigned char* base = resolve_scratch(c, dev);
+  const HostShells x = parse_shells(b, LMAX_AUX, "basis");
+  constexpr int LDC = 2 * LMAX_AUX + 1;
+  const int n = x.n, nexp = (int)x.exps.size();
+  // layout of the scratch array, each region 256-byte aligned
+  size_t top = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = top;
+    top = (top + bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t oL = carve(n * sizeof(int)), onp = carve(n * sizeof(int)), opp = carve(n * sizeof(int)),
+               of0 = carve(n * sizeof(int)), oxyz = carve(3 * n * sizeof(double)),
+               oex = carve(nexp * sizeof(double)), oco = carve(nexp * sizeof(double)),
+               os1 = carve(n * sizeof(int)), os2 = carve(n * sizeof(int)), onb = carve(n * sizeof(int)),
+               one = carve(n * sizeof(int)), orow = carve(n * sizeof(int64_t)), ocol = carve(n * sizeof(int64_t));
+  const size_t nup = top;
+  const size_t oblk = carve((size_t)x.nbf * LDC * sizeof(double)), oq = carve(n * sizeof(double));
+  if (top > NOALLOC_SCRATCH_BYTES)
+    throw Error{DFHIP_ERR_ARG, "the basis needs " + std::to_string(top) +
+                                   " bytes of device scratch; the allocation-free entry point holds at most " +
+                                   std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                   " (88 bytes per auxiliary function plus about 100 per shell)"};
+  // host image of everything that goes up, in the same layout
+  std::vector<unsigned char> img(nup, 0);
+  auto put = [&](size_t o, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(img.data() + o, src, bytes);
+  };
+  put(oL, x.L.data(), n * sizeof(int));
+  put(onp, x.nprim.data(), n * sizeof(int));
+  put(opp, x.pptr.data(), n * sizeof(int));
+  put(of0, x.f0.data(), n * sizeof(int));
+  put(oxyz, x.xyz.data(), 3 * n * sizeof(double));
+  put(oex, x.exps.data(), nexp * sizeof(double));
+  put(oco, x.coefs.data(), nexp * sizeof(double));
+  // per L: bra shell i of the list meets only its own list entry (window [k, k + 1))
+  std::vector<int> lbeg(LMAX_AUX + 2, 0), kmax(LMAX_AUX + 1, 1);
+  {
+    std::vector<int> s1, s2, nb, ne;
+    std::vector<int64_t> row, col;
+    for (int L = 0; L <= LMAX_AUX; L++) {
+      lbeg[L] = (int)s1.size();
+      int k = 0;
+      for (int i = 0; i < n; i++) {
+        if (x.L[i] != L) continue;
+        s1.push_back(i);
+        s2.push_back(-1);
+        nb.push_back(k);
+        ne.push_back(k + 1);
+        row.push_back(x.f0[i]);
+        col.push_back(0);
+        kmax[L] = std::max(kmax[L], x.nprim[i]);
+        k++;
+      }
+    }
+    lbeg[LMAX_AUX + 1] = (int)s1.size();
+    put(os1, s1.data(), n * sizeof(int));
+    put(os2, s2.data(), n * sizeof(int));
+    put(onb, nb.data(), n * sizeof(int));
+    put(one, ne.data(), n * sizeof(int));
+    put(orow, row.data(), n * sizeof(int64_t));
+    put(ocol, col.data(), n * sizeof(int64_t));
+  }
+  const hipStream_t st = c.lane0.st;
+  HC(hipMemcpyAsync(base, img.data(), nup, hipMemcpyHostToDevice, st));
+  auto at = [&](size_t o) { return base + o; };
+  const DevShells dv{n,
+                     (const int*)at(oL),
+                     (const int*)at(onp),
+                     (const int*)at(opp),
+                     (const int*)at(of0),
+                     (const double*)at(oxyz),
+                     (const double*)at(oex),
+                     (const double*)at(oco)};
+  const int* ds1 = (const int*)at(os1);
+  const int* ds2 = (const int*)at(os2);
+  const int* dnb = (const int*)at(onb);
+  const int* dne = (const int*)at(one);
+  const int64_t* drow = (const int64_t*)at(orow);
+  const int64_t* dcol = (const int64_t*)at(ocol);
+  double* blk = (double*)at(oblk);
+  double* q = (double*)at(oq);
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    const int b0 = lbeg[L], np = lbeg[L + 1] - b0;
+    if (np == 0) continue;
+    PairArgs pa{ds1 + b0, ds2 + b0, drow + b0, dne + b0, np};
+    pa.nbeg = dnb + b0;
+    const AuxArgs xa{ds1 + b0, dcol + b0, 0, np, kmax[L]};
+    c.int3[L][0][L].fn(dv, dv, pa, xa, c.sph.p, c.boys.p, blk, LDC, st);
+    HC(hipGetLastError());
+  }
+  schwarz_aux_from_metric(blk, LDC, dv.L, dv.f0, n, q, st, true);
+  HC(hipGetLastError());
+  HC(hipMemcpyAsync(integrals, q, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
 // The QVL screener records of the device path: per packed pair
 // k = a(a+1)/2 + b (a >= b) [centre x, y, z, extent, (min a + min b)^(1/4),
 // Q_ab, 0, 0] from qvl_pair_kernel, per aux shell [extent, beta, Q_C, 0].
@@ -2512,6 +2656,13 @@ int dfhip_int3c2e_dense(const dfhip_basis* bas, const dfhip_basis* aux, const df
     Outputs out;
     out.dense3c = out3;
     run(*bas, *aux, 0, 0, nullptr, o, out, stats);
+  });
+}
+
+int dfhip_int_schwarz_aux_noalloc(const dfhip_basis* aux, double* integrals) {
+  return guarded([&] {
+    if (!aux || !integrals) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    schwarz_aux_noalloc(*aux, integrals);
   });
 }
 

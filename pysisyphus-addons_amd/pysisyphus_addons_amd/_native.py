@@ -78,7 +78,7 @@ EXPORTED = (
     "f_int_schwarz_aux", "f_int_schwarz_bra", "dfhip_last_status", "dfhip_last_error",
     "dfhip_default_options", "dfhip_contract_coulomb_densities", "dfhip_int3c2e_dense",
     "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
-    "dfhip_qvl_records",
+    "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc",
 )
 
 
@@ -104,6 +104,8 @@ def lib():
         L.dfhip_int3c2e_dense.restype = ctypes.c_int
         L.dfhip_int3c2e_dense.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis),
                                           ctypes.POINTER(Options), vp, ctypes.POINTER(Stats)]
+        L.dfhip_int_schwarz_aux_noalloc.restype = ctypes.c_int
+        L.dfhip_int_schwarz_aux_noalloc.argtypes = [ctypes.POINTER(Basis), vp]
         L.dfhip_qvl_records.restype = ctypes.c_int
         L.dfhip_qvl_records.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis), vp, vp]
         L.dfhip_metric.restype = ctypes.c_int
