@@ -1364,24 +1364,25 @@ void screen_sweep(Plan& pl, const dfhip_options& opt) {
 // Mixed precision: the FP64 part of each screened prefix, the list entries
 // with Q_ab Q_C >= tau (a prefix again, non-increasing along a class), for the
 // classes with FP32 kernels; the rest of the prefix runs in FP32.
-void split_precision(Plan& pl, double tau, bool gen) {
+void split_precision(Plan& pl, double tau, const std::vector<std::vector<char>>& lowp) {
   pl.n64 = pl.nsurv;
-  pl.ncomputed32 = 0;
   for (int L = 0; L <= LMAX_AUX; L++) {
     const auto& v = pl.alist[L];
-    for (int la = 0; la <= LMAX_ORB; la++)
-      for (int lb = 0; lb <= la; lb++) {
-        if (!gen || !gen3_has(la, lb, L)) continue;
-        const int k = pcls(la, lb);
-        int n = (int)v.size();
-        for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
-          const double qa = pl.qab[p];
-          while (n > 0 && qa * pl.qc[v[n - 1]] < tau) n--;
-          pl.n64[L][p] = std::min(n, pl.nsurv[L][p]);
-          pl.ncomputed32 += pl.nsurv[L][p] - pl.n64[L][p];
-        }
+    for (int k = 0; k < NPCLS; k++) {
+      if (!lowp[k][L]) continue;
+      int n = (int)v.size();
+      for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+        const double qa = pl.qab[p];
+        while (n > 0 && qa * pl.qc[v[n - 1]] < tau) n--;
+        pl.n64[L][p] = std::min(n, pl.nsurv[L][p]);
       }
+    }
   }
+}
+
+// Class k, aux L back to one FP64 window (its low-precision kernel cannot run).
+void unsplit_precision(Plan& pl, int k, int L) {
+  for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) pl.n64[L][p] = pl.nsurv[L][p];
 }
 
 // Verbose progress line, flushed at once so that a caller redirecting stdout
@@ -1995,12 +1996,20 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
   DBuf<int> dn64flat;  // FP64 windows of the prefixes, [L][pair] like dnsflat
+  // mixed precision: the low-precision kernel of each (class, aux L), 1 the
+  // generated FP32 kernels, 2 the FP32 matrix-core kernels (0: FP64 only);
+  // the split itself is final once the tables of the FP32 matrix-core
+  // kernels are built, below
+  std::vector<std::vector<char>> lowp(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
   if (mixed) {
-    split_precision(pl, opt.fp32_below, gen);
-    std::vector<int> f((size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1));
-    for (int L = 0; L <= LMAX_AUX; L++)
-      std::copy(pl.n64[L].begin(), pl.n64[L].end(), f.begin() + (size_t)L * pl.ps.n());
-    upload_pinned(dn64flat, f, st);
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++)
+        for (int L = 0; L <= LMAX_AUX; L++) {
+          const int k = pcls(la, lb);
+          if (gen && gen3_has(la, lb, L)) lowp[k][L] = 1;
+          else if (use_mm[k][L] && c.int3m[la][lb][L].fn32 != nullptr) lowp[k][L] = 2;
+        }
+    split_precision(pl, opt.fp32_below, lowp);
   }
   const DView<int> dorig{ab.dev<int>(o_orig)};
   vstep("plan uploads");
@@ -2050,7 +2059,64 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // rows from the class start to the last pair whose screened prefix reaches
   // the tile (under Schwarz screening these pairs are exactly a prefix of
   // the class: pairs are sorted by Q_ab and the prefixes are non-increasing).
-  if (opt.screening == DFHIP_SCREEN_QVL) build_et([&](int k, int L, int p) { return pl.nsurv[L][p] > 0; }, st, false);
+  if (opt.screening == DFHIP_SCREEN_QVL)
+    build_et([&](int k, int L, int p) { return (lowp[k][L] ? pl.n64[L][p] : pl.nsurv[L][p]) > 0; }, st, false);
+  // FP32 Et tables of the FP32 matrix-core windows [n64, nsurv); a class whose
+  // table does not fit (or that lost its FP64 table) stays FP64
+  std::vector<DBuf<float>> etg32(NPCLS);
+  std::vector<DBuf<int64_t>> etoff32(NPCLS);
+  std::vector<DBuf<int>> etlist32(NPCLS);
+  if (mixed) {
+    size_t fr = 0, tot = 0;
+    HC(hipMemGetInfo(&fr, &tot));
+    const size_t cap32 = std::max<size_t>(fr / 6, 1);
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++) {
+        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
+        bool any = false;
+        for (int L = 0; L <= LMAX_AUX; L++) {
+          if (lowp[k][L] == 2 && !(use_mm[k][L] && etg[k].p)) {
+            lowp[k][L] = 0;
+            unsplit_precision(pl, k, L);
+          }
+          any = any || lowp[k][L] == 2;
+        }
+        if (!any || np == 0) continue;
+        std::vector<int> list;
+        std::vector<int64_t> off(np, 0);
+        int64_t tote = 0;
+        for (int p = 0; p < np; p++) {
+          bool nd = false;
+          for (int L = 0; L <= LMAX_AUX && !nd; L++) nd = lowp[k][L] == 2 && pl.nsurv[L][p0 + p] > pl.n64[L][p0 + p];
+          if (!nd) continue;
+          list.push_back(p);
+          off[p] = tote;
+          tote += (int64_t)o.nprim[pl.ps.s1[p0 + p]] * o.nprim[pl.ps.s2[p0 + p]] * c.et[la][lb].etpp;
+        }
+        if ((size_t)tote * sizeof(float) > cap32) {
+          for (int L = 0; L <= LMAX_AUX; L++)
+            if (lowp[k][L] == 2) {
+              lowp[k][L] = 0;
+              unsplit_precision(pl, k, L);
+            }
+          continue;
+        }
+        if (list.empty()) continue;
+        upload_pinned(etlist32[k], list, st);
+        upload_pinned(etoff32[k], off, st);
+        etg32[k].alloc((size_t)tote);
+        c.et[la][lb].fn32(dorb.view, ps1.p + p0, ps2.p + p0, etlist32[k].p, etoff32[k].p, (int)list.size(),
+                          c.sph.p, etg32[k].p, st);
+        HC(hipGetLastError());
+      }
+    pl.ncomputed32 = 0;
+    for (int L = 0; L <= LMAX_AUX; L++)
+      for (int p = 0; p < pl.ps.n(); p++) pl.ncomputed32 += pl.nsurv[L][p] - pl.n64[L][p];
+    std::vector<int> f((size_t)(LMAX_AUX + 1) * std::max(pl.ps.n(), 1));
+    for (int L = 0; L <= LMAX_AUX; L++)
+      std::copy(pl.n64[L].begin(), pl.n64[L].end(), f.begin() + (size_t)L * pl.ps.n());
+    upload_pinned(dn64flat, f, st);
+  }
   vstep("Et tables");
   const int GT = contract_tile();
   // suffix maxima of the prefixes inside each class: under Schwarz screening
@@ -2139,13 +2205,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
                                   st);
             HC(hipGetLastError());
           };
-          if (gen && gen3_has(la, lb, L) && mixed) {
+          if (mixed && lowp[k][L]) {
             PairArgs p64 = pa, p32 = pa;
             p64.nsurv = dn64flat.p + (size_t)L * pl.ps.n() + p0;  // [0, n64): FP64
             p32.nbeg = dn64flat.p + (size_t)L * pl.ps.n() + p0;   // [n64, nsurv): FP32
             if (mmk) launch_mm(p64);
             else launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);
-            launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
+            if (lowp[k][L] == 1) {
+              launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
+            } else {
+              c.int3m[la][lb][L].fn32(dorb.view, daux.view, p32, xa, etg32[k].p, etoff32[k].p, c.sph.p, c.boys.p,
+                                      Tp - cA, w, st);
+              HC(hipGetLastError());
+            }
           } else if (mmk) {
             launch_mm(pa);
           } else if (gen && gen3_has(la, lb, L)) {

@@ -16,6 +16,12 @@
 //     product on v_mfma_f64_16x16x4 with A fragments read straight from the
 //     global Et table and B fragments from LDS; the accumulators stay in
 //     registers over all primitive pairs.
+// Both kernels exist for T = double (the FP64 path) and T = float (the
+// mixed path's low-precision window): with T = float the Et table, R_tuv,
+// the Hobson step and the product (v_mfma_f32_16x16x4f32) run in FP32, while
+// the Boys values and prefactors (the G table) stay FP64 and each primitive
+// pair's FP32 product is added into FP64 accumulators (FP32 sums over at most
+// nherm(la+lb) terms, FP64 across primitive pairs).
 #pragma once
 
 #include "gen3c_common.h"
@@ -24,8 +30,33 @@
 namespace dfhip {
 
 typedef double dbl4m __attribute__((ext_vector_type(4)));
+typedef float flt4m __attribute__((ext_vector_type(4)));
 
-template <int LA, int LB, int LC>
+// The 16x16x4 matrix-core product per arithmetic type.  A fragment: lane l
+// holds A[l % 16][l / 16]; B fragment: B[l / 16][l % 16]; accumulator
+// register r of lane l: column l % 16, row l / 16 + 4 r for FP64
+// (v_mfma_f64_16x16x4f64) and 4 (l / 16) + r for FP32
+// (v_mfma_f32_16x16x4f32).
+template <typename T>
+struct MMOp;
+template <>
+struct MMOp<double> {
+  typedef dbl4m v4;
+  static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __host__ __device__ constexpr int row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct MMOp<float> {
+  typedef flt4m v4;
+  static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __host__ __device__ constexpr int row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <int LA, int LB, int LC, typename T = double>
 struct Cls3m {
   static constexpr int LAB = LA + LB, L = LAB + LC;
   static constexpr int NA = nsph(LA), NB = nsph(LB), NAB = NA * NB, NCC = nsph(LC);
@@ -38,11 +69,11 @@ struct Cls3m {
   // R^(n)_tuv by levels n = L-1 .. 0 in two buffers: even levels in R (all NR
   // indices), odd levels in Rt (orders <= L-1); odd strides per aux shell
   static constexpr int NRP = NR | 1, NRT = nherm(L - 1) | 1;
-  static constexpr int BUD = 18500;       // LDS doubles for R, Rt, G and W
+  static constexpr int BUD = 18500 * 8;   // LDS bytes for R, Rt, G and W
   // G of up to NPG primitive pairs is built once per workgroup (uncontracted aux)
   static constexpr int NPG = 9;
   static constexpr int nc_fit(int nc) {
-    return nc * (NRP + NRT + NPG * GWP) + HK * ((nc * NCC + 15) / 16 * 16) <= BUD;
+    return (int)((nc * (NRP + NRT) + HK * ((nc * NCC + 15) / 16 * 16)) * sizeof(T) + nc * NPG * GWP * 8) <= BUD;
   }
   static constexpr int nc_max() {
     int nc = 32;
@@ -66,15 +97,16 @@ struct Cls3m {
 };
 
 // Et tables of the listed pairs: EtG[off[p] + ij*ETPP + h*NABP + mu*NB + nu],
-// contraction coefficients folded in, zero padding.
-template <int LA, int LB>
+// contraction coefficients folded in, zero padding; evaluated in FP64 and
+// stored as TO (double, or float for the FP32 window of the mixed path).
+template <int LA, int LB, typename TO>
 __global__ __launch_bounds__(256) void int3c2e_et_kernel(DevShells bra, const int* __restrict__ s1v,
                                                          const int* __restrict__ s2v,
                                                          const int* __restrict__ list,
                                                          const int64_t* __restrict__ off,
-                                                         const SphTable* __restrict__ sph, double* __restrict__ EtG) {
+                                                         const SphTable* __restrict__ sph, TO* __restrict__ EtG) {
   using K = Cls3m<LA, LB, 0>;
-  constexpr int H = K::H, HK = K::HK, NAB = K::NAB, NABP = K::NABP, NB = K::NB, ESZ = K::ESZ, NT = 256;
+  constexpr int H = K::H, NAB = K::NAB, NABP = K::NABP, NB = K::NB, ESZ = K::ESZ, NT = 256;
   constexpr int NPPM = 16;  // primitive pairs per pass
   __shared__ double E[NPPM * 3 * ESZ];
   __shared__ int tuv[H];
@@ -85,7 +117,7 @@ __global__ __launch_bounds__(256) void int3c2e_et_kernel(DevShells bra, const in
   const int s1 = s1v[p], s2 = s2v[p];
   const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb;
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
-  double* out = EtG + off[p];
+  TO* out = EtG + off[p];
   for (int h = tid; h < H; h += NT) {
     int t, u, v;
     herm_tuv(h, t, u, v);
@@ -118,27 +150,31 @@ __global__ __launch_bounds__(256) void int3c2e_et_kernel(DevShells bra, const in
               bra_element_lds<LA, LB>(sa, ta, sb, tb, n / NB, n % NB, tv & 255, (tv >> 8) & 255, tv >> 16, Eq,
                                       Eq + ESZ, Eq + 2 * ESZ);
       }
-      out[(int64_t)ij * K::ETPP + r] = val;
+      out[(int64_t)ij * K::ETPP + r] = (TO)val;
     }
   }
 }
 
-template <int LA, int LB, int LC>
+// One shell pair x NC aux shells of one L per workgroup, the list window
+// [nbeg, nsurv) of the pair (nbeg null: from the start), T the arithmetic
+// type of R_tuv, the Hobson step and the product (see the file head).
+template <int LA, int LB, int LC, typename T>
 __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
-                                                         const double* __restrict__ EtG,
+                                                         const T* __restrict__ EtG,
                                                          const int64_t* __restrict__ etoff,
                                                          const SphTable* __restrict__ sph,
                                                          const double* __restrict__ boys, double* __restrict__ out,
                                                          int64_t ldo) {
-  using K = Cls3m<LA, LB, LC>;
+  using K = Cls3m<LA, LB, LC, T>;
+  using Op = MMOp<T>;
   constexpr int L = K::L, H = K::H, HK = K::HK, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NABP = K::NABP;
-  constexpr int NT = K::NT, NW = K::NW, NC = K::NC, NCOLP = K::NCOLP, GW = K::GW, GWP = K::GWP;
+  constexpr int NT = K::NT, NC = K::NC, NCOLP = K::NCOLP, GWP = K::GWP;
   constexpr int NRP = K::NRP, NRT = K::NRT;
   constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL;
   __shared__ double G[K::NPG * NC * GWP];  // per primitive pair and aux shell: scaled Boys values, then 2 PC_d
-  __shared__ double R[NC * NRP];       // per aux shell: R^(n)_tuv of the even levels, finally R_tuv
-  __shared__ double Rt[NC * NRT];      // per aux shell: R^(n)_tuv of the odd levels
-  __shared__ double W[HK * NCOLP];     // W[h][c NCC + m]
+  __shared__ T R[NC * NRP];       // per aux shell: R^(n)_tuv of the even levels, finally R_tuv
+  __shared__ T Rt[NC * NRT];      // per aux shell: R^(n)_tuv of the odd levels
+  __shared__ T W[HK * NCOLP];     // W[h][c NCC + m]
   __shared__ double cxyz[NC * 3];
   __shared__ int cshell[NC], ckeep[NC];
   __shared__ int tuv[NR];
@@ -152,12 +188,13 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   const int pidx = (int)blockIdx.y * 8 + (int)(blockIdx.x & 7);
   if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int c0 = xa.lo + (int)(blockIdx.x >> 3) * NC;
-  if (c0 >= cend) return;
-  const int nc = min(NC, cend - c0);
+  const int cchunk = xa.lo + (int)(blockIdx.x >> 3) * NC;
+  const int c0 = pa.nbeg ? max(cchunk, pa.nbeg[pidx]) : cchunk;
+  if (c0 >= min(cend, cchunk + NC)) return;
+  const int nc = min(cchunk + NC, cend) - c0;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
-  const double* Et0 = EtG + etoff[pidx];
+  const T* Et0 = EtG + etoff[pidx];
 
   for (int h = tid; h < NR; h += NT) {
     int t, u, v;
@@ -192,7 +229,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     cxyz[3 * c + 2] = ket.xyz[3 * sh + 2];
     ckeep[c] = triplet_kept(pa, xa, pidx, sh, LC, ket.xyz[3 * sh + 0], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2]);
   }
-  for (int e = tid; e < HK * NCOLP; e += NT) W[e] = 0.0;  // padding rows and columns stay zero
+  for (int e = tid; e < HK * NCOLP; e += NT) W[e] = (T)0;  // padding rows and columns stay zero
 
   double A[3], B[3];
   for (int d = 0; d < 3; d++) {
@@ -203,53 +240,60 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
   const double lc_sign = (LC & 1) ? -1.0 : 1.0;
 
-  // G of one (primitive pair, aux primitive, aux shell): lane part 0 of 4 the
-  // scaled Boys values g_n = s (-alpha)^n F_n(T) (F_L from the table, the rest
-  // by downward recursion), parts 1-3 2 PC_d; the R recursion is seeded with g_L
-  auto gstep = [&](int k, const double* P, double p, int kp, double* Gd) {
-    const int c = k >> 2, part = k & 3;
+  // G of one (primitive pair, aux primitive, aux shell) in FP64, in two
+  // parts so that every lane of a loop takes the same branch: gboys the
+  // scaled Boys values g_n = s (-alpha)^n F_n(T) (F_L from the table, the
+  // rest by downward recursion), gpc the 2 PC_d
+  auto gboys = [&](int c, const double* P, double p, int kp, double* Gd) {
     const double PC0 = P[0] - cxyz[3 * c + 0], PC1 = P[1] - cxyz[3 * c + 1], PC2 = P[2] - cxyz[3 * c + 2];
     double* gc = Gd + c * GWP;
-    if (part == 0) {
-      const int sh = cshell[c];
-      const bool live = kp < ket.nprim[sh] && ckeep[c];
-      double F[L + 1];
-      double scale = 0.0, alpha = 0.0;
-      if (live) {
-        const double g = ket.exps[ket.pptr[sh] + kp], dk = ket.coefs[ket.pptr[sh] + kp];
-        alpha = p * g / (p + g);
-        const double T = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
-        scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
-        const double inv2g = 0.5 / g;
+    const int sh = cshell[c];
+    const bool live = kp < ket.nprim[sh] && ckeep[c];
+    double F[L + 1];
+    double scale = 0.0, alpha = 0.0;
+    if (live) {
+      const double g = ket.exps[ket.pptr[sh] + kp], dk = ket.coefs[ket.pptr[sh] + kp];
+      alpha = p * g / (p + g);
+      const double Tb = alpha * (PC0 * PC0 + PC1 * PC1 + PC2 * PC2);
+      scale = 2.0 * PI * PI * sqrt(PI) / (p * g * sqrt(p + g)) * dk * lc_sign;
+      const double inv2g = 0.5 / g;
 #pragma unroll
-        for (int e = 0; e < LC; e++) scale *= inv2g;
-        boys_array<L>(T, boys, F);
-      } else {
-#pragma unroll
-        for (int n = 0; n <= L; n++) F[n] = 0.0;
-      }
-#pragma unroll
-      for (int n = 0; n <= L; n++) {
-        gc[n] = scale * F[n];
-        scale *= -alpha;
-      }
-      if (Gd == G) (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gc[L];  // R^(L)_000
+      for (int e = 0; e < LC; e++) scale *= inv2g;
+      boys_array<L>(Tb, boys, F);
     } else {
-      gc[L + part] = 2.0 * (part == 1 ? PC0 : (part == 2 ? PC1 : PC2));
+#pragma unroll
+      for (int n = 0; n <= L; n++) F[n] = 0.0;
+    }
+#pragma unroll
+    for (int n = 0; n <= L; n++) {
+      gc[n] = scale * F[n];
+      scale *= -alpha;
     }
   };
+  auto gpc = [&](int c, int d, const double* P, double* Gd) {
+    Gd[c * GWP + L + 1 + d] = 2.0 * (P[d] - cxyz[3 * c + d]);
+  };
+  auto seed = [&](int c, const double* gb) { (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = (T)gb[c * GWP + L]; };
   // uncontracted aux shells and at most NPG primitive pairs: G of every
   // primitive pair at once (9x the parallelism, one wait on the Boys table)
   const bool pre = xa.kmax == 1 && npp <= K::NPG;
   if (pre) {
     __syncthreads();  // cshell, cxyz and ckeep of the prologue are read below
-    for (int k = tid; k < npp * nc * 4; k += NT) {
-      const int ij = k / (nc * 4), kk = k - ij * (nc * 4);
+    for (int k = tid; k < npp * nc; k += NT) {
+      const int ij = k / nc, c = k - ij * nc;
       const int i = ij / npb, j = ij - i * npb;
       const double a = bra.exps[pa0 + i], b = bra.exps[pb0 + j], p = a + b;
       double P[3];
       for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
-      gstep(kk, P, p, 0, G + ij * NC * GWP);
+      gboys(c, P, p, 0, G + ij * NC * GWP);
+    }
+    for (int k = tid; k < npp * nc * 3; k += NT) {
+      const int ij = k / (nc * 3), r = k - ij * (nc * 3), c = r / 3, d = r - c * 3;
+      const int i = ij / npb, j = ij - i * npb;
+      const double a = bra.exps[pa0 + i], b = bra.exps[pb0 + j], p = a + b;
+      double P[3];
+      for (int e = 0; e < 3; e++) P[e] = (a * A[e] + b * B[e]) / p;
+      gpc(c, d, P, G + ij * NC * GWP);
     }
   }
 
@@ -263,22 +307,25 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     const double a = bra.exps[pa0 + i], b = bra.exps[pb0 + j], p = a + b;
     double P[3];
     for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
-    const double* Et = Et0 + (int64_t)ij * K::ETPP;
+    const T* Et = Et0 + (int64_t)ij * K::ETPP;
     for (int kp = 0; kp < xa.kmax; kp++) {
       __syncthreads();  // tables built (first pass); previous product done with W
-      double* gb = pre ? G + ij * NC * GWP : G;
+      const double* gb = pre ? G + ij * NC * GWP : G;
       if (pre) {
-        for (int c = tid; c < nc; c += NT) (L % 2 == 0 ? R + c * NRP : Rt + c * NRT)[0] = gb[c * GWP + L];
+        for (int c = tid; c < nc; c += NT) seed(c, gb);
       } else {
-        for (int k = tid; k < nc * 4; k += NT) gstep(k, P, p, kp, G);
+        for (int c = tid; c < nc; c += NT) gboys(c, P, p, kp, G);
+        for (int k = tid; k < nc * 3; k += NT) gpc(k / 3, k % 3, P, G);
+        __syncthreads();
+        for (int c = tid; c < nc; c += NT) seed(c, gb);
       }
       __syncthreads();
       // levels n = L-1 .. 0: every entry of a level is independent (aux shells
       // on lanes, odd strides); level n covers the orders t+u+v <= L-n
 #pragma unroll
       for (int n = L - 1; n >= 0; n--) {
-        const double* So = (n % 2 == 0) ? Rt : R;  // level n+1
-        double* Sn = (n % 2 == 0) ? R : Rt;
+        const T* So = (n % 2 == 0) ? Rt : R;  // level n+1
+        T* Sn = (n % 2 == 0) ? R : Rt;
         const int so = (n % 2 == 0) ? NRT : NRP, sn = (n % 2 == 0) ? NRP : NRT;
         // level 0: only the orders >= LC that the Hobson step reads
         const int hbeg = n == 0 ? K::NRLO : 0, nent = nherm(L - n) - hbeg;
@@ -287,14 +334,14 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
           const int h = hbeg + k / NC, c = k - (h - hbeg) * NC;  // compile-time divisor
           if (c >= nc) continue;
           const double* gc = gb + c * GWP;
-          double val;
+          T val;
           if (h == 0) {
-            val = gc[n];
+            val = (T)gc[n];
           } else {
             const int rc = rec[h];
             const int h1 = rc & 1023, h2 = (rc >> 10) & 1023, dir = (rc >> 20) & 3, cf = rc >> 22;
-            const double* sc0 = So + c * so;
-            val = fma((double)(2 * cf), sc0[h2], gc[L + 1 + dir] * sc0[h1]);
+            const T* sc0 = So + c * so;
+            val = fma((T)(2 * cf), sc0[h2], (T)gc[L + 1 + dir] * sc0[h1]);
           }
           Sn[c * sn + h] = val;
         }
@@ -309,43 +356,57 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
         const int tv = tuv[h];
         const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
         const int q0 = sc[m], nq = sc[m + 1] - q0;
-        double cq[K::MAXT];
+        T cq[K::MAXT];
         int iq[K::MAXT];
 #pragma unroll
         for (int z = 0; z < K::MAXT; z++) {
           const SphTerm tm = tc[z < nq ? q0 + z : q0];
-          cq[z] = z < nq ? tm.c : 0.0;
+          cq[z] = z < nq ? (T)tm.c : (T)0;
           iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
         }
         for (int c = cg; c < nc; c += K::CG) {
-          const double* Rc = R + c * NRP;
-          double s = 0.0;
+          const T* Rc = R + c * NRP;
+          T s = (T)0;
 #pragma unroll
           for (int z = 0; z < K::MAXT; z++) s = fma(cq[z], Rc[iq[z]], s);
           W[h * NCOLP + c * NCC + m] = s;
         }
       }
       __syncthreads();
-      // acc[nt] += Et[16 mt.., h] W[h, 16 nt..]
+      // acc[nt] += Et[16 mt.., h] W[h, 16 nt..]: FP64 accumulates in place,
+      // FP32 per primitive pair then into the FP64 accumulators
+      // (k step outer: one A fragment in flight, consecutive products on
+      // different accumulators)
       if (mt < MT) {
-        double av[KS];
-        const double* ea = Et + lk * NABP + 16 * mt + lr;
+        const T* ea = Et + lk * NABP + 16 * mt + lr;
+        const T* wb = W + lk * NCOLP + 16 * gi + lr;
+        if constexpr (sizeof(T) == sizeof(double)) {
+#pragma unroll 2
+          for (int s = 0; s < KS; s++) {
+            const double av = ea[4 * s * NABP];
 #pragma unroll
-        for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
+            for (int t = 0; t < NPW; t++)
+              if (gi + GRP * t < NTL) acc[t] = Op::mfma(av, wb[4 * s * NCOLP + 16 * GRP * t], acc[t]);
+          }
+        } else {
+          flt4m c4[NPW];
 #pragma unroll
-        for (int t = 0; t < NPW; t++) {
-          const int nt = gi + GRP * t;
-          if (nt >= NTL) break;
-          const double* wb = W + lk * NCOLP + 16 * nt + lr;
-          dbl4m c4 = acc[t];
+          for (int t = 0; t < NPW; t++) c4[t] = flt4m{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+          for (int s = 0; s < KS; s++) {
+            const float av = ea[4 * s * NABP];
 #pragma unroll
-          for (int s = 0; s < KS; s++) c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], wb[4 * s * NCOLP], c4, 0, 0, 0);
-          acc[t] = c4;
+            for (int t = 0; t < NPW; t++)
+              if (gi + GRP * t < NTL) c4[t] = Op::mfma(av, wb[4 * s * NCOLP + 16 * GRP * t], c4[t]);
+          }
+#pragma unroll
+          for (int t = 0; t < NPW; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[t][r] += (double)c4[t][r];
         }
       }
     }
   }
-  // register r of lane l: row mu nu = 16 mt + l/16 + 4 r, column 16 nt + l%16
   const int64_t row0 = pa.row[pidx];
   if (mt >= MT) return;
 #pragma unroll
@@ -357,52 +418,72 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     const int64_t ocol = xa.col[c0 + c] + m;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int munu = 16 * mt + lk + 4 * r;
+      const int munu = 16 * mt + Op::row(lk, r);
       if (munu < NAB) out[(row0 + munu) * ldo + ocol] = acc[t][r];
     }
   }
 }
 
-struct Int3MLaunch {
-  void (*fn)(DevShells, DevShells, PairArgs, AuxArgs, const double*, const int64_t*, const SphTable*, const double*,
-             double*, int64_t, hipStream_t);
-  int nc;
+template <typename T>
+struct Int3MFn {
+  typedef void (*type)(DevShells, DevShells, PairArgs, AuxArgs, const T*, const int64_t*, const SphTable*,
+                       const double*, double*, int64_t, hipStream_t);
 };
 
-template <int LA, int LB, int LC>
-void launch_int3c2e_mm(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa, const double* EtG, const int64_t* etoff,
+struct Int3MLaunch {
+  Int3MFn<double>::type fn;
+  int nc;
+  Int3MFn<float>::type fn32;  // FP32 window of the mixed path
+  int nc32;
+};
+
+template <int LA, int LB, int LC, typename T>
+void launch_int3c2e_mm(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa, const T* EtG, const int64_t* etoff,
                        const SphTable* sph, const double* boys, double* out, int64_t ldo, hipStream_t st) {
-  using K = Cls3m<LA, LB, LC>;
+  using K = Cls3m<LA, LB, LC, T>;
   const int nchunks = (xa.hi - xa.lo + K::NC - 1) / K::NC;
   if (pa.npairs <= 0 || nchunks <= 0) return;
-  hipLaunchKernelGGL((int3c2e_mm_kernel<LA, LB, LC>), dim3(8 * nchunks, (pa.npairs + 7) / 8), dim3(K::NT), 0, st, bra,
-                     ket, pa, xa, EtG, etoff, sph, boys, out, ldo);
+  hipLaunchKernelGGL((int3c2e_mm_kernel<LA, LB, LC, T>), dim3(8 * nchunks, (pa.npairs + 7) / 8), dim3(K::NT), 0,
+                     st, bra, ket, pa, xa, EtG, etoff, sph, boys, out, ldo);
 }
 
 template <int LA, int LB, int LC>
 Int3MLaunch make_int3m() {
-  if constexpr (Cls3m<LA, LB, LC>::OK)
-    return Int3MLaunch{&launch_int3c2e_mm<LA, LB, LC>, Cls3m<LA, LB, LC>::NC};
-  else
-    return Int3MLaunch{nullptr, 0};
+  Int3MLaunch r{nullptr, 0, nullptr, 0};
+  if constexpr (Cls3m<LA, LB, LC, double>::OK) {
+    r.fn = &launch_int3c2e_mm<LA, LB, LC, double>;
+    r.nc = Cls3m<LA, LB, LC, double>::NC;
+  }
+  if constexpr (Cls3m<LA, LB, LC, float>::OK) {
+    r.fn32 = &launch_int3c2e_mm<LA, LB, LC, float>;
+    r.nc32 = Cls3m<LA, LB, LC, float>::NC;
+  }
+  return r;
 }
 
-struct EtLaunch {
-  void (*fn)(DevShells, const int*, const int*, const int*, const int64_t*, int, const SphTable*, double*,
-             hipStream_t);
-  int64_t etpp;  // doubles per primitive pair
+template <typename TO>
+struct EtFn {
+  typedef void (*type)(DevShells, const int*, const int*, const int*, const int64_t*, int, const SphTable*, TO*,
+                       hipStream_t);
 };
 
-template <int LA, int LB>
+struct EtLaunch {
+  EtFn<double>::type fn;
+  EtFn<float>::type fn32;
+  int64_t etpp;  // elements per primitive pair
+};
+
+template <int LA, int LB, typename TO>
 void launch_et(DevShells bra, const int* s1, const int* s2, const int* list, const int64_t* off, int nlist,
-               const SphTable* sph, double* EtG, hipStream_t st) {
+               const SphTable* sph, TO* EtG, hipStream_t st) {
   if (nlist <= 0) return;
-  hipLaunchKernelGGL((int3c2e_et_kernel<LA, LB>), dim3(nlist), dim3(256), 0, st, bra, s1, s2, list, off, sph, EtG);
+  hipLaunchKernelGGL((int3c2e_et_kernel<LA, LB, TO>), dim3(nlist), dim3(256), 0, st, bra, s1, s2, list, off, sph,
+                     EtG);
 }
 
 template <int LA, int LB>
 EtLaunch make_et() {
-  return EtLaunch{&launch_et<LA, LB>, Cls3m<LA, LB, 0>::ETPP};
+  return EtLaunch{&launch_et<LA, LB, double>, &launch_et<LA, LB, float>, Cls3m<LA, LB, 0>::ETPP};
 }
 
 using Int3MTable = Int3MLaunch[LMAX_ORB + 1][LMAX_ORB + 1][LMAX_AUX + 1];
