@@ -2200,9 +2200,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             xa.thresh = opt.thresh;
           }
           const bool mmk = use_mm[k][L] && etg[k].p;
+          const Int3MLaunch& mml = c.int3m[la][lb][L];
           auto launch_mm = [&](const PairArgs& q) {
-            c.int3m[la][lb][L].fn(dorb.view, daux.view, q, xa, etg[k].p, etoff[k].p, c.sph.p, c.boys.p, Tp - cA, w,
-                                  st);
+            mml.fn(dorb.view, daux.view, q, xa, etg[k].p, etoff[k].p, c.sph.p, c.boys.p, Tp - cA, w, st);
+            HC(hipGetLastError());
+          };
+          auto launch_mm32 = [&](const PairArgs& q) {
+            mml.fn32(dorb.view, daux.view, q, xa, etg32[k].p, etoff32[k].p, c.sph.p, c.boys.p, Tp - cA, w, st);
             HC(hipGetLastError());
           };
           if (mixed && lowp[k][L]) {
@@ -2214,9 +2218,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             if (lowp[k][L] == 1) {
               launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
             } else {
-              c.int3m[la][lb][L].fn32(dorb.view, daux.view, p32, xa, etg32[k].p, etoff32[k].p, c.sph.p, c.boys.p,
-                                      Tp - cA, w, st);
-              HC(hipGetLastError());
+              launch_mm32(p32);
             }
           } else if (mmk) {
             launch_mm(pa);

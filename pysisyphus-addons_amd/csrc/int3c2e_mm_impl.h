@@ -75,9 +75,16 @@ struct Cls3m {
   static constexpr int nc_fit(int nc) {
     return (int)((nc * (NRP + NRT) + HK * ((nc * NCC + 15) / 16 * 16)) * sizeof(T) + nc * NPG * GWP * 8) <= BUD;
   }
+  // accumulator tiles per wave at nc aux shells (at most 4: the FP64
+  // accumulators of more tiles, next to the A fragments, exceed the
+  // registers of 2 waves per SIMD)
+  static constexpr int npw_of(int nc) {
+    const int ntl = (nc * NCC + 15) / 16, grp = 8 / MT >= 1 ? 8 / MT : 1;
+    return (ntl + grp - 1) / grp;
+  }
   static constexpr int nc_max() {
     int nc = 32;
-    while (nc > 1 && !nc_fit(nc)) nc--;
+    while (nc > 1 && (!nc_fit(nc) || npw_of(nc) > 4)) nc--;
     return nc;
   }
   static constexpr int NC = nc_max();
@@ -93,6 +100,11 @@ struct Cls3m {
   static constexpr int CG = NT / (H * NCC) >= 1 ? NT / (H * NCC) : 1;
   static constexpr int LH = L / 2 + 1;
   static constexpr int64_t ETPP = (int64_t)HK * NABP;  // Et table doubles per primitive pair
+  // independent entries per thread in the R levels and the Hobson step, and
+  // where the A fragments are loaded (measured per precision on the 200-atom
+  // g/h workload: FP64 4 and late, FP32 2 and early)
+  static constexpr int UR = sizeof(T) == sizeof(double) ? 4 : 2;
+  static constexpr bool ALATE = sizeof(T) == sizeof(double);
   static constexpr bool OK = nc_fit(NC);
 };
 
@@ -308,6 +320,18 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     double P[3];
     for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
     const T* Et = Et0 + (int64_t)ij * K::ETPP;
+    // FP32: the A fragments of this primitive pair are requested before the R
+    // and Hobson steps, whose barriers cover their memory latency (FP64: right
+    // before the product, which keeps the R steps' registers free for their
+    // four independent entries per thread)
+    T av[KS];
+    if constexpr (!K::ALATE) {
+      if (mt < MT) {
+        const T* ea = Et + lk * NABP + 16 * mt + lr;
+#pragma unroll
+        for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
+      }
+    }
     for (int kp = 0; kp < xa.kmax; kp++) {
       __syncthreads();  // tables built (first pass); previous product done with W
       const double* gb = pre ? G + ij * NC * GWP : G;
@@ -329,7 +353,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
         const int so = (n % 2 == 0) ? NRT : NRP, sn = (n % 2 == 0) ? NRP : NRT;
         // level 0: only the orders >= LC that the Hobson step reads
         const int hbeg = n == 0 ? K::NRLO : 0, nent = nherm(L - n) - hbeg;
-#pragma unroll 2
+#pragma unroll(K::UR)
         for (int k = tid; k < nent * NC; k += NT) {
           const int h = hbeg + k / NC, c = k - (h - hbeg) * NC;  // compile-time divisor
           if (c >= nc) continue;
@@ -364,6 +388,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
           cq[z] = z < nq ? (T)tm.c : (T)0;
           iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
         }
+#pragma unroll(K::UR)
         for (int c = cg; c < nc; c += K::CG) {
           const T* Rc = R + c * NRP;
           T s = (T)0;
@@ -375,34 +400,32 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
       __syncthreads();
       // acc[nt] += Et[16 mt.., h] W[h, 16 nt..]: FP64 accumulates in place,
       // FP32 per primitive pair then into the FP64 accumulators
-      // (k step outer: one A fragment in flight, consecutive products on
-      // different accumulators)
+      // all A fragments of the wave's row tile in flight at once (one memory
+      // latency per primitive pair), then one accumulation chain per column
+      // tile; FP32 chains start from zero and are added into FP64
       if (mt < MT) {
-        const T* ea = Et + lk * NABP + 16 * mt + lr;
-        const T* wb = W + lk * NCOLP + 16 * gi + lr;
-        if constexpr (sizeof(T) == sizeof(double)) {
-#pragma unroll 2
-          for (int s = 0; s < KS; s++) {
-            const double av = ea[4 * s * NABP];
+        if constexpr (K::ALATE) {
+          const T* ea = Et + lk * NABP + 16 * mt + lr;
 #pragma unroll
-            for (int t = 0; t < NPW; t++)
-              if (gi + GRP * t < NTL) acc[t] = Op::mfma(av, wb[4 * s * NCOLP + 16 * GRP * t], acc[t]);
+          for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
+        }
+#pragma unroll
+        for (int t = 0; t < NPW; t++) {
+          const int nt = gi + GRP * t;
+          if (nt >= NTL) break;
+          const T* wb = W + lk * NCOLP + 16 * nt + lr;
+          if constexpr (sizeof(T) == sizeof(double)) {
+            dbl4m c4 = acc[t];
+#pragma unroll
+            for (int s = 0; s < KS; s++) c4 = Op::mfma(av[s], wb[4 * s * NCOLP], c4);
+            acc[t] = c4;
+          } else {
+            flt4m c4 = flt4m{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < KS; s++) c4 = Op::mfma(av[s], wb[4 * s * NCOLP], c4);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[t][r] += (double)c4[r];
           }
-        } else {
-          flt4m c4[NPW];
-#pragma unroll
-          for (int t = 0; t < NPW; t++) c4[t] = flt4m{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 2
-          for (int s = 0; s < KS; s++) {
-            const float av = ea[4 * s * NABP];
-#pragma unroll
-            for (int t = 0; t < NPW; t++)
-              if (gi + GRP * t < NTL) c4[t] = Op::mfma(av, wb[4 * s * NCOLP + 16 * GRP * t], c4[t]);
-          }
-#pragma unroll
-          for (int t = 0; t < NPW; t++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc[t][r] += (double)c4[t][r];
         }
       }
     }
