@@ -67,6 +67,34 @@ void f_int_schwarz_aux(int32_t nshells, int32_t ndata, const int32_t *bas_center
 void f_int_schwarz_bra(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
                        const int32_t *bas_spec, const double *bas_data, double *integrals);
 
+/* df1c_tensor (column-major nbfs x nbfs_aux) = D1 L^-T with D1(mu, Q) the
+ * diagonal bra elements (mu mu|Q) of the Schwarz-screened (aa|C) blocks and L
+ * the Cholesky factor of (P|Q).
+ * Replaces f_intor_df1c, std/intor_wrapper.f90:40-62 (std/intor.f90:15-112). */
+void f_intor_df1c(int32_t nshells, int32_t ndata, const int32_t *bas_centers, const int32_t *bas_spec,
+                  const double *bas_data, int32_t nshells_aux, int32_t ndata_aux,
+                  const int32_t *bas_centers_aux, const int32_t *bas_spec_aux, const double *bas_data_aux,
+                  int32_t nbfs, int32_t nbfs_aux, double *df1c_tensor);
+
+/* eri2c_tensor (column-major nbfs x nbfs) = df1c df1c^T, the density-fitted
+ * (aa|bb).  Replaces f_intor_eri2c, std/intor_wrapper.f90:12-38
+ * (std/intor.f90:114-131). */
+void f_intor_eri2c(int32_t nshells, int32_t ndata, const int32_t *bas_centers, const int32_t *bas_spec,
+                   const double *bas_data, int32_t nshells_aux, int32_t ndata_aux,
+                   const int32_t *bas_centers_aux, const int32_t *bas_spec_aux, const double *bas_data_aux,
+                   int32_t nbfs, double *eri2c_tensor);
+
+/* df2c_mo_tensor (column-major nmos(nmos+1)/2 x nbfs_aux): row i(i+1)/2 + a
+ * (i >= a, 0-based) = sum_Q [sum_{mu nu} C(mu, i) C(nu, a) (mu nu|Q)] L^-T(Q, P),
+ * mo_tensor column-major (nbfs, nmos).  The reference's extra drops of
+ * single values <= 1e-10 (std/intor.f90:244-266) are not applied.
+ * Replaces f_intor_df2c_mo, std/intor_wrapper.f90:64-88 (std/intor.f90:133-292). */
+void f_intor_df2c_mo(int32_t nshells, int32_t ndata, const int32_t *bas_centers, const int32_t *bas_spec,
+                     const double *bas_data, int32_t nshells_aux, int32_t ndata_aux,
+                     const int32_t *bas_centers_aux, const int32_t *bas_spec_aux,
+                     const double *bas_data_aux, int32_t nbfs, int32_t nmos, const double *mo_tensor,
+                     int32_t nbfs_aux, double *df2c_mo_tensor);
+
 /* ---- status ------------------------------------------------------------ */
 
 enum {

@@ -293,3 +293,69 @@ def contract_coulomb_densities_4d(shells, shells_aux, densities, thresh=SCHWARZ_
         info["B"] = B
         return J, info
     return J
+
+
+def dense3c(bas, aux, qbra=None, qaux=None, thresh=SCHWARZ_THRESH):
+    """Schwarz-screened (mu nu|P) as a dense (nbf, nbf, naux) array, both
+    (mu, nu) orders filled, screened-out blocks zero (the triplet loop and
+    skip rule of std/intor.f90:202-230)."""
+    if qbra is None:
+        qbra = schwarz_bra(bas)
+    if qaux is None:
+        qaux = schwarz_aux(aux)
+    out = np.zeros((bas.nbf, bas.nbf, aux.nbf))
+    for a in range(bas.nshells):
+        for b in range(a + 1):
+            k = a * (a + 1) // 2 + b
+            for c in range(aux.nshells):
+                if qbra[k] * qaux[c] < thresh:
+                    continue
+                blk = int3c2e_block(bas, aux, a, b, c)
+                sa = slice(bas.off[a], bas.off[a] + blk.shape[0])
+                sb = slice(bas.off[b], bas.off[b] + blk.shape[1])
+                sc = slice(aux.off[c], aux.off[c] + blk.shape[2])
+                out[sa, sb, sc] = blk
+                out[sb, sa, sc] = blk.transpose(1, 0, 2)
+    return out
+
+
+def _linv_t(aux):
+    """(L^-1)^T of the metric's Cholesky factor, from inv_chol's lower triangle
+    (the dtrmm('R','L','T','N') of std/intor.f90:110,274)."""
+    return np.tril(inv_chol(metric(aux))).T
+
+
+def df1c(shells, shells_aux, thresh=SCHWARZ_THRESH):
+    """std/intor.f90:15-112: D1(mu, Q) = (mu mu|Q) from the diagonal of each
+    Schwarz-screened (aa|C) block, then df1c = D1 L^-T.  Returns (df1c, D1)."""
+    bas, aux = Basis.from_shells(shells), Basis.from_shells(shells_aux)
+    qb, qa = schwarz_bra(bas), schwarz_aux(aux)
+    D1 = np.zeros((bas.nbf, aux.nbf))
+    for a in range(bas.nshells):
+        k = a * (a + 1) // 2 + a
+        for c in range(aux.nshells):
+            if qb[k] * qa[c] < thresh:
+                continue
+            blk = int3c2e_block(bas, aux, a, a, c)
+            n = blk.shape[0]
+            D1[bas.off[a]:bas.off[a] + n, aux.off[c]:aux.off[c] + blk.shape[2]] = blk[np.arange(n), np.arange(n)]
+    return D1 @ _linv_t(aux), D1
+
+
+def eri2c(shells, shells_aux, thresh=SCHWARZ_THRESH):
+    """std/intor.f90:114-131: df1c df1c^T."""
+    d, _ = df1c(shells, shells_aux, thresh)
+    return d @ d.T
+
+
+def df2c_mo(shells, shells_aux, mo_tensor, thresh=SCHWARZ_THRESH):
+    """std/intor.f90:133-292 without its drops of single values <= 1e-10:
+    row i(i+1)/2 + a (i >= a) of sum_{mu nu} C(mu, i) C(nu, a) (mu nu|Q),
+    times L^-T.  Returns (df2c_mo, the unpacked (nmo, nmo, naux) MO tensor)."""
+    bas, aux = Basis.from_shells(shells), Basis.from_shells(shells_aux)
+    T = dense3c(bas, aux, thresh=thresh)
+    C = np.asarray(mo_tensor, dtype=np.float64)
+    X = np.einsum("mnQ,mi,na->iaQ", T, C, C, optimize=True)
+    nmo = C.shape[1]
+    rows = np.array([X[i, a] for i in range(nmo) for a in range(i + 1)])
+    return rows @ _linv_t(aux), X

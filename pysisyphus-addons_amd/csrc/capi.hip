@@ -1509,6 +1509,7 @@ struct Outputs {
   double* J = nullptr;          // host, column-major nstates^4
   int nstates = 0;
   double* dense3c = nullptr;    // host, (nbf, nbf, naux) row-major
+  double* gram = nullptr;       // host, column-major ndens x ndens: B^T B
 };
 
 // take_lock = false: the caller holds the library mutex and has set up the
@@ -1528,7 +1529,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.aux = parse_shells(X, LMAX_AUX, "auxiliary basis");
   const HostShells& o = pl.orb;
   const HostShells& x = pl.aux;
-  const bool need_dens = out.df || out.J;
+  const bool need_dens = out.df || out.J || out.gram;
   if (need_dens) {
     if (ndens < 1) throw Error{DFHIP_ERR_ARG, "ndens must be >= 1"};
     if (nbfs != o.nbf)
@@ -2304,10 +2305,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       bt_to_b(dBT.p, naux, ndens, dB.p, st);
       HC(hipGetLastError());
     }
-    if (out.J) {
+    if (out.J || out.gram) {
       dGp.alloc((size_t)gram_chunks(naux, ndens) * ndens * ndens);
       dG.alloc((size_t)ndens * ndens);
       gram_neg(dBT.p, naux, ndens, dGp.p, dG.p, st);
+      HC(hipGetLastError());
+    }
+    if (out.J) {
       const size_t nj = (size_t)out.nstates * out.nstates * out.nstates * out.nstates;
       dJ.alloc(nj);
       HC(hipMemsetAsync(dJ.p, 0, nj * sizeof(double), st));
@@ -2318,8 +2322,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   if (out.df) HC(hipMemcpyAsync(out.df, dB.p, (size_t)naux * ndens * sizeof(double), hipMemcpyDeviceToHost, st));
   if (out.J) HC(hipMemcpyAsync(out.J, dJ.p, dJ.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (out.gram)
+    HC(hipMemcpyAsync(out.gram, dG.p, (size_t)ndens * ndens * sizeof(double), hipMemcpyDeviceToHost, st));
   const int e9 = tm.mark();
   HC(hipStreamSynchronize(st));
+  if (out.gram)  // the Gram kernels give -B^T B (the Coulomb tensor's sign)
+    for (size_t i = 0; i < (size_t)ndens * ndens; i++) out.gram[i] = -out.gram[i];
   if (need_dens && lane.hinfo[0] != 0)
     throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
                                        std::to_string(lane.hinfo[0]) + ")"};
@@ -2605,6 +2613,80 @@ std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
   return h;
 }
 
+// ---------------------------------------------------------------------------
+// The std/ DF tensors (std/intor.f90:15-292) through the Coulomb path: each
+// is d(P, I) = sum_{mu nu} (mu nu|P) D_I(mu, nu) for a set of model densities,
+// B = L^-1 d, under the reference's Schwarz rule Q_ab Q_C < 1e-10 (the
+// single-precision literal, std/intor.f90:78,223).
+//   df1c(mu, P)  = B(P, mu) with D_mu = e_mu e_mu^T: the diagonal bra
+//                  elements (mu mu|Q) of the (aa|C) blocks (:67-103), times L^-T;
+//   eri2c        = df1c df1c^T = B^T B (:116-130);
+//   df2c_mo(ia, P) = B(P, ia) with D_ia = (C_i C_a^T + C_a C_i^T) / 2 for the
+//                  packed MO pairs i >= a (:176-274): sum_{mu nu} C_mu,i C_nu,a
+//                  (mu nu|Q) for the symmetric (mu nu|Q).
+// The reference also drops single integrals and half-transformed elements of
+// magnitude <= 1e-10 inside df2c_mo (:244-266); this evaluation keeps them.
+
+// D[(mu nbf + mu) nbf + mu] = 1 (D cleared before)
+__global__ void unit_diag_densities_kernel(double* __restrict__ D, int nbf) {
+  const int64_t mu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mu < nbf) D[(mu * nbf + mu) * nbf + mu] = 1.0;
+}
+
+// D[pack][mu][nu] = (C[mu, i] C[nu, a] + C[nu, i] C[mu, a]) / 2, pack = i(i+1)/2 + a,
+// C column-major (nbf, nmo)
+__global__ void mo_pair_densities_kernel(const double* __restrict__ C, int nbf, int nmo, double* __restrict__ D) {
+  const int64_t npack = (int64_t)nmo * (nmo + 1) / 2, nn = (int64_t)nbf * nbf, tot = npack * nn;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pk = e / nn, r = e - pk * nn;
+    const int mu = (int)(r / nbf), nu = (int)(r - (int64_t)mu * nbf);
+    int i = (int)((std::sqrt(8.0 * (double)pk + 1.0) - 1.0) * 0.5);
+    while ((int64_t)i * (i + 1) / 2 > pk) i--;
+    while ((int64_t)(i + 1) * (i + 2) / 2 <= pk) i++;
+    const int a = (int)(pk - (int64_t)i * (i + 1) / 2);
+    D[e] = 0.5 * (C[mu + (int64_t)nbf * i] * C[nu + (int64_t)nbf * a] + C[nu + (int64_t)nbf * i] * C[mu + (int64_t)nbf * a]);
+  }
+}
+
+int basis_nsph(const dfhip_basis& b) {
+  int n = 0;
+  for (int i = 0; i < b.nshells; i++) n += 2 * b.bas_spec[i + b.nshells] + 1;
+  return n;
+}
+
+// The model densities of kind 0 (df1c / eri2c) or 1 (df2c_mo) on the device,
+// then the Coulomb path into out (df and/or gram).
+void std_tensors(const dfhip_basis& b, const dfhip_basis& x, int kind, int nmo, const double* C, Outputs out) {
+  if (b.nshells < 1 || x.nshells < 1 || !b.bas_spec || !x.bas_spec)
+    throw Error{DFHIP_ERR_ARG, "empty basis"};
+  const int nbf = basis_nsph(b);
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  const hipStream_t st = cur_lane().st;
+  cur_lane().arena.reset();
+  int ndens;
+  DBuf<double> D, dC;
+  if (kind == 0) {
+    ndens = nbf;
+    D.alloc((size_t)nbf * nbf * nbf);
+    HC(hipMemsetAsync(D.p, 0, D.n * sizeof(double), st));
+    hipLaunchKernelGGL(unit_diag_densities_kernel, dim3((nbf + 255) / 256), dim3(256), 0, st, D.p, nbf);
+  } else {
+    if (nmo < 1 || !C) throw Error{DFHIP_ERR_ARG, "mo_tensor is empty"};
+    ndens = nmo * (nmo + 1) / 2;
+    std::vector<double> hc(C, C + (size_t)nbf * nmo);
+    upload(dC, hc, st);
+    D.alloc((size_t)ndens * nbf * nbf);
+    hipLaunchKernelGGL(mo_pair_densities_kernel, dim3(2048), dim3(256), 0, st, dC.p, nbf, nmo, D.p);
+  }
+  HC(hipGetLastError());
+  HC(hipStreamSynchronize(st));
+  dfhip_options o = defaults();
+  o.dens_layout = DFHIP_DENS_C;
+  o.dens_on_device = 1;
+  run(b, x, ndens, nbf, D.p, o, out, nullptr, false);
+}
+
 }  // namespace
 }  // namespace dfhip
 
@@ -2810,6 +2892,59 @@ void f_contract_coulomb_densities_4d(int32_t nshells, int32_t ndata, const int32
     out.J = coulomb_tensor;
     out.nstates = nstates;
     run(b, x, ndens, nbfs, densities, o, out, nullptr);
+  });
+}
+
+void f_intor_df1c(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                  const double* bas_data, int32_t nshells_aux, int32_t ndata_aux, const int32_t* bas_centers_aux,
+                  const int32_t* bas_spec_aux, const double* bas_data_aux, int32_t nbfs, int32_t nbfs_aux,
+                  double* df1c_tensor) {
+  guarded([&] {
+    const dfhip_basis b = mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data);
+    const dfhip_basis x = mkbasis(nshells_aux, ndata_aux, bas_centers_aux, bas_spec_aux, bas_data_aux);
+    if (!df1c_tensor) throw Error{DFHIP_ERR_ARG, "df1c_tensor is NULL"};
+    if (nbfs != basis_nsph(b) || nbfs_aux != basis_nsph(x))
+      throw Error{DFHIP_ERR_ARG, "nbfs / nbfs_aux do not match the spherical functions of the bases"};
+    std::vector<double> B((size_t)nbfs_aux * nbfs);
+    Outputs out;
+    out.df = B.data();
+    std_tensors(b, x, 0, 0, nullptr, out);
+    for (int64_t mu = 0; mu < nbfs; mu++)
+      for (int64_t P = 0; P < nbfs_aux; P++) df1c_tensor[mu + (int64_t)nbfs * P] = B[P + (int64_t)nbfs_aux * mu];
+  });
+}
+
+void f_intor_eri2c(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                   const double* bas_data, int32_t nshells_aux, int32_t ndata_aux, const int32_t* bas_centers_aux,
+                   const int32_t* bas_spec_aux, const double* bas_data_aux, int32_t nbfs, double* eri2c_tensor) {
+  guarded([&] {
+    const dfhip_basis b = mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data);
+    const dfhip_basis x = mkbasis(nshells_aux, ndata_aux, bas_centers_aux, bas_spec_aux, bas_data_aux);
+    if (!eri2c_tensor) throw Error{DFHIP_ERR_ARG, "eri2c_tensor is NULL"};
+    if (nbfs != basis_nsph(b)) throw Error{DFHIP_ERR_ARG, "nbfs does not match the spherical functions of the basis"};
+    Outputs out;
+    out.gram = eri2c_tensor;
+    std_tensors(b, x, 0, 0, nullptr, out);
+  });
+}
+
+void f_intor_df2c_mo(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                     const double* bas_data, int32_t nshells_aux, int32_t ndata_aux, const int32_t* bas_centers_aux,
+                     const int32_t* bas_spec_aux, const double* bas_data_aux, int32_t nbfs, int32_t nmos,
+                     const double* mo_tensor, int32_t nbfs_aux, double* df2c_mo_tensor) {
+  guarded([&] {
+    const dfhip_basis b = mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data);
+    const dfhip_basis x = mkbasis(nshells_aux, ndata_aux, bas_centers_aux, bas_spec_aux, bas_data_aux);
+    if (!df2c_mo_tensor || !mo_tensor) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    if (nbfs != basis_nsph(b) || nbfs_aux != basis_nsph(x))
+      throw Error{DFHIP_ERR_ARG, "nbfs / nbfs_aux do not match the spherical functions of the bases"};
+    const int64_t npack = (int64_t)nmos * (nmos + 1) / 2;
+    std::vector<double> B((size_t)nbfs_aux * npack);
+    Outputs out;
+    out.df = B.data();
+    std_tensors(b, x, 1, nmos, mo_tensor, out);
+    for (int64_t pk = 0; pk < npack; pk++)
+      for (int64_t P = 0; P < nbfs_aux; P++) df2c_mo_tensor[pk + npack * P] = B[P + (int64_t)nbfs_aux * pk];
   });
 }
 

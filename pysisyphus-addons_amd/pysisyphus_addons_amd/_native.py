@@ -78,7 +78,7 @@ EXPORTED = (
     "f_int_schwarz_aux", "f_int_schwarz_bra", "dfhip_last_status", "dfhip_last_error",
     "dfhip_default_options", "dfhip_contract_coulomb_densities", "dfhip_int3c2e_dense",
     "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
-    "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc",
+    "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc", "f_intor_df1c", "f_intor_eri2c", "f_intor_df2c_mo",
 )
 
 
@@ -115,6 +115,9 @@ def lib():
         args13 = [i32, i32, vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, vp]
         L.f_contract_coulomb_densities_2d.argtypes = args13 + [i32, vp]
         L.f_contract_coulomb_densities_4d.argtypes = args13 + [i32, vp]
+        L.f_intor_df1c.argtypes = args13[:10] + [i32, i32, vp]
+        L.f_intor_eri2c.argtypes = args13[:10] + [i32, vp]
+        L.f_intor_df2c_mo.argtypes = args13[:10] + [i32, i32, vp, i32, vp]
         L.f_int_schwarz_aux.argtypes = [i32, i32, vp, vp, vp, vp]
         L.f_int_schwarz_bra.argtypes = [i32, i32, vp, vp, vp, vp]
         _lib = L

@@ -300,3 +300,60 @@ def test_bena2_coulomb_tensor_golden(bena2):
     J = O.contract_coulomb_densities_4d(shells, shells, bena2["densities"])
     assert J.shape == (2, 2, 2, 2)
     np.testing.assert_allclose(J, bena2["J_ref"], atol=1e-10)
+
+
+def _std_system():
+    rng = np.random.default_rng(31)
+    orb, aux = [], []
+    for ci, C in enumerate(rng.uniform(-1.5, 1.5, (3, 3))):
+        for L in (0, 1, 2):
+            e = 0.4 * 3.0 ** np.arange(2) * rng.uniform(0.5, 2.0)
+            orb.append(Shell(L=L, center=C, exps=e, coeffs=rng.uniform(0.2, 1.0, 2), center_ind=ci))
+        for L in (0, 1, 2, 3):
+            aux.append(Shell(L=L, center=C, exps=[rng.uniform(0.3, 3.0)], coeffs=[1.0], center_ind=ci))
+    return Shells(orb), Shells(aux)
+
+
+def test_std_df1c_eri2c_definitions():
+    """tests/test_std/test_std.py:13-57: the DF (aa|bb) equals
+    int3c_flat pinv((P|Q)) int3c_flat^T, int3c_flat the diagonal bra elements
+    of (aa|P) -- an independent route (pseudo-inverse instead of the Cholesky
+    factor); and df1c L^T gives back those diagonal elements."""
+    sh, ax = _std_system()
+    d, D1 = O.df1c(sh, ax)
+    M = O.metric(O.Basis.from_shells(ax))
+    ref = D1 @ np.linalg.pinv(M) @ D1.T
+    np.testing.assert_allclose(O.eri2c(sh, ax), ref, atol=1e-11)
+    np.testing.assert_allclose(d @ np.linalg.cholesky(M).T, D1, atol=1e-12)
+    # D1 really is the diagonal of the shell-diagonal (aa|P) blocks
+    T = O.dense3c(O.Basis.from_shells(sh), O.Basis.from_shells(ax))
+    bas = O.Basis.from_shells(sh)
+    for a in range(bas.nshells):
+        n = 2 * bas.L[a] + 1
+        for i in range(n):
+            mu = bas.off[a] + i
+            np.testing.assert_array_equal(D1[mu], T[mu, mu])
+
+
+def test_std_df2c_mo_definitions():
+    """tests/test_std/test_std.py:60-113: with B the MO-transformed integrals
+    times L^-T, B B^T equals int3c_mo pinv((P|Q)) int3c_mo^T, and row
+    i(i+1)/2 + a of df2c_mo is B[i, a]."""
+    sh, ax = _std_system()
+    rng = np.random.default_rng(5)
+    C = np.linalg.qr(rng.standard_normal((sh.nbf, sh.nbf)))[0][:, :6]
+    d2, X = O.df2c_mo(sh, ax, C)
+    M = O.metric(O.Basis.from_shells(ax))
+    L = np.linalg.cholesky(M)
+    B = np.einsum("iaQ,PQ->iaP", X, np.linalg.inv(L))
+    eri_mo = np.einsum("ijP,klP->ijkl", B, B)
+    eri_ref = np.einsum("ijP,PQ,klQ->ijkl", X, np.linalg.pinv(M), X)
+    np.testing.assert_allclose(eri_mo, eri_ref, atol=1e-10)
+    for i in range(6):
+        for a in range(i + 1):
+            np.testing.assert_allclose(d2[i * (i + 1) // 2 + a], B[i, a], atol=1e-11)
+    # the MO transform itself against a direct double loop for one element
+    bas, aux = O.Basis.from_shells(sh), O.Basis.from_shells(ax)
+    T = O.dense3c(bas, aux)
+    assert X[3, 1, 7] == pytest.approx(sum(C[m, 3] * C[n, 1] * T[m, n, 7] for m in range(sh.nbf)
+                                           for n in range(sh.nbf)), rel=1e-12)
