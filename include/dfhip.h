@@ -95,6 +95,19 @@ void f_intor_df2c_mo(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
                      const double *bas_data_aux, int32_t nbfs, int32_t nmos, const double *mo_tensor,
                      int32_t nbfs_aux, double *df2c_mo_tensor);
 
+/* Screening-estimate records of all triplets of the pairs a >= b that are
+ * neither concentric nor small (Q_ab <= 1e-10), in the reference's loop order,
+ * written to int_estimates.bin in the working directory as 4 float64 per
+ * triplet: Schwarz estimate, QVL estimate, ||(ab|C)||_F, |contracted centre - C|;
+ * the reference's log lines go to stdout.  The metric and its inverse square
+ * root are computed (and timed) as in the reference.
+ * Replaces f_benchmark_int3c2e, intor/benchmark_wrapper.f90:12-38
+ * (intor/benchmark.f90:15-132). */
+void f_benchmark_int3c2e(int32_t nshells, int32_t ndata, const int32_t *bas_centers,
+                         const int32_t *bas_spec, const double *bas_data, int32_t nshells_aux,
+                         int32_t ndata_aux, const int32_t *bas_centers_aux, const int32_t *bas_spec_aux,
+                         const double *bas_data_aux);
+
 /* ---- status ------------------------------------------------------------ */
 
 enum {
@@ -225,6 +238,15 @@ int dfhip_qvl_records(const dfhip_basis *bas, const dfhip_basis *aux, double *pa
  * Calls in one process are serialized by the library; the array keeps the last
  * call's contents. */
 int dfhip_int_schwarz_aux_noalloc(const dfhip_basis *aux, double *integrals);
+
+/* (P|Q)^power, naux x naux, by the eigen route of matrix_powerh
+ * (common/linalg.f90:36-58): rocSOLVER dsyevd, then V diag(lambda^power) V^T. */
+int dfhip_metric_powerh(const dfhip_basis *aux, double power, double *out);
+
+/* The records of f_benchmark_int3c2e in memory: records (4 float64 each) may be
+ * NULL to count them; *nrecords receives the count. */
+int dfhip_int3c2e_estimates(const dfhip_basis *bas, const dfhip_basis *aux, double *records,
+                            int64_t capacity, int64_t *nrecords);
 
 /* Metric (P|Q), naux x naux (symmetric). */
 int dfhip_metric(const dfhip_basis *aux, double *out);

@@ -359,3 +359,30 @@ def df2c_mo(shells, shells_aux, mo_tensor, thresh=SCHWARZ_THRESH):
     nmo = C.shape[1]
     rows = np.array([X[i, a] for i in range(nmo) for a in range(i + 1)])
     return rows @ _linv_t(aux), X
+
+
+def int3c2e_estimates(shells, shells_aux):
+    """intor/benchmark.f90:15-132 restated: for a >= b (the reference's
+    1-based loops a = 1..n, b = 1..a), pairs neither concentric (|A - B| <=
+    1e-6) nor small (Q_ab <= 1e-10), every aux shell c: [Q_ab Q_C, QVL
+    estimate, ||(ab|C)||_F, |contracted centre - C|] (screening.f90:315-326).
+    Returns (records (n, 4), nskipped with QVL estimate < the real(4) 1e-10)."""
+    bas, aux = Basis.from_shells(shells), Basis.from_shells(shells_aux)
+    qb, qa = schwarz_bra(bas), schwarz_aux(aux)
+    pr, xr = qvl_records(bas, aux, qb, qa)
+    recs, nskip = [], 0
+    for a in range(bas.nshells):
+        A = np.asarray(shells.shells[a].center, dtype=float)
+        for b in range(a + 1):
+            B = np.asarray(shells.shells[b].center, dtype=float)
+            k = a * (a + 1) // 2 + b
+            if np.linalg.norm(A - B) <= 1e-6 or qb[k] <= 1e-10:
+                continue
+            for c in range(aux.nshells):
+                C = np.ascontiguousarray(shells_aux.shells[c].center, dtype=float)
+                q = lib().orc_qvl_estimate(np.ascontiguousarray(pr[k]), np.ascontiguousarray(xr[c]), C,
+                                           int(aux.L[c]))
+                nskip += q < SCHWARZ_THRESH
+                recs.append([qb[k] * qa[c], q, np.linalg.norm(int3c2e_block(bas, aux, a, b, c)),
+                             np.linalg.norm(pr[k, :3] - C)])
+    return np.array(recs).reshape(-1, 4), nskip

@@ -7,6 +7,8 @@
 // parses the basis, sorts shell pairs / auxiliary shells by their Schwarz
 // norms and derives the screened prefix lengths.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -403,6 +405,7 @@ struct Ctx {
   Schwarz4MTable s4m{};
   Schwarz4STable s4s{};
   Gen3Table gen3{};
+  rocblas_handle blas = nullptr;  // rocBLAS / rocSOLVER handle of the eigen route (created on first use)
 };
 
 template <class T>
@@ -1510,7 +1513,30 @@ struct Outputs {
   int nstates = 0;
   double* dense3c = nullptr;    // host, (nbf, nbf, naux) row-major
   double* gram = nullptr;       // host, column-major ndens x ndens: B^T B
+  double* blocknorm = nullptr;  // host, [a(a+1)/2 + b][aux shell c]: ||(ab|C)||_F of the computed blocks
 };
+
+// Frobenius norm of every (pair, aux shell) block of one column block of the
+// staging matrix (rows of pair p from prow[p], the block's aux shells at
+// columns bcol[q] - c0): norms[p nshx + bsh[q]].  One thread per block.
+__global__ void block_norm_kernel(const double* __restrict__ T, int64_t ldt, int64_t c0,
+                                  const int64_t* __restrict__ prow, const int* __restrict__ s1,
+                                  const int* __restrict__ s2, const int* __restrict__ Lorb, int npairs,
+                                  const int* __restrict__ bsh, const int64_t* __restrict__ bcol,
+                                  const int* __restrict__ bL, int nb, int nshx, double* __restrict__ norms) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)npairs * nb) return;
+  const int p = (int)(e / nb), q = (int)(e - (int64_t)p * nb);
+  const int nab = (2 * Lorb[s1[p]] + 1) * (2 * Lorb[s2[p]] + 1), nc = 2 * bL[q] + 1;
+  const double* t = T + prow[p] * ldt + (bcol[q] - c0);
+  double s = 0.0;
+  for (int r = 0; r < nab; r++)
+    for (int m = 0; m < nc; m++) {
+      const double v = t[(int64_t)r * ldt + m];
+      s += v * v;
+    }
+  norms[(int64_t)p * nshx + bsh[q]] = sqrt(s);
+}
 
 // take_lock = false: the caller holds the library mutex and has set up the
 // context (a batch worker on its own lane).
@@ -1530,6 +1556,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   const HostShells& o = pl.orb;
   const HostShells& x = pl.aux;
   const bool need_dens = out.df || out.J || out.gram;
+  if (out.blocknorm && opt.screening != DFHIP_SCREEN_NONE)
+    throw Error{DFHIP_ERR_ARG, "block norms need the unscreened integrals"};
   if (need_dens) {
     if (ndens < 1) throw Error{DFHIP_ERR_ARG, "ndens must be >= 1"};
     if (nbfs != o.nbf)
@@ -2178,6 +2206,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   double ms_int = 0.0, ms_con = 0.0;
   std::vector<double> dense_stage;
   vstep("integral set-up");
+  DBuf<double> dnorm;  // block norms, [sorted pair][aux shell]
+  if (out.blocknorm) dnorm.alloc((size_t)std::max(pl.ps.n(), 1) * x.n);
   for (int blk = 0; blk < nblocks; blk++) {
     const int64_t cA = bounds[blk], cB = bounds[blk + 1], w = cB - cA;
     // only the dense dump reads screened-out entries; the contraction masks them
@@ -2240,6 +2270,29 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       }
     }
     const int b1 = tm.mark();
+    if (out.blocknorm) {
+      std::vector<int> bsh, bL;
+      std::vector<int64_t> bcol;
+      for (int L = 0; L <= LMAX_AUX; L++)
+        for (size_t i = 0; i < pl.acol[L].size(); i++)
+          if (pl.acol[L][i] >= cA && pl.acol[L][i] < cB) {
+            bsh.push_back(pl.alist[L][i]);
+            bcol.push_back(pl.acol[L][i]);
+            bL.push_back(L);
+          }
+      DBuf<int> dbsh, dbL;
+      DBuf<int64_t> dbcol;
+      upload(dbsh, bsh, st);
+      upload(dbL, bL, st);
+      upload(dbcol, bcol, st);
+      const int64_t nt = (int64_t)pl.ps.n() * (int64_t)bsh.size();
+      if (nt > 0)
+        hipLaunchKernelGGL(block_norm_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, Tp, w, cA,
+                           prow.p, ps1.p, ps2.p, dorb.L.p, pl.ps.n(), dbsh.p, dbcol.p, dbL.p, (int)bsh.size(), x.n,
+                           dnorm.p);
+      HC(hipGetLastError());
+      HC(hipStreamSynchronize(st));  // the small uploads above go back to the pool
+    }
     if (need_dens) {
       // the packed densities come from the side stream, and d^T's rows were
       // cleared there before the metric
@@ -2328,6 +2381,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   HC(hipStreamSynchronize(st));
   if (out.gram)  // the Gram kernels give -B^T B (the Coulomb tensor's sign)
     for (size_t i = 0; i < (size_t)ndens * ndens; i++) out.gram[i] = -out.gram[i];
+  if (out.blocknorm) {
+    std::vector<double> hn((size_t)pl.ps.n() * x.n);
+    HC(hipMemcpy(hn.data(), dnorm.p, hn.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int p = 0; p < pl.ps.n(); p++) {
+      const int a = std::max(pl.ps.s1[p], pl.ps.s2[p]), b = std::min(pl.ps.s1[p], pl.ps.s2[p]);
+      std::memcpy(out.blocknorm + ((size_t)a * (a + 1) / 2 + b) * x.n, &hn[(size_t)p * x.n], x.n * sizeof(double));
+    }
+  }
   if (need_dens && lane.hinfo[0] != 0)
     throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
                                        std::to_string(lane.hinfo[0]) + ")"};
@@ -2614,6 +2675,137 @@ std::vector<double> metric_host(const dfhip_basis& b, std::vector<double>* qc) {
 }
 
 // ---------------------------------------------------------------------------
+// (P|Q)^power by the eigen route of common/linalg.f90:36-58 (matrix_powerh):
+// A = V diag(lambda) V^T with rocSOLVER's divide-and-conquer dsyevd (the
+// reference's LAPACK dsyevd, upper triangle), then V diag(lambda^power) V^T
+// with one rocBLAS dgemm.  A (n x n, column-major, device) is overwritten by
+// V; out receives the power.  A negative eigenvalue with a fractional power
+// gives NaN, as in the reference.
+
+__global__ void scale_columns_pow_kernel(const double* __restrict__ V, const double* __restrict__ lam, int n,
+                                         double power, double* __restrict__ W) {
+  const int64_t tot = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x)
+    W[e] = V[e] * pow(lam[e / n], power);
+}
+
+#define RBC(x)                                                                                          \
+  do {                                                                                                  \
+    rocblas_status s_ = (x);                                                                            \
+    if (s_ != rocblas_status_success)                                                                   \
+      throw Error{DFHIP_ERR_HIP, std::string(#x) + ": " + rocblas_status_to_string(s_)};               \
+  } while (0)
+
+void matrix_powerh_dev(Ctx& c, double* A, int n, double power, double* out, hipStream_t st) {
+  if (!c.blas) RBC(rocblas_create_handle(&c.blas));
+  RBC(rocblas_set_stream(c.blas, st));
+  DBuf<double> lam(n), E(n), W((size_t)n * n);
+  DBuf<int> info(1);
+  RBC(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, lam.p, E.p, info.p));
+  hipLaunchKernelGGL(scale_columns_pow_kernel, dim3(1024), dim3(256), 0, st, A, lam.p, n, power, W.p);
+  HC(hipGetLastError());
+  const double one = 1.0, zero = 0.0;
+  RBC(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, W.p, n, A, n, &zero,
+                    out, n));
+  int hinfo = 0;
+  HC(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  if (hinfo != 0) throw Error{DFHIP_ERR_NOT_SPD, "dsyevd did not converge (info " + std::to_string(hinfo) + ")"};
+}
+
+// (P|Q)^power of an aux basis into host out (naux x naux).
+void metric_powerh_host(const dfhip_basis& b, double power, double* out) {
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  Ctx& c = ctx();
+  HostShells x = parse_shells(b, LMAX_AUX, "basis");
+  cur_lane().arena.reset();
+  const hipStream_t st = cur_lane().st;
+  DevShellBufs d;
+  d.upload_from(x, st);
+  const int n = x.nbf;
+  DBuf<double> M((size_t)n * n), P((size_t)n * n);
+  metric_dev(c, x, d.view, M.p, n);
+  matrix_powerh_dev(c, M.p, n, power, P.p, st);
+  HC(hipMemcpyAsync(out, P.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// The screening-estimate records of intor/benchmark.f90:15-132
+// (benchmark_int3c2e): for a = 1..n, b = 1..a (1-based, the reference's
+// loops), pairs that are neither concentric (|A - B| <= 1e-6) nor small
+// (Q_ab <= 1d-10), and every aux shell c: [Schwarz estimate Q_ab Q_C, QVL
+// estimate, ||(ab|C)||_F of the unscreened block, R = |contracted centre - C|]
+// (screening.f90:315-326).  Also the counts and phase times the reference
+// logs.
+struct Estimates {
+  std::vector<double> rec;  // 4 per record
+  int64_t ntriplets = 0, nskipped = 0;
+  int nbf = 0, naux = 0, nshells = 0, nshells_aux = 0;
+  double t_screener = 0, t_metric = 0, t_powerh = 0, t_ints = 0;  // seconds
+};
+
+Estimates int3c2e_estimates(const dfhip_basis& b, const dfhip_basis& xb) {
+  using clk = std::chrono::steady_clock;
+  auto secs = [](clk::time_point a, clk::time_point z) { return std::chrono::duration<double>(z - a).count(); };
+  Estimates es;
+  const HostShells o = parse_shells(b, LMAX_ORB, "orbital basis");
+  const HostShells x = parse_shells(xb, LMAX_AUX, "auxiliary basis");
+  es.nbf = o.nbf;
+  es.naux = x.nbf;
+  es.nshells = o.n;
+  es.nshells_aux = x.n;
+  const size_t npk = (size_t)o.n * (o.n + 1) / 2;
+  auto t0 = clk::now();
+  std::vector<double> pr(8 * npk), xr(4 * (size_t)x.n);
+  qvl_records_host(b, xb, pr.data(), xr.data());
+  auto t1 = clk::now();
+  es.t_screener = secs(t0, t1);
+  {
+    // the metric and its inverse square root (timed, not used further: the
+    // reference computes them the same way)
+    std::vector<double> P((size_t)x.nbf * x.nbf);
+    const auto m0 = clk::now();
+    metric_host(xb, nullptr);
+    const auto m1 = clk::now();
+    metric_powerh_host(xb, -0.5, P.data());
+    es.t_metric = secs(m0, m1);
+    es.t_powerh = secs(m1, clk::now());
+  }
+  std::vector<double> norms(npk * (size_t)x.n, 0.0);
+  const auto i0 = clk::now();
+  {
+    dfhip_options opt = defaults();
+    opt.screening = DFHIP_SCREEN_NONE;
+    Outputs out;
+    out.blocknorm = norms.data();
+    run(b, xb, 0, 0, nullptr, opt, out, nullptr);
+  }
+  es.ntriplets = (int64_t)npk * x.n;
+  const double f32_1e10 = (double)1e-10f;  // intor/benchmark.f90:100, a real(4) literal
+  for (int a = 0; a < o.n; a++)
+    for (int bb = 0; bb <= a; bb++) {
+      const size_t k = (size_t)a * (a + 1) / 2 + bb;
+      const double dx = o.xyz[3 * a] - o.xyz[3 * bb], dy = o.xyz[3 * a + 1] - o.xyz[3 * bb + 1],
+                   dz = o.xyz[3 * a + 2] - o.xyz[3 * bb + 2];
+      if (std::sqrt(dx * dx + dy * dy + dz * dz) <= 1e-6 || pr[8 * k + 5] <= 1e-10) continue;
+      for (int c = 0; c < x.n; c++) {
+        const double* C = &x.xyz[3 * c];
+        const double qvl = qvl_estimate(&pr[8 * k], &xr[4 * c], C[0], C[1], C[2], x.L[c]);
+        const double rx = pr[8 * k] - C[0], ry = pr[8 * k + 1] - C[1], rz = pr[8 * k + 2] - C[2];
+        if (qvl < f32_1e10) es.nskipped++;
+        es.rec.push_back(pr[8 * k + 5] * xr[4 * c + 2]);
+        es.rec.push_back(qvl);
+        es.rec.push_back(norms[k * x.n + c]);
+        es.rec.push_back(std::sqrt(rx * rx + ry * ry + rz * rz));
+      }
+    }
+  es.t_ints = secs(i0, clk::now());
+  return es;
+}
+
+// ---------------------------------------------------------------------------
 // The std/ DF tensors (std/intor.f90:15-292) through the Coulomb path: each
 // is d(P, I) = sum_{mu nu} (mu nu|P) D_I(mu, nu) for a set of model densities,
 // B = L^-1 d, under the reference's Schwarz rule Q_ab Q_C < 1e-10 (the
@@ -2819,6 +3011,54 @@ int dfhip_int_schwarz_aux_noalloc(const dfhip_basis* aux, double* integrals) {
   return guarded([&] {
     if (!aux || !integrals) throw Error{DFHIP_ERR_ARG, "NULL argument"};
     schwarz_aux_noalloc(*aux, integrals);
+  });
+}
+
+int dfhip_metric_powerh(const dfhip_basis* aux, double power, double* out) {
+  return guarded([&] {
+    if (!aux || !out) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    metric_powerh_host(*aux, power, out);
+  });
+}
+
+int dfhip_int3c2e_estimates(const dfhip_basis* bas, const dfhip_basis* aux, double* records, int64_t capacity,
+                            int64_t* nrecords) {
+  return guarded([&] {
+    if (!bas || !aux || !nrecords) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    const Estimates es = int3c2e_estimates(*bas, *aux);
+    const int64_t n = (int64_t)es.rec.size() / 4;
+    *nrecords = n;
+    if (records) {
+      if (capacity < n)
+        throw Error{DFHIP_ERR_ARG, "records holds " + std::to_string(capacity) + " records, " + std::to_string(n) +
+                                       " are needed"};
+      std::memcpy(records, es.rec.data(), es.rec.size() * sizeof(double));
+    }
+  });
+}
+
+void f_benchmark_int3c2e(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                         const double* bas_data, int32_t nshells_aux, int32_t ndata_aux,
+                         const int32_t* bas_centers_aux, const int32_t* bas_spec_aux, const double* bas_data_aux) {
+  guarded([&] {
+    const dfhip_basis b = mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data);
+    const dfhip_basis x = mkbasis(nshells_aux, ndata_aux, bas_centers_aux, bas_spec_aux, bas_data_aux);
+    const Estimates es = int3c2e_estimates(b, x);
+    std::printf("Main basis:%6d shells, %6d basis functions\n", es.nshells, es.nbf);
+    std::printf("Aux. basis:%6d shells, %6d basis functions\n", es.nshells_aux, es.naux);
+    std::printf("Initializing the DF-integral screener took %6.2f s\n", es.t_screener);
+    std::printf("Calculation of density fitting metric (P|Q) took%8.4f s.\n", es.t_metric);
+    std::printf("Calculation of (P|Q)**-0.5 took%8.4f s.\n", es.t_powerh);
+    std::printf("Density contraction took%8.4f s.\n", es.t_ints);
+    std::printf("Skipped %12lld of %12lld integral triplets (%8.2f%%)\n", (long long)es.nskipped,
+                (long long)es.ntriplets, 100.0 * (double)es.nskipped / (double)std::max<int64_t>(es.ntriplets, 1));
+    std::fflush(stdout);
+    // intor/benchmark.f90:43,117: a stream of float64 records in the working directory
+    FILE* f = std::fopen("int_estimates.bin", "wb");
+    if (!f) throw Error{DFHIP_ERR_ARG, "cannot open int_estimates.bin for writing"};
+    const size_t nw = std::fwrite(es.rec.data(), sizeof(double), es.rec.size(), f);
+    std::fclose(f);
+    if (nw != es.rec.size()) throw Error{DFHIP_ERR_ARG, "short write to int_estimates.bin"};
   });
 }
 

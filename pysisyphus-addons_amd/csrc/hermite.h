@@ -193,20 +193,20 @@ __device__ inline double bra_element_lds(const int* sa, const SphTerm* ta, const
 // The triplet is kept unless the estimate is below thresh.  Evaluated in
 // the reference's operation order without contraction, so host and device
 // take the same decisions.
-__host__ __device__ inline bool qvl_keep(const double* pr, const double* xr, double Cx, double Cy, double Cz, int L,
-                                         double thresh) {
+__host__ __device__ inline double qvl_estimate(const double* pr, const double* xr, double Cx, double Cy, double Cz,
+                                               int L) {
 #pragma clang fp contract(off)
   const double dx = pr[0] - Cx, dy = pr[1] - Cy, dz = pr[2] - Cz;
   const double R = sqrt(dx * dx + dy * dy + dz * dz);
-  double est;
-  if (R <= pr[3] + xr[0]) {
-    est = pr[5] * xr[2];
-  } else {
-    double Rp = R;
-    for (int i = 0; i < L; i++) Rp = Rp * R;
-    est = pr[5] * PI * sqrt(2.0) * xr[1] / (pr[4] * Rp);
-  }
-  return !(est < thresh);
+  if (R <= pr[3] + xr[0]) return pr[5] * xr[2];
+  double Rp = R;
+  for (int i = 0; i < L; i++) Rp = Rp * R;
+  return pr[5] * PI * sqrt(2.0) * xr[1] / (pr[4] * Rp);
+}
+
+__host__ __device__ inline bool qvl_keep(const double* pr, const double* xr, double Cx, double Cy, double Cz, int L,
+                                         double thresh) {
+  return !(qvl_estimate(pr, xr, Cx, Cy, Cz, L) < thresh);
 }
 
 }  // namespace dfhip

@@ -79,6 +79,7 @@ EXPORTED = (
     "dfhip_default_options", "dfhip_contract_coulomb_densities", "dfhip_int3c2e_dense",
     "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
     "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc", "f_intor_df1c", "f_intor_eri2c", "f_intor_df2c_mo",
+    "f_benchmark_int3c2e", "dfhip_metric_powerh", "dfhip_int3c2e_estimates",
 )
 
 
@@ -118,6 +119,11 @@ def lib():
         L.f_intor_df1c.argtypes = args13[:10] + [i32, i32, vp]
         L.f_intor_eri2c.argtypes = args13[:10] + [i32, vp]
         L.f_intor_df2c_mo.argtypes = args13[:10] + [i32, i32, vp, i32, vp]
+        L.f_benchmark_int3c2e.argtypes = args13[:10]
+        L.dfhip_metric_powerh.restype = ctypes.c_int
+        L.dfhip_metric_powerh.argtypes = [ctypes.POINTER(Basis), ctypes.c_double, vp]
+        L.dfhip_int3c2e_estimates.restype = ctypes.c_int
+        L.dfhip_int3c2e_estimates.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis), vp, i64, vp]
         L.f_int_schwarz_aux.argtypes = [i32, i32, vp, vp, vp, vp]
         L.f_int_schwarz_bra.argtypes = [i32, i32, vp, vp, vp, vp]
         _lib = L

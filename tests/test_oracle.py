@@ -357,3 +357,22 @@ def test_std_df2c_mo_definitions():
     T = O.dense3c(bas, aux)
     assert X[3, 1, 7] == pytest.approx(sum(C[m, 3] * C[n, 1] * T[m, n, 7] for m in range(sh.nbf)
                                            for n in range(sh.nbf)), rel=1e-12)
+
+
+def test_estimate_records_properties():
+    """intor/benchmark.f90:15-132 restated: one record per aux shell for each
+    pair that is neither concentric nor small; the Frobenius norm of every
+    block obeys the Schwarz bound ||(ab|C)||_F <= Q_ab Q_C; inside the extents
+    the QVL estimate equals the Schwarz estimate; R is the distance of the
+    |d_a d_b|-weighted pair centre from C."""
+    sh, ax, _ = synthetic.workload(0)
+    recs, nskip = O.int3c2e_estimates(sh, ax)
+    n = len(sh.shells)
+    cents = [np.asarray(s.center) for s in sh.shells]
+    npairs = sum(1 for a in range(n) for b in range(a + 1) if np.linalg.norm(cents[a] - cents[b]) > 1e-6)
+    assert recs.shape[0] <= npairs * len(ax.shells) and recs.shape[0] % len(ax.shells) == 0
+    assert np.all(recs[:, 2] <= recs[:, 0] * (1 + 1e-12))
+    near = recs[:, 1] == recs[:, 0]
+    assert near.any() and (~near).any()
+    assert 0 < nskip < recs.shape[0]
+    assert nskip == int(np.sum(recs[:, 1] < O.SCHWARZ_THRESH))
