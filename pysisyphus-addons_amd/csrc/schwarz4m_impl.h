@@ -291,17 +291,23 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
     }
   }
   __syncthreads();
+  // the waves add their partial Z one after another (a fixed order, so Q_ab
+  // is reproducible bit for bit)
+  for (int w = 0; w < NW; w++) {
+    if (wave == w) {
 #pragma unroll
-  for (int a = 0; a < NCT; a++)
+      for (int a = 0; a < NCT; a++)
 #pragma unroll
-    for (int j = 0; j < NCT; j++)
+        for (int j = 0; j < NCT; j++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        double* z = &scr[(16 * a + lk + 4 * r) * ZL + 16 * j + lr];
-        if constexpr (NW == 1) *z = zacc[a][j][r];
-        else atomicAdd(z, zacc[a][j][r]);
-      }
-  __syncthreads();
+          for (int r = 0; r < 4; r++) {
+            double* z = &scr[(16 * a + lk + 4 * r) * ZL + 16 * j + lr];
+            if (w == 0) *z = zacc[a][j][r];
+            else *z += zacc[a][j][r];
+          }
+    }
+    __syncthreads();
+  }
   double ss = 0.0;
   for (int e = tid; e < NAB * NAB; e += NT) {
     const int n1 = e / NAB, n2 = e - n1 * NAB;

@@ -277,13 +277,20 @@ __global__ __launch_bounds__((Cls4s<LA, LB>::NT)) void schwarz4s_kernel(DevShell
         }
         __builtin_amdgcn_sched_barrier(0);  // keeps the loads of later tiles from piling up in registers
       }
+      // the waves add their partial tiles one after another (a fixed order,
+      // so Q_ab is reproducible bit for bit)
+      for (int w = 0; w < NW; w++) {
+        if (wave == w) {
 #pragma unroll
-      for (int z = 0; z < 2; z++)
+          for (int z = 0; z < 2; z++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int n1 = 16 * (a0 + z) + lk + 4 * r, n2 = 16 * ch + lr;
-          if (n1 < NAB && n2 < NAB) atomicAdd(&Z[n1 * NAB + n2], zacc[z][r]);
+            for (int r = 0; r < 4; r++) {
+              const int n1 = 16 * (a0 + z) + lk + 4 * r, n2 = 16 * ch + lr;
+              if (n1 < NAB && n2 < NAB) Z[n1 * NAB + n2] += zacc[z][r];
+            }
         }
+        __syncthreads();
+      }
     }
   }
   __syncthreads();

@@ -463,6 +463,21 @@ def test_bitwise_reproducible():
     assert np.array_equal(J1, J2)
 
 
+def test_bitwise_reproducible_g_shells():
+    """g orbital shells take the multi-wave and the streamed Schwarz kernels
+    ((ff|ff) and larger): their Schwarz norms, and so the pair order, the
+    screened set and J, are the same bit for bit from call to call."""
+    shells, aux, dens = synthetic.workload4_reduced(6)
+    q1, q2 = sintor.int_schwarz_bra(shells), sintor.int_schwarz_bra(shells)
+    assert np.array_equal(q1, q2)
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    opt = N.default_options(screening=N.SCREEN_QVL, thresh=1e-10)
+    _, J1, _ = _intor.contract(b, x, dens, nstates=20, want_df=False, want_j=True, options=opt)
+    _, J2, _ = _intor.contract(b, x, dens, nstates=20, want_df=False, want_j=True, options=opt)
+    assert np.array_equal(J1, J2)
+
+
 def test_generated_vs_lds_kernels(monkeypatch):
     """The register-resident generated kernels and the LDS kernels agree on
     every class both implement (workload 0 plus an all-class system)."""
