@@ -108,6 +108,21 @@ void f_benchmark_int3c2e(int32_t nshells, int32_t ndata, const int32_t *bas_cent
                          int32_t ndata_aux, const int32_t *bas_centers_aux, const int32_t *bas_spec_aux,
                          const double *bas_data_aux);
 
+/* grid_densities (column-major ndens x npoints, overwritten) = the densities
+ * of the Cartesian AO density matrices (column-major ndens x nbfs x nbfs,
+ * element (j, mu, nu) read for mu >= nu; nbfs = Cartesian functions of the
+ * basis, pysisyphus order) at the points grid3d (column-major 3 x npoints).
+ * Points go in blocks of blk_size; a pair (mu, nu) enters a block's points
+ * when |f D_j(mu, nu) mean_nu mean_mu| >= thresh (f = 2 off the diagonal,
+ * means of |chi| over the block).  Shells up to L = 4 (the reference returns
+ * NaN above; this library reports DFHIP_ERR_UNSUPPORTED).
+ * Replaces f_eval_densities, grid/grid_wrapper.f90:10-39
+ * (grid/grid.f90:112-286). */
+void f_eval_densities(int32_t nshells, int32_t ndata, const int32_t *bas_centers, const int32_t *bas_spec,
+                      const double *bas_data, int32_t npoints, const double *grid3d, int32_t ndens,
+                      int32_t nbfs, const double *densities, double *grid_densities, int32_t blk_size,
+                      double thresh);
+
 /* ---- status ------------------------------------------------------------ */
 
 enum {

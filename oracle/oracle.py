@@ -386,3 +386,91 @@ def int3c2e_estimates(shells, shells_aux):
                 recs.append([qb[k] * qa[c], q, np.linalg.norm(int3c2e_block(bas, aux, a, b, c)),
                              np.linalg.norm(pr[k, :3] - C)])
     return np.array(recs).reshape(-1, 4), nskip
+
+
+# grid/grid.f90:9-13: primitives with -a r^2 below CUTOFF are skipped; the
+# Cartesian components carry 1/sqrt((2lx-1)!! (2ly-1)!! (2lz-1)!!)
+GRID_CUTOFF = -36.0
+_DFACT_ODD = (1.0, 1.0, 3.0, 15.0, 105.0)
+
+
+def cart_exponents(L):
+    """Cartesian order of a shell (lx = L..0, then ly = L-lx..0): xx, xy, xz,
+    yy, yz, zz for L = 2 (grid.f90:64-99)."""
+    return [(lx, ly, L - lx - ly) for lx in range(L, -1, -1) for ly in range(L - lx, -1, -1)]
+
+
+def cart_norm(lx, ly, lz):
+    return 1.0 / math.sqrt(_DFACT_ODD[lx] * _DFACT_ODD[ly] * _DFACT_ODD[lz])
+
+
+def grid_shells(bas_centers, bas_spec, bas_data):
+    """(L, centre, exponents, normalized coefficients, first Cartesian index)
+    per shell from the reference basis arrays (common/shells.f90:97-187); the
+    coefficients come from the oracle's norm_cgto_coeffs."""
+    bas = Basis(bas_centers, bas_spec, bas_data)
+    bc, bs, bd = bas.bas_centers, bas.bas_spec, bas.bas_data
+    out, c0 = [], 0
+    for i in range(bas.nshells):
+        L, npr, pp = int(bs[i, 1]), int(bs[i, 2]), int(bs[i, 4]) - 1
+        cp = int(bc[i, 2]) - 1
+        coefs = np.zeros(npr)
+        lib().orc_basis_normed_coefs(bas.h, i, coefs)
+        out.append((L, bd[cp:cp + 3].copy(), bd[pp + npr:pp + 2 * npr].copy(), coefs, c0))
+        c0 += ncart(L)
+    return out, c0
+
+
+def cart_ao_values(shells, ncart_total, points):
+    """chi[i, c] of the Cartesian functions at points (n, 3): cart_gto3d_rel
+    (grid.f90:17-110) restated with numpy over the points."""
+    points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    chi = np.zeros((len(points), ncart_total))
+    for L, A, exps, coefs, c0 in shells:
+        d = points - A
+        dist = (d * d).sum(axis=1)
+        rad = np.zeros(len(points))
+        for a, c in zip(exps, coefs):
+            arg = -a * dist
+            keep = arg >= GRID_CUTOFF
+            rad[keep] += c * np.exp(arg[keep])
+        for k, (lx, ly, lz) in enumerate(cart_exponents(L)):
+            chi[:, c0 + k] = cart_norm(lx, ly, lz) * d[:, 0] ** lx * d[:, 1] ** ly * d[:, 2] ** lz * rad
+    return chi
+
+
+def eval_densities(bas_centers, bas_spec, bas_data, grid3d, densities, blk_size=100, thresh=1e-8,
+                   return_masks=False):
+    """grid/grid.f90:112-286 restated: the points in blocks of blk_size; per
+    block the mean |chi_nu| (:222-229), the pairs nu <= mu whose
+    |((f D_j(mu, nu)) mean_nu) mean_mu| >= thresh (f = 2 off the diagonal,
+    :234-258), and rho_j(i) = sum over the kept pairs of f D_j(mu, nu)
+    chi_mu(i) chi_nu(i) (:260-271).  densities (ndens, nbfs, nbfs) is read as
+    element [j, mu, nu] for mu >= nu.  Returns (ndens, npoints); with
+    return_masks also the per-block contrib arrays (for margin checks)."""
+    shells, nc = grid_shells(bas_centers, bas_spec, bas_data)
+    grid3d = np.asarray(grid3d, dtype=np.float64).reshape(-1, 3)
+    D = np.asarray(densities, dtype=np.float64)
+    ndens, npts = D.shape[0], len(grid3d)
+    assert D.shape[1] == D.shape[2] == nc
+    low = np.tril(np.ones((nc, nc), dtype=bool))            # [mu, nu] with mu >= nu
+    fac = np.where(np.eye(nc, dtype=bool), 1.0, 2.0)
+    FD = np.where(low[None], fac[None] * D, 0.0)             # f D_j(mu, nu), mu >= nu
+    rho = np.zeros((ndens, npts))
+    contribs = []
+    for b0 in range(0, npts, blk_size):
+        chi = cart_ao_values(shells, nc, grid3d[b0:b0 + blk_size])
+        cur = len(chi)
+        mean = np.zeros(nc)
+        for i in range(cur):                                 # summed in point order
+            mean += np.abs(chi[i])
+        mean /= cur
+        contrib = FD * mean[None, None, :] * mean[None, :, None]   # ((f D) mean_nu) mean_mu
+        M = np.where(low[None] & (np.abs(contrib) >= thresh), FD, 0.0)
+        if return_masks:
+            contribs.append(np.where(low[None], np.abs(contrib), np.nan))
+        # rho_j(i) = sum_{mu, nu} M_j[mu, nu] chi[i, mu] chi[i, nu]
+        rho[:, b0:b0 + cur] = np.einsum("im,jmn,in->ji", chi, M, chi, optimize=True)
+    if return_masks:
+        return rho, contribs
+    return rho

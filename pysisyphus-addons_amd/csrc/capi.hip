@@ -28,6 +28,7 @@
 
 #include "../../include/dfhip.h"
 #include "gen3c_classes.h"
+#include "grid.h"
 #include "int3c2e_impl.h"
 #include "int3c2e_mm_impl.h"
 #include "linalg.h"
@@ -2879,6 +2880,70 @@ void std_tensors(const dfhip_basis& b, const dfhip_basis& x, int kind, int nmo, 
   run(b, x, ndens, nbf, D.p, o, out, nullptr, false);
 }
 
+// grid/grid.f90:112-286 (eval_densities): out[j + ndens p] = density j at
+// point p (grid3d[3 p + d]) from the Cartesian density matrices dens
+// (Fortran (ndens, nbfs, nbfs), element (j, mu, nu) read for mu >= nu) in
+// blocks of bs points, pairs screened per block by thresh.  The points are
+// walked in batches of whole blocks whose AO values fit a fixed budget.
+void eval_densities_host(const dfhip_basis& b, int64_t npoints, const double* grid3d, int ndens, int nbfs,
+                         const double* dens, double* out, int bs, double thresh) {
+  if (!out) throw Error{DFHIP_ERR_ARG, "grid_densities is NULL"};
+  if (bs < 1) throw Error{DFHIP_ERR_ARG, "blk_size must be >= 1"};
+  if (npoints < 0 || ndens < 0) throw Error{DFHIP_ERR_ARG, "npoints and ndens must be >= 0"};
+  // cart_gto3d_rel (grid.f90:17-110) covers L <= 4 and returns NaN above
+  HostShells s = parse_shells(b, 4, "basis");
+  std::vector<int> c0(s.n);
+  int ncart = 0;
+  for (int i = 0; i < s.n; i++) {
+    c0[i] = ncart;
+    ncart += (s.L[i] + 1) * (s.L[i] + 2) / 2;
+  }
+  if (nbfs != ncart)
+    throw Error{DFHIP_ERR_ARG, "nbfs=" + std::to_string(nbfs) + " but the basis has " + std::to_string(ncart) +
+                                   " Cartesian functions"};
+  if (npoints == 0 || ndens == 0) return;
+  if (!grid3d || !dens) throw Error{DFHIP_ERR_ARG, "grid3d or densities is NULL"};
+  std::lock_guard<std::mutex> lock(ctx_mutex());
+  ensure_ctx();
+  cur_lane().arena.reset();
+  const hipStream_t st = cur_lane().st;
+  DBuf<double> dxyz, dexp, dcoef, dpts, dD, dDd, drho;
+  DBuf<int> dL, dnp, dpp, dc0;
+  upload(dxyz, s.xyz, st);
+  upload(dexp, s.exps, st);
+  upload(dcoef, s.coefs, st);
+  upload(dL, s.L, st);
+  upload(dnp, s.nprim, st);
+  upload(dpp, s.pptr, st);
+  upload(dc0, c0, st);
+  dpts.alloc((size_t)3 * npoints);
+  HC(hipMemcpyAsync(dpts.p, grid3d, dpts.n * sizeof(double), hipMemcpyHostToDevice, st));
+  const size_t nd = (size_t)ndens * ncart * ncart;
+  dD.alloc(nd);
+  dDd.alloc(nd);
+  HC(hipMemcpyAsync(dD.p, dens, nd * sizeof(double), hipMemcpyHostToDevice, st));
+  grid_transpose_densities(dD.p, ndens, ncart, dDd.p, st);
+  HC(hipGetLastError());
+  HC(hipStreamSynchronize(st));
+  dD.release();
+  drho.alloc((size_t)npoints * ndens);
+  const GridShells gs{dxyz.p, dexp.p, dcoef.p, dL.p, dnp.p, dpp.p, dc0.p, s.n, ncart};
+  const int64_t nblk = (npoints + bs - 1) / bs;
+  const size_t per_blk = (size_t)ncart * bs;
+  const int64_t bmax = std::max<int64_t>(1, std::min<int64_t>(nblk, ((size_t)1 << 28) / per_blk));
+  DBuf<double> dX((size_t)bmax * per_blk), dm((size_t)bmax * ncart);
+  for (int64_t b0 = 0; b0 < nblk; b0 += bmax) {
+    const int nb = (int)std::min(bmax, nblk - b0);
+    const int64_t p0 = b0 * bs, np = std::min<int64_t>((int64_t)nb * bs, npoints - p0);
+    grid_ao_values(gs, dpts.p, p0, np, bs, dX.p, st);
+    grid_block_means(dX.p, nb, ncart, bs, np, dm.p, st);
+    grid_block_densities(dX.p, dm.p, dDd.p, ndens, ncart, nb, bs, p0, np, thresh, drho.p, st);
+    HC(hipGetLastError());
+  }
+  HC(hipMemcpyAsync(out, drho.p, drho.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
 }  // namespace
 }  // namespace dfhip
 
@@ -3202,6 +3267,15 @@ void f_int_schwarz_bra(int32_t nshells, int32_t ndata, const int32_t* bas_center
   guarded([&] {
     std::vector<double> q = schwarz_bra_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data));
     std::memcpy(integrals, q.data(), q.size() * sizeof(double));
+  });
+}
+
+void f_eval_densities(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,
+                      const double* bas_data, int32_t npoints, const double* grid3d, int32_t ndens, int32_t nbfs,
+                      const double* densities, double* grid_densities, int32_t blk_size, double thresh) {
+  guarded([&] {
+    eval_densities_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data), npoints, grid3d, ndens, nbfs,
+                        densities, grid_densities, blk_size, thresh);
   });
 }
 

@@ -79,7 +79,7 @@ EXPORTED = (
     "dfhip_default_options", "dfhip_contract_coulomb_densities", "dfhip_int3c2e_dense",
     "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
     "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc", "f_intor_df1c", "f_intor_eri2c", "f_intor_df2c_mo",
-    "f_benchmark_int3c2e", "dfhip_metric_powerh", "dfhip_int3c2e_estimates",
+    "f_benchmark_int3c2e", "dfhip_metric_powerh", "dfhip_int3c2e_estimates", "f_eval_densities",
 )
 
 
@@ -126,6 +126,7 @@ def lib():
         L.dfhip_int3c2e_estimates.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis), vp, i64, vp]
         L.f_int_schwarz_aux.argtypes = [i32, i32, vp, vp, vp, vp]
         L.f_int_schwarz_bra.argtypes = [i32, i32, vp, vp, vp, vp]
+        L.f_eval_densities.argtypes = [i32, i32, vp, vp, vp, i32, vp, i32, i32, vp, vp, i32, ctypes.c_double]
         _lib = L
     return _lib
 

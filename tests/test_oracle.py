@@ -376,3 +376,121 @@ def test_estimate_records_properties():
     assert near.any() and (~near).any()
     assert 0 < nskip < recs.shape[0]
     assert nskip == int(np.sum(recs[:, 1] < O.SCHWARZ_THRESH))
+
+
+# ---- grid densities (grid/grid.f90:17-286) ---------------------------------
+
+def _grid_basis(seed=3, lmax=4):
+    rng = np.random.default_rng(seed)
+    shells = []
+    for ci, A in enumerate(([0.0, 0.0, 0.0], [1.1, -0.4, 0.7])):
+        for L in range(lmax + 1):
+            exps = np.sort(rng.uniform(0.4, 1.6, 2))[::-1]
+            shells.append(Shell(L=L, center=A, exps=exps, coeffs=rng.uniform(0.3, 1.0, 2), center_ind=ci))
+    return Shells(shells)
+
+
+def _sph_from_cart(L):
+    """T[m, c] with phi_m = sum_c T[m, c] chi_c for the grid's individually
+    normalized Cartesian functions chi_c (the oracle's solid-harmonic table
+    acts on the bare monomials of the contracted shell)."""
+    S = O.sph_coeffs(L)
+    nrm = np.array([O.cart_norm(*e) for e in O.cart_exponents(L)])
+    return S / nrm[None, :]
+
+
+def test_grid_ao_values_integrate_to_overlap():
+    """Gauss-Legendre quadrature of the grid's AO values, turned spherical,
+    reproduces the oracle's analytic overlap (itself pinned by ORCA's S):
+    pins the Cartesian order, the per-component factors of grid.f90:10-13 and
+    the coefficient normalization for L = 0..4."""
+    sh = _grid_basis()
+    arr = sh.as_arrays(fortran=True)
+    shells, nc = O.grid_shells(*arr)
+    # composite Gauss-Legendre, 4 panels of 20 points over [-7, 7] per axis
+    xg, wg = np.polynomial.legendre.leggauss(20)
+    e = np.linspace(-7.0, 7.0, 5)
+    x = np.concatenate([0.5 * (e[k + 1] - e[k]) * xg + 0.5 * (e[k + 1] + e[k]) for k in range(4)])
+    w = np.concatenate([0.5 * (e[k + 1] - e[k]) * wg for k in range(4)])
+    g = np.stack(np.meshgrid(x + 0.55, x - 0.2, x + 0.35, indexing="ij"), -1).reshape(-1, 3)
+    wt = np.einsum("i,j,k->ijk", w, w, w).ravel()
+    chi = O.cart_ao_values(shells, nc, g)
+    G = chi.T @ (wt[:, None] * chi)
+    blocks = [_sph_from_cart(L) for L, *_ in shells]
+    nsph = sum(b.shape[0] for b in blocks)
+    T = np.zeros((nsph, nc))
+    r = 0
+    for (L, _, _, _, c0), b in zip(shells, blocks):
+        T[r:r + b.shape[0], c0:c0 + b.shape[1]] = b
+        r += b.shape[0]
+    S = O.overlap(O.Basis(*arr))
+    np.testing.assert_allclose(T @ G @ T.T, S, atol=1e-9)
+    # and each Cartesian function is normalized on its own
+    np.testing.assert_allclose(np.diag(G), 1.0, atol=1e-9)
+
+
+def _sym_density(nc, ndens, rng):
+    D = rng.uniform(-1.0, 1.0, (ndens, nc, nc))
+    return 0.5 * (D + D.transpose(0, 2, 1))
+
+
+def test_grid_unscreened_is_the_full_quadratic_form():
+    """thresh = 0 keeps every pair: rho_j = chi D_j chi^T for symmetric D_j
+    (the factor 2 on the pairs nu < mu and the triangle are exact)."""
+    sh = _grid_basis(lmax=3)
+    arr = sh.as_arrays(fortran=True)
+    shells, nc = O.grid_shells(*arr)
+    rng = np.random.default_rng(0)
+    pts = rng.uniform(-2.5, 3.0, (237, 3))
+    D = _sym_density(nc, 2, rng)
+    rho = O.eval_densities(*arr, pts, D, blk_size=50, thresh=0.0)
+    chi = O.cart_ao_values(shells, nc, pts)
+    ref = np.einsum("im,jmn,in->ji", chi, D, chi)
+    np.testing.assert_allclose(rho, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    # only the mu >= nu triangle is read
+    Dl = np.tril(D[0])
+    rho_l = O.eval_densities(*arr, pts, Dl[None] + np.triu(rng.uniform(size=(nc, nc)), 1)[None], blk_size=50,
+                             thresh=0.0)
+    ref_l = np.einsum("im,mn,in->i", chi, Dl + np.tril(Dl, -1).T, chi)
+    np.testing.assert_allclose(rho_l[0], ref_l, rtol=0, atol=1e-13 * np.abs(ref_l).max())
+
+
+def test_grid_screen_one_point_blocks():
+    """blk_size = 1: a block's mean |chi| is |chi| at its point, so a pair
+    enters exactly when its own term |f D chi_mu chi_nu| reaches thresh."""
+    sh = _grid_basis(lmax=2)
+    arr = sh.as_arrays(fortran=True)
+    shells, nc = O.grid_shells(*arr)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(-3.0, 4.0, (40, 3))
+    D = _sym_density(nc, 1, rng)
+    thresh = 1e-3
+    rho = O.eval_densities(*arr, pts, D, blk_size=1, thresh=thresh)[0]
+    chi = O.cart_ao_values(shells, nc, pts)
+    ref = np.zeros(len(pts))
+    nkept = ndrop = 0
+    for i in range(len(pts)):
+        for mu in range(nc):
+            for nu in range(mu + 1):
+                term = (1.0 if mu == nu else 2.0) * D[0, mu, nu] * chi[i, mu] * chi[i, nu]
+                if abs(term) >= thresh:
+                    ref[i] += term
+                    nkept += 1
+                else:
+                    ndrop += 1
+    assert nkept > 100 and ndrop > 100
+    np.testing.assert_allclose(rho, ref, rtol=1e-13, atol=1e-15)
+    # a threshold above every term leaves nothing
+    assert not O.eval_densities(*arr, pts, D, blk_size=1, thresh=1e6).any()
+
+
+def test_grid_cutoff_drops_primitives():
+    """A primitive whose -a r^2 is below -36 contributes nothing (grid.f90:58)."""
+    sh = Shells([Shell(L=0, center=[0.0, 0.0, 0.0], exps=[4.0], coeffs=[1.0])])
+    arr = sh.as_arrays(fortran=True)
+    shells, nc = O.grid_shells(*arr)
+    r = math.sqrt(36.0 / 4.0)
+    chi = O.cart_ao_values(shells, nc, [[r * 0.999, 0, 0], [r * 1.001, 0, 0]])
+    assert chi[0, 0] > 0 and chi[1, 0] == 0.0
+    # the normalized s function at the origin: (2a/pi)^(3/4)
+    assert O.cart_ao_values(shells, nc, [[0, 0, 0]])[0, 0] == pytest.approx((8.0 / math.pi) ** 0.75, rel=1e-14)
