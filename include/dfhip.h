@@ -269,6 +269,30 @@ int dfhip_metric(const dfhip_basis *aux, double *out);
 /* F[i] = F_{n[i]}(T[i]), 0 <= n[i] <= 17, T[i] >= 0, on the device. */
 int dfhip_boys(int64_t npts, const int32_t *n, const double *T, double *F);
 
+/* ---- autotuner (no counterpart in the reference) -------------------------
+ * dfhip_autotune measures, on the caller's workload (arguments as for
+ * dfhip_contract_coulomb_densities; opt selects the precision and screening
+ * that are tuned), the kernel choices that the library otherwise takes from
+ * fixed rules:
+ *   - per (la, lb, lc) class that has both a generated VALU kernel and a
+ *     matrix-core kernel, and per precision window (0 = FP64, 1 = the FP32
+ *     window of the mixed path), which of the two runs it;
+ *   - for this density count, the densities per thread of the contraction's
+ *     output tile (1-5 or 7) and the scale of its split-K depth (1/2, 1, 2).
+ * One full contraction per candidate, timed with HIP events per class; the
+ * rule's choice stays unless another is at least 2 % faster.  The table
+ * replaces this process's table and is written to path (NULL: not written).
+ * dfhip_load_tuning reads such a file (NULL clears the table); the library
+ * also reads the file named by the environment variable DFHIP_TUNING when it
+ * sets up its device.  Results do not depend on the table beyond rounding. */
+int dfhip_autotune(const dfhip_basis *bas, const dfhip_basis *aux, int32_t ndens, int32_t nbfs,
+                   const double *densities, const dfhip_options *opt, const char *path);
+int dfhip_load_tuning(const char *path);
+/* 1 generated, 2 matrix cores, 0 not tuned, -1 invalid arguments */
+int32_t dfhip_tuning_choice(int32_t la, int32_t lb, int32_t lc, int32_t window);
+/* 1 and *ipt, *split_scale when the table has an entry for ndens, else 0 */
+int32_t dfhip_tuning_contract(int32_t ndens, int32_t *ipt, double *split_scale);
+
 #ifdef __cplusplus
 }
 #endif

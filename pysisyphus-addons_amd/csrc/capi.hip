@@ -502,6 +502,8 @@ unsigned char* resolve_scratch(Ctx& c, int dev) {
   return c.scratch_dev[dev];
 }
 
+void load_env_tuning();  // the table named by DFHIP_TUNING, if any
+
 void ensure_ctx() {
   Ctx& c = ctx();
   if (c.ready) return;
@@ -509,6 +511,7 @@ void ensure_ctx() {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error{DFHIP_ERR_HIP, "no HIP device available"};
   HC(hipGetDevice(&c.device));
   resolve_scratch(c, c.device);
+  load_env_tuning();
   HC(hipStreamCreateWithFlags(&c.lane0.st, hipStreamNonBlocking));
   const hipStream_t st0 = c.lane0.st;
   std::vector<double> tab;
@@ -901,6 +904,52 @@ inline bool gen3_has(int la, int lb, int lc) {
 // against 776 us, (dd|f) 388 against 557 us); in mixed mode their FP64
 // window goes to the matrix-core kernel, the FP32 window stays generated.
 inline bool prefer_mm(int la, int lb, int lc) { return la == 2 && lb == 2 && lc >= 2; }
+
+// Kernel choices measured by dfhip_autotune or loaded from its file.
+// fam[k][L][w]: for window w (0 = FP64, 1 = the FP32 window of the mixed
+// path) of class (pair class k, aux L) 1 = generated VALU kernel, 2 =
+// matrix-core kernel, 0 = not tuned (the rule above decides).  contract:
+// per density count, the densities per thread of the contraction's output
+// tile and the scale of its split-K depth.
+struct Tuning {
+  signed char fam[NPCLS][LMAX_AUX + 1][2] = {};
+  double ms[NPCLS][LMAX_AUX + 1][2][2] = {};  // measured [w][gen, mm] (for the file)
+  struct Contract {
+    int ndens, ipt;
+    double nscale, ms;
+  };
+  std::vector<Contract> contract;
+  const Contract* contract_for(int ndens) const {
+    for (const auto& c : contract)
+      if (c.ndens == ndens) return &c;
+    return nullptr;
+  }
+};
+Tuning& tuning() {
+  static Tuning* t = new Tuning();
+  return *t;
+}
+std::mutex& tuning_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+// Set by dfhip_autotune on its thread for the duration of one measured run:
+// forced choices (0 = the table / rule) and the measured times.
+struct TuneProbe {
+  int fam64 = 0, fam32 = 0, ipt = 0;
+  double nscale = 0.0;
+  double ms[NPCLS][LMAX_AUX + 1][2] = {};
+  bool seen[NPCLS][LMAX_AUX + 1][2] = {};
+};
+thread_local TuneProbe* t_probe = nullptr;
+
+// family of window w of class (k, L): the probe's, else the table's, else 0
+int tuned_family(int k, int L, int w) {
+  if (t_probe && (w == 0 ? t_probe->fam64 : t_probe->fam32)) return w == 0 ? t_probe->fam64 : t_probe->fam32;
+  std::lock_guard<std::mutex> g(tuning_mutex());
+  return tuning().fam[k][L][w];
+}
 
 void launch_bra_records(int la, int lb, DevShells bra, const int* s1, const int* s2, const int64_t* recoff, int rs,
                         int npairs, double* recs, hipStream_t st) {
@@ -1858,9 +1907,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       for (int lb = 0; lb <= la; lb++) {
         const int k = pcls(la, lb);
         if (pl.ps.cls_end[k] == pl.ps.cls_begin[k] || c.et[la][lb].fn == nullptr) continue;
-        for (int L = 0; L <= LMAX_AUX; L++)
-          use_mm[k][L] = haveL[L] && (!(gen && gen3_has(la, lb, L)) || prefer_mm(la, lb, L)) &&
-                         c.int3m[la][lb][L].fn != nullptr;
+        for (int L = 0; L <= LMAX_AUX; L++) {
+          // where both kernels exist: the tuned family, else the measured rule
+          const int f = tuned_family(k, L, 0);
+          const bool mm_first = f ? f == 2 : prefer_mm(la, lb, L);
+          use_mm[k][L] = haveL[L] && (!(gen && gen3_has(la, lb, L)) || mm_first) && c.int3m[la][lb][L].fn != nullptr;
+        }
       }
   }
   auto build_et = [&](auto need, hipStream_t es, bool defer) {
@@ -1916,7 +1968,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++)
         for (int L = 0; L <= LMAX_AUX; L++)
-          if (haveL[L] && gen && gen3_has(la, lb, L) && prefer_mm(la, lb, L)) early[pcls(la, lb)] = 1;
+          if (haveL[L] && gen && gen3_has(la, lb, L) && use_mm[pcls(la, lb)][L]) early[pcls(la, lb)] = 1;
     build_et([&](int k, int, int) { return early[k] != 0; }, st, true);
   }
   vstep("pair plan, bra records");
@@ -2036,8 +2088,9 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       for (int lb = 0; lb <= la; lb++)
         for (int L = 0; L <= LMAX_AUX; L++) {
           const int k = pcls(la, lb);
-          if (gen && gen3_has(la, lb, L)) lowp[k][L] = 1;
-          else if (use_mm[k][L] && c.int3m[la][lb][L].fn32 != nullptr) lowp[k][L] = 2;
+          const bool mm32 = use_mm[k][L] && c.int3m[la][lb][L].fn32 != nullptr;
+          if (gen && gen3_has(la, lb, L)) lowp[k][L] = (mm32 && tuned_family(k, L, 1) == 2) ? 2 : 1;
+          else if (mm32) lowp[k][L] = 2;
         }
     split_precision(pl, opt.fp32_below, lowp);
   }
@@ -2079,9 +2132,24 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   ScreenMask mk{};
   if (need_dens) mk = ScreenMask{drowpair.p, drowperm.p, ab.dev<int>(o_colL), ab.dev<int>(o_colia), dnsflat.p, pl.ps.n()};
   vstep("column blocks, mask");
+  // density-tile width and split-K depth of the contraction: the probe's, the
+  // tuned ones for this density count, or the defaults
+  int cipt = 0;
+  double cscale = 1.0;
+  if (t_probe && t_probe->ipt) {
+    cipt = t_probe->ipt;
+    if (t_probe->nscale > 0.0) cscale = t_probe->nscale;
+  } else {
+    std::lock_guard<std::mutex> g(tuning_mutex());
+    if (const Tuning::Contract* tc = tuning().contract_for(ndens)) {
+      cipt = tc->ipt;
+      cscale = tc->nscale;
+    }
+  }
   if (need_dens) {
-    const int64_t tiles = contract_ntiles(widest, ndens);
-    nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(512, 2048 / std::max<int64_t>(tiles, 1)));
+    const int64_t tiles = contract_ntiles(widest, ndens, cipt);
+    nsplit = (int)std::max<int64_t>(
+        1, std::min<int64_t>(512, (int64_t)(cscale * 2048.0 / (double)std::max<int64_t>(tiles, 1))));
     nsplit = (int)std::min<int64_t>(nsplit, std::max<int64_t>(1, (pl.npf + 15) / 16));
     partp = reserve(cur_lane().wsPart, (size_t)nsplit * widest * ndens);
   }
@@ -2209,6 +2277,28 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   vstep("integral set-up");
   DBuf<double> dnorm;  // block norms, [sorted pair][aux shell]
   if (out.blocknorm) dnorm.alloc((size_t)std::max(pl.ps.n(), 1) * x.n);
+  // under dfhip_autotune: events around each class's launches, read after
+  // each column block
+  struct ProbeEvents {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> tag;
+    size_t used = 0;
+    ~ProbeEvents() {
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+  } pev;
+  TuneProbe* const probe = t_probe;
+  auto probe_mark = [&](int tagv) {
+    if (!probe) return;
+    if (pev.used == pev.ev.size()) {
+      hipEvent_t e;
+      HC(hipEventCreate(&e));
+      pev.ev.push_back(e);
+      pev.tag.push_back(0);
+    }
+    pev.tag[pev.used] = tagv;
+    HC(hipEventRecord(pev.ev[pev.used++], st));
+  };
   for (int blk = 0; blk < nblocks; blk++) {
     const int64_t cA = bounds[blk], cB = bounds[blk + 1], w = cB - cA;
     // only the dense dump reads screened-out entries; the contraction masks them
@@ -2241,24 +2331,33 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             mml.fn32(dorb.view, daux.view, q, xa, etg32[k].p, etoff32[k].p, c.sph.p, c.boys.p, Tp - cA, w, st);
             HC(hipGetLastError());
           };
+          const int tag0 = (k * (LMAX_AUX + 1) + L) * 2;  // probe tags: window 0 / 1
           if (mixed && lowp[k][L]) {
             PairArgs p64 = pa, p32 = pa;
             p64.nsurv = dn64flat.p + (size_t)L * pl.ps.n() + p0;  // [0, n64): FP64
             p32.nbeg = dn64flat.p + (size_t)L * pl.ps.n() + p0;   // [n64, nsurv): FP32
+            probe_mark(tag0);
             if (mmk) launch_mm(p64);
             else launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);
+            probe_mark(-1);
+            probe_mark(tag0 + 1);
             if (lowp[k][L] == 1) {
               launch_gen3(c, la, lb, L, brs[k], daux.view, p32, xa, Tp - cA, w, true);
             } else {
               launch_mm32(p32);
             }
-          } else if (mmk) {
-            launch_mm(pa);
-          } else if (gen && gen3_has(la, lb, L)) {
-            launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
+            probe_mark(-1);
           } else {
-            c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, Tp - cA, w, st);
-            HC(hipGetLastError());
+            probe_mark(tag0);
+            if (mmk) {
+              launch_mm(pa);
+            } else if (gen && gen3_has(la, lb, L)) {
+              launch_gen3(c, la, lb, L, brs[k], daux.view, pa, xa, Tp - cA, w);
+            } else {
+              c.int3[la][lb][L].fn(dorb.view, daux.view, pa, xa, c.sph.p, c.boys.p, Tp - cA, w, st);
+              HC(hipGetLastError());
+            }
+            probe_mark(-1);
           }
         }
     }
@@ -2308,7 +2407,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       mb.colL = mk.colL + cA;
       mb.colia = mk.colia + cA;
       contract_T(Tp, w, Dp, ndens, (int)w, ndens, NPCLS, dseg.p, mb, partp, nsplit, dorig.p + cA, Ap + 2 * naux, lda, 1,
-                 st);
+                 st, cipt);
       HC(hipGetLastError());
     }
     const int b2 = tm.mark();
@@ -2335,6 +2434,15 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     HC(hipStreamSynchronize(st));
     ms_int += tm.ms(b0, b1);
     ms_con += tm.ms(b1, b2);
+    // the probe's class times of this block: pairs (tag, -1) of marks
+    for (size_t i = 0; probe && i + 1 < pev.used; i += 2) {
+      float ms = 0.0f;
+      HC(hipEventElapsedTime(&ms, pev.ev[i], pev.ev[i + 1]));
+      const int t = pev.tag[i], kk = t / 2 / (LMAX_AUX + 1), LL = t / 2 % (LMAX_AUX + 1), ww = t % 2;
+      probe->ms[kk][LL][ww] += ms;
+      probe->seen[kk][LL][ww] = true;
+    }
+    pev.used = 0;
     vlog(opt, "Column block %d of %d: integrals %.1f ms, contraction %.1f ms.\n", blk + 1, nblocks, tm.ms(b0, b1),
          tm.ms(b1, b2));
   }
@@ -2880,6 +2988,195 @@ void std_tensors(const dfhip_basis& b, const dfhip_basis& x, int kind, int nmo, 
   run(b, x, ndens, nbf, D.p, o, out, nullptr, false);
 }
 
+// ---- tuning table: file, measurement ------------------------------------
+
+void tuning_save(const Tuning& t, const char* path) {
+  FILE* f = std::fopen(path, "w");
+  if (!f) throw Error{DFHIP_ERR_ARG, std::string("cannot write ") + path};
+  std::fprintf(f, "# libdfhip tuning table (dfhip_autotune)\n");
+  std::fprintf(f, "# int3c2e la lb lc window family(1 generated, 2 matrix cores) ms_generated ms_matrix_cores\n");
+  std::fprintf(f, "# contract ndens densities_per_thread split_scale ms\n");
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++)
+      for (int L = 0; L <= LMAX_AUX; L++)
+        for (int w = 0; w < 2; w++) {
+          const int k = pcls(la, lb);
+          if (t.fam[k][L][w])
+            std::fprintf(f, "int3c2e %d %d %d %d %d %.6f %.6f\n", la, lb, L, w, t.fam[k][L][w], t.ms[k][L][w][0],
+                         t.ms[k][L][w][1]);
+        }
+  for (const auto& c : t.contract) std::fprintf(f, "contract %d %d %.3f %.6f\n", c.ndens, c.ipt, c.nscale, c.ms);
+  const bool ok = std::fclose(f) == 0;
+  if (!ok) throw Error{DFHIP_ERR_ARG, std::string("cannot write ") + path};
+}
+
+Tuning tuning_load(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) throw Error{DFHIP_ERR_ARG, std::string("cannot read ") + path};
+  Tuning t;
+  char line[512];
+  int lineno = 0;
+  std::string bad;
+  while (std::fgets(line, sizeof(line), f)) {
+    lineno++;
+    char kind[32] = {};
+    if (line[0] == '#' || std::sscanf(line, "%31s", kind) != 1) continue;
+    if (std::strcmp(kind, "int3c2e") == 0) {
+      int la, lb, L, w, fam;
+      double m0 = 0.0, m1 = 0.0;
+      if (std::sscanf(line, "%*s %d %d %d %d %d %lf %lf", &la, &lb, &L, &w, &fam, &m0, &m1) < 5 || la < 0 ||
+          la > LMAX_ORB || lb < 0 || lb > la || L < 0 || L > LMAX_AUX || w < 0 || w > 1 || fam < 1 || fam > 2) {
+        bad = "line " + std::to_string(lineno);
+        break;
+      }
+      const int k = pcls(la, lb);
+      t.fam[k][L][w] = (signed char)fam;
+      t.ms[k][L][w][0] = m0;
+      t.ms[k][L][w][1] = m1;
+    } else if (std::strcmp(kind, "contract") == 0) {
+      Tuning::Contract c{};
+      if (std::sscanf(line, "%*s %d %d %lf %lf", &c.ndens, &c.ipt, &c.nscale, &c.ms) < 3 || c.ndens < 1 ||
+          !contract_ipt_ok(c.ipt) || !(c.nscale > 0.0) || c.nscale > 64.0) {
+        bad = "line " + std::to_string(lineno);
+        break;
+      }
+      t.contract.push_back(c);
+    } else {
+      bad = "line " + std::to_string(lineno);
+      break;
+    }
+  }
+  std::fclose(f);
+  if (!bad.empty()) throw Error{DFHIP_ERR_ARG, std::string(path) + ": malformed " + bad};
+  return t;
+}
+
+// a tuning table named by DFHIP_TUNING (dfhip_autotune's file), read when
+// the library sets up its device context
+void load_env_tuning() {
+  const char* tp = std::getenv("DFHIP_TUNING");
+  if (!tp || !*tp) return;
+  Tuning t = tuning_load(tp);
+  std::lock_guard<std::mutex> g(tuning_mutex());
+  tuning() = t;
+}
+
+// dfhip_autotune: measured runs of the contraction on the caller's workload
+// (densities as for dfhip_contract_coulomb_densities; the Gram matrix is the
+// only output), one per candidate:
+//   1. every class with both kernels on the generated one, then on the
+//      matrix-core one (mixed: also the FP32 window on the matrix cores);
+//      per (la, lb, lc) and window the faster family is kept, the rule's
+//      choice unless the other is at least 2 % faster;
+//   2. with those families, the contraction's densities per thread over the
+//      supported widths, then its split-K depth at 1/2 and 2 times the default.
+// A first run with the current choices warms the tables and the workspaces.
+Tuning autotune_host(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const double* dens,
+                     const dfhip_options& opt) {
+  const bool mixed = opt.precision == DFHIP_MIXED;
+  std::vector<double> gram((size_t)ndens * ndens);
+  Outputs out;
+  out.gram = gram.data();
+  struct Reset {
+    ~Reset() { t_probe = nullptr; }
+  } reset;
+  auto measure = [&](TuneProbe& p) {
+    dfhip_stats st{};
+    t_probe = &p;
+    run(B, X, ndens, nbfs, dens, opt, out, &st);
+    t_probe = nullptr;
+    return st.ms_contract;
+  };
+  {
+    TuneProbe warm;
+    measure(warm);
+  }
+  TuneProbe gen64, mm64, mm32;
+  gen64.fam64 = 1;
+  gen64.fam32 = 1;
+  mm64.fam64 = 2;
+  mm64.fam32 = 1;
+  mm32.fam64 = 2;
+  mm32.fam32 = 2;
+  measure(gen64);
+  measure(mm64);
+  if (mixed) measure(mm32);
+  Tuning t;
+  const bool gen = use_generated();
+  Ctx& c = ctx();
+  for (int la = 0; la <= LMAX_ORB; la++)
+    for (int lb = 0; lb <= la; lb++)
+      for (int L = 0; L <= LMAX_AUX; L++) {
+        const int k = pcls(la, lb);
+        if (!(gen && gen3_has(la, lb, L)) || c.int3m[la][lb][L].fn == nullptr) continue;
+        if (gen64.seen[k][L][0] && mm64.seen[k][L][0]) {
+          const double tg = gen64.ms[k][L][0], tm = mm64.ms[k][L][0];
+          const bool rule_mm = prefer_mm(la, lb, L);
+          const bool pick_mm = rule_mm ? !(tg < 0.98 * tm) : tm < 0.98 * tg;
+          t.fam[k][L][0] = pick_mm ? 2 : 1;
+          t.ms[k][L][0][0] = tg;
+          t.ms[k][L][0][1] = tm;
+        }
+        // the FP32 window on the matrix cores needs the FP64 window there
+        if (mixed && t.fam[k][L][0] == 2 && c.int3m[la][lb][L].fn32 != nullptr && mm64.seen[k][L][1] &&
+            mm32.seen[k][L][1]) {
+          const double tg = mm64.ms[k][L][1], tm = mm32.ms[k][L][1];
+          t.fam[k][L][1] = tm < 0.98 * tg ? 2 : 1;
+          t.ms[k][L][1][0] = tg;
+          t.ms[k][L][1][1] = tm;
+        }
+      }
+  // the contraction with the families above in place
+  Tuning saved;
+  {
+    std::lock_guard<std::mutex> g(tuning_mutex());
+    saved = tuning();
+    std::memcpy(tuning().fam, t.fam, sizeof(t.fam));
+    tuning().contract.clear();
+  }
+  struct Restore {
+    Tuning& s;
+    bool on = true;
+    ~Restore() {
+      if (!on) return;
+      std::lock_guard<std::mutex> g(tuning_mutex());
+      tuning() = s;
+    }
+  } restore{saved};
+  // the default width is kept unless another one is at least 2 % faster
+  const int ipt0 = contract_ipt(ndens);
+  std::vector<std::pair<int, double>> cand;
+  for (int ipt : {1, 2, 3, 4, 5, 7}) {
+    // tiles from a quarter to twice the densities (the default always)
+    if (ipt != ipt0 && (16 * ipt > 2 * std::max(ndens, 16) || 64 * ipt < ndens)) continue;
+    TuneProbe p;
+    p.ipt = ipt;
+    cand.push_back({ipt, measure(p)});
+  }
+  int best_ipt = ipt0;
+  double best = 0.0, best_scale = 1.0;
+  for (const auto& cd : cand)
+    if (cd.first == ipt0) best = cd.second;
+  for (const auto& cd : cand)
+    if (cd.second < 0.98 * best) {
+      best = cd.second;
+      best_ipt = cd.first;
+    }
+  for (double sc : {0.5, 2.0}) {
+    TuneProbe p;
+    p.ipt = best_ipt;
+    p.nscale = sc;
+    const double ms = measure(p);
+    if (ms < 0.98 * best) {
+      best = ms;
+      best_scale = sc;
+    }
+  }
+  t.contract.push_back(Tuning::Contract{ndens, best_ipt, best_scale, best});
+  restore.on = false;
+  return t;
+}
+
 // grid/grid.f90:112-286 (eval_densities): out[j + ndens p] = density j at
 // point p (grid3d[3 p + d]) from the Cartesian density matrices dens
 // (Fortran (ndens, nbfs, nbfs), element (j, mu, nu) read for mu >= nu) in
@@ -3268,6 +3565,41 @@ void f_int_schwarz_bra(int32_t nshells, int32_t ndata, const int32_t* bas_center
     std::vector<double> q = schwarz_bra_host(mkbasis(nshells, ndata, bas_centers, bas_spec, bas_data));
     std::memcpy(integrals, q.data(), q.size() * sizeof(double));
   });
+}
+
+int dfhip_autotune(const dfhip_basis* bas, const dfhip_basis* aux, int32_t ndens, int32_t nbfs,
+                   const double* densities, const dfhip_options* opt, const char* path) {
+  return guarded([&] {
+    if (!bas || !aux || !densities) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    const dfhip_options o = opt ? *opt : defaults();
+    Tuning t = autotune_host(*bas, *aux, ndens, nbfs, densities, o);
+    if (path && *path) tuning_save(t, path);
+    std::lock_guard<std::mutex> g(tuning_mutex());
+    tuning() = t;
+  });
+}
+
+int dfhip_load_tuning(const char* path) {
+  return guarded([&] {
+    Tuning t = (path && *path) ? tuning_load(path) : Tuning();
+    std::lock_guard<std::mutex> g(tuning_mutex());
+    tuning() = t;
+  });
+}
+
+int32_t dfhip_tuning_choice(int32_t la, int32_t lb, int32_t lc, int32_t window) {
+  if (la < 0 || la > LMAX_ORB || lb < 0 || lb > la || lc < 0 || lc > LMAX_AUX || window < 0 || window > 1) return -1;
+  std::lock_guard<std::mutex> g(tuning_mutex());
+  return tuning().fam[pcls(la, lb)][lc][window];
+}
+
+int32_t dfhip_tuning_contract(int32_t ndens, int32_t* ipt, double* split_scale) {
+  std::lock_guard<std::mutex> g(tuning_mutex());
+  const Tuning::Contract* c = tuning().contract_for(ndens);
+  if (!c) return 0;
+  if (ipt) *ipt = c->ipt;
+  if (split_scale) *split_scale = c->nscale;
+  return 1;
 }
 
 void f_eval_densities(int32_t nshells, int32_t ndata, const int32_t* bas_centers, const int32_t* bas_spec,

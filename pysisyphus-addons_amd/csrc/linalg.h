@@ -37,12 +37,16 @@ struct ScreenMask {
   int npairs;
 };
 
+// ipt: densities per thread of the output tile (0 or an unsupported value:
+// contract_ipt(ndens))
 void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int ncol, int ndens, int nseg,
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
-                int64_t sP, int64_t sI, hipStream_t st);
+                int64_t sP, int64_t sI, hipStream_t st, int ipt = 0);
 int contract_tile();
+int contract_ipt(int ndens);
+bool contract_ipt_ok(int ipt);
 // output tiles of one contraction (column tiles x density tiles)
-int64_t contract_ntiles(int64_t ncol, int ndens);
+int64_t contract_ntiles(int64_t ncol, int ndens, int ipt = 0);
 // Cholesky of [[M], [I]] (n columns, 2n rows, lda): L in the leading block,
 // L^-T in the border rows n..2n-1.
 void cholesky_inverse_rows(double* A, int64_t lda, int64_t n, int* info, hipStream_t st);

@@ -302,11 +302,13 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
     }
 }
 
-// densities per thread of the contraction's output tile
+// densities per thread of the contraction's output tile (the default; the
+// tuner may pick another of contract_ipt_ok)
 int contract_ipt(int ndens) { return ndens <= 16 ? 1 : ndens <= 32 ? 2 : ndens <= 48 ? 3 : ndens <= 64 ? 4 : 7; }
+bool contract_ipt_ok(int ipt) { return (ipt >= 1 && ipt <= 5) || ipt == 7; }
 
-int64_t contract_ntiles(int64_t ncol, int ndens) {
-  const int gi = 16 * contract_ipt(ndens);
+int64_t contract_ntiles(int64_t ncol, int ndens, int ipt) {
+  const int gi = 16 * (contract_ipt_ok(ipt) ? ipt : contract_ipt(ndens));
   return ((ncol + GP - 1) / GP) * ((ndens + gi - 1) / gi);
 }
 
@@ -324,9 +326,9 @@ __global__ void reduce_unpermute_kernel(const double* __restrict__ part, int nsp
 
 void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int ncol, int ndens, int nseg,
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
-                int64_t sP, int64_t sI, hipStream_t st) {
+                int64_t sP, int64_t sI, hipStream_t st, int ipt_req) {
   if (ncol <= 0) return;
-  const int ipt = contract_ipt(ndens), gi = 16 * ipt;
+  const int ipt = contract_ipt_ok(ipt_req) ? ipt_req : contract_ipt(ndens), gi = 16 * ipt;
   dim3 grid((ncol + GP - 1) / GP, (ndens + gi - 1) / gi, nsplit);
   switch (ipt) {
     case 1:
@@ -343,6 +345,10 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
       break;
     case 4:
       hipLaunchKernelGGL(gemm_tn_splitk_kernel<4>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
+                         nsplit, mk, part);
+      break;
+    case 5:
+      hipLaunchKernelGGL(gemm_tn_splitk_kernel<5>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
                          nsplit, mk, part);
       break;
     default:  // more than 64 densities: tiles of 112, T read half as often as with 64

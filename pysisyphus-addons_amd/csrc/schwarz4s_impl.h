@@ -131,18 +131,22 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
   }
   __syncthreads();
   // R^(n) by levels n = L .. 0 in the pair's scratch (L is even: level n in R
-  // if n is even, else in the F region, which is built afterwards), quartets
-  // on lanes; every entry of a level is independent.  Global memory written
-  // before __syncthreads() is visible to the whole workgroup after it.
+  // if n is even, else in the F region, which is built afterwards); every
+  // entry of a level is independent.  Levels n >= 1 are stored entry-major
+  // ([h][quartet], the quartets of one entry on neighbouring lanes, so that
+  // the stores of a wave are contiguous); level 0 goes quartet-major
+  // ([quartet][h], the layout the main kernel stages), with the entries of
+  // one quartet on neighbouring lanes.  Global memory written before
+  // __syncthreads() is visible to the whole workgroup after it.
   {
+    constexpr int NQS = K::NQS;
     double* Rt = F;
-    for (int n = L; n >= 0; n--) {
-      const double* So = (n % 2 == 0) ? Rt : R;  // level n+1
+    for (int n = L; n >= 1; n--) {
+      const double* So = (n % 2 == 0) ? Rt : R;  // level n+1, [h][q]
       double* Sn = (n % 2 == 0) ? R : Rt;
-      const int so = (n % 2 == 0) ? K::NRL : NR1, sn = (n % 2 == 0) ? NR1 : K::NRL;
       const int nent = nherm(L - n);
-      for (int e = tid; e < nent * K::NQS; e += NT) {
-        const int h = e / K::NQS, q = e - h * K::NQS;  // compile-time divisor
+      for (int e = tid; e < nent * NQS; e += NT) {
+        const int h = e / NQS, q = e - h * NQS;  // compile-time divisor
         if (q >= nq) continue;
         const double* g = G + q * GW;
         double val;
@@ -151,14 +155,29 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
         } else {
           const int rc = rec[h];
           const int h1 = rc & 2047, h2 = (rc >> 11) & 2047, dir = (rc >> 22) & 3, cf = rc >> 24;
-          const double* s0 = So + (size_t)q * so;
-          val = fma((double)(2 * cf), s0[h2], g[L + 1 + dir] * s0[h1]);
+          val = fma((double)(2 * cf), So[h2 * NQS + q], g[L + 1 + dir] * So[h1 * NQS + q]);
         }
-        Sn[(size_t)q * sn + h] = val;
+        Sn[h * NQS + q] = val;
       }
       __syncthreads();
     }
-    for (int q = tid; q < nq; q += NT) R[(size_t)q * NR1 + NR] = 0.0;  // the h2 >= H pad
+    // level 0 from level 1 (in Rt); the entries NR of every quartet are the h2 >= H pad
+    for (int e = tid; e < nq * NR1; e += NT) {
+      const int q = e / NR1, h = e - q * NR1;  // compile-time divisor
+      const double* g = G + q * GW;
+      double val;
+      if (h == 0) {
+        val = g[0];
+      } else if (h == NR) {
+        val = 0.0;
+      } else {
+        const int rc = rec[h];
+        const int h1 = rc & 2047, h2 = (rc >> 11) & 2047, dir = (rc >> 22) & 3, cf = rc >> 24;
+        val = fma((double)(2 * cf), Rt[h2 * NQS + q], g[L + 1 + dir] * Rt[h1 * NQS + q]);
+      }
+      R[(size_t)q * NR1 + h] = val;
+    }
+    __syncthreads();
   }
   // F rows beyond npp H (padding read by the main kernel) are zero
   for (int64_t e = tid; e < K::FPAIR; e += NT) {

@@ -80,6 +80,7 @@ EXPORTED = (
     "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
     "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc", "f_intor_df1c", "f_intor_eri2c", "f_intor_df2c_mo",
     "f_benchmark_int3c2e", "dfhip_metric_powerh", "dfhip_int3c2e_estimates", "f_eval_densities",
+    "dfhip_autotune", "dfhip_load_tuning", "dfhip_tuning_choice", "dfhip_tuning_contract",
 )
 
 
@@ -127,6 +128,15 @@ def lib():
         L.f_int_schwarz_aux.argtypes = [i32, i32, vp, vp, vp, vp]
         L.f_int_schwarz_bra.argtypes = [i32, i32, vp, vp, vp, vp]
         L.f_eval_densities.argtypes = [i32, i32, vp, vp, vp, i32, vp, i32, i32, vp, vp, i32, ctypes.c_double]
+        L.dfhip_autotune.restype = ctypes.c_int
+        L.dfhip_autotune.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis), i32, i32, vp,
+                                     ctypes.POINTER(Options), ctypes.c_char_p]
+        L.dfhip_load_tuning.restype = ctypes.c_int
+        L.dfhip_load_tuning.argtypes = [ctypes.c_char_p]
+        L.dfhip_tuning_choice.restype = i32
+        L.dfhip_tuning_choice.argtypes = [i32, i32, i32, i32]
+        L.dfhip_tuning_contract.restype = i32
+        L.dfhip_tuning_contract.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 

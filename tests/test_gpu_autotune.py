@@ -1,0 +1,91 @@
+"""The autotuner (include/dfhip.h, dfhip_autotune) on the 200-atom recipe
+(orbital s-g, aux s-h, 20 states, QVL 1e-10) at 10 atoms, mixed precision:
+
+* the table it writes covers every class with both a generated and a
+  matrix-core kernel (FP64 window) and the contraction at 210 densities, and
+  each choice agrees with the times it recorded (the rule's family unless the
+  other is at least 2 % faster);
+* the Coulomb tensor with the table loaded from the file agrees with the
+  untuned one within 1e-9 (a class's FP32 window may move between the
+  generated and the matrix-core FP32 kernels, which round differently) and
+  with the FP64 oracle within 1e-8.
+The table is cleared afterwards, so the tests after this one run untuned.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pysisyphus_addons_amd import _native as N
+from pysisyphus_addons_amd import synthetic, tuning
+from pysisyphus_addons_amd.diabatization import _intor
+
+pytestmark = pytest.mark.gpu
+
+GEN3 = {"la_max": 2, "lc_max": 3}  # classes with a generated kernel: la <= 2, lc <= 3
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _run(shells, aux, dens):
+    b = N.BasisArrays(*shells.as_arrays())
+    x = N.BasisArrays(*aux.as_arrays())
+    opt = N.default_options(precision=N.MIXED, screening=N.SCREEN_QVL, thresh=1e-10)
+    _, J, st = _intor.contract(b, x, dens, nstates=20, want_df=False, want_j=True, options=opt)
+    return J, st
+
+
+def _rows(path):
+    ints, cons = [], []
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] == "int3c2e":
+            ints.append((tuple(int(v) for v in f[1:6]), float(f[6]), float(f[7])))
+        elif f[0] == "contract":
+            cons.append((int(f[1]), int(f[2]), float(f[3]), float(f[4])))
+    return ints, cons
+
+
+def test_autotune_table_and_results(tmp_path):
+    shells, aux, dens = synthetic.workload4_reduced(10)
+    J0, st0 = _run(shells, aux, dens)
+    path = tmp_path / "tune.txt"
+    try:
+        tab = tuning.autotune(shells, aux, dens, path=str(path), precision=N.MIXED, screening=N.SCREEN_QVL,
+                              thresh=1e-10)
+        ints, cons = _rows(path)
+        _say(f"autotune: {len(ints)} class entries, contraction {cons}")
+        for (la, lb, lc, w, fam), tg, tm in ints:
+            _say(f"  ({la}{lb}|{lc}) window {w}: family {fam}, generated {tg:.3f} ms, matrix cores {tm:.3f} ms")
+        lmax_aux = max(s.L for s in aux.shells)
+        want = {(la, lb, lc) for la in range(GEN3["la_max"] + 1) for lb in range(la + 1)
+                for lc in range(min(GEN3["lc_max"], lmax_aux) + 1)}
+        got = {k[:3] for k, _, _ in ints if k[3] == 0}
+        assert want == got
+        for (la, lb, lc, w, fam), tg, tm in ints:
+            if w == 0:
+                rule_mm = la == 2 and lb == 2 and lc >= 2
+                assert fam == (2 if (not tg < 0.98 * tm if rule_mm else tm < 0.98 * tg) else 1)
+            else:
+                assert fam == (2 if tm < 0.98 * tg else 1)
+        assert len(cons) == 1 and cons[0][0] == 210 and cons[0][1] in (1, 2, 3, 4, 5, 7)
+        assert cons[0][2] in (0.5, 1.0, 2.0)
+        assert tab["contract"] == (cons[0][1], cons[0][2])
+        # a fresh load of the file gives the same table, and the tuned run the same tensor
+        tuning.load(None)
+        tuning.load(str(path))
+        assert tuning.table(210) == tab
+        J1, st1 = _run(shells, aux, dens)
+        assert st1["ncomputed"] == st0["ncomputed"]
+        d = np.abs(J1 - J0).max()
+        _say(f"tuned vs untuned: max |dJ| {d:.2e}; tuned {st1['ms_total']:.1f} ms, untuned {st0['ms_total']:.1f} ms")
+        assert d <= 1e-9
+        Jo = O.contract_coulomb_densities_4d(shells, aux, dens, screening="qvl")
+        assert np.abs(J1 - Jo).max() <= 1e-8
+    finally:
+        tuning.load(None)
+    assert tuning.table() == {"int3c2e": {}, "contract": None}

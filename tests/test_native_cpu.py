@@ -113,3 +113,35 @@ def test_invalid_basis_reports_status():
     with pytest.raises(N.DfhipError) as e:
         _intor.contract_coulomb_densities_4d(*a, *x, np.ascontiguousarray(dens[:, :-1, :-1]))
     assert e.value.status == 1
+
+
+def test_tuning_file_roundtrip(tmp_path):
+    """The autotuner's table file (include/dfhip.h, dfhip_load_tuning) loads
+    without a device, answers per class and window, and is rejected when
+    malformed; loading None clears it."""
+    from pysisyphus_addons_amd import tuning
+
+    f = tmp_path / "tune.txt"
+    f.write_text("# comment\n"
+                 "int3c2e 2 1 3 0 2 0.5 0.25\n"
+                 "int3c2e 2 2 2 1 1 0.1 0.2\n"
+                 "contract 210 5 2.000 12.5\n")
+    try:
+        tuning.load(str(f))
+        t = tuning.table(210)
+        assert t["int3c2e"] == {(2, 1, 3, 0): 2, (2, 2, 2, 1): 1}
+        assert t["contract"] == (5, 2.0)
+        assert tuning.table(36)["contract"] is None
+        assert N.lib().dfhip_tuning_choice(1, 2, 0, 0) == -1  # lb > la
+        bad = tmp_path / "bad.txt"
+        bad.write_text("int3c2e 2 1 3 0 7\n")
+        with pytest.raises(N.DfhipError, match="malformed"):
+            tuning.load(str(bad))
+        bad.write_text("contract 210 6 1.0\n")  # no tile of 6 densities per thread
+        with pytest.raises(N.DfhipError, match="malformed"):
+            tuning.load(str(bad))
+        # a failed load keeps the previous table
+        assert tuning.table()["int3c2e"] == {(2, 1, 3, 0): 2, (2, 2, 2, 1): 1}
+    finally:
+        tuning.load(None)
+    assert tuning.table(210) == {"int3c2e": {}, "contract": None}
