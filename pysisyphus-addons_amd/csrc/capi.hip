@@ -1495,6 +1495,10 @@ double erfinv_pos(double y) {
 // buffers are the records the 3c2e kernels test each triplet against.
 struct QvlRecords {
   DBuf<double> pair, aux;
+  // per aux L: the kept entries of every pair's prefix (qvl_compact), for the
+  // matrix-core kernels
+  DBuf<int64_t> koff[LMAX_AUX + 1];
+  DBuf<int> kidx[LMAX_AUX + 1];
 };
 
 // Aux records [extent, beta, Q_C, 0] per aux shell (common/screening.f90:107-115):
@@ -1545,6 +1549,14 @@ void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int
     HC(hipMemcpyAsync(h.data(), ns.p, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
     HC(hipStreamSynchronize(st));
     std::copy(h.begin(), h.begin() + np, pl.nsurv[L].begin());
+    {
+      std::vector<int64_t> ko((size_t)np + 1, 0);
+      for (int p = 0; p < np; p++) ko[p + 1] = ko[p] + h[np + p];
+      upload(qr.koff[L], ko, st);
+      qr.kidx[L].alloc((size_t)std::max<int64_t>(ko[np], 1));
+      qvl_compact(qr.pair.p, np, daux, dalist[L].p, L, qr.aux.p, thresh, ns.p, qr.koff[L].p, qr.kidx[L].p, st);
+      HC(hipGetLastError());
+    }
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
         const int k = pcls(la, lb);
@@ -2057,7 +2069,16 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   ab.commit(st);
   QvlRecords qvl;
   const bool use_qvl = opt.screening == DFHIP_SCREEN_QVL;
-  if (use_qvl) apply_qvl(pl, dorb.view, daux.view, ps1.p, ps2.p, dalist, opt.thresh, qvl, st);
+  if (use_qvl) {
+    apply_qvl(pl, dorb.view, daux.view, ps1.p, ps2.p, dalist, opt.thresh, qvl, st);
+    if (opt.verbose >= 1) {
+      int64_t npre = 0;
+      for (int L = 0; L <= LMAX_AUX; L++)
+        for (int p = 0; p < pl.ps.n(); p++) npre += pl.nsurv[L][p];
+      vlog(opt, "QVL: %lld triplets kept, %lld in the screened prefixes.\n", (long long)pl.ncomputed,
+           (long long)npre);
+    }
+  }
   // screened prefixes, one flat array [L][pair] for the kernels and the
   // contraction's mask
   DBuf<int> dnsflat;
@@ -2320,6 +2341,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             pa.qvl = qvl.pair.p + (size_t)8 * p0;
             xa.qvl = qvl.aux.p;
             xa.thresh = opt.thresh;
+            pa.koff = qvl.koff[L].p + p0;  // read by the matrix-core kernels only
+            pa.kidx = qvl.kidx[L].p;
           }
           const bool mmk = use_mm[k][L] && etg[k].p;
           const Int3MLaunch& mml = c.int3m[la][lb][L];

@@ -22,6 +22,11 @@ struct PairArgs {
   int npairs;
   const double* qvl = nullptr;  // QVL pair records (8 doubles per pair), or null for Schwarz
   const int* nbeg = nullptr;    // first aux-list entry per pair (generated and LDS kernels; null = 0)
+  // QVL only, matrix-core kernels: the kept list entries of each pair's
+  // prefix, kidx[koff[p] .. koff[p + 1]) ascending (null: every prefix entry
+  // is computed, the dropped ones as zeros)
+  const int64_t* koff = nullptr;
+  const int* kidx = nullptr;
 };
 
 struct AuxArgs {

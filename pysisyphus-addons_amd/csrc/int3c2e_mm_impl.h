@@ -200,11 +200,56 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   const int pidx = (int)blockIdx.y * 8 + (int)(blockIdx.x & 7);
   if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
-  const int cchunk = xa.lo + (int)(blockIdx.x >> 3) * NC;
-  const int c0 = pa.nbeg ? max(cchunk, pa.nbeg[pidx]) : cchunk;
-  if (c0 >= min(cend, cchunk + NC)) return;
-  const int nc = min(cchunk + NC, cend) - c0;
+  const int chunk = (int)(blockIdx.x >> 3);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  __shared__ int cidx[NC];  // list entry of each aux shell of the chunk
+  int nc;
+  if (pa.kidx) {
+    // chunk = NC consecutive kept entries of the window [cbeg, cend); the
+    // dropped entries from this chunk's first kept one (the window start for
+    // chunk 0) to the next chunk's first kept one (the window end for the
+    // last) get zero blocks from this workgroup
+    const int cbeg = pa.nbeg ? max(xa.lo, pa.nbeg[pidx]) : xa.lo;
+    if (cbeg >= cend) return;
+    const int* kl = pa.kidx + pa.koff[pidx];
+    const int nk = (int)(pa.koff[pidx + 1] - pa.koff[pidx]);
+    auto lower = [&](int v) {
+      int lo = 0, hi = nk;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (kl[mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      return lo;
+    };
+    const int ja = lower(cbeg), jb = lower(cend);
+    const int j0 = ja + chunk * NC;
+    if (j0 >= jb && chunk > 0) return;
+    const int j1 = min(jb, j0 + NC);
+    nc = max(0, j1 - j0);
+    const int zb = chunk == 0 ? cbeg : kl[j0], ze = j1 < jb ? kl[j1] : cend;
+    const int64_t zrow = pa.row[pidx];
+    for (int e = tid; e < (ze - zb) * NAB * NCC; e += NT) {
+      const int i = zb + e / (NAB * NCC), r = e % (NAB * NCC), munu = r / NCC, m = r - munu * NCC;
+      int lo = j0, hi = j1;  // is entry i one of the chunk's kept entries?
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (kl[mid] < i) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < j1 && kl[lo] == i) continue;
+      out[(zrow + munu) * ldo + xa.col[i] + m] = 0.0;
+    }
+    if (nc == 0) return;
+    for (int c = tid; c < nc; c += NT) cidx[c] = kl[j0 + c];
+  } else {
+    const int cchunk = xa.lo + chunk * NC;
+    const int c0 = pa.nbeg ? max(cchunk, pa.nbeg[pidx]) : cchunk;
+    if (c0 >= min(cend, cchunk + NC)) return;
+    nc = min(cchunk + NC, cend) - c0;
+    for (int c = tid; c < nc; c += NT) cidx[c] = c0 + c;
+  }
+  __syncthreads();
   const int s1 = pa.s1[pidx], s2 = pa.s2[pidx];
   const T* Et0 = EtG + etoff[pidx];
 
@@ -234,12 +279,14 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     for (int e = tid; e < nsph_terms(LC); e += NT) tc[e] = sph->terms[x0 + e];
   }
   for (int c = tid; c < nc; c += NT) {
-    const int sh = xa.shell[c0 + c];
+    const int sh = xa.shell[cidx[c]];
     cshell[c] = sh;
     cxyz[3 * c + 0] = ket.xyz[3 * sh + 0];
     cxyz[3 * c + 1] = ket.xyz[3 * sh + 1];
     cxyz[3 * c + 2] = ket.xyz[3 * sh + 2];
-    ckeep[c] = triplet_kept(pa, xa, pidx, sh, LC, ket.xyz[3 * sh + 0], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2]);
+    // the compact lists hold kept entries only (the test of qvl_compact)
+    ckeep[c] = pa.kidx ? 1
+                       : triplet_kept(pa, xa, pidx, sh, LC, ket.xyz[3 * sh + 0], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2]);
   }
   for (int e = tid; e < HK * NCOLP; e += NT) W[e] = (T)0;  // padding rows and columns stay zero
 
@@ -438,7 +485,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
     if (nt >= NTL) break;
     const int col = 16 * nt + lr, c = col / NCC, m = col - c * NCC;
     if (c >= nc) continue;
-    const int64_t ocol = xa.col[c0 + c] + m;
+    const int64_t ocol = xa.col[cidx[c]] + m;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int munu = 16 * mt + Op::row(lk, r);

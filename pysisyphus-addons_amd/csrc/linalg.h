@@ -72,6 +72,10 @@ void qvl_pair_records(const DevShells& bra, const int* s1, const int* s2, const 
                       double* rec, hipStream_t st);
 void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int* alist, int nlist, int L,
                 const double* xrec, double thresh, int* nsurv, int* nkeep, int* nprimkeep, hipStream_t st);
+// kidx[koff[p] ..] = the kept list entries (ascending) of pair p's screened-in
+// prefix nsurv[p] of the aux list of angular momentum L
+void qvl_compact(const double* prec, int npairs, const DevShells& ket, const int* alist, int L, const double* xrec,
+                 double thresh, const int* nsurv, const int64_t* koff, int* kidx, hipStream_t st);
 
 // Schwarz screened prefixes (screen.hip) of every pair and aux list L into
 // nsflat[L * npairs + p]: qab per pair, qcs the Q_C of the lists (sorted

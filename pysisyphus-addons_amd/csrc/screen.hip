@@ -82,6 +82,27 @@ __global__ void qvl_screen_kernel(const double* __restrict__ prec, int npairs, D
   nprimkeep[p] = pc;
 }
 
+// The kept entries of every pair's screened-in prefix, in list order:
+// kidx[koff[p] ..] for pair p (koff: exclusive prefix sums of the kept
+// counts of qvl_screen_kernel).  The same test as there, one thread per pair.
+__global__ void qvl_compact_kernel(const double* __restrict__ prec, int npairs, DevShells ket,
+                                   const int* __restrict__ alist, int L, const double* __restrict__ xrec,
+                                   double thresh, const int* __restrict__ nsurv, const int64_t* __restrict__ koff,
+                                   int* __restrict__ kidx) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  double pr[6];
+  for (int k = 0; k < 6; k++) pr[k] = prec[8 * (size_t)p + k];
+  int* out = kidx + koff[p];
+  const int n = nsurv[p];
+  int j = 0;
+  for (int i = 0; i < n; i++) {
+    const int sh = alist[i];
+    if (qvl_keep(pr, xrec + 4 * (size_t)sh, ket.xyz[3 * sh], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2], L, thresh))
+      out[j++] = i;
+  }
+}
+
 // Schwarz screened prefixes (common/screening.f90 test Q_ab Q_C >= thresh,
 // the comparison of the host sweep): for pair p and aux list L (Q_C sorted
 // non-increasing in qcs[loff[L] .. loff[L] + nl[L])), the number of list
@@ -129,6 +150,13 @@ void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int*
   if (npairs <= 0) return;
   hipLaunchKernelGGL(qvl_screen_kernel, dim3((npairs + 127) / 128), dim3(128), 0, st, prec, npairs, ket, alist, nlist,
                      L, xrec, thresh, nsurv, nkeep, nprimkeep);
+}
+
+void qvl_compact(const double* prec, int npairs, const DevShells& ket, const int* alist, int L, const double* xrec,
+                 double thresh, const int* nsurv, const int64_t* koff, int* kidx, hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(qvl_compact_kernel, dim3((npairs + 127) / 128), dim3(128), 0, st, prec, npairs, ket, alist, L,
+                     xrec, thresh, nsurv, koff, kidx);
 }
 
 }  // namespace dfhip
