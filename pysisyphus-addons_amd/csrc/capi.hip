@@ -807,9 +807,9 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
   for (int k = 0; k < NPCLS; k++) poff[k + 1] += poff[k];
   DBuf<double>& part = sp.part;
   part.alloc(std::max<size_t>(poff[NPCLS], 1));
-  // scratch of the streamed kernels: all pairs of the largest class, at most 1 GiB
+  // scratch of the streamed kernels: all pairs of the largest class, at most 4 GiB
   DBuf<double>& scr = sp.scr;
-  scr.alloc(std::min<size_t>(scr_need, (size_t)1 << 27));
+  scr.alloc(std::min<size_t>(scr_need, (size_t)1 << 29));
   double* hp = lane.hpart.get<double>(poff[NPCLS]);
   // shipments: the last NSHIP - 1 non-empty classes one each, the classes
   // before them in the first (shipment i ends after class shipend[i])
