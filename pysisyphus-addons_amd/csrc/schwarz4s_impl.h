@@ -36,6 +36,7 @@ struct Cls4s {
   static constexpr int NRT1 = (H + 15) / 16;    // row tiles of V_pq
   static constexpr int NW = 4, NT = 64 * NW;
   static constexpr int TPW = (NRT1 + NW - 1) / NW;  // row tiles per wave
+  static constexpr int ZC = NCT < 2 ? NCT : 2;       // Z column tiles per pass (accumulator registers)
   static constexpr int NPPS = 9, NQS = NPPS * (NPPS + 1) / 2;
   static constexpr int GW = (L + 4) | 1;  // per quartet: g_0..g_L, then 2 PQ_x, 2 PQ_y, 2 PQ_z (odd stride)
   // LDS of the main kernel in doubles: R (even levels, pad), R (odd levels), Z, G
@@ -93,13 +94,13 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
 }
 
 template <int LA, int LB>
-__global__ __launch_bounds__(256) void schwarz4s_kernel(DevShells bra, const int* __restrict__ s1v,
+__global__ __launch_bounds__(256, 2) void schwarz4s_kernel(DevShells bra, const int* __restrict__ s1v,
                                                         const int* __restrict__ s2v,
                                                         const double* __restrict__ boys,
                                                         const double* __restrict__ scr, double* __restrict__ part) {
   using K = Cls4s<LA, LB>;
   constexpr int L = K::L, H = K::H, NR = K::NR, NAB = K::NAB, NT = K::NT, NW = K::NW, GW = K::GW;
-  constexpr int NCT = K::NCT, NS = K::NS, TPW = K::TPW, NRT1 = K::NRT1;
+  constexpr int NCT = K::NCT, NS = K::NS, TPW = K::TPW, NRT1 = K::NRT1, ZC = K::ZC;
   __shared__ double Rq[NR + 1];  // even levels of the current quartet, finally R; Rq[NR] = 0 pads h >= H
   __shared__ double Ro[K::NRO > 0 ? K::NRO : 1];  // odd levels
   __shared__ double Z[NAB * NAB];
@@ -245,38 +246,43 @@ __global__ __launch_bounds__(256) void schwarz4s_kernel(DevShells bra, const int
       }
       __syncthreads();  // R of this quartet read by all waves before the next quartet's levels
     }
-    // Z[n1][n2] += sum_h Et_pq[n1][h] V_pq[h][n2], one n1 tile at a time
+    // Z[n1][n2] += sum_h Et_pq[n1][h] V_pq[h][n2], one n1 tile and ZC n2
+    // tiles at a time (few accumulator registers besides V's)
     const double* Fpq = F + (size_t)pq * H * NAB;
     for (int a0 = 0; a0 < NCT; a0++) {
-      dbl4 zacc[NCT];
 #pragma unroll
-      for (int ch = 0; ch < NCT; ch++) zacc[ch] = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int cz = 0; cz < NCT; cz += ZC) {
+        dbl4 zacc[ZC];
 #pragma unroll
-      for (int tt = 0; tt < TPW; tt++) {
-        const int t = wave + NW * tt;
-        if (t < NRT1) {  // wave-uniform
+        for (int z = 0; z < ZC; z++) zacc[z] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int row = 16 * t + 4 * r + lk, n1 = 16 * a0 + lr;
-            const double ea = (row < H && n1 < NAB) ? Fpq[(size_t)row * NAB + n1] : 0.0;
-#pragma unroll
-            for (int ch = 0; ch < NCT; ch++)
-              zacc[ch] = __builtin_amdgcn_mfma_f64_16x16x4f64(ea, acc[tt][ch][r], zacc[ch], 0, 0, 0);
-          }
-        }
-      }
-      // the waves add their partial tiles one after another (a fixed order)
-      for (int wv = 0; wv < NW; wv++) {
-        if (wave == wv) {
-#pragma unroll
-          for (int ch = 0; ch < NCT; ch++)
+        for (int tt = 0; tt < TPW; tt++) {
+          const int t = wave + NW * tt;
+          if (t < NRT1) {  // wave-uniform
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-              const int n1 = 16 * a0 + lk + 4 * r, n2 = 16 * ch + lr;
-              if (n1 < NAB && n2 < NAB) Z[n1 * NAB + n2] += zacc[ch][r];
+              const int row = 16 * t + 4 * r + lk, n1 = 16 * a0 + lr;
+              const double ea = (row < H && n1 < NAB) ? Fpq[(size_t)row * NAB + n1] : 0.0;
+#pragma unroll
+              for (int z = 0; z < ZC; z++)
+                if (cz + z < NCT)
+                  zacc[z] = __builtin_amdgcn_mfma_f64_16x16x4f64(ea, acc[tt][cz + z][r], zacc[z], 0, 0, 0);
             }
+          }
         }
-        __syncthreads();
+        // the waves add their partial tiles one after another (a fixed order)
+        for (int wv = 0; wv < NW; wv++) {
+          if (wave == wv) {
+#pragma unroll
+            for (int z = 0; z < ZC; z++)
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int n1 = 16 * a0 + lk + 4 * r, n2 = 16 * (cz + z) + lr;
+                if (cz + z < NCT && n1 < NAB && n2 < NAB) Z[n1 * NAB + n2] += zacc[z][r];
+              }
+          }
+          __syncthreads();
+        }
       }
     }
   }
