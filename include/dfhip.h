@@ -277,8 +277,10 @@ int dfhip_boys(int64_t npts, const int32_t *n, const double *T, double *F);
  *   - per (la, lb, lc) class that has both a generated VALU kernel and a
  *     matrix-core kernel, and per precision window (0 = FP64, 1 = the FP32
  *     window of the mixed path), which of the two runs it;
- *   - for this density count, the densities per thread of the contraction's
- *     output tile (1-5 or 7) and the scale of its split-K depth (1/2, 1, 2).
+ *   - for this density count, the contraction's output tile (code 1-5 or 7:
+ *     the FP64 VALU kernel with 16 x code densities per tile; 102-107: the
+ *     FP64 matrix-core kernel with 32 x (code - 100)) and the scale of its
+ *     split-K depth (1/2, 1, 2).
  * One full contraction per candidate, timed with HIP events per class; the
  * rule's choice stays unless another is at least 2 % faster.  The table
  * replaces this process's table and is written to path (NULL: not written).
@@ -290,7 +292,7 @@ int dfhip_autotune(const dfhip_basis *bas, const dfhip_basis *aux, int32_t ndens
 int dfhip_load_tuning(const char *path);
 /* 1 generated, 2 matrix cores, 0 not tuned, -1 invalid arguments */
 int32_t dfhip_tuning_choice(int32_t la, int32_t lb, int32_t lc, int32_t window);
-/* 1 and *ipt, *split_scale when the table has an entry for ndens, else 0 */
+/* 1 and *ipt (tile code), *split_scale when the table has an entry for ndens, else 0 */
 int32_t dfhip_tuning_contract(int32_t ndens, int32_t *ipt, double *split_scale);
 
 #ifdef __cplusplus

@@ -3169,9 +3169,11 @@ Tuning autotune_host(const dfhip_basis& B, const dfhip_basis& X, int ndens, int 
   // the default width is kept unless another one is at least 2 % faster
   const int ipt0 = contract_ipt(ndens);
   std::vector<std::pair<int, double>> cand;
-  for (int ipt : {1, 2, 3, 4, 5, 7}) {
-    // tiles from a quarter to twice the densities (the default always)
-    if (ipt != ipt0 && (16 * ipt > 2 * std::max(ndens, 16) || 64 * ipt < ndens)) continue;
+  for (int ipt : {1, 2, 3, 4, 5, 7, 102, 103, 104, 105, 106, 107}) {
+    // tiles from a quarter to twice the densities (the default always): VALU
+    // codes are 16 densities per unit, matrix-core codes 100 + 32-density units
+    const int wd = ipt > 100 ? 32 * (ipt - 100) : 16 * ipt;
+    if (ipt != ipt0 && (wd > 2 * std::max(ndens, 16) || 4 * wd < ndens)) continue;
     TuneProbe p;
     p.ipt = ipt;
     cand.push_back({ipt, measure(p)});

@@ -302,13 +302,135 @@ __global__ __launch_bounds__(256) void gemm_tn_splitk_kernel(const double* __res
     }
 }
 
-// densities per thread of the contraction's output tile (the default; the
-// tuner may pick another of contract_ipt_ok)
-int contract_ipt(int ndens) { return ndens <= 16 ? 1 : ndens <= 32 ? 2 : ndens <= 48 ? 3 : ndens <= 64 ? 4 : 7; }
-bool contract_ipt_ok(int ipt) { return (ipt >= 1 && ipt <= 5) || ipt == 7; }
+// The same contraction on the FP64 matrix cores (v_mfma_f64_16x16x4):
+// output tile MP = 128 columns P x 32 NIT densities per workgroup of 8 waves;
+// wave w takes the column tiles 2 (w % 4), 2 (w % 4) + 1 and the NIT density
+// tiles from NIT (w / 4) on, 2 NIT accumulator tiles held over the whole K
+// range.  K steps of MK = 16 rows staged in LDS as in gemm_tn_splitk_kernel
+// (the same masks, the next block's global loads issued before the current
+// block's products); per 4-row step a wave reads 2 + NIT fragments from LDS
+// for 2 NIT products, where the VALU kernel reads 8 + IPT values per 8 IPT
+// FMAs.  A[m][k] = T[k][P], lane l (m = l%16, k = l/16); B[k][n] = Dp[k][I],
+// lane l (k = l/16, n = l%16); register r of lane l holds the output
+// (P row l/16 + 4 r, density l%16) of its tile.
+typedef double gemm_dbl4 __attribute__((ext_vector_type(4)));
+constexpr int MP = 128, MK = 16;
+
+template <int NIT>
+__global__ __launch_bounds__(512) void gemm_tn_mfma_kernel(const double* __restrict__ T, int64_t ldt,
+                                                           const double* __restrict__ Dp, int ldd, int ncol,
+                                                           int ndens, int nseg, const int64_t* __restrict__ seg,
+                                                           int nsplit, const ScreenMask mk,
+                                                           double* __restrict__ part) {
+  constexpr int MI = 32 * NIT, NT = 512;
+  __shared__ double sT[MK][MP + 1];
+  __shared__ double sD[MK][MI + 1];
+  const int tcol = blockIdx.x;
+  const int tp = blockIdx.x * MP, ti = blockIdx.y * MI, s = blockIdx.z;
+  const int64_t* sg = seg + (size_t)tcol * (2 * nseg + 1);  // [total, lo0, hi0, lo1, hi1, ...]
+  const int64_t tot = sg[0];
+  const int64_t chunk = ((tot + nsplit - 1) / nsplit + MK - 1) / MK * MK;
+  int64_t q0 = (int64_t)s * chunk, q1 = min(tot, q0 + chunk);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int pt0 = 2 * (wave & 3), it0 = NIT * (wave >> 2);
+  // this lane always loads column tp + (tid % MP) of T: its screened prefix test
+  const int mcol = tp + (tid % MP);
+  const bool colok = mcol < ncol;
+  const int* __restrict__ ns_col = colok ? mk.nsurv + (size_t)mk.colL[mcol] * mk.npairs : mk.nsurv;
+  const int ia_col = colok ? mk.colia[mcol] : 0;
+  gemm_dbl4 acc[2][NIT];
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < NIT; i++) acc[j][i] = gemm_dbl4{0.0, 0.0, 0.0, 0.0};
+  int64_t base = 0;
+  for (int g = 0; g < nseg && q0 < q1; g++) {
+    const int64_t lo = sg[1 + 2 * g], hi = sg[2 + 2 * g], len = hi - lo;
+    if (q0 >= base + len) {
+      base += len;
+      continue;
+    }
+    const int64_t a0 = lo + (q0 - base), a1 = lo + min(len, q1 - base);
+    const int mcolc = colok ? mcol : ncol - 1;
+    constexpr int NZ = MK * MP / NT, NZD = (MK * MI + NT - 1) / NT;
+    double tv[NZ], dv[NZD];
+    int ns[NZ];
+    auto load = [&](int64_t kb) {
+#pragma unroll
+      for (int z = 0; z < NZ; z++) {
+        const int e = tid + NT * z, kk = e / MP;
+        const int64_t kc = min(kb + kk, a1 - 1);
+        tv[z] = T[kc * ldt + mcolc];
+        ns[z] = ns_col[mk.rowpair[kc]];
+      }
+#pragma unroll
+      for (int z = 0; z < NZD; z++) {
+        const int e = min(tid + NT * z, MK * MI - 1), kk = e / MI, c = e - kk * MI;
+        const int64_t kc = min(kb + kk, a1 - 1);
+        dv[z] = Dp[(int64_t)mk.rowperm[kc] * ldd + min(ti + c, ndens - 1)];
+      }
+    };
+    load(a0);
+    for (int64_t kb = a0; kb < a1; kb += MK) {
+#pragma unroll
+      for (int z = 0; z < NZ; z++) {
+        const int e = tid + NT * z, kk = e / MP, c = e - kk * MP;
+        const int64_t k = kb + kk;
+        // entries of screened-out triplets are never written: mask them here
+        const bool live = k < a1 && colok && ia_col < ns[z];
+        sT[kk][c] = live ? tv[z] : 0.0;
+      }
+#pragma unroll
+      for (int z = 0; z < NZD; z++) {
+        const int e = tid + NT * z, kk = e / MI, c = e - kk * MI;
+        if (e < MK * MI) sD[kk][c] = (kb + kk < a1 && ti + c < ndens) ? dv[z] : 0.0;
+      }
+      __syncthreads();
+      if (kb + MK < a1) load(kb + MK);
+#pragma unroll
+      for (int k4 = 0; k4 < MK / 4; k4++) {
+        const int kr = 4 * k4 + lk;
+        double a[2], b[NIT];
+#pragma unroll
+        for (int j = 0; j < 2; j++) a[j] = sT[kr][16 * (pt0 + j) + lr];
+#pragma unroll
+        for (int i = 0; i < NIT; i++) b[i] = sD[kr][16 * (it0 + i) + lr];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int i = 0; i < NIT; i++) acc[j][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[i], acc[j][i], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+    q0 = base + len;
+    base += len;
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < NIT; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int P = tp + 16 * (pt0 + j) + lk + 4 * r, I = ti + 16 * (it0 + i) + lr;
+        if (P < ncol && I < ndens) part[((size_t)s * ncol + P) * ndens + I] = acc[j][i][r];
+      }
+}
+
+// Output-tile codes of the contraction: 1-5 and 7 = densities per thread of
+// the VALU kernel (tiles of 16 x code densities), 100 + NIT (NIT = 2..7) =
+// the matrix-core kernel with tiles of 32 NIT densities.  The default: VALU
+// up to 64 densities, the matrix cores above.
+int contract_ipt(int ndens) {
+  if (ndens > 64) return 100 + std::min(7, (ndens + 31) / 32);
+  return ndens <= 16 ? 1 : ndens <= 32 ? 2 : ndens <= 48 ? 3 : 4;
+}
+bool contract_ipt_ok(int ipt) { return (ipt >= 1 && ipt <= 5) || ipt == 7 || (ipt >= 102 && ipt <= 107); }
+static int contract_width(int ipt) { return ipt > 100 ? 32 * (ipt - 100) : 16 * ipt; }
 
 int64_t contract_ntiles(int64_t ncol, int ndens, int ipt) {
-  const int gi = 16 * (contract_ipt_ok(ipt) ? ipt : contract_ipt(ndens));
+  const int code = contract_ipt_ok(ipt) ? ipt : contract_ipt(ndens);
+  const int gi = contract_width(code);
+  static_assert(GP == MP, "both kernels tile 128 columns");
   return ((ncol + GP - 1) / GP) * ((ndens + gi - 1) / gi);
 }
 
@@ -328,9 +450,21 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
                 const int64_t* seg, const ScreenMask& mk, double* part, int nsplit, const int* orig, double* d,
                 int64_t sP, int64_t sI, hipStream_t st, int ipt_req) {
   if (ncol <= 0) return;
-  const int ipt = contract_ipt_ok(ipt_req) ? ipt_req : contract_ipt(ndens), gi = 16 * ipt;
+  const int ipt = contract_ipt_ok(ipt_req) ? ipt_req : contract_ipt(ndens), gi = contract_width(ipt);
   dim3 grid((ncol + GP - 1) / GP, (ndens + gi - 1) / gi, nsplit);
+#define DFHIP_MFMA(N)                                                                                          \
+  case 100 + N:                                                                                                \
+    hipLaunchKernelGGL(gemm_tn_mfma_kernel<N>, grid, dim3(512), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg, \
+                       nsplit, mk, part);                                                                      \
+    break;
   switch (ipt) {
+    DFHIP_MFMA(2)
+    DFHIP_MFMA(3)
+    DFHIP_MFMA(4)
+    DFHIP_MFMA(5)
+    DFHIP_MFMA(6)
+    DFHIP_MFMA(7)
+#undef DFHIP_MFMA
     case 1:
       hipLaunchKernelGGL(gemm_tn_splitk_kernel<1>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
                          nsplit, mk, part);
@@ -351,7 +485,7 @@ void contract_T(const double* T, int64_t ldt, const double* Dp, int ldd, int nco
       hipLaunchKernelGGL(gemm_tn_splitk_kernel<5>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
                          nsplit, mk, part);
       break;
-    default:  // more than 64 densities: tiles of 112, T read half as often as with 64
+    default:  // 7: tiles of 112 densities
       hipLaunchKernelGGL(gemm_tn_splitk_kernel<7>, grid, dim3(256), 0, st, T, ldt, Dp, ldd, ncol, ndens, nseg, seg,
                          nsplit, mk, part);
   }

@@ -125,13 +125,15 @@ def test_tuning_file_roundtrip(tmp_path):
     f.write_text("# comment\n"
                  "int3c2e 2 1 3 0 2 0.5 0.25\n"
                  "int3c2e 2 2 2 1 1 0.1 0.2\n"
-                 "contract 210 5 2.000 12.5\n")
+                 "contract 210 5 2.000 12.5\n"
+                 "contract 36 103 1.000 1.5\n")
     try:
         tuning.load(str(f))
         t = tuning.table(210)
         assert t["int3c2e"] == {(2, 1, 3, 0): 2, (2, 2, 2, 1): 1}
         assert t["contract"] == (5, 2.0)
-        assert tuning.table(36)["contract"] is None
+        assert tuning.table(36)["contract"] == (103, 1.0)
+        assert tuning.table(20)["contract"] is None
         assert N.lib().dfhip_tuning_choice(1, 2, 0, 0) == -1  # lb > la
         bad = tmp_path / "bad.txt"
         bad.write_text("int3c2e 2 1 3 0 7\n")
