@@ -2099,6 +2099,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
   DBuf<int> dn64flat;  // FP64 windows of the prefixes, [L][pair] like dnsflat
+  DBuf<unsigned long long> dn32count;  // QVL: kept triplets of the FP32 windows
   // mixed precision: the low-precision kernel of each (class, aux L), 1 the
   // generated FP32 kernels, 2 the FP32 matrix-core kernels (0: FP64 only);
   // the split itself is final once the tables of the FP32 matrix-core
@@ -2235,6 +2236,16 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     for (int L = 0; L <= LMAX_AUX; L++)
       std::copy(pl.n64[L].begin(), pl.n64[L].end(), f.begin() + (size_t)L * pl.ps.n());
     upload_pinned(dn64flat, f, st);
+    // under QVL the windows [n64, nsurv) hold dropped entries too: count the
+    // kept ones on the device (read with the other results at the end)
+    if (use_qvl && pl.ps.n() > 0) {
+      dn32count.alloc(1);
+      HC(hipMemsetAsync(dn32count.p, 0, sizeof(unsigned long long), st));
+      for (int L = 0; L <= LMAX_AUX; L++)
+        if (!pl.alist[L].empty())
+          qvl_count_from(qvl.koff[L].p, qvl.kidx[L].p, dn64flat.p + (size_t)L * pl.ps.n(), pl.ps.n(), dn32count.p, st);
+      HC(hipGetLastError());
+    }
   }
   vstep("Et tables");
   const int GT = contract_tile();
@@ -2525,6 +2536,11 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     throw Error{DFHIP_ERR_NOT_SPD, "metric (P|Q) is not positive definite (leading minor " +
                                        std::to_string(lane.hinfo[0]) + ")"};
 
+  if (dn32count.p) {
+    unsigned long long n32 = 0;
+    HC(hipMemcpy(&n32, dn32count.p, sizeof(n32), hipMemcpyDeviceToHost));
+    pl.ncomputed32 = (int64_t)n32;
+  }
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->ntriplets = pl.ntriplets;

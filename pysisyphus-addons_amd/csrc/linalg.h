@@ -76,6 +76,9 @@ void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int*
 // prefix nsurv[p] of the aux list of angular momentum L
 void qvl_compact(const double* prec, int npairs, const DevShells& ket, const int* alist, int L, const double* xrec,
                  double thresh, const int* nsurv, const int64_t* koff, int* kidx, hipStream_t st);
+// *count += the kept entries of every pair at or after list entry nfrom[p]
+void qvl_count_from(const int64_t* koff, const int* kidx, const int* nfrom, int npairs, unsigned long long* count,
+                    hipStream_t st);
 
 // Schwarz screened prefixes (screen.hip) of every pair and aux list L into
 // nsflat[L * npairs + p]: qab per pair, qcs the Q_C of the lists (sorted

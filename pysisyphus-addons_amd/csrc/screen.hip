@@ -159,4 +159,29 @@ void qvl_compact(const double* prec, int npairs, const DevShells& ket, const int
                      xrec, thresh, nsurv, koff, kidx);
 }
 
+// *count += the kept entries kidx[koff[p] ..] at or after nfrom[p], over the
+// pairs (the kept triplets of the windows [nfrom, nsurv)).  One thread per pair.
+__global__ void qvl_count_from_kernel(const int64_t* __restrict__ koff, const int* __restrict__ kidx,
+                                      const int* __restrict__ nfrom, int npairs,
+                                      unsigned long long* __restrict__ count) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int* kl = kidx + koff[p];
+  const int nk = (int)(koff[p + 1] - koff[p]), v = nfrom[p];
+  int lo = 0, hi = nk;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (kl[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  if (nk > lo) atomicAdd(count, (unsigned long long)(nk - lo));
+}
+
+void qvl_count_from(const int64_t* koff, const int* kidx, const int* nfrom, int npairs, unsigned long long* count,
+                    hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(qvl_count_from_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st, koff, kidx, nfrom, npairs,
+                     count);
+}
+
 }  // namespace dfhip

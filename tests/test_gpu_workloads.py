@@ -100,6 +100,7 @@ def test_mixed_vs_oracle(key):
          f"low-precision triplets {st['ncomputed_fp32']} / {st['ncomputed']}")
     assert st["ncomputed"] == info["ncomputed"]
     _check_counts(st)
+    assert 0 < st["ncomputed_fp32"] <= st["ncomputed"]
     assert err.max() <= 1e-8
 
 
@@ -206,6 +207,7 @@ def test_workload4_full_properties():
     for s in (s64, sm):
         _check_counts(s)
     assert sm["ncomputed"] == s64["ncomputed"]
+    assert 0 < sm["ncomputed_fp32"] <= sm["ncomputed"] and s64["ncomputed_fp32"] == 0
     assert _eightfold(J64) and _eightfold(Jm)
     err = np.abs(Jm - J64).max()
     _say(f"workload 4: max |dJ| mixed vs FP64 {err:.2e} (|J|max {np.abs(J64).max():.3e}), low-precision "
