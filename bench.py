@@ -131,6 +131,42 @@ def cpu_baseline(shells, aux, wid, screening, ncomputed_full):
     return out
 
 
+def mfma_calibration():
+    """One grid-density call (4 atoms with s-d shells, 56 Cartesian
+    functions, 10,000 points in blocks of 100, one density, thresh 0) and the
+    exact number of v_mfma_f64_16x16x4f64 its density kernel issues: with
+    thresh 0 no 4 x 16 slice of the density is screened out, so per block and
+    64-point group it runs ceil((ncart - 16 t) / 4) K steps for every
+    16-column tile t, each on every 16-point row tile of the group."""
+    import numpy as np
+
+    from pysisyphus_addons_amd.grid import grid
+    from pysisyphus_addons_amd.shells import Shell, Shells
+
+    rng = np.random.default_rng(5)
+    shs = []
+    for ci in range(4):
+        A = rng.uniform(0.0, 6.0, 3)
+        for L in (0, 0, 1, 1, 2):
+            shs.append(Shell(L=L, center=A, exps=np.sort(rng.uniform(0.15, 6.0, 3))[::-1],
+                             coeffs=rng.uniform(0.2, 1.0, 3), center_ind=ci))
+    sh = Shells(shs)
+    nc = sum((s.L + 1) * (s.L + 2) // 2 for s in sh.shells)
+    D = rng.uniform(-1.0, 1.0, (1, nc, nc))
+    D = 0.5 * (D + D.transpose(0, 2, 1))
+    pts = rng.uniform(-2.0, 8.0, (10000, 3))
+    bs = 100
+    grid.eval_densities(sh, pts, D, blk_size=bs, thresh=0.0)
+    n = 0
+    for b0 in range(0, len(pts), bs):
+        cur = min(bs, len(pts) - b0)
+        for g0 in range(0, cur, 64):
+            nit = (min(64, cur - g0) + 15) // 16
+            n += nit * sum((nc - 16 * t + 3) // 4 for t in range((nc + 15) // 16))
+    print(json.dumps({"kernel": "grid_density_kernel", "ncart": nc, "mfma_f64_16x16x4": n, "flops": 2048 * n}),
+          flush=True)
+
+
 def bench_batch(args):
     """Batched form: 256 independent molecules (5-40 atoms, cube side
     4 N^(1/3), orbital 3s2p1d, aux 4s3p2d1f, 4 states) per step, FP64 and
@@ -195,6 +231,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 2 GiB device-to-device copy: the reference run that scales the FETCH_SIZE and "
                          "WRITE_SIZE counters of the roofline's traffic figure")
+    ap.add_argument("--mfma-calibration", action="store_true",
+                    help="only one f_eval_densities call whose density kernel issues a known number of FP64 "
+                         "MFMA instructions (printed): the reference run for the unit of "
+                         "SQ_INSTS_VALU_MFMA_MOPS_F64")
     ap.add_argument("--workload", type=int, default=4, choices=(1, 2, 4),
                     help="4: 200 atoms s-g / aux s-h, 20 states, QVL 1e-10, densities generated on the device "
                          "(default); 1: 60 atoms s-d / aux s-f, 8 states; 2: 60 atoms s-f / aux s-g, 8 states")
@@ -217,6 +257,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copied_bytes": x.numel() * 8, "check": float(y[-1])}), flush=True)
         return None
+    if args.mfma_calibration:
+        return mfma_calibration()
     if args.batch:
         return bench_batch(args)
 
