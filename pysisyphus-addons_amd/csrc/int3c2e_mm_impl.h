@@ -10,18 +10,20 @@
 //     the pairs that have screened-in triplets to a global table, h-major and
 //     padded to MFMA fragments: EtG[ij][h < HK][mu nu < NABP];
 //   * int3c2e_mm_kernel takes one shell pair x NC aux shells of one L per
-//     workgroup (8 waves): per primitive pair and aux primitive it builds the
-//     Boys values, R_tuv and the Hobson-transformed W[h][(C, m)] for all NC
-//     shells in LDS, then the waves run the (NAB x H) . (H x NC nsph(lc))
-//     product on v_mfma_f64_16x16x4 with A fragments read straight from the
-//     global Et table and B fragments from LDS; the accumulators stay in
-//     registers over all primitive pairs.
-// Both kernels exist for T = double (the FP64 path) and T = float (the
-// mixed path's low-precision window): with T = float the Et table, R_tuv,
-// the Hobson step and the product (v_mfma_f32_16x16x4f32) run in FP32, while
-// the Boys values and prefactors (the G table) stay FP64 and each primitive
-// pair's FP32 product is added into FP64 accumulators (FP32 sums over at most
-// nherm(la+lb) terms, FP64 across primitive pairs).
+//     workgroup: per primitive pair and aux primitive each wave builds R_tuv
+//     of its own aux shells by levels in LDS (synchronized within the wave
+//     only, so one wave's levels run beside the other waves' products), the
+//     workgroup builds the Hobson-transformed W[h][(C, m)] of all NC shells,
+//     then the waves run the (NAB x H) . (H x NC nsph(lc)) product on the
+//     matrix cores with A fragments read straight from the global Et table
+//     and B fragments from LDS; the accumulators stay in registers over all
+//     primitive pairs.  Two workgroup barriers per primitive pair.
+// Both kernels exist for T = double (the FP64 path: 8 waves, one workgroup
+// per CU, v_mfma_f64_16x16x4) and T = float (the mixed path's low-precision
+// window: 4 waves, two workgroups per CU, v_mfma_f32_16x16x4f32): with
+// T = float the Et table, R_tuv, the Hobson step, the product and its
+// accumulation over the primitive pairs run in FP32, while the Boys values
+// and prefactors (the G table) stay FP64; the block is stored as FP64.
 #pragma once
 
 #include "gen3c_common.h"
@@ -56,6 +58,14 @@ struct MMOp<float> {
   static __host__ __device__ constexpr int row(int lk, int r) { return 4 * lk + r; }
 };
 
+// LDS writes of a wave visible to all its lanes (LDS operations of one wave
+// complete in order; the fences keep the compiler from moving accesses across)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int LA, int LB, int LC, typename T = double>
 struct Cls3m {
   static constexpr int LAB = LA + LB, L = LAB + LC;
@@ -69,27 +79,38 @@ struct Cls3m {
   // R^(n)_tuv by levels n = L-1 .. 0 in two buffers: even levels in R (all NR
   // indices), odd levels in Rt (orders <= L-1); odd strides per aux shell
   static constexpr int NRP = NR | 1, NRT = nherm(L - 1) | 1;
-  static constexpr int BUD = 18500 * 8;   // LDS bytes for R, Rt, G and W
+  // FP64: one workgroup of 8 waves per CU.  FP32: the tables take half the
+  // LDS, so two workgroups of 4 waves share a CU (each wave keeps the 256
+  // registers of 2 waves per SIMD) and one's barriers overlap the other's
+  // products
+  static constexpr int OCC = sizeof(T) == sizeof(double) ? 1 : 2;
+  static constexpr int NW = OCC == 1 ? 8 : 4, NT = 64 * NW;
+  static constexpr int BUD = OCC == 1 ? 18500 * 8 : 160 * 1024 / OCC - 6 * 1024;  // LDS bytes for R, Rt, G and W
   // G of up to NPG primitive pairs is built once per workgroup (uncontracted aux)
   static constexpr int NPG = 9;
   static constexpr int nc_fit(int nc) {
     return (int)((nc * (NRP + NRT) + HK * ((nc * NCC + 15) / 16 * 16)) * sizeof(T) + nc * NPG * GWP * 8) <= BUD;
   }
-  // accumulator tiles per wave at nc aux shells (at most 4: the FP64
-  // accumulators of more tiles, next to the A fragments, exceed the
-  // registers of 2 waves per SIMD)
+  // accumulator tiles per wave at nc aux shells.  OCC 1: wave w takes row
+  // tile w / GRP and every GRP-th column tile, at most 4 (the FP64
+  // accumulators of more tiles, next to the A fragments, exceed the registers
+  // of 2 waves per SIMD).  OCC 2: the MT x NTL tiles in row-major order, an
+  // equal run of at most 8 per wave
   static constexpr int npw_of(int nc) {
-    const int ntl = (nc * NCC + 15) / 16, grp = 8 / MT >= 1 ? 8 / MT : 1;
-    return (ntl + grp - 1) / grp;
+    const int ntl = (nc * NCC + 15) / 16, grp = NW / MT >= 1 ? NW / MT : 1;
+    return OCC == 1 ? (ntl + grp - 1) / grp : (MT * ntl + NW - 1) / NW;
   }
   static constexpr int nc_max() {
     int nc = 32;
-    while (nc > 1 && (!nc_fit(nc) || npw_of(nc) > 4)) nc--;
+    while (nc > 1 && (!nc_fit(nc) || npw_of(nc) > (OCC == 1 ? 4 : 8))) nc--;
     return nc;
   }
   static constexpr int NC = nc_max();
   static constexpr int NCOL = NC * NCC, NTL = (NCOL + 15) / 16, NCOLP = NTL * 16;
-  static constexpr int NW = 8, NT = 64 * NW;
+  // OCC 2: tiles per wave and the most row tiles one wave's run touches
+  static constexpr int TPW = (MT * NTL + NW - 1) / NW;
+  static constexpr int RPM = (TPW + NTL - 2) / NTL + 1;
+  static constexpr int NCW = (NC + NW - 1) / NW;  // aux shells per wave in the R levels
   // wave w works on row tile w / GRP with the column tiles w % GRP, w % GRP + GRP,
   // ...: it loads its A fragments once per primitive pair and reuses them
   static constexpr int GRP = NW / MT >= 1 ? NW / MT : 1;
@@ -100,12 +121,66 @@ struct Cls3m {
   static constexpr int CG = NT / (H * NCC) >= 1 ? NT / (H * NCC) : 1;
   static constexpr int LH = L / 2 + 1;
   static constexpr int64_t ETPP = (int64_t)HK * NABP;  // Et table doubles per primitive pair
-  // independent entries per thread in the R levels and the Hobson step, and
-  // where the A fragments are loaded (measured per precision on the 200-atom
-  // g/h workload: FP64 4 and late, FP32 2 and early)
-  static constexpr int UR = sizeof(T) == sizeof(double) ? 4 : 2;
-  static constexpr bool ALATE = sizeof(T) == sizeof(double);
+  // rounds of independent entries per lane in flight in the R levels and
+  // aux shells per thread in the Hobson step
+  static constexpr int UR = 8;
   static constexpr bool OK = nc_fit(NC);
+};
+
+// R levels of one wave's aux shells (c = wave, wave + NW, ...; at most NCW
+// of them), level N down to 0.  Level N covers the orders t+u+v <= L-N (level
+// 0 only those >= LC, which the Hobson step reads), from level N+1 in the
+// other buffer:
+//   R^(N)_tuv = 2 (t-1) R^(N+1)_{t-2,u,v} + 2 PC_x R^(N+1)_{t-1,u,v}
+// along the first non-zero index (rec[h]: parents, direction, coefficient;
+// rec[0] is a harmless dummy), R^(N)_000 = g_N, in the scale of the G table.
+// Every level has a compile-time size, so its rounds of 64 entries are
+// straight-line code: UR rounds at a time issue their reads together, entries
+// past the end or of shells >= nc read clamped indices and skip the store.
+template <int L, int NW, int NCW, int NRP, int NRT, int GWP, int NRLO, int UR, typename T>
+struct MMLevels {
+  template <int N>
+  static __device__ __forceinline__ void run(T* __restrict__ R, T* __restrict__ Rt, const int* __restrict__ rec,
+                                             const double* __restrict__ gb, int lane, int wave, int nc) {
+    static_assert(N >= 0 && N < L, "R levels run from L-1 down to 0");
+    constexpr bool EVEN = N % 2 == 0;
+    const T* So = EVEN ? Rt : R;  // level N+1
+    T* Sn = EVEN ? R : Rt;
+    constexpr int SO = EVEN ? NRT : NRP, SN = EVEN ? NRP : NRT;
+    constexpr int HB = N == 0 ? NRLO : 0, NE = nherm(L - N) - HB, TOT = NE * NCW, NRND = (TOT + 63) / 64;
+#pragma unroll
+    for (int r0 = 0; r0 < NRND; r0 += UR) {
+      T val[UR];
+      int dst[UR];
+      bool live[UR];
+#pragma unroll
+      for (int u = 0; u < UR; u++) {
+        const int r = r0 + u;
+        if (r >= NRND) break;
+        const int k0 = lane + 64 * r;
+        const bool ok = r < NRND - 1 || k0 < TOT;
+        const int k = ok ? k0 : TOT - 1;
+        const int e = k / NCW, cl = k - e * NCW, h = HB + e;  // compile-time divisor
+        const int c = wave + NW * cl;
+        live[u] = ok && c < nc;
+        const int cc = c < nc ? c : 0;
+        const double* gc = gb + cc * GWP;
+        const int rc = rec[h];
+        const int h1 = rc & 1023, h2 = (rc >> 10) & 1023, dir = (rc >> 20) & 3, cf = rc >> 22;
+        const T* sc0 = So + cc * SO;
+        const T v = fma((T)(2 * cf), sc0[h2], (T)gc[L + 1 + dir] * sc0[h1]);
+        val[u] = h == 0 ? (T)gc[N] : v;
+        dst[u] = cc * SN + h;
+      }
+#pragma unroll
+      for (int u = 0; u < UR; u++) {
+        if (r0 + u >= NRND) break;
+        if (live[u]) Sn[dst[u]] = val[u];
+      }
+    }
+    wave_lds_sync();
+    if constexpr (N > 0) run<N - 1>(R, Rt, rec, gb, lane, wave, nc);
+  }
 };
 
 // Et tables of the listed pairs: EtG[off[p] + ij*ETPP + h*NABP + mu*NB + nu],
@@ -171,7 +246,7 @@ __global__ __launch_bounds__(256) void int3c2e_et_kernel(DevShells bra, const in
 // [nbeg, nsurv) of the pair (nbeg null: from the start), T the arithmetic
 // type of R_tuv, the Hobson step and the product (see the file head).
 template <int LA, int LB, int LC, typename T>
-__global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
+__global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
                                                          const T* __restrict__ EtG,
                                                          const int64_t* __restrict__ etoff,
                                                          const SphTable* __restrict__ sph,
@@ -182,7 +257,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   constexpr int L = K::L, H = K::H, HK = K::HK, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NABP = K::NABP;
   constexpr int NT = K::NT, NC = K::NC, NCOLP = K::NCOLP, GWP = K::GWP;
   constexpr int NRP = K::NRP, NRT = K::NRT;
-  constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL;
+  constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL, NW = K::NW, NCW = K::NCW;
   __shared__ double G[K::NPG * NC * GWP];  // per primitive pair and aux shell: scaled Boys values, then 2 PC_d
   __shared__ T R[NC * NRP];       // per aux shell: R^(n)_tuv of the even levels, finally R_tuv
   __shared__ T Rt[NC * NRT];      // per aux shell: R^(n)_tuv of the odd levels
@@ -260,6 +335,7 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   }
   // R^(n)_tuv = 2 (t-1) R^(n+1)_{t-2,u,v} + 2 PC_x R^(n+1)_{t-1,u,v} along the first
   // non-zero index (here in the scale of the G table: R^(n)_000 = g_n)
+  if (tid == 0) rec[0] = 0;  // dummy parents of h = 0 (its value is g_n)
   for (int h = 1 + tid; h < NR; h += NT) {
     int t, u, v;
     herm_tuv(h, t, u, v);
@@ -357,127 +433,181 @@ __global__ __launch_bounds__(512) void int3c2e_mm_kernel(DevShells bra, DevShell
   }
 
   const int mt = wave / GRP, gi = wave % GRP;  // mt >= MT: this wave only helps with the tables
-  dbl4m acc[NPW];
+  dbl4m acc[K::OCC == 2 ? 1 : NPW];
+  flt4m acc32[K::OCC == 2 ? K::TPW : 1];
 #pragma unroll
-  for (int t = 0; t < NPW; t++) acc[t] = dbl4m{0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < (K::OCC == 2 ? 1 : NPW); t++) acc[t] = dbl4m{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < (K::OCC == 2 ? K::TPW : 1); t++) acc32[t] = flt4m{0.0f, 0.0f, 0.0f, 0.0f};
 
+  // OCC 2: A fragments of one chunk of 8 K steps for the row tiles
+  // mt0 .. mt0 + RPM - 1 (clamped), double-buffered across chunks.  The Et
+  // table is read-only: without the opaque offset the compiler hoists every
+  // chunk's loads above the barriers of the table phase and spills them
+  T ak[K::OCC == 2 ? 2 : 1][K::OCC == 2 ? K::RPM : 1][8];
+  auto load_a = [&](T (&dst)[K::OCC == 2 ? K::RPM : 1][8], const T* Etb, int ch, int mt0) {
+    int zo = 0;
+    asm volatile("" : "+s"(zo));
+#pragma unroll
+    for (int r = 0; r < (K::OCC == 2 ? K::RPM : 1); r++) {
+      const int mtr = min(mt0 + r, MT - 1);
+      const T* ea = Etb + zo + lk * NABP + 16 * mtr + lr;
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        if (8 * ch + s < KS) dst[r][s] = ea[4 * (8 * ch + s) * NABP];
+    }
+  };
+  __syncthreads();  // prologue tables, G, zeroed W
   for (int ij = 0; ij < npp; ij++) {
     const int i = ij / npb, j = ij - i * npb;
     const double a = bra.exps[pa0 + i], b = bra.exps[pb0 + j], p = a + b;
     double P[3];
     for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
     const T* Et = Et0 + (int64_t)ij * K::ETPP;
-    // FP32: the A fragments of this primitive pair are requested before the R
-    // and Hobson steps, whose barriers cover their memory latency (FP64: right
-    // before the product, which keeps the R steps' registers free for their
-    // four independent entries per thread)
-    T av[KS];
-    if constexpr (!K::ALATE) {
-      if (mt < MT) {
-        const T* ea = Et + lk * NABP + 16 * mt + lr;
-#pragma unroll
-        for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
-      }
-    }
+    // OCC 1 (FP64): all A fragments of the wave's row tile, requested right
+    // before the product (the R levels keep their registers until then)
+    T av[K::OCC == 2 ? 1 : KS];
     for (int kp = 0; kp < xa.kmax; kp++) {
-      __syncthreads();  // tables built (first pass); previous product done with W
       const double* gb = pre ? G + ij * NC * GWP : G;
-      if (pre) {
-        for (int c = tid; c < nc; c += NT) seed(c, gb);
-      } else {
+      if (!pre) {
+        // G of this (primitive pair, aux primitive) for all shells: every
+        // wave is past the previous Hobson step, so G is free
         for (int c = tid; c < nc; c += NT) gboys(c, P, p, kp, G);
         for (int k = tid; k < nc * 3; k += NT) gpc(k / 3, k % 3, P, G);
         __syncthreads();
-        for (int c = tid; c < nc; c += NT) seed(c, gb);
       }
-      __syncthreads();
-      // levels n = L-1 .. 0: every entry of a level is independent (aux shells
-      // on lanes, odd strides); level n covers the orders t+u+v <= L-n
-#pragma unroll
-      for (int n = L - 1; n >= 0; n--) {
-        const T* So = (n % 2 == 0) ? Rt : R;  // level n+1
-        T* Sn = (n % 2 == 0) ? R : Rt;
-        const int so = (n % 2 == 0) ? NRT : NRP, sn = (n % 2 == 0) ? NRP : NRT;
-        // level 0: only the orders >= LC that the Hobson step reads
-        const int hbeg = n == 0 ? K::NRLO : 0, nent = nherm(L - n) - hbeg;
-#pragma unroll(K::UR)
-        for (int k = tid; k < nent * NC; k += NT) {
-          const int h = hbeg + k / NC, c = k - (h - hbeg) * NC;  // compile-time divisor
-          if (c >= nc) continue;
-          const double* gc = gb + c * GWP;
-          T val;
-          if (h == 0) {
-            val = (T)gc[n];
-          } else {
-            const int rc = rec[h];
-            const int h1 = rc & 1023, h2 = (rc >> 10) & 1023, dir = (rc >> 20) & 3, cf = rc >> 22;
-            const T* sc0 = So + c * so;
-            val = fma((T)(2 * cf), sc0[h2], (T)gc[L + 1 + dir] * sc0[h1]);
-          }
-          Sn[c * sn + h] = val;
-        }
-        if (n > 0) __syncthreads();
+      // R levels of the wave's own aux shells, synchronized within the wave
+      // only: a wave starts them as soon as its product of the previous
+      // primitive pair is issued, beside the other waves' products
+      if (lane < NCW) {
+        const int c = wave + NW * lane;
+        if (c < nc) seed(c, gb);
       }
+      wave_lds_sync();
+      // (L = 0: the seed g_0 is R itself)
+      if constexpr (L > 0)
+        MMLevels<L, NW, NCW, NRP, NRT, GWP, K::NRLO, K::UR, T>::template run<L - 1>(R, Rt, rec, gb, lane, wave, nc);
+      // R of every shell is there, and every wave's previous product is done
+      // with W
       __syncthreads();
+      if constexpr (K::OCC == 2) load_a(ak[0], Et, 0, wave * K::TPW / NTL);
       // W[h][c NCC + m] = sum_q c_q R_c[(t, u, v) + l_q] over the solid-harmonic terms
       // q of (LC, m): the term indices are resolved once per (h, m) and reused
-      // for every aux shell of the thread's group
-      for (int k = tid; k < H * NCC * K::CG; k += NT) {
-        const int hm = k % (H * NCC), cg = k / (H * NCC), h = hm / NCC, m = hm - h * NCC;
-        const int tv = tuv[h];
-        const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
-        const int q0 = sc[m], nq = sc[m + 1] - q0;
-        T cq[K::MAXT];
-        int iq[K::MAXT];
+      // for every aux shell of the thread's group.  Compile-time trip counts:
+      // shells c >= nc read the last live shell (finite values in columns no
+      // output reads), the tail item skips its stores.
+      {
+        constexpr int NHM = H * NCC, CG = K::CG, NIT = (NHM * CG + NT - 1) / NT, NCG = (NC + CG - 1) / CG;
 #pragma unroll
-        for (int z = 0; z < K::MAXT; z++) {
-          const SphTerm tm = tc[z < nq ? q0 + z : q0];
-          cq[z] = z < nq ? (T)tm.c : (T)0;
-          iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
-        }
-#pragma unroll(K::UR)
-        for (int c = cg; c < nc; c += K::CG) {
-          const T* Rc = R + c * NRP;
-          T s = (T)0;
+        for (int r = 0; r < NIT; r++) {
+          const int k0 = tid + NT * r;
+          const bool ok = r < NIT - 1 || k0 < NHM * CG;
+          const int k = ok ? k0 : NHM * CG - 1;
+          const int hm = k % NHM, cg = k / NHM, h = hm / NCC, m = hm - h * NCC;
+          const int tv = tuv[h];
+          const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
+          const int q0 = sc[m], nq = sc[m + 1] - q0;
+          T cq[K::MAXT];
+          int iq[K::MAXT];
 #pragma unroll
-          for (int z = 0; z < K::MAXT; z++) s = fma(cq[z], Rc[iq[z]], s);
-          W[h * NCOLP + c * NCC + m] = s;
+          for (int z = 0; z < K::MAXT; z++) {
+            const SphTerm tm = tc[z < nq ? q0 + z : q0];
+            cq[z] = z < nq ? (T)tm.c : (T)0;
+            iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
+          }
+          T* wr = W + h * NCOLP + m;
+#pragma unroll
+          for (int j0 = 0; j0 < NCG; j0 += K::UR) {
+            T sv[K::UR];
+#pragma unroll
+            for (int j = j0; j < j0 + K::UR && j < NCG; j++) {
+              const int c = min(cg + CG * j, nc - 1);
+              const T* Rc = R + c * NRP;
+              T sum = (T)0;
+#pragma unroll
+              for (int z = 0; z < K::MAXT; z++) sum = fma(cq[z], Rc[iq[z]], sum);
+              sv[j - j0] = sum;
+            }
+#pragma unroll
+            for (int j = j0; j < j0 + K::UR && j < NCG; j++) {
+              const int c = cg + CG * j;
+              if (ok && c < NC) wr[c * NCC] = sv[j - j0];
+            }
+          }
         }
       }
       __syncthreads();
-      // acc[nt] += Et[16 mt.., h] W[h, 16 nt..]: FP64 accumulates in place,
-      // FP32 per primitive pair then into the FP64 accumulators
-      // all A fragments of the wave's row tile in flight at once (one memory
-      // latency per primitive pair), then one accumulation chain per column
-      // tile; FP32 chains start from zero and are added into FP64
-      if (mt < MT) {
-        if constexpr (K::ALATE) {
+      // acc[tile] += Et[16 mt.., h] W[h, 16 nt..] over all primitive pairs.
+      // OCC 1: all A fragments of the wave's row tile in flight at once (one
+      // memory latency per primitive pair), then one FP64 chain per column
+      // tile.
+      if (K::OCC == 2 || mt < MT) {
+        if constexpr (K::OCC == 2) {
+          // the wave's run of tiles [tb, te), in chunks of 8 K steps: the A
+          // fragments of the next chunk (every row tile the run touches) are
+          // requested before this chunk's products, chunk 0 before the
+          // Hobson step; one FP32 accumulation chain per tile
+          constexpr int TPW = K::TPW, RPM = K::RPM, NCH = (KS + 7) / 8;
+          const int tb = wave * TPW, te = min(tb + TPW, MT * NTL), mt0 = tb / NTL;
+#pragma unroll
+          for (int ch = 0; ch < NCH; ch++) {
+            if (ch + 1 < NCH) load_a(ak[(ch + 1) & 1], Et, ch + 1, mt0);
+            const int s0 = 8 * ch;
+#pragma unroll
+            for (int q = 0; q < TPW; q++) {
+              const int tau = tb + q;
+              if (tau >= te) break;
+              const int mtq = tau / NTL, nt = tau - mtq * NTL;
+              const T* wb = W + lk * NCOLP + 16 * nt + lr;
+              flt4m c4 = acc32[q];
+#pragma unroll
+              for (int s = 0; s < 8; s++) {
+                if (s0 + s >= KS) break;
+                T a = ak[ch & 1][0][s];
+#pragma unroll
+                for (int r = 1; r < RPM; r++) a = (mtq == mt0 + r) ? ak[ch & 1][r][s] : a;
+                c4 = Op::mfma(a, wb[4 * (s0 + s) * NCOLP], c4);
+              }
+              acc32[q] = c4;
+            }
+          }
+        } else {
           const T* ea = Et + lk * NABP + 16 * mt + lr;
 #pragma unroll
           for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
-        }
 #pragma unroll
-        for (int t = 0; t < NPW; t++) {
-          const int nt = gi + GRP * t;
-          if (nt >= NTL) break;
-          const T* wb = W + lk * NCOLP + 16 * nt + lr;
-          if constexpr (sizeof(T) == sizeof(double)) {
+          for (int t = 0; t < NPW; t++) {
+            const int nt = gi + GRP * t;
+            if (nt >= NTL) break;
+            const T* wb = W + lk * NCOLP + 16 * nt + lr;
             dbl4m c4 = acc[t];
 #pragma unroll
             for (int s = 0; s < KS; s++) c4 = Op::mfma(av[s], wb[4 * s * NCOLP], c4);
             acc[t] = c4;
-          } else {
-            flt4m c4 = flt4m{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < KS; s++) c4 = Op::mfma(av[s], wb[4 * s * NCOLP], c4);
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc[t][r] += (double)c4[r];
           }
         }
       }
     }
   }
   const int64_t row0 = pa.row[pidx];
+  if constexpr (K::OCC == 2) {
+    const int tb = wave * K::TPW, te = min(tb + K::TPW, MT * NTL);
+#pragma unroll
+    for (int q = 0; q < K::TPW; q++) {
+      const int tau = tb + q;
+      if (tau >= te) break;
+      const int mtq = tau / NTL, nt = tau - mtq * NTL;
+      const int col = 16 * nt + lr, c = col / NCC, m = col - c * NCC;
+      if (c >= nc) continue;
+      const int64_t ocol = xa.col[cidx[c]] + m;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int munu = 16 * mtq + Op::row(lk, r);
+        if (munu < NAB) out[(row0 + munu) * ldo + ocol] = (double)acc32[q][r];
+      }
+    }
+    return;
+  }
   if (mt >= MT) return;
 #pragma unroll
   for (int t = 0; t < NPW; t++) {
