@@ -121,8 +121,10 @@ struct Cls3m {
   static constexpr int CG = NT / (H * NCC) >= 1 ? NT / (H * NCC) : 1;
   static constexpr int LH = L / 2 + 1;
   static constexpr int64_t ETPP = (int64_t)HK * NABP;  // Et table doubles per primitive pair
-  // rounds of independent entries per lane in flight in the R levels and
-  // aux shells per thread in the Hobson step
+  // rounds of independent entries per lane in flight in the R levels, and
+  // the unrolling of the Hobson step's loop over aux shells (measured on the
+  // 200-atom g/h workload; compile-time trip counts over all NC shells were
+  // slower there, the last chunks of a pair hold fewer)
   static constexpr int UR = 8;
   static constexpr bool OK = nc_fit(NC);
 };
@@ -493,47 +495,27 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kern
       if constexpr (K::OCC == 2) load_a(ak[0], Et, 0, wave * K::TPW / NTL);
       // W[h][c NCC + m] = sum_q c_q R_c[(t, u, v) + l_q] over the solid-harmonic terms
       // q of (LC, m): the term indices are resolved once per (h, m) and reused
-      // for every aux shell of the thread's group.  Compile-time trip counts:
-      // shells c >= nc read the last live shell (finite values in columns no
-      // output reads), the tail item skips its stores.
-      {
-        constexpr int NHM = H * NCC, CG = K::CG, NIT = (NHM * CG + NT - 1) / NT, NCG = (NC + CG - 1) / CG;
+      // for every aux shell of the thread's group
+      for (int k = tid; k < H * NCC * K::CG; k += NT) {
+        const int hm = k % (H * NCC), cg = k / (H * NCC), h = hm / NCC, m = hm - h * NCC;
+        const int tv = tuv[h];
+        const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
+        const int q0 = sc[m], nq = sc[m + 1] - q0;
+        T cq[K::MAXT];
+        int iq[K::MAXT];
 #pragma unroll
-        for (int r = 0; r < NIT; r++) {
-          const int k0 = tid + NT * r;
-          const bool ok = r < NIT - 1 || k0 < NHM * CG;
-          const int k = ok ? k0 : NHM * CG - 1;
-          const int hm = k % NHM, cg = k / NHM, h = hm / NCC, m = hm - h * NCC;
-          const int tv = tuv[h];
-          const int t = tv & 255, u = (tv >> 8) & 255, v = tv >> 16;
-          const int q0 = sc[m], nq = sc[m + 1] - q0;
-          T cq[K::MAXT];
-          int iq[K::MAXT];
+        for (int z = 0; z < K::MAXT; z++) {
+          const SphTerm tm = tc[z < nq ? q0 + z : q0];
+          cq[z] = z < nq ? (T)tm.c : (T)0;
+          iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
+        }
+#pragma unroll(K::UR)
+        for (int c = cg; c < nc; c += K::CG) {
+          const T* Rc = R + c * NRP;
+          T s = (T)0;
 #pragma unroll
-          for (int z = 0; z < K::MAXT; z++) {
-            const SphTerm tm = tc[z < nq ? q0 + z : q0];
-            cq[z] = z < nq ? (T)tm.c : (T)0;
-            iq[z] = herm_index(t + tm.lx, u + tm.ly, v + tm.lz);
-          }
-          T* wr = W + h * NCOLP + m;
-#pragma unroll
-          for (int j0 = 0; j0 < NCG; j0 += K::UR) {
-            T sv[K::UR];
-#pragma unroll
-            for (int j = j0; j < j0 + K::UR && j < NCG; j++) {
-              const int c = min(cg + CG * j, nc - 1);
-              const T* Rc = R + c * NRP;
-              T sum = (T)0;
-#pragma unroll
-              for (int z = 0; z < K::MAXT; z++) sum = fma(cq[z], Rc[iq[z]], sum);
-              sv[j - j0] = sum;
-            }
-#pragma unroll
-            for (int j = j0; j < j0 + K::UR && j < NCG; j++) {
-              const int c = cg + CG * j;
-              if (ok && c < NC) wr[c * NCC] = sv[j - j0];
-            }
-          }
+          for (int z = 0; z < K::MAXT; z++) s = fma(cq[z], Rc[iq[z]], s);
+          W[h * NCOLP + c * NCC + m] = s;
         }
       }
       __syncthreads();
