@@ -18,12 +18,12 @@
 //     matrix cores with A fragments read straight from the global Et table
 //     and B fragments from LDS; the accumulators stay in registers over all
 //     primitive pairs.  Two workgroup barriers per primitive pair.
-// Both kernels exist for T = double (the FP64 path: 8 waves, one workgroup
-// per CU, v_mfma_f64_16x16x4) and T = float (the mixed path's low-precision
-// window: 4 waves, two workgroups per CU, v_mfma_f32_16x16x4f32): with
-// T = float the Et table, R_tuv, the Hobson step, the product and its
-// accumulation over the primitive pairs run in FP32, while the Boys values
-// and prefactors (the G table) stay FP64; the block is stored as FP64.
+// Workgroups of 4 waves, two per CU for T = double (the FP64 path,
+// v_mfma_f64_16x16x4) and three for T = float (the mixed path's
+// low-precision window, v_mfma_f32_16x16x4f32, half the LDS): with T = float
+// the Et table, R_tuv, the Hobson step, the product and its accumulation over
+// the primitive pairs run in FP32, while the Boys values and prefactors (the
+// G table) stay FP64; the block is stored as FP64.
 #pragma once
 
 #include "gen3c_common.h"
@@ -79,42 +79,33 @@ struct Cls3m {
   // R^(n)_tuv by levels n = L-1 .. 0 in two buffers: even levels in R (all NR
   // indices), odd levels in Rt (orders <= L-1); odd strides per aux shell
   static constexpr int NRP = NR | 1, NRT = nherm(L - 1) | 1;
-  // FP64: one workgroup of 8 waves per CU.  FP32: the tables take half the
-  // LDS, so two workgroups of 4 waves share a CU (each wave keeps the 256
-  // registers of 2 waves per SIMD) and one's barriers overlap the other's
-  // products
-  static constexpr int OCC = sizeof(T) == sizeof(double) ? 1 : 2;
-  static constexpr int NW = OCC == 1 ? 8 : 4, NT = 64 * NW;
-  static constexpr int BUD = OCC == 1 ? 18500 * 8 : 160 * 1024 / OCC - 6 * 1024;  // LDS bytes for R, Rt, G and W
+  // Workgroups of 4 waves; OCC of them share a CU: two for FP64, three for
+  // FP32, whose tables take half the LDS (measured on the 200-atom g/h
+  // workload: one 8-wave FP64 workgroup per CU took 3.70 s of 3c2e time per
+  // step, two 4-wave ones 2.99 s; three for FP32 saved another 5 %, three for
+  // FP64 nothing).  One workgroup's barriers overlap another's products.
+  static constexpr int OCC = sizeof(T) == sizeof(double) ? 2 : 3;
+  static constexpr int NW = 4, NT = 64 * NW;
+  static constexpr int BUD = 160 * 1024 / OCC - 6 * 1024;  // LDS bytes for R, Rt, G and W
   // G of up to NPG primitive pairs is built once per workgroup (uncontracted aux)
   static constexpr int NPG = 9;
   static constexpr int nc_fit(int nc) {
     return (int)((nc * (NRP + NRT) + HK * ((nc * NCC + 15) / 16 * 16)) * sizeof(T) + nc * NPG * GWP * 8) <= BUD;
   }
-  // accumulator tiles per wave at nc aux shells.  OCC 1: wave w takes row
-  // tile w / GRP and every GRP-th column tile, at most 4 (the FP64
-  // accumulators of more tiles, next to the A fragments, exceed the registers
-  // of 2 waves per SIMD).  OCC 2: the MT x NTL tiles in row-major order, an
-  // equal run of at most 8 per wave
-  static constexpr int npw_of(int nc) {
-    const int ntl = (nc * NCC + 15) / 16, grp = NW / MT >= 1 ? NW / MT : 1;
-    return OCC == 1 ? (ntl + grp - 1) / grp : (MT * ntl + NW - 1) / NW;
-  }
+  // accumulator tiles per wave at nc aux shells: the MT x NTL tiles in
+  // row-major order, an equal run of at most 8 per wave
+  static constexpr int npw_of(int nc) { return (MT * ((nc * NCC + 15) / 16) + NW - 1) / NW; }
   static constexpr int nc_max() {
     int nc = 32;
-    while (nc > 1 && (!nc_fit(nc) || npw_of(nc) > (OCC == 1 ? 4 : 8))) nc--;
+    while (nc > 1 && (!nc_fit(nc) || npw_of(nc) > 8)) nc--;
     return nc;
   }
   static constexpr int NC = nc_max();
   static constexpr int NCOL = NC * NCC, NTL = (NCOL + 15) / 16, NCOLP = NTL * 16;
-  // OCC 2: tiles per wave and the most row tiles one wave's run touches
+  // tiles per wave and the most row tiles one wave's run touches
   static constexpr int TPW = (MT * NTL + NW - 1) / NW;
   static constexpr int RPM = (TPW + NTL - 2) / NTL + 1;
   static constexpr int NCW = (NC + NW - 1) / NW;  // aux shells per wave in the R levels
-  // wave w works on row tile w / GRP with the column tiles w % GRP, w % GRP + GRP,
-  // ...: it loads its A fragments once per primitive pair and reuses them
-  static constexpr int GRP = NW / MT >= 1 ? NW / MT : 1;
-  static constexpr int NPW = (NTL + GRP - 1) / GRP;
   static constexpr int KS = HK / 4;
   // Hobson step: one thread per (h, m) and group of aux shells
   static constexpr int MAXT = sph_max_terms(LC);
@@ -248,7 +239,7 @@ __global__ __launch_bounds__(256) void int3c2e_et_kernel(DevShells bra, const in
 // [nbeg, nsurv) of the pair (nbeg null: from the start), T the arithmetic
 // type of R_tuv, the Hobson step and the product (see the file head).
 template <int LA, int LB, int LC, typename T>
-__global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
+__global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), (Cls3m<LA, LB, LC, T>::OCC)) void int3c2e_mm_kernel(DevShells bra, DevShells ket, PairArgs pa, AuxArgs xa,
                                                          const T* __restrict__ EtG,
                                                          const int64_t* __restrict__ etoff,
                                                          const SphTable* __restrict__ sph,
@@ -259,7 +250,7 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kern
   constexpr int L = K::L, H = K::H, HK = K::HK, NR = K::NR, NCC = K::NCC, NAB = K::NAB, NABP = K::NABP;
   constexpr int NT = K::NT, NC = K::NC, NCOLP = K::NCOLP, GWP = K::GWP;
   constexpr int NRP = K::NRP, NRT = K::NRT;
-  constexpr int MT = K::MT, GRP = K::GRP, NPW = K::NPW, KS = K::KS, NTL = K::NTL, NW = K::NW, NCW = K::NCW;
+  constexpr int MT = K::MT, KS = K::KS, NTL = K::NTL, NW = K::NW, NCW = K::NCW;
   __shared__ double G[K::NPG * NC * GWP];  // per primitive pair and aux shell: scaled Boys values, then 2 PC_d
   __shared__ T R[NC * NRP];       // per aux shell: R^(n)_tuv of the even levels, finally R_tuv
   __shared__ T Rt[NC * NRT];      // per aux shell: R^(n)_tuv of the odd levels
@@ -434,24 +425,20 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kern
     }
   }
 
-  const int mt = wave / GRP, gi = wave % GRP;  // mt >= MT: this wave only helps with the tables
-  dbl4m acc[K::OCC == 2 ? 1 : NPW];
-  flt4m acc32[K::OCC == 2 ? K::TPW : 1];
+  typename Op::v4 acc[K::TPW];
 #pragma unroll
-  for (int t = 0; t < (K::OCC == 2 ? 1 : NPW); t++) acc[t] = dbl4m{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int t = 0; t < (K::OCC == 2 ? K::TPW : 1); t++) acc32[t] = flt4m{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < K::TPW; t++) acc[t] = typename Op::v4{(T)0, (T)0, (T)0, (T)0};
 
-  // OCC 2: A fragments of one chunk of 8 K steps for the row tiles
+  // A fragments of one chunk of 8 K steps for the row tiles
   // mt0 .. mt0 + RPM - 1 (clamped), double-buffered across chunks.  The Et
   // table is read-only: without the opaque offset the compiler hoists every
   // chunk's loads above the barriers of the table phase and spills them
-  T ak[K::OCC == 2 ? 2 : 1][K::OCC == 2 ? K::RPM : 1][8];
-  auto load_a = [&](T (&dst)[K::OCC == 2 ? K::RPM : 1][8], const T* Etb, int ch, int mt0) {
+  T ak[2][K::RPM][8];
+  auto load_a = [&](T (&dst)[K::RPM][8], const T* Etb, int ch, int mt0) {
     int zo = 0;
     asm volatile("" : "+s"(zo));
 #pragma unroll
-    for (int r = 0; r < (K::OCC == 2 ? K::RPM : 1); r++) {
+    for (int r = 0; r < K::RPM; r++) {
       const int mtr = min(mt0 + r, MT - 1);
       const T* ea = Etb + zo + lk * NABP + 16 * mtr + lr;
 #pragma unroll
@@ -466,9 +453,6 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kern
     double P[3];
     for (int d = 0; d < 3; d++) P[d] = (a * A[d] + b * B[d]) / p;
     const T* Et = Et0 + (int64_t)ij * K::ETPP;
-    // OCC 1 (FP64): all A fragments of the wave's row tile, requested right
-    // before the product (the R levels keep their registers until then)
-    T av[K::OCC == 2 ? 1 : KS];
     for (int kp = 0; kp < xa.kmax; kp++) {
       const double* gb = pre ? G + ij * NC * GWP : G;
       if (!pre) {
@@ -492,7 +476,7 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kern
       // R of every shell is there, and every wave's previous product is done
       // with W
       __syncthreads();
-      if constexpr (K::OCC == 2) load_a(ak[0], Et, 0, wave * K::TPW / NTL);
+      load_a(ak[0], Et, 0, wave * K::TPW / NTL);
       // W[h][c NCC + m] = sum_q c_q R_c[(t, u, v) + l_q] over the solid-harmonic terms
       // q of (LC, m): the term indices are resolved once per (h, m) and reused
       // for every aux shell of the thread's group
@@ -519,89 +503,51 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), 2) void int3c2e_mm_kern
         }
       }
       __syncthreads();
-      // acc[tile] += Et[16 mt.., h] W[h, 16 nt..] over all primitive pairs.
-      // OCC 1: all A fragments of the wave's row tile in flight at once (one
-      // memory latency per primitive pair), then one FP64 chain per column
-      // tile.
-      if (K::OCC == 2 || mt < MT) {
-        if constexpr (K::OCC == 2) {
-          // the wave's run of tiles [tb, te), in chunks of 8 K steps: the A
-          // fragments of the next chunk (every row tile the run touches) are
-          // requested before this chunk's products, chunk 0 before the
-          // Hobson step; one FP32 accumulation chain per tile
-          constexpr int TPW = K::TPW, RPM = K::RPM, NCH = (KS + 7) / 8;
-          const int tb = wave * TPW, te = min(tb + TPW, MT * NTL), mt0 = tb / NTL;
+      // acc[tile] += Et[16 mt.., h] W[h, 16 nt..] over all primitive pairs:
+      // the wave's run of tiles [tb, te), in chunks of 8 K steps: the A
+      // fragments of the next chunk (every row tile the run touches) are
+      // requested before this chunk's products, chunk 0 before the
+      // Hobson step; one accumulation chain per tile (in the arithmetic type)
+      constexpr int TPW = K::TPW, RPM = K::RPM, NCH = (KS + 7) / 8;
+      const int tb = wave * TPW, te = min(tb + TPW, MT * NTL), mt0 = tb / NTL;
 #pragma unroll
-          for (int ch = 0; ch < NCH; ch++) {
-            if (ch + 1 < NCH) load_a(ak[(ch + 1) & 1], Et, ch + 1, mt0);
-            const int s0 = 8 * ch;
+      for (int ch = 0; ch < NCH; ch++) {
+        if (ch + 1 < NCH) load_a(ak[(ch + 1) & 1], Et, ch + 1, mt0);
+        const int s0 = 8 * ch;
 #pragma unroll
-            for (int q = 0; q < TPW; q++) {
-              const int tau = tb + q;
-              if (tau >= te) break;
-              const int mtq = tau / NTL, nt = tau - mtq * NTL;
-              const T* wb = W + lk * NCOLP + 16 * nt + lr;
-              flt4m c4 = acc32[q];
+        for (int q = 0; q < TPW; q++) {
+          const int tau = tb + q;
+          if (tau >= te) break;
+          const int mtq = tau / NTL, nt = tau - mtq * NTL;
+          const T* wb = W + lk * NCOLP + 16 * nt + lr;
+          typename Op::v4 c4 = acc[q];
 #pragma unroll
-              for (int s = 0; s < 8; s++) {
-                if (s0 + s >= KS) break;
-                T a = ak[ch & 1][0][s];
+          for (int s = 0; s < 8; s++) {
+            if (s0 + s >= KS) break;
+            T a = ak[ch & 1][0][s];
 #pragma unroll
-                for (int r = 1; r < RPM; r++) a = (mtq == mt0 + r) ? ak[ch & 1][r][s] : a;
-                c4 = Op::mfma(a, wb[4 * (s0 + s) * NCOLP], c4);
-              }
-              acc32[q] = c4;
-            }
+            for (int r = 1; r < RPM; r++) a = (mtq == mt0 + r) ? ak[ch & 1][r][s] : a;
+            c4 = Op::mfma(a, wb[4 * (s0 + s) * NCOLP], c4);
           }
-        } else {
-          const T* ea = Et + lk * NABP + 16 * mt + lr;
-#pragma unroll
-          for (int s = 0; s < KS; s++) av[s] = ea[4 * s * NABP];
-#pragma unroll
-          for (int t = 0; t < NPW; t++) {
-            const int nt = gi + GRP * t;
-            if (nt >= NTL) break;
-            const T* wb = W + lk * NCOLP + 16 * nt + lr;
-            dbl4m c4 = acc[t];
-#pragma unroll
-            for (int s = 0; s < KS; s++) c4 = Op::mfma(av[s], wb[4 * s * NCOLP], c4);
-            acc[t] = c4;
-          }
+          acc[q] = c4;
         }
       }
     }
   }
   const int64_t row0 = pa.row[pidx];
-  if constexpr (K::OCC == 2) {
-    const int tb = wave * K::TPW, te = min(tb + K::TPW, MT * NTL);
+  const int tb = wave * K::TPW, te = min(tb + K::TPW, MT * NTL);
 #pragma unroll
-    for (int q = 0; q < K::TPW; q++) {
-      const int tau = tb + q;
-      if (tau >= te) break;
-      const int mtq = tau / NTL, nt = tau - mtq * NTL;
-      const int col = 16 * nt + lr, c = col / NCC, m = col - c * NCC;
-      if (c >= nc) continue;
-      const int64_t ocol = xa.col[cidx[c]] + m;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int munu = 16 * mtq + Op::row(lk, r);
-        if (munu < NAB) out[(row0 + munu) * ldo + ocol] = (double)acc32[q][r];
-      }
-    }
-    return;
-  }
-  if (mt >= MT) return;
-#pragma unroll
-  for (int t = 0; t < NPW; t++) {
-    const int nt = gi + GRP * t;
-    if (nt >= NTL) break;
+  for (int q = 0; q < K::TPW; q++) {
+    const int tau = tb + q;
+    if (tau >= te) break;
+    const int mtq = tau / NTL, nt = tau - mtq * NTL;
     const int col = 16 * nt + lr, c = col / NCC, m = col - c * NCC;
     if (c >= nc) continue;
     const int64_t ocol = xa.col[cidx[c]] + m;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int munu = 16 * mt + Op::row(lk, r);
-      if (munu < NAB) out[(row0 + munu) * ldo + ocol] = acc[t][r];
+      const int munu = 16 * mtq + Op::row(lk, r);
+      if (munu < NAB) out[(row0 + munu) * ldo + ocol] = (double)acc[q][r];
     }
   }
 }
