@@ -432,6 +432,10 @@ Lane& cur_lane() { return t_lane ? *t_lane : ctx().lane0; }
 // lane's own stream instead, so that the streams of all lanes stay within
 // the hardware queues (work of streams that share a queue serializes).
 thread_local bool t_few_streams = false;
+// lanes of the batch call running beside this one: the automatic HBM
+// budgets of a call (staging matrix, Et tables) are shares of the free memory
+// divided by it, so that concurrent lanes do not over-commit the device
+thread_local int t_lanes = 1;
 hipStream_t copy_stream(Lane& l) { return t_few_streams ? l.st : l.st3; }
 
 // Uploads through the lane's pinned arena, for uploads queued behind running
@@ -1931,7 +1935,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     if (et_cap == 0) {
       size_t fr = 0, tot = 0;
       HC(hipMemGetInfo(&fr, &tot));
-      et_cap = std::max<size_t>((fr + cur_lane().wsT.n * sizeof(double)) / 6, 1);
+      et_cap = std::max<size_t>((fr + cur_lane().wsT.n * sizeof(double)) / 6 / t_lanes, 1);
     }
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
@@ -2128,7 +2132,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     // the staging matrix this lane already holds counts as free: a repeated
     // call gets the same column blocks as the first one
     fr += cur_lane().wsT.n * sizeof(double);
-    cap = std::max<size_t>((size_t)(0.5 * fr), (size_t)1 << 28);
+    cap = std::max<size_t>((size_t)(0.5 * fr / t_lanes), (size_t)1 << 28);
   }
   const int64_t rowbytes = std::max<int64_t>(pl.npf, 1) * (int64_t)sizeof(double);
   int64_t maxcols = std::max<int64_t>(1, (int64_t)(cap / rowbytes));
@@ -2189,7 +2193,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (mixed) {
     size_t fr = 0, tot = 0;
     HC(hipMemGetInfo(&fr, &tot));
-    const size_t cap32 = std::max<size_t>(fr / 6, 1);
+    const size_t cap32 = std::max<size_t>(fr / 6 / t_lanes, 1);
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
         const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
@@ -3355,8 +3359,12 @@ int dfhip_contract_coulomb_densities_batch(int32_t nmol, const dfhip_basis* bas,
       t_lane = w == 0 ? &c.lane0 : c.lanes[w - 1].get();
       t_pool = w == 0 ? nullptr : &t_lane->pool;
       t_few_streams = nl > 1;
+      t_lanes = nl;
       struct Reset {
-        ~Reset() { t_few_streams = false; }
+        ~Reset() {
+          t_few_streams = false;
+          t_lanes = 1;
+        }
       } reset_streams;
       if (hipSetDevice(c.device) != hipSuccess) {
         std::lock_guard<std::mutex> g(emu);
