@@ -171,6 +171,8 @@ typedef struct {
     int64_t max_tile_bytes; /* cap of the (ab|P) staging matrix, 0 = automatic */
     int32_t dens_on_device; /* densities is a device pointer (HBM-resident input) */
     int32_t reserved;
+    /* (a class whose FP32 Et table exceeds a sixth of the free HBM, or the
+       bytes in the environment variable DFHIP_ET32_CAP, runs FP64) */
     double fp32_below;   /* DFHIP_MIXED: FP32 below this Schwarz product (default
                             1e-3: J within 1e-10 of the FP64 path on the test
                             systems, 8 to 200 atoms; 1e-2 already moves J by

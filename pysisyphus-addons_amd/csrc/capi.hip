@@ -2193,7 +2193,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   if (mixed) {
     size_t fr = 0, tot = 0;
     HC(hipMemGetInfo(&fr, &tot));
-    const size_t cap32 = std::max<size_t>(fr / 6 / t_lanes, 1);
+    size_t cap32 = std::max<size_t>(fr / 6 / t_lanes, 1);
+    // DFHIP_ET32_CAP (bytes) replaces the budget of one class's FP32 table:
+    // a small value sends the FP32 windows back to FP64 (for tests)
+    if (const char* e = std::getenv("DFHIP_ET32_CAP"))
+      if (*e) cap32 = (size_t)std::strtoull(e, nullptr, 10);
+    bool refit64 = false;
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
         const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
@@ -2223,6 +2228,13 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
               lowp[k][L] = 0;
               unsplit_precision(pl, k, L);
             }
+          // the FP64 table holds only the pairs with an FP64 window: built
+          // again below for the whole prefixes (or the class leaves the
+          // matrix cores)
+          if (etg[k].p) {
+            etg[k].release();
+            refit64 = true;
+          }
           continue;
         }
         if (list.empty()) continue;
@@ -2233,6 +2245,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
                           c.sph.p, etg32[k].p, st);
         HC(hipGetLastError());
       }
+    if (refit64)
+      build_et([&](int k, int L, int p) { return (lowp[k][L] ? pl.n64[L][p] : pl.nsurv[L][p]) > 0; }, st, false);
     pl.ncomputed32 = 0;
     for (int L = 0; L <= LMAX_AUX; L++)
       for (int p = 0; p < pl.ps.n(); p++) pl.ncomputed32 += pl.nsurv[L][p] - pl.n64[L][p];

@@ -7,7 +7,8 @@ through size-independent properties:
   mixed path at 1e-8;
 * the 200-atom recipe (orbital 4s3p2d1f1g, aux 5s4p3d2f1g1h, 20 states,
   QVL at 1e-10) on 10 atoms: FP64 at 1e-10 with identical kept/skipped
-  counts, mixed at 1e-8;
+  counts, mixed at 1e-8, also with the FP32 Et tables refused (the FP64
+  fallback of the matrix-core windows);
 * the 200-atom workload itself: FP64 against mixed at 1e-8, the 8-fold
   symmetry of J, the class histogram summing to the screened count;
 * the QVL screener records of the device path against the oracle's.
@@ -102,6 +103,26 @@ def test_mixed_vs_oracle(key):
     _check_counts(st)
     assert 0 < st["ncomputed_fp32"] <= st["ncomputed"]
     assert err.max() <= 1e-8
+
+
+def test_mixed_fp32_table_fallback(monkeypatch):
+    """Mixed path on the 200-atom recipe (10 atoms, QVL) with the FP32 Et
+    table budget at one byte (DFHIP_ET32_CAP): every matrix-core class falls
+    back to a single FP64 window, whose Et table is then built again for the
+    whole prefixes (it first held only the pairs with an FP64 window); J
+    against the oracle at 1e-8 with identical counts, and fewer FP32
+    triplets than the default run."""
+    Jo, info = _oracle("w4r")
+    J0, st0 = _gpu("w4r", N.MIXED)
+    monkeypatch.setenv("DFHIP_ET32_CAP", "1")
+    J, st = _gpu("w4r", N.MIXED)
+    err = np.abs(J - Jo).max()
+    _say(f"w4r mixed, FP32 tables refused: max |dJ| {err:.2e}, low-precision triplets {st['ncomputed_fp32']} "
+         f"(default run {st0['ncomputed_fp32']}) / {st['ncomputed']}")
+    assert st["ncomputed"] == info["ncomputed"]
+    _check_counts(st)
+    assert st["ncomputed_fp32"] < st0["ncomputed_fp32"]
+    assert err <= 1e-8
 
 
 def test_mixed_blocks_fg_classes():
