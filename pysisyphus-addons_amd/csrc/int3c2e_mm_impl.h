@@ -98,6 +98,9 @@ struct Cls3m {
   static constexpr int nc_max() {
     int nc = 32;
     while (nc > 1 && (!nc_fit(nc) || npw_of(nc) > 8)) nc--;
+    // a multiple of the waves, so that every wave owns as many shells in the
+    // R levels (200-atom workload: 3c2e phase 2.99 -> 2.90 s)
+    if (nc >= NW) nc -= nc % NW;
     return nc;
   }
   static constexpr int NC = nc_max();
