@@ -6,7 +6,9 @@ densities, B = L^-1 d, the 8-fold J assembly).  The default workload is the
 200-atom system (34 bohr cube, orbital 4s3p2d1f1g of 3-primitive
 contractions, aux 5s4p3d2f1g1h uncontracted, QVL screening at 1e-10, 20
 states = 210 densities drawn on the device), with basis and densities
-resident in HBM when the timed region starts.  The same step runs on the
+resident in HBM when the timed region starts, the densities in the layout
+the reference's Cython passes (Fortran order of (ndens, nbf, nbf), the
+density index fastest; `--dens-layout c` for row-major).  The same step runs on the
 plain FP64 path (the drop-in) and on the mixed-precision path.
 
 `--workload 1` and `--workload 2` run the 60-atom systems (orbital s-d / aux
@@ -235,6 +237,9 @@ def main():
                     help="only one f_eval_densities call whose density kernel issues a known number of FP64 "
                          "MFMA instructions (printed): the reference run for the unit of "
                          "SQ_INSTS_VALU_MFMA_MOPS_F64")
+    ap.add_argument("--dens-layout", choices=("c", "fortran"), default="fortran",
+                    help="layout of the device-resident densities: c = (ndens, nbf, nbf) row-major, fortran = "
+                         "(ndens, nbf, nbf) column-major (what the reference's Cython passes)")
     ap.add_argument("--workload", type=int, default=4, choices=(1, 2, 4),
                     help="4: 200 atoms s-g / aux s-h, 20 states, QVL 1e-10, densities generated on the device "
                          "(default); 1: 60 atoms s-d / aux s-f, 8 states; 2: 60 atoms s-f / aux s-g, 8 states")
@@ -263,7 +268,8 @@ def main():
         return bench_batch(args)
 
     dev = torch.device("cuda:0")
-    base_opt = dict(dens_layout=N.DENS_C, dens_on_device=1)
+    fortran = args.dens_layout == "fortran"
+    base_opt = dict(dens_layout=N.DENS_FORTRAN if fortran else N.DENS_C, dens_on_device=1)
     wid = args.workload
     nstates = synthetic.WORKLOADS[wid]["nstates"]
     if wid == 4:
@@ -273,18 +279,24 @@ def main():
         nbf, ndens = shells.nbf, nstates * (nstates + 1) // 2
         gen = torch.Generator(device=dev)
         gen.manual_seed(synthetic.SEED_BASE + 4)
-        dens_d = torch.empty((ndens, nbf, nbf), dtype=torch.float64, device=dev)
+        # C layout: [I][i][j]; Fortran layout of (ndens, nbf, nbf): the density
+        # index fastest, element (I, i, j) at I + ndens (i + nbf j), which is
+        # [j][i][I] of a C-contiguous (nbf, nbf, ndens) tensor
+        dens_d = torch.empty((nbf, nbf, ndens) if fortran else (ndens, nbf, nbf), dtype=torch.float64, device=dev)
         for i in range(ndens):
             a = torch.rand((nbf, nbf), dtype=torch.float64, device=dev, generator=gen)
             a = 0.5 * (a + a.T)
-            dens_d[i] = a / torch.trace(a)
+            if fortran:
+                dens_d[:, :, i] = a / torch.trace(a)  # symmetric: [j][i] = [i][j]
+            else:
+                dens_d[i] = a / torch.trace(a)
             del a
         base_opt.update(screening=N.SCREEN_QVL, thresh=1e-10)
         screening = "qvl"
     else:
         shells, aux, dens = synthetic.workload(wid)
         ndens, nbf = dens.shape[0], dens.shape[1]
-        dens_d = torch.from_numpy(dens).to(dev)
+        dens_d = torch.from_numpy(np.ascontiguousarray(dens.transpose(2, 1, 0)) if fortran else dens).to(dev)
         screening = "schwarz"
     b = N.BasisArrays(*shells.as_arrays())
     x = N.BasisArrays(*aux.as_arrays())
@@ -362,6 +374,7 @@ def main():
         "config": {
             "workload": WORKLOADS[wid],
             "path": head,
+            "density_layout": args.dens_layout,
             "nbf": nbf,
             "naux": aux.nbf,
             "ndens": ndens,
