@@ -1039,8 +1039,11 @@ void launch_gen3(Ctx& c, int la, int lb, int lc, const BraRecords& br, DevShells
   const int nchunks = (xa.hi - xa.lo + 63) / 64;
   if (pa.npairs <= 0 || nchunks <= 0) return;
   if (fp32 && !br.recs32.p) throw Error{DFHIP_ERR_HIP, "FP32 bra records missing"};
-  hipLaunchKernelGGL(fp32 ? g.kernel32 : g.kernel, dim3(pa.npairs, nchunks, fp32 ? g.groups32 : g.groups), dim3(64),
-                     0, st, br.recs.p,
+  // x: the m-groups of a pair interleaved by XCD (8 (G (pidx / 8) + group) +
+  // pidx % 8, the pairs padded to a multiple of 8; the kernel decodes it)
+  const int64_t gx = (int64_t)(fp32 ? g.groups32 : g.groups) * ((pa.npairs + 7) / 8 * 8);
+  if (gx > INT32_MAX) throw Error{DFHIP_ERR_ARG, "too many shell pairs in one class"};
+  hipLaunchKernelGGL(fp32 ? g.kernel32 : g.kernel, dim3((unsigned)gx, nchunks), dim3(64), 0, st, br.recs.p,
                      br.recs32.p, br.off, br.npp, ket, pa, xa, c.boys.p, out, ldo);
   HC(hipGetLastError());
 }

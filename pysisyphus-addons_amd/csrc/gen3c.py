@@ -276,7 +276,15 @@ def gen_class(la, lb, lc, real="double"):
     w(f"__global__ __launch_bounds__({bounds}) void {name}(const double* __restrict__ recs, const float* __restrict__ recs32,")
     w(f"    const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,")
     w(f"    const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {{")
-    w(f"  const int pidx = blockIdx.x;")
+    # blockIdx.x = 8 (G (pidx / 8) + group) + pidx % 8: the m-groups of one
+    # (pair, aux chunk) block get workgroup ids congruent mod 8 and adjacent in
+    # that residue's sequence, so they run side by side on one XCD (workgroups
+    # go to the 8 XCDs round robin) and their interleaved column stores into
+    # the same lines merge in that XCD's L2 instead of reaching HBM as partial
+    # lines, one per group
+    w(f"  const int xs = (int)(blockIdx.x >> 3), grp = xs % {len(groups)};")
+    w(f"  const int pidx = (xs / {len(groups)}) * 8 + (int)(blockIdx.x & 7);")
+    w(f"  if (pidx >= pa.npairs) return;")
     w(f"  const int cend = min(xa.hi, pa.nsurv[pidx]);")
     w(f"  const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;")
     w(f"  const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;")
@@ -292,7 +300,7 @@ def gen_class(la, lb, lc, real="double"):
         w(f"  const float* __restrict__ recp32 = recs32 + recoff[pidx];")
     w(f"  const int64_t row0 = pa.row[pidx];")
     w(f"  const int64_t col = active ? xa.col[c] : 0;")
-    w(f"  switch (blockIdx.z) {{")
+    w(f"  switch (grp) {{")
     for gi, ms in enumerate(groups):
         nacc = na * nb * len(ms)
         w(f"  case {gi}: {{")

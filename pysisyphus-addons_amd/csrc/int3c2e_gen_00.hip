@@ -7,7 +7,9 @@ namespace dfhip {
 __global__ __launch_bounds__(64) void int3c2e_gen_000(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -21,7 +23,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_000(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[1];
 #pragma unroll
@@ -72,7 +74,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_000(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_000(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_000(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[1];
 #pragma unroll
@@ -145,7 +149,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_000(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_001(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_001(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[3];
 #pragma unroll
@@ -232,7 +238,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_001(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_001(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_001(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[3];
 #pragma unroll
@@ -327,7 +335,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_001(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_002(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -341,7 +351,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_002(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[5];
 #pragma unroll
@@ -448,7 +458,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_002(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_002(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -463,7 +475,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_002(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[5];
 #pragma unroll
@@ -577,7 +589,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_002(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_003(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -591,7 +605,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_003(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[7];
 #pragma unroll
@@ -742,7 +756,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_003(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_003(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -757,7 +773,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_003(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[7];
 #pragma unroll

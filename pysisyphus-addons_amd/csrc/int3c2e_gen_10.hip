@@ -7,7 +7,9 @@ namespace dfhip {
 __global__ __launch_bounds__(64) void int3c2e_gen_100(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -21,7 +23,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_100(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[3];
 #pragma unroll
@@ -110,7 +112,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_100(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_100(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -125,7 +129,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_100(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[3];
 #pragma unroll
@@ -221,7 +225,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_100(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_101(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_101(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[9];
 #pragma unroll
@@ -381,7 +387,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_101(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_101(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -396,7 +404,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_101(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[9];
 #pragma unroll
@@ -549,7 +557,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_101(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_102(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -563,7 +573,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_102(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[15];
 #pragma unroll
@@ -787,7 +797,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_102(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_102(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -802,7 +814,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_102(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[15];
 #pragma unroll
@@ -1033,7 +1045,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_102(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_103(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1047,7 +1061,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_103(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[12];
 #pragma unroll
@@ -1525,7 +1539,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_103(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_103(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1540,7 +1556,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_103(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[21];
 #pragma unroll

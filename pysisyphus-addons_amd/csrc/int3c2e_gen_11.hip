@@ -7,7 +7,9 @@ namespace dfhip {
 __global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -21,7 +23,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[9];
 #pragma unroll
@@ -236,7 +238,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_110(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_110(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_110(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[9];
 #pragma unroll
@@ -473,7 +477,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_110(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -487,7 +493,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -1016,7 +1022,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_111(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_111(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1031,7 +1039,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_111(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[27];
 #pragma unroll
@@ -1403,7 +1411,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_111(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 3;
+  const int pidx = (xs / 3) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1417,7 +1427,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -2407,7 +2417,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_112(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_112(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -2422,7 +2434,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_112(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[27];
 #pragma unroll
@@ -3229,7 +3241,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_112(const double* __restrict
 __global__ __launch_bounds__(64, 3) void int3c2e_gen_113(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 4;
+  const int pidx = (xs / 4) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -3243,7 +3257,7 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_113(const double* __restric
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -4828,7 +4842,9 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_113(const double* __restric
 __global__ __launch_bounds__(64) void int3c2e_gen32_113(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -4843,7 +4859,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_113(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[36];
 #pragma unroll

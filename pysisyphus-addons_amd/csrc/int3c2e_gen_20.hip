@@ -7,7 +7,9 @@ namespace dfhip {
 __global__ __launch_bounds__(64) void int3c2e_gen_200(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -21,7 +23,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_200(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[6];
 #pragma unroll
@@ -196,7 +198,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_200(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_200(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -211,7 +215,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_200(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[6];
 #pragma unroll
@@ -393,7 +397,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_200(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_201(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -407,7 +413,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_201(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -700,7 +706,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_201(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_201(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -715,7 +723,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_201(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -1015,7 +1023,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_201(const double* __restrict
 __global__ __launch_bounds__(64, 3) void int3c2e_gen_202(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1029,7 +1039,7 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_202(const double* __restric
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -1694,7 +1704,9 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_202(const double* __restric
 __global__ __launch_bounds__(64) void int3c2e_gen32_202(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1709,7 +1721,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_202(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[30];
 #pragma unroll
@@ -2169,7 +2181,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_202(const double* __restrict
 __global__ __launch_bounds__(64, 3) void int3c2e_gen_203(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 3;
+  const int pidx = (xs / 3) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -2183,7 +2197,7 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_203(const double* __restric
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -3301,7 +3315,9 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_203(const double* __restric
 __global__ __launch_bounds__(64) void int3c2e_gen32_203(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -3316,7 +3332,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_203(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[24];
 #pragma unroll

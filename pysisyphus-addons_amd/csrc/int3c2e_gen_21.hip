@@ -7,7 +7,9 @@ namespace dfhip {
 __global__ __launch_bounds__(64) void int3c2e_gen_210(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -21,7 +23,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_210(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -525,7 +527,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_210(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_210(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 1;
+  const int pidx = (xs / 1) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -540,7 +544,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_210(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -1051,7 +1055,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_210(const double* __restrict
 __global__ __launch_bounds__(64) void int3c2e_gen_211(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 3;
+  const int pidx = (xs / 3) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -1065,7 +1071,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen_211(const double* __restrict__
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -2625,7 +2631,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen_211(const double* __restrict__
 __global__ __launch_bounds__(64) void int3c2e_gen32_211(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 2;
+  const int pidx = (xs / 2) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -2640,7 +2648,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_211(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[36];
 #pragma unroll
@@ -3870,7 +3878,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_211(const double* __restrict
 __global__ __launch_bounds__(64, 3) void int3c2e_gen_212(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 5;
+  const int pidx = (xs / 5) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -3884,7 +3894,7 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_212(const double* __restric
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -6801,7 +6811,9 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_212(const double* __restric
 __global__ __launch_bounds__(64) void int3c2e_gen32_212(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 3;
+  const int pidx = (xs / 3) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -6816,7 +6828,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_212(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[36];
 #pragma unroll
@@ -8992,7 +9004,9 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_212(const double* __restrict
 __global__ __launch_bounds__(64, 3) void int3c2e_gen_213(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 7;
+  const int pidx = (xs / 7) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -9006,7 +9020,7 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_213(const double* __restric
   const double* __restrict__ recp = recs + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[18];
 #pragma unroll
@@ -13600,7 +13614,9 @@ __global__ __launch_bounds__(64, 3) void int3c2e_gen_213(const double* __restric
 __global__ __launch_bounds__(64) void int3c2e_gen32_213(const double* __restrict__ recs, const float* __restrict__ recs32,
     const int64_t* __restrict__ recoff, const int* __restrict__ npp_v, DevShells ket, PairArgs pa, AuxArgs xa,
     const double* __restrict__ boys, double* __restrict__ out, int64_t ldo) {
-  const int pidx = blockIdx.x;
+  const int xs = (int)(blockIdx.x >> 3), grp = xs % 4;
+  const int pidx = (xs / 4) * 8 + (int)(blockIdx.x & 7);
+  if (pidx >= pa.npairs) return;
   const int cend = min(xa.hi, pa.nsurv[pidx]);
   const int cbeg = pa.nbeg ? pa.nbeg[pidx] : 0;
   const int c0 = xa.lo + (int)blockIdx.y * 64, c = c0 + (int)threadIdx.x;
@@ -13615,7 +13631,7 @@ __global__ __launch_bounds__(64) void int3c2e_gen32_213(const double* __restrict
   const float* __restrict__ recp32 = recs32 + recoff[pidx];
   const int64_t row0 = pa.row[pidx];
   const int64_t col = active ? xa.col[c] : 0;
-  switch (blockIdx.z) {
+  switch (grp) {
   case 0: {
     double acc[36];
 #pragma unroll
