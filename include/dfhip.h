@@ -201,6 +201,8 @@ typedef struct {
     int64_t class_triplets[15 * 6];
     int64_t class_prim_triplets[15 * 6];
     int64_t ncomputed_fp32; /* DFHIP_MIXED: triplets inside the FP32 windows */
+    int64_t npairs_zero_q;  /* shell pairs with Q_ab = 0: dropped by their bound before the
+                               (ab|ab) block (see dfhip_schwarz_bounds), or underflowing */
 } dfhip_stats;
 
 void dfhip_default_options(dfhip_options *opt);
@@ -267,6 +269,15 @@ int dfhip_metric_powerh(const dfhip_basis *aux, double power, double *out);
  * NULL to count them; *nrecords receives the count. */
 int dfhip_int3c2e_estimates(const dfhip_basis *bas, const dfhip_basis *aux, double *records,
                             int64_t capacity, int64_t *nrecords);
+
+/* Rigorous upper bounds of the Schwarz norms, on the host: pair_bounds[k]
+ * >= Q_ab for the packed pair k = a(a+1)/2 + b, a >= b (0-based), and
+ * aux_bounds[c] >= Q_C.  No counterpart in the reference: the screened
+ * contraction drops, before their (ab|ab) blocks are evaluated, the pairs
+ * whose bound times the largest factor of their triplets' estimates is
+ * below half the threshold (DFHIP_NO_PAIR_SKIP=1 evaluates every pair). */
+int dfhip_schwarz_bounds(const dfhip_basis *bas, const dfhip_basis *aux, double *pair_bounds,
+                         double *aux_bounds);
 
 /* Metric (P|Q), naux x naux (symmetric). */
 int dfhip_metric(const dfhip_basis *aux, double *out);

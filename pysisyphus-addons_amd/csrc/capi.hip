@@ -11,6 +11,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -760,7 +761,10 @@ struct SchwarzPending {
   bool shipped[Lane::NSHIP] = {};
 };
 
-void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const PairClasses& pc, SchwarzPending& sp) {
+// With sk.thresh > 0 the pairs whose bound on Q_ab keeps no triplet get a
+// zero sum without their quartets being evaluated.
+void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const PairClasses& pc, SchwarzPending& sp,
+                        const PairSkip& sk = PairSkip{}) {
   hipStream_t st = cur_lane().st;
   // the pair list (and each pair's first row) generated on the device
   const int npairs = pc.t.begin[NPCLS];
@@ -846,13 +850,13 @@ void schwarz_bra_launch(Ctx& c, const HostShells& o, const DevShells& dev, const
       const int k = pcls(la, lb), b0 = ps.cls_begin[k], np = ps.cls_end[k] - b0;
       if (np == 0) continue;
       if (kind[k] == 3)
-        c.s4m[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+        c.s4m[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], sk, st);
       else if (kind[k] == 2)
-        c.s4s[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], scr.p, scr.n, st);
+        c.s4s[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], scr.p, scr.n, sk, st);
       else if (kind[k] == 1)
         c.s4c[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
       else
-        c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], st);
+        c.s4[la][lb].fn(dev, s1.p + b0, s2.p + b0, np, c.sph.p, c.boys.p, part.p + poff[k], sk, st);
       HC(hipGetLastError());
       for (int i = 0; i < NS - 1; i++)
         if (k == shipend[i]) ship(i, i > 0 && shipend[i - 1] >= 0 ? poff[shipend[i - 1] + 1] : 0, poff[k + 1]);
@@ -1531,6 +1535,61 @@ std::vector<double> aux_qvl_records(const HostShells& x, const std::vector<doubl
   return xr;
 }
 
+// S_l of pair_bound.h: per degree the largest sum of |coefficients| of one
+// real solid harmonic of the table.
+const double* sph_abs_sums() {
+  static const std::array<double, LMAX_SPH + 1> sl = [] {
+    std::array<double, LMAX_SPH + 1> r{};
+    std::unique_ptr<SphTable> t(new SphTable());
+    build_sph_table(*t);
+    for (int L = 0; L <= LMAX_SPH; L++)
+      for (int m = 0; m < 2 * L + 1; m++) {
+        double s = 0.0;
+        for (int e = t->start[L][m]; e < t->start[L][m + 1]; e++) s += std::fabs(t->terms[e].c);
+        r[L] = std::max(r[L], s);
+      }
+    return r;
+  }();
+  return sl.data();
+}
+
+DevShells host_view(const HostShells& s) {
+  return DevShells{s.n, s.L.data(), s.nprim.data(), s.pptr.data(), s.f0.data(), s.xyz.data(), s.exps.data(),
+                   s.coefs.data()};
+}
+
+// Upper bound of Q_C of aux shell sh (pair_bound.h, one centre).
+double aux_schwarz_bound(const HostShells& x, int sh) {
+  const int L = x.L[sh];
+  double u = 0.0;
+  for (int i = 0; i < x.nprim[sh]; i++) {
+    const double b = x.exps[x.pptr[sh] + i], q = 0.5 * b;
+    const double C = L > 0 ? std::pow(L / b, 0.5 * L) * std::exp(-0.5 * L) : 1.0;
+    u += std::fabs(x.coefs[x.pptr[sh] + i]) * C * std::sqrt(2.0 * std::pow(PI, 2.5) / (q * q * std::sqrt(2.0 * q)));
+  }
+  return std::sqrt(2.0 * L + 1.0) * sph_abs_sums()[L] * u;
+}
+
+// Drop-test parameters of a screened contraction (pair_bound.h): the bound
+// on max Q_C and, under QVL, the far-field factor from the aux extents and
+// betas the screener uses.  DFHIP_NO_PAIR_SKIP=1 turns the test off.
+PairSkip pair_skip_params(const HostShells& x, const dfhip_options& opt) {
+  PairSkip sk;
+  const char* v = std::getenv("DFHIP_NO_PAIR_SKIP");
+  if (opt.screening == DFHIP_SCREEN_NONE || (v && *v && *v != '0') || !(opt.thresh > 0.0)) return sk;
+  for (int l = 0; l <= LMAX_SPH; l++) sk.sl[l] = sph_abs_sums()[l];
+  for (int sh = 0; sh < x.n; sh++) sk.qc = std::max(sk.qc, aux_schwarz_bound(x, sh));
+  if (opt.screening == DFHIP_SCREEN_QVL) {
+    const std::vector<double> xr = aux_qvl_records(x, std::vector<double>(x.n, 0.0));
+    double g = 0.0;
+    for (int sh = 0; sh < x.n; sh++) g = std::max(g, xr[4 * sh + 1] / std::pow(xr[4 * sh], x.L[sh] + 1));
+    sk.qvl = PI * std::sqrt(2.0) * g;
+  }
+  if (!std::isfinite(sk.qc) || !std::isfinite(sk.qvl)) return PairSkip{};
+  sk.thresh = opt.thresh;
+  return sk;
+}
+
 // erfinv(1 - thresh_ws) of the pair extents, thresh_ws = 1d-4
 // (t_df_screener(shells, shells_aux, 1d-4), diabatization/intor.f90:84)
 double qvl_erfi() { return erfinv_pos(1.0 - 1e-4); }
@@ -1692,7 +1751,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   vstep("shells");
   const PairClasses pcl = pair_classes(o);
   SchwarzPending spend;
-  schwarz_bra_launch(c, o, dorb.view, pcl, spend);
+  schwarz_bra_launch(c, o, dorb.view, pcl, spend, pair_skip_params(x, opt));
   const int e2 = tm.mark();
   vstep("Schwarz queued");
   pl.ps = make_pairs(o, pcl);
@@ -2567,6 +2626,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
     stats->ntriplets = pl.ntriplets;
     stats->ncomputed = pl.ncomputed;
     stats->ncomputed_fp32 = pl.ncomputed32;
+    stats->npairs_zero_q = std::count(pl.qab.begin(), pl.qab.end(), 0.0);
     stats->nskipped = pl.ntriplets - pl.ncomputed;
     stats->nbf = o.nbf;
     stats->naux = naux;
@@ -3494,6 +3554,18 @@ int dfhip_qvl_records(const dfhip_basis* bas, const dfhip_basis* aux, double* pa
   return guarded([&] {
     if (!bas || !aux || !pair_records || !aux_records) throw Error{DFHIP_ERR_ARG, "NULL argument"};
     qvl_records_host(*bas, *aux, pair_records, aux_records);
+  });
+}
+
+int dfhip_schwarz_bounds(const dfhip_basis* bas, const dfhip_basis* aux, double* pair_bounds, double* aux_bounds) {
+  return guarded([&] {
+    if (!bas || !aux || !pair_bounds || !aux_bounds) throw Error{DFHIP_ERR_ARG, "NULL argument"};
+    const HostShells o = parse_shells(*bas, LMAX_ORB, "orbital basis");
+    const HostShells x = parse_shells(*aux, LMAX_AUX, "auxiliary basis");
+    const DevShells v = host_view(o);
+    for (int a = 0; a < o.n; a++)
+      for (int b = 0; b <= a; b++) pair_bounds[(size_t)a * (a + 1) / 2 + b] = schwarz_pair_bound(v, a, b, sph_abs_sums());
+    for (int sh = 0; sh < x.n; sh++) aux_bounds[sh] = aux_schwarz_bound(x, sh);
   });
 }
 

@@ -10,6 +10,7 @@
 #pragma once
 
 #include "hermite.h"
+#include "pair_bound.h"
 
 namespace dfhip {
 
@@ -35,7 +36,8 @@ struct Cls4 {
 template <int LA, int LB>
 __global__ __launch_bounds__(128) void schwarz4c_kernel(DevShells bra, const int* __restrict__ s1v,
                                                         const int* __restrict__ s2v, const SphTable* __restrict__ sph,
-                                                        const double* __restrict__ boys, double* __restrict__ part) {
+                                                        const double* __restrict__ boys, double* __restrict__ part,
+                                                        const PairSkip sk) {
   using K = Cls4<LA, LB>;
   constexpr int L = K::L, H = K::H, NR = K::NR, NAB = K::NAB, NT = K::NT, NB2 = K::NB2;
   __shared__ double lds[K::LDS];
@@ -43,6 +45,11 @@ __global__ __launch_bounds__(128) void schwarz4c_kernel(DevShells bra, const int
   const int s1 = s1v[pidx], s2 = s2v[pidx];
   const int n2lo = chunk * NB2, n2n = min(NB2, NAB - n2lo);
   const int tid = threadIdx.x;
+  // a pair whose bound on Q_ab keeps no triplet: zero sum, no quartets
+  if (pair_negligible_wave(bra, s1, s2, sk, tid & 63)) {
+    if (tid == 0) part[(size_t)pidx * K::NCH + chunk] = 0.0;
+    return;
+  }
 
   double* Ebra = lds;
   double* Eket = lds + 3 * K::ESZ;
@@ -163,17 +170,18 @@ __global__ __launch_bounds__(128) void schwarz4c_kernel(DevShells bra, const int
 }
 
 struct Schwarz4Launch {
-  void (*fn)(DevShells, const int*, const int*, int, const SphTable*, const double*, double*, hipStream_t);
+  void (*fn)(DevShells, const int*, const int*, int, const SphTable*, const double*, double*, const PairSkip&,
+             hipStream_t);
   int nch;
 };
 
 template <int LA, int LB>
 void launch_schwarz4c(DevShells bra, const int* s1, const int* s2, int npairs, const SphTable* sph,
-                      const double* boys, double* part, hipStream_t st) {
+                      const double* boys, double* part, const PairSkip& sk, hipStream_t st) {
   using K = Cls4<LA, LB>;
   if (npairs <= 0) return;
   hipLaunchKernelGGL((schwarz4c_kernel<LA, LB>), dim3(npairs, K::NCH), dim3(K::NT), 0, st, bra, s1, s2, sph, boys,
-                     part);
+                     part, sk);
 }
 
 template <int LA, int LB>

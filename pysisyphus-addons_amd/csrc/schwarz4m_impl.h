@@ -22,6 +22,7 @@
 
 #include "gen3c_common.h"
 #include "hermite.h"
+#include "pair_bound.h"
 
 namespace dfhip {
 
@@ -94,7 +95,7 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
                                                                       const int* __restrict__ s2v,
                                                                       const SphTable* __restrict__ sph,
                                                                       const double* __restrict__ boys,
-                                                                      double* __restrict__ part) {
+                                                                      double* __restrict__ part, const PairSkip sk) {
   using K = Cls4m<LA, LB>;
   constexpr int L = K::L, H = K::H, NR = K::NR, NR1 = K::NR1, NAB = K::NAB, NB = K::NB, NT = K::NT, NW = K::NW;
   constexpr int NCT = K::NCT, ZL = K::ZL, NS = K::NS, GW = K::GW, ESZ = K::ESZ;
@@ -109,6 +110,11 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
 
   const int tid = threadIdx.x, pair = blockIdx.x;
   const int s1 = s1v[pair], s2 = s2v[pair];
+  // a pair whose bound on Q_ab keeps no triplet: zero sum, no quartets
+  if (pair_negligible_wave(bra, s1, s2, sk, tid & 63)) {
+    if (tid == 0) part[pair] = 0.0;
+    return;
+  }
   const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb, nq = npp * (npp + 1) / 2;
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
   double* Esc = scr;                    // [pq][d][ESZ]
@@ -328,14 +334,15 @@ __global__ __launch_bounds__((Cls4m<LA, LB>::NT)) void schwarz4m_kernel(DevShell
 
 template <int LA, int LB>
 void launch_schwarz4m(DevShells bra, const int* s1, const int* s2, int npairs, const SphTable* sph,
-                      const double* boys, double* part, hipStream_t st) {
+                      const double* boys, double* part, const PairSkip& sk, hipStream_t st) {
   using K = Cls4m<LA, LB>;
   if (npairs <= 0) return;
-  hipLaunchKernelGGL((schwarz4m_kernel<LA, LB>), dim3(npairs), dim3(K::NT), 0, st, bra, s1, s2, sph, boys, part);
+  hipLaunchKernelGGL((schwarz4m_kernel<LA, LB>), dim3(npairs), dim3(K::NT), 0, st, bra, s1, s2, sph, boys, part, sk);
 }
 
 struct Schwarz4MLaunch {
-  void (*fn)(DevShells, const int*, const int*, int, const SphTable*, const double*, double*, hipStream_t);
+  void (*fn)(DevShells, const int*, const int*, int, const SphTable*, const double*, double*, const PairSkip&,
+             hipStream_t);
   int nppm;  // largest primitive-pair count handled; 0 = class not instantiated
 };
 

@@ -50,7 +50,7 @@ template <int LA, int LB>
 __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, const int* __restrict__ s1v,
                                                              const int* __restrict__ s2v,
                                                              const SphTable* __restrict__ sph,
-                                                             double* __restrict__ scr) {
+                                                             double* __restrict__ scr, const PairSkip sk) {
   using K = Cls4s<LA, LB>;
   constexpr int H = K::H, NAB = K::NAB, NB = K::NB, ESZ = K::ESZ;
   constexpr int NT = 256;
@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void schwarz4s_prep_kernel(DevShells bra, cons
   __shared__ SphTerm ta[nsph(LA) * ncart(LA)], tb[nsph(LB) * ncart(LB)];
   const int tid = threadIdx.x, pair = blockIdx.x;
   const int s1 = s1v[pair], s2 = s2v[pair];
+  if (pair_negligible_wave(bra, s1, s2, sk, tid & 63)) return;  // the main kernel drops it too
   const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb;
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
   double* F = scr + (size_t)pair * K::PERPAIR;
@@ -97,7 +98,8 @@ template <int LA, int LB>
 __global__ __launch_bounds__(256, 2) void schwarz4s_kernel(DevShells bra, const int* __restrict__ s1v,
                                                         const int* __restrict__ s2v,
                                                         const double* __restrict__ boys,
-                                                        const double* __restrict__ scr, double* __restrict__ part) {
+                                                        const double* __restrict__ scr, double* __restrict__ part,
+                                                        const PairSkip sk) {
   using K = Cls4s<LA, LB>;
   constexpr int L = K::L, H = K::H, NR = K::NR, NAB = K::NAB, NT = K::NT, NW = K::NW, GW = K::GW;
   constexpr int NCT = K::NCT, NS = K::NS, TPW = K::TPW, NRT1 = K::NRT1, ZC = K::ZC;
@@ -111,6 +113,11 @@ __global__ __launch_bounds__(256, 2) void schwarz4s_kernel(DevShells bra, const 
   const int tid = threadIdx.x, pair = blockIdx.x;
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const int s1 = s1v[pair], s2 = s2v[pair];
+  // a pair whose bound on Q_ab keeps no triplet: zero sum, no quartets
+  if (pair_negligible_wave(bra, s1, s2, sk, tid & 63)) {
+    if (tid == 0) part[pair] = 0.0;
+    return;
+  }
   const int npb = bra.nprim[s2], npp = bra.nprim[s1] * npb, nq = npp * (npp + 1) / 2;
   const int pa0 = bra.pptr[s1], pb0 = bra.pptr[s2];
   const double* F = scr + (size_t)pair * K::PERPAIR;
@@ -307,21 +314,22 @@ __global__ __launch_bounds__(256, 2) void schwarz4s_kernel(DevShells bra, const 
 // Runs the class over npairs pairs in batches that fit scratch (nscr doubles).
 template <int LA, int LB>
 void launch_schwarz4s(DevShells bra, const int* s1, const int* s2, int npairs, const SphTable* sph, const double* boys,
-                      double* part, double* scr, size_t nscr, hipStream_t st) {
+                      double* part, double* scr, size_t nscr, const PairSkip& sk, hipStream_t st) {
   using K = Cls4s<LA, LB>;
   if (npairs <= 0) return;
   const int batch = (int)std::min<int64_t>(npairs, std::max<int64_t>(1, (int64_t)(nscr / K::PERPAIR)));
   for (int p0 = 0; p0 < npairs; p0 += batch) {
     const int nb = std::min(batch, npairs - p0);
-    hipLaunchKernelGGL((schwarz4s_prep_kernel<LA, LB>), dim3(nb), dim3(256), 0, st, bra, s1 + p0, s2 + p0, sph, scr);
+    hipLaunchKernelGGL((schwarz4s_prep_kernel<LA, LB>), dim3(nb), dim3(256), 0, st, bra, s1 + p0, s2 + p0, sph, scr,
+                       sk);
     hipLaunchKernelGGL((schwarz4s_kernel<LA, LB>), dim3(nb), dim3(K::NT), 0, st, bra, s1 + p0, s2 + p0, boys,
-                       (const double*)scr, part + p0);
+                       (const double*)scr, part + p0, sk);
   }
 }
 
 struct Schwarz4SLaunch {
   void (*fn)(DevShells, const int*, const int*, int, const SphTable*, const double*, double*, double*, size_t,
-             hipStream_t);
+             const PairSkip&, hipStream_t);
   int npps;          // largest primitive-pair count handled; 0 = class not instantiated
   int64_t perpair;   // scratch doubles per pair
 };

@@ -59,6 +59,7 @@ class Stats(ctypes.Structure):
         ("class_triplets", ctypes.c_int64 * 90),
         ("class_prim_triplets", ctypes.c_int64 * 90),
         ("ncomputed_fp32", ctypes.c_int64),
+        ("npairs_zero_q", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -80,7 +81,7 @@ EXPORTED = (
     "dfhip_metric", "dfhip_boys", "dfhip_release_workspace", "dfhip_contract_coulomb_densities_batch",
     "dfhip_qvl_records", "dfhip_int_schwarz_aux_noalloc", "f_intor_df1c", "f_intor_eri2c", "f_intor_df2c_mo",
     "f_benchmark_int3c2e", "dfhip_metric_powerh", "dfhip_int3c2e_estimates", "f_eval_densities",
-    "dfhip_autotune", "dfhip_load_tuning", "dfhip_tuning_choice", "dfhip_tuning_contract",
+    "dfhip_autotune", "dfhip_load_tuning", "dfhip_tuning_choice", "dfhip_tuning_contract", "dfhip_schwarz_bounds",
 )
 
 
@@ -112,6 +113,8 @@ def lib():
         L.dfhip_qvl_records.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis), vp, vp]
         L.dfhip_metric.restype = ctypes.c_int
         L.dfhip_metric.argtypes = [ctypes.POINTER(Basis), vp]
+        L.dfhip_schwarz_bounds.restype = ctypes.c_int
+        L.dfhip_schwarz_bounds.argtypes = [ctypes.POINTER(Basis), ctypes.POINTER(Basis), vp, vp]
         L.dfhip_boys.restype = ctypes.c_int
         L.dfhip_boys.argtypes = [i64, vp, vp, vp]
         args13 = [i32, i32, vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, vp]

@@ -125,6 +125,36 @@ def test_mixed_fp32_table_fallback(monkeypatch):
     assert err <= 1e-8
 
 
+@pytest.mark.parametrize("key", ["w1", "w4r40"])
+def test_pair_drop_changes_nothing(key, monkeypatch):
+    """The pairs dropped by their Schwarz bound before the (ab|ab) blocks
+    (dfhip_schwarz_bounds) keep no triplet: J and every count equal those of
+    the run that evaluates every pair, bit for bit.  w1: the 60-atom s-d
+    workload, Schwarz; w4r40: the 200-atom recipe on 40 atoms, QVL; both
+    FP64 and mixed.  Most pairs of either are dropped."""
+    if key == "w1":
+        shells, aux, dens = synthetic.workload(1)
+        nstates, kw = 8, {}
+    else:
+        shells, aux, dens = synthetic.workload4_reduced(40)
+        nstates, kw = 20, dict(screening=N.SCREEN_QVL, thresh=1e-10)
+    b, x = _arrays(shells, aux)
+    npairs = len(shells.shells) * (len(shells.shells) + 1) // 2
+    for prec in (N.FP64, N.MIXED):
+        opt = N.default_options(precision=prec, **kw)
+        monkeypatch.delenv("DFHIP_NO_PAIR_SKIP", raising=False)
+        _, J, st = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True, options=opt)
+        monkeypatch.setenv("DFHIP_NO_PAIR_SKIP", "1")
+        _, J0, st0 = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True, options=opt)
+        _say(f"{key} precision {prec}: pairs with Q_ab = 0: {st['npairs_zero_q']} dropped run, "
+             f"{st0['npairs_zero_q']} full run, of {npairs}; Schwarz {st['ms_schwarz']:.2f} ms vs "
+             f"{st0['ms_schwarz']:.2f} ms; max |dJ| {np.abs(J - J0).max():.1e}")
+        assert st["npairs_zero_q"] > npairs // 2 > st0["npairs_zero_q"]
+        for k in ("ncomputed", "nskipped", "ncomputed_fp32", "class_triplets", "class_prim_triplets"):
+            assert st[k] == st0[k], k
+        assert np.array_equal(J, J0)
+
+
 def test_mixed_blocks_fg_classes():
     """(ab|P) blocks of the mixed path on the f/g-orbital, g/h-aux classes (the
     200-atom recipe on 4 atoms), against the oracle: max abs and rel error."""

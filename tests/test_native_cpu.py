@@ -147,3 +147,41 @@ def test_tuning_file_roundtrip(tmp_path):
     finally:
         tuning.load(None)
     assert tuning.table(210) == {"int3c2e": {}, "contract": None}
+
+
+def _schwarz_bounds(shells, aux):
+    b, x = N.BasisArrays(*shells.as_arrays()), N.BasisArrays(*aux.as_arrays())
+    n = len(shells.shells)
+    pb, xb = np.zeros(n * (n + 1) // 2), np.zeros(len(aux.shells))
+    assert N.lib().dfhip_schwarz_bounds(b.struct, x.struct, pb.ctypes.data, xb.ctypes.data) == 0
+    return pb, xb
+
+
+@pytest.mark.parametrize("case", ["bena2", "w4r6", "spread"])
+def test_schwarz_bounds_dominate_exact_norms(case, bena2):
+    """The host bounds behind the contraction's pair drop (no GPU): every
+    exact Q_ab and Q_C of the oracle lies below its bound; the bound is not
+    vacuous (within 3x of Q for the tightest pair, 2e3 for the aux shells);
+    and far-apart pairs get bounds small enough to be dropped.
+    bena2: contracted shells of a real molecule; w4r6: the 200-atom recipe
+    (s-g, aux s-h) on 6 atoms; spread: 8 atoms of s-d shells over a 40 bohr
+    cube, where most pairs are far apart."""
+    import oracle as O
+
+    if case == "bena2":
+        shells = aux = Shells.from_arrays(bena2["bas_centers"], bena2["bas_spec"], bena2["bas_data"])
+    elif case == "w4r6":
+        shells, aux, _ = synthetic.workload4_reduced(6)
+    else:
+        shells, aux, _ = synthetic.molecule(8, 40.0, nstates=1, seed=7, with_densities=False)
+    pb, xb = _schwarz_bounds(shells, aux)
+    qb = O.schwarz_bra(O.Basis.from_shells(shells))
+    qa = O.schwarz_aux(O.Basis.from_shells(aux))
+    assert np.all(qb <= pb) and np.all(qa <= xb)
+    live = qb > 1e-12
+    assert np.min(pb[live] / qb[live]) < 3.0
+    # one aux primitive of degree l: the bound is a fixed multiple of Q_C, 2.4 (s) .. 1.4e3 (h)
+    assert np.max(xb / qa) < 2e3
+    if case == "spread":
+        # pairs on different atoms 10+ bohr apart with all exponents >= 0.1
+        assert np.mean(pb < 1e-14) > 0.5
