@@ -2382,6 +2382,38 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       sg[0] = tot;
     }
   };
+  // Pairs of class k launched for aux L (a leading run: the pairs are sorted
+  // by Q_ab descending, so the Schwarz prefixes, and under QVL their suffix
+  // maxima, are non-increasing along a class) and the end of the longest
+  // prefix among them; the whole class without screening.
+  auto live_window = [&](int k, int L, int& np, int& hi_live) {
+    const int p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
+    np = p1 - p0;
+    if (opt.screening == DFHIP_SCREEN_NONE || np == 0) return;
+    const auto& v = pl.alist[L];
+    if (v.empty()) {
+      np = 0;
+      return;
+    }
+    if (use_qvl) {
+      const std::vector<int>& ns = nsmax_qvl[L];
+      np = (int)(std::partition_point(ns.begin() + p0, ns.begin() + p1, [](int n) { return n > 0; }) -
+                 (ns.begin() + p0));
+      if (np > 0) hi_live = std::min(hi_live, ns[p0]);
+    } else {
+      const double qm = pl.qc[v[0]];
+      np = (int)(std::partition_point(pl.qab.begin() + p0, pl.qab.begin() + p1,
+                                      [&](double q) { return !(q * qm < opt.thresh); }) -
+                 (pl.qab.begin() + p0));
+      if (np > 0) {
+        const double qa = pl.qab[p0];
+        const int n = (int)(std::partition_point(v.begin(), v.end(),
+                                                 [&](int sh) { return !(qa * pl.qc[sh] < opt.thresh); }) -
+                            v.begin());
+        hi_live = std::min(hi_live, n);
+      }
+    }
+  };
   std::vector<int64_t> hseg;
   DBuf<int64_t> dseg;
   double ms_int = 0.0, ms_con = 0.0;
@@ -2424,10 +2456,14 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       if (lo >= hi) continue;
       for (int la = 0; la <= LMAX_ORB; la++)
         for (int lb = 0; lb <= la; lb++) {
-          const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], np = pl.ps.cls_end[k] - p0;
-          if (np == 0) continue;
+          const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k];
+          // no workgroups for the pairs past the class's last one with a kept
+          // entry of L, nor for list entries past the longest prefix
+          int np = 0, hi_live = hi;
+          live_window(k, L, np, hi_live);
+          if (np == 0 || hi_live <= lo) continue;
           PairArgs pa{ps1.p + p0, ps2.p + p0, prow.p + p0, dns(L) + p0, np};
-          AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi, kmax[L]};
+          AuxArgs xa{dalist[L].p, dacol[L].p, lo, hi_live, kmax[L]};
           if (use_qvl) {
             pa.qvl = qvl.pair.p + (size_t)8 * p0;
             xa.qvl = qvl.aux.p;
