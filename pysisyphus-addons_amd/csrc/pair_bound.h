@@ -9,10 +9,11 @@
 // Pointwise |x^i y^j z^k| <= r^(i+j+k), so a real solid harmonic of degree l
 // is bounded by S_l r^l (S_l: the largest sum of |coefficients| of one of
 // them), and with p = a + b, K = exp(-ab/p R_AB^2) and the product centre P,
-// r_A, r_B <= r_P + m (m = max(|PA|, |PB|)), n = la + lb:
-//   r_A^la r_B^lb exp(-a r_A^2 - b r_B^2) <= K (r_P + m)^n exp(-p r_P^2)
-//                                         <= K C exp(-p/2 r_P^2),
-//   C = max_s (s + m)^n exp(-p/2 s^2), taken at s = (sqrt(m^2 + 4n/p) - m)/2.
+// r_A <= r_P + |PA|, r_B <= r_P + |PB|, n = la + lb, w = la / n:
+//   r_A^la r_B^lb exp(-a r_A^2 - b r_B^2)
+//     <= K (r_P + |PA|)^la (r_P + |PB|)^lb exp(-p r_P^2) <= K C exp(-p/2 r_P^2),
+//   C = c(|PA|, la, w p/2) c(|PB|, lb, (1 - w) p/2),
+//   c(d, l, q) = max_s (s + d)^l exp(-q s^2), at s = (sqrt(d^2 + 2l/q) - d)/2.
 // The Coulomb kernel is positive, so |f| <= h pointwise gives |f|_C <= |h|_C,
 // the norm is subadditive over the primitive pairs, and
 //   |K C exp(-q r^2)|_C^2 = (K C)^2 2 pi^(5/2) / (q^2 sqrt(2q)),  q = p/2.
@@ -38,18 +39,21 @@ struct PairSkip {
   double sl[LMAX_SPH + 1] = {};   // S_l
 };
 
+// max_s (s + d)^l exp(-q s^2) over s >= 0 (1 for l = 0).
+__host__ __device__ inline double poly_gauss_max(double d, int l, double q) {
+  if (l == 0) return 1.0;
+  const double s = 0.5 * (sqrt(d * d + 2.0 * l / q) - d);
+  return pow(s + d, (double)l) * exp(-q * s * s);
+}
+
 // Contribution of primitive pair (a, b) with coefficients ca, cb at squared
 // distance R2 to the bound of max_mn |phi_mn|_C (without S_la S_lb).
-__host__ __device__ inline double pair_bound_term(double a, double b, double ca, double cb, double R2, int n) {
+__host__ __device__ inline double pair_bound_term(double a, double b, double ca, double cb, double R2, int la, int lb) {
   const double p = a + b, R = sqrt(R2);
   const double K = exp(-a * b / p * R2);
-  const double m = fmax(a, b) * R / p;
-  double C = 1.0;
-  if (n > 0) {
-    const double s = 0.5 * (sqrt(m * m + 4.0 * n / p) - m);
-    C = pow(s + m, (double)n) * exp(-0.5 * p * s * s);
-  }
-  const double q = 0.5 * p;
+  const int n = la + lb;
+  const double w = n > 0 ? (double)la / n : 0.5, q = 0.5 * p;
+  const double C = poly_gauss_max(b * R / p, la, w * q) * poly_gauss_max(a * R / p, lb, (1.0 - w) * q);
   return fabs(ca * cb) * K * C * sqrt(2.0 * pow(PI, 2.5) / (q * q * sqrt(2.0 * q)));
 }
 
@@ -63,7 +67,7 @@ __host__ __device__ inline double schwarz_pair_bound(const DevShells& s, int s1,
   double u = 0.0;
   for (int i = 0; i < s.nprim[s1]; i++)
     for (int j = 0; j < s.nprim[s2]; j++)
-      u += pair_bound_term(s.exps[a0 + i], s.exps[b0 + j], s.coefs[a0 + i], s.coefs[b0 + j], R2, la + lb);
+      u += pair_bound_term(s.exps[a0 + i], s.exps[b0 + j], s.coefs[a0 + i], s.coefs[b0 + j], R2, la, lb);
   return sqrt((double)((2 * la + 1) * (2 * lb + 1))) * sl[la] * sl[lb] * u;
 }
 
@@ -92,7 +96,7 @@ __device__ inline bool pair_negligible_wave(const DevShells& s, int s1, int s2, 
   double u = 0.0;
   for (int e = lane; e < npp; e += 64) {
     const int i = e / nb, j = e - i * nb;
-    u += pair_bound_term(s.exps[a0 + i], s.exps[b0 + j], s.coefs[a0 + i], s.coefs[b0 + j], R2, la + lb);
+    u += pair_bound_term(s.exps[a0 + i], s.exps[b0 + j], s.coefs[a0 + i], s.coefs[b0 + j], R2, la, lb);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) u += __shfl_xor(u, off);
