@@ -1948,11 +1948,19 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   // of the sorted pairs
   const hipStream_t s3 = copy_stream(lane);
   pb.commit(s3);
+  // FP32 copies of the records only for classes that may get an FP32 window
+  // of the generated kernels (the tuning table's choice or DFHIP_GEN32=1)
+  const char* g32env = std::getenv("DFHIP_GEN32");
+  const bool gen32_all = g32env && *g32env && *g32env != '0';
   for (int la = 0; la <= LMAX_ORB; la++)
     for (int lb = 0; lb <= la; lb++) {
       const int k = pcls(la, lb), p0 = ps0.cls_begin[k], n = ps0.cls_end[k] - p0;
-      if (n > 0 && gen && gen3_has(la, lb, 0))
-        fill_bra_records(c, la, lb, dorb.view, ps1.p + p0, ps2.p + p0, n, brtot[k], brs[k], mixed, s3);
+      if (n > 0 && gen && gen3_has(la, lb, 0)) {
+        bool rec32 = false;
+        for (int L = 0; L <= LMAX_AUX && mixed; L++)
+          rec32 = rec32 || (gen3_has(la, lb, L) && (gen32_all || tuned_family(k, L, 1) == 1));
+        fill_bra_records(c, la, lb, dorb.view, ps1.p + p0, ps2.p + p0, n, brtot[k], brs[k], rec32, s3);
+      }
     }
   pl.npf = npf0;
   // per staging-matrix row (sorted pairs): its pair, for the contraction's
@@ -2076,11 +2084,35 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   const auto hp1 = std::chrono::steady_clock::now();
   build_aux_lists(pl);
-  // Schwarz screening (or none) in FP64: the prefixes come from the device,
-  // and the host's sweep (the counts, and the prefixes that the contraction's
-  // row segments use) runs once the integrals are queued
+  // mixed precision: the low-precision kernel of each (class, aux L), 1 the
+  // generated FP32 kernels, 2 the FP32 matrix-core kernels (0: FP64 only);
+  // the split itself is final once the tables of the FP32 matrix-core
+  // kernels are built, below.  A class of the generated kernels keeps one
+  // FP64 window unless the tuning table (or DFHIP_GEN32=1) picks its FP32
+  // kernel: those run at the FP64 rate (FP32 VALU without packed math) and
+  // the split only adds a launch (60-atom s-d workload: 8.0 ms mixed with
+  // the FP32 windows, 6.9 ms FP64)
+  std::vector<std::vector<char>> lowp(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
+  if (mixed) {
+    for (int la = 0; la <= LMAX_ORB; la++)
+      for (int lb = 0; lb <= la; lb++)
+        for (int L = 0; L <= LMAX_AUX; L++) {
+          const int k = pcls(la, lb);
+          const bool mm32 = use_mm[k][L] && c.int3m[la][lb][L].fn32 != nullptr;
+          const int tf = tuned_family(k, L, 1);
+          if (gen && gen3_has(la, lb, L))
+            lowp[k][L] = (mm32 && tf == 2) ? 2 : (tf == 1 || gen32_all) ? 1 : 0;
+          else if (mm32) lowp[k][L] = 2;
+        }
+  }
+  bool any_low = false;
+  for (const auto& v : lowp)
+    for (char w : v) any_low = any_low || w != 0;
+  // Schwarz screening (or none) without FP32 windows: the prefixes come from
+  // the device, and the host's sweep (the counts, and the prefixes that the
+  // contraction's row segments use) runs once the integrals are queued
   const bool dev_prefix =
-      !mixed && (opt.screening == DFHIP_SCREEN_SCHWARZ || opt.screening == DFHIP_SCREEN_NONE);
+      !any_low && (opt.screening == DFHIP_SCREEN_SCHWARZ || opt.screening == DFHIP_SCREEN_NONE);
   if (!dev_prefix) screen_sweep(pl, opt);
   ms_plan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp1).count();
   vlog(opt, "Main basis: %6d shells, %6d basis functions\n", o.n, o.nbf);
@@ -2166,22 +2198,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   auto dns = [&](int L) { return dnsflat.p + (size_t)L * pl.ps.n(); };
   DBuf<int> dn64flat;  // FP64 windows of the prefixes, [L][pair] like dnsflat
   DBuf<unsigned long long> dn32count;  // QVL: kept triplets of the FP32 windows
-  // mixed precision: the low-precision kernel of each (class, aux L), 1 the
-  // generated FP32 kernels, 2 the FP32 matrix-core kernels (0: FP64 only);
-  // the split itself is final once the tables of the FP32 matrix-core
-  // kernels are built, below
-  std::vector<std::vector<char>> lowp(NPCLS, std::vector<char>(LMAX_AUX + 1, 0));
-  if (mixed) {
-    for (int la = 0; la <= LMAX_ORB; la++)
-      for (int lb = 0; lb <= la; lb++)
-        for (int L = 0; L <= LMAX_AUX; L++) {
-          const int k = pcls(la, lb);
-          const bool mm32 = use_mm[k][L] && c.int3m[la][lb][L].fn32 != nullptr;
-          if (gen && gen3_has(la, lb, L)) lowp[k][L] = (mm32 && tuned_family(k, L, 1) == 2) ? 2 : 1;
-          else if (mm32) lowp[k][L] = 2;
-        }
-    split_precision(pl, opt.fp32_below, lowp);
-  }
+  if (any_low) split_precision(pl, opt.fp32_below, lowp);
   const DView<int> dorig{ab.dev<int>(o_orig)};
   vstep("plan uploads");
   const int e5 = tm.mark();
@@ -2252,7 +2269,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   std::vector<DBuf<float>> etg32(NPCLS);
   std::vector<DBuf<int64_t>> etoff32(NPCLS);
   std::vector<DBuf<int>> etlist32(NPCLS);
-  if (mixed) {
+  if (any_low) {
     size_t fr = 0, tot = 0;
     HC(hipMemGetInfo(&fr, &tot));
     size_t cap32 = std::max<size_t>(fr / 6 / t_lanes, 1);

@@ -349,10 +349,21 @@ def test_workload1_vs_oracle():
 
 
 @pytest.mark.parametrize("wid,nstates", [(0, 3), (1, 8)])
-def test_mixed_precision_j(wid, nstates):
+def test_mixed_precision_j(wid, nstates, monkeypatch):
     """Mixed precision (FP32 integral arithmetic below the Schwarz product
-    1e-3): Coulomb-tensor elements within 1e-8 Eh of the FP64 oracle."""
+    1e-3): Coulomb-tensor elements within 1e-8 Eh of the FP64 oracle.  These
+    workloads have generated classes only, which keep one FP64 window unless
+    DFHIP_GEN32=1 (set here) or the tuning table asks for their FP32 kernels;
+    without it the mixed run equals the FP64 one."""
     shells, aux, dens = synthetic.workload(wid)
+    if wid == 1:
+        b = N.BasisArrays(*shells.as_arrays())
+        x = N.BasisArrays(*aux.as_arrays())
+        _, J0, st0 = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True,
+                                     options=N.default_options(precision=N.MIXED))
+        _, J64, _ = _intor.contract(b, x, dens, nstates=nstates, want_df=False, want_j=True)
+        assert st0["ncomputed_fp32"] == 0 and np.array_equal(J0, J64)
+    monkeypatch.setenv("DFHIP_GEN32", "1")
     b = N.BasisArrays(*shells.as_arrays())
     x = N.BasisArrays(*aux.as_arrays())
     opt = N.default_options(precision=N.MIXED)
@@ -366,9 +377,11 @@ def test_mixed_precision_j(wid, nstates):
     assert errJ <= 1e-8
 
 
-def test_mixed_precision_blocks():
-    """(ab|P) blocks of the mixed path: FP64 blocks exact, FP32 blocks (all
-    with Schwarz product < 1e-3) within 2e-9 absolute."""
+def test_mixed_precision_blocks(monkeypatch):
+    """(ab|P) blocks of the mixed path with the FP32 windows of the generated
+    kernels (DFHIP_GEN32=1): FP64 blocks exact, FP32 blocks (all with
+    Schwarz product < 1e-3) within 2e-9 absolute."""
+    monkeypatch.setenv("DFHIP_GEN32", "1")
     shells, aux, _ = synthetic.workload(0)
     b = N.BasisArrays(*shells.as_arrays())
     x = N.BasisArrays(*aux.as_arrays())

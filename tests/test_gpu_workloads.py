@@ -266,18 +266,22 @@ def test_workload4_full_properties():
     assert err <= 1e-8
 
 
-def test_batch_all_256_molecules():
+def test_batch_all_256_molecules(monkeypatch):
     """The batched workload in full: 256 molecules of 5-40 atoms (4 states)
-    in one call, FP64 and mixed; per molecule the mixed J within 1e-8 of its
-    FP64 J, 8-fold symmetry and consistent counts; the two smallest molecules
-    against the oracle at 1e-10."""
+    in one call, FP64 and mixed (with the FP32 windows of the generated
+    kernels, DFHIP_GEN32=1: the default mixed run keeps these classes in
+    FP64); per molecule the mixed J within 1e-8 of its FP64 J, 8-fold
+    symmetry and consistent counts; the two smallest molecules against the
+    oracle at 1e-10."""
     raw, mols = [], []
     for i in range(256):
         sh, ax, dens = synthetic.batch_molecule(i)
         raw.append((sh, ax, dens))
         mols.append((N.BasisArrays(*sh.as_arrays()), N.BasisArrays(*ax.as_arrays()), dens))
     J64, st64 = _intor.contract_batch(mols, [4] * 256, options=N.default_options(precision=N.FP64))
+    monkeypatch.setenv("DFHIP_GEN32", "1")
     Jm, stm = _intor.contract_batch(mols, [4] * 256, options=N.default_options(precision=N.MIXED))
+    assert sum(s["ncomputed_fp32"] for s in stm) > 0
     worst = 0.0
     for k in range(256):
         _check_counts(st64[k])
