@@ -298,10 +298,14 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), (Cls3m<LA, LB, LC, T>::
     if (j0 >= jb && chunk > 0) return;
     const int j1 = min(jb, j0 + NC);
     nc = max(0, j1 - j0);
+    // zero blocks of the dropped entries in [zb, ze): the range's columns are
+    // contiguous (the list's columns are), so consecutive threads store
+    // consecutive columns of each block row
     const int zb = chunk == 0 ? cbeg : kl[j0], ze = j1 < jb ? kl[j1] : cend;
-    const int64_t zrow = pa.row[pidx];
-    for (int e = tid; e < (ze - zb) * NAB * NCC; e += NT) {
-      const int i = zb + e / (NAB * NCC), r = e % (NAB * NCC), munu = r / NCC, m = r - munu * NCC;
+    const int span = (ze - zb) * NCC;
+    double* zout = out + pa.row[pidx] * ldo + xa.col[zb];
+    for (int q = tid; q < span; q += NT) {
+      const int i = zb + q / NCC;
       int lo = j0, hi = j1;  // is entry i one of the chunk's kept entries?
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -309,10 +313,10 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), (Cls3m<LA, LB, LC, T>::
         else hi = mid;
       }
       if (lo < j1 && kl[lo] == i) continue;
-      out[(zrow + munu) * ldo + xa.col[i] + m] = 0.0;
+      for (int munu = 0; munu < NAB; munu++) zout[munu * ldo + q] = 0.0;
     }
-    if (nc == 0) return;
     for (int c = tid; c < nc; c += NT) cidx[c] = kl[j0 + c];
+    if (nc == 0) return;
   } else {
     const int cchunk = xa.lo + chunk * NC;
     const int c0 = pa.nbeg ? max(cchunk, pa.nbeg[pidx]) : cchunk;
