@@ -2433,6 +2433,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   };
   std::vector<int64_t> hseg;
   DBuf<int64_t> dseg;
+  DBuf<int> dkjab;  // QVL: kept-list windows per pair, [window kind][ja, jb][pair]
+  if (use_qvl) dkjab.alloc((size_t)6 * std::max(pl.ps.n(), 1));
   double ms_int = 0.0, ms_con = 0.0;
   std::vector<double> dense_stage;
   vstep("integral set-up");
@@ -2471,6 +2473,22 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       const int lo = (int)(std::lower_bound(cols.begin(), cols.end(), cA) - cols.begin());
       const int hi = (int)(std::lower_bound(cols.begin(), cols.end(), cB) - cols.begin());
       if (lo >= hi) continue;
+      // QVL: each pair's kept-list window for the matrix-core launches of this
+      // L and column block (the whole prefix, and with FP32 windows [0, n64)
+      // and [n64, nsurv)), so that a workgroup reads it instead of searching
+      const int* kja[3] = {}, *kjb[3] = {};
+      if (use_qvl && pl.ps.n() > 0) {
+        const int npall = pl.ps.n();
+        const int* n64v = any_low ? dn64flat.p + (size_t)L * npall : nullptr;
+        for (int wk = 0; wk < (any_low ? 3 : 1); wk++) {
+          int* a0 = dkjab.p + (size_t)(2 * wk) * npall;
+          qvl_windows(qvl.koff[L].p, qvl.kidx[L].p, npall, lo, hi, wk == 2 ? n64v : nullptr, wk == 1 ? n64v : dns(L),
+                      a0, a0 + npall, st);
+          kja[wk] = a0;
+          kjb[wk] = a0 + npall;
+        }
+        HC(hipGetLastError());
+      }
       for (int la = 0; la <= LMAX_ORB; la++)
         for (int lb = 0; lb <= la; lb++) {
           const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k];
@@ -2487,6 +2505,8 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             xa.thresh = opt.thresh;
             pa.koff = qvl.koff[L].p + p0;  // read by the matrix-core kernels only
             pa.kidx = qvl.kidx[L].p;
+            pa.kja = kja[0] + p0;
+            pa.kjb = kjb[0] + p0;
           }
           const bool mmk = use_mm[k][L] && etg[k].p;
           const Int3MLaunch& mml = c.int3m[la][lb][L];
@@ -2503,6 +2523,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
             PairArgs p64 = pa, p32 = pa;
             p64.nsurv = dn64flat.p + (size_t)L * pl.ps.n() + p0;  // [0, n64): FP64
             p32.nbeg = dn64flat.p + (size_t)L * pl.ps.n() + p0;   // [n64, nsurv): FP32
+            if (use_qvl) {
+              p64.kja = kja[1] + p0;
+              p64.kjb = kjb[1] + p0;
+              p32.kja = kja[2] + p0;
+              p32.kjb = kjb[2] + p0;
+            }
             probe_mark(tag0);
             if (mmk) launch_mm(p64);
             else launch_gen3(c, la, lb, L, brs[k], daux.view, p64, xa, Tp - cA, w);

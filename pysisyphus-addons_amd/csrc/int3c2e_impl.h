@@ -27,6 +27,10 @@ struct PairArgs {
   // is computed, the dropped ones as zeros)
   const int64_t* koff = nullptr;
   const int* kidx = nullptr;
+  // optional, with kidx: the kept-list window of each pair for this launch
+  // (qvl_windows), instead of two searches per workgroup
+  const int* kja = nullptr;
+  const int* kjb = nullptr;
 };
 
 struct AuxArgs {

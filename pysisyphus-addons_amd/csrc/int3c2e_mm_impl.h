@@ -293,7 +293,7 @@ __global__ __launch_bounds__((Cls3m<LA, LB, LC, T>::NT), (Cls3m<LA, LB, LC, T>::
       }
       return lo;
     };
-    const int ja = lower(cbeg), jb = lower(cend);
+    const int ja = pa.kja ? pa.kja[pidx] : lower(cbeg), jb = pa.kja ? pa.kjb[pidx] : lower(cend);
     const int j0 = ja + chunk * NC;
     if (j0 >= jb && chunk > 0) return;
     const int j1 = min(jb, j0 + NC);

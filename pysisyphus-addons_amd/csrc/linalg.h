@@ -77,6 +77,8 @@ void qvl_screen(const double* prec, int npairs, const DevShells& ket, const int*
 void qvl_compact(const double* prec, int npairs, const DevShells& ket, const int* alist, int L, const double* xrec,
                  double thresh, const int* nsurv, const int64_t* koff, int* kidx, hipStream_t st);
 // *count += the kept entries of every pair at or after list entry nfrom[p]
+void qvl_windows(const int64_t* koff, const int* kidx, int npairs, int lo, int hi, const int* nbeg, const int* nend,
+                 int* ja, int* jb, hipStream_t st);
 void qvl_count_from(const int64_t* koff, const int* kidx, const int* nfrom, int npairs, unsigned long long* count,
                     hipStream_t st);
 

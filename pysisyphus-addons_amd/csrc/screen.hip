@@ -159,6 +159,45 @@ void qvl_compact(const double* prec, int npairs, const DevShells& ket, const int
                      xrec, thresh, nsurv, koff, kidx);
 }
 
+// Kept-list window of every pair for one launch of the matrix-core kernels:
+// with the list window [cbeg, cend), cbeg = max(lo, nbeg[p]) (lo without
+// nbeg) and cend = min(hi, nend[p]), ja[p] / jb[p] = the first kept entries
+// kidx[koff[p] ..] at or after cbeg / cend (both 0 when the window is empty),
+// so that each workgroup of the pair reads two ints instead of searching its
+// kept list.  One thread per pair.
+__global__ void qvl_window_kernel(const int64_t* __restrict__ koff, const int* __restrict__ kidx, int npairs, int lo,
+                                  int hi, const int* __restrict__ nbeg, const int* __restrict__ nend,
+                                  int* __restrict__ ja, int* __restrict__ jb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int cbeg = nbeg ? max(lo, nbeg[p]) : lo, cend = min(hi, nend[p]);
+  if (cbeg >= cend) {
+    ja[p] = 0;
+    jb[p] = 0;
+    return;
+  }
+  const int* kl = kidx + koff[p];
+  const int nk = (int)(koff[p + 1] - koff[p]);
+  auto lower = [&](int v) {
+    int a = 0, b = nk;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (kl[mid] < v) a = mid + 1;
+      else b = mid;
+    }
+    return a;
+  };
+  ja[p] = lower(cbeg);
+  jb[p] = lower(cend);
+}
+
+void qvl_windows(const int64_t* koff, const int* kidx, int npairs, int lo, int hi, const int* nbeg, const int* nend,
+                 int* ja, int* jb, hipStream_t st) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(qvl_window_kernel, dim3((npairs + 255) / 256), dim3(256), 0, st, koff, kidx, npairs, lo, hi, nbeg,
+                     nend, ja, jb);
+}
+
 // *count += the kept entries kidx[koff[p] ..] at or after nfrom[p], over the
 // pairs (the kept triplets of the windows [nfrom, nsurv)).  One thread per pair.
 __global__ void qvl_count_from_kernel(const int64_t* __restrict__ koff, const int* __restrict__ kidx,
