@@ -69,7 +69,9 @@ __global__ void qvl_screen_kernel(const double* __restrict__ prec, int npairs, D
   double pr[6];
   for (int k = 0; k < 6; k++) pr[k] = prec[8 * (size_t)p + k];
   int last = 0, cnt = 0, pc = 0;
-  for (int i = 0; i < nlist; i++) {
+  // Q_ab = 0 (a pair dropped by its Schwarz bound): every estimate is 0
+  const int n = (thresh > 0.0 && !(pr[5] > 0.0)) ? 0 : nlist;
+  for (int i = 0; i < n; i++) {
     const int sh = alist[i];
     if (qvl_keep(pr, xrec + 4 * (size_t)sh, ket.xyz[3 * sh], ket.xyz[3 * sh + 1], ket.xyz[3 * sh + 2], L, thresh)) {
       last = i + 1;
