@@ -337,7 +337,7 @@ struct Lane {
   // pair part of the plan while the Schwarz kernels of later classes and the
   // metric still run
   hipEvent_t ev_met = nullptr;
-  PinnedBuf hpart, hqc;
+  PinnedBuf hpart, hqc, hqvl;
   PinnedArena arena;
   // copy stream: the metric's inputs go up on it while the Schwarz kernels
   // run (ev_mprep); the Schwarz sums come down on it in NSHIP shipments,
@@ -371,6 +371,7 @@ struct Lane {
     arena.release();
     hpart.release();
     hqc.release();
+    hqvl.release();
     wsT.release();
     wsA.release();
     wsDp.release();
@@ -1506,6 +1507,10 @@ double erfinv_pos(double y) {
 // buffers are the records the 3c2e kernels test each triplet against.
 struct QvlRecords {
   DBuf<double> pair, aux;
+  // per aux L and pair: the prefix end, the kept count and the kept
+  // primitives ([L][3][npairs]); the prefixes feed the kernels' masks
+  DBuf<int> counts;
+  size_t per = 0;
   // per aux L: the kept entries of every pair's prefix (qvl_compact), for the
   // matrix-core kernels
   DBuf<int64_t> koff[LMAX_AUX + 1];
@@ -1604,34 +1609,64 @@ void apply_qvl(Plan& pl, const DevShells& dorb, const DevShells& daux, const int
   qr.pair.alloc((size_t)8 * std::max(np, 1));
   qvl_pair_records(dorb, ps1, ps2, dq.p, np, qvl_erfi(), qr.pair.p, st);
   HC(hipGetLastError());
-  DBuf<int> ns((size_t)3 * std::max(np, 1));
-  std::vector<int> h((size_t)3 * np);
+  // the screens of all aux L first, one copy of their counts into pinned
+  // memory and one wait; then per L the kept-list offsets go up through the
+  // pinned arena and the compaction is queued, without further waits.  Host
+  // loops cover the pairs with Q_ab > 0 (a leading run of each class; the
+  // others keep nothing)
+  const size_t per = (size_t)3 * std::max(np, 1);
+  qr.per = per;
+  qr.counts.alloc(per * (LMAX_AUX + 1));
+  DBuf<int>& ns = qr.counts;
   pl.ncomputed = 0;
+  if (np == 0) return;
   for (int L = 0; L <= LMAX_AUX; L++) {
     const int nl = (int)pl.alist[L].size();
-    if (nl == 0 || np == 0) continue;
-    qvl_screen(qr.pair.p, np, daux, dalist[L].p, nl, L, qr.aux.p, thresh, ns.p, ns.p + np, ns.p + 2 * np, st);
-    HC(hipGetLastError());
-    HC(hipMemcpyAsync(h.data(), ns.p, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-    HC(hipStreamSynchronize(st));
-    std::copy(h.begin(), h.begin() + np, pl.nsurv[L].begin());
-    {
-      std::vector<int64_t> ko((size_t)np + 1, 0);
-      for (int p = 0; p < np; p++) ko[p + 1] = ko[p] + h[np + p];
-      upload(qr.koff[L], ko, st);
-      qr.kidx[L].alloc((size_t)std::max<int64_t>(ko[np], 1));
-      qvl_compact(qr.pair.p, np, daux, dalist[L].p, L, qr.aux.p, thresh, ns.p, qr.koff[L].p, qr.kidx[L].p, st);
-      HC(hipGetLastError());
+    int* n3 = ns.p + per * L;
+    if (nl == 0) {
+      HC(hipMemsetAsync(n3, 0, per * sizeof(int), st));
+      continue;
     }
+    qvl_screen(qr.pair.p, np, daux, dalist[L].p, nl, L, qr.aux.p, thresh, n3, n3 + np, n3 + 2 * np, st);
+    HC(hipGetLastError());
+  }
+  std::vector<int> live(NPCLS);
+  for (int k = 0; k < NPCLS; k++)
+    live[k] = thresh > 0.0 ? (int)(std::partition_point(pl.qab.begin() + pl.ps.cls_begin[k],
+                                                        pl.qab.begin() + pl.ps.cls_end[k],
+                                                        [](double q) { return q > 0.0; }) -
+                                   pl.qab.begin())
+                           : pl.ps.cls_end[k];
+  int* hall = cur_lane().hqvl.get<int>(per * (LMAX_AUX + 1));
+  HC(hipMemcpyAsync(hall, ns.p, per * (LMAX_AUX + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  for (int L = 0; L <= LMAX_AUX; L++) {
+    if (pl.alist[L].empty()) continue;
+    const int* h = hall + per * L;
+    int* n3 = ns.p + per * L;
+    int* nsv = pl.nsurv[L].data();
+    int64_t* ko = cur_lane().arena.alloc<int64_t>((size_t)np + 1);
+    ko[0] = 0;
     for (int la = 0; la <= LMAX_ORB; la++)
       for (int lb = 0; lb <= la; lb++) {
-        const int k = pcls(la, lb);
-        for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
+        const int k = pcls(la, lb), p0 = pl.ps.cls_begin[k], p1 = pl.ps.cls_end[k];
+        for (int p = p0; p < live[k]; p++) {
+          nsv[p] = h[p];
+          ko[p + 1] = ko[p] + h[np + p];
           pl.class_trip[k * 6 + L] += h[np + p];
           pl.class_prim[k * 6 + L] += (int64_t)o.nprim[pl.ps.s1[p]] * o.nprim[pl.ps.s2[p]] * h[2 * np + p];
-          pl.ncomputed += h[np + p];
+        }
+        for (int p = std::max(p0, live[k]); p < p1; p++) {
+          nsv[p] = 0;
+          ko[p + 1] = ko[p];
         }
       }
+    pl.ncomputed += ko[np];
+    qr.koff[L].alloc((size_t)np + 1);
+    HC(hipMemcpyAsync(qr.koff[L].p, ko, ((size_t)np + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    qr.kidx[L].alloc((size_t)std::max<int64_t>(ko[np], 1));
+    qvl_compact(qr.pair.p, np, daux, dalist[L].p, L, qr.aux.p, thresh, n3, qr.koff[L].p, qr.kidx[L].p, st);
+    HC(hipGetLastError());
   }
 }
 
@@ -2188,6 +2223,11 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
       schwarz_prefixes(dqab.p, pl.ps.n(), ab.dev<double>(o_qcs), dl, dl + LMAX_AUX + 1, LMAX_AUX + 1, opt.thresh,
                        opt.screening == DFHIP_SCREEN_NONE, dnsflat.p, st);
       HC(hipGetLastError());
+    } else if (use_qvl && pl.ps.n() > 0) {
+      // the prefix ends of the QVL screen, already on the device
+      for (int L = 0; L <= LMAX_AUX; L++)
+        HC(hipMemcpyAsync(dnsflat.p + (size_t)L * pl.ps.n(), qvl.counts.p + qvl.per * L,
+                          (size_t)pl.ps.n() * sizeof(int), hipMemcpyDeviceToDevice, st));
     } else {
       int* nsflat = lane.arena.alloc<int>(nf);
       for (int L = 0; L <= LMAX_AUX; L++)
