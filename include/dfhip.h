@@ -174,9 +174,11 @@ typedef struct {
     /* (a class whose FP32 Et table exceeds a sixth of the free HBM, or the
        bytes in the environment variable DFHIP_ET32_CAP, runs FP64) */
     double fp32_below;   /* DFHIP_MIXED: FP32 below this Schwarz product (default
-                            1e-3: J within 1e-10 of the FP64 path on the test
-                            systems, 8 to 200 atoms; 1e-2 already moves J by
-                            4e-8 on one 40-atom molecule) */
+                            1e-2: J within 1e-9 of the FP64 path on the test
+                            systems, 8 to 200 atoms); the FP32 windows of the
+                            generated kernels (tuning table or DFHIP_GEN32=1)
+                            stay below 1e-3, where they move J by at most
+                            6e-11 (4.4e-8 on one batch molecule at 1e-2) */
 } dfhip_options;
 
 typedef struct {

@@ -1219,7 +1219,7 @@ dfhip_options defaults() {
   const char* v = std::getenv("DFHIP_VERBOSE");
   o.verbose = (v && *v) ? std::max(0, std::atoi(v)) : 0;
   o.max_tile_bytes = 0;
-  o.fp32_below = 1e-3;
+  o.fp32_below = 1e-2;
   return o;
 }
 
@@ -1426,6 +1426,12 @@ void screen_sweep(Plan& pl, const dfhip_options& opt) {
   for (int L = 0; L <= LMAX_AUX; L++) pl.ncomputed += ncomp[L];
 }
 
+// The FP32 windows of the generated kernels (taken only when the tuning
+// table or DFHIP_GEN32=1 asks for them) stay below this Schwarz product
+// whatever fp32_below says: at 1e-2 one molecule of the 256-molecule batch
+// moves J by 4.4e-8 through them, at 1e-3 the worst is 6e-11.
+constexpr double GEN32_TAU_MAX = 1e-3;
+
 // Mixed precision: the FP64 part of each screened prefix, the list entries
 // with Q_ab Q_C >= tau (a prefix again, non-increasing along a class), for the
 // classes with FP32 kernels; the rest of the prefix runs in FP32.
@@ -1435,10 +1441,11 @@ void split_precision(Plan& pl, double tau, const std::vector<std::vector<char>>&
     const auto& v = pl.alist[L];
     for (int k = 0; k < NPCLS; k++) {
       if (!lowp[k][L]) continue;
+      const double t = lowp[k][L] == 1 ? std::min(tau, GEN32_TAU_MAX) : tau;
       int n = (int)v.size();
       for (int p = pl.ps.cls_begin[k]; p < pl.ps.cls_end[k]; p++) {
         const double qa = pl.qab[p];
-        while (n > 0 && qa * pl.qc[v[n - 1]] < tau) n--;
+        while (n > 0 && qa * pl.qc[v[n - 1]] < t) n--;
         pl.n64[L][p] = std::min(n, pl.nsurv[L][p]);
       }
     }
