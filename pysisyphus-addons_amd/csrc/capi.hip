@@ -1570,9 +1570,16 @@ DevShells host_view(const HostShells& s) {
                    s.coefs.data()};
 }
 
-// Upper bound of Q_C of aux shell sh (pair_bound.h, one centre).
+// Upper bound of Q_C of aux shell sh (pair_bound.h, one centre).  A single
+// normalized primitive has it in closed form: its components are orthonormal
+// and, on one centre with one exponent, Coulomb-orthogonal with the same
+// self-energy, which the Fourier transform of r^l Y_lm exp(-a r^2) gives as
+// 2 pi / (a (l + 1/2)), so Q_C = (2l+1)^(1/4) sqrt(2 pi / (a (l + 1/2)))
+// (1 % added for rounding).
 double aux_schwarz_bound(const HostShells& x, int sh) {
   const int L = x.L[sh];
+  if (x.nprim[sh] == 1)
+    return 1.01 * std::pow(2.0 * L + 1.0, 0.25) * std::sqrt(2.0 * PI / (x.exps[x.pptr[sh]] * (L + 0.5)));
   double u = 0.0;
   for (int i = 0; i < x.nprim[sh]; i++) {
     const double b = x.exps[x.pptr[sh] + i], q = 0.5 * b;

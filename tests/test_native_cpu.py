@@ -161,7 +161,8 @@ def _schwarz_bounds(shells, aux):
 def test_schwarz_bounds_dominate_exact_norms(case, bena2):
     """The host bounds behind the contraction's pair drop (no GPU): every
     exact Q_ab and Q_C of the oracle lies below its bound; the bound is not
-    vacuous (within 3x of Q for the tightest pair, 2e3 for the aux shells);
+    vacuous (within 3x of Q for the tightest pair; 1 % above Q_C for a single
+    aux primitive, within 2e3 for contracted aux shells);
     and far-apart pairs get bounds small enough to be dropped.
     bena2: contracted shells of a real molecule; w4r6: the 200-atom recipe
     (s-g, aux s-h) on 6 atoms; spread: 8 atoms of s-d shells over a 40 bohr
@@ -180,7 +181,12 @@ def test_schwarz_bounds_dominate_exact_norms(case, bena2):
     assert np.all(qb <= pb) and np.all(qa <= xb)
     live = qb > 1e-12
     assert np.min(pb[live] / qb[live]) < 3.0
-    # one aux primitive of degree l: the bound is a fixed multiple of Q_C, 2.4 (s) .. 1.4e3 (h)
+    # a single aux primitive has the closed form (1 % above Q_C); a contracted
+    # shell the primitive-wise bound, a fixed multiple per l (2.4 for s .. 1.4e3 for h)
+    single = np.array([len(s.exps) == 1 for s in aux.shells])
+    if single.any():
+        r = xb[single] / qa[single]
+        assert r.min() >= 1.0 and r.max() <= 1.0101, (r.min(), r.max())
     assert np.max(xb / qa) < 2e3
     if case == "spread":
         # pairs on different atoms 10+ bohr apart with all exponents >= 0.1
