@@ -2269,6 +2269,10 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   }
   const int64_t rowbytes = std::max<int64_t>(pl.npf, 1) * (int64_t)sizeof(double);
   int64_t maxcols = std::max<int64_t>(1, (int64_t)(cap / rowbytes));
+  // a whole number of the contraction's column tiles per block: the GEMM runs
+  // every tile of a block in full, so a block of 287 columns (200 atoms)
+  // paid for 3 tiles of 128 where 2.24 hold columns
+  if (maxcols >= contract_tile()) maxcols = maxcols / contract_tile() * contract_tile();
   // block boundaries at aux-shell granularity over the permuted column order
   std::vector<int64_t> bounds{0};
   {
