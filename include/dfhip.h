@@ -185,7 +185,8 @@ typedef struct {
     int64_t ntriplets;   /* nshells(nshells+1)/2 * nshells_aux */
     int64_t ncomputed;   /* screened-in triplets */
     int64_t nskipped;
-    int64_t nbf, naux, npair_functions;
+    int64_t nbf, naux, npair_functions; /* rows of the staging matrix: the functions of
+                                           the shell pairs with Q_ab > 0 */
     int32_t ncolumn_blocks;
     /* phase times in ms (HIP events): ms_schwarz, ms_pack (the set-up from
        the end of the Schwarz kernels to the first 3c2e kernel),

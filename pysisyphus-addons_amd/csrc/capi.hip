@@ -1332,10 +1332,13 @@ bool big_system(const PairSet& ps, std::vector<char>* big_cls) {
 // pl.qab, pl.from and pl.prow at the class's range (these have their full
 // sizes).  ps0 / q0: the pairs in make_pairs order and their Q_ab; row0: the
 // class's first row (all pairs of a class have the same number of functions).
-void plan_pair_class(Plan& pl, const PairSet& ps0, const std::vector<double>& q0, int k, int64_t row0) {
+// compact: pairs with Q_ab = 0 (they keep no triplet) get no rows of the
+// staging matrix; their prow is the class's row end.  Returns that end.
+int64_t plan_pair_class(Plan& pl, const PairSet& ps0, const std::vector<double>& q0, int k, int64_t row0,
+                        bool compact) {
   const HostShells& o = pl.orb;
   const int p0 = ps0.cls_begin[k], p1 = ps0.cls_end[k];
-  if (p1 == p0) return;
+  if (p1 == p0) return row0;
   std::vector<int> idx(p1 - p0);
   for (int p = p0; p < p1; p++) idx[p - p0] = p;
   sort_desc(q0.data(), idx);
@@ -1347,8 +1350,16 @@ void plan_pair_class(Plan& pl, const PairSet& ps0, const std::vector<double>& q0
     pl.ps.packed[p0 + r] = ps0.packed[p];
     pl.from[p0 + r] = p;
     pl.qab[p0 + r] = q0[p];
-    pl.prow[p0 + r] = row0 + r * w;
   }
+  int64_t row = row0;
+  for (int r = 0; r < p1 - p0; r++) {
+    pl.prow[p0 + r] = row;
+    if (!compact || pl.qab[p0 + r] > 0.0) row += w;
+  }
+  if (compact)  // the zero-Q pairs sort last: they all start at the end
+    for (int r = 0; r < p1 - p0; r++)
+      if (!(pl.qab[p0 + r] > 0.0)) pl.prow[p0 + r] = row;
+  return row;
 }
 
 // Aux lists of the plan (needs Q_C): per L sorted by Q_C descending, their
@@ -1942,6 +1953,12 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
   pl.qab.assign(npairs, 0.0);
   pl.from.assign(npairs, 0);
   pl.prow.assign(npairs, 0);
+  // the staging matrix holds rows of the pairs with Q_ab > 0 only (the others
+  // keep no triplet under a positive threshold), so its column blocks are
+  // wider; the dense and block-norm dumps read every pair's rows
+  const bool compact_rows = opt.screening != DFHIP_SCREEN_NONE && opt.thresh > 0.0 && !out.dense3c &&
+                            !out.blocknorm;
+  int64_t rows_next = 0;
   const bool gen = use_generated();
   UploadBatch pb;
   const size_t o_s1 = pb.add<int>(npairs), o_s2 = pb.add<int>(npairs), o_prow = pb.add<int64_t>(npairs);
@@ -1973,7 +1990,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
                                           "class 4 sorted", "class 5 sorted", "class 6+ sorted"};
       if (opt.verbose >= 3) vstep(wname[std::min(k, 6)]);
       const auto hp0 = std::chrono::steady_clock::now();
-      plan_pair_class(pl, ps0, q0, k, prow0[p0]);
+      rows_next = plan_pair_class(pl, ps0, q0, k, rows_next, compact_rows);
       ms_plan += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hp0).count();
       if (opt.verbose >= 3) vstep(sname[std::min(k, 6)]);
       std::memcpy(pb.host<int>(o_s1) + p0, pl.ps.s1.data() + p0, n * sizeof(int));
@@ -2011,7 +2028,7 @@ void run(const dfhip_basis& B, const dfhip_basis& X, int ndens, int nbfs, const 
         fill_bra_records(c, la, lb, dorb.view, ps1.p + p0, ps2.p + p0, n, brtot[k], brs[k], rec32, s3);
       }
     }
-  pl.npf = npf0;
+  pl.npf = rows_next;
   // per staging-matrix row (sorted pairs): its pair, for the contraction's
   // mask, and the packed density row it reads
   DBuf<int> drowpair, drowperm;
