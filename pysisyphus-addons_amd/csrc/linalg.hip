@@ -665,26 +665,39 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
     }
   }
   __syncthreads();
-  const int tx = tid & 15, ty = tid >> 4;
-  double acc[4][4] = {};
-  for (int l = 0; l < nb; l++) {
-    double a[4], b[4];
+  // the 64 x 64 product Pr Pc^T on the matrix cores: wave w owns rows
+  // 32 (w / 2) .. + 31 and columns 32 (w % 2) .. + 31 as 2 x 2 tiles of
+  // v_mfma_f64_16x16x4 (A: row lane % 16, k lane / 16; B: column lane % 16,
+  // k lane / 16; D: row lane / 16 + 4 i, column lane % 16); the padding
+  // columns l >= nb of Pr and Pc are zero
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int mr0 = 32 * (wave >> 1), nc0 = 32 * (wave & 1);
+  gemm_dbl4 acc[2][2];
 #pragma unroll
-    for (int z = 0; z < 4; z++) {
-      a[z] = Pr[ty * 4 + z][l];
-      b[z] = Pc[tx * 4 + z][l];
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = gemm_dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int k4 = 0; k4 < (nb + 3) / 4; k4++) {
+    const int kk = 4 * k4 + lk;
+    double a[2], b[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      a[x] = Pr[mr0 + 16 * x + lr][kk];
+      b[x] = Pc[nc0 + 16 * x + lr][kk];
     }
 #pragma unroll
-    for (int x = 0; x < 4; x++)
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-      for (int y = 0; y < 4; y++) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
   }
   // A22 -= acc through LDS, so that the read-modify-write runs along columns
   __syncthreads();
 #pragma unroll
-  for (int x = 0; x < 4; x++)
+  for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int y = 0; y < 4; y++) Pr[ty * 4 + x][tx * 4 + y] = acc[x][y];
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) Pr[mr0 + 16 * x + lk + 4 * i][nc0 + 16 * y + lr] = acc[x][y][i];
   __syncthreads();
   for (int e = tid; e < CB * CB; e += 256) {
     const int rr = e & (CB - 1), cc = e >> 6;
